@@ -1,0 +1,259 @@
+/*
+ * raptr_hip.h -- C ABI of the MI355X (gfx950) raptr reconstruction hot path.
+ *
+ * Every entry point is re-entrant: geometry travels as a POD argument (the
+ * reference keeps it in process-global __constant__ memory,
+ * device_constant_memory.hpp:34-46), work is enqueued on the caller's HIP
+ * stream (NULL = the null stream), device buffers belong to the caller, and
+ * scratch comes from a caller-provided workspace whose size is queried first.
+ * Functions return RAPTR_OK (0) or a status code; raptr_last_error() gives
+ * the message of the calling thread's last failure.
+ *
+ * Numerics: every kernel is compiled without FP contraction and with
+ * correctly rounded FP32 divide and square root, in the reference's operation
+ * order, so the workqueue and the matrix values are bit-identical to the
+ * host restatement of the reference on the same inputs.
+ */
+#ifndef RAPTR_HIP_H
+#define RAPTR_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ status */
+enum {
+  RAPTR_OK = 0,
+  RAPTR_ERR_INVALID_ARG = 1,
+  RAPTR_ERR_DEVICE = 2,
+  RAPTR_ERR_OUT_OF_MEMORY = 3,
+  RAPTR_ERR_BUDGET = 4,
+  RAPTR_ERR_WORKSPACE = 5,
+  RAPTR_ERR_NOT_SUPPORTED = 6
+};
+
+enum { RAPTR_RAYGEN_RAND = 0, RAPTR_RAYGEN_REGU = 1 };
+enum { RAPTR_OP_NON_TRANSPOSE = 0, RAPTR_OP_TRANSPOSE = 1 };
+enum { RAPTR_MAX_ANGLES = 360 };
+
+typedef void* raptr_stream_t; /* hipStream_t */
+
+/* DefaultMeasurementSetup<float> (MeasurementSetup.hpp:181-220): head x
+ * positions, angle count, pixels per head (z, y), angular step [deg],
+ * pixel edge lengths (x = depth, y, z). */
+typedef struct raptr_setup_t {
+  float pos0x, pos1x;
+  int na, n0z, n0y, n1z, n1y;
+  float da, segx, segy, segz;
+} raptr_setup_t;
+
+/* DefaultVoxelGrid<float> (VoxelGrid.hpp:151-176). */
+typedef struct raptr_grid_t {
+  float o[3];
+  float d[3];
+  int n[3];
+} raptr_grid_t;
+
+const char* raptr_status_string(int status);
+const char* raptr_last_error(void);
+int raptr_version(void);
+
+/* ------------------------------------------------- device plumbing */
+int raptr_device_count(int* count);
+int raptr_set_device(int device);
+int raptr_device_name(char* buf, int len);
+int raptr_malloc(void** ptr, size_t bytes);
+int raptr_free(void* ptr);
+int raptr_memcpy_h2d(void* dst, const void* src, size_t bytes, raptr_stream_t s);
+int raptr_memcpy_d2h(void* dst, const void* src, size_t bytes, raptr_stream_t s);
+int raptr_memcpy_d2d(void* dst, const void* src, size_t bytes, raptr_stream_t s);
+int raptr_memset(void* dst, int value, size_t bytes, raptr_stream_t s);
+int raptr_stream_create(raptr_stream_t* s);
+int raptr_stream_destroy(raptr_stream_t s);
+int raptr_stream_sync(raptr_stream_t s);
+int raptr_device_sync(void);
+int raptr_event_create(void** ev);
+int raptr_event_destroy(void* ev);
+int raptr_event_record(void* ev, raptr_stream_t s);
+int raptr_event_elapsed_ms(float* ms, void* start, void* stop);
+int raptr_mem_info(size_t* free_bytes, size_t* total_bytes);
+
+/* ----------------------------------------------- geometry (host side) */
+/* Per-angle sin/cos table shared by every kernel: the float argument
+ * pi*ida*da/180 of MeasurementSetupTrafo2CartCoord.hpp:110-111, with the
+ * correctly rounded float of its double-precision sin/cos. */
+int raptr_angle_table(const raptr_setup_t* setup, float* sin_out, float* cos_out);
+/* 0.5*(|voxel edges| + |pixel edges|), getWorkqueue.hpp:74-76. */
+float raptr_sum_radii(const raptr_setup_t* setup, const raptr_grid_t* grid);
+
+/* ------------------------------------------------- A6/A7: workqueue */
+/* Replaces getWorkqueue (getWorkqueue.hpp:135-191 bounded, :193-243
+ * unbounded).  Scans (list entry, voxel) pairs from the cursor
+ * (*list_id, *vxl_id) in list-major, voxel-minor order and writes up to n
+ * candidate pairs (n < 0: all) to d_cnl/d_vxl; advances the cursor exactly
+ * as the reference does.  *n_found receives the count.  d_list is device
+ * memory.  Workspace size from raptr_workqueue_workspace_size. */
+size_t raptr_workqueue_workspace_size(const raptr_grid_t* grid, int list_size);
+int raptr_workqueue(const raptr_setup_t* setup, const raptr_grid_t* grid,
+                    const int* d_list, int list_size, int* list_id, int* vxl_id,
+                    int64_t n, int* d_cnl, int* d_vxl, int64_t capacity,
+                    int64_t* n_found, void* d_workspace, size_t workspace_bytes,
+                    raptr_stream_t stream);
+/* Candidate count of every list entry over the full voxel range
+ * (d_counts[list_size], int64). */
+int raptr_workqueue_counts(const raptr_setup_t* setup, const raptr_grid_t* grid,
+                           const int* d_list, int list_size, int64_t* d_counts,
+                           raptr_stream_t stream);
+
+/* ------------------------------------- A8-A10: matrix from workqueue */
+/* Replaces the getSystemMatrixFromWorkqueue kernel
+ * (getSystemMatrixFromWorkqueue.cu:116-281): d_val[w] = mean over nrays rays
+ * of the slab intersection length of voxel d_vxl[w] with the rays of channel
+ * d_cnl[w].  Rays: Philox4x32-10, key (seed, 0), counter (channel, ray, 0|1,
+ * 0), uniforms ((x>>8)+1)*2^-24, or pixel-centre rays (REGU). */
+int raptr_sm_from_wq(const raptr_setup_t* setup, const raptr_grid_t* grid,
+                     const int* d_cnl, const int* d_vxl, float* d_val,
+                     int64_t wq_length, int nrays, uint32_t seed, int raygen,
+                     raptr_stream_t stream);
+/* Ray end points of (channel, ray) pairs, d_rays[6*k]: the RayGen functor
+ * (RayGenerators.hpp:36-99) evaluated on the device. */
+int raptr_generate_rays(const raptr_setup_t* setup, const int* d_cnl, int n_cnl,
+                        int nrays, uint32_t seed, int raygen, float* d_rays,
+                        raptr_stream_t stream);
+
+/* ---------------------- A12/A13/A23: fused matrix build, chunked CSR */
+/* Builds the matrix of the listed channels (rows in list order) with exact
+ * zeros dropped, columns ascending: the result of the reference's
+ * systemMatrixCalculation (wrappers.hpp:185-266: getSystemMatrix, cooSort,
+ * coo2csr, csr2ecsr) without its zero-valued candidates, computed in
+ * channel chunks sized from the workspace.  Two calls: first with
+ * d_col == NULL to get d_row_ptr (int64, list_size+1) and *nnz (count
+ * mode), or directly with d_col/d_val of `capacity` entries (fill mode;
+ * RAPTR_ERR_WORKSPACE if the matrix does not fit).  Values are
+ * bit-identical to raptr_sm_from_wq on the same (channel, voxel) pairs. */
+size_t raptr_sm_build_workspace_size(const raptr_grid_t* grid, int nrays,
+                                     int64_t staging_entries);
+int raptr_sm_build_csr(const raptr_setup_t* setup, const raptr_grid_t* grid,
+                       const int* d_list, int list_size, int nrays,
+                       uint32_t seed, int raygen, int64_t* d_row_ptr,
+                       int* d_col, float* d_val, int64_t capacity,
+                       int64_t* nnz, void* d_workspace, size_t workspace_bytes,
+                       raptr_stream_t stream);
+/* Sensitivity image s_j = sum over ALL channels of the setup of A_ij, in
+ * FP64 (d_sens[nvox]); the rows are generated and reduced on the fly, never
+ * stored.  Replaces the sensitivity file the reference reads
+ * (reco.cu:171-204).  Accumulation uses FP64 atomics (order-dependent in the
+ * last bits of FP64 only). */
+int raptr_sensitivity_f64(const raptr_setup_t* setup, const raptr_grid_t* grid,
+                          int nrays, uint32_t seed, int raygen, double* d_sens,
+                          raptr_stream_t stream);
+
+/* Matrix-free forward projection: d_out[i] = sum_j A_ij x_j in FP64 for the
+ * listed channels (d_list == NULL: all channels of the setup, list_size
+ * ignored), rows generated on the fly exactly as raptr_sm_build_csr does
+ * (the reference's on-the-fly system matrix, reco.cu:246-259). */
+int raptr_forward_project_f64(const raptr_setup_t* setup, const raptr_grid_t* grid,
+                              const int* d_list, int list_size, int nrays,
+                              uint32_t seed, int raygen, const float* d_x,
+                              double* d_out, raptr_stream_t stream);
+
+/* --------------------------------------- A14-A16: sparse formats */
+/* Replaces cooSort<T> (cooSort.hpp:75-99): in-place stable sort of COO
+ * entries by (row, col); values follow.  Indices must be >= 0. */
+size_t raptr_coo_sort_workspace_size(int64_t n);
+int raptr_coo_sort(float* d_val, int* d_row, int* d_col, int64_t n,
+                   void* d_workspace, size_t workspace_bytes,
+                   raptr_stream_t stream);
+/* Replaces convertCoo2Csr (cusparseWrapper.hpp:53-60): zero-based row
+ * pointer (m+1) of row-sorted COO row ids. */
+int raptr_coo2csr(int* d_row_ptr, const int* d_row_id, int64_t nnz, int m,
+                  raptr_stream_t stream);
+/* Replaces convertCsr2Ecsr (convertCsr2Ecsr.hpp:51-64):
+ * ecsr[i] = csr[y_row_id[i]] (i < nnzr), ecsr[nnzr] = csr[m]. */
+int raptr_csr2ecsr(int* d_ecsr, const int* d_y_row_id, int nnzr,
+                   const int* d_csr, int m, raptr_stream_t stream);
+/* CSR (64-bit row pointers) -> CSR of the transpose, stable: entries of
+ * each transposed row keep ascending original-row order. */
+size_t raptr_csr_transpose_workspace_size(int64_t nnz, int m, int n);
+int raptr_csr_transpose(int m, int n, int64_t nnz, const int64_t* d_row_ptr,
+                        const int* d_col, const float* d_val,
+                        int64_t* d_t_row_ptr, int* d_t_col, float* d_t_val,
+                        void* d_workspace, size_t workspace_bytes,
+                        raptr_stream_t stream);
+
+/* ----------------------------------------------------- A17: SpMV */
+/* Replaces CSRmv<T> (csrmv.hpp:37-99, cusparse{S,D}csrmv):
+ * y = alpha*op(A)*x + beta*y, A m x n zero-based CSR with int32 pointers;
+ * beta == 0 does not read y.  The transposed product scatters with atomics
+ * (like the legacy library routine); the reconstruction path uses a stored
+ * transpose and raptr_spmv_csr64 instead. */
+int raptr_csrmv_f32(int op, int m, int n, int64_t nnz, float alpha,
+                    const float* d_val, const int* d_row_ptr, const int* d_col,
+                    const float* d_x, float beta, float* d_y,
+                    raptr_stream_t stream);
+int raptr_csrmv_f64(int op, int m, int n, int64_t nnz, double alpha,
+                    const double* d_val, const int* d_row_ptr, const int* d_col,
+                    const double* d_x, double beta, double* d_y,
+                    raptr_stream_t stream);
+/* Deterministic y = A x for a CSR with 64-bit row pointers, FP32 values,
+ * FP32 (acc64 = 0) or FP64 (acc64 = 1, y is double*) accumulation. */
+int raptr_spmv_csr64(int m, const int64_t* d_row_ptr, const int* d_col,
+                     const float* d_val, const float* d_x, void* d_y,
+                     int acc64, raptr_stream_t stream);
+
+/* ------------------------------------------- A18-A20: MLEM ops */
+/* mlemOperations.hpp:44-54, :89-104, :147-153, :164-169. */
+int raptr_divides(float* d_res, const float* d_dividend, const float* d_divisor,
+                  int n, raptr_stream_t stream);
+int raptr_divides_multiplies(float* d_res, const float* d_a, const float* d_b,
+                             const float* d_c, int n, raptr_stream_t stream);
+int raptr_scales(float* d_x, float alpha, int n, raptr_stream_t stream);
+/* Sum (deterministic fixed-order tree with FP64 partials; the FP32 result
+ * is written to *h_sum after a stream sync, like thrust::reduce returning
+ * to the host).  Workspace from raptr_sum_workspace_size. */
+size_t raptr_sum_workspace_size(int n);
+int raptr_sum(const float* d_x, int n, float* h_sum, void* d_workspace,
+              size_t workspace_bytes, raptr_stream_t stream);
+
+/* ---------------------------------------- A21/A22: fused MLEM */
+/* Column sums in FP64 (sensitivity image s_j = sum_i A_ij) of a CSR that
+ * stores the transpose (one row per voxel). */
+int raptr_column_sums_f64(int n, const int64_t* d_t_row_ptr,
+                          const float* d_t_val, double* d_out,
+                          raptr_stream_t stream);
+
+/* Plain MLEM (reco.cu:230-303) on a device-resident matrix held both as CSR
+ * (rows = measured channels) and as its transpose.  Per iteration:
+ * yt = A x, e = y/yt (0 where yt == 0), c = A^T e, x = x*c/(s + (s==0)),
+ * x *= 1/sum(x) with the sum in FP64.  s is the raw sensitivity in FP64,
+ * Sigma-normalised inside.  d_x holds the start image and receives the
+ * result.  Work buffers come from the workspace.  use_graph != 0 captures
+ * the iteration into a HIP graph and replays it. */
+size_t raptr_mlem_workspace_size(int m, int n);
+/* The two fused kernels of one iteration, exposed for drivers and timing:
+ * forward + ratio  e_i = y_i / (A x)_i (0 where (A x)_i == 0);
+ * backward + update  xx_j = x_j (A^T e)_j / (s_j + (s_j==0)) with FP64
+ * partial sums of xx per block in d_partials (<= 2048 blocks).  s is the
+ * Sigma-normalised FP32 sensitivity.  lanes: lanes per row (4..64, 0 =
+ * default 64). */
+int raptr_mlem_forward_ratio(int m, const int64_t* d_row_ptr, const int* d_col,
+                             const float* d_val, const float* d_x, const float* d_y,
+                             float* d_e, int lanes, raptr_stream_t stream);
+int raptr_mlem_backward_update(int n, const int64_t* d_t_row_ptr, const int* d_t_col,
+                               const float* d_t_val, const float* d_e, const float* d_x,
+                               const float* d_s, float* d_xx, double* d_partials,
+                               int lanes, raptr_stream_t stream);
+int raptr_mlem(int m, int n, const int64_t* d_row_ptr, const int* d_col,
+               const float* d_val, const int64_t* d_t_row_ptr,
+               const int* d_t_col, const float* d_t_val, const float* d_y,
+               const double* d_sens, float* d_x, int niter, int use_graph,
+               void* d_workspace, size_t workspace_bytes,
+               raptr_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

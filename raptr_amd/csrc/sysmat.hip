@@ -1,0 +1,664 @@
+// sysmat.hip -- Monte Carlo ray-sampled system matrix on gfx950.
+//
+// Two entry points follow the reference:
+//  * raptr_sm_from_wq: the two-stage path (getSystemMatrixFromWorkqueue.cu:
+//    116-281), one thread per workqueue entry, rays of the entry's channel
+//    regenerated from the counter-based Philox stream.
+//  * raptr_sm_build_csr: the production path (what getSystemMatrix +
+//    cooSort + coo2csr + csr2ecsr produce, getSystemMatrixDeviceOnly.cu:
+//    357-562, wrappers.hpp:185-266), one workgroup per channel:
+//      1. the channel's rays are generated once into LDS, with per-ray plane
+//         tables alpha[dim][k] = (o + k*d - r0)/(r1 - r0) for every grid plane
+//         (the voxel-box divisions of the slab method depend only on the
+//         plane index, so one division per plane and ray replaces six per
+//         voxel and ray, with identical float results);
+//      2. the cylinder cull runs over the grid in rounds of 1024 voxels and
+//         compacts candidates into LDS with wave ballots (ascending voxel
+//         order preserved);
+//      3. candidates are sampled densely (all lanes busy), exact zeros are
+//         dropped with a second ballot compaction into an LDS row buffer;
+//      4. row buffers are flushed as pieces into a staging area and packed
+//         into a CSR with 64-bit row pointers per channel chunk.
+#include "primitives.hpp"
+
+namespace raptr {
+namespace {
+
+constexpr int kSmBlock = 256;
+constexpr int kCandCap = 3072;  // >= 2048 + one cull round (1024)
+constexpr int kCandFlush = 2048;
+constexpr int kRowBuf = 3072;   // >= kCandCap: one batch always fits
+constexpr int kCullRound = 1024;
+
+// ------------------------------------------------- two-stage kernels
+__global__ __launch_bounds__(256) void k_sm_from_wq(Geom G, const int* cnl, const int* vxl,
+                                                    float* val, int64_t n, int nrays,
+                                                    uint32_t seed, int raygen) {
+  for (int64_t w = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; w < n;
+       w += int64_t(gridDim.x) * blockDim.x) {
+    int c = cnl[w], vv = vxl[w];
+    if (c < 0 || c >= G.nch || vv < 0 || vv >= G.nvox) {
+      val[w] = 0.0f;  // out-of-range pair: no geometry, no value
+      continue;
+    }
+    int ix, iy, iz;
+    vxl_split(G.g, vv, ix, iy, iz);
+    float box[6] = {plane(G.g, 0, ix),     plane(G.g, 1, iy),     plane(G.g, 2, iz),
+                    plane(G.g, 0, ix + 1), plane(G.g, 1, iy + 1), plane(G.g, 2, iz + 1)};
+    float a = 0.0f;
+    for (int r = 0; r < nrays; ++r) {
+      float ray[6];
+      make_ray(G, raygen, seed, c, r, ray);
+      a += slab_length(box, ray);
+    }
+    a /= float(nrays);
+    val[w] = a;
+  }
+}
+
+__global__ void k_generate_rays(Geom G, const int* cnl, int n_cnl, int nrays, uint32_t seed,
+                                int raygen, float* rays) {
+  int64_t total = int64_t(n_cnl) * nrays;
+  for (int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; k < total;
+       k += int64_t(gridDim.x) * blockDim.x) {
+    int64_t ci = k / nrays;
+    int r = int(k - ci * nrays);
+    int c = cnl[ci];
+    if (c < 0 || c >= G.nch) {
+      for (int q = 0; q < 6; ++q) rays[6 * k + q] = 0.0f;
+      continue;
+    }
+    make_ray(G, raygen, seed, c, r, rays + 6 * k);
+  }
+}
+
+// ----------------------------------------------------- fused build
+struct Piece {
+  int row;       // row within chunk
+  int within;    // offset of the piece inside its row
+  int64_t src;   // staging offset
+  int len;
+  int pad;
+};
+
+struct BuildArgs {
+  Geom G;
+  FastDiv div_yz, div_z;
+  const int* list;  // nullptr: channel = list index (all channels)
+  int first;        // first list entry of this launch
+  int rows;         // list entries in this launch
+  int nrays;
+  uint32_t seed;
+  int raygen;
+  int pl;           // planes per ray (nx+1 + ny+1 + nz+1)
+  // outputs
+  int* row_nnz;     // [rows]
+  int* st_col;      // staging
+  float* st_val;
+  int64_t st_cap;
+  unsigned long long* st_count;
+  Piece* pieces;
+  int piece_cap;
+  unsigned* piece_count;
+  unsigned* flags;  // bit0 staging overflow, bit1 piece overflow
+  double* colsum;   // sensitivity mode
+  const float* xin; // forward-projection mode: image
+  double* fwd;      // forward-projection mode: per-row FP64 projection
+};
+
+enum { kModeCount = 0, kModeFill = 1, kModeColsum = 2, kModeFwd = 3 };
+
+__device__ inline float sample_voxel(const float* rinfo, const float* alpha, int nrays,
+                                     int pl, int ix, int iy, int iz, int off1, int off2) {
+  float a = 0.0f;
+  for (int r = 0; r < nrays; ++r) {
+    const float* ri = rinfo + 8 * r;
+    const float* al = alpha + size_t(r) * pl;
+    int bits = __float_as_int(ri[7]);
+    float amin = 0.0f, amax = 0.0f;
+    bool good = false;
+    if (bits & 1) {
+      float lo = al[ix], hi = al[ix + 1];
+      if (hi < lo) {
+        float t = lo;
+        lo = hi;
+        hi = t;
+      }
+      amin = lo;
+      amax = hi;
+      good = true;
+    }
+    if (bits & 2) {
+      float lo = al[off1 + iy], hi = al[off1 + iy + 1];
+      if (hi < lo) {
+        float t = lo;
+        lo = hi;
+        hi = t;
+      }
+      amin = good ? (amin < lo ? lo : amin) : lo;
+      amax = good ? (hi < amax ? hi : amax) : hi;
+      good = true;
+    }
+    if (bits & 4) {
+      float lo = al[off2 + iz], hi = al[off2 + iz + 1];
+      if (hi < lo) {
+        float t = lo;
+        lo = hi;
+        hi = t;
+      }
+      amin = good ? (amin < lo ? lo : amin) : lo;
+      amax = good ? (hi < amax ? hi : amax) : hi;
+    }
+    float len = (amin < amax) ? (amax - amin) * ri[6] : 0.0f;
+    a += len;
+  }
+  return a / float(nrays);
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kSmBlock) void k_sm_build(BuildArgs A) {
+  extern __shared__ float smem[];
+  __shared__ int wcnt[kSmBlock / kWave];
+  __shared__ long long s_off;
+  const int nr = A.nrays, pl = A.pl;
+  float* rinfo = smem;                                   // nr * 8
+  float* alpha = rinfo + 8 * nr;                         // nr * pl
+  int* cand = reinterpret_cast<int*>(alpha + size_t(nr) * pl);
+  float* cval = reinterpret_cast<float*>(cand + kCandCap);
+  int* rb_col = reinterpret_cast<int*>(cval + kCandCap);
+  float* rb_val = reinterpret_cast<float*>(rb_col + kRowBuf);
+
+  const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
+  const int nx = A.G.g.n[0], ny = A.G.g.n[1];
+  const int off1 = nx + 1, off2 = nx + 1 + ny + 1;
+  const int nvox = A.G.nvox;
+
+  for (int rl = blockIdx.x; rl < A.rows; rl += gridDim.x) {
+    const int li = A.first + rl;
+    int cnl = A.list ? A.list[li] : li;
+    // an out-of-range channel id yields an empty row (uniform branch)
+    const bool bad = cnl < 0 || cnl >= A.G.nch;
+    if (bad) cnl = 0;
+    // 1. rays and plane tables
+    for (int r = t; r < nr; r += kSmBlock) {
+      float ray[6];
+      make_ray(A.G, A.raygen, A.seed, cnl, r, ray);
+      float dx = ray[3] - ray[0], dy = ray[4] - ray[1], dz = ray[5] - ray[2];
+      int bits = (ray[0] != ray[3] ? 1 : 0) | (ray[1] != ray[4] ? 2 : 0) |
+                 (ray[2] != ray[5] ? 4 : 0);
+      float* ri = rinfo + 8 * r;
+      ri[0] = ray[0];
+      ri[1] = ray[1];
+      ri[2] = ray[2];
+      ri[3] = dx;
+      ri[4] = dy;
+      ri[5] = dz;
+      ri[6] = sqrtf(dx * dx + dy * dy + dz * dz);
+      ri[7] = __int_as_float(bits);
+    }
+    __syncthreads();
+    for (int k = t; k < nr * pl; k += kSmBlock) {
+      int r = k / pl, p = k - r * pl;
+      int dim = p < off1 ? 0 : (p < off2 ? 1 : 2);
+      int idx = p - (dim == 0 ? 0 : (dim == 1 ? off1 : off2));
+      const float* ri = rinfo + 8 * r;
+      int bits = __float_as_int(ri[7]);
+      float v = 0.0f;
+      if (bits & (1 << dim)) v = (plane(A.G.g, dim, idx) - ri[dim]) / ri[3 + dim];
+      alpha[k] = v;
+    }
+    float p0[3], p1[3];
+    channel_centres(A.G, cnl, p0, p1);
+    __syncthreads();
+
+    int cand_n = 0, rb_n = 0, flushed = 0;
+    double proj = 0.0;
+    for (int base = 0; base < (bad ? 0 : nvox); base += kCullRound) {
+      // 2. cull one round of 1024 voxels, compact into cand[]
+      for (int r = 0; r < kCullRound / kSmBlock; ++r) {
+        int v = base + r * kSmBlock + t;
+        bool pass = false;
+        int ix = 0, iy = 0, iz = 0;
+        if (v < nvox) {
+          uint32_t qx = fdiv(uint32_t(v), A.div_yz);
+          uint32_t rem = uint32_t(v) - qx * A.div_yz.d;
+          uint32_t qy = fdiv(rem, A.div_z);
+          ix = int(qx);
+          iy = int(qy);
+          iz = int(rem - qy * A.div_z.d);
+          float c[3];
+          voxel_centre(A.G.g, ix, iy, iz, c);
+          pass = distance_point_line(p0, p1, c) < A.G.sumRadii;
+        }
+        uint64_t m = __ballot(pass);
+        if (lane == 0) wcnt[w] = __popcll(m);
+        __syncthreads();
+        int pre = 0, tot = 0;
+        for (int ww = 0; ww < kSmBlock / kWave; ++ww) {
+          int c = wcnt[ww];
+          pre += ww < w ? c : 0;
+          tot += c;
+        }
+        if (pass) cand[cand_n + pre + __popcll(m & lanemask_lt())] = v;
+        cand_n += tot;
+        __syncthreads();
+      }
+      bool last = base + kCullRound >= nvox;
+      if (cand_n == 0 || (cand_n < kCandFlush && !last)) continue;
+      // 3. sample candidates densely
+      for (int c = t; c < cand_n; c += kSmBlock) {
+        int v = cand[c];
+        uint32_t qx = fdiv(uint32_t(v), A.div_yz);
+        uint32_t rem = uint32_t(v) - qx * A.div_yz.d;
+        uint32_t qy = fdiv(rem, A.div_z);
+        cval[c] = sample_voxel(rinfo, alpha, nr, pl, int(qx), int(qy),
+                               int(rem - qy * A.div_z.d), off1, off2);
+      }
+      __syncthreads();
+      if (MODE == kModeColsum) {
+        for (int c = t; c < cand_n; c += kSmBlock) {
+          float a = cval[c];
+          if (a != 0.0f) atomicAdd(&A.colsum[cand[c]], double(a));
+        }
+        __syncthreads();
+        cand_n = 0;
+        continue;
+      }
+      if (MODE == kModeFwd) {
+        for (int c = t; c < cand_n; c += kSmBlock)
+          proj += double(cval[c]) * double(A.xin[cand[c]]);
+        __syncthreads();
+        cand_n = 0;
+        continue;
+      }
+      // flush the row buffer first if the candidates could overflow it
+      if (MODE == kModeFill && rb_n + cand_n > kRowBuf) {
+        if (t == 0) {
+          unsigned long long off = atomicAdd(A.st_count, (unsigned long long)rb_n);
+          long long o = (long long)off;
+          if (o + rb_n > A.st_cap) {
+            atomicOr(A.flags, 1u);
+            o = -1;
+          } else {
+            unsigned pc = atomicAdd(A.piece_count, 1u);
+            if (int(pc) >= A.piece_cap) {
+              atomicOr(A.flags, 2u);
+              o = -1;
+            } else {
+              A.pieces[pc] = Piece{rl, flushed, o, rb_n, 0};
+            }
+          }
+          s_off = o;
+        }
+        __syncthreads();
+        long long o = s_off;
+        if (o >= 0)
+          for (int k = t; k < rb_n; k += kSmBlock) {
+            A.st_col[o + k] = rb_col[k];
+            A.st_val[o + k] = rb_val[k];
+          }
+        flushed += rb_n;
+        rb_n = 0;
+        __syncthreads();
+      }
+      // compact non-zeros in candidate order
+      for (int c0 = 0; c0 < cand_n; c0 += kSmBlock) {
+        int c = c0 + t;
+        bool nz = c < cand_n && cval[c] != 0.0f;
+        uint64_t m = __ballot(nz);
+        if (lane == 0) wcnt[w] = __popcll(m);
+        __syncthreads();
+        int pre = 0, tot = 0;
+        for (int ww = 0; ww < kSmBlock / kWave; ++ww) {
+          int cc = wcnt[ww];
+          pre += ww < w ? cc : 0;
+          tot += cc;
+        }
+        if (MODE == kModeFill && nz) {
+          int pos = rb_n + pre + __popcll(m & lanemask_lt());
+          rb_col[pos] = cand[c];
+          rb_val[pos] = cval[c];
+        }
+        rb_n += tot;
+        __syncthreads();
+      }
+      if (MODE == kModeCount) {
+        flushed += rb_n;
+        rb_n = 0;
+      }
+      cand_n = 0;
+    }
+    if (MODE == kModeFill && rb_n > 0) {
+      if (t == 0) {
+        unsigned long long off = atomicAdd(A.st_count, (unsigned long long)rb_n);
+        long long o = (long long)off;
+        if (o + rb_n > A.st_cap) {
+          atomicOr(A.flags, 1u);
+          o = -1;
+        } else {
+          unsigned pc = atomicAdd(A.piece_count, 1u);
+          if (int(pc) >= A.piece_cap) {
+            atomicOr(A.flags, 2u);
+            o = -1;
+          } else {
+            A.pieces[pc] = Piece{rl, flushed, o, rb_n, 0};
+          }
+        }
+        s_off = o;
+      }
+      __syncthreads();
+      long long o = s_off;
+      if (o >= 0)
+        for (int k = t; k < rb_n; k += kSmBlock) {
+          A.st_col[o + k] = rb_col[k];
+          A.st_val[o + k] = rb_val[k];
+        }
+      flushed += rb_n;
+      rb_n = 0;
+    }
+    if (MODE == kModeFwd) {
+      // fixed-order block reduction of the per-thread FP64 partials
+      proj = wave_reduce_sum(proj);
+      __shared__ double wred[kSmBlock / kWave];
+      if (lane == 0) wred[w] = proj;
+      __syncthreads();
+      if (t == 0) {
+        double tot = 0.0;
+        for (int ww = 0; ww < kSmBlock / kWave; ++ww) tot += wred[ww];
+        A.fwd[li] = tot;
+      }
+    }
+    if ((MODE == kModeCount || MODE == kModeFill) && t == 0) A.row_nnz[rl] = flushed;
+    __syncthreads();
+  }
+}
+
+// row_ptr[first + i] = base + excl[i]
+__global__ void k_rowptr_fill(const int64_t* excl, int rows, int64_t base, int64_t* row_ptr) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < rows) row_ptr[i] = base + excl[i];
+}
+
+// one block per piece: copy staging -> final CSR position
+__global__ __launch_bounds__(256) void k_pieces_copy(const Piece* pieces, int npieces,
+                                                     const int64_t* row_ptr_chunk,
+                                                     const int* st_col, const float* st_val,
+                                                     int* col, float* val) {
+  for (int p = blockIdx.x; p < npieces; p += gridDim.x) {
+    Piece pc = pieces[p];
+    int64_t dst = row_ptr_chunk[pc.row] + pc.within;
+    for (int k = threadIdx.x; k < pc.len; k += 256) {
+      col[dst + k] = st_col[pc.src + k];
+      val[dst + k] = st_val[pc.src + k];
+    }
+  }
+}
+
+size_t build_lds_bytes(int nrays, int pl) {
+  return size_t(nrays) * (8 + size_t(pl)) * sizeof(float) +
+         size_t(kCandCap) * (sizeof(int) + sizeof(float)) +
+         size_t(kRowBuf) * (sizeof(int) + sizeof(float));
+}
+
+int make_build_args(const raptr_setup_t* setup, const raptr_grid_t* grid, int nrays,
+                    uint32_t seed, int raygen, BuildArgs* A, size_t* lds) {
+  std::memset(A, 0, sizeof(BuildArgs));
+  RAPTR_TRY(make_geom(setup, grid, &A->G));
+  if (nrays <= 0) return fail(RAPTR_ERR_INVALID_ARG, "nrays must be positive");
+  if (raygen != RAPTR_RAYGEN_RAND && raygen != RAPTR_RAYGEN_REGU)
+    return fail(RAPTR_ERR_INVALID_ARG, "unknown ray generator %d", raygen);
+  A->div_yz = make_fastdiv(uint32_t(grid->n[1] * grid->n[2]));
+  A->div_z = make_fastdiv(uint32_t(grid->n[2]));
+  A->nrays = nrays;
+  A->seed = seed;
+  A->raygen = raygen;
+  A->pl = grid->n[0] + grid->n[1] + grid->n[2] + 3;
+  *lds = build_lds_bytes(nrays, A->pl);
+  if (*lds > 160 * 1024)
+    return fail(RAPTR_ERR_NOT_SUPPORTED,
+                "nrays=%d x planes=%d needs %zu B of LDS (> 160 KiB)", nrays, A->pl, *lds);
+  return RAPTR_OK;
+}
+
+template <int MODE>
+int launch_build(const BuildArgs& A, size_t lds, hipStream_t st) {
+  static bool configured = false;
+  if (!configured) {
+    // Raise the dynamic-LDS limit once; not fatal where the default suffices.
+    if (hipFuncSetAttribute((const void*)k_sm_build<MODE>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                            160 * 1024) != hipSuccess)
+      (void)hipGetLastError();
+    configured = true;
+  }
+  unsigned g = unsigned(A.rows);
+  hipLaunchKernelGGL(k_sm_build<MODE>, dim3(g), dim3(kSmBlock), lds, st, A);
+  RAPTR_LAUNCH_CHECK();
+  return RAPTR_OK;
+}
+
+
+// Workspace layout of the fused build for a staging area of S entries:
+// counters, staging (col, val), per-row nnz and scan, piece table, scan ws.
+struct BuildLayout {
+  int64_t staging, max_rows, piece_cap;
+};
+
+inline BuildLayout build_layout(int64_t S) {
+  BuildLayout L;
+  L.staging = S;
+  L.max_rows = S / 16 + 1;
+  if (L.max_rows > (1 << 20)) L.max_rows = 1 << 20;
+  L.piece_cap = 2 * L.max_rows + 1024;
+  return L;
+}
+
+template <class C>
+void carve_build(C& c, const BuildLayout& L) {
+  c.template take<unsigned long long>(4);
+  c.template take<int>(size_t(L.staging));
+  c.template take<float>(size_t(L.staging));
+  c.template take<int>(size_t(L.max_rows));
+  c.template take<int64_t>(size_t(L.max_rows));
+  c.template take<char>(size_t(L.piece_cap) * 24);
+  c.template take<char>(scan_workspace_bytes(L.max_rows));
+}
+
+inline size_t build_bytes(int64_t S) {
+  Sizer z;
+  carve_build(z, build_layout(S));
+  return z.bytes();
+}
+
+}  // namespace
+}  // namespace raptr
+
+using namespace raptr;
+
+extern "C" {
+
+int raptr_sm_from_wq(const raptr_setup_t* setup, const raptr_grid_t* grid, const int* d_cnl,
+                     const int* d_vxl, float* d_val, int64_t wq_length, int nrays,
+                     uint32_t seed, int raygen, raptr_stream_t stream) {
+  Geom G;
+  RAPTR_TRY(make_geom(setup, grid, &G));
+  if (nrays <= 0) return fail(RAPTR_ERR_INVALID_ARG, "nrays must be positive");
+  if (wq_length < 0) return fail(RAPTR_ERR_INVALID_ARG, "negative length");
+  if (wq_length == 0) return RAPTR_OK;
+  if (!d_cnl || !d_vxl || !d_val) return fail(RAPTR_ERR_INVALID_ARG, "null buffer");
+  hipLaunchKernelGGL(k_sm_from_wq, dim3(grid_for(wq_length, 256, 1 << 16)), dim3(256), 0,
+                     S(stream), G, d_cnl, d_vxl, d_val, wq_length, nrays, seed, raygen);
+  RAPTR_LAUNCH_CHECK();
+  return RAPTR_OK;
+}
+
+int raptr_generate_rays(const raptr_setup_t* setup, const int* d_cnl, int n_cnl, int nrays,
+                        uint32_t seed, int raygen, float* d_rays, raptr_stream_t stream) {
+  raptr_grid_t g = {{0, 0, 0}, {1, 1, 1}, {1, 1, 1}};
+  Geom G;
+  RAPTR_TRY(make_geom(setup, &g, &G));
+  if (n_cnl <= 0 || nrays <= 0) return RAPTR_OK;
+  hipLaunchKernelGGL(k_generate_rays, dim3(grid_for(int64_t(n_cnl) * nrays, 256, 1 << 16)),
+                     dim3(256), 0, S(stream), G, d_cnl, n_cnl, nrays, seed, raygen, d_rays);
+  RAPTR_LAUNCH_CHECK();
+  return RAPTR_OK;
+}
+
+size_t raptr_sm_build_workspace_size(const raptr_grid_t* grid, int nrays,
+                                     int64_t staging_entries) {
+  (void)grid;
+  (void)nrays;
+  if (staging_entries < kRowBuf) staging_entries = kRowBuf;
+  return build_bytes(staging_entries);
+}
+
+int raptr_sm_build_csr(const raptr_setup_t* setup, const raptr_grid_t* grid,
+                       const int* d_list, int list_size, int nrays, uint32_t seed,
+                       int raygen, int64_t* d_row_ptr, int* d_col, float* d_val,
+                       int64_t capacity, int64_t* nnz, void* d_ws, size_t ws_bytes,
+                       raptr_stream_t stream) {
+  if (!nnz || !d_row_ptr) return fail(RAPTR_ERR_INVALID_ARG, "null output");
+  *nnz = 0;
+  if (list_size < 0 || (list_size > 0 && !d_list))
+    return fail(RAPTR_ERR_INVALID_ARG, "bad list");
+  BuildArgs A;
+  size_t lds = 0;
+  RAPTR_TRY(make_build_args(setup, grid, nrays, seed, raygen, &A, &lds));
+  hipStream_t st = S(stream);
+  const bool fill = d_col != nullptr;
+  if (fill && !d_val) return fail(RAPTR_ERR_INVALID_ARG, "null values");
+  // largest staging area whose layout fits the workspace
+  int64_t lo = 0, hi = int64_t(ws_bytes / 8) + 1;
+  while (lo + 1 < hi) {
+    int64_t mid = lo + (hi - lo) / 2;
+    if (build_bytes(mid) <= ws_bytes) lo = mid; else hi = mid;
+  }
+  if (lo < kRowBuf || !d_ws)
+    return fail(RAPTR_ERR_WORKSPACE, "sm_build workspace too small (%zu B)", ws_bytes);
+  BuildLayout L = build_layout(lo);
+  Carver cv(d_ws, ws_bytes);
+  unsigned long long* counters = cv.take<unsigned long long>(4);
+  int* st_col = cv.take<int>(size_t(L.staging));
+  float* st_val = cv.take<float>(size_t(L.staging));
+  int* row_nnz = cv.take<int>(size_t(L.max_rows));
+  int64_t* excl = cv.take<int64_t>(size_t(L.max_rows));
+  Piece* pieces = reinterpret_cast<Piece*>(cv.take<char>(size_t(L.piece_cap) * 24));
+  size_t sb = scan_workspace_bytes(L.max_rows);
+  void* scan_ws = cv.take<char>(sb);
+  if (!cv.ok) return fail(RAPTR_ERR_WORKSPACE, "sm_build workspace carve failed");
+  const int64_t staging = L.staging, max_rows = L.max_rows;
+  A.list = d_list;
+  A.row_nnz = row_nnz;
+  A.st_col = st_col;
+  A.st_val = st_val;
+  A.st_cap = staging;
+  A.st_count = &counters[0];
+  A.piece_count = reinterpret_cast<unsigned*>(&counters[1]);
+  A.flags = reinterpret_cast<unsigned*>(&counters[2]);
+  A.pieces = pieces;
+
+  int64_t total = 0;
+  int first = 0;
+  int rows = int(max_rows < 4096 ? max_rows : 4096);
+  if (rows > list_size) rows = list_size;
+  while (first < list_size) {
+    if (rows > list_size - first) rows = list_size - first;
+    A.first = first;
+    A.rows = rows;
+    A.piece_cap = int(L.piece_cap);
+    RAPTR_HIP_TRY(hipMemsetAsync(counters, 0, 4 * sizeof(unsigned long long), st));
+    if (fill)
+      RAPTR_TRY(launch_build<kModeFill>(A, lds, st));
+    else
+      RAPTR_TRY(launch_build<kModeCount>(A, lds, st));
+    unsigned long long hc[4];
+    RAPTR_HIP_TRY(hipMemcpyAsync(hc, counters, sizeof(hc), hipMemcpyDeviceToHost, st));
+    RAPTR_HIP_TRY(hipStreamSynchronize(st));
+    unsigned flags = unsigned(hc[2] & 0xffffffffu);
+    if (flags) {
+      if (rows == 1)
+        return fail(RAPTR_ERR_WORKSPACE, "row %d does not fit the staging area", first);
+      rows = rows / 2;
+      continue;
+    }
+    int64_t chunk_nnz = 0;
+    RAPTR_TRY(scan_exclusive_i32_i64(row_nnz, excl, rows, &reinterpret_cast<int64_t*>(counters)[3],
+                                     scan_ws, sb, st));
+    hipLaunchKernelGGL(k_rowptr_fill, dim3(grid_for(rows, 256)), dim3(256), 0, st,
+                       (const int64_t*)excl, rows, total, d_row_ptr + first);
+    RAPTR_LAUNCH_CHECK();
+    RAPTR_HIP_TRY(hipMemcpyAsync(&chunk_nnz, &counters[3], sizeof(int64_t),
+                                 hipMemcpyDeviceToHost, st));
+    RAPTR_HIP_TRY(hipStreamSynchronize(st));
+    if (fill) {
+      if (total + chunk_nnz > capacity)
+        return fail(RAPTR_ERR_WORKSPACE, "matrix exceeds capacity %lld after row %d",
+                    (long long)capacity, first + rows);
+      unsigned np = unsigned(hc[1] & 0xffffffffu);
+      if (np)
+        hipLaunchKernelGGL(k_pieces_copy, dim3(np < 65536 ? np : 65536), dim3(256), 0, st,
+                           (const Piece*)pieces, int(np), (const int64_t*)(d_row_ptr + first),
+                           (const int*)st_col, (const float*)st_val, d_col, d_val);
+      RAPTR_LAUNCH_CHECK();
+    }
+    total += chunk_nnz;
+    first += rows;
+    // next chunk: fill ~60% of the staging at the observed density
+    double avg = rows ? double(chunk_nnz) / rows : 0.0;
+    int64_t want = avg > 0 ? int64_t(0.6 * double(staging) / (avg * 2.0)) : max_rows;
+    if (want > max_rows) want = max_rows;
+    if (want < 1) want = 1;
+    rows = int(want);
+  }
+  int64_t tot = total;
+  RAPTR_HIP_TRY(hipMemcpyAsync(d_row_ptr + list_size, &tot, sizeof(int64_t),
+                               hipMemcpyHostToDevice, st));
+  RAPTR_HIP_TRY(hipStreamSynchronize(st));
+  *nnz = total;
+  return RAPTR_OK;
+}
+
+int raptr_sensitivity_f64(const raptr_setup_t* setup, const raptr_grid_t* grid, int nrays,
+                          uint32_t seed, int raygen, double* d_sens, raptr_stream_t stream) {
+  BuildArgs A;
+  size_t lds = 0;
+  RAPTR_TRY(make_build_args(setup, grid, nrays, seed, raygen, &A, &lds));
+  if (!d_sens) return fail(RAPTR_ERR_INVALID_ARG, "null output");
+  hipStream_t st = S(stream);
+  RAPTR_HIP_TRY(hipMemsetAsync(d_sens, 0, size_t(A.G.nvox) * sizeof(double), st));
+  int64_t nch = int64_t(setup->na) * setup->n0z * setup->n0y * setup->n1z * setup->n1y;
+  A.list = nullptr;
+  A.colsum = d_sens;
+  const int64_t kRowsPerLaunch = 1 << 20;
+  for (int64_t f = 0; f < nch; f += kRowsPerLaunch) {
+    A.first = int(f);
+    A.rows = int(nch - f < kRowsPerLaunch ? nch - f : kRowsPerLaunch);
+    RAPTR_TRY(launch_build<kModeColsum>(A, lds, st));
+  }
+  return RAPTR_OK;
+}
+
+int raptr_forward_project_f64(const raptr_setup_t* setup, const raptr_grid_t* grid,
+                              const int* d_list, int list_size, int nrays, uint32_t seed,
+                              int raygen, const float* d_x, double* d_out,
+                              raptr_stream_t stream) {
+  BuildArgs A;
+  size_t lds = 0;
+  RAPTR_TRY(make_build_args(setup, grid, nrays, seed, raygen, &A, &lds));
+  if (!d_x || !d_out) return fail(RAPTR_ERR_INVALID_ARG, "null buffer");
+  int64_t nch = int64_t(setup->na) * setup->n0z * setup->n0y * setup->n1z * setup->n1y;
+  int64_t rows = d_list ? int64_t(list_size) : nch;
+  if (rows < 0) return fail(RAPTR_ERR_INVALID_ARG, "negative list size");
+  A.list = d_list;
+  A.xin = d_x;
+  A.fwd = d_out;
+  const int64_t kRowsPerLaunch = 1 << 20;
+  for (int64_t f = 0; f < rows; f += kRowsPerLaunch) {
+    A.first = int(f);
+    A.rows = int(rows - f < kRowsPerLaunch ? rows - f : kRowsPerLaunch);
+    RAPTR_TRY(launch_build<kModeFwd>(A, lds, S(stream)));
+  }
+  return RAPTR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,142 @@
+"""Workqueue and system-matrix construction on the GPU (Python side of the C
+ABI).  Mirrors getWorkqueue (getWorkqueue.hpp:135-243), the
+getSystemMatrixFromWorkqueue kernel (getSystemMatrixFromWorkqueue.cu:
+116-281) and systemMatrixCalculation (wrappers.hpp:185-266)."""
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _capi as capi
+from .device import DeviceArray, Workspace
+
+
+def workqueue(setup, grid, cnl_list, list_id=0, vxl_id=0, n=-1, stream=None):
+    """Resumable (optionally bounded) workqueue.  Returns host arrays
+    (cnl, vxl) and the advanced cursor (list_id, vxl_id)."""
+    lst = np.ascontiguousarray(cnl_list, np.int32)
+    d_list = DeviceArray.from_numpy(lst)
+    s, g = setup.struct(), grid.struct()
+    L = capi.lib()
+    ws = Workspace(L.raptr_workqueue_workspace_size(C.byref(g), len(lst)))
+    if n >= 0:
+        cap = int(n)
+    else:
+        cnt = workqueue_counts(setup, grid, lst[list_id:] if list_id < len(lst) else lst[:0],
+                               d_list=None)
+        cap = int(cnt.sum())
+    d_cnl = DeviceArray(max(cap, 1), np.int32)
+    d_vxl = DeviceArray(max(cap, 1), np.int32)
+    li, vi, found = C.c_int(list_id), C.c_int(vxl_id), C.c_int64()
+    capi.call("raptr_workqueue", C.byref(s), C.byref(g), d_list.ptr, len(lst), C.byref(li),
+              C.byref(vi), int(n), d_cnl.ptr, d_vxl.ptr, cap, C.byref(found), ws.ptr,
+              ws.nbytes, stream)
+    k = found.value
+    return d_cnl.numpy(k), d_vxl.numpy(k), li.value, vi.value
+
+
+def workqueue_counts(setup, grid, cnl_list, d_list=None, stream=None):
+    lst = np.ascontiguousarray(cnl_list, np.int32)
+    if len(lst) == 0:
+        return np.zeros(0, np.int64)
+    d_list = d_list or DeviceArray.from_numpy(lst)
+    out = DeviceArray(len(lst), np.int64)
+    s, g = setup.struct(), grid.struct()
+    capi.call("raptr_workqueue_counts", C.byref(s), C.byref(g), d_list.ptr, len(lst), out.ptr,
+              stream)
+    return out.numpy()
+
+
+def sm_from_wq(setup, grid, cnl, vxl, nrays, seed=1, raygen=capi.RAYGEN_RAND, stream=None):
+    cnl = np.ascontiguousarray(cnl, np.int32)
+    vxl = np.ascontiguousarray(vxl, np.int32)
+    d_c, d_v = DeviceArray.from_numpy(cnl), DeviceArray.from_numpy(vxl)
+    d_val = DeviceArray(max(len(cnl), 1), np.float32)
+    s, g = setup.struct(), grid.struct()
+    capi.call("raptr_sm_from_wq", C.byref(s), C.byref(g), d_c.ptr, d_v.ptr, d_val.ptr,
+              len(cnl), int(nrays), int(seed), int(raygen), stream)
+    return d_val.numpy(len(cnl))
+
+
+def generate_rays(setup, cnl, nrays, seed=1, raygen=capi.RAYGEN_RAND, stream=None):
+    cnl = np.ascontiguousarray(cnl, np.int32)
+    d_c = DeviceArray.from_numpy(cnl)
+    d_r = DeviceArray(max(len(cnl) * nrays * 6, 1), np.float32)
+    s = setup.struct()
+    capi.call("raptr_generate_rays", C.byref(s), d_c.ptr, len(cnl), int(nrays), int(seed),
+              int(raygen), d_r.ptr, stream)
+    return d_r.numpy(len(cnl) * nrays * 6).reshape(len(cnl), nrays, 6)
+
+
+@dataclass
+class DeviceCSR:
+    """CSR with 64-bit row pointers held on the device."""
+    m: int
+    n: int
+    nnz: int
+    row_ptr: DeviceArray
+    col: DeviceArray
+    val: DeviceArray
+
+    def to_host(self):
+        return (self.row_ptr.numpy(self.m + 1), self.col.numpy(self.nnz),
+                self.val.numpy(self.nnz))
+
+    @property
+    def bytes(self):
+        return self.nnz * (self.val.dtype.itemsize + 4) + (self.m + 1) * 8
+
+
+DEFAULT_STAGING = 64 << 20  # staging entries of the chunked build
+
+
+def build_csr(setup, grid, cnl_list, nrays, seed=1, raygen=capi.RAYGEN_RAND, stream=None,
+              staging=DEFAULT_STAGING, capacity=None, d_list=None):
+    """Matrix of the listed channels (rows in list order), zeros dropped.
+    Without ``capacity`` a count pass sizes the output exactly."""
+    lst = np.ascontiguousarray(cnl_list, np.int32)
+    m = len(lst)
+    d_list = d_list or DeviceArray.from_numpy(lst)
+    s, g = setup.struct(), grid.struct()
+    L = capi.lib()
+    ws = Workspace(L.raptr_sm_build_workspace_size(C.byref(g), int(nrays), int(staging)))
+    rp = DeviceArray(m + 1, np.int64)
+    nnz = C.c_int64()
+    if capacity is None:
+        capi.call("raptr_sm_build_csr", C.byref(s), C.byref(g), d_list.ptr, m, int(nrays),
+                  int(seed), int(raygen), rp.ptr, None, None, 0, C.byref(nnz), ws.ptr,
+                  ws.nbytes, stream)
+        capacity = nnz.value
+    col = DeviceArray(max(capacity, 1), np.int32)
+    val = DeviceArray(max(capacity, 1), np.float32)
+    capi.call("raptr_sm_build_csr", C.byref(s), C.byref(g), d_list.ptr, m, int(nrays),
+              int(seed), int(raygen), rp.ptr, col.ptr, val.ptr, int(capacity), C.byref(nnz),
+              ws.ptr, ws.nbytes, stream)
+    return DeviceCSR(m, grid.nvox, nnz.value, rp, col, val)
+
+
+def sensitivity(setup, grid, nrays, seed=1, raygen=capi.RAYGEN_RAND, stream=None):
+    """FP64 column sums over all channels of the setup (device array)."""
+    out = DeviceArray(grid.nvox, np.float64)
+    s, g = setup.struct(), grid.struct()
+    capi.call("raptr_sensitivity_f64", C.byref(s), C.byref(g), int(nrays), int(seed),
+              int(raygen), out.ptr, stream)
+    return out
+
+
+def forward_project(setup, grid, x, nrays, cnl_list=None, seed=1, raygen=capi.RAYGEN_RAND,
+                    stream=None):
+    """FP64 projection of image x onto the listed channels (all if None),
+    matrix-free.  Returns a host float64 array."""
+    dx = x if isinstance(x, DeviceArray) else DeviceArray.from_numpy(x, np.float32)
+    if cnl_list is None:
+        d_list, m = None, setup.nchannels
+    else:
+        lst = np.ascontiguousarray(cnl_list, np.int32)
+        d_list, m = DeviceArray.from_numpy(lst), len(lst)
+    out = DeviceArray(max(m, 1), np.float64)
+    s, g = setup.struct(), grid.struct()
+    capi.call("raptr_forward_project_f64", C.byref(s), C.byref(g),
+              d_list.ptr if d_list is not None else None, m, int(nrays), int(seed),
+              int(raygen), dx.ptr, out.ptr, stream)
+    return out.numpy(m)
