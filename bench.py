@@ -101,12 +101,16 @@ def main():
     d_x = DeviceArray.from_numpy(x0)
     L = capi.lib()
     ws = Workspace(L.raptr_mlem_workspace_size(m, n))
+    from raptr_amd import mlem as mlem_mod
+    cfg, tune_ms = mlem_mod.autotune(A, At, d_y, d_x, reps=3, stream=st)
+    log(f"autotune: forward cfg {cfg[0]} ({tune_ms[0]:.3f} ms), backward cfg {cfg[1]} "
+        f"({tune_ms[1]:.3f} ms)")
 
     def frame():
         d_x.copy_from(x0, stream=st)
         capi.call("raptr_mlem", m, n, A.row_ptr.ptr, A.col.ptr, A.val.ptr, At.row_ptr.ptr,
-                  At.col.ptr, At.val.ptr, d_y.ptr, sens.ptr, d_x.ptr, niter, 1, ws.ptr,
-                  ws.nbytes, st)
+                  At.col.ptr, At.val.ptr, d_y.ptr, sens.ptr, d_x.ptr, niter, 1, cfg[0], cfg[1],
+                  ws.ptr, ws.nbytes, st)
 
     for _ in range(args.warmup):
         frame()
@@ -125,12 +129,11 @@ def main():
     log(f"MLEM: {ms_per_step:.2f} ms/frame ({niter} it), {fps:.2f} frames/s, wall {wall_s:.2f}s")
 
     # ---- dominant kernels timed alone with HIP events on the same stream
-    lf = 64 if A.nnz / m >= 48 else 32
-    lb = 64
+    lf, lb = cfg
     d_e = DeviceArray(m, np.float32)
     d_s = DeviceArray.from_numpy((sens.numpy() / sens.numpy().sum()).astype(np.float32))
     d_xx = DeviceArray(n, np.float32)
-    d_part = DeviceArray(2048, np.float64)
+    d_part = DeviceArray(L.raptr_mlem_partial_slots(n), np.float64)
     R = args.kernel_reps
 
     def fwd():
@@ -145,7 +148,7 @@ def main():
     ms_f = timed(lambda: [fwd() for _ in range(R)]) / R
     ms_b = timed(lambda: [bwd() for _ in range(R)]) / R
     bytes_f = 8 * A.nnz + 8 * (m + 1) + 4 * m + 4 * m + 4 * n
-    bytes_b = 8 * A.nnz + 8 * (n + 1) + 4 * m + 4 * n * 3 + 8 * 2048
+    bytes_b = 8 * A.nnz + 8 * (n + 1) + 4 * m + 4 * n * 3 + 8 * (n // 4)
     gbs_f = bytes_f / (ms_f * 1e-3) / 1e9
     gbs_b = bytes_b / (ms_b * 1e-3) / 1e9
     dom = ("mlem_forward_ratio", ms_f, bytes_f, gbs_f) if ms_f >= ms_b else \
@@ -195,7 +198,8 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(dom[3] / HBM_PEAK_GBS, 4),
                      "traffic": None, "algorithmic_bytes": dom[2],
                      "avg_launch_ms": round(dom[1], 4)},
-        "kernels": {"forward_ratio_ms": round(ms_f, 4), "forward_GBps": round(gbs_f, 1),
+        "kernels": {"forward_cfg": cfg[0], "backward_cfg": cfg[1],
+                    "forward_ratio_ms": round(ms_f, 4), "forward_GBps": round(gbs_f, 1),
                     "backward_update_ms": round(ms_b, 4), "backward_GBps": round(gbs_b, 1),
                     "iteration_bytes": bytes_it,
                     "iteration_GBps": round(bytes_it / (ms_per_step / niter * 1e-3) / 1e9, 1)},

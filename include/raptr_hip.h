@@ -236,22 +236,35 @@ size_t raptr_mlem_workspace_size(int m, int n);
 /* The two fused kernels of one iteration, exposed for drivers and timing:
  * forward + ratio  e_i = y_i / (A x)_i (0 where (A x)_i == 0);
  * backward + update  xx_j = x_j (A^T e)_j / (s_j + (s_j==0)) with FP64
- * partial sums of xx per block in d_partials (<= 2048 blocks).  s is the
- * Sigma-normalised FP32 sensitivity.  lanes: lanes per row (4..64, 0 =
- * default 64). */
+ * partial sums of xx per block in d_partials (raptr_mlem_partial_slots(n)
+ * slots).  s is the Sigma-normalised FP32 sensitivity.  (0 =
+ * default).  cfg: kernel configuration code (see raptr_mlem). */
 int raptr_mlem_forward_ratio(int m, const int64_t* d_row_ptr, const int* d_col,
                              const float* d_val, const float* d_x, const float* d_y,
-                             float* d_e, int lanes, raptr_stream_t stream);
+                             float* d_e, int cfg, raptr_stream_t stream);
 int raptr_mlem_backward_update(int n, const int64_t* d_t_row_ptr, const int* d_t_col,
                                const float* d_t_val, const float* d_e, const float* d_x,
                                const float* d_s, float* d_xx, double* d_partials,
-                               int lanes, raptr_stream_t stream);
+                               int cfg, raptr_stream_t stream);
 int raptr_mlem(int m, int n, const int64_t* d_row_ptr, const int* d_col,
                const float* d_val, const int64_t* d_t_row_ptr,
                const int* d_t_col, const float* d_t_val, const float* d_y,
                const double* d_sens, float* d_x, int niter, int use_graph,
-               void* d_workspace, size_t workspace_bytes,
-               raptr_stream_t stream);
+               int fwd_cfg, int bwd_cfg, void* d_workspace,
+               size_t workspace_bytes, raptr_stream_t stream);
+/* Kernel configurations are codes lanes*16 + unroll: lanes per row
+ * (8, 16, 32, 64) and independent loads in flight per lane (1, 2, 4, 8);
+ * 0 picks one from the mean row length.  The autotuner times every
+ * supported code on the caller's matrix (reps launches each, HIP events)
+ * and returns the fastest forward and backward codes and their times. */
+int raptr_mlem_autotune(int m, int n, const int64_t* d_row_ptr, const int* d_col,
+                        const float* d_val, const int64_t* d_t_row_ptr,
+                        const int* d_t_col, const float* d_t_val, const float* d_y,
+                        const float* d_x, int reps, int* fwd_cfg, int* bwd_cfg,
+                        float* fwd_ms, float* bwd_ms, void* d_workspace,
+                        size_t workspace_bytes, raptr_stream_t stream);
+/* Partial-sum slots d_partials must hold for raptr_mlem_backward_update. */
+int raptr_mlem_partial_slots(int n);
 
 #ifdef __cplusplus
 }

@@ -101,7 +101,11 @@ _SIGS = {
     "raptr_mlem_workspace_size": (sz, [i32, i32]),
     "raptr_mlem_forward_ratio": (i32, [i32, vp, vp, vp, vp, vp, vp, i32, vp]),
     "raptr_mlem_backward_update": (i32, [i32, vp, vp, vp, vp, vp, vp, vp, vp, i32, vp]),
-    "raptr_mlem": (i32, [i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, i32, i32, vp, sz, vp]),
+    "raptr_mlem": (i32, [i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, i32, i32, i32, i32, vp,
+                         sz, vp]),
+    "raptr_mlem_autotune": (i32, [i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, i32, P(i32), P(i32),
+                                  P(f32), P(f32), vp, sz, vp]),
+    "raptr_mlem_partial_slots": (i32, [i32]),
 }
 
 _lib = None

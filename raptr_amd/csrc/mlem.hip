@@ -93,49 +93,84 @@ __global__ void k_sens_norm(const double* sens, const double* total, int n, floa
     s[i] = float(sens[i] / tt);
 }
 
-template <int LANES>
-__global__ __launch_bounds__(256) void k_mlem_fwd(int64_t m, const int64_t* rp, const int* col,
-                                                  const float* val, const float* x,
-                                                  const float* y, float* e) {
-  const int g = threadIdx.x % LANES;
-  const int64_t groups = int64_t(gridDim.x) * (256 / LANES);
-  for (int64_t i = int64_t(blockIdx.x) * (256 / LANES) + threadIdx.x / LANES; i < m;
-       i += groups) {
-    int64_t b = rp[i], en = rp[i + 1];
-    float acc = 0.0f;
-    for (int64_t k = b + g; k < en; k += LANES) acc += val[k] * x[col[k]];
+// ---------------------------------------------------------------------
+// Fused iteration kernels.  LANES lanes of a wave64 share one row (column of
+// A for the backward pass); each lane keeps UNROLL independent value/index
+// loads in flight per step (the matrix stream is latency-bound at one load
+// pair per lane), the matrix is read with non-temporal loads so that it
+// does not evict the gathered vector from L2, and partial sums are combined
+// with wave shuffles in a fixed order (deterministic results).
+template <int LANES, int UNROLL>
+__device__ inline float row_dot(int64_t b, int64_t en, int g, const int* __restrict__ col,
+                                const float* __restrict__ val, const float* __restrict__ x) {
+  float acc = 0.0f;
+  int64_t k = b + g;
+  constexpr int64_t kStep = int64_t(LANES) * UNROLL;
+  for (; k + int64_t(UNROLL - 1) * LANES < en; k += kStep) {
+    float v[UNROLL];
+    int c[UNROLL];
 #pragma unroll
-    for (int off = LANES / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LANES);
-    if (g == 0) e[i] = acc != 0.0f ? y[i] / acc : 0.0f;
+    for (int u = 0; u < UNROLL; ++u) {
+      v[u] = __builtin_nontemporal_load(val + k + u * LANES);
+      c[u] = __builtin_nontemporal_load(col + k + u * LANES);
+    }
+#pragma unroll
+    for (int u = 0; u < UNROLL; ++u) acc = fmaf(v[u], x[c[u]], acc);
   }
+  for (; k < en; k += LANES) acc = fmaf(val[k], x[col[k]], acc);
+#pragma unroll
+  for (int off = LANES / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LANES);
+  return acc;
 }
 
-template <int LANES>
-__global__ __launch_bounds__(256) void k_mlem_bwd(int64_t n, const int64_t* trp,
-                                                  const int* tcol, const float* tval,
-                                                  const float* e, const float* x,
-                                                  const float* s, float* xx,
-                                                  double* partials) {
+// Block index -> logical block such that each XCD (blocks are dealt to the
+// 8 XCDs round robin) works on one contiguous range of logical blocks, so
+// neighbouring columns share their XCD's L2.
+__device__ inline int xcd_block(int b, int G) {
+  constexpr int kXcd = 8;
+  int x = b % kXcd, q = G / kXcd, r = G % kXcd;
+  return x * q + (x < r ? x : r) + b / kXcd;
+}
+
+template <int LANES, int UNROLL>
+__global__ __launch_bounds__(256) void k_mlem_fwd(int64_t m, const int64_t* __restrict__ rp,
+                                                  const int* __restrict__ col,
+                                                  const float* __restrict__ val,
+                                                  const float* __restrict__ x,
+                                                  const float* __restrict__ y,
+                                                  float* __restrict__ e) {
   const int g = threadIdx.x % LANES;
-  const int64_t groups = int64_t(gridDim.x) * (256 / LANES);
+  const int64_t i = int64_t(blockIdx.x) * (256 / LANES) + threadIdx.x / LANES;
+  if (i >= m) return;
+  float acc = row_dot<LANES, UNROLL>(rp[i], rp[i + 1], g, col, val, x);
+  if (g == 0) e[i] = acc != 0.0f ? y[i] / acc : 0.0f;
+}
+
+template <int LANES, int UNROLL>
+__global__ __launch_bounds__(256) void k_mlem_bwd(int64_t n, const int64_t* __restrict__ trp,
+                                                  const int* __restrict__ tcol,
+                                                  const float* __restrict__ tval,
+                                                  const float* __restrict__ e,
+                                                  const float* __restrict__ x,
+                                                  const float* __restrict__ s,
+                                                  float* __restrict__ xx,
+                                                  double* __restrict__ partials) {
+  const int g = threadIdx.x % LANES;
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t j = int64_t(lb) * (256 / LANES) + threadIdx.x / LANES;
   double part = 0.0;
-  for (int64_t j = int64_t(blockIdx.x) * (256 / LANES) + threadIdx.x / LANES; j < n;
-       j += groups) {
-    int64_t b = trp[j], en = trp[j + 1];
-    float acc = 0.0f;
-    for (int64_t k = b + g; k < en; k += LANES) acc += tval[k] * e[tcol[k]];
-#pragma unroll
-    for (int off = LANES / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LANES);
+  if (j < n) {
+    float acc = row_dot<LANES, UNROLL>(trp[j], trp[j + 1], g, tcol, tval, e);
     if (g == 0) {
       float d = s[j];
       d += float(d == 0.0f);
       float v = x[j] * acc / d;
       xx[j] = v;
-      part += double(v);
+      part = double(v);
     }
   }
   double t = block_sum_f64(part);
-  if (threadIdx.x == 0) partials[blockIdx.x] = t;
+  if (threadIdx.x == 0) partials[lb] = t;
 }
 
 __global__ void k_apply_scale(int n, const float* xx, const float* scale, float* x) {
@@ -153,48 +188,78 @@ struct MlemBufs {
   float* scale;
 };
 
+
+// Tuning configuration code: lanes * 16 + unroll (lanes 8..64, unroll 1..8).
+inline int cfg_lanes(int cfg) { return cfg >> 4; }
+inline int cfg_unroll(int cfg) { return cfg & 15; }
+constexpr int kCfgs[] = {8 * 16 + 4, 16 * 16 + 2, 16 * 16 + 4, 32 * 16 + 2, 32 * 16 + 4,
+                         32 * 16 + 8, 64 * 16 + 1, 64 * 16 + 2, 64 * 16 + 4, 64 * 16 + 8};
+
+int default_cfg(double mean) {
+  if (mean < 24) return 8 * 16 + 4;
+  if (mean < 96) return 32 * 16 + 2;
+  return 64 * 16 + 4;
+}
+
+inline unsigned blocks_for(int64_t rows, int lanes) {
+  return unsigned((rows + (256 / lanes) - 1) / (256 / lanes));
+}
+
+int launch_fwd(int cfg, hipStream_t st, int64_t m, const int64_t* rp, const int* col,
+               const float* val, const float* x, const float* y, float* e) {
+  if (m <= 0) return RAPTR_OK;
+  int L = cfg_lanes(cfg), U = cfg_unroll(cfg);
+  unsigned grid = blocks_for(m, L);
+#define RAPTR_FWD(LL, UU)                                                                  \
+  if (L == LL && U == UU) {                                                                \
+    hipLaunchKernelGGL((k_mlem_fwd<LL, UU>), dim3(grid), dim3(256), 0, st, m, rp, col, val, \
+                       x, y, e);                                                           \
+    RAPTR_LAUNCH_CHECK();                                                                  \
+    return RAPTR_OK;                                                                       \
+  }
+  RAPTR_FWD(8, 4) RAPTR_FWD(16, 2) RAPTR_FWD(16, 4) RAPTR_FWD(32, 2) RAPTR_FWD(32, 4)
+  RAPTR_FWD(32, 8) RAPTR_FWD(64, 1) RAPTR_FWD(64, 2) RAPTR_FWD(64, 4) RAPTR_FWD(64, 8)
+#undef RAPTR_FWD
+  return fail(RAPTR_ERR_INVALID_ARG, "unsupported forward configuration %d", cfg);
+}
+
+int launch_bwd(int cfg, hipStream_t st, int64_t n, const int64_t* trp, const int* tcol,
+               const float* tval, const float* e, const float* x, const float* s, float* xx,
+               double* partials, int* nblocks) {
+  int L = cfg_lanes(cfg), U = cfg_unroll(cfg);
+  unsigned grid = blocks_for(n, L);
+  if (nblocks) *nblocks = int(grid);
+#define RAPTR_BWD(LL, UU)                                                                   \
+  if (L == LL && U == UU) {                                                                 \
+    hipLaunchKernelGGL((k_mlem_bwd<LL, UU>), dim3(grid), dim3(256), 0, st, n, trp, tcol,    \
+                       tval, e, x, s, xx, partials);                                        \
+    RAPTR_LAUNCH_CHECK();                                                                   \
+    return RAPTR_OK;                                                                        \
+  }
+  RAPTR_BWD(8, 4) RAPTR_BWD(16, 2) RAPTR_BWD(16, 4) RAPTR_BWD(32, 2) RAPTR_BWD(32, 4)
+  RAPTR_BWD(32, 8) RAPTR_BWD(64, 1) RAPTR_BWD(64, 2) RAPTR_BWD(64, 4) RAPTR_BWD(64, 8)
+#undef RAPTR_BWD
+  return fail(RAPTR_ERR_INVALID_ARG, "unsupported backward configuration %d", cfg);
+}
+
+// partial-sum slots of the backward kernel: one per block at the smallest
+// block width (lanes = 64 -> 4 columns per block)
+inline int64_t partial_slots(int n) { return int64_t(n) / 4 + 8; }
+
 template <class C>
 void carve_mlem(C& c, int m, int n) {
   c.template take<float>(size_t(m > 0 ? m : 1));
   c.template take<float>(size_t(n));
   c.template take<float>(size_t(n));
-  c.template take<double>(kPartials);
+  c.template take<double>(size_t(partial_slots(n) > kPartials ? partial_slots(n) : kPartials));
   c.template take<double>(4);
   c.template take<float>(4);
 }
 
-int launch_fwd(int lanes, unsigned grid, hipStream_t st, int64_t m, const int64_t* rp,
-               const int* col, const float* val, const float* x, const float* y, float* e) {
-#define RAPTR_FWD(L) \
-  hipLaunchKernelGGL(k_mlem_fwd<L>, dim3(grid), dim3(256), 0, st, m, rp, col, val, x, y, e)
-  switch (lanes) {
-    case 4: RAPTR_FWD(4); break;
-    case 8: RAPTR_FWD(8); break;
-    case 16: RAPTR_FWD(16); break;
-    case 32: RAPTR_FWD(32); break;
-    default: RAPTR_FWD(64); break;
-  }
-#undef RAPTR_FWD
-  RAPTR_LAUNCH_CHECK();
-  return RAPTR_OK;
-}
-
-int launch_bwd(int lanes, unsigned grid, hipStream_t st, int64_t n, const int64_t* trp,
-               const int* tcol, const float* tval, const float* e, const float* x,
-               const float* s, float* xx, double* partials) {
-#define RAPTR_BWD(L)                                                                  \
-  hipLaunchKernelGGL(k_mlem_bwd<L>, dim3(grid), dim3(256), 0, st, n, trp, tcol, tval, e, \
-                     x, s, xx, partials)
-  switch (lanes) {
-    case 4: RAPTR_BWD(4); break;
-    case 8: RAPTR_BWD(8); break;
-    case 16: RAPTR_BWD(16); break;
-    case 32: RAPTR_BWD(32); break;
-    default: RAPTR_BWD(64); break;
-  }
-#undef RAPTR_BWD
-  RAPTR_LAUNCH_CHECK();
-  return RAPTR_OK;
+bool known_cfg(int cfg) {
+  for (int c : kCfgs)
+    if (c == cfg) return true;
+  return false;
 }
 
 }  // namespace
@@ -278,28 +343,30 @@ int raptr_column_sums_f64(int n, const int64_t* d_t_row_ptr, const float* d_t_va
 }
 
 int raptr_mlem_forward_ratio(int m, const int64_t* d_rp, const int* d_col, const float* d_val,
-                             const float* d_x, const float* d_y, float* d_e, int lanes,
+                             const float* d_x, const float* d_y, float* d_e, int cfg,
                              raptr_stream_t stream) {
   if (m < 0) return fail(RAPTR_ERR_INVALID_ARG, "negative size");
   if (m == 0) return RAPTR_OK;
   if (!d_rp || !d_x || !d_y || !d_e) return fail(RAPTR_ERR_INVALID_ARG, "null buffer");
-  if (lanes <= 0) lanes = 64;
-  unsigned gf = grid_for(int64_t(m) * lanes, 256, 1 << 18);
-  return launch_fwd(lanes, gf, S(stream), m, d_rp, d_col, d_val, d_x, d_y, d_e);
+  if (cfg == 0) cfg = default_cfg(64.0);
+  if (!known_cfg(cfg)) return fail(RAPTR_ERR_INVALID_ARG, "unsupported configuration %d", cfg);
+  return launch_fwd(cfg, S(stream), m, d_rp, d_col, d_val, d_x, d_y, d_e);
 }
 
 int raptr_mlem_backward_update(int n, const int64_t* d_trp, const int* d_tcol,
                                const float* d_tval, const float* d_e, const float* d_x,
-                               const float* d_s, float* d_xx, double* d_partials, int lanes,
+                               const float* d_s, float* d_xx, double* d_partials, int cfg,
                                raptr_stream_t stream) {
   if (n <= 0) return fail(RAPTR_ERR_INVALID_ARG, "bad size");
   if (!d_trp || !d_e || !d_x || !d_s || !d_xx || !d_partials)
     return fail(RAPTR_ERR_INVALID_ARG, "null buffer");
-  if (lanes <= 0) lanes = 64;
-  unsigned gb = grid_for(int64_t(n) * lanes, 256, kPartials);
-  return launch_bwd(lanes, gb, S(stream), n, d_trp, d_tcol, d_tval, d_e, d_x, d_s, d_xx,
-                    d_partials);
+  if (cfg == 0) cfg = default_cfg(64.0);
+  if (!known_cfg(cfg)) return fail(RAPTR_ERR_INVALID_ARG, "unsupported configuration %d", cfg);
+  return launch_bwd(cfg, S(stream), n, d_trp, d_tcol, d_tval, d_e, d_x, d_s, d_xx, d_partials,
+                    nullptr);
 }
+
+int raptr_mlem_partial_slots(int n) { return int(partial_slots(n)); }
 
 size_t raptr_mlem_workspace_size(int m, int n) {
   Sizer z;
@@ -309,8 +376,8 @@ size_t raptr_mlem_workspace_size(int m, int n) {
 
 int raptr_mlem(int m, int n, const int64_t* d_rp, const int* d_col, const float* d_val,
                const int64_t* d_trp, const int* d_tcol, const float* d_tval, const float* d_y,
-               const double* d_sens, float* d_x, int niter, int use_graph, void* d_ws,
-               size_t ws_bytes, raptr_stream_t stream) {
+               const double* d_sens, float* d_x, int niter, int use_graph, int fwd_cfg,
+               int bwd_cfg, void* d_ws, size_t ws_bytes, raptr_stream_t stream) {
   if (m < 0 || n <= 0 || niter < 0) return fail(RAPTR_ERR_INVALID_ARG, "bad sizes");
   if (!d_rp || !d_trp || !d_y || !d_sens || !d_x)
     return fail(RAPTR_ERR_INVALID_ARG, "null buffer");
@@ -319,7 +386,8 @@ int raptr_mlem(int m, int n, const int64_t* d_rp, const int* d_col, const float*
   B.e = cv.take<float>(size_t(m > 0 ? m : 1));
   B.xx = cv.take<float>(size_t(n));
   B.s = cv.take<float>(size_t(n));
-  B.partials = cv.take<double>(kPartials);
+  B.partials = cv.take<double>(
+      size_t(partial_slots(n) > kPartials ? partial_slots(n) : kPartials));
   B.scal = cv.take<double>(4);
   B.scale = cv.take<float>(4);
   if (!cv.ok)
@@ -332,10 +400,10 @@ int raptr_mlem(int m, int n, const int64_t* d_rp, const int* d_col, const float*
   RAPTR_HIP_TRY(hipMemcpyAsync(&tnnz, d_trp + n, sizeof(int64_t), hipMemcpyDeviceToHost, st));
   RAPTR_HIP_TRY(hipStreamSynchronize(st));
   if (nnz != tnnz) return fail(RAPTR_ERR_INVALID_ARG, "matrix and transpose differ in nnz");
-  int lf = pick_lanes(m ? double(nnz) / m : 0.0);
-  int lb = pick_lanes(double(nnz) / n);
-  unsigned gf = grid_for(int64_t(m) * lf, 256, 1 << 18);
-  unsigned gb = grid_for(int64_t(n) * lb, 256, kPartials);
+  if (fwd_cfg == 0) fwd_cfg = default_cfg(m ? double(nnz) / m : 0.0);
+  if (bwd_cfg == 0) bwd_cfg = default_cfg(double(nnz) / n);
+  if (!known_cfg(fwd_cfg) || !known_cfg(bwd_cfg))
+    return fail(RAPTR_ERR_INVALID_ARG, "unsupported configuration %d/%d", fwd_cfg, bwd_cfg);
   // normalised FP32 sensitivity
   unsigned ns = grid_for(n, kEwBlock, kPartials);
   hipLaunchKernelGGL(k_partial_sums<double>, dim3(ns), dim3(kEwBlock), 0, st, d_sens,
@@ -349,12 +417,12 @@ int raptr_mlem(int m, int n, const int64_t* d_rp, const int* d_col, const float*
   RAPTR_LAUNCH_CHECK();
 
   auto iteration = [&](hipStream_t s2) -> int {
-    if (m > 0)
-      RAPTR_TRY(launch_fwd(lf, gf, s2, m, d_rp, d_col, d_val, d_x, d_y, B.e));
-    RAPTR_TRY(launch_bwd(lb, gb, s2, n, d_trp, d_tcol, d_tval, B.e, d_x, B.s, B.xx,
-                         B.partials));
+    RAPTR_TRY(launch_fwd(fwd_cfg, s2, m, d_rp, d_col, d_val, d_x, d_y, B.e));
+    int gb = 0;
+    RAPTR_TRY(launch_bwd(bwd_cfg, s2, n, d_trp, d_tcol, d_tval, B.e, d_x, B.s, B.xx,
+                         B.partials, &gb));
     hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(kEwBlock), 0, s2, (const double*)B.partials,
-                       int(gb), &B.scal[1], B.scale);
+                       gb, &B.scal[1], B.scale);
     RAPTR_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_apply_scale, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s2, n,
                        (const float*)B.xx, (const float*)B.scale, d_x);
@@ -415,6 +483,63 @@ int raptr_mlem(int m, int n, const int64_t* d_rp, const int* d_col, const float*
   (void)hipEventDestroy(ev);
   (void)hipStreamDestroy(cs);
   return rc;
+}
+
+int raptr_mlem_autotune(int m, int n, const int64_t* d_rp, const int* d_col,
+                        const float* d_val, const int64_t* d_trp, const int* d_tcol,
+                        const float* d_tval, const float* d_y, const float* d_x, int reps,
+                        int* fwd_cfg, int* bwd_cfg, float* fwd_ms, float* bwd_ms, void* d_ws,
+                        size_t ws_bytes, raptr_stream_t stream) {
+  if (!fwd_cfg || !bwd_cfg) return fail(RAPTR_ERR_INVALID_ARG, "null result");
+  if (reps <= 0) reps = 5;
+  Carver cv(d_ws, ws_bytes);
+  float* e = cv.take<float>(size_t(m > 0 ? m : 1));
+  float* xx = cv.take<float>(size_t(n));
+  float* s = cv.take<float>(size_t(n));
+  double* part = cv.take<double>(
+      size_t(partial_slots(n) > kPartials ? partial_slots(n) : kPartials));
+  if (!cv.ok) return fail(RAPTR_ERR_WORKSPACE, "autotune workspace too small");
+  hipStream_t st = S(stream);
+  RAPTR_HIP_TRY(hipMemsetAsync(s, 0, size_t(n) * sizeof(float), st));
+  hipEvent_t e0, e1;
+  RAPTR_HIP_TRY(hipEventCreate(&e0));
+  RAPTR_HIP_TRY(hipEventCreate(&e1));
+  float best_f = 1e30f, best_b = 1e30f;
+  int bf = 0, bb = 0, rc = RAPTR_OK;
+  for (int cfg : kCfgs) {
+    if (m > 0) {
+      rc = launch_fwd(cfg, st, m, d_rp, d_col, d_val, d_x, d_y, e);  // warm
+      if (rc) break;
+      (void)hipEventRecord(e0, st);
+      for (int r = 0; r < reps && !rc; ++r)
+        rc = launch_fwd(cfg, st, m, d_rp, d_col, d_val, d_x, d_y, e);
+      (void)hipEventRecord(e1, st);
+      (void)hipEventSynchronize(e1);
+      float ms = 0;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      ms /= float(reps);
+      if (ms < best_f) best_f = ms, bf = cfg;
+    }
+    rc = launch_bwd(cfg, st, n, d_trp, d_tcol, d_tval, e, d_x, s, xx, part, nullptr);
+    if (rc) break;
+    (void)hipEventRecord(e0, st);
+    for (int r = 0; r < reps && !rc; ++r)
+      rc = launch_bwd(cfg, st, n, d_trp, d_tcol, d_tval, e, d_x, s, xx, part, nullptr);
+    (void)hipEventRecord(e1, st);
+    (void)hipEventSynchronize(e1);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    ms /= float(reps);
+    if (ms < best_b) best_b = ms, bb = cfg;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  *fwd_cfg = bf ? bf : default_cfg(64.0);
+  *bwd_cfg = bb;
+  if (fwd_ms) *fwd_ms = bf ? best_f : 0.0f;
+  if (bwd_ms) *bwd_ms = best_b;
+  return RAPTR_OK;
 }
 
 }  // extern "C"

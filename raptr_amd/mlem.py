@@ -36,7 +36,7 @@ def fsum(x, stream=None):
     return float(out.value)
 
 
-def mlem(A, At, y, sens, x0, niter, use_graph=True, stream=None):
+def mlem(A, At, y, sens, x0, niter, use_graph=True, stream=None, cfg=(0, 0)):
     """Plain MLEM.  A, At: DeviceCSR of the measured rows and its transpose;
     y: counts of the measured rows (host or device float32); sens: FP64
     sensitivity (device); x0: start image (host).  Returns the image."""
@@ -45,8 +45,22 @@ def mlem(A, At, y, sens, x0, niter, use_graph=True, stream=None):
     ws = Workspace(capi.lib().raptr_mlem_workspace_size(A.m, A.n))
     capi.call("raptr_mlem", A.m, A.n, A.row_ptr.ptr, A.col.ptr, A.val.ptr, At.row_ptr.ptr,
               At.col.ptr, At.val.ptr, dy.ptr, sens.ptr, dx.ptr, int(niter),
-              1 if use_graph else 0, ws.ptr, ws.nbytes, stream)
+              1 if use_graph else 0, int(cfg[0]), int(cfg[1]), ws.ptr, ws.nbytes, stream)
     return dx.numpy()
+
+
+def autotune(A, At, y, x, reps=5, stream=None):
+    """Fastest (forward, backward) kernel configuration codes and their
+    times in ms on this matrix."""
+    dy = y if isinstance(y, DeviceArray) else DeviceArray.from_numpy(y, np.float32)
+    dx = x if isinstance(x, DeviceArray) else DeviceArray.from_numpy(x, np.float32)
+    ws = Workspace(capi.lib().raptr_mlem_workspace_size(A.m, A.n))
+    f, b = C.c_int(), C.c_int()
+    fm, bm = C.c_float(), C.c_float()
+    capi.call("raptr_mlem_autotune", A.m, A.n, A.row_ptr.ptr, A.col.ptr, A.val.ptr,
+              At.row_ptr.ptr, At.col.ptr, At.val.ptr, dy.ptr, dx.ptr, int(reps), C.byref(f),
+              C.byref(b), C.byref(fm), C.byref(bm), ws.ptr, ws.nbytes, stream)
+    return (f.value, b.value), (fm.value, bm.value)
 
 
 def uniform_image(nvox):
