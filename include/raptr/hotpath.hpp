@@ -1,0 +1,333 @@
+// raptr/hotpath.hpp -- host mirror of the reference's hot-path call
+// signatures, served by the gfx950 kernels behind include/raptr_hip.h:
+//
+//   getWorkqueue               getWorkqueue.hpp:135-243
+//   getSystemMatrixFromWorkqueue  getSystemMatrixFromWorkqueue.cu:116-122
+//                              (a host launcher here: device pointers in,
+//                              geometry passed explicitly, no __constant__)
+//   cooSort<T>                 cooSort.hpp:75-99
+//   customizeMatDescr / convertCoo2Csr   cusparseWrapper.hpp:36-60
+//   convertCsr2Ecsr            convertCsr2Ecsr.hpp:51-64
+//   CSRmv<T>                   csrmv.hpp:37-99
+//   divides / dividesMultiplies / scales / sum   mlemOperations.hpp:44-169
+//   systemMatrixCalculation    wrappers.hpp:185-266
+//   nInChunk / chunkPtr / nChunks   wrappers.hpp:429-464
+//
+// Error behaviour follows the reference: device errors throw
+// raptr::DeviceError (its cudaException, CUDA_HandleError.hpp:39-53);
+// sparse-library style calls return a status; getWorkqueue reports misuse on
+// stderr and exits (getWorkqueue.hpp:109-111, 156-159).  Pointers named
+// *_devi / documented as device are device memory.  Helpers that the
+// reference gets from thrust (temporary storage) allocate and free their
+// own scratch, as thrust does.
+#pragma once
+
+#include <cstdint>
+#include <cstdlib>
+#include <iostream>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "raptr/geometry.hpp"
+#include "raptr_hip.h"
+
+namespace raptr {
+
+class DeviceError : public std::runtime_error {
+ public:
+  DeviceError(int status, const std::string& where)
+      : std::runtime_error(where + ": " + raptr_status_string(status) + ": " +
+                           raptr_last_error()),
+        status_(status) {}
+  int status() const { return status_; }
+
+ private:
+  int status_;
+};
+
+inline void check(int status, const char* where) {
+  if (status != RAPTR_OK) throw DeviceError(status, where);
+}
+
+// Owning device buffer (host helper; the C ABI itself never allocates).
+template <typename T>
+class DeviceBuffer {
+ public:
+  DeviceBuffer() {}
+  explicit DeviceBuffer(size_t n) : n_(n) {
+    void* p = nullptr;
+    check(raptr_malloc(&p, n * sizeof(T) ? n * sizeof(T) : 1), "raptr_malloc");
+    p_ = static_cast<T*>(p);
+  }
+  DeviceBuffer(const DeviceBuffer&) = delete;
+  DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+  DeviceBuffer(DeviceBuffer&& o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; }
+  ~DeviceBuffer() {
+    if (p_) raptr_free(p_);
+  }
+  T* get() const { return p_; }
+  size_t size() const { return n_; }
+  void upload(const T* h, size_t n) { check(raptr_memcpy_h2d(p_, h, n * sizeof(T), nullptr), "h2d"); }
+  void download(T* h, size_t n) const {
+    check(raptr_memcpy_d2h(h, p_, n * sizeof(T), nullptr), "d2h");
+  }
+
+ private:
+  T* p_ = nullptr;
+  size_t n_ = 0;
+};
+
+// The precompiled kernels implement the Default* plugin family.
+template <typename MS, typename VG>
+void require_default_geometry() {
+  static_assert(std::is_same<MS, DefaultMeasurementSetup<float> >::value,
+                "the gfx950 kernels implement DefaultMeasurementSetup<float>");
+  static_assert(std::is_same<VG, DefaultVoxelGrid<float> >::value,
+                "the gfx950 kernels implement DefaultVoxelGrid<float>");
+}
+
+}  // namespace raptr
+
+// ------------------------------------------------------------ workqueue
+// Resumable scan of (list entry, voxel) pairs from the cursor (listId,
+// vxlId); appends up to n (or all) candidate pairs and advances the cursor.
+template <typename T, typename ConcreteList, typename ConcreteVG, typename ConcreteVGidx,
+          typename ConcreteVGidy, typename ConcreteVGidz, typename ConcreteMS,
+          typename ConcreteMSid0z, typename ConcreteMSid0y, typename ConcreteMSid1z,
+          typename ConcreteMSid1y, typename ConcreteMSida, typename Trafo0, typename Trafo1>
+int getWorkqueue(std::vector<int>& returnCnlId, std::vector<int>& returnVxlId, int& listId,
+                 int& vxlId, ConcreteList const* const list, ConcreteVG const* const grid,
+                 ConcreteMS const* const setup, int const n = -1) {
+  raptr::require_default_geometry<ConcreteMS, ConcreteVG>();
+  if (!returnCnlId.empty() || !returnVxlId.empty()) {
+    std::cerr << "raptr: getWorkqueue needs empty output vectors" << std::endl;
+    std::exit(EXIT_FAILURE);
+  }
+  const int size = list->size();
+  raptr_setup_t s = setup->to_c();
+  raptr_grid_t g = grid->to_c();
+  if (vxlId < 0 || vxlId >= grid->nVoxels()) {
+    std::cerr << "raptr: getWorkqueue cursor voxel " << vxlId << " out of range" << std::endl;
+    std::exit(EXIT_FAILURE);
+  }
+  if (listId >= size) return 0;
+  raptr::DeviceBuffer<int> d_list(size_t(size) + 1);
+  d_list.upload(list->cnlData(), size_t(size));
+  std::vector<int64_t> counts(size_t(size - listId));
+  raptr::DeviceBuffer<int64_t> d_counts(counts.size());
+  raptr::check(raptr_workqueue_counts(&s, &g, d_list.get() + listId, size - listId,
+                                      d_counts.get(), nullptr),
+               "raptr_workqueue_counts");
+  d_counts.download(counts.data(), counts.size());
+  int64_t cap = 0;
+  for (int64_t c : counts) cap += c;
+  if (n >= 0 && cap > n) cap = n;
+  raptr::DeviceBuffer<int> d_cnl(size_t(cap) + 1), d_vxl(size_t(cap) + 1);
+  size_t wsb = raptr_workqueue_workspace_size(&g, size);
+  raptr::DeviceBuffer<char> ws(wsb);
+  int64_t found = 0;
+  raptr::check(raptr_workqueue(&s, &g, d_list.get(), size, &listId, &vxlId, n, d_cnl.get(),
+                               d_vxl.get(), cap, &found, ws.get(), wsb, nullptr),
+               "raptr_workqueue");
+  returnCnlId.resize(size_t(found));
+  returnVxlId.resize(size_t(found));
+  if (found) {
+    d_cnl.download(returnCnlId.data(), size_t(found));
+    d_vxl.download(returnVxlId.data(), size_t(found));
+  }
+  return int(found);
+}
+
+// ---------------------------------------------- matrix from workqueue
+// Device pointers cnl, vxl (wqLength entries) in, val out.
+template <typename T, typename ConcreteVG, typename ConcreteVGIdx, typename ConcreteVGIdy,
+          typename ConcreteVGIdz, typename ConcreteMS, typename ConcreteMSid0z,
+          typename ConcreteMSid0y, typename ConcreteMSid1z, typename ConcreteMSid1y,
+          typename ConcreteMSida, typename Trafo0, typename Trafo1>
+void getSystemMatrixFromWorkqueue(int const* const globalCnl, int const* const globalVxl,
+                                  T* const globalVal, int const wqLength, int const nrays,
+                                  ConcreteVG const& grid, ConcreteMS const& setup,
+                                  uint32_t seed = 1, int raygen = RAPTR_RAYGEN_RAND,
+                                  raptr_stream_t stream = nullptr) {
+  static_assert(std::is_same<T, float>::value, "matrix values are float");
+  raptr::require_default_geometry<ConcreteMS, ConcreteVG>();
+  raptr_setup_t s = setup.to_c();
+  raptr_grid_t g = grid.to_c();
+  raptr::check(raptr_sm_from_wq(&s, &g, globalCnl, globalVxl, globalVal, wqLength, nrays,
+                                seed, raygen, stream),
+               "raptr_sm_from_wq");
+}
+
+// ------------------------------------------------------- sparse formats
+template <typename T>
+void cooSort(T* const cooVal, int* const cooRowId, int* const cooColId, int const N) {
+  static_assert(std::is_same<T, float>::value, "COO values are float");
+  if (N <= 0 || !cooVal) return;
+  size_t wsb = raptr_coo_sort_workspace_size(N);
+  raptr::DeviceBuffer<char> ws(wsb);
+  raptr::check(raptr_coo_sort(cooVal, cooRowId, cooColId, N, ws.get(), wsb, nullptr),
+               "raptr_coo_sort");
+}
+
+// Sparse "library" handle and descriptor of this project (no CUDA types).
+struct raptr_sparse_context {
+  raptr_stream_t stream = nullptr;
+};
+typedef raptr_sparse_context* raptr_sparse_handle_t;
+struct raptr_mat_descr {
+  int general = 1;
+  int index_base = 0;
+};
+typedef raptr_mat_descr* raptr_mat_descr_t;
+enum raptr_sparse_status_t { RAPTR_SPARSE_STATUS_SUCCESS = 0, RAPTR_SPARSE_STATUS_ERROR = 1 };
+enum raptr_operation_t {
+  RAPTR_SPARSE_OPERATION_NON_TRANSPOSE = RAPTR_OP_NON_TRANSPOSE,
+  RAPTR_SPARSE_OPERATION_TRANSPOSE = RAPTR_OP_TRANSPOSE
+};
+
+inline raptr_sparse_status_t raptr_sparse_create(raptr_sparse_handle_t* h) {
+  *h = new raptr_sparse_context();
+  return RAPTR_SPARSE_STATUS_SUCCESS;
+}
+inline void raptr_sparse_destroy(raptr_sparse_handle_t h) { delete h; }
+inline raptr_sparse_status_t raptr_sparse_create_descr(raptr_mat_descr_t* d) {
+  *d = new raptr_mat_descr();
+  return RAPTR_SPARSE_STATUS_SUCCESS;
+}
+inline void raptr_sparse_destroy_descr(raptr_mat_descr_t d) { delete d; }
+
+// general matrix, zero-based indices
+inline raptr_sparse_status_t customizeMatDescr(raptr_mat_descr_t& descr,
+                                               raptr_sparse_handle_t const& handle) {
+  (void)handle;
+  descr->general = 1;
+  descr->index_base = 0;
+  return RAPTR_SPARSE_STATUS_SUCCESS;
+}
+
+inline raptr_sparse_status_t convertCoo2Csr(int* const csrRowPtr, int const* const cooRowId,
+                                            raptr_sparse_handle_t const& handle,
+                                            int const nnz, int const m) {
+  int st = raptr_coo2csr(csrRowPtr, cooRowId, nnz, m, handle ? handle->stream : nullptr);
+  return st == RAPTR_OK ? RAPTR_SPARSE_STATUS_SUCCESS : RAPTR_SPARSE_STATUS_ERROR;
+}
+
+inline void convertCsr2Ecsr(int* const ecsrRowPtr, int* const yRowId, int const nnzr,
+                            int* const csrRowPtr, int const m) {
+  raptr::check(raptr_csr2ecsr(ecsrRowPtr, yRowId, nnzr, csrRowPtr, m, nullptr),
+               "raptr_csr2ecsr");
+}
+
+// y = alpha*op(A)*x + beta*y; alpha and beta are host pointers
+template <typename T>
+struct CSRmv;
+
+template <>
+struct CSRmv<float> {
+  raptr_sparse_status_t operator()(raptr_sparse_handle_t handle, raptr_operation_t transA,
+                                   int m, int n, int nnz, float* const alpha,
+                                   raptr_mat_descr_t descrA, float* const csrValA,
+                                   int* const csrRowPtrA, int* const csrColIdA, float* const x,
+                                   float* const beta, float* const y) {
+    (void)descrA;
+    int st = raptr_csrmv_f32(int(transA), m, n, nnz, *alpha, csrValA, csrRowPtrA, csrColIdA, x,
+                             *beta, y, handle ? handle->stream : nullptr);
+    return st == RAPTR_OK ? RAPTR_SPARSE_STATUS_SUCCESS : RAPTR_SPARSE_STATUS_ERROR;
+  }
+};
+
+template <>
+struct CSRmv<double> {
+  raptr_sparse_status_t operator()(raptr_sparse_handle_t handle, raptr_operation_t transA,
+                                   int m, int n, int nnz, double* const alpha,
+                                   raptr_mat_descr_t descrA, double* const csrValA,
+                                   int* const csrRowPtrA, int* const csrColIdA, double* const x,
+                                   double* const beta, double* const y) {
+    (void)descrA;
+    int st = raptr_csrmv_f64(int(transA), m, n, nnz, *alpha, csrValA, csrRowPtrA, csrColIdA, x,
+                             *beta, y, handle ? handle->stream : nullptr);
+    return st == RAPTR_OK ? RAPTR_SPARSE_STATUS_SUCCESS : RAPTR_SPARSE_STATUS_ERROR;
+  }
+};
+
+// ----------------------------------------------------------- MLEM ops
+template <typename T>
+void divides(T* const& result, T* const& dividend, T* const& divisor, int const n) {
+  static_assert(std::is_same<T, float>::value, "float vectors");
+  raptr::check(raptr_divides(result, dividend, divisor, n, nullptr), "raptr_divides");
+}
+
+template <typename T>
+void dividesMultiplies(T* const& result, T* const& a, T* const& b, T* const& c, int const n) {
+  static_assert(std::is_same<T, float>::value, "float vectors");
+  raptr::check(raptr_divides_multiplies(result, a, b, c, n, nullptr),
+               "raptr_divides_multiplies");
+}
+
+template <typename T>
+void scales(T* const& x, T const alpha, int const n) {
+  static_assert(std::is_same<T, float>::value, "float vectors");
+  raptr::check(raptr_scales(x, alpha, n, nullptr), "raptr_scales");
+}
+
+template <typename T>
+T sum(T* const& x, int const n) {
+  static_assert(std::is_same<T, float>::value, "float vectors");
+  size_t wsb = raptr_sum_workspace_size(n);
+  raptr::DeviceBuffer<char> ws(wsb);
+  float out = 0.0f;
+  raptr::check(raptr_sum(x, n, &out, ws.get(), wsb, nullptr), "raptr_sum");
+  return out;
+}
+
+// ------------------------------------------------------- chunk math
+template <typename ListSizeType, typename ChunkGridSizeType>
+ListSizeType nInChunk(ChunkGridSizeType idChunk, ListSizeType const nAll,
+                      ListSizeType const nFullChunk) {
+  ChunkGridSizeType nFull = ChunkGridSizeType(nAll / nFullChunk);
+  ChunkGridSizeType nTotal = ChunkGridSizeType((nAll + nFullChunk - 1) / nFullChunk);
+  if (idChunk < nFull) return nFullChunk;
+  if (idChunk < nTotal) return nAll % nFullChunk;
+  return 0;
+}
+
+template <typename ListSizeType, typename ChunkGridSizeType>
+ListSizeType chunkPtr(ChunkGridSizeType const chunkId, ListSizeType const nFullChunk) {
+  return ListSizeType(chunkId) * nFullChunk;
+}
+
+template <typename ChunkGridSizeType, typename MemArrSizeType>
+ChunkGridSizeType nChunks(MemArrSizeType const maxNnz, MemArrSizeType const maxNChunk) {
+  return ChunkGridSizeType((maxNnz + maxNChunk - 1) / maxNChunk);
+}
+
+// ------------------------------------------- system matrix of a chunk
+// The matrix of the nY listed channels (device yRowId_devi) as ECSR:
+// aEcsrCnlPtr_devi[nY+1] (int32 row pointers), aVxlId_devi / aVal_devi
+// (capacity entries), *nnz_host receives the count.  Zero-valued candidates
+// are not stored (the reference stores them); the non-zero values are
+// identical.
+inline void systemMatrixCalculation(int* aEcsrCnlPtr_devi, int* aVxlId_devi, float* aVal_devi,
+                                    int64_t capacity, int64_t* nnz_host,
+                                    const int* yRowId_devi, int nY,
+                                    const DefaultMeasurementSetup<float>& setup,
+                                    const DefaultVoxelGrid<float>& grid, int nrays,
+                                    uint32_t seed = 1, raptr_stream_t stream = nullptr) {
+  raptr_setup_t s = setup.to_c();
+  raptr_grid_t g = grid.to_c();
+  raptr::DeviceBuffer<int64_t> rp(size_t(nY) + 1);
+  size_t wsb = raptr_sm_build_workspace_size(&g, nrays, 1 << 22);
+  raptr::DeviceBuffer<char> ws(wsb);
+  raptr::check(raptr_sm_build_csr(&s, &g, yRowId_devi, nY, nrays, seed, RAPTR_RAYGEN_RAND,
+                                  rp.get(), aVxlId_devi, aVal_devi, capacity, nnz_host,
+                                  ws.get(), wsb, stream),
+               "raptr_sm_build_csr");
+  if (*nnz_host > INT32_MAX) throw std::runtime_error("ECSR chunk exceeds int32 pointers");
+  std::vector<int64_t> h(size_t(nY) + 1);
+  rp.download(h.data(), h.size());
+  std::vector<int> e(h.begin(), h.end());
+  raptr::check(raptr_memcpy_h2d(aEcsrCnlPtr_devi, e.data(), e.size() * sizeof(int), stream),
+               "h2d");
+}

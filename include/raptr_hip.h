@@ -266,6 +266,47 @@ int raptr_mlem_autotune(int m, int n, const int64_t* d_row_ptr, const int* d_col
 /* Partial-sum slots d_partials must hold for raptr_mlem_backward_update. */
 int raptr_mlem_partial_slots(int n);
 
+/* ------------------------- reconstruction plan (plain, batched, mixed) */
+/* A device-resident MLEM plan built from the CSR of the measured channels
+ * (64-bit row pointers, ascending columns) and the FP64 sensitivity.  The
+ * plan owns two derived layouts: a row-padded CSR (16-byte aligned rows,
+ * vector loads) for the forward pass and the transpose cut into row blocks
+ * of block_rows channels (0: sized so the block's ratio slice fits one
+ * XCD's L2) for the atomic-free backward pass.  value_type: 0 = FP32,
+ * 1 = FP16, 2 = BF16 (of value/row maximum), 3 = UNORM16 (fixed point
+ * round(65535*value/row maximum)); 16-bit types carry a per-row FP32
+ * scale and accumulate in FP32.  max_frames: batch capacity (power of two <= 64).  The
+ * plan allocates its own device memory (raptr_recon_info reports it). */
+typedef struct raptr_recon raptr_recon;
+enum { RAPTR_VALUE_F32 = 0, RAPTR_VALUE_F16 = 1, RAPTR_VALUE_BF16 = 2, RAPTR_VALUE_U16 = 3 };
+int raptr_recon_create(raptr_recon** plan, int m, int n, int64_t nnz,
+                       const int64_t* d_row_ptr, const int* d_col,
+                       const float* d_val, const double* d_sens, int value_type,
+                       int max_frames, int block_rows, raptr_stream_t stream);
+int raptr_recon_destroy(raptr_recon* plan);
+int raptr_recon_info(const raptr_recon* plan, int64_t* device_bytes,
+                     int64_t* padded_nnz, int* row_blocks, int* fwd_lanes,
+                     int* bwd_unroll);
+/* Kernel shape: lanes per forward row (16, 32, 64; single frame) and
+ * independent loads per lane of the backward pass (2, 4, 8). */
+int raptr_recon_set_config(raptr_recon* plan, int fwd_lanes, int bwd_unroll);
+/* niter MLEM iterations (reco.cu:230-303) of nframes frames at once
+ * (power of two <= max_frames).  d_y: counts [m][nframes]; d_x: images
+ * [n][nframes], start images in, results out.  Per frame and iteration:
+ * yt = A x, e = y/yt (0 where y or yt is 0), c = A^T e,
+ * x = x*c/(s + (s==0)), x *= 1/sum(x) (FP64 sum).  With use_graph and a
+ * non-null stream the iteration is captured once into a HIP graph (cached
+ * per frame count and buffers) and replayed. */
+int raptr_recon_mlem(raptr_recon* plan, int nframes, const float* d_y, float* d_x,
+                     int niter, int use_graph, raptr_stream_t stream);
+/* The two passes alone (timing, tests): forward + ratio into d_e
+ * ([m][nframes]); backward over all row blocks with the update into the
+ * plan's internal image buffer. */
+int raptr_recon_forward(raptr_recon* plan, int nframes, const float* d_x,
+                        const float* d_y, float* d_e, raptr_stream_t stream);
+int raptr_recon_backward(raptr_recon* plan, int nframes, const float* d_x,
+                         raptr_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -766,6 +766,12 @@ void ro_mlem_f64(int m, int nvox, const int64_t* rp, const int* col,
   double* s = (double*)malloc(sizeof(double) * nvox);
   double* c = (double*)malloc(sizeof(double) * nvox);
   double* e = (double*)malloc(sizeof(double) * (m > 0 ? m : 1));
+  int nt = 1;
+#ifdef _OPENMP
+  nt = omp_get_max_threads();
+#endif
+  /* per-thread partial back-projections, reduced in thread order */
+  double* part = (double*)malloc(sizeof(double) * (size_t)nvox * (size_t)nt);
   double ss = 0.0;
   for (int j = 0; j < nvox; ++j) ss += sens[j];
   for (int j = 0; j < nvox; ++j) s[j] = sens[j] / ss;
@@ -776,9 +782,25 @@ void ro_mlem_f64(int m, int nvox, const int64_t* rp, const int* col,
       for (int64_t k = rp[i]; k < rp[i + 1]; ++k) acc += val[k] * x[col[k]];
       e[i] = acc != 0.0 ? y[i] / acc : 0.0;
     }
-    for (int j = 0; j < nvox; ++j) c[j] = 0.0;
-    for (int i = 0; i < m; ++i)
-      for (int64_t k = rp[i]; k < rp[i + 1]; ++k) c[col[k]] += val[k] * e[i];
+#pragma omp parallel num_threads(nt)
+    {
+      int t = 0, T = 1;
+#ifdef _OPENMP
+      t = omp_get_thread_num();
+      T = omp_get_num_threads();
+#endif
+      double* p = part + (size_t)nvox * (size_t)t;
+      for (int j = 0; j < nvox; ++j) p[j] = 0.0;
+      int64_t lo = (int64_t)m * t / T, hi = (int64_t)m * (t + 1) / T;
+      for (int64_t i = lo; i < hi; ++i)
+        for (int64_t k = rp[i]; k < rp[i + 1]; ++k) p[col[k]] += val[k] * e[i];
+    }
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < nvox; ++j) {
+      double acc = 0.0;
+      for (int t = 0; t < nt; ++t) acc += part[(size_t)nvox * (size_t)t + j];
+      c[j] = acc;
+    }
     double norm = 0.0;
     for (int j = 0; j < nvox; ++j) {
       double d = s[j];
@@ -788,6 +810,7 @@ void ro_mlem_f64(int m, int nvox, const int64_t* rp, const int* col,
     }
     for (int j = 0; j < nvox; ++j) x[j] = x[j] * (1.0 / norm);
   }
+  free(part);
   free(s);
   free(c);
   free(e);

@@ -70,7 +70,24 @@ def build(verbose=False, jobs=None):
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_cpp_tests()
     return LIB
+
+
+CPP_TESTS = [os.path.join(ROOT, "tests", "cpp", "test_reference_api.cpp")]
+
+
+def build_cpp_tests():
+    """C++ programs written against include/raptr/*.hpp, linked to the HIP
+    library (rpath to raptr_amd/lib)."""
+    for src in CPP_TESTS:
+        out = os.path.join(LIBDIR, os.path.basename(src)[:-4])
+        cmd = ["g++", "-O2", "-std=c++17", "-Wall", f"-I{os.path.join(ROOT, 'include')}", src,
+               "-o", out + ".tmp", f"-L{LIBDIR}", "-lraptr_hip", "-Wl,-rpath,$ORIGIN"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"g++ failed for {src}:\n{r.stdout}\n{r.stderr}")
+        os.replace(out + ".tmp", out)
 
 
 if __name__ == "__main__":

@@ -88,4 +88,15 @@ int scan_exclusive_u32(const uint32_t* in, uint32_t* out, int64_t n,
                        uint32_t* d_total, void* ws, size_t ws_bytes,
                        hipStream_t st);
 
+// Stable LSD radix sort of (key, value) pairs by bits [0, bits) of the key;
+// ping-pong buffers k0/k1, v0/v1; histogram and scan scratch come from cv.
+// *result = 0 or 1: which buffer pair holds the sorted data.
+int radix_sort_pairs(uint64_t* k0, uint64_t* k1, uint32_t* v0, uint32_t* v1, int64_t n,
+                     int bits, hipStream_t st, int* result, Carver& cv);
+
+// Native vector types (accepted by the non-temporal load builtin).
+typedef float vf4 __attribute__((ext_vector_type(4)));
+typedef int vi4 __attribute__((ext_vector_type(4)));
+typedef unsigned int vu2 __attribute__((ext_vector_type(2)));
+
 }  // namespace raptr

@@ -1,0 +1,756 @@
+// recon.hip -- device-resident MLEM reconstruction plan on gfx950.
+//
+// A plan holds the system matrix of the measured channels in two layouts
+// tuned for the two passes of an MLEM iteration (reco.cu:246-303):
+//  * forward: CSR with every row padded to a multiple of 4 entries and
+//    16-byte aligned, so each lane streams 4 values and 4 column indices per
+//    load (row sums are short, ~360 entries, and scalar traversal loses a
+//    quarter of the bandwidth to row overheads);
+//  * backward: the transpose cut into row blocks (channel ranges) whose ratio
+//    vector e fits in one XCD's L2.  Each block is one launch, so at any time
+//    all waves gather from the same L2-sized slice of e; the column sums are
+//    carried across blocks in a fixed order (deterministic).
+// Values are FP32, or 16-bit with a per-row FP32 scale: FP16 or BF16 of
+// value/rowmax, or unsigned 16-bit fixed point round(65535*value/rowmax)
+// (absolute error <= rowmax/131070, uniform over the row).  The scale is
+// applied once per row in the forward pass and folded into the ratio
+// (e'_i = scale_i * e_i) for the backward pass.  Vectors hold F frames
+// per index ([index][frame]); one launch handles all frames, so the matrix
+// stream is shared by the batch.  The normalisation sums are FP64.
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
+
+#include <algorithm>
+#include <map>
+#include <vector>
+
+#include "primitives.hpp"
+
+namespace raptr {
+namespace {
+
+
+// --------------------------------------------------------- value types
+enum { kF32 = 0, kF16 = 1, kBF16 = 2, kU16 = 3 };
+
+__device__ __forceinline__ float h2f(uint16_t h, int vt) {
+  if (vt == kF16) return __half2float(__ushort_as_half(h));
+  if (vt == kU16) return float(h);
+  return __uint_as_float(uint32_t(h) << 16);
+}
+
+// f in [0, 1] (value / row maximum)
+__device__ __forceinline__ uint16_t f2h(float f, int vt) {
+  if (vt == kF16) return __half_as_ushort(__float2half_rn(f));
+  if (vt == kU16) return uint16_t(fminf(rintf(f * 65535.0f), 65535.0f));
+  // BF16, round to nearest even
+  uint32_t u = __float_as_uint(f);
+  uint32_t r = 0x7fffu + ((u >> 16) & 1u);
+  return uint16_t((u + r) >> 16);
+}
+
+template <int VT>
+struct Val;
+template <>
+struct Val<kF32> {
+  using T = float;
+  __device__ static float get(const float* p, int64_t k) { return __builtin_nontemporal_load(p + k); }
+};
+template <>
+struct Val<kF16> {
+  using T = uint16_t;
+  __device__ static float get(const uint16_t* p, int64_t k) {
+    return __half2float(__ushort_as_half(__builtin_nontemporal_load(p + k)));
+  }
+};
+template <>
+struct Val<kBF16> {
+  using T = uint16_t;
+  __device__ static float get(const uint16_t* p, int64_t k) {
+    return __uint_as_float(uint32_t(__builtin_nontemporal_load(p + k)) << 16);
+  }
+};
+template <>
+struct Val<kU16> {
+  using T = uint16_t;
+  __device__ static float get(const uint16_t* p, int64_t k) {
+    return float(__builtin_nontemporal_load(p + k));
+  }
+};
+
+// ------------------------------------------------- layout construction
+__global__ void k_padded_len(const int64_t* rp, int m, int64_t* plen) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x)
+    plen[i] = (rp[i + 1] - rp[i] + 3) & ~int64_t(3);
+}
+
+__global__ void k_set_last(int64_t* p, int m, const int64_t* total) { p[m] = *total; }
+
+// One wave per row: row maximum -> scale (mixed), copy with zero padding.
+__global__ __launch_bounds__(256) void k_pad_copy(const int64_t* rp, const int* col,
+                                                  const float* val, int m, int vt,
+                                                  const int64_t* prp, int* pcol, void* pval,
+                                                  float* scale) {
+  int64_t wid = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
+  int lane = threadIdx.x & (kWave - 1);
+  int64_t nw = int64_t(gridDim.x) * blockDim.x / kWave;
+  for (int64_t i = wid; i < m; i += nw) {
+    int64_t b = rp[i], e = rp[i + 1], pb = prp[i], pe = prp[i + 1];
+    float mx = 0.0f;
+    if (vt != kF32) {
+      for (int64_t k = b + lane; k < e; k += kWave) mx = fmaxf(mx, fabsf(val[k]));
+      for (int off = kWave / 2; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, kWave));
+    }
+    float sc = mx > 0.0f ? mx : 1.0f;
+    if (vt != kF32 && lane == 0) scale[i] = vt == kU16 ? sc * (1.0f / 65535.0f) : sc;
+    int lastc = e > b ? col[e - 1] : 0;
+    for (int64_t k = lane; k < pe - pb; k += kWave) {
+      bool real = b + k < e;
+      int c = real ? col[b + k] : lastc;
+      float v = real ? val[b + k] : 0.0f;
+      pcol[pb + k] = c;
+      if (vt == kF32)
+        static_cast<float*>(pval)[pb + k] = v;
+      else
+        static_cast<uint16_t*>(pval)[pb + k] = f2h(v / sc, vt);
+    }
+  }
+}
+
+// transpose keys: (row block << colbits) | col, payload = entry index
+__global__ void k_bt_keys(const int64_t* rp, const int* col, int m, int blk_rows, int colbits,
+                          uint64_t* keys, uint32_t* idx) {
+  int64_t wid = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
+  int lane = threadIdx.x & (kWave - 1);
+  int64_t nw = int64_t(gridDim.x) * blockDim.x / kWave;
+  for (int64_t i = wid; i < m; i += nw) {
+    uint64_t hi = uint64_t(i / blk_rows) << colbits;
+    for (int64_t k = rp[i] + lane; k < rp[i + 1]; k += kWave) {
+      keys[k] = hi | uint64_t(uint32_t(col[k]));
+      idx[k] = uint32_t(k);
+    }
+  }
+}
+
+__global__ void k_expand_rows32(const int64_t* rp, int m, uint32_t* rowid) {
+  int64_t wid = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
+  int lane = threadIdx.x & (kWave - 1);
+  int64_t nw = int64_t(gridDim.x) * blockDim.x / kWave;
+  for (int64_t i = wid; i < m; i += nw)
+    for (int64_t k = rp[i] + lane; k < rp[i + 1]; k += kWave) rowid[k] = uint32_t(i);
+}
+
+__global__ void k_bt_gather(const uint32_t* idx, int64_t nnz, const uint32_t* rowid,
+                            const float* val, const float* scale, int vt, int* brow,
+                            void* bval) {
+  for (int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; j < nnz;
+       j += int64_t(gridDim.x) * blockDim.x) {
+    uint32_t k = idx[j];
+    uint32_t r = rowid[k];
+    brow[j] = int(r);
+    float v = val[k];
+    if (vt == kF32)
+      static_cast<float*>(bval)[j] = v;
+    else
+      static_cast<uint16_t*>(bval)[j] =
+          f2h(vt == kU16 ? v / (scale[r] * 65535.0f) : v / scale[r], vt);
+  }
+}
+
+// segment pointers of the blocked transpose: ptr[b*(n+1) + j] = first entry
+// with key >= (b << colbits | j)
+__global__ void k_bt_ptr(const uint64_t* keys, int64_t nnz, int nblk, int n, int colbits,
+                         int64_t* ptr) {
+  int64_t total = int64_t(nblk) * (n + 1);
+  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < total;
+       t += int64_t(gridDim.x) * blockDim.x) {
+    int64_t b = t / (n + 1), j = t - b * (n + 1);
+    uint64_t key = (uint64_t(b) << colbits) + uint64_t(j);
+    int64_t lo = 0, hi = nnz;
+    while (lo < hi) {
+      int64_t mid = lo + ((hi - lo) >> 1);
+      if (keys[mid] < key) lo = mid + 1; else hi = mid;
+    }
+    ptr[t] = lo;
+  }
+}
+
+// ------------------------------------------------------ MLEM kernels
+// Frames per launch FR (1 or a power of two up to 64).  A wave handles one
+// row (forward) / column (backward); its 64 lanes form 64/FR entry slots
+// x FR frames.  F = 1 uses vector loads of 4 entries per lane (forward).
+
+template <int VT, int LANES>
+__global__ __launch_bounds__(256) void k_fwd1(int64_t m, const int64_t* __restrict__ rp,
+                                              const int* __restrict__ col,
+                                              const typename Val<VT>::T* __restrict__ val,
+                                              const float* __restrict__ rscale,
+                                              const float* __restrict__ x,
+                                              const float* __restrict__ y,
+                                              float* __restrict__ e) {
+  const int g = threadIdx.x % LANES;
+  const int64_t i = int64_t(blockIdx.x) * (256 / LANES) + threadIdx.x / LANES;
+  if (i >= m) return;
+  const int64_t b = rp[i], en = rp[i + 1];  // multiples of 4
+  float acc = 0.0f;
+  for (int64_t k = b + 4 * g; k < en; k += 4 * LANES) {
+    vi4 c = __builtin_nontemporal_load(reinterpret_cast<const vi4*>(col + k));
+    float v0, v1, v2, v3;
+    if (VT == kF32) {
+      vf4 v = __builtin_nontemporal_load(reinterpret_cast<const vf4*>(
+          reinterpret_cast<const float*>(val) + k));
+      v0 = v.x, v1 = v.y, v2 = v.z, v3 = v.w;
+    } else {
+      vu2 h = __builtin_nontemporal_load(reinterpret_cast<const vu2*>(
+          reinterpret_cast<const uint16_t*>(val) + k));
+      v0 = h2f(uint16_t(h.x & 0xffff), VT), v1 = h2f(uint16_t(h.x >> 16), VT);
+      v2 = h2f(uint16_t(h.y & 0xffff), VT), v3 = h2f(uint16_t(h.y >> 16), VT);
+    }
+    acc = fmaf(v0, x[c.x], acc);
+    acc = fmaf(v1, x[c.y], acc);
+    acc = fmaf(v2, x[c.z], acc);
+    acc = fmaf(v3, x[c.w], acc);
+  }
+#pragma unroll
+  for (int off = LANES / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LANES);
+  if (g == 0) {
+    float sc = VT == kF32 ? 1.0f : rscale[i];
+    float yt = sc * acc;
+    float r = yt != 0.0f ? y[i] / yt : 0.0f;
+    e[i] = VT == kF32 ? r : sc * r;  // ratio folded with the row scale
+  }
+}
+
+template <int VT, int FR>
+__global__ __launch_bounds__(256) void k_fwdF(int64_t m, const int64_t* __restrict__ rp,
+                                              const int* __restrict__ col,
+                                              const typename Val<VT>::T* __restrict__ val,
+                                              const float* __restrict__ rscale,
+                                              const float* __restrict__ x,
+                                              const float* __restrict__ y,
+                                              float* __restrict__ e) {
+  constexpr int SLOTS = kWave / FR;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int f = lane % FR, slot = lane / FR;
+  const int64_t i = int64_t(blockIdx.x) * (256 / kWave) + threadIdx.x / kWave;
+  if (i >= m) return;
+  const int64_t b = rp[i], en = rp[i + 1];
+  float acc = 0.0f;
+  for (int64_t k = b + slot; k < en; k += SLOTS) {
+    float v = Val<VT>::get(val, k);
+    int c = __builtin_nontemporal_load(col + k);
+    acc = fmaf(v, x[int64_t(c) * FR + f], acc);
+  }
+#pragma unroll
+  for (int off = kWave / 2; off >= FR; off >>= 1) acc += __shfl_xor(acc, off, kWave);
+  if (slot == 0) {
+    float sc = VT == kF32 ? 1.0f : rscale[i];
+    float yt = sc * acc;
+    float yy = y[i * FR + f];
+    float r = (yt != 0.0f && yy != 0.0f) ? yy / yt : 0.0f;
+    e[i * FR + f] = VT == kF32 ? r : sc * r;
+  }
+}
+
+// Backward over one row block: c[j][f] (+)= sum over the block's entries of
+// column j of v * e'[row][f]; the last block applies the update
+// xx = x * c / (s + (s==0)) and writes FP64 partial sums per block/frame.
+template <int VT, int FR, int UNROLL>
+__global__ __launch_bounds__(256) void k_bwd(int n, const int64_t* __restrict__ ptr,
+                                             const int* __restrict__ brow,
+                                             const typename Val<VT>::T* __restrict__ val,
+                                             const float* __restrict__ e, float* __restrict__ c,
+                                             int first, int last, const float* __restrict__ x,
+                                             const float* __restrict__ s,
+                                             float* __restrict__ xx,
+                                             double* __restrict__ partials) {
+  constexpr int SLOTS = kWave / FR;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int f = lane % FR, slot = lane / FR;
+  const int w = threadIdx.x / kWave;
+  // XCD-contiguous column ranges
+  const int G = gridDim.x, bx = blockIdx.x;
+  const int xcd = bx % 8, q = G / 8, r8 = G % 8;
+  const int lb = xcd * q + (xcd < r8 ? xcd : r8) + bx / 8;
+  const int j = lb * (256 / kWave) + w;
+  float acc = 0.0f;
+  if (j < n) {
+    const int64_t b = ptr[j], en = ptr[j + 1];
+    int64_t k = b + slot;
+    for (; k + int64_t(UNROLL - 1) * SLOTS < en; k += int64_t(UNROLL) * SLOTS) {
+      float v[UNROLL];
+      int rr[UNROLL];
+#pragma unroll
+      for (int u = 0; u < UNROLL; ++u) {
+        v[u] = Val<VT>::get(val, k + u * SLOTS);
+        rr[u] = __builtin_nontemporal_load(brow + k + u * SLOTS);
+      }
+#pragma unroll
+      for (int u = 0; u < UNROLL; ++u) acc = fmaf(v[u], e[int64_t(rr[u]) * FR + f], acc);
+    }
+    for (; k < en; k += SLOTS)
+      acc = fmaf(Val<VT>::get(val, k), e[int64_t(brow[k]) * FR + f], acc);
+#pragma unroll
+    for (int off = kWave / 2; off >= FR; off >>= 1) acc += __shfl_xor(acc, off, kWave);
+  }
+  double part = 0.0;
+  if (j < n && slot == 0) {
+    int64_t o = int64_t(j) * FR + f;
+    float cv = first ? acc : c[o] + acc;
+    if (!last) {
+      c[o] = cv;
+    } else {
+      float d = s[j];
+      d += float(d == 0.0f);
+      float v = x[o] * cv / d;
+      xx[o] = v;
+      part = double(v);
+    }
+  }
+  if (last) {
+    // per-frame block sums: lanes with slot 0 of each wave hold frame f
+    __shared__ double red[256 / kWave][64];
+    if (slot == 0) red[w][f] = part;
+    __syncthreads();
+    if (threadIdx.x < FR) {
+      double t = 0.0;
+      for (int ww = 0; ww < 256 / kWave; ++ww) t += red[ww][threadIdx.x];
+      partials[int64_t(lb) * FR + threadIdx.x] = t;
+    }
+  }
+}
+
+// scale_f = 1/sum_b partials[b][f] (FP64, fixed order); 0 for an empty frame
+__global__ void k_norms(const double* partials, int nb, int FR, float* scale, double* norm) {
+  __shared__ double red[256];
+  for (int f = 0; f < FR; ++f) {
+    double t = 0.0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) t += partials[int64_t(b) * FR + f];
+    red[threadIdx.x] = t;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+      if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      double tot = red[0];
+      norm[f] = tot;
+      scale[f] = tot != 0.0 ? float(1.0 / tot) : 0.0f;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_scale_frames(int64_t n, int FR, const float* xx, const float* scale,
+                               float* x) {
+  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < n * FR;
+       t += int64_t(gridDim.x) * blockDim.x)
+    x[t] = scale[t % FR] * xx[t];
+}
+
+__global__ void k_sens_norm32(const double* sens, int n, float* s) {
+  __shared__ double red[256];
+  double t = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) t += sens[i];
+  red[threadIdx.x] = t;
+  __syncthreads();
+  for (int k = blockDim.x / 2; k > 0; k >>= 1) {
+    if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  double tot = red[0];
+  for (int i = threadIdx.x; i < n; i += blockDim.x) s[i] = float(sens[i] / tot);
+}
+
+}  // namespace
+}  // namespace raptr
+
+using namespace raptr;
+
+struct raptr_recon {
+  int m = 0, n = 0, vt = kF32, fmax = 1;
+  int64_t nnz = 0, pnnz = 0;
+  int blk_rows = 0, nblk = 0;
+  // forward (padded CSR)
+  int64_t* frp = nullptr;
+  int* fcol = nullptr;
+  void* fval = nullptr;
+  float* rscale = nullptr;
+  // backward (row-blocked transpose)
+  int64_t* bptr = nullptr;  // nblk * (n+1)
+  int* brow = nullptr;
+  void* bval = nullptr;
+  // sensitivity and work vectors (sized for fmax frames)
+  float* s = nullptr;
+  float* e = nullptr;
+  float* c = nullptr;
+  float* xx = nullptr;
+  double* partials = nullptr;
+  float* scale = nullptr;
+  double* norm = nullptr;
+  int fwd_lanes = 64;
+  int bwd_unroll = 4;
+  size_t bytes = 0;
+  std::vector<void*> allocs;
+  std::map<long long, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
+};
+
+namespace raptr {
+namespace {
+
+template <class T>
+int ralloc(raptr_recon* R, T** p, size_t count) {
+  void* q = nullptr;
+  hipError_t e = hipMalloc(&q, std::max<size_t>(count * sizeof(T), 256));
+  if (e == hipErrorOutOfMemory) {
+    (void)hipGetLastError();
+    return fail(RAPTR_ERR_OUT_OF_MEMORY, "recon: out of device memory (%zu B)",
+                count * sizeof(T));
+  }
+  RAPTR_HIP_TRY(e);
+  R->allocs.push_back(q);
+  R->bytes += count * sizeof(T);
+  *p = static_cast<T*>(q);
+  return RAPTR_OK;
+}
+
+int vbytes(int vt) { return vt == kF32 ? 4 : 2; }
+
+// ---------------------------------------------------------- launches
+int launch_fwd_any(const raptr_recon* R, int F, const float* x, const float* y, float* e,
+                   hipStream_t st) {
+  if (R->m == 0) return RAPTR_OK;
+  const int64_t m = R->m;
+  if (F == 1) {
+    int L = R->fwd_lanes;
+    unsigned g = unsigned((m + (256 / L) - 1) / (256 / L));
+#define RAPTR_F1(VV, LL)                                                                     \
+  if (R->vt == VV && L == LL) {                                                              \
+    hipLaunchKernelGGL((k_fwd1<VV, LL>), dim3(g), dim3(256), 0, st, m, R->frp, R->fcol,       \
+                       (const typename Val<VV>::T*)R->fval, R->rscale, x, y, e);             \
+    RAPTR_LAUNCH_CHECK();                                                                    \
+    return RAPTR_OK;                                                                         \
+  }
+    RAPTR_F1(kF32, 16) RAPTR_F1(kF32, 32) RAPTR_F1(kF32, 64)
+    RAPTR_F1(kF16, 16) RAPTR_F1(kF16, 32) RAPTR_F1(kF16, 64)
+    RAPTR_F1(kBF16, 16) RAPTR_F1(kBF16, 32) RAPTR_F1(kBF16, 64)
+    RAPTR_F1(kU16, 16) RAPTR_F1(kU16, 32) RAPTR_F1(kU16, 64)
+#undef RAPTR_F1
+    return fail(RAPTR_ERR_INVALID_ARG, "forward lanes %d unsupported", L);
+  }
+  unsigned g = unsigned((m + 3) / 4);
+#define RAPTR_FF(VV, FF)                                                                   \
+  if (R->vt == VV && F == FF) {                                                            \
+    hipLaunchKernelGGL((k_fwdF<VV, FF>), dim3(g), dim3(256), 0, st, m, R->frp, R->fcol,     \
+                       (const typename Val<VV>::T*)R->fval, R->rscale, x, y, e);           \
+    RAPTR_LAUNCH_CHECK();                                                                  \
+    return RAPTR_OK;                                                                       \
+  }
+  RAPTR_FF(kF32, 2) RAPTR_FF(kF32, 4) RAPTR_FF(kF32, 8) RAPTR_FF(kF32, 16) RAPTR_FF(kF32, 32)
+  RAPTR_FF(kF32, 64) RAPTR_FF(kF16, 2) RAPTR_FF(kF16, 4) RAPTR_FF(kF16, 8) RAPTR_FF(kF16, 16)
+  RAPTR_FF(kF16, 32) RAPTR_FF(kF16, 64) RAPTR_FF(kBF16, 2) RAPTR_FF(kBF16, 4) RAPTR_FF(kBF16, 8)
+  RAPTR_FF(kBF16, 16) RAPTR_FF(kBF16, 32) RAPTR_FF(kBF16, 64)
+  RAPTR_FF(kU16, 2) RAPTR_FF(kU16, 4) RAPTR_FF(kU16, 8) RAPTR_FF(kU16, 16) RAPTR_FF(kU16, 32)
+  RAPTR_FF(kU16, 64)
+#undef RAPTR_FF
+  return fail(RAPTR_ERR_INVALID_ARG, "frame count %d unsupported (1..64, power of two)", F);
+}
+
+int launch_bwd_any(const raptr_recon* R, int F, int blk, const float* x, hipStream_t st,
+                   int* nb_out) {
+  const int n = R->n;
+  unsigned g = unsigned((n + 3) / 4);
+  if (nb_out) *nb_out = int(g);
+  const int64_t* ptr = R->bptr + int64_t(blk) * (n + 1);
+  int first = blk == 0, last = blk == R->nblk - 1;
+  int U = R->bwd_unroll;
+#define RAPTR_B(VV, FF, UU)                                                                 \
+  if (R->vt == VV && F == FF && U == UU) {                                                  \
+    hipLaunchKernelGGL((k_bwd<VV, FF, UU>), dim3(g), dim3(256), 0, st, n, ptr, R->brow,      \
+                       (const typename Val<VV>::T*)R->bval, R->e, R->c, first, last, x,     \
+                       R->s, R->xx, R->partials);                                           \
+    RAPTR_LAUNCH_CHECK();                                                                   \
+    return RAPTR_OK;                                                                        \
+  }
+#define RAPTR_BU(VV, FF) RAPTR_B(VV, FF, 2) RAPTR_B(VV, FF, 4) RAPTR_B(VV, FF, 8)
+  RAPTR_BU(kF32, 1) RAPTR_BU(kF32, 2) RAPTR_BU(kF32, 4) RAPTR_BU(kF32, 8) RAPTR_BU(kF32, 16)
+  RAPTR_BU(kF32, 32) RAPTR_BU(kF32, 64)
+  RAPTR_BU(kF16, 1) RAPTR_BU(kF16, 2) RAPTR_BU(kF16, 4) RAPTR_BU(kF16, 8) RAPTR_BU(kF16, 16)
+  RAPTR_BU(kF16, 32) RAPTR_BU(kF16, 64)
+  RAPTR_BU(kBF16, 1) RAPTR_BU(kBF16, 2) RAPTR_BU(kBF16, 4) RAPTR_BU(kBF16, 8)
+  RAPTR_BU(kBF16, 16) RAPTR_BU(kBF16, 32) RAPTR_BU(kBF16, 64)
+  RAPTR_BU(kU16, 1) RAPTR_BU(kU16, 2) RAPTR_BU(kU16, 4) RAPTR_BU(kU16, 8) RAPTR_BU(kU16, 16)
+  RAPTR_BU(kU16, 32) RAPTR_BU(kU16, 64)
+#undef RAPTR_BU
+#undef RAPTR_B
+  return fail(RAPTR_ERR_INVALID_ARG, "backward configuration unsupported");
+}
+
+int iteration(raptr_recon* R, int F, const float* y, float* x, hipStream_t st) {
+  RAPTR_TRY(launch_fwd_any(R, F, x, y, R->e, st));
+  int nb = 0;
+  for (int b = 0; b < R->nblk; ++b) RAPTR_TRY(launch_bwd_any(R, F, b, x, st, &nb));
+  hipLaunchKernelGGL(k_norms, dim3(1), dim3(256), 0, st, (const double*)R->partials, nb, F,
+                     R->scale, R->norm);
+  RAPTR_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_scale_frames, dim3(grid_for(int64_t(R->n) * F, 256, 8192)), dim3(256),
+                     0, st, int64_t(R->n), F, (const float*)R->xx, (const float*)R->scale, x);
+  RAPTR_LAUNCH_CHECK();
+  return RAPTR_OK;
+}
+
+bool valid_frames(int F) { return F >= 1 && F <= 64 && (F & (F - 1)) == 0; }
+
+}  // namespace
+}  // namespace raptr
+
+extern "C" {
+
+int raptr_recon_create(raptr_recon** out, int m, int n, int64_t nnz, const int64_t* d_rp,
+                       const int* d_col, const float* d_val, const double* d_sens,
+                       int value_type, int max_frames, int block_rows,
+                       raptr_stream_t stream) {
+  if (!out) return fail(RAPTR_ERR_INVALID_ARG, "null output");
+  *out = nullptr;
+  if (m < 0 || n <= 0 || nnz < 0) return fail(RAPTR_ERR_INVALID_ARG, "bad sizes");
+  if (value_type < kF32 || value_type > kU16)
+    return fail(RAPTR_ERR_INVALID_ARG, "unknown value type %d", value_type);
+  if (!valid_frames(max_frames))
+    return fail(RAPTR_ERR_INVALID_ARG, "max_frames must be a power of two in [1, 64]");
+  if (!d_rp || !d_sens || (nnz > 0 && (!d_col || !d_val)))
+    return fail(RAPTR_ERR_INVALID_ARG, "null buffer");
+  if (nnz > int64_t(UINT32_MAX) - 16)
+    return fail(RAPTR_ERR_INVALID_ARG, "nnz exceeds 2^32");
+  hipStream_t st = S(stream);
+  raptr_recon* R = new raptr_recon();
+  R->m = m;
+  R->n = n;
+  R->nnz = nnz;
+  R->vt = value_type;
+  R->fmax = max_frames;
+  auto bail = [&](int rc) {
+    raptr_recon_destroy(R);
+    return rc;
+  };
+  // row block: ratio slice of <= 2 MiB per launch (one XCD's L2 holds it)
+  if (block_rows <= 0) block_rows = std::max(1, (2 << 20) / (4 * max_frames));
+  R->blk_rows = block_rows;
+  R->nblk = std::max(1, int((int64_t(m) + block_rows - 1) / block_rows));
+  int rc;
+  // ---- forward: padded CSR
+  int64_t* plen = nullptr;
+  int64_t* tot = nullptr;
+  if ((rc = ralloc(R, &R->frp, size_t(m) + 1))) return bail(rc);
+  {
+    void* tmp = nullptr;
+    size_t sb = scan_workspace_bytes(m);
+    RAPTR_HIP_TRY(hipMalloc(&tmp, size_t(m) * 8 + 512 + sb));
+    plen = static_cast<int64_t*>(tmp);
+    tot = reinterpret_cast<int64_t*>(static_cast<char*>(tmp) + ((size_t(m) * 8 + 255) & ~size_t(255)));
+    char* sws = reinterpret_cast<char*>(tot) + 256;
+    if (m > 0) {
+      hipLaunchKernelGGL(k_padded_len, dim3(grid_for(m, 256, 8192)), dim3(256), 0, st, d_rp,
+                         m, plen);
+      rc = scan_exclusive_i64(plen, R->frp, m, tot, sws, sb, st);
+      if (!rc) hipLaunchKernelGGL(k_set_last, dim3(1), dim3(1), 0, st, R->frp, m, (const int64_t*)tot);
+    } else {
+      (void)hipMemsetAsync(R->frp, 0, 8, st);
+      (void)hipMemsetAsync(tot, 0, 8, st);
+    }
+    int64_t h = 0;
+    (void)hipMemcpyAsync(&h, tot, 8, hipMemcpyDeviceToHost, st);
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(tmp);
+    if (rc) return bail(rc);
+    R->pnnz = h;
+  }
+  if ((rc = ralloc(R, &R->fcol, size_t(R->pnnz) + 4))) return bail(rc);
+  if (value_type == kF32) {
+    float* p;
+    if ((rc = ralloc(R, &p, size_t(R->pnnz) + 4))) return bail(rc);
+    R->fval = p;
+  } else {
+    uint16_t* p;
+    if ((rc = ralloc(R, &p, size_t(R->pnnz) + 8))) return bail(rc);
+    R->fval = p;
+  }
+  if ((rc = ralloc(R, &R->rscale, size_t(m) + 1))) return bail(rc);
+  if (m > 0) {
+    hipLaunchKernelGGL(k_pad_copy, dim3(grid_for(int64_t(m) * kWave, 256, 1 << 16)), dim3(256),
+                       0, st, d_rp, d_col, d_val, m, value_type, (const int64_t*)R->frp,
+                       R->fcol, R->fval, R->rscale);
+    hipError_t le = hipGetLastError();
+    if (le != hipSuccess) return bail(fail(RAPTR_ERR_DEVICE, "pad: %s", hipGetErrorString(le)));
+  }
+  // ---- backward: row-blocked transpose via a stable radix sort
+  if ((rc = ralloc(R, &R->bptr, size_t(R->nblk) * (size_t(n) + 1)))) return bail(rc);
+  if ((rc = ralloc(R, &R->brow, size_t(nnz) + 1))) return bail(rc);
+  if (value_type == kF32) {
+    float* p;
+    if ((rc = ralloc(R, &p, size_t(nnz) + 1))) return bail(rc);
+    R->bval = p;
+  } else {
+    uint16_t* p;
+    if ((rc = ralloc(R, &p, size_t(nnz) + 1))) return bail(rc);
+    R->bval = p;
+  }
+  if (nnz > 0) {
+    int colbits = 1;
+    while ((int64_t(1) << colbits) <= n) ++colbits;
+    int blkbits = 0;
+    while ((int64_t(1) << blkbits) < R->nblk) ++blkbits;
+    size_t sort_ws = raptr_csr_transpose_workspace_size(nnz, m, n);  // >= radix buffers
+    void* tmp = nullptr;
+    RAPTR_HIP_TRY(hipMalloc(&tmp, sort_ws));
+    // carve: rowid | key32 (unused) | sort buffers, in the order the
+    // transpose helper lays them out
+    Carver cv(tmp, sort_ws);
+    uint32_t* rowid = cv.take<uint32_t>(size_t(nnz));
+    (void)cv.take<int>(size_t(nnz));
+    rc = RAPTR_OK;
+    uint64_t* k0 = cv.take<uint64_t>(size_t(nnz));
+    uint64_t* k1 = cv.take<uint64_t>(size_t(nnz));
+    uint32_t* v0 = cv.take<uint32_t>(size_t(nnz));
+    uint32_t* v1 = cv.take<uint32_t>(size_t(nnz));
+    if (!cv.ok) rc = fail(RAPTR_ERR_WORKSPACE, "transpose scratch carve failed");
+    if (!rc) {
+      hipLaunchKernelGGL(k_bt_keys, dim3(grid_for(int64_t(m) * kWave, 256, 1 << 16)), dim3(256),
+                         0, st, d_rp, d_col, m, block_rows, colbits, k0, v0);
+      hipLaunchKernelGGL(k_expand_rows32, dim3(grid_for(int64_t(m) * kWave, 256, 1 << 16)),
+                         dim3(256), 0, st, d_rp, m, rowid);
+      int res = 0;
+      rc = radix_sort_pairs(k0, k1, v0, v1, nnz, colbits + blkbits, st, &res, cv);
+      if (!rc) {
+        uint64_t* keys = res ? k1 : k0;
+        uint32_t* idx = res ? v1 : v0;
+        hipLaunchKernelGGL(k_bt_gather, dim3(grid_for(nnz, 256, 1 << 16)), dim3(256), 0, st,
+                           (const uint32_t*)idx, nnz, (const uint32_t*)rowid, d_val,
+                           (const float*)R->rscale, value_type, R->brow, R->bval);
+        hipLaunchKernelGGL(k_bt_ptr,
+                           dim3(grid_for(int64_t(R->nblk) * (n + 1), 256, 1 << 16)), dim3(256),
+                           0, st, (const uint64_t*)keys, nnz, R->nblk, n, colbits, R->bptr);
+        hipError_t le = hipGetLastError();
+        if (le != hipSuccess) rc = fail(RAPTR_ERR_DEVICE, "transpose: %s", hipGetErrorString(le));
+      }
+    }
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(tmp);
+    if (rc) return bail(rc);
+  } else {
+    (void)hipMemsetAsync(R->bptr, 0, sizeof(int64_t) * size_t(R->nblk) * (n + 1), st);
+  }
+  // ---- sensitivity and work vectors
+  const int F = max_frames;
+  if ((rc = ralloc(R, &R->s, size_t(n)))) return bail(rc);
+  if ((rc = ralloc(R, &R->e, (size_t(m) + 1) * F))) return bail(rc);
+  if ((rc = ralloc(R, &R->c, size_t(n) * F))) return bail(rc);
+  if ((rc = ralloc(R, &R->xx, size_t(n) * F))) return bail(rc);
+  if ((rc = ralloc(R, &R->partials, (size_t(n) / 4 + 8) * F))) return bail(rc);
+  if ((rc = ralloc(R, &R->scale, size_t(F)))) return bail(rc);
+  if ((rc = ralloc(R, &R->norm, size_t(F)))) return bail(rc);
+  hipLaunchKernelGGL(k_sens_norm32, dim3(1), dim3(256), 0, st, d_sens, n, R->s);
+  hipError_t le = hipGetLastError();
+  if (le != hipSuccess) return bail(fail(RAPTR_ERR_DEVICE, "sens: %s", hipGetErrorString(le)));
+  hipError_t se = hipStreamSynchronize(st);
+  if (se != hipSuccess) return bail(fail(RAPTR_ERR_DEVICE, "create: %s", hipGetErrorString(se)));
+  R->fwd_lanes = double(nnz) / std::max(m, 1) >= 192 ? 32 : 16;
+  *out = R;
+  return RAPTR_OK;
+}
+
+int raptr_recon_destroy(raptr_recon* R) {
+  if (!R) return RAPTR_OK;
+  for (auto& kv : R->graphs) {
+    if (kv.second.second) (void)hipGraphExecDestroy(kv.second.second);
+    if (kv.second.first) (void)hipGraphDestroy(kv.second.first);
+  }
+  for (void* p : R->allocs) (void)hipFree(p);
+  delete R;
+  return RAPTR_OK;
+}
+
+int raptr_recon_info(const raptr_recon* R, int64_t* device_bytes, int64_t* padded_nnz,
+                     int* row_blocks, int* fwd_lanes, int* bwd_unroll) {
+  if (!R) return fail(RAPTR_ERR_INVALID_ARG, "null plan");
+  if (device_bytes) *device_bytes = int64_t(R->bytes);
+  if (padded_nnz) *padded_nnz = R->pnnz;
+  if (row_blocks) *row_blocks = R->nblk;
+  if (fwd_lanes) *fwd_lanes = R->fwd_lanes;
+  if (bwd_unroll) *bwd_unroll = R->bwd_unroll;
+  return RAPTR_OK;
+}
+
+int raptr_recon_set_config(raptr_recon* R, int fwd_lanes, int bwd_unroll) {
+  if (!R) return fail(RAPTR_ERR_INVALID_ARG, "null plan");
+  if (fwd_lanes != 16 && fwd_lanes != 32 && fwd_lanes != 64)
+    return fail(RAPTR_ERR_INVALID_ARG, "forward lanes must be 16, 32 or 64");
+  if (bwd_unroll != 2 && bwd_unroll != 4 && bwd_unroll != 8)
+    return fail(RAPTR_ERR_INVALID_ARG, "backward unroll must be 2, 4 or 8");
+  if (R->fwd_lanes != fwd_lanes || R->bwd_unroll != bwd_unroll) {
+    for (auto& kv : R->graphs) {
+      if (kv.second.second) (void)hipGraphExecDestroy(kv.second.second);
+      if (kv.second.first) (void)hipGraphDestroy(kv.second.first);
+    }
+    R->graphs.clear();
+  }
+  R->fwd_lanes = fwd_lanes;
+  R->bwd_unroll = bwd_unroll;
+  return RAPTR_OK;
+}
+
+int raptr_recon_mlem(raptr_recon* R, int nframes, const float* d_y, float* d_x, int niter,
+                     int use_graph, raptr_stream_t stream) {
+  if (!R) return fail(RAPTR_ERR_INVALID_ARG, "null plan");
+  if (!valid_frames(nframes) || nframes > R->fmax)
+    return fail(RAPTR_ERR_INVALID_ARG, "nframes %d not a power of two <= %d", nframes, R->fmax);
+  if (niter < 0 || !d_y || !d_x) return fail(RAPTR_ERR_INVALID_ARG, "bad arguments");
+  hipStream_t st = S(stream);
+  if (!use_graph || niter < 2 || !st) {
+    for (int it = 0; it < niter; ++it) RAPTR_TRY(iteration(R, nframes, d_y, d_x, st));
+    return RAPTR_OK;
+  }
+  long long key = (long long)nframes ^ ((long long)(uintptr_t)d_y << 7) ^
+                  ((long long)(uintptr_t)d_x << 19);
+  auto it = R->graphs.find(key);
+  hipGraphExec_t exec = nullptr;
+  if (it == R->graphs.end()) {
+    hipGraph_t graph = nullptr;
+    RAPTR_HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    int irc = iteration(R, nframes, d_y, d_x, st);
+    hipError_t ce = hipStreamEndCapture(st, &graph);
+    if (irc) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return irc;
+    }
+    RAPTR_HIP_TRY(ce);
+    hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (ie != hipSuccess) {
+      (void)hipGraphDestroy(graph);
+      return fail(RAPTR_ERR_DEVICE, "graph instantiate: %s", hipGetErrorString(ie));
+    }
+    R->graphs[key] = {graph, exec};
+  } else {
+    exec = it->second.second;
+  }
+  for (int i = 0; i < niter; ++i) RAPTR_HIP_TRY(hipGraphLaunch(exec, st));
+  return RAPTR_OK;
+}
+
+// One forward (+ratio) or one full backward (all row blocks, with update)
+// of the plan, for timing and tests.
+int raptr_recon_forward(raptr_recon* R, int nframes, const float* d_x, const float* d_y,
+                        float* d_e, raptr_stream_t stream) {
+  if (!R || !valid_frames(nframes) || nframes > R->fmax)
+    return fail(RAPTR_ERR_INVALID_ARG, "bad plan or frame count");
+  return launch_fwd_any(R, nframes, d_x, d_y, d_e, S(stream));
+}
+
+int raptr_recon_backward(raptr_recon* R, int nframes, const float* d_x, raptr_stream_t stream) {
+  if (!R || !valid_frames(nframes) || nframes > R->fmax)
+    return fail(RAPTR_ERR_INVALID_ARG, "bad plan or frame count");
+  for (int b = 0; b < R->nblk; ++b)
+    RAPTR_TRY(launch_bwd_any(R, nframes, b, d_x, S(stream), nullptr));
+  return RAPTR_OK;
+}
+
+}  // extern "C"

@@ -227,6 +227,23 @@ __global__ void k_t_gather(const uint64_t* keys, const uint32_t* idx, int64_t nn
 }
 
 }  // namespace
+
+int radix_sort_pairs(uint64_t* k0, uint64_t* k1, uint32_t* v0, uint32_t* v1, int64_t n,
+                     int bits, hipStream_t st, int* result, Carver& cv) {
+  SortBufs b;
+  int64_t ntiles = (n + kRsTile - 1) / kRsTile;
+  b.k[0] = k0;
+  b.k[1] = k1;
+  b.v[0] = v0;
+  b.v[1] = v1;
+  b.hist = cv.take<uint32_t>(size_t(ntiles * kRsBins));
+  b.offs = cv.take<uint32_t>(size_t(ntiles * kRsBins));
+  b.scan_bytes = scan_workspace_bytes(ntiles * kRsBins);
+  b.scan_ws = cv.take<char>(b.scan_bytes);
+  if (!cv.ok) return fail(RAPTR_ERR_WORKSPACE, "radix sort scratch too small");
+  return radix_sort(b, n, bits, st, result);
+}
+
 }  // namespace raptr
 
 using namespace raptr;

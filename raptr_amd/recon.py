@@ -1,0 +1,90 @@
+"""Device-resident MLEM reconstruction plan (Python side of raptr_recon_*).
+
+A plan takes the CSR of the measured channels (from ``sysmat.build_csr``)
+and the FP64 sensitivity, builds the forward (row-padded CSR) and backward
+(row-blocked transpose) layouts once, and runs plain or batched MLEM
+(reco.cu:230-303) on F frames at a time.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _capi as capi
+from .device import DeviceArray
+
+VALUE_TYPES = {"f32": 0, "f16": 1, "bf16": 2, "u16": 3}
+
+
+class Reconstructor:
+    def __init__(self, A, sens, value_type="f32", max_frames=1, block_rows=0, stream=None):
+        self.m, self.n, self.nnz = A.m, A.n, A.nnz
+        self.value_type = value_type
+        self.max_frames = max_frames
+        p = C.c_void_p()
+        capi.call("raptr_recon_create", C.byref(p), A.m, A.n, A.nnz, A.row_ptr.ptr, A.col.ptr,
+                  A.val.ptr, sens.ptr, VALUE_TYPES[value_type], int(max_frames),
+                  int(block_rows), stream)
+        self.handle = p.value
+
+    def info(self):
+        b, p = C.c_int64(), C.c_int64()
+        nb, fl, bu = C.c_int(), C.c_int(), C.c_int()
+        capi.call("raptr_recon_info", self.handle, C.byref(b), C.byref(p), C.byref(nb),
+                  C.byref(fl), C.byref(bu))
+        return {"device_bytes": b.value, "padded_nnz": p.value, "row_blocks": nb.value,
+                "fwd_lanes": fl.value, "bwd_unroll": bu.value}
+
+    def set_config(self, fwd_lanes, bwd_unroll):
+        capi.call("raptr_recon_set_config", self.handle, int(fwd_lanes), int(bwd_unroll))
+
+    def run(self, y, x, niter, nframes=1, use_graph=True, stream=None):
+        """In-place MLEM on device arrays y ([m][F]) and x ([n][F])."""
+        capi.call("raptr_recon_mlem", self.handle, int(nframes), y.ptr, x.ptr, int(niter),
+                  1 if use_graph else 0, stream)
+
+    def mlem(self, y, x0, niter, use_graph=True, stream=None):
+        """Host convenience: y [m] or [m, F]; x0 [n] or [n, F]; returns x."""
+        y = np.asarray(y, np.float32)
+        x0 = np.asarray(x0, np.float32)
+        F = 1 if y.ndim == 1 else y.shape[1]
+        # every copy on the work stream: a non-blocking stream is not ordered
+        # with the null stream
+        dy = DeviceArray.from_numpy(np.ascontiguousarray(y).reshape(-1), stream=stream)
+        dx = DeviceArray.from_numpy(np.ascontiguousarray(x0).reshape(-1), stream=stream)
+        self.run(dy, dx, niter, F, use_graph=use_graph and stream is not None, stream=stream)
+        out = dx.numpy(stream=stream)
+        return out if F == 1 else out.reshape(self.n, F)
+
+    def autotune(self, y, x, nframes=1, reps=3, stream=None):
+        """Time every (forward lanes, backward unroll) shape on this plan with
+        HIP events and keep the fastest; returns (config, ms per iteration)."""
+        from .device import Event
+        e0, e1 = Event(), Event()
+        x_save = DeviceArray(x.count, np.float32)
+        capi.call("raptr_memcpy_d2d", x_save.ptr, x.ptr, x.nbytes, stream)
+        best = None
+        lanes = (16, 32, 64) if nframes == 1 else (self.info()["fwd_lanes"],)
+        for fl in lanes:
+            for bu in (2, 4, 8):
+                self.set_config(fl, bu)
+                self.run(y, x, 1, nframes, use_graph=False, stream=stream)
+                e0.record(stream)
+                self.run(y, x, reps, nframes, use_graph=False, stream=stream)
+                e1.record(stream)
+                ms = e0.elapsed_ms(e1) / reps
+                if best is None or ms < best[1]:
+                    best = ((fl, bu), ms)
+        self.set_config(*best[0])
+        capi.call("raptr_memcpy_d2d", x.ptr, x_save.ptr, x.nbytes, stream)
+        return best
+
+    def close(self):
+        if getattr(self, "handle", None):
+            capi.lib().raptr_recon_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

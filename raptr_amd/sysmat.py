@@ -93,15 +93,41 @@ DEFAULT_STAGING = 64 << 20  # staging entries of the chunked build
 def build_csr(setup, grid, cnl_list, nrays, seed=1, raygen=capi.RAYGEN_RAND, stream=None,
               staging=DEFAULT_STAGING, capacity=None, d_list=None):
     """Matrix of the listed channels (rows in list order), zeros dropped.
-    Without ``capacity`` a count pass sizes the output exactly."""
+    capacity: None = exact count pass first; "auto" = estimate from a count
+    pass over every 64th row (falls back to the exact pass if exceeded);
+    an int = caller's bound."""
     lst = np.ascontiguousarray(cnl_list, np.int32)
     m = len(lst)
-    d_list = d_list or DeviceArray.from_numpy(lst)
+    if d_list is None:
+        d_list = DeviceArray.from_numpy(lst)
     s, g = setup.struct(), grid.struct()
     L = capi.lib()
     ws = Workspace(L.raptr_sm_build_workspace_size(C.byref(g), int(nrays), int(staging)))
     rp = DeviceArray(m + 1, np.int64)
     nnz = C.c_int64()
+    if capacity == "auto":
+        if m < 4096:
+            capacity = None
+        else:
+            sample = lst[::64].copy()
+            d_s = DeviceArray.from_numpy(sample)
+            rps = DeviceArray(len(sample) + 1, np.int64)
+            capi.call("raptr_sm_build_csr", C.byref(s), C.byref(g), d_s.ptr, len(sample),
+                      int(nrays), int(seed), int(raygen), rps.ptr, None, None, 0, C.byref(nnz),
+                      ws.ptr, ws.nbytes, stream)
+            est = nnz.value * m / len(sample)
+            capacity = int(est * 1.03) + 65536
+            col = DeviceArray(capacity, np.int32)
+            val = DeviceArray(capacity, np.float32)
+            st = L.raptr_sm_build_csr(C.byref(s), C.byref(g), d_list.ptr, m, int(nrays),
+                                      int(seed), int(raygen), rp.ptr, col.ptr, val.ptr,
+                                      int(capacity), C.byref(nnz), ws.ptr, ws.nbytes, stream)
+            if st == capi.RAPTR_OK:
+                return DeviceCSR(m, grid.nvox, nnz.value, rp, col, val)
+            if st != capi.RAPTR_ERR_WORKSPACE:
+                capi.check(st, "raptr_sm_build_csr")
+            del col, val
+            capacity = None
     if capacity is None:
         capi.call("raptr_sm_build_csr", C.byref(s), C.byref(g), d_list.ptr, m, int(nrays),
                   int(seed), int(raygen), rp.ptr, None, None, 0, C.byref(nnz), ws.ptr,
