@@ -148,13 +148,25 @@ def main():
         info = R.info()
         modes[name] = {"value_type": vt, "frames_per_batch": nf, "ms_per_batch": round(ms, 3),
                        "frames_per_s": round(fps, 3), "ms_per_iteration": round(ms / niter, 4),
-                       "config": {"fwd_lanes": cfg[0], "bwd_unroll": cfg[1],
-                                  "row_blocks": info["row_blocks"]},
+                       "config": dict(cfg, row_blocks=info["row_blocks"]),
                        "plan_build_s": round(t_plan, 2),
                        "plan_device_GB": round(info["device_bytes"] / 1e9, 2),
                        "image0": xr[:, 0].copy()}
+        de_m = DeviceArray(m * nf, np.float32)
+
+        def pf():
+            capi.call("raptr_recon_forward", R.handle, nf, dx.ptr, dy.ptr, de_m.ptr, st)
+
+        def pb():
+            capi.call("raptr_recon_backward", R.handle, nf, dx.ptr, st)
+
+        pf(), pb()
+        modes[name]["passes_ms"] = {"forward": round(timed(lambda: [pf() for _ in range(3)]) / 3, 3),
+                                    "backward": round(timed(lambda: [pb() for _ in range(3)]) / 3, 3)}
+        del de_m
         log(f"{name}: {ms:.1f} ms/batch of {nf} -> {fps:.2f} frames/s "
-            f"(cfg {cfg}, {info['row_blocks']} row blocks, plan {t_plan:.1f} s)")
+            f"(cfg {cfg}, {info['row_blocks']} row blocks, plan {t_plan:.1f} s, "
+            f"passes {modes[name]['passes_ms']})")
         if name != "plain":
             # each batched frame against the plain single-frame run of frame 0
             modes[name]["rel_l2_vs_plain_frame0"] = rel_l2(xr[:, 0], modes["plain"]["image0"])
@@ -179,16 +191,22 @@ def main():
     nb = info["row_blocks"]
     # algorithmic bytes: matrix (value + column index), row pointers, vectors
     bytes_f = 8 * info["padded_nnz"] + 8 * (m + 1) + 4 * m + 4 * m + 4 * n
-    bytes_b = 8 * A.nnz + 8 * nb * (n + 1) + 4 * m + 4 * n * (2 * nb + 2) + 8 * (n // 4)
+    if info["bwd_staged"]:
+        # one launch; the ratio vector read once per workgroup from L2 (counted
+        # once), column sums in LDS, x/s read and x' written once
+        bytes_b = 8 * A.nnz + 8 * nb * (n + 1) + 4 * m + 12 * n + 8 * info["bwd_workgroups"]
+        bwd_name = f"k_bwd_lds (backward + update, one launch over {nb} LDS row slices)"
+    else:
+        bytes_b = 8 * A.nnz + 8 * nb * (n + 1) + 4 * m + 4 * n * (2 * nb + 2) + 8 * (n // 4)
+        bwd_name = f"k_bwd (backward + update, pass of {nb} row-block launches)"
     gbs_f = bytes_f / (ms_f * 1e-3) / 1e9
     gbs_b = bytes_b / (ms_b * 1e-3) / 1e9
-    log(f"forward pass {ms_f:.3f} ms ({gbs_f:.0f} GB/s), backward pass {ms_b:.3f} ms over "
-        f"{nb} launches ({gbs_b:.0f} GB/s)")
+    log(f"forward pass {ms_f:.3f} ms ({gbs_f:.0f} GB/s), backward pass {ms_b:.3f} ms "
+        f"({bwd_name}, {gbs_b:.0f} GB/s)")
     dom = {"kernel": "k_fwd1 (forward + ratio, one launch)", "ms": ms_f, "bytes": bytes_f,
            "gbs": gbs_f}
     if ms_b > ms_f:
-        dom = {"kernel": f"k_bwd (backward + update, pass of {nb} row-block launches)",
-               "ms": ms_b, "bytes": bytes_b, "gbs": gbs_b}
+        dom = {"kernel": bwd_name, "ms": ms_b, "bytes": bytes_b, "gbs": gbs_b}
 
     # ------------------------ FP64 host recurrence: errors + CPU baseline
     cpu = None
@@ -256,7 +274,7 @@ def main():
                      "algorithmic_bytes": dom["bytes"], "avg_launch_ms": round(dom["ms"], 4)},
         "passes": {"forward_ms": round(ms_f, 4), "forward_GBps": round(gbs_f, 1),
                    "backward_ms": round(ms_b, 4), "backward_GBps": round(gbs_b, 1),
-                   "backward_launches": nb},
+                   "backward_kernel": bwd_name},
         "build": {"matrix_ms": round(ms_build, 1), "sensitivity_ms": round(ms_sens, 1),
                   "data_s": round(t_data, 2)},
         "batched_small": small,

@@ -286,10 +286,22 @@ int raptr_recon_create(raptr_recon** plan, int m, int n, int64_t nnz,
 int raptr_recon_destroy(raptr_recon* plan);
 int raptr_recon_info(const raptr_recon* plan, int64_t* device_bytes,
                      int64_t* padded_nnz, int* row_blocks, int* fwd_lanes,
-                     int* bwd_unroll);
-/* Kernel shape: lanes per forward row (16, 32, 64; single frame) and
- * independent loads per lane of the backward pass (2, 4, 8). */
-int raptr_recon_set_config(raptr_recon* plan, int fwd_lanes, int bwd_unroll);
+                     int* fwd_unroll, int* bwd_unroll);
+/* Kernel shape: lanes per forward row (16, 32, 64; single frame), forward
+ * loads in flight per lane (single frame: 1 or 2 groups of 4 entries;
+ * batched: 2, 4 or 8 entries) and backward loads per lane (2, 4, 8). */
+int raptr_recon_set_config(raptr_recon* plan, int fwd_lanes, int fwd_unroll,
+                           int bwd_unroll);
+/* Single-frame backward pass staged through LDS (staged != 0; needs
+ * block_rows <= 24576 and a multiple of 4): each of `workgroups` workgroups owns a contiguous
+ * column range, loads each row block's ratio slice into LDS and gathers
+ * from there with lanes_per_column (8, 16, 32) lanes per column; column sums
+ * carried in LDS across blocks.  Single-frame plans created with such
+ * block_rows start with it on; others start with the block-by-block pass. */
+int raptr_recon_set_backward(raptr_recon* plan, int staged, int lanes_per_column,
+                             int workgroups);
+int raptr_recon_get_backward(const raptr_recon* plan, int* staged, int* lanes_per_column,
+                             int* workgroups);
 /* niter MLEM iterations (reco.cu:230-303) of nframes frames at once
  * (power of two <= max_frames).  d_y: counts [m][nframes]; d_x: images
  * [n][nframes], start images in, results out.  Per frame and iteration:

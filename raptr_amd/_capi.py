@@ -108,8 +108,10 @@ _SIGS = {
     "raptr_mlem_partial_slots": (i32, [i32]),
     "raptr_recon_create": (i32, [P(vp), i32, i32, i64, vp, vp, vp, vp, i32, i32, i32, vp]),
     "raptr_recon_destroy": (i32, [vp]),
-    "raptr_recon_info": (i32, [vp, P(i64), P(i64), P(i32), P(i32), P(i32)]),
-    "raptr_recon_set_config": (i32, [vp, i32, i32]),
+    "raptr_recon_info": (i32, [vp, P(i64), P(i64), P(i32), P(i32), P(i32), P(i32)]),
+    "raptr_recon_set_config": (i32, [vp, i32, i32, i32]),
+    "raptr_recon_set_backward": (i32, [vp, i32, i32, i32]),
+    "raptr_recon_get_backward": (i32, [vp, vp, vp, vp]),
     "raptr_recon_mlem": (i32, [vp, i32, vp, vp, i32, i32, vp]),
     "raptr_recon_forward": (i32, [vp, i32, vp, vp, vp, vp]),
     "raptr_recon_backward": (i32, [vp, i32, vp, vp]),

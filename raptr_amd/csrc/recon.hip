@@ -180,7 +180,24 @@ __global__ void k_bt_ptr(const uint64_t* keys, int64_t nnz, int nblk, int n, int
 // row (forward) / column (backward); its 64 lanes form 64/FR entry slots
 // x FR frames.  F = 1 uses vector loads of 4 entries per lane (forward).
 
-template <int VT, int LANES>
+template <int VT>
+__device__ __forceinline__ void load4(const typename Val<VT>::T* __restrict__ val, int64_t k,
+                                      float& v0, float& v1, float& v2, float& v3) {
+  if (VT == kF32) {
+    vf4 v = __builtin_nontemporal_load(
+        reinterpret_cast<const vf4*>(reinterpret_cast<const float*>(val) + k));
+    v0 = v.x, v1 = v.y, v2 = v.z, v3 = v.w;
+  } else {
+    vu2 h = __builtin_nontemporal_load(
+        reinterpret_cast<const vu2*>(reinterpret_cast<const uint16_t*>(val) + k));
+    v0 = h2f(uint16_t(h.x & 0xffff), VT), v1 = h2f(uint16_t(h.x >> 16), VT);
+    v2 = h2f(uint16_t(h.y & 0xffff), VT), v3 = h2f(uint16_t(h.y >> 16), VT);
+  }
+}
+
+// Single frame: each lane streams U groups of 4 entries per step (vector
+// loads), so 4*U gathers of x are in flight per lane.
+template <int VT, int LANES, int U>
 __global__ __launch_bounds__(256) void k_fwd1(int64_t m, const int64_t* __restrict__ rp,
                                               const int* __restrict__ col,
                                               const typename Val<VT>::T* __restrict__ val,
@@ -193,19 +210,27 @@ __global__ __launch_bounds__(256) void k_fwd1(int64_t m, const int64_t* __restri
   if (i >= m) return;
   const int64_t b = rp[i], en = rp[i + 1];  // multiples of 4
   float acc = 0.0f;
-  for (int64_t k = b + 4 * g; k < en; k += 4 * LANES) {
+  int64_t k = b + 4 * g;
+  for (; k + int64_t(4) * LANES * (U - 1) < en; k += int64_t(4) * LANES * U) {
+    vi4 c[U];
+    float v[U][4];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      c[u] = __builtin_nontemporal_load(reinterpret_cast<const vi4*>(col + k + 4 * LANES * u));
+      load4<VT>(val, k + 4 * LANES * u, v[u][0], v[u][1], v[u][2], v[u][3]);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      acc = fmaf(v[u][0], x[c[u].x], acc);
+      acc = fmaf(v[u][1], x[c[u].y], acc);
+      acc = fmaf(v[u][2], x[c[u].z], acc);
+      acc = fmaf(v[u][3], x[c[u].w], acc);
+    }
+  }
+  for (; k < en; k += int64_t(4) * LANES) {
     vi4 c = __builtin_nontemporal_load(reinterpret_cast<const vi4*>(col + k));
     float v0, v1, v2, v3;
-    if (VT == kF32) {
-      vf4 v = __builtin_nontemporal_load(reinterpret_cast<const vf4*>(
-          reinterpret_cast<const float*>(val) + k));
-      v0 = v.x, v1 = v.y, v2 = v.z, v3 = v.w;
-    } else {
-      vu2 h = __builtin_nontemporal_load(reinterpret_cast<const vu2*>(
-          reinterpret_cast<const uint16_t*>(val) + k));
-      v0 = h2f(uint16_t(h.x & 0xffff), VT), v1 = h2f(uint16_t(h.x >> 16), VT);
-      v2 = h2f(uint16_t(h.y & 0xffff), VT), v3 = h2f(uint16_t(h.y >> 16), VT);
-    }
+    load4<VT>(val, k, v0, v1, v2, v3);
     acc = fmaf(v0, x[c.x], acc);
     acc = fmaf(v1, x[c.y], acc);
     acc = fmaf(v2, x[c.z], acc);
@@ -221,7 +246,9 @@ __global__ __launch_bounds__(256) void k_fwd1(int64_t m, const int64_t* __restri
   }
 }
 
-template <int VT, int FR>
+// F frames: a wave's 64 lanes are 64/FR entry slots x FR frames; each lane
+// keeps U entries (value, index, gathered x) in flight per step.
+template <int VT, int FR, int U>
 __global__ __launch_bounds__(256) void k_fwdF(int64_t m, const int64_t* __restrict__ rp,
                                               const int* __restrict__ col,
                                               const typename Val<VT>::T* __restrict__ val,
@@ -236,7 +263,19 @@ __global__ __launch_bounds__(256) void k_fwdF(int64_t m, const int64_t* __restri
   if (i >= m) return;
   const int64_t b = rp[i], en = rp[i + 1];
   float acc = 0.0f;
-  for (int64_t k = b + slot; k < en; k += SLOTS) {
+  int64_t k = b + slot;
+  for (; k + int64_t(SLOTS) * (U - 1) < en; k += int64_t(SLOTS) * U) {
+    float v[U];
+    int c[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      v[u] = Val<VT>::get(val, k + u * SLOTS);
+      c[u] = __builtin_nontemporal_load(col + k + u * SLOTS);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) acc = fmaf(v[u], x[int64_t(c[u]) * FR + f], acc);
+  }
+  for (; k < en; k += SLOTS) {
     float v = Val<VT>::get(val, k);
     int c = __builtin_nontemporal_load(col + k);
     acc = fmaf(v, x[int64_t(c) * FR + f], acc);
@@ -320,6 +359,90 @@ __global__ __launch_bounds__(256) void k_bwd(int n, const int64_t* __restrict__ 
   }
 }
 
+// Single-frame backward staged through LDS: each workgroup owns a
+// contiguous range of columns (voxels) and walks the row blocks in order;
+// block b's slice of the ratio vector is loaded into LDS once per workgroup
+// and the column segments of that block gather from LDS instead of L2.
+// Column sums stay in LDS across blocks (fixed order, no atomics); the last
+// step applies the update and writes one FP64 partial sum per workgroup.
+template <int VT, int LPC, int U>
+__global__ __launch_bounds__(256) void k_bwd_lds(int n, int nblk, int blk_rows, int64_t m,
+                                                 int cols_per_wg,
+                                                 const int64_t* __restrict__ ptr,
+                                                 const int* __restrict__ brow,
+                                                 const typename Val<VT>::T* __restrict__ val,
+                                                 const float* __restrict__ e,
+                                                 const float* __restrict__ x,
+                                                 const float* __restrict__ s,
+                                                 float* __restrict__ xx,
+                                                 double* __restrict__ partials) {
+  extern __shared__ float lds[];
+  float* eslice = lds;               // blk_rows
+  float* cacc = lds + blk_rows;      // cols_per_wg
+  constexpr int GROUPS = 256 / LPC;
+  const int t = threadIdx.x, grp = t / LPC, g = t % LPC;
+  const int G = gridDim.x, bx = blockIdx.x;
+  const int xcd = bx % 8, q = G / 8, r8 = G % 8;
+  const int wg = xcd * q + (xcd < r8 ? xcd : r8) + bx / 8;
+  const int c0 = wg * cols_per_wg;
+  const int c1 = min(n, c0 + cols_per_wg);
+  for (int k = t; k < cols_per_wg; k += 256) cacc[k] = 0.0f;
+  for (int b = 0; b < nblk; ++b) {
+    const int64_t r0 = int64_t(b) * blk_rows;
+    const int rows = int(min(int64_t(blk_rows), m - r0));
+    __syncthreads();
+    for (int k = 4 * t; k < rows; k += 1024) {
+      if (k + 3 < rows) {
+        vf4 v = *reinterpret_cast<const vf4*>(e + r0 + k);
+        eslice[k] = v.x, eslice[k + 1] = v.y, eslice[k + 2] = v.z, eslice[k + 3] = v.w;
+      } else {
+        for (int q2 = k; q2 < rows; ++q2) eslice[q2] = e[r0 + q2];
+      }
+    }
+    __syncthreads();
+    const int64_t* pb = ptr + int64_t(b) * (n + 1);
+    for (int j = c0 + grp; j < c1; j += GROUPS) {
+      const int64_t k0 = pb[j], k1 = pb[j + 1];
+      float acc = 0.0f;
+      int64_t k = k0 + g;
+      for (; k + int64_t(LPC) * (U - 1) < k1; k += int64_t(LPC) * U) {
+        float v[U];
+        int r[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          v[u] = Val<VT>::get(val, k + u * LPC);
+          r[u] = __builtin_nontemporal_load(brow + k + u * LPC);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) acc = fmaf(v[u], eslice[r[u] - int(r0)], acc);
+      }
+      for (; k < k1; k += LPC) acc = fmaf(Val<VT>::get(val, k), eslice[brow[k] - int(r0)], acc);
+#pragma unroll
+      for (int off = LPC / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LPC);
+      if (g == 0) cacc[j - c0] += acc;
+    }
+  }
+  __syncthreads();
+  double part = 0.0;
+  for (int k = t; k < c1 - c0; k += 256) {
+    const int j = c0 + k;
+    float d = s[j];
+    d += float(d == 0.0f);
+    float v = x[j] * cacc[k] / d;
+    xx[j] = v;
+    part += double(v);
+  }
+  __shared__ double red[256 / kWave];
+  part = wave_reduce_sum(part);
+  if ((t & (kWave - 1)) == 0) red[t / kWave] = part;
+  __syncthreads();
+  if (t == 0) {
+    double tot = 0.0;
+    for (int w = 0; w < 256 / kWave; ++w) tot += red[w];
+    partials[wg] = tot;
+  }
+}
+
 // scale_f = 1/sum_b partials[b][f] (FP64, fixed order); 0 for an empty frame
 __global__ void k_norms(const double* partials, int nb, int FR, float* scale, double* norm) {
   __shared__ double red[256];
@@ -389,7 +512,11 @@ struct raptr_recon {
   float* scale = nullptr;
   double* norm = nullptr;
   int fwd_lanes = 64;
+  int fwd_unroll = 1;  // vec4 groups (1 frame) or entries (F frames) in flight
   int bwd_unroll = 4;
+  int bwd_lds = 0;     // single-frame backward staged through LDS (0: off)
+  int bwd_lpc = 16;    // lanes per column of the LDS-staged backward
+  int bwd_wgs = 512;   // workgroups of the LDS-staged backward
   size_t bytes = 0;
   std::vector<void*> allocs;
   std::map<long long, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
@@ -421,39 +548,42 @@ int launch_fwd_any(const raptr_recon* R, int F, const float* x, const float* y, 
                    hipStream_t st) {
   if (R->m == 0) return RAPTR_OK;
   const int64_t m = R->m;
+  const int U = R->fwd_unroll;
   if (F == 1) {
-    int L = R->fwd_lanes;
+    const int L = R->fwd_lanes;
     unsigned g = unsigned((m + (256 / L) - 1) / (256 / L));
-#define RAPTR_F1(VV, LL)                                                                     \
-  if (R->vt == VV && L == LL) {                                                              \
-    hipLaunchKernelGGL((k_fwd1<VV, LL>), dim3(g), dim3(256), 0, st, m, R->frp, R->fcol,       \
+#define RAPTR_F1(VV, LL, UU)                                                                 \
+  if (R->vt == VV && L == LL && U == UU) {                                                   \
+    hipLaunchKernelGGL((k_fwd1<VV, LL, UU>), dim3(g), dim3(256), 0, st, m, R->frp, R->fcol,   \
                        (const typename Val<VV>::T*)R->fval, R->rscale, x, y, e);             \
     RAPTR_LAUNCH_CHECK();                                                                    \
     return RAPTR_OK;                                                                         \
   }
-    RAPTR_F1(kF32, 16) RAPTR_F1(kF32, 32) RAPTR_F1(kF32, 64)
-    RAPTR_F1(kF16, 16) RAPTR_F1(kF16, 32) RAPTR_F1(kF16, 64)
-    RAPTR_F1(kBF16, 16) RAPTR_F1(kBF16, 32) RAPTR_F1(kBF16, 64)
-    RAPTR_F1(kU16, 16) RAPTR_F1(kU16, 32) RAPTR_F1(kU16, 64)
+#define RAPTR_F1V(VV)                                                                        \
+  RAPTR_F1(VV, 16, 1) RAPTR_F1(VV, 32, 1) RAPTR_F1(VV, 64, 1) RAPTR_F1(VV, 16, 2)            \
+  RAPTR_F1(VV, 32, 2) RAPTR_F1(VV, 64, 2)
+    RAPTR_F1V(kF32) RAPTR_F1V(kF16) RAPTR_F1V(kBF16) RAPTR_F1V(kU16)
+#undef RAPTR_F1V
 #undef RAPTR_F1
-    return fail(RAPTR_ERR_INVALID_ARG, "forward lanes %d unsupported", L);
+    return fail(RAPTR_ERR_INVALID_ARG, "forward shape lanes %d unroll %d unsupported", L, U);
   }
   unsigned g = unsigned((m + 3) / 4);
-#define RAPTR_FF(VV, FF)                                                                   \
-  if (R->vt == VV && F == FF) {                                                            \
-    hipLaunchKernelGGL((k_fwdF<VV, FF>), dim3(g), dim3(256), 0, st, m, R->frp, R->fcol,     \
+#define RAPTR_FF(VV, FF, UU)                                                               \
+  if (R->vt == VV && F == FF && U == UU) {                                                 \
+    hipLaunchKernelGGL((k_fwdF<VV, FF, UU>), dim3(g), dim3(256), 0, st, m, R->frp, R->fcol, \
                        (const typename Val<VV>::T*)R->fval, R->rscale, x, y, e);           \
     RAPTR_LAUNCH_CHECK();                                                                  \
     return RAPTR_OK;                                                                       \
   }
-  RAPTR_FF(kF32, 2) RAPTR_FF(kF32, 4) RAPTR_FF(kF32, 8) RAPTR_FF(kF32, 16) RAPTR_FF(kF32, 32)
-  RAPTR_FF(kF32, 64) RAPTR_FF(kF16, 2) RAPTR_FF(kF16, 4) RAPTR_FF(kF16, 8) RAPTR_FF(kF16, 16)
-  RAPTR_FF(kF16, 32) RAPTR_FF(kF16, 64) RAPTR_FF(kBF16, 2) RAPTR_FF(kBF16, 4) RAPTR_FF(kBF16, 8)
-  RAPTR_FF(kBF16, 16) RAPTR_FF(kBF16, 32) RAPTR_FF(kBF16, 64)
-  RAPTR_FF(kU16, 2) RAPTR_FF(kU16, 4) RAPTR_FF(kU16, 8) RAPTR_FF(kU16, 16) RAPTR_FF(kU16, 32)
-  RAPTR_FF(kU16, 64)
+#define RAPTR_FFU(VV, FF) RAPTR_FF(VV, FF, 2) RAPTR_FF(VV, FF, 4) RAPTR_FF(VV, FF, 8)
+#define RAPTR_FFV(VV)                                                                      \
+  RAPTR_FFU(VV, 2) RAPTR_FFU(VV, 4) RAPTR_FFU(VV, 8) RAPTR_FFU(VV, 16) RAPTR_FFU(VV, 32)    \
+  RAPTR_FFU(VV, 64)
+  RAPTR_FFV(kF32) RAPTR_FFV(kF16) RAPTR_FFV(kBF16) RAPTR_FFV(kU16)
+#undef RAPTR_FFV
+#undef RAPTR_FFU
 #undef RAPTR_FF
-  return fail(RAPTR_ERR_INVALID_ARG, "frame count %d unsupported (1..64, power of two)", F);
+  return fail(RAPTR_ERR_INVALID_ARG, "frame count %d / unroll %d unsupported", F, U);
 }
 
 int launch_bwd_any(const raptr_recon* R, int F, int blk, const float* x, hipStream_t st,
@@ -486,10 +616,53 @@ int launch_bwd_any(const raptr_recon* R, int F, int blk, const float* x, hipStre
   return fail(RAPTR_ERR_INVALID_ARG, "backward configuration unsupported");
 }
 
+// LDS capacity for the staged backward (one e slice + column sums)
+constexpr int kLdsRowsMax = 24576;
+
+int launch_bwd_lds(const raptr_recon* R, const float* x, hipStream_t st, int* nb_out) {
+  const int n = R->n;
+  int G = std::min(R->bwd_wgs, std::max(1, (n + 63) / 64));
+  const int cpw = (n + G - 1) / G;
+  G = (n + cpw - 1) / cpw;
+  if (nb_out) *nb_out = G;
+  size_t lds = size_t(R->blk_rows + cpw) * sizeof(float);
+  if (lds > size_t(160) * 1024)
+    return fail(RAPTR_ERR_INVALID_ARG, "staged backward needs %zu B of LDS (> 160 KiB)", lds);
+  const int L = R->bwd_lpc, U = R->bwd_unroll > 4 ? 4 : R->bwd_unroll;
+#define RAPTR_BL(VV, LL, UU)                                                                \
+  if (R->vt == VV && L == LL && U == UU) {                                                  \
+    static bool attr = false;                                                               \
+    if (!attr) {                                                                            \
+      if (hipFuncSetAttribute((const void*)k_bwd_lds<VV, LL, UU>,                           \
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) !=    \
+          hipSuccess)                                                                       \
+        (void)hipGetLastError();                                                            \
+      attr = true;                                                                          \
+    }                                                                                       \
+    hipLaunchKernelGGL((k_bwd_lds<VV, LL, UU>), dim3(G), dim3(256), lds, st, n, R->nblk,     \
+                       R->blk_rows, int64_t(R->m), cpw, (const int64_t*)R->bptr,            \
+                       (const int*)R->brow, (const typename Val<VV>::T*)R->bval,            \
+                       (const float*)R->e, x, (const float*)R->s, R->xx, R->partials);      \
+    RAPTR_LAUNCH_CHECK();                                                                   \
+    return RAPTR_OK;                                                                        \
+  }
+#define RAPTR_BLV(VV)                                                                       \
+  RAPTR_BL(VV, 8, 2) RAPTR_BL(VV, 8, 4) RAPTR_BL(VV, 16, 2) RAPTR_BL(VV, 16, 4)              \
+  RAPTR_BL(VV, 32, 2) RAPTR_BL(VV, 32, 4)
+  RAPTR_BLV(kF32) RAPTR_BLV(kF16) RAPTR_BLV(kBF16) RAPTR_BLV(kU16)
+#undef RAPTR_BLV
+#undef RAPTR_BL
+  return fail(RAPTR_ERR_INVALID_ARG, "staged backward shape unsupported");
+}
+
 int iteration(raptr_recon* R, int F, const float* y, float* x, hipStream_t st) {
   RAPTR_TRY(launch_fwd_any(R, F, x, y, R->e, st));
   int nb = 0;
-  for (int b = 0; b < R->nblk; ++b) RAPTR_TRY(launch_bwd_any(R, F, b, x, st, &nb));
+  if (F == 1 && R->bwd_lds) {
+    RAPTR_TRY(launch_bwd_lds(R, x, st, &nb));
+  } else {
+    for (int b = 0; b < R->nblk; ++b) RAPTR_TRY(launch_bwd_any(R, F, b, x, st, &nb));
+  }
   hipLaunchKernelGGL(k_norms, dim3(1), dim3(256), 0, st, (const double*)R->partials, nb, F,
                      R->scale, R->norm);
   RAPTR_LAUNCH_CHECK();
@@ -654,6 +827,8 @@ int raptr_recon_create(raptr_recon** out, int m, int n, int64_t nnz, const int64
   hipError_t se = hipStreamSynchronize(st);
   if (se != hipSuccess) return bail(fail(RAPTR_ERR_DEVICE, "create: %s", hipGetErrorString(se)));
   R->fwd_lanes = double(nnz) / std::max(m, 1) >= 192 ? 32 : 16;
+  R->fwd_unroll = max_frames == 1 ? 1 : 4;
+  R->bwd_lds = (max_frames == 1 && block_rows <= kLdsRowsMax && block_rows % 4 == 0) ? 1 : 0;
   *out = R;
   return RAPTR_OK;
 }
@@ -670,23 +845,30 @@ int raptr_recon_destroy(raptr_recon* R) {
 }
 
 int raptr_recon_info(const raptr_recon* R, int64_t* device_bytes, int64_t* padded_nnz,
-                     int* row_blocks, int* fwd_lanes, int* bwd_unroll) {
+                     int* row_blocks, int* fwd_lanes, int* fwd_unroll, int* bwd_unroll) {
   if (!R) return fail(RAPTR_ERR_INVALID_ARG, "null plan");
   if (device_bytes) *device_bytes = int64_t(R->bytes);
   if (padded_nnz) *padded_nnz = R->pnnz;
   if (row_blocks) *row_blocks = R->nblk;
   if (fwd_lanes) *fwd_lanes = R->fwd_lanes;
+  if (fwd_unroll) *fwd_unroll = R->fwd_unroll;
   if (bwd_unroll) *bwd_unroll = R->bwd_unroll;
   return RAPTR_OK;
 }
 
-int raptr_recon_set_config(raptr_recon* R, int fwd_lanes, int bwd_unroll) {
+int raptr_recon_set_config(raptr_recon* R, int fwd_lanes, int fwd_unroll, int bwd_unroll) {
   if (!R) return fail(RAPTR_ERR_INVALID_ARG, "null plan");
   if (fwd_lanes != 16 && fwd_lanes != 32 && fwd_lanes != 64)
     return fail(RAPTR_ERR_INVALID_ARG, "forward lanes must be 16, 32 or 64");
+  bool fu_ok = R->fmax == 1 ? (fwd_unroll == 1 || fwd_unroll == 2)
+                            : (fwd_unroll == 2 || fwd_unroll == 4 || fwd_unroll == 8);
+  if (!fu_ok)
+    return fail(RAPTR_ERR_INVALID_ARG,
+                "forward unroll must be 1 or 2 (one frame) or 2, 4, 8 (batched)");
   if (bwd_unroll != 2 && bwd_unroll != 4 && bwd_unroll != 8)
     return fail(RAPTR_ERR_INVALID_ARG, "backward unroll must be 2, 4 or 8");
-  if (R->fwd_lanes != fwd_lanes || R->bwd_unroll != bwd_unroll) {
+  if (R->fwd_lanes != fwd_lanes || R->fwd_unroll != fwd_unroll ||
+      R->bwd_unroll != bwd_unroll) {
     for (auto& kv : R->graphs) {
       if (kv.second.second) (void)hipGraphExecDestroy(kv.second.second);
       if (kv.second.first) (void)hipGraphDestroy(kv.second.first);
@@ -694,7 +876,39 @@ int raptr_recon_set_config(raptr_recon* R, int fwd_lanes, int bwd_unroll) {
     R->graphs.clear();
   }
   R->fwd_lanes = fwd_lanes;
+  R->fwd_unroll = fwd_unroll;
   R->bwd_unroll = bwd_unroll;
+  return RAPTR_OK;
+}
+
+int raptr_recon_set_backward(raptr_recon* R, int staged, int lanes_per_column, int workgroups) {
+  if (!R) return fail(RAPTR_ERR_INVALID_ARG, "null plan");
+  if (staged && (R->fmax != 1 || R->blk_rows > kLdsRowsMax || R->blk_rows % 4))
+    return fail(RAPTR_ERR_INVALID_ARG,
+                "staged backward needs a single-frame plan with block_rows <= %d, "
+                "a multiple of 4", kLdsRowsMax);
+  if (lanes_per_column != 8 && lanes_per_column != 16 && lanes_per_column != 32)
+    return fail(RAPTR_ERR_INVALID_ARG, "lanes per column must be 8, 16 or 32");
+  if (workgroups < 1) return fail(RAPTR_ERR_INVALID_ARG, "workgroups must be positive");
+  if (R->bwd_lds != staged || R->bwd_lpc != lanes_per_column || R->bwd_wgs != workgroups) {
+    for (auto& kv : R->graphs) {
+      if (kv.second.second) (void)hipGraphExecDestroy(kv.second.second);
+      if (kv.second.first) (void)hipGraphDestroy(kv.second.first);
+    }
+    R->graphs.clear();
+  }
+  R->bwd_lds = staged ? 1 : 0;
+  R->bwd_lpc = lanes_per_column;
+  R->bwd_wgs = workgroups;
+  return RAPTR_OK;
+}
+
+int raptr_recon_get_backward(const raptr_recon* R, int* staged, int* lanes_per_column,
+                             int* workgroups) {
+  if (!R) return fail(RAPTR_ERR_INVALID_ARG, "null plan");
+  if (staged) *staged = R->bwd_lds;
+  if (lanes_per_column) *lanes_per_column = R->bwd_lpc;
+  if (workgroups) *workgroups = R->bwd_wgs;
   return RAPTR_OK;
 }
 
@@ -748,6 +962,7 @@ int raptr_recon_forward(raptr_recon* R, int nframes, const float* d_x, const flo
 int raptr_recon_backward(raptr_recon* R, int nframes, const float* d_x, raptr_stream_t stream) {
   if (!R || !valid_frames(nframes) || nframes > R->fmax)
     return fail(RAPTR_ERR_INVALID_ARG, "bad plan or frame count");
+  if (nframes == 1 && R->bwd_lds) return launch_bwd_lds(R, d_x, S(stream), nullptr);
   for (int b = 0; b < R->nblk; ++b)
     RAPTR_TRY(launch_bwd_any(R, nframes, b, d_x, S(stream), nullptr));
   return RAPTR_OK;

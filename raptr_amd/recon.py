@@ -28,14 +28,26 @@ class Reconstructor:
 
     def info(self):
         b, p = C.c_int64(), C.c_int64()
-        nb, fl, bu = C.c_int(), C.c_int(), C.c_int()
+        nb, fl, fu, bu = C.c_int(), C.c_int(), C.c_int(), C.c_int()
         capi.call("raptr_recon_info", self.handle, C.byref(b), C.byref(p), C.byref(nb),
-                  C.byref(fl), C.byref(bu))
+                  C.byref(fl), C.byref(fu), C.byref(bu))
+        st, lpc, wgs = C.c_int(), C.c_int(), C.c_int()
+        capi.call("raptr_recon_get_backward", self.handle, C.byref(st), C.byref(lpc),
+                  C.byref(wgs))
         return {"device_bytes": b.value, "padded_nnz": p.value, "row_blocks": nb.value,
-                "fwd_lanes": fl.value, "bwd_unroll": bu.value}
+                "fwd_lanes": fl.value, "fwd_unroll": fu.value, "bwd_unroll": bu.value,
+                "bwd_staged": st.value, "bwd_lanes_per_column": lpc.value,
+                "bwd_workgroups": wgs.value}
 
-    def set_config(self, fwd_lanes, bwd_unroll):
-        capi.call("raptr_recon_set_config", self.handle, int(fwd_lanes), int(bwd_unroll))
+    def set_config(self, fwd_lanes, fwd_unroll, bwd_unroll):
+        capi.call("raptr_recon_set_config", self.handle, int(fwd_lanes), int(fwd_unroll),
+                  int(bwd_unroll))
+
+    def set_backward(self, staged, lanes_per_column=16, workgroups=512):
+        """Single-frame backward through LDS slices of the ratio vector
+        (``staged``) or block by block through L2."""
+        capi.call("raptr_recon_set_backward", self.handle, 1 if staged else 0,
+                  int(lanes_per_column), int(workgroups))
 
     def run(self, y, x, niter, nframes=1, use_graph=True, stream=None):
         """In-place MLEM on device arrays y ([m][F]) and x ([n][F])."""
@@ -56,27 +68,62 @@ class Reconstructor:
         return out if F == 1 else out.reshape(self.n, F)
 
     def autotune(self, y, x, nframes=1, reps=3, stream=None):
-        """Time every (forward lanes, backward unroll) shape on this plan with
-        HIP events and keep the fastest; returns (config, ms per iteration)."""
+        """Time the kernel shapes on this plan with HIP events and keep the
+        fastest: forward shapes first (lanes per row, unroll) with the
+        current backward, then backward shapes (unroll; for one frame also
+        the LDS-staged variant's lanes per column and workgroup count).
+        Returns (config dict, ms per iteration)."""
         from .device import Event
         e0, e1 = Event(), Event()
         x_save = DeviceArray(x.count, np.float32)
         capi.call("raptr_memcpy_d2d", x_save.ptr, x.ptr, x.nbytes, stream)
+
+        def timed():
+            self.run(y, x, 1, nframes, use_graph=False, stream=stream)
+            e0.record(stream)
+            self.run(y, x, reps, nframes, use_graph=False, stream=stream)
+            e1.record(stream)
+            return e0.elapsed_ms(e1) / reps
+
+        inf = self.info()
+        bu = inf["bwd_unroll"]
+        if nframes == 1:
+            fshapes = [(fl, fu) for fl in (16, 32, 64) for fu in (1, 2)]
+        else:
+            fshapes = [(inf["fwd_lanes"], fu) for fu in (2, 4, 8)]
+        best_f = None
+        for fl, fu in fshapes:
+            self.set_config(fl, fu, bu)
+            ms = timed()
+            if best_f is None or ms < best_f[1]:
+                best_f = ((fl, fu), ms)
+        fl, fu = best_f[0]
+        bshapes = [(0, 16, 512, u) for u in (2, 4, 8)]
+        staged_ok = nframes == 1 and self.max_frames == 1
+        if staged_ok:
+            try:  # the plan's row blocks may be too large for LDS slices
+                self.set_backward(True)
+            except capi.RaptrError:
+                staged_ok = False
+        if staged_ok:
+            bshapes += [(1, lpc, wgs, u) for lpc in (8, 16, 32) for wgs in (256, 512, 1024)
+                        for u in (2, 4)]
         best = None
-        lanes = (16, 32, 64) if nframes == 1 else (self.info()["fwd_lanes"],)
-        for fl in lanes:
-            for bu in (2, 4, 8):
-                self.set_config(fl, bu)
-                self.run(y, x, 1, nframes, use_graph=False, stream=stream)
-                e0.record(stream)
-                self.run(y, x, reps, nframes, use_graph=False, stream=stream)
-                e1.record(stream)
-                ms = e0.elapsed_ms(e1) / reps
-                if best is None or ms < best[1]:
-                    best = ((fl, bu), ms)
-        self.set_config(*best[0])
+        for st, lpc, wgs, u in bshapes:
+            if staged_ok:
+                self.set_backward(st, lpc, wgs)
+            self.set_config(fl, fu, u)
+            ms = timed()
+            if best is None or ms < best[1]:
+                best = ((st, lpc, wgs, u), ms)
+        st, lpc, wgs, u = best[0]
+        if staged_ok:
+            self.set_backward(st, lpc, wgs)
+        self.set_config(fl, fu, u)
         capi.call("raptr_memcpy_d2d", x.ptr, x_save.ptr, x.nbytes, stream)
-        return best
+        cfg = {"fwd_lanes": fl, "fwd_unroll": fu, "bwd_unroll": u, "bwd_staged": st,
+               "bwd_lanes_per_column": lpc, "bwd_workgroups": wgs}
+        return cfg, best[1]
 
     def close(self):
         if getattr(self, "handle", None):

@@ -61,6 +61,42 @@ def test_recon_f32_vs_fp64_oracle(gpu, oracle, small_system, block_rows):
         assert abs(float(x.astype(np.float64).sum()) - 1.0) < 1e-5
 
 
+@pytest.mark.parametrize("lpc,wgs,unroll", [(8, 512, 2), (16, 64, 4), (32, 1024, 4)])
+def test_recon_staged_backward(gpu, oracle, small_system, lpc, wgs, unroll):
+    """The LDS-staged single-frame backward (column ranges per workgroup,
+    ratio slices through LDS) against the FP64 oracle and against the
+    block-by-block backward (different summation order: 1e-6)."""
+    from raptr_amd.recon import Reconstructor
+    S = small_system
+    n = S["grid"].nvox
+    x0 = np.full(n, 1.0 / n, np.float32)
+    x64 = _oracle_mlem(oracle, S, S["y"], 5)
+    R = Reconstructor(S["A"], S["sens"], "f32", 1, block_rows=800)
+    assert R.info()["bwd_staged"] == 1 and R.info()["row_blocks"] > 8
+    R.set_config(16, 1, unroll)
+    R.set_backward(True, lpc, wgs)
+    a = R.mlem(S["y"], x0, 5)
+    R.set_backward(False)
+    b = R.mlem(S["y"], x0, 5)
+    assert _rel(a, x64) <= 1e-4
+    assert _rel(a, b) <= 1e-6
+    with pytest.raises(Exception):
+        R.set_backward(True, 12, 512)
+
+
+def test_recon_staged_backward_needs_small_slices(gpu, small_system):
+    from raptr_amd import RaptrError
+    from raptr_amd.recon import Reconstructor
+    S = small_system
+    R = Reconstructor(S["A"], S["sens"], "f32", 1, block_rows=30000)
+    assert R.info()["bwd_staged"] == 0
+    with pytest.raises(RaptrError):
+        R.set_backward(True)
+    Rb = Reconstructor(S["A"], S["sens"], "f32", 2)
+    with pytest.raises(RaptrError):
+        Rb.set_backward(True)
+
+
 def test_recon_row_block_cut_is_immaterial(gpu, small_system):
     from raptr_amd.recon import Reconstructor
     S = small_system
