@@ -5,6 +5,7 @@ import this module.  It loads ``oracle/_build/libraptr_oracle.so`` (the C
 restatement) and, when it has been built in a container that holds the raptr
 sources, ``oracle/_ref/libraptr_ref.so`` (the reference's own getWorkqueue).
 """
+import contextlib
 import ctypes as C
 import os
 
@@ -53,51 +54,67 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"oracle library missing: {LIB_PATH} (run make -C oracle)")
-        L = C.CDLL(LIB_PATH)
-        P = C.POINTER
-        S, G = P(Setup), P(Grid)
-        i64, f32, f64, i32, u32 = C.c_int64, C.c_float, C.c_double, C.c_int, C.c_uint32
-        sig = {
-            "ro_nchannels": (i64, [S]),
-            "ro_cnl_split": (None, [S, i32] + [P(i32)] * 5),
-            "ro_cnl_lin": (i32, [S, i32, i32, i32, i32, i32]),
-            "ro_vxl_split": (None, [G, i32, P(i32), P(i32), P(i32)]),
-            "ro_vxl_lin": (i32, [G, i32, i32, i32]),
-            "ro_angle_table": (None, [S, P(f32), P(f32)]),
-            "ro_trafo": (None, [S, i32, f32, f32, i32, i32, P(f32), P(f32)]),
-            "ro_distance": (f32, [P(f32), P(f32), P(f32)]),
-            "ro_workqueue": (i64, [S, G, P(i32), i32, P(i32), P(i32), i64, P(i32), P(i32)]),
-            "ro_workqueue_counts": (None, [S, G, P(i32), i32, P(i64)]),
-            "ro_philox4x32_10": (None, [P(u32), P(u32), P(u32)]),
-            "ro_u01": (f32, [u32]),
-            "ro_set_threads": (None, [i32]),
-            "ro_get_threads": (i32, []),
-            "ro_ray": (None, [S, P(f32), P(f32), i32, u32, i32, i32, P(f32)]),
-            "ro_intersection_length": (f32, [P(f32), P(f32)]),
-            "ro_intersection_length_f64": (f64, [P(f64), P(f64)]),
-            "ro_voxel_box": (None, [G, i32, P(f32)]),
-            "ro_sm_from_wq": (None, [S, G, P(i32), P(i32), P(f32), i64, i32, u32, i32]),
-            "ro_sm_from_wq_f64": (None, [S, G, P(i32), P(i32), P(f64), i64, i32, u32, i32]),
-            "ro_get_chords": (i32, [P(f32), P(i32), i32, P(f32), P(f32), P(f32), P(i32)]),
-            "ro_build_csr": (i64, [S, G, P(i32), i32, i32, u32, i32, P(i64), P(i32), P(f32), P(f64)]),
-            "ro_coo_sort": (None, [P(f32), P(i32), P(i32), i64]),
-            "ro_coo2csr": (None, [P(i32), P(i32), i64, i32]),
-            "ro_csr2ecsr": (None, [P(i32), P(i32), i32, P(i32), i32]),
-            "ro_csrmv_f32": (None, [i32, i32, i32, f32, P(f32), P(i32), P(i32), P(f32), f32, P(f32)]),
-            "ro_csrmv_f64": (None, [i32, i32, i32, f64, P(f64), P(i64), P(i32), P(f64), f64, P(f64)]),
-            "ro_divides": (None, [P(f32), P(f32), P(f32), i32]),
-            "ro_divides_multiplies": (None, [P(f32), P(f32), P(f32), P(f32), i32]),
-            "ro_scales": (None, [P(f32), f32, i32]),
-            "ro_sum": (f32, [P(f32), i32]),
-            "ro_mlem_f64": (None, [i32, i32, P(i64), P(i32), P(f64), P(f64), P(f64), P(f64), i32]),
-            "ro_sensitivity_f64": (None, [i32, i32, P(i64), P(i32), P(f64), P(f64)]),
-        }
-        for name, (res, args) in sig.items():
-            fn = getattr(L, name)
-            fn.restype = res
-            fn.argtypes = args
-        _lib = L
+        _lib = _bind(C.CDLL(LIB_PATH))
     return _lib
+
+
+@contextlib.contextmanager
+def swap_library(path):
+    """Run the bindings against another build of the oracle (the pin tests
+    load builds with a planted mistake)."""
+    global _lib
+    saved = _lib
+    _lib = _bind(C.CDLL(path))
+    try:
+        yield _lib
+    finally:
+        _lib = saved
+
+
+def _bind(L):
+    P = C.POINTER
+    S, G = P(Setup), P(Grid)
+    i64, f32, f64, i32, u32 = C.c_int64, C.c_float, C.c_double, C.c_int, C.c_uint32
+    sig = {
+        "ro_nchannels": (i64, [S]),
+        "ro_cnl_split": (None, [S, i32] + [P(i32)] * 5),
+        "ro_cnl_lin": (i32, [S, i32, i32, i32, i32, i32]),
+        "ro_vxl_split": (None, [G, i32, P(i32), P(i32), P(i32)]),
+        "ro_vxl_lin": (i32, [G, i32, i32, i32]),
+        "ro_angle_table": (None, [S, P(f32), P(f32)]),
+        "ro_trafo": (None, [S, i32, f32, f32, i32, i32, P(f32), P(f32)]),
+        "ro_distance": (f32, [P(f32), P(f32), P(f32)]),
+        "ro_workqueue": (i64, [S, G, P(i32), i32, P(i32), P(i32), i64, P(i32), P(i32)]),
+        "ro_workqueue_counts": (None, [S, G, P(i32), i32, P(i64)]),
+        "ro_philox4x32_10": (None, [P(u32), P(u32), P(u32)]),
+        "ro_u01": (f32, [u32]),
+        "ro_set_threads": (None, [i32]),
+        "ro_get_threads": (i32, []),
+        "ro_ray": (None, [S, P(f32), P(f32), i32, u32, i32, i32, P(f32)]),
+        "ro_intersection_length": (f32, [P(f32), P(f32)]),
+        "ro_intersection_length_f64": (f64, [P(f64), P(f64)]),
+        "ro_voxel_box": (None, [G, i32, P(f32)]),
+        "ro_sm_from_wq": (None, [S, G, P(i32), P(i32), P(f32), i64, i32, u32, i32]),
+        "ro_sm_from_wq_f64": (None, [S, G, P(i32), P(i32), P(f64), i64, i32, u32, i32]),
+        "ro_get_chords": (i32, [P(f32), P(i32), i32, P(f32), P(f32), P(f32), P(i32)]),
+        "ro_build_csr": (i64, [S, G, P(i32), i32, i32, u32, i32, P(i64), P(i32), P(f32), P(f64)]),
+        "ro_coo_sort": (None, [P(f32), P(i32), P(i32), i64]),
+        "ro_coo2csr": (None, [P(i32), P(i32), i64, i32]),
+        "ro_csr2ecsr": (None, [P(i32), P(i32), i32, P(i32), i32]),
+        "ro_csrmv_f32": (None, [i32, i32, i32, f32, P(f32), P(i32), P(i32), P(f32), f32, P(f32)]),
+        "ro_csrmv_f64": (None, [i32, i32, i32, f64, P(f64), P(i64), P(i32), P(f64), f64, P(f64)]),
+        "ro_divides": (None, [P(f32), P(f32), P(f32), i32]),
+        "ro_divides_multiplies": (None, [P(f32), P(f32), P(f32), P(f32), i32]),
+        "ro_scales": (None, [P(f32), f32, i32]),
+        "ro_sum": (f32, [P(f32), i32]),
+        "ro_mlem_f64": (None, [i32, i32, P(i64), P(i32), P(f64), P(f64), P(f64), P(f64), i32]),
+        "ro_sensitivity_f64": (None, [i32, i32, P(i64), P(i32), P(f64), P(f64)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    return L
 
 
 def ref_available():

@@ -18,6 +18,12 @@
 #include <omp.h>
 #endif
 
+/* Planted mistakes for the pin tests (tests/test_oracle_pins.py): a build
+ * with -DRO_MUTANT=k carries mistake k; the default build has none. */
+#ifndef RO_MUTANT
+#define RO_MUTANT 0
+#endif
+
 #define RO_PI_F 3.14159265358979323846f
 
 /* ======================================================================
@@ -44,6 +50,11 @@ void ro_cnl_split(const ro_setup* s, int cnl, int* ida, int* id0z, int* id0y,
   t %= p2;
   *id1z = t / p1;
   *id1y = t % p1;
+  if (RO_MUTANT == 2) { /* transposed index */
+    int q = *id1z;
+    *id1z = *id1y;
+    *id1y = q;
+  }
 }
 
 int ro_cnl_lin(const ro_setup* s, int id0z, int id0y, int id1z, int id1y,
@@ -111,7 +122,7 @@ void ro_trafo(const ro_setup* s, int det, float sn, float cs, int idz, int idy,
   t[5] = e1;
   t[6] = 0.0f;
   t[7] = p1 - 0.5f * e1;
-  t[8] = -sn * e0;
+  t[8] = RO_MUTANT == 1 ? sn * e0 : -sn * e0; /* 1: wrong sign */
   t[9] = 0.0f;
   t[10] = cs * e2;
   t[11] = -sn * (p0 - 0.5f * e0) + cs * (p2 - 0.5f * e2);
@@ -137,7 +148,7 @@ float ro_distance(const float a[3], const float b[3], const float p[3]) {
   }
   for (int d = 0; d < 3; ++d)
     c[d] = ab[(d + 1) % 3] * ap[(d + 2) % 3] - ab[(d + 2) % 3] * ap[(d + 1) % 3];
-  return ro_absolute(c) / ro_absolute(ab);
+  return ro_absolute(c) / ro_absolute(RO_MUTANT == 9 ? ap : ab); /* 9: wrong operand */
 }
 
 /* ======================================================================
@@ -153,7 +164,7 @@ typedef struct {
 static float ro_sum_radii(const ro_setup* s, const ro_grid* g) {
   float ve[3] = {g->d[0], g->d[1], g->d[2]};
   float pe[3] = {s->segx, s->segy, s->segz};
-  return 0.5f * (ro_absolute(ve) + ro_absolute(pe));
+  return 0.5f * (ro_absolute(ve) + (RO_MUTANT == 3 ? 0.0f : ro_absolute(pe))); /* 3: dropped term */
 }
 
 static void ro_channel_centres(const ro_setup* s, const float* sn,
@@ -247,7 +258,7 @@ void ro_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2],
                       uint32_t out[4]) {
   uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
   uint32_t k0 = key[0], k1 = key[1];
-  for (int r = 0; r < 10; ++r) {
+  for (int r = 0; r < (RO_MUTANT == 7 ? 9 : 10); ++r) { /* 7: a round short */
     if (r) {
       k0 += 0x9E3779B9u;
       k1 += 0xBB67AE85u;
@@ -350,6 +361,7 @@ float ro_intersection_length(const float v[6], const float r[6]) {
   }
   if (!(amin < amax)) return 0.0f;
   float dx = r[3] - r[0], dy = r[4] - r[1], dz = r[5] - r[2];
+  if (RO_MUTANT == 4) return amax - amin; /* dropped factor |ray| */
   return (amax - amin) * sqrtf(dx * dx + dy * dy + dz * dz);
 }
 
@@ -641,6 +653,7 @@ typedef struct {
 static int ro_coo_cmp(const void* pa, const void* pb) {
   const ro_coo_item* a = (const ro_coo_item*)pa;
   const ro_coo_item* b = (const ro_coo_item*)pb;
+  if (RO_MUTANT == 8 && a->col != b->col) return a->col < b->col ? -1 : 1; /* keys swapped */
   if (a->row != b->row) return a->row < b->row ? -1 : 1;
   if (a->col != b->col) return a->col < b->col ? -1 : 1;
   return a->pos < b->pos ? -1 : (a->pos > b->pos);
@@ -675,7 +688,7 @@ void ro_coo2csr(int* row_ptr, const int* row_id, int64_t nnz, int m) {
 
 /* A16: effective row pointer, convertCsr2Ecsr.hpp:57-63. */
 void ro_csr2ecsr(int* ecsr, const int* y, int nnzr, const int* csr, int m) {
-  for (int i = 0; i < nnzr; ++i) ecsr[i] = csr[y[i]];
+  for (int i = 0; i < nnzr; ++i) ecsr[i] = csr[RO_MUTANT == 5 ? i : y[i]]; /* 5: wrong index */
   ecsr[nnzr] = csr[m];
 }
 
@@ -775,6 +788,7 @@ void ro_mlem_f64(int m, int nvox, const int64_t* rp, const int* col,
   double ss = 0.0;
   for (int j = 0; j < nvox; ++j) ss += sens[j];
   for (int j = 0; j < nvox; ++j) s[j] = sens[j] / ss;
+  int used = 1; /* team size of the back-projection region */
   for (int it = 0; it < niter; ++it) {
 #pragma omp parallel for schedule(static)
     for (int i = 0; i < m; ++i) {
@@ -789,6 +803,7 @@ void ro_mlem_f64(int m, int nvox, const int64_t* rp, const int* col,
       t = omp_get_thread_num();
       T = omp_get_num_threads();
 #endif
+      if (t == 0) used = T;
       double* p = part + (size_t)nvox * (size_t)t;
       for (int j = 0; j < nvox; ++j) p[j] = 0.0;
       int64_t lo = (int64_t)m * t / T, hi = (int64_t)m * (t + 1) / T;
@@ -798,7 +813,7 @@ void ro_mlem_f64(int m, int nvox, const int64_t* rp, const int* col,
 #pragma omp parallel for schedule(static)
     for (int j = 0; j < nvox; ++j) {
       double acc = 0.0;
-      for (int t = 0; t < nt; ++t) acc += part[(size_t)nvox * (size_t)t + j];
+      for (int t = 0; t < used; ++t) acc += part[(size_t)nvox * (size_t)t + j];
       c[j] = acc;
     }
     double norm = 0.0;
@@ -808,7 +823,8 @@ void ro_mlem_f64(int m, int nvox, const int64_t* rp, const int* col,
       x[j] = x[j] * c[j] / d;
       norm += x[j];
     }
-    for (int j = 0; j < nvox; ++j) x[j] = x[j] * (1.0 / norm);
+    if (RO_MUTANT != 6) /* 6: normalisation dropped */
+      for (int j = 0; j < nvox; ++j) x[j] = x[j] * (1.0 / norm);
   }
   free(part);
   free(s);
