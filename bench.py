@@ -56,6 +56,10 @@ def parse():
                     help="skip the FP64 host recurrence (no error figures either)")
     ap.add_argument("--no-small", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=10)
+    ap.add_argument("--modes", default="plain,batched,mixed_u16,mixed_f16,mixed_bf16",
+                    help="comma-separated subset of the modes to run")
+    ap.add_argument("--block-slices-mb", default="2,8,32",
+                    help="ratio-slice sizes (MB) of the batched plans' row blocks to try")
     return ap.parse_args()
 
 
@@ -87,6 +91,12 @@ def main():
         fn()
         ev1.record(st)
         return ev0.elapsed_ms(ev1)
+
+    copy = copy_peak(timed, st)
+    log(f"device copy: {copy['GBps']:.0f} GB/s (read + write) over {copy['bytes'] >> 20} MiB")
+    if args.modes == "copy":
+        print(json.dumps({"partial": True, "copy": copy, "device": dev}), flush=True)
+        return
 
     # ------------------------------------------------------------ data
     setup, grid, nrays = G.config_full()
@@ -122,16 +132,43 @@ def main():
 
     modes = {}
     plans = {}
+    wanted = args.modes.split(",")
+    slices_mb = [float(v) for v in args.block_slices_mb.split(",")]
+    best_rows = None
     for name, vt, nf in (("plain", "f32", 1), ("batched", "f32", F), ("mixed_u16", "u16", F),
                          ("mixed_f16", "f16", F), ("mixed_bf16", "bf16", F)):
-        t = time.perf_counter()
-        R = Reconstructor(A, sens, vt, nf, stream=st)
-        t_plan = time.perf_counter() - t
+        if name not in wanted:
+            continue
         yy = Y[:, 0].copy() if nf == 1 else Y
         dy = DeviceArray.from_numpy(yy.reshape(-1))
         xx0 = x0 if nf == 1 else np.repeat(x0[:, None], nf, axis=1)
         dx = DeviceArray.from_numpy(xx0.reshape(-1))
-        cfg, ms_it = R.autotune(dy, dx, nf, reps=2, stream=st)
+        # row blocks of the backward: for F frames try several ratio-slice
+        # sizes (the first batched mode picks, the others reuse its choice)
+        if nf == 1:
+            rows_try = [0]
+        elif best_rows is not None:
+            rows_try = [best_rows]
+        else:
+            rows_try = [max(4, int(mb * 2**20 / (4 * nf)) // 4 * 4) for mb in slices_mb]
+        R, t_plan, cfg, ms_it = None, 0.0, None, None
+        for rows in rows_try:
+            t = time.perf_counter()
+            Rc = Reconstructor(A, sens, vt, nf, block_rows=rows, stream=st)
+            tp = time.perf_counter() - t
+            cc, mc = Rc.autotune(dy, dx, nf, reps=2, stream=st)
+            if len(rows_try) > 1:
+                log(f"{name}: block_rows {rows} ({Rc.info()['row_blocks']} blocks) -> "
+                    f"{mc:.2f} ms/iteration")
+            if ms_it is None or mc < ms_it:
+                if R is not None:
+                    R.close()
+                R, t_plan, cfg, ms_it = Rc, tp, cc, mc
+                chosen_rows = rows
+            else:
+                Rc.close()
+        if nf > 1:
+            best_rows = chosen_rows
         plans[name] = (R, dy, dx, xx0, nf)
         for _ in range(args.warmup):
             dx.copy_from(xx0, stream=st)
@@ -169,9 +206,18 @@ def main():
             f"passes {modes[name]['passes_ms']})")
         if name != "plain":
             # each batched frame against the plain single-frame run of frame 0
-            modes[name]["rel_l2_vs_plain_frame0"] = rel_l2(xr[:, 0], modes["plain"]["image0"])
+            if "plain" in modes:
+                modes[name]["rel_l2_vs_plain_frame0"] = rel_l2(xr[:, 0],
+                                                               modes["plain"]["image0"])
             R.close()
             del plans[name], R, dy, dx
+
+    if "plain" not in modes:
+        # a partial run (profiling a subset of modes): per-mode figures only
+        for name in modes:
+            modes[name].pop("image0")
+        print(json.dumps({"partial": True, "modes": modes, "device": dev}), flush=True)
+        return
 
     # ------------------------------------------- roofline of the plain path
     R, dy, dx, _, _ = plans["plain"]
@@ -278,10 +324,27 @@ def main():
         "build": {"matrix_ms": round(ms_build, 1), "sensitivity_ms": round(ms_sens, 1),
                   "data_s": round(t_data, 2)},
         "batched_small": small,
+        "device_copy": copy,
         "cpu_baseline": cpu,
         "device": dev,
     }
     print(json.dumps(out), flush=True)
+
+
+def copy_peak(timed, st, nbytes=2 << 30, reps=5):
+    """Device-to-device copy of nbytes (the HIP runtime's copy kernel): the
+    box's attainable read+write bandwidth, and the known-size dispatch that
+    calibrates the FETCH_SIZE / WRITE_SIZE counters."""
+    from raptr_amd import _capi as capi
+    from raptr_amd.device import DeviceArray
+    a = DeviceArray(nbytes // 4, np.float32)
+    b = DeviceArray(nbytes // 4, np.float32)
+    capi.call("raptr_memset", a.ptr, 0, nbytes, st)
+    capi.call("raptr_memcpy_d2d", b.ptr, a.ptr, nbytes, st)
+    ms = timed(lambda: [capi.call("raptr_memcpy_d2d", b.ptr, a.ptr, nbytes, st)
+                        for _ in range(reps)]) / reps
+    del a, b
+    return {"bytes": nbytes, "ms": round(ms, 4), "GBps": round(2 * nbytes / (ms * 1e-3) / 1e9, 1)}
 
 
 def bench_small(log, timed, stream, st):

@@ -246,8 +246,53 @@ __global__ __launch_bounds__(256) void k_fwd1(int64_t m, const int64_t* __restri
   }
 }
 
-// F frames: a wave's 64 lanes are 64/FR entry slots x FR frames; each lane
-// keeps U entries (value, index, gathered x) in flight per step.
+// Frame vectors: a lane carries V = min(F, 4) consecutive frames of one
+// matrix entry (one 4/8/16-byte gather), a wave's 64 lanes are SLOTS entry
+// slots x Q frame vectors.  The value and index of an entry are loaded by
+// the Q lanes of its slot.
+template <int FR>
+struct FrameVec {
+  static constexpr int V = FR >= 4 ? 4 : FR;
+  static constexpr int Q = FR / V;
+  static constexpr int SLOTS = kWave / Q;
+};
+
+template <int V>
+__device__ __forceinline__ void load_frames(const float* __restrict__ p, float (&o)[V]) {
+  if constexpr (V == 4) {
+    vf4 t = *reinterpret_cast<const vf4*>(p);
+    o[0] = t.x, o[1] = t.y, o[2] = t.z, o[3] = t.w;
+  } else if constexpr (V == 2) {
+    typedef float vf2 __attribute__((ext_vector_type(2)));
+    vf2 t = *reinterpret_cast<const vf2*>(p);
+    o[0] = t.x, o[1] = t.y;
+  } else {
+    o[0] = *p;
+  }
+}
+
+template <int V>
+__device__ __forceinline__ void store_frames(float* __restrict__ p, const float (&o)[V]) {
+  if constexpr (V == 4) {
+    *reinterpret_cast<vf4*>(p) = vf4{o[0], o[1], o[2], o[3]};
+  } else if constexpr (V == 2) {
+    typedef float vf2 __attribute__((ext_vector_type(2)));
+    *reinterpret_cast<vf2*>(p) = vf2{o[0], o[1]};
+  } else {
+    *p = o[0];
+  }
+}
+
+// XCD-contiguous block index: consecutive workgroups of one XCD (dispatch
+// is round-robin over the 8 XCDs) take consecutive logical blocks, so the
+// rows an XCD works on at any moment are neighbours and share its L2.
+__device__ __forceinline__ int64_t xcd_block(int64_t bx, int64_t G) {
+  const int64_t xcd = bx % 8, q = G / 8, r8 = G % 8;
+  return xcd * q + (xcd < r8 ? xcd : r8) + bx / 8;
+}
+
+// F frames forward + ratio: one wave per row, each lane keeps U entries
+// (value, index, gathered frame vector) in flight per step.
 template <int VT, int FR, int U>
 __global__ __launch_bounds__(256) void k_fwdF(int64_t m, const int64_t* __restrict__ rp,
                                               const int* __restrict__ col,
@@ -256,44 +301,64 @@ __global__ __launch_bounds__(256) void k_fwdF(int64_t m, const int64_t* __restri
                                               const float* __restrict__ x,
                                               const float* __restrict__ y,
                                               float* __restrict__ e) {
-  constexpr int SLOTS = kWave / FR;
+  using FV = FrameVec<FR>;
+  constexpr int V = FV::V, Q = FV::Q, SLOTS = FV::SLOTS;
   const int lane = threadIdx.x & (kWave - 1);
-  const int f = lane % FR, slot = lane / FR;
-  const int64_t i = int64_t(blockIdx.x) * (256 / kWave) + threadIdx.x / kWave;
+  const int q = lane % Q, slot = lane / Q;
+  const int64_t i = xcd_block(blockIdx.x, gridDim.x) * (256 / kWave) + threadIdx.x / kWave;
   if (i >= m) return;
   const int64_t b = rp[i], en = rp[i + 1];
-  float acc = 0.0f;
+  float acc[V];
+#pragma unroll
+  for (int v = 0; v < V; ++v) acc[v] = 0.0f;
+  const float* xq = x + V * q;
   int64_t k = b + slot;
   for (; k + int64_t(SLOTS) * (U - 1) < en; k += int64_t(SLOTS) * U) {
-    float v[U];
+    float w[U];
     int c[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      v[u] = Val<VT>::get(val, k + u * SLOTS);
+      w[u] = Val<VT>::get(val, k + u * SLOTS);
       c[u] = __builtin_nontemporal_load(col + k + u * SLOTS);
     }
+    float g[U][V];
 #pragma unroll
-    for (int u = 0; u < U; ++u) acc = fmaf(v[u], x[int64_t(c[u]) * FR + f], acc);
+    for (int u = 0; u < U; ++u) load_frames<V>(xq + int64_t(c[u]) * FR, g[u]);
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int v = 0; v < V; ++v) acc[v] = fmaf(w[u], g[u][v], acc[v]);
   }
   for (; k < en; k += SLOTS) {
-    float v = Val<VT>::get(val, k);
+    float w = Val<VT>::get(val, k);
     int c = __builtin_nontemporal_load(col + k);
-    acc = fmaf(v, x[int64_t(c) * FR + f], acc);
+    float g[V];
+    load_frames<V>(xq + int64_t(c) * FR, g);
+#pragma unroll
+    for (int v = 0; v < V; ++v) acc[v] = fmaf(w, g[v], acc[v]);
   }
 #pragma unroll
-  for (int off = kWave / 2; off >= FR; off >>= 1) acc += __shfl_xor(acc, off, kWave);
+  for (int off = kWave / 2; off >= Q; off >>= 1)
+#pragma unroll
+    for (int v = 0; v < V; ++v) acc[v] += __shfl_xor(acc[v], off, kWave);
   if (slot == 0) {
-    float sc = VT == kF32 ? 1.0f : rscale[i];
-    float yt = sc * acc;
-    float yy = y[i * FR + f];
-    float r = (yt != 0.0f && yy != 0.0f) ? yy / yt : 0.0f;
-    e[i * FR + f] = VT == kF32 ? r : sc * r;
+    const float sc = VT == kF32 ? 1.0f : rscale[i];
+    float yy[V], r[V];
+    load_frames<V>(y + i * FR + V * q, yy);
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      float yt = sc * acc[v];
+      float t = (yt != 0.0f && yy[v] != 0.0f) ? yy[v] / yt : 0.0f;
+      r[v] = VT == kF32 ? t : sc * t;  // ratio folded with the row scale
+    }
+    store_frames<V>(e + i * FR + V * q, r);
   }
 }
 
 // Backward over one row block: c[j][f] (+)= sum over the block's entries of
 // column j of v * e'[row][f]; the last block applies the update
 // xx = x * c / (s + (s==0)) and writes FP64 partial sums per block/frame.
+// One wave per column, lanes as in the forward (frame vectors of V).
 template <int VT, int FR, int UNROLL>
 __global__ __launch_bounds__(256) void k_bwd(int n, const int64_t* __restrict__ ptr,
                                              const int* __restrict__ brow,
@@ -303,53 +368,85 @@ __global__ __launch_bounds__(256) void k_bwd(int n, const int64_t* __restrict__ 
                                              const float* __restrict__ s,
                                              float* __restrict__ xx,
                                              double* __restrict__ partials) {
-  constexpr int SLOTS = kWave / FR;
+  using FV = FrameVec<FR>;
+  constexpr int V = FV::V, Q = FV::Q, SLOTS = FV::SLOTS;
   const int lane = threadIdx.x & (kWave - 1);
-  const int f = lane % FR, slot = lane / FR;
+  const int q = lane % Q, slot = lane / Q;
   const int w = threadIdx.x / kWave;
-  // XCD-contiguous column ranges
-  const int G = gridDim.x, bx = blockIdx.x;
-  const int xcd = bx % 8, q = G / 8, r8 = G % 8;
-  const int lb = xcd * q + (xcd < r8 ? xcd : r8) + bx / 8;
+  const int lb = int(xcd_block(blockIdx.x, gridDim.x));  // XCD-contiguous column ranges
   const int j = lb * (256 / kWave) + w;
-  float acc = 0.0f;
+  float acc[V];
+#pragma unroll
+  for (int v = 0; v < V; ++v) acc[v] = 0.0f;
   if (j < n) {
+    const float* eq = e + V * q;
     const int64_t b = ptr[j], en = ptr[j + 1];
     int64_t k = b + slot;
     for (; k + int64_t(UNROLL - 1) * SLOTS < en; k += int64_t(UNROLL) * SLOTS) {
-      float v[UNROLL];
+      float wv[UNROLL];
       int rr[UNROLL];
 #pragma unroll
       for (int u = 0; u < UNROLL; ++u) {
-        v[u] = Val<VT>::get(val, k + u * SLOTS);
+        wv[u] = Val<VT>::get(val, k + u * SLOTS);
         rr[u] = __builtin_nontemporal_load(brow + k + u * SLOTS);
       }
+      float g[UNROLL][V];
 #pragma unroll
-      for (int u = 0; u < UNROLL; ++u) acc = fmaf(v[u], e[int64_t(rr[u]) * FR + f], acc);
+      for (int u = 0; u < UNROLL; ++u) load_frames<V>(eq + int64_t(rr[u]) * FR, g[u]);
+#pragma unroll
+      for (int u = 0; u < UNROLL; ++u)
+#pragma unroll
+        for (int v = 0; v < V; ++v) acc[v] = fmaf(wv[u], g[u][v], acc[v]);
     }
-    for (; k < en; k += SLOTS)
-      acc = fmaf(Val<VT>::get(val, k), e[int64_t(brow[k]) * FR + f], acc);
+    for (; k < en; k += SLOTS) {
+      float wv = Val<VT>::get(val, k);
+      float g[V];
+      load_frames<V>(eq + int64_t(brow[k]) * FR, g);
 #pragma unroll
-    for (int off = kWave / 2; off >= FR; off >>= 1) acc += __shfl_xor(acc, off, kWave);
+      for (int v = 0; v < V; ++v) acc[v] = fmaf(wv, g[v], acc[v]);
+    }
+#pragma unroll
+    for (int off = kWave / 2; off >= Q; off >>= 1)
+#pragma unroll
+      for (int v = 0; v < V; ++v) acc[v] += __shfl_xor(acc[v], off, kWave);
   }
-  double part = 0.0;
+  double part[V];
+#pragma unroll
+  for (int v = 0; v < V; ++v) part[v] = 0.0;
   if (j < n && slot == 0) {
-    int64_t o = int64_t(j) * FR + f;
-    float cv = first ? acc : c[o] + acc;
+    const int64_t o = int64_t(j) * FR + V * q;
+    float cv[V];
+    if (first) {
+#pragma unroll
+      for (int v = 0; v < V; ++v) cv[v] = acc[v];
+    } else {
+      load_frames<V>(c + o, cv);
+#pragma unroll
+      for (int v = 0; v < V; ++v) cv[v] += acc[v];
+    }
     if (!last) {
-      c[o] = cv;
+      store_frames<V>(c + o, cv);
     } else {
       float d = s[j];
       d += float(d == 0.0f);
-      float v = x[o] * cv / d;
-      xx[o] = v;
-      part = double(v);
+      float xv[V], nv[V];
+      load_frames<V>(x + o, xv);
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        nv[v] = xv[v] * cv[v] / d;
+        part[v] = double(nv[v]);
+      }
+      store_frames<V>(xx + o, nv);
     }
   }
   if (last) {
-    // per-frame block sums: lanes with slot 0 of each wave hold frame f
+    // per-frame block sums: lanes with slot 0 of each wave hold frames
+    // V*q .. V*q+V-1
     __shared__ double red[256 / kWave][64];
-    if (slot == 0) red[w][f] = part;
+    if (slot == 0) {
+#pragma unroll
+      for (int v = 0; v < V; ++v) red[w][V * q + v] = part[v];
+    }
     __syncthreads();
     if (threadIdx.x < FR) {
       double t = 0.0;
@@ -674,6 +771,12 @@ int iteration(raptr_recon* R, int F, const float* y, float* x, hipStream_t st) {
 
 bool valid_frames(int F) { return F >= 1 && F <= 64 && (F & (F - 1)) == 0; }
 
+// frame vectors are loaded with 4*min(F,4)-byte vector accesses
+bool frames_aligned(const void* p, int F) {
+  const uintptr_t a = uintptr_t(4) * uintptr_t(F >= 4 ? 4 : F);
+  return p == nullptr || uintptr_t(p) % a == 0;
+}
+
 }  // namespace
 }  // namespace raptr
 
@@ -918,6 +1021,9 @@ int raptr_recon_mlem(raptr_recon* R, int nframes, const float* d_y, float* d_x, 
   if (!valid_frames(nframes) || nframes > R->fmax)
     return fail(RAPTR_ERR_INVALID_ARG, "nframes %d not a power of two <= %d", nframes, R->fmax);
   if (niter < 0 || !d_y || !d_x) return fail(RAPTR_ERR_INVALID_ARG, "bad arguments");
+  if (!frames_aligned(d_y, nframes) || !frames_aligned(d_x, nframes))
+    return fail(RAPTR_ERR_INVALID_ARG, "y and x must be aligned to %d bytes for %d frames",
+                4 * (nframes >= 4 ? 4 : nframes), nframes);
   hipStream_t st = S(stream);
   if (!use_graph || niter < 2 || !st) {
     for (int it = 0; it < niter; ++it) RAPTR_TRY(iteration(R, nframes, d_y, d_x, st));
@@ -956,12 +1062,17 @@ int raptr_recon_forward(raptr_recon* R, int nframes, const float* d_x, const flo
                         float* d_e, raptr_stream_t stream) {
   if (!R || !valid_frames(nframes) || nframes > R->fmax)
     return fail(RAPTR_ERR_INVALID_ARG, "bad plan or frame count");
+  if (!frames_aligned(d_x, nframes) || !frames_aligned(d_y, nframes) ||
+      !frames_aligned(d_e, nframes))
+    return fail(RAPTR_ERR_INVALID_ARG, "x, y and e must be aligned to the frame vector");
   return launch_fwd_any(R, nframes, d_x, d_y, d_e, S(stream));
 }
 
 int raptr_recon_backward(raptr_recon* R, int nframes, const float* d_x, raptr_stream_t stream) {
   if (!R || !valid_frames(nframes) || nframes > R->fmax)
     return fail(RAPTR_ERR_INVALID_ARG, "bad plan or frame count");
+  if (!frames_aligned(d_x, nframes))
+    return fail(RAPTR_ERR_INVALID_ARG, "x must be aligned to the frame vector");
   if (nframes == 1 && R->bwd_lds) return launch_bwd_lds(R, d_x, S(stream), nullptr);
   for (int b = 0; b < R->nblk; ++b)
     RAPTR_TRY(launch_bwd_any(R, nframes, b, d_x, S(stream), nullptr));
