@@ -5,6 +5,8 @@
 //   getSystemMatrixFromWorkqueue  getSystemMatrixFromWorkqueue.cu:116-122
 //                              (a host launcher here: device pointers in,
 //                              geometry passed explicitly, no __constant__)
+//   getChords<T>               ChordsCalc_lowlevel.hpp:194-204 (a batched
+//                              device call here: nRays rays, nChords per ray)
 //   cooSort<T>                 cooSort.hpp:75-99
 //   customizeMatDescr / convertCoo2Csr   cusparseWrapper.hpp:36-60
 //   convertCsr2Ecsr            convertCsr2Ecsr.hpp:51-64
@@ -158,6 +160,24 @@ void getSystemMatrixFromWorkqueue(int const* const globalCnl, int const* const g
   raptr::check(raptr_sm_from_wq(&s, &g, globalCnl, globalVxl, globalVal, wqLength, nrays,
                                 seed, raygen, stream),
                "raptr_sm_from_wq");
+}
+
+// Siddon chords of nRays rays (device pointers): per ray up to nChords
+// (length, ix, iy, iz); returns nothing, counts per ray in nFound_devi.
+// The reference's form is one ray per call with host or device pointers
+// (ChordsCalc_lowlevel.hpp:194-204); a device batch replaces its loop.
+template <typename T>
+void getChords(T* const chords_devi, int* const voxelIds_devi, int const nChords,
+               T const* const rays_devi, int const nRays, T const* const gridO,
+               T const* const gridD, int const* const gridN, int* const nFound_devi,
+               raptr_stream_t stream = nullptr) {
+  static_assert(std::is_same<T, float>::value, "rays are float");
+  raptr_grid_t g = {{gridO[0], gridO[1], gridO[2]},
+                    {gridD[0], gridD[1], gridD[2]},
+                    {gridN[0], gridN[1], gridN[2]}};
+  raptr::check(raptr_get_chords(rays_devi, nRays, &g, nChords, chords_devi, voxelIds_devi,
+                                nFound_devi, stream),
+               "raptr_get_chords");
 }
 
 // ------------------------------------------------------- sparse formats

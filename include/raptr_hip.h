@@ -124,6 +124,20 @@ int raptr_generate_rays(const raptr_setup_t* setup, const int* d_cnl, int n_cnl,
                         int nrays, uint32_t seed, int raygen, float* d_rays,
                         raptr_stream_t stream);
 
+/* ------------------------------------------------- A11: Siddon chords */
+/* getChords (ChordsCalc_lowlevel.hpp:194-379) for nrays rays d_rays[6*r]
+ * (start xyz, end xyz): per ray up to cap (chord length, voxel index ix, iy,
+ * iz) in traversal order at d_chords[r*cap + k], d_voxel_ids[3*(r*cap + k)];
+ * d_counts[r] = chords written (min(traversed voxels, cap)).  A ray whose
+ * end points are not both outside the grid along its path, or that misses
+ * it, gives 0.  cap = nx + ny + nz + 4 always suffices.  Each axis starts at
+ * its first plane after the grid entry (the reference starts one plane past
+ * the axis' first grid plane and yields negative chords for axes other than
+ * the entry axis). */
+int raptr_get_chords(const float* d_rays, int nrays, const raptr_grid_t* grid, int cap,
+                     float* d_chords, int* d_voxel_ids, int* d_counts,
+                     raptr_stream_t stream);
+
 /* ---------------------- A12/A13/A23: fused matrix build, chunked CSR */
 /* Builds the matrix of the listed channels (rows in list order) with exact
  * zeros dropped, columns ascending: the result of the reference's

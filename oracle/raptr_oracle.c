@@ -484,7 +484,8 @@ void ro_sm_from_wq_f64(const ro_setup* s, const ro_grid* g, const int* cnl,
  * every axis other than the entry axis and yields negative chords; here each
  * axis starts at its first plane strictly after the entry parameter.  The
  * output is bounded by n_chords_cap (the reference ignores nChords) and the
- * last chord is clipped at the exit parameter.  Returns the chord count.
+ * last chord is clipped at the exit parameter; the plane stepping is bounded
+ * by the grid's plane count.  Returns the chord count (at most cap).
  * ==================================================================== */
 int ro_get_chords(float* chords, int* voxel_ids, int cap, const float ray[6],
                   const float go[3], const float gd[3], const int gn[3]) {
@@ -524,7 +525,7 @@ int ro_get_chords(float* chords, int* voxel_ids, int cap, const float ray[6],
   for (int d = 0; d < 3; ++d) {
     next[d] = amin_d[d] + up[d];
     if (crosses[d])
-      while (next[d] <= amin) next[d] += up[d];
+      for (int k = 0; next[d] <= amin && k < gn[d] + 4; ++k) next[d] += up[d];
   }
   float anext = 2.0f;
   for (int d = 0; d < 3; ++d)
@@ -537,7 +538,9 @@ int ro_get_chords(float* chords, int* voxel_ids, int cap, const float ray[6],
   float length = sqrtf(dx * dx + dy * dy + dz * dz);
   float acur = amin;
   int n = 0;
-  while (acur < amax) {
+  /* every step crosses at least one grid plane: bounded by the plane count */
+  const int max_steps = gn[0] + gn[1] + gn[2] + 8;
+  for (int step = 0; acur < amax && step < max_steps; ++step) {
     anext = 2.0f;
     for (int d = 0; d < 3; ++d)
       if (crosses[d] && next[d] < anext) anext = next[d];

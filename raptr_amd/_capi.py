@@ -78,6 +78,7 @@ _SIGS = {
     "raptr_workqueue_counts": (i32, [SP, GP, vp, i32, vp, vp]),
     "raptr_sm_from_wq": (i32, [SP, GP, vp, vp, vp, i64, i32, u32, i32, vp]),
     "raptr_generate_rays": (i32, [SP, vp, i32, i32, u32, i32, vp, vp]),
+    "raptr_get_chords": (i32, [vp, i32, GP, i32, vp, vp, vp, vp]),
     "raptr_sm_build_workspace_size": (sz, [GP, i32, i64]),
     "raptr_sm_build_csr": (i32, [SP, GP, vp, i32, i32, u32, i32, vp, vp, vp, i64, P(i64), vp,
                                  sz, vp]),

@@ -68,6 +68,26 @@ def generate_rays(setup, cnl, nrays, seed=1, raygen=capi.RAYGEN_RAND, stream=Non
     return d_r.numpy(len(cnl) * nrays * 6).reshape(len(cnl), nrays, 6)
 
 
+def get_chords(rays, grid, cap=None, stream=None):
+    """Siddon chords of rays [k, 6] (getChords, ChordsCalc_lowlevel.hpp:
+    194-379): returns (chords [k, cap], voxel ids [k, cap, 3], counts [k])."""
+    rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)
+    k = len(rays)
+    cap = int(cap if cap is not None else sum(grid.n) + 4)
+    d_r = DeviceArray.from_numpy(rays.reshape(-1) if k else np.zeros(6, np.float32))
+    d_ch = DeviceArray(max(k * cap, 1), np.float32)
+    d_id = DeviceArray(max(k * cap * 3, 1), np.int32)
+    d_n = DeviceArray(max(k, 1), np.int32)
+    g = grid.struct()
+    capi.call("raptr_get_chords", d_r.ptr, k, C.byref(g), cap, d_ch.ptr, d_id.ptr, d_n.ptr,
+              stream)
+    if k == 0:
+        return (np.zeros((0, cap), np.float32), np.zeros((0, cap, 3), np.int32),
+                np.zeros(0, np.int32))
+    return (d_ch.numpy(k * cap).reshape(k, cap), d_id.numpy(k * cap * 3).reshape(k, cap, 3),
+            d_n.numpy(k))
+
+
 @dataclass
 class DeviceCSR:
     """CSR with 64-bit row pointers held on the device."""

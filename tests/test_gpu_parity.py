@@ -93,6 +93,43 @@ def test_rays_match_oracle(gpu, oracle):
     np.testing.assert_array_equal(r[3, 1], oracle.ray(s, 20479, 1, raygen=oracle.RAYGEN_REGU))
 
 
+# ------------------------------------------------------------ Siddon chords
+@pytest.mark.parametrize("cfg", ["tiny", "small"])
+def test_get_chords_bit_exact(gpu, oracle, cfg):
+    """GPU getChords equals the host restatement bit for bit: sampled
+    channel rays, axis-parallel and diagonal rays, a miss, a ray starting
+    inside the grid, and a capacity smaller than the traversal."""
+    from raptr_amd import sysmat
+    setup, grid, nrays = G.config_tiny() if cfg == "tiny" else G.config_small()
+    s, g = oracle_geom(oracle, setup, grid)
+    rays = [oracle.ray(s, int(c), k) for c in range(0, setup.nchannels, 97)
+            for k in range(nrays)]
+    lo = np.array(grid.o, np.float32)
+    hi = lo + np.array(grid.d, np.float32) * np.array(grid.n, np.float32)
+    mid = 0.5 * (lo + hi)
+    ext = [
+        [lo[0] - 10, mid[1] + 0.3, mid[2] - 0.7, hi[0] + 10, mid[1] + 0.3, mid[2] - 0.7],
+        [*(lo - 5), *(hi + 5)],
+        [lo[0] - 10, hi[1] + 50, mid[2], hi[0] + 10, hi[1] + 50, mid[2]],  # miss
+        [*mid, *(hi + 5)],  # starts inside
+        [hi[0] + 7, lo[1] - 3, mid[2] + 1.1, lo[0] - 9, hi[1] + 4, mid[2] - 2.3],
+    ]
+    rays = np.array(rays + ext, np.float32)
+    for cap in (sum(grid.n) + 4, 3):
+        ch, ids, cnt = sysmat.get_chords(rays, grid, cap=cap)
+        for r, ray in enumerate(rays):
+            wc, wi = oracle.get_chords(ray, g, cap=cap)
+            assert cnt[r] == len(wc), r
+            np.testing.assert_array_equal(ch[r, :cnt[r]], wc)
+            np.testing.assert_array_equal(ids[r, :cnt[r]], wi)
+    # the closed forms of the extra rays
+    ch, ids, cnt = sysmat.get_chords(rays[-5:], grid)
+    assert cnt[2] == 0 and cnt[3] == 0
+    assert cnt[0] == grid.n[0] and np.allclose(ch[0, :cnt[0]], grid.d[0], rtol=1e-4)
+    assert ch[1, :cnt[1]].sum() == pytest.approx(
+        float(np.linalg.norm(hi - lo)), rel=1e-4)
+
+
 # ------------------------------------------------------- matrix from workqueue
 @pytest.mark.parametrize("cfg", ["tiny", "small"])
 def test_sm_from_wq_bit_exact(gpu, oracle, cfg):
