@@ -58,6 +58,9 @@ def parse():
     ap.add_argument("--kernel-reps", type=int, default=10)
     ap.add_argument("--modes", default="plain,batched,mixed_u16,mixed_f16,mixed_bf16",
                     help="comma-separated subset of the modes to run")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serving-requests", type=int, default=120)
+    ap.add_argument("--serving-rate", type=float, default=20.0, help="arrivals per second")
     ap.add_argument("--block-slices-mb", default="2,8,32",
                     help="ratio-slice sizes (MB) of the batched plans' row blocks to try")
     return ap.parse_args()
@@ -96,6 +99,10 @@ def main():
     log(f"device copy: {copy['GBps']:.0f} GB/s (read + write) over {copy['bytes'] >> 20} MiB")
     if args.modes == "copy":
         print(json.dumps({"partial": True, "copy": copy, "device": dev}), flush=True)
+        return
+    if args.modes == "serving":
+        serve = bench_serving(log, args.serving_requests, args.serving_rate)
+        print(json.dumps({"partial": True, "serving": serve, "device": dev}), flush=True)
         return
 
     # ------------------------------------------------------------ data
@@ -293,6 +300,15 @@ def main():
     if not args.no_small:
         small = bench_small(log, timed, stream, st)
 
+    # ------------------------------------------------------------ serving
+    serve = None
+    nnz_full = A.nnz
+    if not args.no_serving:
+        del plans, R, dy, dx, de
+        res.pop("A")  # free the full-size matrix first
+        A = None
+        serve = bench_serving(log, args.serving_requests, args.serving_rate)
+
     out = {
         "metric": "frames/s batched vs plain MLEM at <=1e-4 rel err; p50/p99 latency; HBM GB/s",
         "value": modes[best]["frames_per_s"],
@@ -309,7 +325,7 @@ def main():
         "config": {"workload": "full-size MLEM: 13x13 heads x 180 angles (5,140,980 channels), "
                                "52^3 grid @ 2 mm, 30 rays, 2e7 events, "
                                f"{niter} iterations/frame, mode '{best}'",
-                   "measured_channels": m, "nnz": A.nnz, "voxels": n,
+                   "measured_channels": m, "nnz": nnz_full, "voxels": n,
                    "frames_per_batch": modes[best]["frames_per_batch"]},
         "modes": modes,
         "plain_frames_per_s": plain_fps,
@@ -324,6 +340,7 @@ def main():
         "build": {"matrix_ms": round(ms_build, 1), "sensitivity_ms": round(ms_sens, 1),
                   "data_s": round(t_data, 2)},
         "batched_small": small,
+        "serving": serve,
         "device_copy": copy,
         "cpu_baseline": cpu,
         "device": dev,
@@ -345,6 +362,45 @@ def copy_peak(timed, st, nbytes=2 << 30, reps=5):
                         for _ in range(reps)]) / reps
     del a, b
     return {"bytes": nbytes, "ms": round(ms, 4), "GBps": round(2 * nbytes / (ms * 1e-3) / 1e9, 1)}
+
+
+def bench_serving(log, nreq, rate):
+    """Poisson arrivals of reconstruction requests over three geometries
+    (8x8 heads / 32^3, 13x13 heads / 52^3, 16x16 heads / 96^3; 60 angles at
+    3 deg, 2 mm voxels, 20 rays), 1e4-5e6 events and 10-50 iterations per
+    request, served by continuous batching (16 slots per geometry) from a
+    pre-built device matrix cache.  Latency = arrival to image on the host."""
+    from raptr_amd import geometry as G
+    from raptr_amd import serving, synthetic, sysmat
+    geoms = [serving.Geometry(G.two_head_setup(p, 60, 3.0), G.centred_grid(v, 2.0), 20)
+             for p, v in ((8, 32), (13, 52), (16, 96))]
+    cache = serving.MatrixCache(200 << 30, frames=16)
+    t = time.perf_counter()
+    builds = {}
+    for g in geoms:
+        lane = cache.admit(g)
+        builds[f"{g.setup.n0z}x{g.setup.n0y}/{g.grid.n[0]}^3"] = {
+            "channels": g.nchannels, "nnz": lane.nnz, "build_s": round(lane.build_s, 2),
+            "device_GB": round(lane.bytes / 1e9, 2)}
+        log(f"serving lane {g.setup.n0z}x{g.setup.n0y} heads, {g.grid.n[0]}^3: "
+            f"nnz {lane.nnz}, built in {lane.build_s:.1f} s, {lane.bytes / 1e9:.1f} GB")
+    ybar = {}
+    for g in geoms:
+        img, _ = synthetic.random_ellipsoids(g.grid, n=6, seed=5)
+        ybar[g] = sysmat.forward_project(g.setup, g.grid, img.astype(np.float32), g.nrays)
+    reqs = serving.synthetic_requests(geoms, rate, nreq, seed=11, ybar=ybar)
+    t_prep = time.perf_counter() - t
+    stats = serving.Server(cache, max_chunk=10).serve(reqs, realtime=True)
+    out = stats.summary()
+    out.update({"workload": f"{nreq} requests, Poisson arrivals at {rate}/s, geometries "
+                            "8x8/32^3, 13x13/52^3, 16x16/96^3 (60 angles x 3 deg, 2 mm, 20 rays), "
+                            "1e4-5e6 events, 10-50 iterations, 16 slots per geometry",
+                "lanes": builds, "prep_s": round(t_prep, 1)})
+    log(f"serving: p50 {out['p50_ms']:.0f} ms, p99 {out['p99_ms']:.0f} ms, "
+        f"{out['requests_per_s']:.1f} req/s over {out['wall_s']:.1f} s")
+    for g in list(cache.lanes):
+        cache.lanes.pop(g).close()
+    return out
 
 
 def bench_small(log, timed, stream, st):

@@ -333,6 +333,19 @@ int raptr_recon_forward(raptr_recon* plan, int nframes, const float* d_x,
 int raptr_recon_backward(raptr_recon* plan, int nframes, const float* d_x,
                          raptr_stream_t stream);
 
+/* ----------------------- frame slots of [rows][nframes] arrays (serving) */
+/* No counterpart in the reference (one frame per run there): a request
+ * occupies one frame slot of a batched plan's y ([m][F]) and x ([n][F]).
+ * fill: column `slot` = value.  scatter: column `slot`[d_idx[i]] = d_val[i]
+ * (rows outside [0, rows) are skipped and set *d_bad = 1).  gather:
+ * d_out[i] = column `slot`[i]. */
+int raptr_frame_fill(float* d_a, int64_t rows, int nframes, int slot, float value,
+                     raptr_stream_t stream);
+int raptr_frame_scatter(float* d_a, int64_t rows, int nframes, int slot, const int* d_idx,
+                        const float* d_val, int64_t n, int* d_bad, raptr_stream_t stream);
+int raptr_frame_gather(const float* d_a, int64_t rows, int nframes, int slot, float* d_out,
+                       raptr_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

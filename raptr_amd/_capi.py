@@ -113,6 +113,9 @@ _SIGS = {
     "raptr_recon_set_config": (i32, [vp, i32, i32, i32]),
     "raptr_recon_set_backward": (i32, [vp, i32, i32, i32]),
     "raptr_recon_get_backward": (i32, [vp, vp, vp, vp]),
+    "raptr_frame_fill": (i32, [vp, i64, i32, i32, f32, vp]),
+    "raptr_frame_scatter": (i32, [vp, i64, i32, i32, vp, vp, i64, vp, vp]),
+    "raptr_frame_gather": (i32, [vp, i64, i32, i32, vp, vp]),
     "raptr_recon_mlem": (i32, [vp, i32, vp, vp, i32, i32, vp]),
     "raptr_recon_forward": (i32, [vp, i32, vp, vp, vp, vp]),
     "raptr_recon_backward": (i32, [vp, i32, vp, vp]),

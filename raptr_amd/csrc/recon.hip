@@ -771,6 +771,31 @@ int iteration(raptr_recon* R, int F, const float* y, float* x, hipStream_t st) {
 
 bool valid_frames(int F) { return F >= 1 && F <= 64 && (F & (F - 1)) == 0; }
 
+// frame-slot plumbing of the frames-fastest [rows][F] layout (serving)
+__global__ void k_frame_fill(float* a, int64_t rows, int F, int slot, float v) {
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < rows;
+       i += int64_t(gridDim.x) * blockDim.x)
+    a[i * F + slot] = v;
+}
+
+__global__ void k_frame_scatter(float* a, int64_t rows, int F, int slot, const int* idx,
+                                const float* val, int64_t n, int* bad) {
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    int r = idx[i];
+    if (r >= 0 && r < rows)
+      a[int64_t(r) * F + slot] = val[i];
+    else
+      bad[0] = 1;
+  }
+}
+
+__global__ void k_frame_gather(const float* a, int64_t rows, int F, int slot, float* out) {
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < rows;
+       i += int64_t(gridDim.x) * blockDim.x)
+    out[i] = a[i * F + slot];
+}
+
 // frame vectors are loaded with 4*min(F,4)-byte vector accesses
 bool frames_aligned(const void* p, int F) {
   const uintptr_t a = uintptr_t(4) * uintptr_t(F >= 4 ? 4 : F);
@@ -1076,6 +1101,41 @@ int raptr_recon_backward(raptr_recon* R, int nframes, const float* d_x, raptr_st
   if (nframes == 1 && R->bwd_lds) return launch_bwd_lds(R, d_x, S(stream), nullptr);
   for (int b = 0; b < R->nblk; ++b)
     RAPTR_TRY(launch_bwd_any(R, nframes, b, d_x, S(stream), nullptr));
+  return RAPTR_OK;
+}
+
+// ------------------------------------------------ frame-slot plumbing
+int raptr_frame_fill(float* d_a, int64_t rows, int nframes, int slot, float value,
+                     raptr_stream_t stream) {
+  if (!d_a || rows < 0 || nframes < 1 || slot < 0 || slot >= nframes)
+    return fail(RAPTR_ERR_INVALID_ARG, "bad frame fill arguments");
+  if (rows == 0) return RAPTR_OK;
+  hipLaunchKernelGGL(k_frame_fill, dim3(grid_for(rows, 256, 8192)), dim3(256), 0, S(stream),
+                     d_a, rows, nframes, slot, value);
+  RAPTR_LAUNCH_CHECK();
+  return RAPTR_OK;
+}
+
+int raptr_frame_scatter(float* d_a, int64_t rows, int nframes, int slot, const int* d_idx,
+                        const float* d_val, int64_t n, int* d_bad, raptr_stream_t stream) {
+  if (!d_a || rows < 0 || nframes < 1 || slot < 0 || slot >= nframes || n < 0 ||
+      (n > 0 && (!d_idx || !d_val || !d_bad)))
+    return fail(RAPTR_ERR_INVALID_ARG, "bad frame scatter arguments");
+  if (n == 0) return RAPTR_OK;
+  hipLaunchKernelGGL(k_frame_scatter, dim3(grid_for(n, 256, 8192)), dim3(256), 0, S(stream),
+                     d_a, rows, nframes, slot, d_idx, d_val, n, d_bad);
+  RAPTR_LAUNCH_CHECK();
+  return RAPTR_OK;
+}
+
+int raptr_frame_gather(const float* d_a, int64_t rows, int nframes, int slot, float* d_out,
+                       raptr_stream_t stream) {
+  if (!d_a || !d_out || rows < 0 || nframes < 1 || slot < 0 || slot >= nframes)
+    return fail(RAPTR_ERR_INVALID_ARG, "bad frame gather arguments");
+  if (rows == 0) return RAPTR_OK;
+  hipLaunchKernelGGL(k_frame_gather, dim3(grid_for(rows, 256, 8192)), dim3(256), 0, S(stream),
+                     d_a, rows, nframes, slot, d_out);
+  RAPTR_LAUNCH_CHECK();
   return RAPTR_OK;
 }
 
