@@ -297,6 +297,28 @@ int raptr_recon_create(raptr_recon** plan, int m, int n, int64_t nnz,
                        const int64_t* d_row_ptr, const int* d_col,
                        const float* d_val, const double* d_sens, int value_type,
                        int max_frames, int block_rows, raptr_stream_t stream);
+/* Layout choice.  BLOCKED: the two layouts above (any frame count).
+ * TILED (max_frames = 1): the columns are cut into tiles of at most 36,864
+ * voxels; each row's entries are stored per tile (rows padded to multiples
+ * of 4, 16-bit column offsets within the tile).  A workgroup holds one
+ * tile of x (forward) or of the column sums (backward) in LDS and streams
+ * its rows' segments, so neither pass gathers from L2 and no transposed
+ * copy is kept.  The backward accumulates with LDS atomics: its FP32 sums
+ * are not bitwise reproducible from run to run.  For TILED, block_rows > 0
+ * caps the tile width (columns) below the LDS limit. */
+enum { RAPTR_LAYOUT_BLOCKED = 0, RAPTR_LAYOUT_TILED = 1 };
+int raptr_recon_create_ex(raptr_recon** plan, int m, int n, int64_t nnz,
+                          const int64_t* d_row_ptr, const int* d_col,
+                          const float* d_val, const double* d_sens, int value_type,
+                          int max_frames, int block_rows, int layout,
+                          raptr_stream_t stream);
+int raptr_recon_layout(const raptr_recon* plan);
+/* Tiled layout kernel shape: lanes per row segment (4, 8, 16) and unroll
+ * (1, 2, 4) of the forward and the backward. */
+int raptr_recon_set_tiled(raptr_recon* plan, int fwd_lanes, int fwd_unroll, int bwd_lanes,
+                          int bwd_unroll);
+int raptr_recon_get_tiled(const raptr_recon* plan, int* ntile, int* tile_cols,
+                          int* fwd_lanes, int* fwd_unroll, int* bwd_lanes, int* bwd_unroll);
 int raptr_recon_destroy(raptr_recon* plan);
 int raptr_recon_info(const raptr_recon* plan, int64_t* device_bytes,
                      int64_t* padded_nnz, int* row_blocks, int* fwd_lanes,

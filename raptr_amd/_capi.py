@@ -112,6 +112,10 @@ _SIGS = {
     "raptr_recon_info": (i32, [vp, P(i64), P(i64), P(i32), P(i32), P(i32), P(i32)]),
     "raptr_recon_set_config": (i32, [vp, i32, i32, i32]),
     "raptr_recon_set_backward": (i32, [vp, i32, i32, i32]),
+    "raptr_recon_create_ex": (i32, [vp, i32, i32, i64, vp, vp, vp, vp, i32, i32, i32, i32, vp]),
+    "raptr_recon_layout": (i32, [vp]),
+    "raptr_recon_set_tiled": (i32, [vp, i32, i32, i32, i32]),
+    "raptr_recon_get_tiled": (i32, [vp, vp, vp, vp, vp, vp, vp]),
     "raptr_recon_get_backward": (i32, [vp, vp, vp, vp]),
     "raptr_frame_fill": (i32, [vp, i64, i32, i32, f32, vp]),
     "raptr_frame_scatter": (i32, [vp, i64, i32, i32, vp, vp, i64, vp, vp]),

@@ -17,66 +17,13 @@
 // (e'_i = scale_i * e_i) for the backward pass.  Vectors hold F frames
 // per index ([index][frame]); one launch handles all frames, so the matrix
 // stream is shared by the batch.  The normalisation sums are FP64.
-#include <hip/hip_bf16.h>
-#include <hip/hip_fp16.h>
-
 #include <algorithm>
-#include <map>
-#include <vector>
 
-#include "primitives.hpp"
+#include "recon.hpp"
 
 namespace raptr {
 namespace {
 
-
-// --------------------------------------------------------- value types
-enum { kF32 = 0, kF16 = 1, kBF16 = 2, kU16 = 3 };
-
-__device__ __forceinline__ float h2f(uint16_t h, int vt) {
-  if (vt == kF16) return __half2float(__ushort_as_half(h));
-  if (vt == kU16) return float(h);
-  return __uint_as_float(uint32_t(h) << 16);
-}
-
-// f in [0, 1] (value / row maximum)
-__device__ __forceinline__ uint16_t f2h(float f, int vt) {
-  if (vt == kF16) return __half_as_ushort(__float2half_rn(f));
-  if (vt == kU16) return uint16_t(fminf(rintf(f * 65535.0f), 65535.0f));
-  // BF16, round to nearest even
-  uint32_t u = __float_as_uint(f);
-  uint32_t r = 0x7fffu + ((u >> 16) & 1u);
-  return uint16_t((u + r) >> 16);
-}
-
-template <int VT>
-struct Val;
-template <>
-struct Val<kF32> {
-  using T = float;
-  __device__ static float get(const float* p, int64_t k) { return __builtin_nontemporal_load(p + k); }
-};
-template <>
-struct Val<kF16> {
-  using T = uint16_t;
-  __device__ static float get(const uint16_t* p, int64_t k) {
-    return __half2float(__ushort_as_half(__builtin_nontemporal_load(p + k)));
-  }
-};
-template <>
-struct Val<kBF16> {
-  using T = uint16_t;
-  __device__ static float get(const uint16_t* p, int64_t k) {
-    return __uint_as_float(uint32_t(__builtin_nontemporal_load(p + k)) << 16);
-  }
-};
-template <>
-struct Val<kU16> {
-  using T = uint16_t;
-  __device__ static float get(const uint16_t* p, int64_t k) {
-    return float(__builtin_nontemporal_load(p + k));
-  }
-};
 
 // ------------------------------------------------- layout construction
 __global__ void k_padded_len(const int64_t* rp, int m, int64_t* plen) {
@@ -180,20 +127,6 @@ __global__ void k_bt_ptr(const uint64_t* keys, int64_t nnz, int nblk, int n, int
 // row (forward) / column (backward); its 64 lanes form 64/FR entry slots
 // x FR frames.  F = 1 uses vector loads of 4 entries per lane (forward).
 
-template <int VT>
-__device__ __forceinline__ void load4(const typename Val<VT>::T* __restrict__ val, int64_t k,
-                                      float& v0, float& v1, float& v2, float& v3) {
-  if (VT == kF32) {
-    vf4 v = __builtin_nontemporal_load(
-        reinterpret_cast<const vf4*>(reinterpret_cast<const float*>(val) + k));
-    v0 = v.x, v1 = v.y, v2 = v.z, v3 = v.w;
-  } else {
-    vu2 h = __builtin_nontemporal_load(
-        reinterpret_cast<const vu2*>(reinterpret_cast<const uint16_t*>(val) + k));
-    v0 = h2f(uint16_t(h.x & 0xffff), VT), v1 = h2f(uint16_t(h.x >> 16), VT);
-    v2 = h2f(uint16_t(h.y & 0xffff), VT), v3 = h2f(uint16_t(h.y >> 16), VT);
-  }
-}
 
 // Single frame: each lane streams U groups of 4 entries per step (vector
 // loads), so 4*U gathers of x are in flight per lane.
@@ -587,58 +520,10 @@ __global__ void k_sens_norm32(const double* sens, int n, float* s) {
 
 using namespace raptr;
 
-struct raptr_recon {
-  int m = 0, n = 0, vt = kF32, fmax = 1;
-  int64_t nnz = 0, pnnz = 0;
-  int blk_rows = 0, nblk = 0;
-  // forward (padded CSR)
-  int64_t* frp = nullptr;
-  int* fcol = nullptr;
-  void* fval = nullptr;
-  float* rscale = nullptr;
-  // backward (row-blocked transpose)
-  int64_t* bptr = nullptr;  // nblk * (n+1)
-  int* brow = nullptr;
-  void* bval = nullptr;
-  // sensitivity and work vectors (sized for fmax frames)
-  float* s = nullptr;
-  float* e = nullptr;
-  float* c = nullptr;
-  float* xx = nullptr;
-  double* partials = nullptr;
-  float* scale = nullptr;
-  double* norm = nullptr;
-  int fwd_lanes = 64;
-  int fwd_unroll = 1;  // vec4 groups (1 frame) or entries (F frames) in flight
-  int bwd_unroll = 4;
-  int bwd_lds = 0;     // single-frame backward staged through LDS (0: off)
-  int bwd_lpc = 16;    // lanes per column of the LDS-staged backward
-  int bwd_wgs = 512;   // workgroups of the LDS-staged backward
-  size_t bytes = 0;
-  std::vector<void*> allocs;
-  std::map<long long, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
-};
 
 namespace raptr {
 namespace {
 
-template <class T>
-int ralloc(raptr_recon* R, T** p, size_t count) {
-  void* q = nullptr;
-  hipError_t e = hipMalloc(&q, std::max<size_t>(count * sizeof(T), 256));
-  if (e == hipErrorOutOfMemory) {
-    (void)hipGetLastError();
-    return fail(RAPTR_ERR_OUT_OF_MEMORY, "recon: out of device memory (%zu B)",
-                count * sizeof(T));
-  }
-  RAPTR_HIP_TRY(e);
-  R->allocs.push_back(q);
-  R->bytes += count * sizeof(T);
-  *p = static_cast<T*>(q);
-  return RAPTR_OK;
-}
-
-int vbytes(int vt) { return vt == kF32 ? 4 : 2; }
 
 // ---------------------------------------------------------- launches
 int launch_fwd_any(const raptr_recon* R, int F, const float* x, const float* y, float* e,
@@ -753,11 +638,15 @@ int launch_bwd_lds(const raptr_recon* R, const float* x, hipStream_t st, int* nb
 }
 
 int iteration(raptr_recon* R, int F, const float* y, float* x, hipStream_t st) {
-  RAPTR_TRY(launch_fwd_any(R, F, x, y, R->e, st));
   int nb = 0;
-  if (F == 1 && R->bwd_lds) {
+  if (R->layout == RAPTR_LAYOUT_TILED) {
+    RAPTR_TRY(tiled_forward(R, x, y, R->e, st));
+    RAPTR_TRY(tiled_backward(R, x, R->e, st, &nb));
+  } else if (F == 1 && R->bwd_lds) {
+    RAPTR_TRY(launch_fwd_any(R, F, x, y, R->e, st));
     RAPTR_TRY(launch_bwd_lds(R, x, st, &nb));
   } else {
+    RAPTR_TRY(launch_fwd_any(R, F, x, y, R->e, st));
     for (int b = 0; b < R->nblk; ++b) RAPTR_TRY(launch_bwd_any(R, F, b, x, st, &nb));
   }
   hipLaunchKernelGGL(k_norms, dim3(1), dim3(256), 0, st, (const double*)R->partials, nb, F,
@@ -811,6 +700,14 @@ int raptr_recon_create(raptr_recon** out, int m, int n, int64_t nnz, const int64
                        const int* d_col, const float* d_val, const double* d_sens,
                        int value_type, int max_frames, int block_rows,
                        raptr_stream_t stream) {
+  return raptr_recon_create_ex(out, m, n, nnz, d_rp, d_col, d_val, d_sens, value_type,
+                               max_frames, block_rows, RAPTR_LAYOUT_BLOCKED, stream);
+}
+
+int raptr_recon_create_ex(raptr_recon** out, int m, int n, int64_t nnz, const int64_t* d_rp,
+                          const int* d_col, const float* d_val, const double* d_sens,
+                          int value_type, int max_frames, int block_rows, int layout,
+                          raptr_stream_t stream) {
   if (!out) return fail(RAPTR_ERR_INVALID_ARG, "null output");
   *out = nullptr;
   if (m < 0 || n <= 0 || nnz < 0) return fail(RAPTR_ERR_INVALID_ARG, "bad sizes");
@@ -822,8 +719,13 @@ int raptr_recon_create(raptr_recon** out, int m, int n, int64_t nnz, const int64
     return fail(RAPTR_ERR_INVALID_ARG, "null buffer");
   if (nnz > int64_t(UINT32_MAX) - 16)
     return fail(RAPTR_ERR_INVALID_ARG, "nnz exceeds 2^32");
+  if (layout != RAPTR_LAYOUT_BLOCKED && layout != RAPTR_LAYOUT_TILED)
+    return fail(RAPTR_ERR_INVALID_ARG, "unknown layout %d", layout);
+  if (layout == RAPTR_LAYOUT_TILED && max_frames != 1)
+    return fail(RAPTR_ERR_INVALID_ARG, "the tiled layout is single-frame (max_frames = 1)");
   hipStream_t st = S(stream);
   raptr_recon* R = new raptr_recon();
+  R->layout = layout;
   R->m = m;
   R->n = n;
   R->nnz = nnz;
@@ -838,107 +740,111 @@ int raptr_recon_create(raptr_recon** out, int m, int n, int64_t nnz, const int64
   R->blk_rows = block_rows;
   R->nblk = std::max(1, int((int64_t(m) + block_rows - 1) / block_rows));
   int rc;
-  // ---- forward: padded CSR
-  int64_t* plen = nullptr;
-  int64_t* tot = nullptr;
-  if ((rc = ralloc(R, &R->frp, size_t(m) + 1))) return bail(rc);
-  {
-    void* tmp = nullptr;
-    size_t sb = scan_workspace_bytes(m);
-    RAPTR_HIP_TRY(hipMalloc(&tmp, size_t(m) * 8 + 512 + sb));
-    plen = static_cast<int64_t*>(tmp);
-    tot = reinterpret_cast<int64_t*>(static_cast<char*>(tmp) + ((size_t(m) * 8 + 255) & ~size_t(255)));
-    char* sws = reinterpret_cast<char*>(tot) + 256;
-    if (m > 0) {
-      hipLaunchKernelGGL(k_padded_len, dim3(grid_for(m, 256, 8192)), dim3(256), 0, st, d_rp,
-                         m, plen);
-      rc = scan_exclusive_i64(plen, R->frp, m, tot, sws, sb, st);
-      if (!rc) hipLaunchKernelGGL(k_set_last, dim3(1), dim3(1), 0, st, R->frp, m, (const int64_t*)tot);
-    } else {
-      (void)hipMemsetAsync(R->frp, 0, 8, st);
-      (void)hipMemsetAsync(tot, 0, 8, st);
-    }
-    int64_t h = 0;
-    (void)hipMemcpyAsync(&h, tot, 8, hipMemcpyDeviceToHost, st);
-    (void)hipStreamSynchronize(st);
-    (void)hipFree(tmp);
-    if (rc) return bail(rc);
-    R->pnnz = h;
-  }
-  if ((rc = ralloc(R, &R->fcol, size_t(R->pnnz) + 4))) return bail(rc);
-  if (value_type == kF32) {
-    float* p;
-    if ((rc = ralloc(R, &p, size_t(R->pnnz) + 4))) return bail(rc);
-    R->fval = p;
+  if (layout == RAPTR_LAYOUT_TILED) {
+    if ((rc = tiled_create(R, d_rp, d_col, d_val, st))) return bail(rc);
   } else {
-    uint16_t* p;
-    if ((rc = ralloc(R, &p, size_t(R->pnnz) + 8))) return bail(rc);
-    R->fval = p;
-  }
-  if ((rc = ralloc(R, &R->rscale, size_t(m) + 1))) return bail(rc);
-  if (m > 0) {
-    hipLaunchKernelGGL(k_pad_copy, dim3(grid_for(int64_t(m) * kWave, 256, 1 << 16)), dim3(256),
-                       0, st, d_rp, d_col, d_val, m, value_type, (const int64_t*)R->frp,
-                       R->fcol, R->fval, R->rscale);
-    hipError_t le = hipGetLastError();
-    if (le != hipSuccess) return bail(fail(RAPTR_ERR_DEVICE, "pad: %s", hipGetErrorString(le)));
-  }
-  // ---- backward: row-blocked transpose via a stable radix sort
-  if ((rc = ralloc(R, &R->bptr, size_t(R->nblk) * (size_t(n) + 1)))) return bail(rc);
-  if ((rc = ralloc(R, &R->brow, size_t(nnz) + 1))) return bail(rc);
-  if (value_type == kF32) {
-    float* p;
-    if ((rc = ralloc(R, &p, size_t(nnz) + 1))) return bail(rc);
-    R->bval = p;
-  } else {
-    uint16_t* p;
-    if ((rc = ralloc(R, &p, size_t(nnz) + 1))) return bail(rc);
-    R->bval = p;
-  }
-  if (nnz > 0) {
-    int colbits = 1;
-    while ((int64_t(1) << colbits) <= n) ++colbits;
-    int blkbits = 0;
-    while ((int64_t(1) << blkbits) < R->nblk) ++blkbits;
-    size_t sort_ws = raptr_csr_transpose_workspace_size(nnz, m, n);  // >= radix buffers
-    void* tmp = nullptr;
-    RAPTR_HIP_TRY(hipMalloc(&tmp, sort_ws));
-    // carve: rowid | key32 (unused) | sort buffers, in the order the
-    // transpose helper lays them out
-    Carver cv(tmp, sort_ws);
-    uint32_t* rowid = cv.take<uint32_t>(size_t(nnz));
-    (void)cv.take<int>(size_t(nnz));
-    rc = RAPTR_OK;
-    uint64_t* k0 = cv.take<uint64_t>(size_t(nnz));
-    uint64_t* k1 = cv.take<uint64_t>(size_t(nnz));
-    uint32_t* v0 = cv.take<uint32_t>(size_t(nnz));
-    uint32_t* v1 = cv.take<uint32_t>(size_t(nnz));
-    if (!cv.ok) rc = fail(RAPTR_ERR_WORKSPACE, "transpose scratch carve failed");
-    if (!rc) {
-      hipLaunchKernelGGL(k_bt_keys, dim3(grid_for(int64_t(m) * kWave, 256, 1 << 16)), dim3(256),
-                         0, st, d_rp, d_col, m, block_rows, colbits, k0, v0);
-      hipLaunchKernelGGL(k_expand_rows32, dim3(grid_for(int64_t(m) * kWave, 256, 1 << 16)),
-                         dim3(256), 0, st, d_rp, m, rowid);
-      int res = 0;
-      rc = radix_sort_pairs(k0, k1, v0, v1, nnz, colbits + blkbits, st, &res, cv);
-      if (!rc) {
-        uint64_t* keys = res ? k1 : k0;
-        uint32_t* idx = res ? v1 : v0;
-        hipLaunchKernelGGL(k_bt_gather, dim3(grid_for(nnz, 256, 1 << 16)), dim3(256), 0, st,
-                           (const uint32_t*)idx, nnz, (const uint32_t*)rowid, d_val,
-                           (const float*)R->rscale, value_type, R->brow, R->bval);
-        hipLaunchKernelGGL(k_bt_ptr,
-                           dim3(grid_for(int64_t(R->nblk) * (n + 1), 256, 1 << 16)), dim3(256),
-                           0, st, (const uint64_t*)keys, nnz, R->nblk, n, colbits, R->bptr);
-        hipError_t le = hipGetLastError();
-        if (le != hipSuccess) rc = fail(RAPTR_ERR_DEVICE, "transpose: %s", hipGetErrorString(le));
+    // ---- forward: padded CSR
+    int64_t* plen = nullptr;
+    int64_t* tot = nullptr;
+    if ((rc = ralloc(R, &R->frp, size_t(m) + 1))) return bail(rc);
+    {
+      void* tmp = nullptr;
+      size_t sb = scan_workspace_bytes(m);
+      RAPTR_HIP_TRY(hipMalloc(&tmp, size_t(m) * 8 + 512 + sb));
+      plen = static_cast<int64_t*>(tmp);
+      tot = reinterpret_cast<int64_t*>(static_cast<char*>(tmp) + ((size_t(m) * 8 + 255) & ~size_t(255)));
+      char* sws = reinterpret_cast<char*>(tot) + 256;
+      if (m > 0) {
+        hipLaunchKernelGGL(k_padded_len, dim3(grid_for(m, 256, 8192)), dim3(256), 0, st, d_rp,
+                           m, plen);
+        rc = scan_exclusive_i64(plen, R->frp, m, tot, sws, sb, st);
+        if (!rc) hipLaunchKernelGGL(k_set_last, dim3(1), dim3(1), 0, st, R->frp, m, (const int64_t*)tot);
+      } else {
+        (void)hipMemsetAsync(R->frp, 0, 8, st);
+        (void)hipMemsetAsync(tot, 0, 8, st);
       }
+      int64_t h = 0;
+      (void)hipMemcpyAsync(&h, tot, 8, hipMemcpyDeviceToHost, st);
+      (void)hipStreamSynchronize(st);
+      (void)hipFree(tmp);
+      if (rc) return bail(rc);
+      R->pnnz = h;
     }
-    (void)hipStreamSynchronize(st);
-    (void)hipFree(tmp);
-    if (rc) return bail(rc);
-  } else {
-    (void)hipMemsetAsync(R->bptr, 0, sizeof(int64_t) * size_t(R->nblk) * (n + 1), st);
+    if ((rc = ralloc(R, &R->fcol, size_t(R->pnnz) + 4))) return bail(rc);
+    if (value_type == kF32) {
+      float* p;
+      if ((rc = ralloc(R, &p, size_t(R->pnnz) + 4))) return bail(rc);
+      R->fval = p;
+    } else {
+      uint16_t* p;
+      if ((rc = ralloc(R, &p, size_t(R->pnnz) + 8))) return bail(rc);
+      R->fval = p;
+    }
+    if ((rc = ralloc(R, &R->rscale, size_t(m) + 1))) return bail(rc);
+    if (m > 0) {
+      hipLaunchKernelGGL(k_pad_copy, dim3(grid_for(int64_t(m) * kWave, 256, 1 << 16)), dim3(256),
+                         0, st, d_rp, d_col, d_val, m, value_type, (const int64_t*)R->frp,
+                         R->fcol, R->fval, R->rscale);
+      hipError_t le = hipGetLastError();
+      if (le != hipSuccess) return bail(fail(RAPTR_ERR_DEVICE, "pad: %s", hipGetErrorString(le)));
+    }
+    // ---- backward: row-blocked transpose via a stable radix sort
+    if ((rc = ralloc(R, &R->bptr, size_t(R->nblk) * (size_t(n) + 1)))) return bail(rc);
+    if ((rc = ralloc(R, &R->brow, size_t(nnz) + 1))) return bail(rc);
+    if (value_type == kF32) {
+      float* p;
+      if ((rc = ralloc(R, &p, size_t(nnz) + 1))) return bail(rc);
+      R->bval = p;
+    } else {
+      uint16_t* p;
+      if ((rc = ralloc(R, &p, size_t(nnz) + 1))) return bail(rc);
+      R->bval = p;
+    }
+    if (nnz > 0) {
+      int colbits = 1;
+      while ((int64_t(1) << colbits) <= n) ++colbits;
+      int blkbits = 0;
+      while ((int64_t(1) << blkbits) < R->nblk) ++blkbits;
+      size_t sort_ws = raptr_csr_transpose_workspace_size(nnz, m, n);  // >= radix buffers
+      void* tmp = nullptr;
+      RAPTR_HIP_TRY(hipMalloc(&tmp, sort_ws));
+      // carve: rowid | key32 (unused) | sort buffers, in the order the
+      // transpose helper lays them out
+      Carver cv(tmp, sort_ws);
+      uint32_t* rowid = cv.take<uint32_t>(size_t(nnz));
+      (void)cv.take<int>(size_t(nnz));
+      rc = RAPTR_OK;
+      uint64_t* k0 = cv.take<uint64_t>(size_t(nnz));
+      uint64_t* k1 = cv.take<uint64_t>(size_t(nnz));
+      uint32_t* v0 = cv.take<uint32_t>(size_t(nnz));
+      uint32_t* v1 = cv.take<uint32_t>(size_t(nnz));
+      if (!cv.ok) rc = fail(RAPTR_ERR_WORKSPACE, "transpose scratch carve failed");
+      if (!rc) {
+        hipLaunchKernelGGL(k_bt_keys, dim3(grid_for(int64_t(m) * kWave, 256, 1 << 16)), dim3(256),
+                           0, st, d_rp, d_col, m, block_rows, colbits, k0, v0);
+        hipLaunchKernelGGL(k_expand_rows32, dim3(grid_for(int64_t(m) * kWave, 256, 1 << 16)),
+                           dim3(256), 0, st, d_rp, m, rowid);
+        int res = 0;
+        rc = radix_sort_pairs(k0, k1, v0, v1, nnz, colbits + blkbits, st, &res, cv);
+        if (!rc) {
+          uint64_t* keys = res ? k1 : k0;
+          uint32_t* idx = res ? v1 : v0;
+          hipLaunchKernelGGL(k_bt_gather, dim3(grid_for(nnz, 256, 1 << 16)), dim3(256), 0, st,
+                             (const uint32_t*)idx, nnz, (const uint32_t*)rowid, d_val,
+                             (const float*)R->rscale, value_type, R->brow, R->bval);
+          hipLaunchKernelGGL(k_bt_ptr,
+                             dim3(grid_for(int64_t(R->nblk) * (n + 1), 256, 1 << 16)), dim3(256),
+                             0, st, (const uint64_t*)keys, nnz, R->nblk, n, colbits, R->bptr);
+          hipError_t le = hipGetLastError();
+          if (le != hipSuccess) rc = fail(RAPTR_ERR_DEVICE, "transpose: %s", hipGetErrorString(le));
+        }
+      }
+      (void)hipStreamSynchronize(st);
+      (void)hipFree(tmp);
+      if (rc) return bail(rc);
+    } else {
+      (void)hipMemsetAsync(R->bptr, 0, sizeof(int64_t) * size_t(R->nblk) * (n + 1), st);
+    }
   }
   // ---- sensitivity and work vectors
   const int F = max_frames;
@@ -956,7 +862,8 @@ int raptr_recon_create(raptr_recon** out, int m, int n, int64_t nnz, const int64
   if (se != hipSuccess) return bail(fail(RAPTR_ERR_DEVICE, "create: %s", hipGetErrorString(se)));
   R->fwd_lanes = double(nnz) / std::max(m, 1) >= 192 ? 32 : 16;
   R->fwd_unroll = max_frames == 1 ? 1 : 4;
-  R->bwd_lds = (max_frames == 1 && block_rows <= kLdsRowsMax && block_rows % 4 == 0) ? 1 : 0;
+  R->bwd_lds = (layout == RAPTR_LAYOUT_BLOCKED && max_frames == 1 && block_rows <= kLdsRowsMax &&
+                block_rows % 4 == 0) ? 1 : 0;
   *out = R;
   return RAPTR_OK;
 }
@@ -1011,7 +918,8 @@ int raptr_recon_set_config(raptr_recon* R, int fwd_lanes, int fwd_unroll, int bw
 
 int raptr_recon_set_backward(raptr_recon* R, int staged, int lanes_per_column, int workgroups) {
   if (!R) return fail(RAPTR_ERR_INVALID_ARG, "null plan");
-  if (staged && (R->fmax != 1 || R->blk_rows > kLdsRowsMax || R->blk_rows % 4))
+  if (staged && (R->layout != RAPTR_LAYOUT_BLOCKED || R->fmax != 1 ||
+                 R->blk_rows > kLdsRowsMax || R->blk_rows % 4))
     return fail(RAPTR_ERR_INVALID_ARG,
                 "staged backward needs a single-frame plan with block_rows <= %d, "
                 "a multiple of 4", kLdsRowsMax);
@@ -1028,6 +936,44 @@ int raptr_recon_set_backward(raptr_recon* R, int staged, int lanes_per_column, i
   R->bwd_lds = staged ? 1 : 0;
   R->bwd_lpc = lanes_per_column;
   R->bwd_wgs = workgroups;
+  return RAPTR_OK;
+}
+
+int raptr_recon_layout(const raptr_recon* R) { return R ? R->layout : -1; }
+
+int raptr_recon_set_tiled(raptr_recon* R, int fwd_lanes, int fwd_unroll, int bwd_lanes,
+                          int bwd_unroll) {
+  if (!R) return fail(RAPTR_ERR_INVALID_ARG, "null plan");
+  if (R->layout != RAPTR_LAYOUT_TILED) return fail(RAPTR_ERR_INVALID_ARG, "plan is not tiled");
+  auto lanes_ok = [](int l) { return l == 4 || l == 8 || l == 16; };
+  auto unroll_ok = [](int u) { return u == 1 || u == 2 || u == 4; };
+  if (!lanes_ok(fwd_lanes) || !lanes_ok(bwd_lanes) || !unroll_ok(fwd_unroll) ||
+      !unroll_ok(bwd_unroll))
+    return fail(RAPTR_ERR_INVALID_ARG, "tiled lanes must be 4, 8 or 16 and unroll 1, 2 or 4");
+  if (R->t_fwd_lanes != fwd_lanes || R->t_fwd_unroll != fwd_unroll ||
+      R->t_bwd_lanes != bwd_lanes || R->t_bwd_unroll != bwd_unroll) {
+    for (auto& kv : R->graphs) {
+      if (kv.second.second) (void)hipGraphExecDestroy(kv.second.second);
+      if (kv.second.first) (void)hipGraphDestroy(kv.second.first);
+    }
+    R->graphs.clear();
+  }
+  R->t_fwd_lanes = fwd_lanes;
+  R->t_fwd_unroll = fwd_unroll;
+  R->t_bwd_lanes = bwd_lanes;
+  R->t_bwd_unroll = bwd_unroll;
+  return RAPTR_OK;
+}
+
+int raptr_recon_get_tiled(const raptr_recon* R, int* ntile, int* tile_cols, int* fwd_lanes,
+                          int* fwd_unroll, int* bwd_lanes, int* bwd_unroll) {
+  if (!R) return fail(RAPTR_ERR_INVALID_ARG, "null plan");
+  if (ntile) *ntile = R->ntile;
+  if (tile_cols) *tile_cols = R->tile_cols;
+  if (fwd_lanes) *fwd_lanes = R->t_fwd_lanes;
+  if (fwd_unroll) *fwd_unroll = R->t_fwd_unroll;
+  if (bwd_lanes) *bwd_lanes = R->t_bwd_lanes;
+  if (bwd_unroll) *bwd_unroll = R->t_bwd_unroll;
   return RAPTR_OK;
 }
 
@@ -1090,6 +1036,7 @@ int raptr_recon_forward(raptr_recon* R, int nframes, const float* d_x, const flo
   if (!frames_aligned(d_x, nframes) || !frames_aligned(d_y, nframes) ||
       !frames_aligned(d_e, nframes))
     return fail(RAPTR_ERR_INVALID_ARG, "x, y and e must be aligned to the frame vector");
+  if (R->layout == RAPTR_LAYOUT_TILED) return tiled_forward(R, d_x, d_y, d_e, S(stream));
   return launch_fwd_any(R, nframes, d_x, d_y, d_e, S(stream));
 }
 
@@ -1098,6 +1045,7 @@ int raptr_recon_backward(raptr_recon* R, int nframes, const float* d_x, raptr_st
     return fail(RAPTR_ERR_INVALID_ARG, "bad plan or frame count");
   if (!frames_aligned(d_x, nframes))
     return fail(RAPTR_ERR_INVALID_ARG, "x must be aligned to the frame vector");
+  if (R->layout == RAPTR_LAYOUT_TILED) return tiled_backward(R, d_x, R->e, S(stream), nullptr);
   if (nframes == 1 && R->bwd_lds) return launch_bwd_lds(R, d_x, S(stream), nullptr);
   for (int b = 0; b < R->nblk; ++b)
     RAPTR_TRY(launch_bwd_any(R, nframes, b, d_x, S(stream), nullptr));

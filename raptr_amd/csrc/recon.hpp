@@ -1,0 +1,159 @@
+// recon.hpp -- the reconstruction plan shared by its translation units
+// (recon.hip: plan, blocked layouts, batched kernels; recon_tiled.hip:
+// column-tiled single-frame layout).
+#pragma once
+
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
+
+#include <algorithm>
+#include <map>
+#include <vector>
+
+#include "primitives.hpp"
+
+namespace raptr {
+
+// --------------------------------------------------------- value types
+enum { kF32 = 0, kF16 = 1, kBF16 = 2, kU16 = 3 };
+
+__device__ __forceinline__ float h2f(uint16_t h, int vt) {
+  if (vt == kF16) return __half2float(__ushort_as_half(h));
+  if (vt == kU16) return float(h);
+  return __uint_as_float(uint32_t(h) << 16);
+}
+
+// f in [0, 1] (value / row maximum)
+__device__ __forceinline__ uint16_t f2h(float f, int vt) {
+  if (vt == kF16) return __half_as_ushort(__float2half_rn(f));
+  if (vt == kU16) return uint16_t(fminf(rintf(f * 65535.0f), 65535.0f));
+  // BF16, round to nearest even
+  uint32_t u = __float_as_uint(f);
+  uint32_t r = 0x7fffu + ((u >> 16) & 1u);
+  return uint16_t((u + r) >> 16);
+}
+
+template <int VT>
+struct Val;
+template <>
+struct Val<kF32> {
+  using T = float;
+  __device__ static float get(const float* p, int64_t k) { return __builtin_nontemporal_load(p + k); }
+};
+template <>
+struct Val<kF16> {
+  using T = uint16_t;
+  __device__ static float get(const uint16_t* p, int64_t k) {
+    return __half2float(__ushort_as_half(__builtin_nontemporal_load(p + k)));
+  }
+};
+template <>
+struct Val<kBF16> {
+  using T = uint16_t;
+  __device__ static float get(const uint16_t* p, int64_t k) {
+    return __uint_as_float(uint32_t(__builtin_nontemporal_load(p + k)) << 16);
+  }
+};
+template <>
+struct Val<kU16> {
+  using T = uint16_t;
+  __device__ static float get(const uint16_t* p, int64_t k) {
+    return float(__builtin_nontemporal_load(p + k));
+  }
+};
+
+// four consecutive matrix values (16-byte / 8-byte aligned) as floats
+template <int VT>
+__device__ __forceinline__ void load4(const typename Val<VT>::T* __restrict__ val, int64_t k,
+                                      float& v0, float& v1, float& v2, float& v3) {
+  if (VT == kF32) {
+    vf4 v = __builtin_nontemporal_load(
+        reinterpret_cast<const vf4*>(reinterpret_cast<const float*>(val) + k));
+    v0 = v.x, v1 = v.y, v2 = v.z, v3 = v.w;
+  } else {
+    vu2 h = __builtin_nontemporal_load(
+        reinterpret_cast<const vu2*>(reinterpret_cast<const uint16_t*>(val) + k));
+    v0 = h2f(uint16_t(h.x & 0xffff), VT), v1 = h2f(uint16_t(h.x >> 16), VT);
+    v2 = h2f(uint16_t(h.y & 0xffff), VT), v3 = h2f(uint16_t(h.y >> 16), VT);
+  }
+}
+}  // namespace raptr
+
+struct raptr_recon {
+  int m = 0, n = 0, vt = raptr::kF32, fmax = 1;
+  int64_t nnz = 0, pnnz = 0;
+  int blk_rows = 0, nblk = 0;
+  // forward (padded CSR)
+  int64_t* frp = nullptr;
+  int* fcol = nullptr;
+  void* fval = nullptr;
+  float* rscale = nullptr;
+  // backward (row-blocked transpose)
+  int64_t* bptr = nullptr;  // nblk * (n+1)
+  int* brow = nullptr;
+  void* bval = nullptr;
+  // sensitivity and work vectors (sized for fmax frames)
+  float* s = nullptr;
+  float* e = nullptr;
+  float* c = nullptr;
+  float* xx = nullptr;
+  double* partials = nullptr;
+  float* scale = nullptr;
+  double* norm = nullptr;
+  int fwd_lanes = 64;
+  int fwd_unroll = 1;  // vec4 groups (1 frame) or entries (F frames) in flight
+  int bwd_unroll = 4;
+  int bwd_lds = 0;     // single-frame backward staged through LDS (0: off)
+  int bwd_lpc = 16;    // lanes per column of the LDS-staged backward
+  int bwd_wgs = 512;   // workgroups of the LDS-staged backward
+  // column-tiled single-frame layout (RAPTR_LAYOUT_TILED): tile q holds
+  // columns [q*tile_cols, (q+1)*tile_cols) of every row, rows padded to
+  // multiples of 4 entries, 16-bit column offsets within the tile
+  int layout = RAPTR_LAYOUT_BLOCKED;
+  int ntile = 0, tile_cols = 0, tile_wgs = 0;
+  int64_t tnnz = 0;            // padded entries over all tiles
+  int64_t* tptr = nullptr;     // ntile*m + 1 segment starts, (tile, row) order
+  uint16_t* tidx = nullptr;    // column offset within the tile
+  void* tval = nullptr;        // float or 16-bit
+  int* trow = nullptr;         // ntile*(tile_wgs+1) row bounds per workgroup
+  float* tpart = nullptr;      // ntile*m partial row sums of the forward
+  float* cpart = nullptr;      // ntile*tile_wgs*tile_cols partial column sums
+  int t_fwd_lanes = 8, t_fwd_unroll = 2, t_bwd_lanes = 8, t_bwd_unroll = 2;
+  size_t bytes = 0;
+  std::vector<void*> allocs;
+  std::map<long long, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
+};
+
+namespace raptr {
+
+template <class T>
+int ralloc(raptr_recon* R, T** p, size_t count) {
+  void* q = nullptr;
+  hipError_t e = hipMalloc(&q, std::max<size_t>(count * sizeof(T), 256));
+  if (e == hipErrorOutOfMemory) {
+    (void)hipGetLastError();
+    return fail(RAPTR_ERR_OUT_OF_MEMORY, "recon: out of device memory (%zu B)",
+                count * sizeof(T));
+  }
+  RAPTR_HIP_TRY(e);
+  R->allocs.push_back(q);
+  R->bytes += count * sizeof(T);
+  *p = static_cast<T*>(q);
+  return RAPTR_OK;
+}
+
+inline int vbytes(int vt) { return vt == kF32 ? 4 : 2; }
+
+// column-tiled single-frame layout (recon_tiled.hip)
+int tiled_create(raptr_recon* R, const int64_t* d_rp, const int* d_col, const float* d_val,
+                 hipStream_t st);
+// forward + ratio into e (length m): e = y / (A x) per row, 0 where y or
+// A x is 0, folded with the row scale for 16-bit values
+int tiled_forward(const raptr_recon* R, const float* x, const float* y, float* e,
+                  hipStream_t st);
+// backward with update: xx = x * (A^T e) / (s + (s==0)) and FP64 partial
+// sums of xx (count in *nb) for the normalisation
+int tiled_backward(const raptr_recon* R, const float* x, const float* e, hipStream_t st,
+                   int* nb);
+
+}  // namespace raptr

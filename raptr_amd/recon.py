@@ -13,17 +13,27 @@ from . import _capi as capi
 from .device import DeviceArray
 
 VALUE_TYPES = {"f32": 0, "f16": 1, "bf16": 2, "u16": 3}
+LAYOUTS = {"blocked": 0, "tiled": 1}
 
 
 class Reconstructor:
-    def __init__(self, A, sens, value_type="f32", max_frames=1, block_rows=0, stream=None):
+    """layout: "blocked" (row-padded CSR + row-blocked transpose, any frame
+    count), "tiled" (column tiles through LDS, one frame) or "auto" (tiled
+    for one frame, blocked otherwise).  block_rows: row-block height of the
+    blocked layout; for the tiled layout a cap on the tile width."""
+
+    def __init__(self, A, sens, value_type="f32", max_frames=1, block_rows=0, stream=None,
+                 layout="auto"):
         self.m, self.n, self.nnz = A.m, A.n, A.nnz
         self.value_type = value_type
         self.max_frames = max_frames
+        if layout == "auto":
+            layout = "tiled" if max_frames == 1 else "blocked"
+        self.layout = layout
         p = C.c_void_p()
-        capi.call("raptr_recon_create", C.byref(p), A.m, A.n, A.nnz, A.row_ptr.ptr, A.col.ptr,
-                  A.val.ptr, sens.ptr, VALUE_TYPES[value_type], int(max_frames),
-                  int(block_rows), stream)
+        capi.call("raptr_recon_create_ex", C.byref(p), A.m, A.n, A.nnz, A.row_ptr.ptr,
+                  A.col.ptr, A.val.ptr, sens.ptr, VALUE_TYPES[value_type], int(max_frames),
+                  int(block_rows), LAYOUTS[layout], stream)
         self.handle = p.value
 
     def info(self):
@@ -34,10 +44,20 @@ class Reconstructor:
         st, lpc, wgs = C.c_int(), C.c_int(), C.c_int()
         capi.call("raptr_recon_get_backward", self.handle, C.byref(st), C.byref(lpc),
                   C.byref(wgs))
-        return {"device_bytes": b.value, "padded_nnz": p.value, "row_blocks": nb.value,
-                "fwd_lanes": fl.value, "fwd_unroll": fu.value, "bwd_unroll": bu.value,
-                "bwd_staged": st.value, "bwd_lanes_per_column": lpc.value,
-                "bwd_workgroups": wgs.value}
+        out = {"layout": self.layout, "device_bytes": b.value, "padded_nnz": p.value,
+               "row_blocks": nb.value, "fwd_lanes": fl.value, "fwd_unroll": fu.value,
+               "bwd_unroll": bu.value, "bwd_staged": st.value,
+               "bwd_lanes_per_column": lpc.value, "bwd_workgroups": wgs.value}
+        if self.layout == "tiled":
+            t = [C.c_int() for _ in range(6)]
+            capi.call("raptr_recon_get_tiled", self.handle, *[C.byref(v) for v in t])
+            out.update(dict(zip(("tiles", "tile_cols", "tiled_fwd_lanes", "tiled_fwd_unroll",
+                                 "tiled_bwd_lanes", "tiled_bwd_unroll"), [v.value for v in t])))
+        return out
+
+    def set_tiled(self, fwd_lanes, fwd_unroll, bwd_lanes, bwd_unroll):
+        capi.call("raptr_recon_set_tiled", self.handle, int(fwd_lanes), int(fwd_unroll),
+                  int(bwd_lanes), int(bwd_unroll))
 
     def set_config(self, fwd_lanes, fwd_unroll, bwd_unroll):
         capi.call("raptr_recon_set_config", self.handle, int(fwd_lanes), int(fwd_unroll),
@@ -86,6 +106,28 @@ class Reconstructor:
             return e0.elapsed_ms(e1) / reps
 
         inf = self.info()
+        if self.layout == "tiled":
+            shapes = [(l, u) for l in (4, 8, 16) for u in (1, 2, 4)]
+            fl, fu, bl, bu_ = (inf["tiled_fwd_lanes"], inf["tiled_fwd_unroll"],
+                               inf["tiled_bwd_lanes"], inf["tiled_bwd_unroll"])
+            best = None
+            for l, u in shapes:
+                self.set_tiled(l, u, bl, bu_)
+                ms = timed()
+                if best is None or ms < best[1]:
+                    best = ((l, u), ms)
+            fl, fu = best[0]
+            best = None
+            for l, u in shapes:
+                self.set_tiled(fl, fu, l, u)
+                ms = timed()
+                if best is None or ms < best[1]:
+                    best = ((l, u), ms)
+            bl, bu_ = best[0]
+            self.set_tiled(fl, fu, bl, bu_)
+            capi.call("raptr_memcpy_d2d", x.ptr, x_save.ptr, x.nbytes, stream)
+            return ({"layout": "tiled", "fwd_lanes": fl, "fwd_unroll": fu, "bwd_lanes": bl,
+                     "bwd_unroll": bu_}, best[1])
         bu = inf["bwd_unroll"]
         if nframes == 1:
             fshapes = [(fl, fu) for fl in (16, 32, 64) for fu in (1, 2)]

@@ -43,16 +43,21 @@ def _rel(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
-@pytest.mark.parametrize("block_rows", [0, 1777])
-def test_recon_f32_vs_fp64_oracle(gpu, oracle, small_system, block_rows):
+@pytest.mark.parametrize("layout,block_rows", [("blocked", 0), ("blocked", 1777), ("tiled", 0),
+                                               ("tiled", 5000)])
+def test_recon_f32_vs_fp64_oracle(gpu, oracle, small_system, layout, block_rows):
+    """Plain FP32 MLEM (both layouts; several row blocks, several column
+    tiles) against the FP64 host recurrence."""
     from raptr_amd.device import Stream
     from raptr_amd.recon import Reconstructor
     S = small_system
     n = S["grid"].nvox
     x64 = _oracle_mlem(oracle, S, S["y"], 5)
-    R = Reconstructor(S["A"], S["sens"], "f32", 1, block_rows=block_rows)
-    if block_rows:
+    R = Reconstructor(S["A"], S["sens"], "f32", 1, block_rows=block_rows, layout=layout)
+    if block_rows and layout == "blocked":
         assert R.info()["row_blocks"] > 5
+    if block_rows and layout == "tiled":
+        assert R.info()["tiles"] == 7
     x0 = np.full(n, 1.0 / n, np.float32)
     st = Stream()
     for graph in (False, True):
@@ -71,7 +76,7 @@ def test_recon_staged_backward(gpu, oracle, small_system, lpc, wgs, unroll):
     n = S["grid"].nvox
     x0 = np.full(n, 1.0 / n, np.float32)
     x64 = _oracle_mlem(oracle, S, S["y"], 5)
-    R = Reconstructor(S["A"], S["sens"], "f32", 1, block_rows=800)
+    R = Reconstructor(S["A"], S["sens"], "f32", 1, block_rows=800, layout="blocked")
     assert R.info()["bwd_staged"] == 1 and R.info()["row_blocks"] > 8
     R.set_config(16, 1, unroll)
     R.set_backward(True, lpc, wgs)
@@ -88,13 +93,15 @@ def test_recon_staged_backward_needs_small_slices(gpu, small_system):
     from raptr_amd import RaptrError
     from raptr_amd.recon import Reconstructor
     S = small_system
-    R = Reconstructor(S["A"], S["sens"], "f32", 1, block_rows=30000)
+    R = Reconstructor(S["A"], S["sens"], "f32", 1, block_rows=30000, layout="blocked")
     assert R.info()["bwd_staged"] == 0
     with pytest.raises(RaptrError):
         R.set_backward(True)
     Rb = Reconstructor(S["A"], S["sens"], "f32", 2)
     with pytest.raises(RaptrError):
         Rb.set_backward(True)
+    with pytest.raises(RaptrError):
+        Reconstructor(S["A"], S["sens"], "f32", 1, layout="tiled").set_backward(True)
 
 
 def test_recon_row_block_cut_is_immaterial(gpu, small_system):
@@ -102,9 +109,16 @@ def test_recon_row_block_cut_is_immaterial(gpu, small_system):
     S = small_system
     n = S["grid"].nvox
     x0 = np.full(n, 1.0 / n, np.float32)
-    a = Reconstructor(S["A"], S["sens"], "f32", 1, block_rows=0).mlem(S["y"], x0, 5)
-    b = Reconstructor(S["A"], S["sens"], "f32", 1, block_rows=999).mlem(S["y"], x0, 5)
+    a = Reconstructor(S["A"], S["sens"], "f32", 1, block_rows=0, layout="blocked").mlem(
+        S["y"], x0, 5)
+    b = Reconstructor(S["A"], S["sens"], "f32", 1, block_rows=999, layout="blocked").mlem(
+        S["y"], x0, 5)
     assert _rel(a, b) <= 1e-6
+    # the column-tiled layout (one and several tiles) gives the same images
+    for cap in (0, 3000):
+        c = Reconstructor(S["A"], S["sens"], "f32", 1, block_rows=cap, layout="tiled").mlem(
+            S["y"], x0, 5)
+        assert _rel(c, a) <= 1e-6
 
 
 @pytest.mark.parametrize("F", [2, 8, 16, 64])
@@ -139,8 +153,9 @@ def test_recon_batched_frames_match_single(gpu, oracle, small_system, F):
     assert np.all(np.isfinite(X))
 
 
+@pytest.mark.parametrize("layout", ["blocked", "tiled"])
 @pytest.mark.parametrize("vt,tol", [("u16", 1e-4), ("f16", 4e-4), ("bf16", 5e-3)])
-def test_recon_mixed_precision_error(gpu, oracle, small_system, vt, tol):
+def test_recon_mixed_precision_error(gpu, oracle, small_system, vt, tol, layout):
     """16-bit matrix values with per-row FP32 scale, FP32 accumulation:
     relative L2 against the FP64 oracle after 5 iterations.  Stated bounds:
     UNORM16 1e-4 (measured ~1e-5), FP16 4e-4 (measured 2.0e-4), BF16 5e-3
@@ -149,7 +164,7 @@ def test_recon_mixed_precision_error(gpu, oracle, small_system, vt, tol):
     S = small_system
     n = S["grid"].nvox
     x64 = _oracle_mlem(oracle, S, S["y"], 5)
-    R = Reconstructor(S["A"], S["sens"], vt, 1)
+    R = Reconstructor(S["A"], S["sens"], vt, 1, layout=layout)
     x = R.mlem(S["y"], np.full(n, 1.0 / n, np.float32), 5)
     rel = _rel(x, x64)
     mx = float(np.max(np.abs(x - x64)) / np.max(np.abs(x64)))
@@ -157,14 +172,16 @@ def test_recon_mixed_precision_error(gpu, oracle, small_system, vt, tol):
     assert rel <= tol
 
 
-def test_recon_graph_replays_match_eager(gpu, small_system):
+@pytest.mark.parametrize("layout", ["blocked", "tiled"])
+def test_recon_graph_replays_match_eager(gpu, small_system, layout):
     """Graph replays (with eager work and syncs between them) equal the eager
-    result every time."""
+    result every time: bit for bit for the blocked layout, within 1e-6 for
+    the tiled one (its backward adds with LDS atomics, in no fixed order)."""
     from raptr_amd.device import DeviceArray, Stream
     from raptr_amd.recon import Reconstructor
     S = small_system
     n = S["grid"].nvox
-    R = Reconstructor(S["A"], S["sens"], "f32", 1)
+    R = Reconstructor(S["A"], S["sens"], "f32", 1, layout=layout)
     x0 = np.full(n, 1.0 / n, np.float32)
     eager = R.mlem(S["y"], x0, 4, use_graph=False)
     st = Stream()
@@ -174,11 +191,16 @@ def test_recon_graph_replays_match_eager(gpu, small_system):
         dx.copy_from(x0, stream=st.handle)
         R.run(dy, dx, 4, 1, use_graph=True, stream=st.handle)
         st.sync()
-        np.testing.assert_array_equal(dx.numpy(), eager)
+        if layout == "blocked":
+            np.testing.assert_array_equal(dx.numpy(), eager)
+        else:
+            assert _rel(dx.numpy(), eager) <= 1e-6
         R.mlem(S["y"], x0, 1, use_graph=False)  # eager work between replays
 
 
-def test_recon_forward_ratio(gpu, small_system):
+@pytest.mark.parametrize("layout,cap", [("blocked", 0), ("tiled", 0), ("tiled", 3000)])
+def test_recon_forward_ratio(gpu, small_system, layout, cap):
+    """The forward pass alone: e = y / (A x) against an FP64 host product."""
     from raptr_amd import _capi as capi
     from raptr_amd.device import DeviceArray
     from raptr_amd.recon import Reconstructor
@@ -188,7 +210,7 @@ def test_recon_forward_ratio(gpu, small_system):
     x = np.random.default_rng(3).random(n).astype(np.float32)
     yt = np.array([np.dot(val[rp[i]:rp[i + 1]].astype(np.float64), x[col[rp[i]:rp[i + 1]]])
                    for i in range(A.m)])
-    R = Reconstructor(A, S["sens"], "f32", 1)
+    R = Reconstructor(A, S["sens"], "f32", 1, block_rows=cap, layout=layout)
     dx, dy = DeviceArray.from_numpy(x), DeviceArray.from_numpy(S["y"].astype(np.float32))
     de = DeviceArray(A.m, np.float32)
     capi.call("raptr_recon_forward", R.handle, 1, dx.ptr, dy.ptr, de.ptr, None)
@@ -201,6 +223,8 @@ def test_recon_rejects_bad_arguments(gpu, small_system):
     S = small_system
     with pytest.raises(RaptrError):
         Reconstructor(S["A"], S["sens"], "f32", 3)
+    with pytest.raises(RaptrError):
+        Reconstructor(S["A"], S["sens"], "f32", 2, layout="tiled")
     R = Reconstructor(S["A"], S["sens"], "f32", 2)
     with pytest.raises(RaptrError):
         R.mlem(np.zeros((S["A"].m, 4), np.float32), np.zeros((S["grid"].nvox, 4), np.float32), 1)
