@@ -242,9 +242,25 @@ def main():
     ms_b = timed(lambda: [bwd() for _ in range(reps)]) / reps
     info = R.info()
     nb = info["row_blocks"]
-    # algorithmic bytes: matrix (value + column index), row pointers, vectors
-    bytes_f = 8 * info["padded_nnz"] + 8 * (m + 1) + 4 * m + 4 * m + 4 * n
-    if info["bwd_staged"]:
+    fwd_name = "k_fwd1 (forward + ratio, one launch)"
+    if info["layout"] == "tiled":
+        # bytes each pass must move: padded matrix (4-byte value + 2-byte
+        # column offset), segment starts, and the vectors (y, the per-tile
+        # partial row sums written and re-read by the ratio kernel, e; the
+        # ratio read by every tile, the partial column sums, x, s, x')
+        T, TC = info["tiles"], info["tile_cols"]
+        W = -(-256 // T)  # workgroups per tile on a 256-CU device
+        tn = info["padded_nnz"]
+        bytes_f = 6 * tn + 8 * (T * m + 1) + 4 * m + 4 * T * m + (4 * T * m + 8 * m) + 4 * n
+        bytes_b = 6 * tn + 8 * (T * m + 1) + 4 * T * m + 2 * 4 * T * W * TC + 12 * n
+        fwd_name = f"k_tfwd + k_tratio (forward + ratio over {T} column tiles)"
+        bwd_name = f"k_tbwd + k_tupdate (backward + update over {T} column tiles)"
+    else:
+        # algorithmic bytes: matrix (value + column index), row pointers, vectors
+        bytes_f = 8 * info["padded_nnz"] + 8 * (m + 1) + 4 * m + 4 * m + 4 * n
+    if info["layout"] == "tiled":
+        pass  # set above
+    elif info["bwd_staged"]:
         # one launch; the ratio vector read once per workgroup from L2 (counted
         # once), column sums in LDS, x/s read and x' written once
         bytes_b = 8 * A.nnz + 8 * nb * (n + 1) + 4 * m + 12 * n + 8 * info["bwd_workgroups"]
@@ -256,8 +272,7 @@ def main():
     gbs_b = bytes_b / (ms_b * 1e-3) / 1e9
     log(f"forward pass {ms_f:.3f} ms ({gbs_f:.0f} GB/s), backward pass {ms_b:.3f} ms "
         f"({bwd_name}, {gbs_b:.0f} GB/s)")
-    dom = {"kernel": "k_fwd1 (forward + ratio, one launch)", "ms": ms_f, "bytes": bytes_f,
-           "gbs": gbs_f}
+    dom = {"kernel": fwd_name, "ms": ms_f, "bytes": bytes_f, "gbs": gbs_f}
     if ms_b > ms_f:
         dom = {"kernel": bwd_name, "ms": ms_b, "bytes": bytes_b, "gbs": gbs_b}
 
@@ -334,7 +349,8 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(dom["gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
                      "algorithmic_bytes": dom["bytes"], "avg_launch_ms": round(dom["ms"], 4)},
-        "passes": {"forward_ms": round(ms_f, 4), "forward_GBps": round(gbs_f, 1),
+        "passes": {"forward_kernel": fwd_name,
+                   "forward_ms": round(ms_f, 4), "forward_GBps": round(gbs_f, 1),
                    "backward_ms": round(ms_b, 4), "backward_GBps": round(gbs_b, 1),
                    "backward_kernel": bwd_name},
         "build": {"matrix_ms": round(ms_build, 1), "sensitivity_ms": round(ms_sens, 1),

@@ -159,6 +159,8 @@ template <int MODE>
 __global__ __launch_bounds__(kSmBlock) void k_sm_build(BuildArgs A) {
   extern __shared__ float smem[];
   __shared__ int wcnt[kSmBlock / kWave];
+  __shared__ int s_scan[kSmBlock / kWave];
+  __shared__ int s_lo[kSmBlock], s_pre[kSmBlock];
   __shared__ long long s_off;
   const int nr = A.nrays, pl = A.pl;
   float* rinfo = smem;                                   // nr * 8
@@ -213,38 +215,8 @@ __global__ __launch_bounds__(kSmBlock) void k_sm_build(BuildArgs A) {
 
     int cand_n = 0, rb_n = 0, flushed = 0;
     double proj = 0.0;
-    for (int base = 0; base < (bad ? 0 : nvox); base += kCullRound) {
-      // 2. cull one round of 1024 voxels, compact into cand[]
-      for (int r = 0; r < kCullRound / kSmBlock; ++r) {
-        int v = base + r * kSmBlock + t;
-        bool pass = false;
-        int ix = 0, iy = 0, iz = 0;
-        if (v < nvox) {
-          uint32_t qx = fdiv(uint32_t(v), A.div_yz);
-          uint32_t rem = uint32_t(v) - qx * A.div_yz.d;
-          uint32_t qy = fdiv(rem, A.div_z);
-          ix = int(qx);
-          iy = int(qy);
-          iz = int(rem - qy * A.div_z.d);
-          float c[3];
-          voxel_centre(A.G.g, ix, iy, iz, c);
-          pass = distance_point_line(p0, p1, c) < A.G.sumRadii;
-        }
-        uint64_t m = __ballot(pass);
-        if (lane == 0) wcnt[w] = __popcll(m);
-        __syncthreads();
-        int pre = 0, tot = 0;
-        for (int ww = 0; ww < kSmBlock / kWave; ++ww) {
-          int c = wcnt[ww];
-          pre += ww < w ? c : 0;
-          tot += c;
-        }
-        if (pass) cand[cand_n + pre + __popcll(m & lanemask_lt())] = v;
-        cand_n += tot;
-        __syncthreads();
-      }
-      bool last = base + kCullRound >= nvox;
-      if (cand_n == 0 || (cand_n < kCandFlush && !last)) continue;
+    // sample, then keep (or count / accumulate) the candidates in cand[]
+    auto process = [&]() {
       // 3. sample candidates densely
       for (int c = t; c < cand_n; c += kSmBlock) {
         int v = cand[c];
@@ -262,14 +234,14 @@ __global__ __launch_bounds__(kSmBlock) void k_sm_build(BuildArgs A) {
         }
         __syncthreads();
         cand_n = 0;
-        continue;
+        return;
       }
       if (MODE == kModeFwd) {
         for (int c = t; c < cand_n; c += kSmBlock)
           proj += double(cval[c]) * double(A.xin[cand[c]]);
         __syncthreads();
         cand_n = 0;
-        continue;
+        return;
       }
       // flush the row buffer first if the candidates could overflow it
       if (MODE == kModeFill && rb_n + cand_n > kRowBuf) {
@@ -327,7 +299,92 @@ __global__ __launch_bounds__(kSmBlock) void k_sm_build(BuildArgs A) {
         rb_n = 0;
       }
       cand_n = 0;
+    };
+    // 2. cull.  For a voxel column (ix, iy) the centres closer to the line
+    // than the radius form one z-interval (a convex set meets a line in an
+    // interval); it is found in FP64, widened by a relative radius slack and
+    // one voxel at each end, and every voxel in it gets the exact float test
+    // of the reference.  Columns run in ascending (ix, iy) and z ascends
+    // within a column, so the candidates and their order are those of a
+    // scan over all voxels.
+    const int nz = A.G.g.n[2];
+    const int ncols = nx * ny;
+    double u[3], len2 = 0.0;
+    for (int d = 0; d < 3; ++d) {
+      u[d] = double(p1[d]) - double(p0[d]);
+      len2 += u[d] * u[d];
     }
+    const double il = len2 > 0.0 ? 1.0 / sqrt(len2) : 0.0;
+    for (int d = 0; d < 3; ++d) u[d] *= il;
+    const double rr = double(A.G.sumRadii) * (1.0 + 1e-4) + 1e-4;
+    const double qa = u[0] * u[0] + u[1] * u[1];
+    for (int cb = 0; cb < (bad ? 0 : ncols); cb += kSmBlock) {
+      const int cc = cb + t;
+      int lo = 0, cnt = 0;
+      if (cc < ncols) {
+        const int ix = cc / ny, iy = cc - (cc / ny) * ny;
+        const double ax = double(A.G.g.o[0]) + (ix + 0.5) * double(A.G.g.d[0]) - double(p0[0]);
+        const double ay = double(A.G.g.o[1]) + (iy + 0.5) * double(A.G.g.d[1]) - double(p0[1]);
+        const double sp = ax * u[0] + ay * u[1];
+        const double qc = ax * ax + ay * ay - sp * sp - rr * rr;
+        int hi = -1;
+        if (qa < 1e-12) {
+          if (qc < 0.0) lo = 0, hi = nz - 1;
+        } else {
+          const double qb = -2.0 * sp * u[2];
+          const double disc = qb * qb - 4.0 * qa * qc;
+          if (disc >= 0.0) {
+            const double sq = sqrt(disc);
+            const double z1 = double(p0[2]) + (-qb - sq) / (2.0 * qa);
+            const double z2 = double(p0[2]) + (-qb + sq) / (2.0 * qa);
+            const double f1 = (z1 - double(A.G.g.o[2])) / double(A.G.g.d[2]) - 0.5;
+            const double f2 = (z2 - double(A.G.g.o[2])) / double(A.G.g.d[2]) - 0.5;
+            lo = int(fmax(floor(f1) - 1.0, 0.0));
+            hi = int(fmin(ceil(f2) + 1.0, double(nz - 1)));
+          }
+        }
+        cnt = hi >= lo ? hi - lo + 1 : 0;
+      }
+      int tot;
+      const int pre = block_exclusive_scan<int, kSmBlock>(cnt, s_scan, &tot);
+      s_lo[t] = lo;
+      s_pre[t] = pre;
+      __syncthreads();
+      for (int k0 = 0; k0 < tot; k0 += kSmBlock) {
+        const int k = k0 + t;
+        bool pass = false;
+        int v = 0;
+        if (k < tot) {
+          int jl = 0, jh = kSmBlock - 1;  // last column with s_pre <= k
+          while (jl < jh) {
+            const int mid = (jl + jh + 1) >> 1;
+            if (s_pre[mid] <= k) jl = mid; else jh = mid - 1;
+          }
+          const int col = cb + jl;
+          const int ix = col / ny, iy = col - (col / ny) * ny;
+          const int iz = s_lo[jl] + (k - s_pre[jl]);
+          v = col * nz + iz;
+          float c[3];
+          voxel_centre(A.G.g, ix, iy, iz, c);
+          pass = distance_point_line(p0, p1, c) < A.G.sumRadii;
+        }
+        uint64_t m = __ballot(pass);
+        if (lane == 0) wcnt[w] = __popcll(m);
+        __syncthreads();
+        int wpre = 0, wtot = 0;
+        for (int ww = 0; ww < kSmBlock / kWave; ++ww) {
+          const int c = wcnt[ww];
+          wpre += ww < w ? c : 0;
+          wtot += c;
+        }
+        if (pass) cand[cand_n + wpre + __popcll(m & lanemask_lt())] = v;
+        cand_n += wtot;
+        __syncthreads();
+        if (cand_n >= kCandFlush) process();
+      }
+      __syncthreads();
+    }
+    if (cand_n > 0) process();
     if (MODE == kModeFill && rb_n > 0) {
       if (t == 0) {
         unsigned long long off = atomicAdd(A.st_count, (unsigned long long)rb_n);
