@@ -365,16 +365,17 @@ def main():
 
 
 def copy_peak(timed, st, nbytes=2 << 30, reps=5):
-    """Device-to-device copy of nbytes (the HIP runtime's copy kernel): the
-    box's attainable read+write bandwidth, and the known-size dispatch that
-    calibrates the FETCH_SIZE / WRITE_SIZE counters."""
+    """Device-to-device copy of nbytes by the library's streaming kernel
+    (k_stream_copy): the box's attainable read+write bandwidth, and the
+    known-size dispatch that calibrates the FETCH_SIZE / WRITE_SIZE
+    counters."""
     from raptr_amd import _capi as capi
     from raptr_amd.device import DeviceArray
     a = DeviceArray(nbytes // 4, np.float32)
     b = DeviceArray(nbytes // 4, np.float32)
     capi.call("raptr_memset", a.ptr, 0, nbytes, st)
-    capi.call("raptr_memcpy_d2d", b.ptr, a.ptr, nbytes, st)
-    ms = timed(lambda: [capi.call("raptr_memcpy_d2d", b.ptr, a.ptr, nbytes, st)
+    capi.call("raptr_stream_copy", b.ptr, a.ptr, nbytes // 4, st)
+    ms = timed(lambda: [capi.call("raptr_stream_copy", b.ptr, a.ptr, nbytes // 4, st)
                         for _ in range(reps)]) / reps
     del a, b
     return {"bytes": nbytes, "ms": round(ms, 4), "GBps": round(2 * nbytes / (ms * 1e-3) / 1e9, 1)}

@@ -71,6 +71,10 @@ int raptr_memcpy_h2d(void* dst, const void* src, size_t bytes, raptr_stream_t s)
 int raptr_memcpy_d2h(void* dst, const void* src, size_t bytes, raptr_stream_t s);
 int raptr_memcpy_d2d(void* dst, const void* src, size_t bytes, raptr_stream_t s);
 int raptr_memset(void* dst, int value, size_t bytes, raptr_stream_t s);
+/* dst[i] = src[i], i < n, by a plain streaming kernel of the library (no
+ * counterpart in the reference): the known-size dispatch that calibrates
+ * bandwidth and the memory-traffic counters. */
+int raptr_stream_copy(float* d_dst, const float* d_src, int64_t n, raptr_stream_t s);
 int raptr_stream_create(raptr_stream_t* s);
 int raptr_stream_destroy(raptr_stream_t s);
 int raptr_stream_sync(raptr_stream_t s);

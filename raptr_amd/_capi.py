@@ -61,6 +61,7 @@ _SIGS = {
     "raptr_memcpy_d2h": (i32, [vp, vp, sz, vp]),
     "raptr_memcpy_d2d": (i32, [vp, vp, sz, vp]),
     "raptr_memset": (i32, [vp, i32, sz, vp]),
+    "raptr_stream_copy": (i32, [vp, vp, i64, vp]),
     "raptr_stream_create": (i32, [P(vp)]),
     "raptr_stream_destroy": (i32, [vp]),
     "raptr_stream_sync": (i32, [vp]),

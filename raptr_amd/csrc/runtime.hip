@@ -54,6 +54,25 @@ int make_geom(const raptr_setup_t* setup, const raptr_grid_t* grid, Geom* G) {
 
 using namespace raptr;
 
+namespace {
+// plain streaming copy (16-byte accesses when aligned): the known-size
+// dispatch for bandwidth and traffic-counter calibration
+__global__ void k_stream_copy(float* __restrict__ dst, const float* __restrict__ src, int64_t n,
+                              int vec) {
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  const int64_t t0 = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (vec) {
+    typedef float vf4_t __attribute__((ext_vector_type(4)));
+    const int64_t n4 = n / 4;
+    for (int64_t i = t0; i < n4; i += stride)
+      reinterpret_cast<vf4_t*>(dst)[i] = reinterpret_cast<const vf4_t*>(src)[i];
+    for (int64_t i = 4 * n4 + t0; i < n; i += stride) dst[i] = src[i];
+  } else {
+    for (int64_t i = t0; i < n; i += stride) dst[i] = src[i];
+  }
+}
+}  // namespace
+
 extern "C" {
 
 const char* raptr_status_string(int st) {
@@ -120,6 +139,16 @@ int raptr_memcpy_d2d(void* dst, const void* src, size_t bytes, raptr_stream_t s)
   RAPTR_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, S(s)));
   return RAPTR_OK;
 }
+int raptr_stream_copy(float* dst, const float* src, int64_t n, raptr_stream_t s) {
+  if (n < 0 || (n > 0 && (!dst || !src))) return fail(RAPTR_ERR_INVALID_ARG, "bad copy");
+  if (n == 0) return RAPTR_OK;
+  const bool vec = ((reinterpret_cast<uintptr_t>(dst) | reinterpret_cast<uintptr_t>(src)) & 15) == 0;
+  hipLaunchKernelGGL(k_stream_copy, dim3(grid_for(n / 4 + 1, 256, 1 << 16)), dim3(256), 0, S(s),
+                     dst, src, n, vec ? 1 : 0);
+  RAPTR_LAUNCH_CHECK();
+  return RAPTR_OK;
+}
+
 int raptr_memset(void* dst, int value, size_t bytes, raptr_stream_t s) {
   if (!bytes) return RAPTR_OK;
   RAPTR_HIP_TRY(hipMemsetAsync(dst, value, bytes, S(s)));
