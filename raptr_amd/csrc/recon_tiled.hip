@@ -163,44 +163,52 @@ __global__ __launch_bounds__(kTileBlock) void k_tfwd(int m, int n, int tcols, in
   const int g = threadIdx.x % LANES, grp = threadIdx.x / LANES;
   const int64_t* P = tptr + int64_t(q) * m;
   float* out = tpart + int64_t(q) * m;
-  for (int i = r0 + grp; i < r1; i += GROUPS) {
-    if (y[i] == 0.0f) {  // e = 0 whatever A x is
-      if (g == 0) out[i] = 0.0f;
-      continue;
-    }
-    const int64_t b = P[i], en = P[i + 1];
+  // the next row's segment bounds and count are loaded while the current
+  // row streams, so a row costs one memory round trip, not two
+  int i = r0 + grp;
+  int64_t b = 0, en = 0;
+  float yi = 0.0f;
+  if (i < r1) b = P[i], en = P[i + 1], yi = y[i];
+  for (; i < r1; i += GROUPS) {
+    const int inx = i + GROUPS;
+    int64_t bn = 0, enn = 0;
+    float yn = 0.0f;
+    if (inx < r1) bn = P[inx], enn = P[inx + 1], yn = y[inx];
     float acc = 0.0f;
-    int64_t k = b + 4 * g;
-    for (; k + int64_t(4) * LANES * (U - 1) < en; k += int64_t(4) * LANES * U) {
-      float v[U][4];
-      int c[U][4];
+    if (yi != 0.0f) {  // e = 0 whatever A x is when y = 0
+      int64_t k = b + 4 * g;
+      for (; k + int64_t(4) * LANES * (U - 1) < en; k += int64_t(4) * LANES * U) {
+        float v[U][4];
+        int c[U][4];
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int64_t kk = k + int64_t(4) * LANES * u;
-        load4<VT>(tval, kk, v[u][0], v[u][1], v[u][2], v[u][3]);
-        idx4(tidx, kk, c[u][0], c[u][1], c[u][2], c[u][3]);
+        for (int u = 0; u < U; ++u) {
+          const int64_t kk = k + int64_t(4) * LANES * u;
+          load4<VT>(tval, kk, v[u][0], v[u][1], v[u][2], v[u][3]);
+          idx4(tidx, kk, c[u][0], c[u][1], c[u][2], c[u][3]);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          acc = fmaf(v[u][0], xs[c[u][0]], acc);
+          acc = fmaf(v[u][1], xs[c[u][1]], acc);
+          acc = fmaf(v[u][2], xs[c[u][2]], acc);
+          acc = fmaf(v[u][3], xs[c[u][3]], acc);
+        }
+      }
+      for (; k < en; k += int64_t(4) * LANES) {
+        float v0, v1, v2, v3;
+        int c0_, c1_, c2_, c3_;
+        load4<VT>(tval, k, v0, v1, v2, v3);
+        idx4(tidx, k, c0_, c1_, c2_, c3_);
+        acc = fmaf(v0, xs[c0_], acc);
+        acc = fmaf(v1, xs[c1_], acc);
+        acc = fmaf(v2, xs[c2_], acc);
+        acc = fmaf(v3, xs[c3_], acc);
       }
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        acc = fmaf(v[u][0], xs[c[u][0]], acc);
-        acc = fmaf(v[u][1], xs[c[u][1]], acc);
-        acc = fmaf(v[u][2], xs[c[u][2]], acc);
-        acc = fmaf(v[u][3], xs[c[u][3]], acc);
-      }
+      for (int off = LANES / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LANES);
     }
-    for (; k < en; k += int64_t(4) * LANES) {
-      float v0, v1, v2, v3;
-      int c0_, c1_, c2_, c3_;
-      load4<VT>(tval, k, v0, v1, v2, v3);
-      idx4(tidx, k, c0_, c1_, c2_, c3_);
-      acc = fmaf(v0, xs[c0_], acc);
-      acc = fmaf(v1, xs[c1_], acc);
-      acc = fmaf(v2, xs[c2_], acc);
-      acc = fmaf(v3, xs[c3_], acc);
-    }
-#pragma unroll
-    for (int off = LANES / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LANES);
     if (g == 0) out[i] = acc;
+    b = bn, en = enn, yi = yn;
   }
 }
 
@@ -239,35 +247,43 @@ __global__ __launch_bounds__(kTileBlock) void k_tbwd(int m, int n, int tcols, in
   constexpr int GROUPS = kTileBlock / LANES;
   const int g = threadIdx.x % LANES, grp = threadIdx.x / LANES;
   const int64_t* P = tptr + int64_t(q) * m;
-  for (int i = r0 + grp; i < r1; i += GROUPS) {
-    const float ei = e[i];
-    if (ei == 0.0f) continue;
-    const int64_t b = P[i], en = P[i + 1];
-    int64_t k = b + 4 * g;
-    for (; k + int64_t(4) * LANES * (U - 1) < en; k += int64_t(4) * LANES * U) {
-      float v[U][4];
-      int c[U][4];
+  int i = r0 + grp;
+  int64_t b = 0, en = 0;
+  float ei = 0.0f;
+  if (i < r1) b = P[i], en = P[i + 1], ei = e[i];
+  for (; i < r1; i += GROUPS) {
+    const int inx = i + GROUPS;
+    int64_t bn = 0, enn = 0;
+    float en_ = 0.0f;
+    if (inx < r1) bn = P[inx], enn = P[inx + 1], en_ = e[inx];
+    if (ei != 0.0f) {
+      int64_t k = b + 4 * g;
+      for (; k + int64_t(4) * LANES * (U - 1) < en; k += int64_t(4) * LANES * U) {
+        float v[U][4];
+        int c[U][4];
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int64_t kk = k + int64_t(4) * LANES * u;
-        load4<VT>(tval, kk, v[u][0], v[u][1], v[u][2], v[u][3]);
-        idx4(tidx, kk, c[u][0], c[u][1], c[u][2], c[u][3]);
+        for (int u = 0; u < U; ++u) {
+          const int64_t kk = k + int64_t(4) * LANES * u;
+          load4<VT>(tval, kk, v[u][0], v[u][1], v[u][2], v[u][3]);
+          idx4(tidx, kk, c[u][0], c[u][1], c[u][2], c[u][3]);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+          for (int j = 0; j < 4; ++j)
+            if (v[u][j] != 0.0f) atomicAdd(&cs[c[u][j]], v[u][j] * ei);
       }
-#pragma unroll
-      for (int u = 0; u < U; ++u)
+      for (; k < en; k += int64_t(4) * LANES) {
+        float v[4];
+        int c[4];
+        load4<VT>(tval, k, v[0], v[1], v[2], v[3]);
+        idx4(tidx, k, c[0], c[1], c[2], c[3]);
 #pragma unroll
         for (int j = 0; j < 4; ++j)
-          if (v[u][j] != 0.0f) atomicAdd(&cs[c[u][j]], v[u][j] * ei);
+          if (v[j] != 0.0f) atomicAdd(&cs[c[j]], v[j] * ei);
+      }
     }
-    for (; k < en; k += int64_t(4) * LANES) {
-      float v[4];
-      int c[4];
-      load4<VT>(tval, k, v[0], v[1], v[2], v[3]);
-      idx4(tidx, k, c[0], c[1], c[2], c[3]);
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-        if (v[j] != 0.0f) atomicAdd(&cs[c[j]], v[j] * ei);
-    }
+    b = bn, en = enn, ei = en_;
   }
   __syncthreads();
   float* out = cpart + (int64_t(q) * W + w) * tcols;
