@@ -342,6 +342,11 @@ int raptr_recon_set_backward(raptr_recon* plan, int staged, int lanes_per_column
                              int workgroups);
 int raptr_recon_get_backward(const raptr_recon* plan, int* staged, int* lanes_per_column,
                              int* workgroups);
+/* Row blocks per backward launch of the blocked layout (1..64): a wave
+ * walks its column through that many consecutive blocks, keeping the sum in
+ * registers (longer waves, fewer launches). */
+int raptr_recon_set_blocks_per_launch(raptr_recon* plan, int blocks);
+int raptr_recon_get_blocks_per_launch(const raptr_recon* plan);
 /* niter MLEM iterations (reco.cu:230-303) of nframes frames at once
  * (power of two <= max_frames).  d_y: counts [m][nframes]; d_x: images
  * [n][nframes], start images in, results out.  Per frame and iteration:

@@ -293,7 +293,7 @@ __global__ __launch_bounds__(256) void k_fwdF(int64_t m, const int64_t* __restri
 // xx = x * c / (s + (s==0)) and writes FP64 partial sums per block/frame.
 // One wave per column, lanes as in the forward (frame vectors of V).
 template <int VT, int FR, int UNROLL>
-__global__ __launch_bounds__(256) void k_bwd(int n, const int64_t* __restrict__ ptr,
+__global__ __launch_bounds__(256) void k_bwd(int n, const int64_t* __restrict__ ptr, int nbl,
                                              const int* __restrict__ brow,
                                              const typename Val<VT>::T* __restrict__ val,
                                              const float* __restrict__ e, float* __restrict__ c,
@@ -311,9 +311,16 @@ __global__ __launch_bounds__(256) void k_bwd(int n, const int64_t* __restrict__ 
   float acc[V];
 #pragma unroll
   for (int v = 0; v < V; ++v) acc[v] = 0.0f;
-  if (j < n) {
-    const float* eq = e + V * q;
-    const int64_t b = ptr[j], en = ptr[j + 1];
+  // segment bounds of column j in the launch's nbl row blocks, one block
+  // per lane, loaded together and broadcast block by block
+  int64_t pb = 0, pe = 0;
+  if (j < n && lane < nbl) {
+    pb = ptr[int64_t(lane) * (n + 1) + j];
+    pe = ptr[int64_t(lane) * (n + 1) + j + 1];
+  }
+  const float* eq = e + V * q;
+  for (int bb = 0; bb < nbl && j < n; ++bb) {
+    const int64_t b = __shfl(pb, bb, kWave), en = __shfl(pe, bb, kWave);
     int64_t k = b + slot;
     for (; k + int64_t(UNROLL - 1) * SLOTS < en; k += int64_t(UNROLL) * SLOTS) {
       float wv[UNROLL];
@@ -338,6 +345,8 @@ __global__ __launch_bounds__(256) void k_bwd(int n, const int64_t* __restrict__ 
 #pragma unroll
       for (int v = 0; v < V; ++v) acc[v] = fmaf(wv, g[v], acc[v]);
     }
+  }
+  if (j < n) {
 #pragma unroll
     for (int off = kWave / 2; off >= Q; off >>= 1)
 #pragma unroll
@@ -568,17 +577,18 @@ int launch_fwd_any(const raptr_recon* R, int F, const float* x, const float* y, 
   return fail(RAPTR_ERR_INVALID_ARG, "frame count %d / unroll %d unsupported", F, U);
 }
 
-int launch_bwd_any(const raptr_recon* R, int F, int blk, const float* x, hipStream_t st,
-                   int* nb_out) {
+// one launch over row blocks [blk, blk + nbl)
+int launch_bwd_any(const raptr_recon* R, int F, int blk, int nbl, const float* x,
+                   hipStream_t st, int* nb_out) {
   const int n = R->n;
   unsigned g = unsigned((n + 3) / 4);
   if (nb_out) *nb_out = int(g);
   const int64_t* ptr = R->bptr + int64_t(blk) * (n + 1);
-  int first = blk == 0, last = blk == R->nblk - 1;
+  int first = blk == 0, last = blk + nbl == R->nblk;
   int U = R->bwd_unroll;
 #define RAPTR_B(VV, FF, UU)                                                                 \
   if (R->vt == VV && F == FF && U == UU) {                                                  \
-    hipLaunchKernelGGL((k_bwd<VV, FF, UU>), dim3(g), dim3(256), 0, st, n, ptr, R->brow,      \
+    hipLaunchKernelGGL((k_bwd<VV, FF, UU>), dim3(g), dim3(256), 0, st, n, ptr, nbl, R->brow, \
                        (const typename Val<VV>::T*)R->bval, R->e, R->c, first, last, x,     \
                        R->s, R->xx, R->partials);                                           \
     RAPTR_LAUNCH_CHECK();                                                                   \
@@ -647,7 +657,8 @@ int iteration(raptr_recon* R, int F, const float* y, float* x, hipStream_t st) {
     RAPTR_TRY(launch_bwd_lds(R, x, st, &nb));
   } else {
     RAPTR_TRY(launch_fwd_any(R, F, x, y, R->e, st));
-    for (int b = 0; b < R->nblk; ++b) RAPTR_TRY(launch_bwd_any(R, F, b, x, st, &nb));
+    for (int b = 0; b < R->nblk; b += R->bwd_bpl)
+      RAPTR_TRY(launch_bwd_any(R, F, b, std::min(R->bwd_bpl, R->nblk - b), x, st, &nb));
   }
   hipLaunchKernelGGL(k_norms, dim3(1), dim3(256), 0, st, (const double*)R->partials, nb, F,
                      R->scale, R->norm);
@@ -941,6 +952,23 @@ int raptr_recon_set_backward(raptr_recon* R, int staged, int lanes_per_column, i
 
 int raptr_recon_layout(const raptr_recon* R) { return R ? R->layout : -1; }
 
+int raptr_recon_set_blocks_per_launch(raptr_recon* R, int blocks) {
+  if (!R) return fail(RAPTR_ERR_INVALID_ARG, "null plan");
+  if (blocks < 1 || blocks > 64)
+    return fail(RAPTR_ERR_INVALID_ARG, "blocks per launch must be in [1, 64]");
+  if (R->bwd_bpl != blocks) {
+    for (auto& kv : R->graphs) {
+      if (kv.second.second) (void)hipGraphExecDestroy(kv.second.second);
+      if (kv.second.first) (void)hipGraphDestroy(kv.second.first);
+    }
+    R->graphs.clear();
+  }
+  R->bwd_bpl = blocks;
+  return RAPTR_OK;
+}
+
+int raptr_recon_get_blocks_per_launch(const raptr_recon* R) { return R ? R->bwd_bpl : -1; }
+
 int raptr_recon_set_tiled(raptr_recon* R, int fwd_lanes, int fwd_unroll, int bwd_lanes,
                           int bwd_unroll) {
   if (!R) return fail(RAPTR_ERR_INVALID_ARG, "null plan");
@@ -1047,8 +1075,9 @@ int raptr_recon_backward(raptr_recon* R, int nframes, const float* d_x, raptr_st
     return fail(RAPTR_ERR_INVALID_ARG, "x must be aligned to the frame vector");
   if (R->layout == RAPTR_LAYOUT_TILED) return tiled_backward(R, d_x, R->e, S(stream), nullptr);
   if (nframes == 1 && R->bwd_lds) return launch_bwd_lds(R, d_x, S(stream), nullptr);
-  for (int b = 0; b < R->nblk; ++b)
-    RAPTR_TRY(launch_bwd_any(R, nframes, b, d_x, S(stream), nullptr));
+  for (int b = 0; b < R->nblk; b += R->bwd_bpl)
+    RAPTR_TRY(launch_bwd_any(R, nframes, b, std::min(R->bwd_bpl, R->nblk - b), d_x, S(stream),
+                             nullptr));
   return RAPTR_OK;
 }
 

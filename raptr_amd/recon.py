@@ -47,13 +47,17 @@ class Reconstructor:
         out = {"layout": self.layout, "device_bytes": b.value, "padded_nnz": p.value,
                "row_blocks": nb.value, "fwd_lanes": fl.value, "fwd_unroll": fu.value,
                "bwd_unroll": bu.value, "bwd_staged": st.value,
-               "bwd_lanes_per_column": lpc.value, "bwd_workgroups": wgs.value}
+               "bwd_lanes_per_column": lpc.value, "bwd_workgroups": wgs.value,
+               "bwd_blocks_per_launch": capi.lib().raptr_recon_get_blocks_per_launch(self.handle)}
         if self.layout == "tiled":
             t = [C.c_int() for _ in range(6)]
             capi.call("raptr_recon_get_tiled", self.handle, *[C.byref(v) for v in t])
             out.update(dict(zip(("tiles", "tile_cols", "tiled_fwd_lanes", "tiled_fwd_unroll",
                                  "tiled_bwd_lanes", "tiled_bwd_unroll"), [v.value for v in t])))
         return out
+
+    def set_blocks_per_launch(self, blocks):
+        capi.call("raptr_recon_set_blocks_per_launch", self.handle, int(blocks))
 
     def set_tiled(self, fwd_lanes, fwd_unroll, bwd_lanes, bwd_unroll):
         capi.call("raptr_recon_set_tiled", self.handle, int(fwd_lanes), int(fwd_unroll),
@@ -140,7 +144,8 @@ class Reconstructor:
             if best_f is None or ms < best_f[1]:
                 best_f = ((fl, fu), ms)
         fl, fu = best_f[0]
-        bshapes = [(0, 16, 512, u) for u in (2, 4, 8)]
+        bshapes = [(0, 16, 512, u, bpl) for u in (2, 4, 8) for bpl in (1, 4, 16)
+                   if bpl == 1 or bpl < inf["row_blocks"]]
         staged_ok = nframes == 1 and self.max_frames == 1
         if staged_ok:
             try:  # the plan's row blocks may be too large for LDS slices
@@ -148,23 +153,25 @@ class Reconstructor:
             except capi.RaptrError:
                 staged_ok = False
         if staged_ok:
-            bshapes += [(1, lpc, wgs, u) for lpc in (8, 16, 32) for wgs in (256, 512, 1024)
+            bshapes += [(1, lpc, wgs, u, 1) for lpc in (8, 16, 32) for wgs in (256, 512, 1024)
                         for u in (2, 4)]
         best = None
-        for st, lpc, wgs, u in bshapes:
+        for st, lpc, wgs, u, bpl in bshapes:
             if staged_ok:
                 self.set_backward(st, lpc, wgs)
             self.set_config(fl, fu, u)
+            self.set_blocks_per_launch(bpl)
             ms = timed()
             if best is None or ms < best[1]:
-                best = ((st, lpc, wgs, u), ms)
-        st, lpc, wgs, u = best[0]
+                best = ((st, lpc, wgs, u, bpl), ms)
+        st, lpc, wgs, u, bpl = best[0]
         if staged_ok:
             self.set_backward(st, lpc, wgs)
         self.set_config(fl, fu, u)
+        self.set_blocks_per_launch(bpl)
         capi.call("raptr_memcpy_d2d", x.ptr, x_save.ptr, x.nbytes, stream)
         cfg = {"fwd_lanes": fl, "fwd_unroll": fu, "bwd_unroll": u, "bwd_staged": st,
-               "bwd_lanes_per_column": lpc, "bwd_workgroups": wgs}
+               "bwd_lanes_per_column": lpc, "bwd_workgroups": wgs, "bwd_blocks_per_launch": bpl}
         return cfg, best[1]
 
     def close(self):

@@ -114,6 +114,12 @@ def test_recon_row_block_cut_is_immaterial(gpu, small_system):
     b = Reconstructor(S["A"], S["sens"], "f32", 1, block_rows=999, layout="blocked").mlem(
         S["y"], x0, 5)
     assert _rel(a, b) <= 1e-6
+    # several row blocks per backward launch (sums carried in registers)
+    Rb = Reconstructor(S["A"], S["sens"], "f32", 1, block_rows=999, layout="blocked")
+    assert Rb.info()["row_blocks"] > 8
+    for bpl in (3, 64):
+        Rb.set_blocks_per_launch(bpl)
+        assert _rel(Rb.mlem(S["y"], x0, 5), a) <= 1e-6
     # the column-tiled layout (one and several tiles) gives the same images
     for cap in (0, 3000):
         c = Reconstructor(S["A"], S["sens"], "f32", 1, block_rows=cap, layout="tiled").mlem(
