@@ -173,6 +173,14 @@ int raptr_sensitivity_f64(const raptr_setup_t* setup, const raptr_grid_t* grid,
  * listed channels (d_list == NULL: all channels of the setup, list_size
  * ignored), rows generated on the fly exactly as raptr_sm_build_csr does
  * (the reference's on-the-fly system matrix, reco.cu:246-259). */
+/* Summed back-projection c = A^T w of the listed channels with weights
+ * d_weights (the measurement), FP64, matrix-free: the quantity the
+ * reference's backprojection driver accumulates chunk by chunk with
+ * CSRmv(TRANSPOSE, beta = 1) (tests/backprojection.cu:144-158) before it
+ * normalises by the sum.  d_out: n voxels, overwritten. */
+int raptr_backproject_f64(const raptr_setup_t* setup, const raptr_grid_t* grid,
+                          const int* d_list, const float* d_weights, int list_size, int nrays,
+                          uint32_t seed, int raygen, double* d_out, raptr_stream_t stream);
 int raptr_forward_project_f64(const raptr_setup_t* setup, const raptr_grid_t* grid,
                               const int* d_list, int list_size, int nrays,
                               uint32_t seed, int raygen, const float* d_x,

@@ -84,6 +84,7 @@ _SIGS = {
     "raptr_sm_build_csr": (i32, [SP, GP, vp, i32, i32, u32, i32, vp, vp, vp, i64, P(i64), vp,
                                  sz, vp]),
     "raptr_sensitivity_f64": (i32, [SP, GP, i32, u32, i32, vp, vp]),
+    "raptr_backproject_f64": (i32, [SP, GP, vp, vp, i32, i32, u32, i32, vp, vp]),
     "raptr_forward_project_f64": (i32, [SP, GP, vp, i32, i32, u32, i32, vp, vp, vp]),
     "raptr_coo_sort_workspace_size": (sz, [i64]),
     "raptr_coo_sort": (i32, [vp, vp, vp, i64, vp, sz, vp]),

@@ -101,7 +101,8 @@ struct BuildArgs {
   int piece_cap;
   unsigned* piece_count;
   unsigned* flags;  // bit0 staging overflow, bit1 piece overflow
-  double* colsum;   // sensitivity mode
+  double* colsum;   // sensitivity / back-projection mode
+  const float* wgt; // back-projection mode: weight per list entry (nullptr: 1)
   const float* xin; // forward-projection mode: image
   double* fwd;      // forward-projection mode: per-row FP64 projection
 };
@@ -228,9 +229,10 @@ __global__ __launch_bounds__(kSmBlock) void k_sm_build(BuildArgs A) {
       }
       __syncthreads();
       if (MODE == kModeColsum) {
+        const double wt = A.wgt ? double(A.wgt[li]) : 1.0;
         for (int c = t; c < cand_n; c += kSmBlock) {
           float a = cval[c];
-          if (a != 0.0f) atomicAdd(&A.colsum[cand[c]], double(a));
+          if (a != 0.0f && wt != 0.0) atomicAdd(&A.colsum[cand[c]], double(a) * wt);
         }
         __syncthreads();
         cand_n = 0;
@@ -685,11 +687,34 @@ int raptr_sensitivity_f64(const raptr_setup_t* setup, const raptr_grid_t* grid, 
   RAPTR_HIP_TRY(hipMemsetAsync(d_sens, 0, size_t(A.G.nvox) * sizeof(double), st));
   int64_t nch = int64_t(setup->na) * setup->n0z * setup->n0y * setup->n1z * setup->n1y;
   A.list = nullptr;
+  A.wgt = nullptr;
   A.colsum = d_sens;
   const int64_t kRowsPerLaunch = 1 << 20;
   for (int64_t f = 0; f < nch; f += kRowsPerLaunch) {
     A.first = int(f);
     A.rows = int(nch - f < kRowsPerLaunch ? nch - f : kRowsPerLaunch);
+    RAPTR_TRY(launch_build<kModeColsum>(A, lds, st));
+  }
+  return RAPTR_OK;
+}
+
+int raptr_backproject_f64(const raptr_setup_t* setup, const raptr_grid_t* grid,
+                          const int* d_list, const float* d_weights, int list_size, int nrays,
+                          uint32_t seed, int raygen, double* d_out, raptr_stream_t stream) {
+  BuildArgs A;
+  size_t lds = 0;
+  RAPTR_TRY(make_build_args(setup, grid, nrays, seed, raygen, &A, &lds));
+  if (!d_out || list_size < 0 || (list_size > 0 && (!d_list || !d_weights)))
+    return fail(RAPTR_ERR_INVALID_ARG, "bad back-projection arguments");
+  hipStream_t st = S(stream);
+  RAPTR_HIP_TRY(hipMemsetAsync(d_out, 0, size_t(A.G.nvox) * sizeof(double), st));
+  A.list = d_list;
+  A.wgt = d_weights;
+  A.colsum = d_out;
+  const int64_t kRowsPerLaunch = 1 << 20;
+  for (int64_t f = 0; f < list_size; f += kRowsPerLaunch) {
+    A.first = int(f);
+    A.rows = int(list_size - f < kRowsPerLaunch ? list_size - f : kRowsPerLaunch);
     RAPTR_TRY(launch_build<kModeColsum>(A, lds, st));
   }
   return RAPTR_OK;

@@ -170,6 +170,28 @@ def sensitivity(setup, grid, nrays, seed=1, raygen=capi.RAYGEN_RAND, stream=None
     return out
 
 
+def backproject(setup, grid, nrays, channels, counts, seed=1, raygen=capi.RAYGEN_RAND,
+                normalise=True, stream=None):
+    """Summed back-projection A^T y of the measured channels (FP64,
+    matrix-free), divided by its sum when normalise (the reference's
+    backprojection driver, tests/backprojection.cu:144-185)."""
+    ch = np.ascontiguousarray(channels, np.int32)
+    w = np.ascontiguousarray(counts, np.float32)
+    if len(ch) != len(w):
+        raise ValueError("channels and counts differ in length")
+    out = DeviceArray(grid.nvox, np.float64)
+    d_c = DeviceArray.from_numpy(ch if len(ch) else np.zeros(1, np.int32))
+    d_w = DeviceArray.from_numpy(w if len(w) else np.zeros(1, np.float32))
+    s, g = setup.struct(), grid.struct()
+    capi.call("raptr_backproject_f64", C.byref(s), C.byref(g), d_c.ptr, d_w.ptr, len(ch),
+              int(nrays), int(seed), int(raygen), out.ptr, stream)
+    x = out.numpy(stream=stream)
+    if normalise:
+        tot = x.sum()
+        x = x / tot if tot != 0 else x
+    return x
+
+
 def forward_project(setup, grid, x, nrays, cnl_list=None, seed=1, raygen=capi.RAYGEN_RAND,
                     stream=None):
     """FP64 projection of image x onto the listed channels (all if None),

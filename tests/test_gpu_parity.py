@@ -93,6 +93,24 @@ def test_rays_match_oracle(gpu, oracle):
     np.testing.assert_array_equal(r[3, 1], oracle.ray(s, 20479, 1, raygen=oracle.RAYGEN_REGU))
 
 
+def test_backprojection_matches_oracle(gpu, oracle):
+    """Matrix-free summed back-projection A^T y against the oracle's FP64
+    transposed product on the oracle's FP32 matrix values."""
+    from raptr_amd import sysmat
+    setup, grid, nrays = G.config_small()
+    s, g = oracle_geom(oracle, setup, grid)
+    rng = np.random.default_rng(5)
+    chans = np.sort(rng.choice(setup.nchannels, 3000, replace=False)).astype(np.int32)
+    y = rng.poisson(7.0, len(chans)).astype(np.float32)
+    rp, col, val = oracle.build_csr(s, g, chans, nrays)[:3]
+    want = oracle.csrmv_f64(1, len(chans), grid.nvox, 1.0, val.astype(np.float64), rp, col,
+                            y.astype(np.float64), 0.0, np.zeros(grid.nvox))
+    got = sysmat.backproject(setup, grid, nrays, chans, y, normalise=False)
+    np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-9)
+    xn = sysmat.backproject(setup, grid, nrays, chans, y)
+    assert abs(xn.sum() - 1.0) < 1e-12
+
+
 # ------------------------------------------------------------ Siddon chords
 @pytest.mark.parametrize("cfg", ["tiny", "small"])
 def test_get_chords_bit_exact(gpu, oracle, cfg):
