@@ -289,6 +289,12 @@ def main():
         x64 = O.mlem_f64(rp, col, v64, y.astype(np.float64), sens.numpy(), x0.astype(np.float64),
                          niter)
         cpu_s = time.perf_counter() - t
+        # one host thread on a bounded sample: 3 iterations, scaled to niter
+        O.lib().ro_set_threads(1)
+        t = time.perf_counter()
+        O.mlem_f64(rp, col, v64, y.astype(np.float64), sens.numpy(), x0.astype(np.float64), 3)
+        cpu1_s = (time.perf_counter() - t) * niter / 3.0
+        O.lib().ro_set_threads(threads)
         del rp, col, v64
         for name in modes:
             modes[name]["rel_l2_vs_fp64"] = rel_l2(modes[name]["image0"], x64)
@@ -300,7 +306,10 @@ def main():
                "sample": f"one full frame ({niter} MLEM iterations) of the same workload by the "
                          f"FP64 C oracle (OpenMP) on the same {A.nnz}-entry matrix: "
                          f"{cpu_s:.1f} s"}
-        log(f"cpu baseline: {cpu_s:.1f} s/frame on {threads} threads")
+        cpu["single_thread"] = {"value": round(1.0 / cpu1_s, 6), "unit": "frames/s", "cores": 1,
+                                "sample": f"3 iterations on one thread, scaled to {niter}"}
+        log(f"cpu baseline: {cpu_s:.1f} s/frame on {threads} threads, {cpu1_s:.1f} s/frame "
+            "on one thread")
 
     for name in modes:
         modes[name].pop("image0")
@@ -313,7 +322,9 @@ def main():
     # ------------------------------------------------ batched small workload
     small = None
     if not args.no_small:
-        small = bench_small(log, timed, stream, st)
+        small = bench_small(log, timed, stream, st,
+                            0 if args.no_cpu_baseline else
+                            int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
 
     # ------------------------------------------------------------ serving
     serve = None
@@ -420,7 +431,7 @@ def bench_serving(log, nreq, rate):
     return out
 
 
-def bench_small(log, timed, stream, st):
+def bench_small(log, timed, stream, st, cpu_threads=0):
     """64 dynamic frames on the small geometry (shared matrix of the union of
     measured channels), 30 iterations, FP32, one batch, vs one frame at a time."""
     from raptr_amd import geometry as G
@@ -475,7 +486,28 @@ def bench_small(log, timed, stream, st):
         out[f"{name}_frames_per_s"] = round(F * 1000.0 / ms, 1)
         out[f"{name}_ms_for_64_frames"] = round(ms, 2)
         log(f"small {name}: {ms:.1f} ms for 64 frames")
+        if nf > 1:
+            Xb = dx.numpy().reshape(n, F)
     out["speedup"] = round(out["batched_frames_per_s"] / out["plain_frames_per_s"], 2)
+    if cpu_threads:
+        # FP64 oracle on the same matrix: frames 0 and 63 (error of the batched
+        # images) and the host time of those two frames, scaled to 64
+        from oracle import pyoracle as O
+        rp, col, val = A.to_host()
+        v64 = val.astype(np.float64)
+        O.lib().ro_set_threads(cpu_threads)
+        t = time.perf_counter()
+        errs = []
+        for f in (0, F - 1):
+            x64 = O.mlem_f64(rp, col, v64, Y[:, f].astype(np.float64), sens.numpy(),
+                             np.full(n, 1.0 / n), niter)
+            errs.append(float(np.linalg.norm(Xb[:, f] - x64) / np.linalg.norm(x64)))
+        cpu_s = (time.perf_counter() - t) * F / 2
+        out["batched_rel_l2_vs_fp64"] = max(errs)
+        out["cpu_baseline"] = {"value": round(F / cpu_s, 2), "unit": "frames/s",
+                               "cores": cpu_threads, "kind": "port",
+                               "sample": "frames 0 and 63 by the FP64 C oracle, scaled to 64"}
+        log(f"small: batched rel L2 vs FP64 {max(errs):.2e}; cpu {F / cpu_s:.1f} frames/s")
     return out
 
 
