@@ -263,10 +263,12 @@ def main():
     elif info["bwd_staged"]:
         # one launch; the ratio vector read once per workgroup from L2 (counted
         # once), column sums in LDS, x/s read and x' written once
-        bytes_b = 8 * A.nnz + 8 * nb * (n + 1) + 4 * m + 12 * n + 8 * info["bwd_workgroups"]
+        bytes_b = (8 * info["transpose_nnz"] + 8 * nb * (n + 1) + 4 * m + 12 * n
+                   + 8 * info["bwd_workgroups"])
         bwd_name = f"k_bwd_lds (backward + update, one launch over {nb} LDS row slices)"
     else:
-        bytes_b = 8 * A.nnz + 8 * nb * (n + 1) + 4 * m + 4 * n * (2 * nb + 2) + 8 * (n // 4)
+        bytes_b = (8 * info["transpose_nnz"] + 8 * nb * (n + 1) + 4 * m
+                   + 4 * n * (2 * nb + 2) + 8 * (n // 4))
         bwd_name = f"k_bwd (backward + update, pass of {nb} row-block launches)"
     gbs_f = bytes_f / (ms_f * 1e-3) / 1e9
     gbs_b = bytes_b / (ms_b * 1e-3) / 1e9

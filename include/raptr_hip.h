@@ -335,9 +335,10 @@ int raptr_recon_destroy(raptr_recon* plan);
 int raptr_recon_info(const raptr_recon* plan, int64_t* device_bytes,
                      int64_t* padded_nnz, int* row_blocks, int* fwd_lanes,
                      int* fwd_unroll, int* bwd_unroll);
-/* Kernel shape: lanes per forward row (16, 32, 64; single frame), forward
- * loads in flight per lane (single frame: 1 or 2 groups of 4 entries;
- * batched: 2, 4 or 8 entries) and backward loads per lane (2, 4, 8). */
+/* Kernel shape: lanes per forward row (16, 32, 64; single frame), and the
+ * groups of 4 entries in flight per lane or entry slot in the forward and
+ * the backward (1, 2, 4).  The transposed segments of the blocked layout
+ * are padded to multiples of 4 entries (value 0) for 16-byte loads. */
 int raptr_recon_set_config(raptr_recon* plan, int fwd_lanes, int fwd_unroll,
                            int bwd_unroll);
 /* Single-frame backward pass staged through LDS (staged != 0; needs
@@ -355,6 +356,8 @@ int raptr_recon_get_backward(const raptr_recon* plan, int* staged, int* lanes_pe
  * registers (longer waves, fewer launches). */
 int raptr_recon_set_blocks_per_launch(raptr_recon* plan, int blocks);
 int raptr_recon_get_blocks_per_launch(const raptr_recon* plan);
+/* Entries of the padded row-blocked transpose (0 for the tiled layout). */
+int64_t raptr_recon_transpose_nnz(const raptr_recon* plan);
 /* niter MLEM iterations (reco.cu:230-303) of nframes frames at once
  * (power of two <= max_frames).  d_y: counts [m][nframes]; d_x: images
  * [n][nframes], start images in, results out.  Per frame and iteration:

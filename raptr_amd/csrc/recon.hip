@@ -87,20 +87,56 @@ __global__ void k_expand_rows32(const int64_t* rp, int m, uint32_t* rowid) {
     for (int64_t k = rp[i] + lane; k < rp[i + 1]; k += kWave) rowid[k] = uint32_t(i);
 }
 
-__global__ void k_bt_gather(const uint32_t* idx, int64_t nnz, const uint32_t* rowid,
-                            const float* val, const float* scale, int vt, int* brow,
-                            void* bval) {
+// padded segment lengths of the blocked transpose: segment (b, j) of
+// column j in row block b padded to a multiple of 4 entries; the (b, n)
+// slot is empty so an exclusive scan gives every block's start
+__global__ void k_bt_padlen(const int64_t* uptr, int64_t total, int n, int64_t* plen) {
+  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < total;
+       t += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t j = t % (n + 1);
+    plen[t] = j < n ? ((uptr[t + 1] - uptr[t] + 3) & ~int64_t(3)) : 0;
+  }
+}
+
+// sorted entry j -> its place in the padded segment of its (block, column)
+__global__ void k_bt_scatter(const uint64_t* keys, const uint32_t* idx, int64_t nnz, int n,
+                             int colbits, const int64_t* uptr, const int64_t* pptr,
+                             const uint32_t* rowid, const float* val, const float* scale,
+                             int vt, int* brow, void* bval) {
+  const uint64_t cmask = (uint64_t(1) << colbits) - 1;
   for (int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; j < nnz;
        j += int64_t(gridDim.x) * blockDim.x) {
-    uint32_t k = idx[j];
-    uint32_t r = rowid[k];
-    brow[j] = int(r);
-    float v = val[k];
+    const uint64_t key = keys[j];
+    const int64_t t = int64_t(key >> colbits) * (n + 1) + int64_t(key & cmask);
+    const int64_t d = pptr[t] + (j - uptr[t]);
+    const uint32_t k = idx[j];
+    const uint32_t r = rowid[k];
+    brow[d] = int(r);
+    const float v = val[k];
     if (vt == kF32)
-      static_cast<float*>(bval)[j] = v;
+      static_cast<float*>(bval)[d] = v;
     else
-      static_cast<uint16_t*>(bval)[j] =
+      static_cast<uint16_t*>(bval)[d] =
           f2h(vt == kU16 ? v / (scale[r] * 65535.0f) : v / scale[r], vt);
+  }
+}
+
+// padding entries: value 0, row = the block's first row (inside the block)
+__global__ void k_bt_pad(const int64_t* uptr, const int64_t* pptr, int64_t total, int n,
+                         int blk_rows, int* brow, void* bval, int vt) {
+  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < total;
+       t += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t b = t / (n + 1), j = t - b * (n + 1);
+    if (j >= n) continue;
+    const int64_t len = uptr[t + 1] - uptr[t];
+    const int64_t plen = (len + 3) & ~int64_t(3);
+    for (int64_t p = pptr[t] + len; p < pptr[t] + plen; ++p) {
+      brow[p] = int(b * blk_rows);
+      if (vt == kF32)
+        static_cast<float*>(bval)[p] = 0.0f;
+      else
+        static_cast<uint16_t*>(bval)[p] = 0;
+    }
   }
 }
 
@@ -224,8 +260,12 @@ __device__ __forceinline__ int64_t xcd_block(int64_t bx, int64_t G) {
   return xcd * q + (xcd < r8 ? xcd : r8) + bx / 8;
 }
 
-// F frames forward + ratio: one wave per row, each lane keeps U entries
-// (value, index, gathered frame vector) in flight per step.
+// F frames forward + ratio: one wave per row.  Each entry slot takes 4
+// consecutive entries per step (one 16-byte value load and one 16-byte
+// index load for the 4, rows are padded to multiples of 4), then gathers
+// the 4 entries' frame vectors; U groups of 4 are in flight per step.  This
+// halves the vector-memory instructions per entry against one entry per
+// slot, and those instructions (not bytes) bound the batched passes.
 template <int VT, int FR, int U>
 __global__ __launch_bounds__(256) void k_fwdF(int64_t m, const int64_t* __restrict__ rp,
                                               const int* __restrict__ col,
@@ -240,35 +280,49 @@ __global__ __launch_bounds__(256) void k_fwdF(int64_t m, const int64_t* __restri
   const int q = lane % Q, slot = lane / Q;
   const int64_t i = xcd_block(blockIdx.x, gridDim.x) * (256 / kWave) + threadIdx.x / kWave;
   if (i >= m) return;
-  const int64_t b = rp[i], en = rp[i + 1];
+  const int64_t b = rp[i], en = rp[i + 1];  // multiples of 4
   float acc[V];
 #pragma unroll
   for (int v = 0; v < V; ++v) acc[v] = 0.0f;
   const float* xq = x + V * q;
-  int64_t k = b + slot;
-  for (; k + int64_t(SLOTS) * (U - 1) < en; k += int64_t(SLOTS) * U) {
-    float w[U];
-    int c[U];
+  constexpr int64_t STEP = 4 * SLOTS;
+  int64_t k = b + 4 * slot;
+  for (; k + STEP * (U - 1) < en; k += STEP * U) {
+    float w[U][4];
+    vi4 c[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      w[u] = Val<VT>::get(val, k + u * SLOTS);
-      c[u] = __builtin_nontemporal_load(col + k + u * SLOTS);
+      load4<VT>(val, k + STEP * u, w[u][0], w[u][1], w[u][2], w[u][3]);
+      c[u] = __builtin_nontemporal_load(reinterpret_cast<const vi4*>(col + k + STEP * u));
     }
-    float g[U][V];
+    float g[U][4][V];
 #pragma unroll
-    for (int u = 0; u < U; ++u) load_frames<V>(xq + int64_t(c[u]) * FR, g[u]);
+    for (int u = 0; u < U; ++u) {
+      load_frames<V>(xq + int64_t(c[u].x) * FR, g[u][0]);
+      load_frames<V>(xq + int64_t(c[u].y) * FR, g[u][1]);
+      load_frames<V>(xq + int64_t(c[u].z) * FR, g[u][2]);
+      load_frames<V>(xq + int64_t(c[u].w) * FR, g[u][3]);
+    }
 #pragma unroll
     for (int u = 0; u < U; ++u)
 #pragma unroll
-      for (int v = 0; v < V; ++v) acc[v] = fmaf(w[u], g[u][v], acc[v]);
-  }
-  for (; k < en; k += SLOTS) {
-    float w = Val<VT>::get(val, k);
-    int c = __builtin_nontemporal_load(col + k);
-    float g[V];
-    load_frames<V>(xq + int64_t(c) * FR, g);
+      for (int j = 0; j < 4; ++j)
 #pragma unroll
-    for (int v = 0; v < V; ++v) acc[v] = fmaf(w, g[v], acc[v]);
+        for (int v = 0; v < V; ++v) acc[v] = fmaf(w[u][j], g[u][j][v], acc[v]);
+  }
+  for (; k < en; k += STEP) {
+    float w[4];
+    load4<VT>(val, k, w[0], w[1], w[2], w[3]);
+    const vi4 c = __builtin_nontemporal_load(reinterpret_cast<const vi4*>(col + k));
+    float g[4][V];
+    load_frames<V>(xq + int64_t(c.x) * FR, g[0]);
+    load_frames<V>(xq + int64_t(c.y) * FR, g[1]);
+    load_frames<V>(xq + int64_t(c.z) * FR, g[2]);
+    load_frames<V>(xq + int64_t(c.w) * FR, g[3]);
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int v = 0; v < V; ++v) acc[v] = fmaf(w[j], g[j][v], acc[v]);
   }
 #pragma unroll
   for (int off = kWave / 2; off >= Q; off >>= 1)
@@ -288,10 +342,12 @@ __global__ __launch_bounds__(256) void k_fwdF(int64_t m, const int64_t* __restri
   }
 }
 
-// Backward over one row block: c[j][f] (+)= sum over the block's entries of
-// column j of v * e'[row][f]; the last block applies the update
-// xx = x * c / (s + (s==0)) and writes FP64 partial sums per block/frame.
-// One wave per column, lanes as in the forward (frame vectors of V).
+// Backward over nbl consecutive row blocks: c[j][f] (+)= sum over those
+// blocks' entries of column j of v * e'[row][f]; the launch that ends with
+// the last block applies the update xx = x * c / (s + (s==0)) and writes
+// FP64 partial sums per block/frame.  One wave per column, lanes as in the
+// forward; column segments are padded to multiples of 4 (value 0, row in
+// the block) and read 4 entries per slot and step.
 template <int VT, int FR, int UNROLL>
 __global__ __launch_bounds__(256) void k_bwd(int n, const int64_t* __restrict__ ptr, int nbl,
                                              const int* __restrict__ brow,
@@ -319,31 +375,46 @@ __global__ __launch_bounds__(256) void k_bwd(int n, const int64_t* __restrict__ 
     pe = ptr[int64_t(lane) * (n + 1) + j + 1];
   }
   const float* eq = e + V * q;
+  constexpr int64_t STEP = 4 * SLOTS;
   for (int bb = 0; bb < nbl && j < n; ++bb) {
     const int64_t b = __shfl(pb, bb, kWave), en = __shfl(pe, bb, kWave);
-    int64_t k = b + slot;
-    for (; k + int64_t(UNROLL - 1) * SLOTS < en; k += int64_t(UNROLL) * SLOTS) {
-      float wv[UNROLL];
-      int rr[UNROLL];
+    int64_t k = b + 4 * slot;
+    for (; k + STEP * (UNROLL - 1) < en; k += STEP * UNROLL) {
+      float wv[UNROLL][4];
+      vi4 rr[UNROLL];
 #pragma unroll
       for (int u = 0; u < UNROLL; ++u) {
-        wv[u] = Val<VT>::get(val, k + u * SLOTS);
-        rr[u] = __builtin_nontemporal_load(brow + k + u * SLOTS);
+        load4<VT>(val, k + STEP * u, wv[u][0], wv[u][1], wv[u][2], wv[u][3]);
+        rr[u] = __builtin_nontemporal_load(reinterpret_cast<const vi4*>(brow + k + STEP * u));
       }
-      float g[UNROLL][V];
+      float g[UNROLL][4][V];
 #pragma unroll
-      for (int u = 0; u < UNROLL; ++u) load_frames<V>(eq + int64_t(rr[u]) * FR, g[u]);
+      for (int u = 0; u < UNROLL; ++u) {
+        load_frames<V>(eq + int64_t(rr[u].x) * FR, g[u][0]);
+        load_frames<V>(eq + int64_t(rr[u].y) * FR, g[u][1]);
+        load_frames<V>(eq + int64_t(rr[u].z) * FR, g[u][2]);
+        load_frames<V>(eq + int64_t(rr[u].w) * FR, g[u][3]);
+      }
 #pragma unroll
       for (int u = 0; u < UNROLL; ++u)
 #pragma unroll
-        for (int v = 0; v < V; ++v) acc[v] = fmaf(wv[u], g[u][v], acc[v]);
-    }
-    for (; k < en; k += SLOTS) {
-      float wv = Val<VT>::get(val, k);
-      float g[V];
-      load_frames<V>(eq + int64_t(brow[k]) * FR, g);
+        for (int jj = 0; jj < 4; ++jj)
 #pragma unroll
-      for (int v = 0; v < V; ++v) acc[v] = fmaf(wv, g[v], acc[v]);
+          for (int v = 0; v < V; ++v) acc[v] = fmaf(wv[u][jj], g[u][jj][v], acc[v]);
+    }
+    for (; k < en; k += STEP) {
+      float wv[4];
+      load4<VT>(val, k, wv[0], wv[1], wv[2], wv[3]);
+      const vi4 rr = __builtin_nontemporal_load(reinterpret_cast<const vi4*>(brow + k));
+      float g[4][V];
+      load_frames<V>(eq + int64_t(rr.x) * FR, g[0]);
+      load_frames<V>(eq + int64_t(rr.y) * FR, g[1]);
+      load_frames<V>(eq + int64_t(rr.z) * FR, g[2]);
+      load_frames<V>(eq + int64_t(rr.w) * FR, g[3]);
+#pragma unroll
+      for (int jj = 0; jj < 4; ++jj)
+#pragma unroll
+        for (int v = 0; v < V; ++v) acc[v] = fmaf(wv[jj], g[jj][v], acc[v]);
     }
   }
   if (j < n) {
@@ -552,7 +623,8 @@ int launch_fwd_any(const raptr_recon* R, int F, const float* x, const float* y, 
   }
 #define RAPTR_F1V(VV)                                                                        \
   RAPTR_F1(VV, 16, 1) RAPTR_F1(VV, 32, 1) RAPTR_F1(VV, 64, 1) RAPTR_F1(VV, 16, 2)            \
-  RAPTR_F1(VV, 32, 2) RAPTR_F1(VV, 64, 2)
+  RAPTR_F1(VV, 32, 2) RAPTR_F1(VV, 64, 2) RAPTR_F1(VV, 16, 4) RAPTR_F1(VV, 32, 4)            \
+  RAPTR_F1(VV, 64, 4)
     RAPTR_F1V(kF32) RAPTR_F1V(kF16) RAPTR_F1V(kBF16) RAPTR_F1V(kU16)
 #undef RAPTR_F1V
 #undef RAPTR_F1
@@ -566,7 +638,7 @@ int launch_fwd_any(const raptr_recon* R, int F, const float* x, const float* y, 
     RAPTR_LAUNCH_CHECK();                                                                  \
     return RAPTR_OK;                                                                       \
   }
-#define RAPTR_FFU(VV, FF) RAPTR_FF(VV, FF, 2) RAPTR_FF(VV, FF, 4) RAPTR_FF(VV, FF, 8)
+#define RAPTR_FFU(VV, FF) RAPTR_FF(VV, FF, 1) RAPTR_FF(VV, FF, 2) RAPTR_FF(VV, FF, 4)
 #define RAPTR_FFV(VV)                                                                      \
   RAPTR_FFU(VV, 2) RAPTR_FFU(VV, 4) RAPTR_FFU(VV, 8) RAPTR_FFU(VV, 16) RAPTR_FFU(VV, 32)    \
   RAPTR_FFU(VV, 64)
@@ -594,7 +666,7 @@ int launch_bwd_any(const raptr_recon* R, int F, int blk, int nbl, const float* x
     RAPTR_LAUNCH_CHECK();                                                                   \
     return RAPTR_OK;                                                                        \
   }
-#define RAPTR_BU(VV, FF) RAPTR_B(VV, FF, 2) RAPTR_B(VV, FF, 4) RAPTR_B(VV, FF, 8)
+#define RAPTR_BU(VV, FF) RAPTR_B(VV, FF, 1) RAPTR_B(VV, FF, 2) RAPTR_B(VV, FF, 4)
   RAPTR_BU(kF32, 1) RAPTR_BU(kF32, 2) RAPTR_BU(kF32, 4) RAPTR_BU(kF32, 8) RAPTR_BU(kF32, 16)
   RAPTR_BU(kF32, 32) RAPTR_BU(kF32, 64)
   RAPTR_BU(kF16, 1) RAPTR_BU(kF16, 2) RAPTR_BU(kF16, 4) RAPTR_BU(kF16, 8) RAPTR_BU(kF16, 16)
@@ -799,28 +871,23 @@ int raptr_recon_create_ex(raptr_recon** out, int m, int n, int64_t nnz, const in
       hipError_t le = hipGetLastError();
       if (le != hipSuccess) return bail(fail(RAPTR_ERR_DEVICE, "pad: %s", hipGetErrorString(le)));
     }
-    // ---- backward: row-blocked transpose via a stable radix sort
-    if ((rc = ralloc(R, &R->bptr, size_t(R->nblk) * (size_t(n) + 1)))) return bail(rc);
-    if ((rc = ralloc(R, &R->brow, size_t(nnz) + 1))) return bail(rc);
-    if (value_type == kF32) {
-      float* p;
-      if ((rc = ralloc(R, &p, size_t(nnz) + 1))) return bail(rc);
-      R->bval = p;
-    } else {
-      uint16_t* p;
-      if ((rc = ralloc(R, &p, size_t(nnz) + 1))) return bail(rc);
-      R->bval = p;
-    }
+    // ---- backward: row-blocked transpose via a stable radix sort, every
+    // (block, column) segment padded to a multiple of 4 entries
+    const int64_t T = int64_t(R->nblk) * (n + 1);
+    if ((rc = ralloc(R, &R->bptr, size_t(T) + 1))) return bail(rc);
     if (nnz > 0) {
       int colbits = 1;
       while ((int64_t(1) << colbits) <= n) ++colbits;
       int blkbits = 0;
       while ((int64_t(1) << blkbits) < R->nblk) ++blkbits;
-      size_t sort_ws = raptr_csr_transpose_workspace_size(nnz, m, n);  // >= radix buffers
+      const size_t sort_ws = raptr_csr_transpose_workspace_size(nnz, m, n);  // radix buffers
+      const size_t sb = scan_workspace_bytes(T);
+      const size_t tb = ((size_t(T) + 1) * 8 + 255) & ~size_t(255);
       void* tmp = nullptr;
-      RAPTR_HIP_TRY(hipMalloc(&tmp, sort_ws));
+      RAPTR_HIP_TRY(hipMalloc(&tmp, sort_ws + 2 * tb + 256 + sb));
       // carve: rowid | key32 (unused) | sort buffers, in the order the
-      // transpose helper lays them out
+      // transpose helper lays them out, then the unpadded pointers, padded
+      // lengths, scan total and scan scratch
       Carver cv(tmp, sort_ws);
       uint32_t* rowid = cv.take<uint32_t>(size_t(nnz));
       (void)cv.take<int>(size_t(nnz));
@@ -829,7 +896,15 @@ int raptr_recon_create_ex(raptr_recon** out, int m, int n, int64_t nnz, const in
       uint64_t* k1 = cv.take<uint64_t>(size_t(nnz));
       uint32_t* v0 = cv.take<uint32_t>(size_t(nnz));
       uint32_t* v1 = cv.take<uint32_t>(size_t(nnz));
+      char* extra = static_cast<char*>(tmp) + sort_ws;
+      int64_t* uptr = reinterpret_cast<int64_t*>(extra);
+      int64_t* plen = reinterpret_cast<int64_t*>(extra + tb);
+      int64_t* tot = reinterpret_cast<int64_t*>(extra + 2 * tb);
+      char* sws = extra + 2 * tb + 256;
       if (!cv.ok) rc = fail(RAPTR_ERR_WORKSPACE, "transpose scratch carve failed");
+      uint64_t* keys = nullptr;
+      uint32_t* idx = nullptr;
+      int64_t h = 0;
       if (!rc) {
         hipLaunchKernelGGL(k_bt_keys, dim3(grid_for(int64_t(m) * kWave, 256, 1 << 16)), dim3(256),
                            0, st, d_rp, d_col, m, block_rows, colbits, k0, v0);
@@ -838,23 +913,55 @@ int raptr_recon_create_ex(raptr_recon** out, int m, int n, int64_t nnz, const in
         int res = 0;
         rc = radix_sort_pairs(k0, k1, v0, v1, nnz, colbits + blkbits, st, &res, cv);
         if (!rc) {
-          uint64_t* keys = res ? k1 : k0;
-          uint32_t* idx = res ? v1 : v0;
-          hipLaunchKernelGGL(k_bt_gather, dim3(grid_for(nnz, 256, 1 << 16)), dim3(256), 0, st,
-                             (const uint32_t*)idx, nnz, (const uint32_t*)rowid, d_val,
-                             (const float*)R->rscale, value_type, R->brow, R->bval);
-          hipLaunchKernelGGL(k_bt_ptr,
-                             dim3(grid_for(int64_t(R->nblk) * (n + 1), 256, 1 << 16)), dim3(256),
-                             0, st, (const uint64_t*)keys, nnz, R->nblk, n, colbits, R->bptr);
-          hipError_t le = hipGetLastError();
-          if (le != hipSuccess) rc = fail(RAPTR_ERR_DEVICE, "transpose: %s", hipGetErrorString(le));
+          keys = res ? k1 : k0;
+          idx = res ? v1 : v0;
+          hipLaunchKernelGGL(k_bt_ptr, dim3(grid_for(T, 256, 1 << 16)), dim3(256), 0, st,
+                             (const uint64_t*)keys, nnz, R->nblk, n, colbits, uptr);
+          hipLaunchKernelGGL(k_bt_padlen, dim3(grid_for(T, 256, 1 << 16)), dim3(256), 0, st,
+                             (const int64_t*)uptr, T, n, plen);
+          rc = scan_exclusive_i64(plen, R->bptr, T, tot, sws, sb, st);
         }
+        if (!rc) {
+          hipLaunchKernelGGL(k_set_last, dim3(1), dim3(1), 0, st, R->bptr, int(T),
+                             (const int64_t*)tot);
+          (void)hipMemcpyAsync(&h, tot, 8, hipMemcpyDeviceToHost, st);
+          hipError_t se = hipStreamSynchronize(st);
+          if (se != hipSuccess) rc = fail(RAPTR_ERR_DEVICE, "transpose: %s", hipGetErrorString(se));
+        }
+      }
+      if (!rc) {
+        R->bnnz = h;
+        if (!(rc = ralloc(R, &R->brow, size_t(h) + 4))) {
+          if (value_type == kF32) {
+            float* p;
+            if (!(rc = ralloc(R, &p, size_t(h) + 4))) R->bval = p;
+          } else {
+            uint16_t* p;
+            if (!(rc = ralloc(R, &p, size_t(h) + 8))) R->bval = p;
+          }
+        }
+      }
+      if (!rc) {
+        hipLaunchKernelGGL(k_bt_scatter, dim3(grid_for(nnz, 256, 1 << 16)), dim3(256), 0, st,
+                           (const uint64_t*)keys, (const uint32_t*)idx, nnz, n, colbits,
+                           (const int64_t*)uptr, (const int64_t*)R->bptr,
+                           (const uint32_t*)rowid, d_val, (const float*)R->rscale, value_type,
+                           R->brow, R->bval);
+        hipLaunchKernelGGL(k_bt_pad, dim3(grid_for(T, 256, 1 << 16)), dim3(256), 0, st,
+                           (const int64_t*)uptr, (const int64_t*)R->bptr, T, n, block_rows,
+                           R->brow, R->bval, value_type);
+        hipError_t le = hipGetLastError();
+        if (le != hipSuccess) rc = fail(RAPTR_ERR_DEVICE, "transpose: %s", hipGetErrorString(le));
       }
       (void)hipStreamSynchronize(st);
       (void)hipFree(tmp);
       if (rc) return bail(rc);
     } else {
-      (void)hipMemsetAsync(R->bptr, 0, sizeof(int64_t) * size_t(R->nblk) * (n + 1), st);
+      (void)hipMemsetAsync(R->bptr, 0, sizeof(int64_t) * (size_t(T) + 1), st);
+      float* p;
+      if ((rc = ralloc(R, &R->brow, 4))) return bail(rc);
+      if ((rc = ralloc(R, &p, 4))) return bail(rc);
+      R->bval = p;
     }
   }
   // ---- sensitivity and work vectors
@@ -872,7 +979,7 @@ int raptr_recon_create_ex(raptr_recon** out, int m, int n, int64_t nnz, const in
   hipError_t se = hipStreamSynchronize(st);
   if (se != hipSuccess) return bail(fail(RAPTR_ERR_DEVICE, "create: %s", hipGetErrorString(se)));
   R->fwd_lanes = double(nnz) / std::max(m, 1) >= 192 ? 32 : 16;
-  R->fwd_unroll = max_frames == 1 ? 1 : 4;
+  R->fwd_unroll = max_frames == 1 ? 1 : 2;
   R->bwd_lds = (layout == RAPTR_LAYOUT_BLOCKED && max_frames == 1 && block_rows <= kLdsRowsMax &&
                 block_rows % 4 == 0) ? 1 : 0;
   *out = R;
@@ -906,13 +1013,11 @@ int raptr_recon_set_config(raptr_recon* R, int fwd_lanes, int fwd_unroll, int bw
   if (!R) return fail(RAPTR_ERR_INVALID_ARG, "null plan");
   if (fwd_lanes != 16 && fwd_lanes != 32 && fwd_lanes != 64)
     return fail(RAPTR_ERR_INVALID_ARG, "forward lanes must be 16, 32 or 64");
-  bool fu_ok = R->fmax == 1 ? (fwd_unroll == 1 || fwd_unroll == 2)
-                            : (fwd_unroll == 2 || fwd_unroll == 4 || fwd_unroll == 8);
-  if (!fu_ok)
-    return fail(RAPTR_ERR_INVALID_ARG,
-                "forward unroll must be 1 or 2 (one frame) or 2, 4, 8 (batched)");
-  if (bwd_unroll != 2 && bwd_unroll != 4 && bwd_unroll != 8)
-    return fail(RAPTR_ERR_INVALID_ARG, "backward unroll must be 2, 4 or 8");
+  // unroll counts groups of 4 entries in flight per lane (entry slot)
+  if (fwd_unroll != 1 && fwd_unroll != 2 && fwd_unroll != 4)
+    return fail(RAPTR_ERR_INVALID_ARG, "forward unroll must be 1, 2 or 4");
+  if (bwd_unroll != 1 && bwd_unroll != 2 && bwd_unroll != 4)
+    return fail(RAPTR_ERR_INVALID_ARG, "backward unroll must be 1, 2 or 4");
   if (R->fwd_lanes != fwd_lanes || R->fwd_unroll != fwd_unroll ||
       R->bwd_unroll != bwd_unroll) {
     for (auto& kv : R->graphs) {
@@ -968,6 +1073,8 @@ int raptr_recon_set_blocks_per_launch(raptr_recon* R, int blocks) {
 }
 
 int raptr_recon_get_blocks_per_launch(const raptr_recon* R) { return R ? R->bwd_bpl : -1; }
+
+int64_t raptr_recon_transpose_nnz(const raptr_recon* R) { return R ? R->bnnz : -1; }
 
 int raptr_recon_set_tiled(raptr_recon* R, int fwd_lanes, int fwd_unroll, int bwd_lanes,
                           int bwd_unroll) {

@@ -92,6 +92,7 @@ struct raptr_recon {
   int64_t* bptr = nullptr;  // nblk * (n+1)
   int* brow = nullptr;
   void* bval = nullptr;
+  int64_t bnnz = 0;  // padded entries of the transpose
   // sensitivity and work vectors (sized for fmax frames)
   float* s = nullptr;
   float* e = nullptr;
@@ -101,8 +102,8 @@ struct raptr_recon {
   float* scale = nullptr;
   double* norm = nullptr;
   int fwd_lanes = 64;
-  int fwd_unroll = 1;  // vec4 groups (1 frame) or entries (F frames) in flight
-  int bwd_unroll = 4;
+  int fwd_unroll = 1;  // groups of 4 entries in flight per lane / entry slot
+  int bwd_unroll = 2;
   int bwd_lds = 0;     // single-frame backward staged through LDS (0: off)
   int bwd_lpc = 16;    // lanes per column of the LDS-staged backward
   int bwd_wgs = 512;   // workgroups of the LDS-staged backward

@@ -48,7 +48,8 @@ class Reconstructor:
                "row_blocks": nb.value, "fwd_lanes": fl.value, "fwd_unroll": fu.value,
                "bwd_unroll": bu.value, "bwd_staged": st.value,
                "bwd_lanes_per_column": lpc.value, "bwd_workgroups": wgs.value,
-               "bwd_blocks_per_launch": capi.lib().raptr_recon_get_blocks_per_launch(self.handle)}
+               "bwd_blocks_per_launch": capi.lib().raptr_recon_get_blocks_per_launch(self.handle),
+               "transpose_nnz": capi.lib().raptr_recon_transpose_nnz(self.handle)}
         if self.layout == "tiled":
             t = [C.c_int() for _ in range(6)]
             capi.call("raptr_recon_get_tiled", self.handle, *[C.byref(v) for v in t])
@@ -134,9 +135,9 @@ class Reconstructor:
                      "bwd_unroll": bu_}, best[1])
         bu = inf["bwd_unroll"]
         if nframes == 1:
-            fshapes = [(fl, fu) for fl in (16, 32, 64) for fu in (1, 2)]
+            fshapes = [(fl, fu) for fl in (16, 32, 64) for fu in (1, 2, 4)]
         else:
-            fshapes = [(inf["fwd_lanes"], fu) for fu in (2, 4, 8)]
+            fshapes = [(inf["fwd_lanes"], fu) for fu in (1, 2, 4)]
         best_f = None
         for fl, fu in fshapes:
             self.set_config(fl, fu, bu)
@@ -144,7 +145,7 @@ class Reconstructor:
             if best_f is None or ms < best_f[1]:
                 best_f = ((fl, fu), ms)
         fl, fu = best_f[0]
-        bshapes = [(0, 16, 512, u, bpl) for u in (2, 4, 8) for bpl in (1, 4, 16)
+        bshapes = [(0, 16, 512, u, bpl) for u in (1, 2, 4) for bpl in (1, 4, 16)
                    if bpl == 1 or bpl < inf["row_blocks"]]
         staged_ok = nframes == 1 and self.max_frames == 1
         if staged_ok:
