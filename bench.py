@@ -39,6 +39,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 ERR_BAR = 1e-4
+ERR_ITERS = 30  # iterations of the mixed-precision error figures
 
 
 def log(*a):
@@ -51,7 +52,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=2, help="timed batches per mode")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--iters", type=int, default=50, help="MLEM iterations per frame")
-    ap.add_argument("--frames", type=int, default=16, help="frames per batch")
+    ap.add_argument("--frames", type=int, default=64, help="frames per batch")
     ap.add_argument("--no-cpu-baseline", action="store_true",
                     help="skip the FP64 host recurrence (no error figures either)")
     ap.add_argument("--no-small", action="store_true")
@@ -61,7 +62,7 @@ def parse():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serving-requests", type=int, default=120)
     ap.add_argument("--serving-rate", type=float, default=20.0, help="arrivals per second")
-    ap.add_argument("--block-slices-mb", default="2,8,32",
+    ap.add_argument("--block-slices-mb", default="8,32",
                     help="ratio-slice sizes (MB) of the batched plans' row blocks to try")
     return ap.parse_args()
 
@@ -152,26 +153,29 @@ def main():
         dx = DeviceArray.from_numpy(xx0.reshape(-1))
         # row blocks of the backward: for F frames try several ratio-slice
         # sizes (the first batched mode picks, the others reuse its choice)
+        # one frame: both plan layouts (column tiles, row blocks) are tried
         if nf == 1:
-            rows_try = [0]
+            rows_try = [("tiled", 0), ("blocked", 0)]
         elif best_rows is not None:
-            rows_try = [best_rows]
+            rows_try = [("blocked", best_rows)]
         else:
-            rows_try = [max(4, int(mb * 2**20 / (4 * nf)) // 4 * 4) for mb in slices_mb]
+            rows_try = [("blocked", max(4, int(mb * 2**20 / (4 * nf)) // 4 * 4))
+                        for mb in slices_mb]
         R, t_plan, cfg, ms_it = None, 0.0, None, None
-        for rows in rows_try:
+        for layout, rows in rows_try:
             t = time.perf_counter()
-            Rc = Reconstructor(A, sens, vt, nf, block_rows=rows, stream=st)
+            Rc = Reconstructor(A, sens, vt, nf, block_rows=rows, stream=st, layout=layout)
             tp = time.perf_counter() - t
             cc, mc = Rc.autotune(dy, dx, nf, reps=2, stream=st)
             if len(rows_try) > 1:
-                log(f"{name}: block_rows {rows} ({Rc.info()['row_blocks']} blocks) -> "
-                    f"{mc:.2f} ms/iteration")
+                log(f"{name}: {layout} layout, block_rows {rows} "
+                    f"({Rc.info()['row_blocks']} blocks) -> {mc:.2f} ms/iteration")
             if ms_it is None or mc < ms_it:
                 if R is not None:
                     R.close()
                 R, t_plan, cfg, ms_it = Rc, tp, cc, mc
                 chosen_rows = rows
+                cfg = dict(cfg, layout=layout)
             else:
                 Rc.close()
         if nf > 1:
@@ -189,13 +193,17 @@ def main():
         ms = timed(lambda: [batch() for _ in range(args.steps)]) / args.steps
         xr = dx.numpy().reshape(n, nf)
         fps = nf * 1000.0 / ms
+        # the mixed-precision error figure is quoted at 30 iterations
+        dx.copy_from(xx0, stream=st)
+        R.run(dy, dx, ERR_ITERS, nf, use_graph=True, stream=st)
+        x30 = dx.numpy(stream=st).reshape(n, nf)[:, 0].copy()
         info = R.info()
         modes[name] = {"value_type": vt, "frames_per_batch": nf, "ms_per_batch": round(ms, 3),
                        "frames_per_s": round(fps, 3), "ms_per_iteration": round(ms / niter, 4),
                        "config": dict(cfg, row_blocks=info["row_blocks"]),
                        "plan_build_s": round(t_plan, 2),
                        "plan_device_GB": round(info["device_bytes"] / 1e9, 2),
-                       "image0": xr[:, 0].copy()}
+                       "image0": xr[:, 0].copy(), "image0_30": x30}
         de_m = DeviceArray(m * nf, np.float32)
 
         def pf():
@@ -223,6 +231,7 @@ def main():
         # a partial run (profiling a subset of modes): per-mode figures only
         for name in modes:
             modes[name].pop("image0")
+            modes[name].pop("image0_30")
         print(json.dumps({"partial": True, "modes": modes, "device": dev}), flush=True)
         return
 
@@ -288,8 +297,10 @@ def main():
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         O.lib().ro_set_threads(threads)
         t = time.perf_counter()
-        x64 = O.mlem_f64(rp, col, v64, y.astype(np.float64), sens.numpy(), x0.astype(np.float64),
-                         niter)
+        x64_30 = O.mlem_f64(rp, col, v64, y.astype(np.float64), sens.numpy(),
+                            x0.astype(np.float64), ERR_ITERS)
+        x64 = O.mlem_f64(rp, col, v64, y.astype(np.float64), sens.numpy(), x64_30,
+                         niter - ERR_ITERS)
         cpu_s = time.perf_counter() - t
         # one host thread on a bounded sample: 3 iterations, scaled to niter
         O.lib().ro_set_threads(1)
@@ -300,6 +311,9 @@ def main():
         del rp, col, v64
         for name in modes:
             modes[name]["rel_l2_vs_fp64"] = rel_l2(modes[name]["image0"], x64)
+            modes[name][f"rel_l2_vs_fp64_{ERR_ITERS}it"] = rel_l2(modes[name]["image0_30"], x64_30)
+            modes[name][f"max_voxel_err_vs_fp64_{ERR_ITERS}it"] = float(
+                np.max(np.abs(modes[name]["image0_30"] - x64_30)) / np.max(np.abs(x64_30)))
             modes[name]["max_voxel_err_vs_fp64"] = float(
                 np.max(np.abs(modes[name]["image0"] - x64)) / np.max(np.abs(x64)))
             log(f"{name}: rel L2 vs FP64 oracle {modes[name]['rel_l2_vs_fp64']:.2e}")
@@ -315,6 +329,7 @@ def main():
 
     for name in modes:
         modes[name].pop("image0")
+        modes[name].pop("image0_30")
     ok = [k for k in modes if modes[k].get("rel_l2_vs_fp64", 0.0) <= ERR_BAR]
     best = max(ok, key=lambda k: modes[k]["frames_per_s"]) if ok else "plain"
     plain_fps = modes["plain"]["frames_per_s"]
@@ -396,14 +411,15 @@ def copy_peak(timed, st, nbytes=2 << 30, reps=5):
 
 def bench_serving(log, nreq, rate):
     """Poisson arrivals of reconstruction requests over three geometries
-    (8x8 heads / 32^3, 13x13 heads / 52^3, 16x16 heads / 96^3; 60 angles at
-    3 deg, 2 mm voxels, 20 rays), 1e4-5e6 events and 10-50 iterations per
-    request, served by continuous batching (16 slots per geometry) from a
-    pre-built device matrix cache.  Latency = arrival to image on the host."""
+    (8x8 heads / 32^3 / 180 angles at 1 deg, 13x13 / 52^3 / 120 at 1.5 deg,
+    16x16 / 96^3 / 60 at 3 deg; 2 mm voxels, 20 rays), 1e4-5e6 events and
+    10-50 iterations per request, served by continuous batching (16 slots per
+    geometry) from a pre-built device matrix cache.  Latency = arrival to
+    image on the host."""
     from raptr_amd import geometry as G
     from raptr_amd import serving, synthetic, sysmat
-    geoms = [serving.Geometry(G.two_head_setup(p, 60, 3.0), G.centred_grid(v, 2.0), 20)
-             for p, v in ((8, 32), (13, 52), (16, 96))]
+    geoms = [serving.Geometry(G.two_head_setup(p, na, 180.0 / na), G.centred_grid(v, 2.0), 20)
+             for p, v, na in ((8, 32, 180), (13, 52, 120), (16, 96, 60))]
     cache = serving.MatrixCache(200 << 30, frames=16)
     t = time.perf_counter()
     builds = {}
@@ -423,8 +439,8 @@ def bench_serving(log, nreq, rate):
     stats = serving.Server(cache, max_chunk=10).serve(reqs, realtime=True)
     out = stats.summary()
     out.update({"workload": f"{nreq} requests, Poisson arrivals at {rate}/s, geometries "
-                            "8x8/32^3, 13x13/52^3, 16x16/96^3 (60 angles x 3 deg, 2 mm, 20 rays), "
-                            "1e4-5e6 events, 10-50 iterations, 16 slots per geometry",
+                            "8x8/32^3/180 angles, 13x13/52^3/120, 16x16/96^3/60 (2 mm voxels, "
+                            "20 rays), 1e4-5e6 events, 10-50 iterations, 16 slots per geometry",
                 "lanes": builds, "prep_s": round(t_prep, 1)})
     log(f"serving: p50 {out['p50_ms']:.0f} ms, p99 {out['p99_ms']:.0f} ms, "
         f"{out['requests_per_s']:.1f} req/s over {out['wall_s']:.1f} s")

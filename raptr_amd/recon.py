@@ -18,9 +18,10 @@ LAYOUTS = {"blocked": 0, "tiled": 1}
 
 class Reconstructor:
     """layout: "blocked" (row-padded CSR + row-blocked transpose, any frame
-    count), "tiled" (column tiles through LDS, one frame) or "auto" (tiled
-    for one frame, blocked otherwise).  block_rows: row-block height of the
-    blocked layout; for the tiled layout a cap on the tile width."""
+    count; the faster one), "tiled" (column tiles through LDS, one frame,
+    ~40% of the device memory) or "auto" (= blocked).  block_rows: row-block
+    height of the blocked layout; for the tiled layout a cap on the tile
+    width."""
 
     def __init__(self, A, sens, value_type="f32", max_frames=1, block_rows=0, stream=None,
                  layout="auto"):
@@ -28,7 +29,7 @@ class Reconstructor:
         self.value_type = value_type
         self.max_frames = max_frames
         if layout == "auto":
-            layout = "tiled" if max_frames == 1 else "blocked"
+            layout = "blocked"
         self.layout = layout
         p = C.c_void_p()
         capi.call("raptr_recon_create_ex", C.byref(p), A.m, A.n, A.nnz, A.row_ptr.ptr,
