@@ -356,6 +356,11 @@ int raptr_recon_get_backward(const raptr_recon* plan, int* staged, int* lanes_pe
  * registers (longer waves, fewer launches). */
 int raptr_recon_set_blocks_per_launch(raptr_recon* plan, int blocks);
 int raptr_recon_get_blocks_per_launch(const raptr_recon* plan);
+/* 64-frame batches: kernels that load the values and indices of 64
+ * entries with one instruction each and broadcast them across the 16 lanes
+ * of an entry slot (on != 0), or the general kernels (0). */
+int raptr_recon_set_broadcast(raptr_recon* plan, int on);
+int raptr_recon_get_broadcast(const raptr_recon* plan);
 /* Entries of the padded row-blocked transpose (0 for the tiled layout). */
 int64_t raptr_recon_transpose_nnz(const raptr_recon* plan);
 /* niter MLEM iterations (reco.cu:230-303) of nframes frames at once

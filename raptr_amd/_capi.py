@@ -119,6 +119,8 @@ _SIGS = {
     "raptr_recon_set_blocks_per_launch": (i32, [vp, i32]),
     "raptr_recon_get_blocks_per_launch": (i32, [vp]),
     "raptr_recon_transpose_nnz": (i64, [vp]),
+    "raptr_recon_set_broadcast": (i32, [vp, i32]),
+    "raptr_recon_get_broadcast": (i32, [vp]),
     "raptr_recon_set_tiled": (i32, [vp, i32, i32, i32, i32]),
     "raptr_recon_get_tiled": (i32, [vp, vp, vp, vp, vp, vp, vp]),
     "raptr_recon_get_backward": (i32, [vp, vp, vp, vp]),

@@ -342,6 +342,63 @@ __global__ __launch_bounds__(256) void k_fwdF(int64_t m, const int64_t* __restri
   }
 }
 
+// End of a backward launch for column j: reduce over entry slots (done by
+// the caller), carry c across launches, and in the last launch apply
+// x' = x*c/(s + (s==0)) and write the workgroup's FP64 partial sums.
+template <int FR>
+__device__ __forceinline__ void bwd_finish(float (&acc)[FrameVec<FR>::V], int j, int n, int q,
+                                           int slot, int w, int lb, int first, int last,
+                                           float* __restrict__ c, const float* __restrict__ x,
+                                           const float* __restrict__ s,
+                                           float* __restrict__ xx,
+                                           double* __restrict__ partials) {
+  constexpr int V = FrameVec<FR>::V;
+  double part[V];
+#pragma unroll
+  for (int v = 0; v < V; ++v) part[v] = 0.0;
+  if (j < n && slot == 0) {
+    const int64_t o = int64_t(j) * FR + V * q;
+    float cv[V];
+    if (first) {
+#pragma unroll
+      for (int v = 0; v < V; ++v) cv[v] = acc[v];
+    } else {
+      load_frames<V>(c + o, cv);
+#pragma unroll
+      for (int v = 0; v < V; ++v) cv[v] += acc[v];
+    }
+    if (!last) {
+      store_frames<V>(c + o, cv);
+    } else {
+      float d = s[j];
+      d += float(d == 0.0f);
+      float xv[V], nv[V];
+      load_frames<V>(x + o, xv);
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        nv[v] = xv[v] * cv[v] / d;
+        part[v] = double(nv[v]);
+      }
+      store_frames<V>(xx + o, nv);
+    }
+  }
+  if (last) {
+    // per-frame block sums: lanes with slot 0 of each wave hold frames
+    // V*q .. V*q+V-1
+    __shared__ double red[256 / kWave][64];
+    if (slot == 0) {
+#pragma unroll
+      for (int v = 0; v < V; ++v) red[w][V * q + v] = part[v];
+    }
+    __syncthreads();
+    if (threadIdx.x < FR) {
+      double t = 0.0;
+      for (int ww = 0; ww < 256 / kWave; ++ww) t += red[ww][threadIdx.x];
+      partials[int64_t(lb) * FR + threadIdx.x] = t;
+    }
+  }
+}
+
 // Backward over nbl consecutive row blocks: c[j][f] (+)= sum over those
 // blocks' entries of column j of v * e'[row][f]; the launch that ends with
 // the last block applies the update xx = x * c / (s + (s==0)) and writes
@@ -423,50 +480,129 @@ __global__ __launch_bounds__(256) void k_bwd(int n, const int64_t* __restrict__ 
 #pragma unroll
       for (int v = 0; v < V; ++v) acc[v] += __shfl_xor(acc[v], off, kWave);
   }
-  double part[V];
+  bwd_finish<FR>(acc, j, n, q, slot, w, lb, first, last, c, x, s, xx, partials);
+}
+
+// 64-frame batches (16-lane frame groups): the 16 lanes of an entry slot
+// form one DPP row.  A wave loads the values and indices of 64 consecutive
+// entries with one 4-byte load each (lane (slot, q) holds entry
+// 16*(q/4) + 4*slot + q%4 of the chunk) and broadcasts them inside each row
+// with row_newbcast, so a chunk costs 2 index/value loads instead of 8; the
+// 16 gathers of 1 KiB each (4 entries x 256 B per instruction) remain.
+__device__ __forceinline__ int row_bcast_i(int v, int lane16) {
+  switch (lane16) {
+#define RAPTR_RB(L) \
+  case L:           \
+    return __builtin_amdgcn_update_dpp(0, v, 0x150 + L, 0xf, 0xf, false);
+    RAPTR_RB(0) RAPTR_RB(1) RAPTR_RB(2) RAPTR_RB(3) RAPTR_RB(4) RAPTR_RB(5) RAPTR_RB(6)
+    RAPTR_RB(7) RAPTR_RB(8) RAPTR_RB(9) RAPTR_RB(10) RAPTR_RB(11) RAPTR_RB(12) RAPTR_RB(13)
+    RAPTR_RB(14) RAPTR_RB(15)
+#undef RAPTR_RB
+  }
+  return v;
+}
+
+// one 64-entry chunk [k0, k0 + 64) of a padded segment ending at en: the
+// lane's frame vector gathers from base + index*64 are accumulated
+template <int VT>
+__device__ __forceinline__ void chunk64(const typename Val<VT>::T* __restrict__ val,
+                                        const int* __restrict__ idx, int64_t k0, int64_t en,
+                                        int slot, int q, const float* __restrict__ base,
+                                        float (&acc)[4]) {
+  const int64_t kk = k0 + 16 * (q >> 2) + 4 * slot + (q & 3);
+  const bool in = kk < en;
+  const float wv = in ? Val<VT>::get(val, kk) : 0.0f;
+  const int cv = in ? __builtin_nontemporal_load(idx + kk) : 0;
+  float g[4][4][4];
+  float w[4][4];
 #pragma unroll
-  for (int v = 0; v < V; ++v) part[v] = 0.0;
-  if (j < n && slot == 0) {
-    const int64_t o = int64_t(j) * FR + V * q;
-    float cv[V];
-    if (first) {
+  for (int t = 0; t < 4; ++t) {
 #pragma unroll
-      for (int v = 0; v < V; ++v) cv[v] = acc[v];
-    } else {
-      load_frames<V>(c + o, cv);
-#pragma unroll
-      for (int v = 0; v < V; ++v) cv[v] += acc[v];
-    }
-    if (!last) {
-      store_frames<V>(c + o, cv);
-    } else {
-      float d = s[j];
-      d += float(d == 0.0f);
-      float xv[V], nv[V];
-      load_frames<V>(x + o, xv);
-#pragma unroll
-      for (int v = 0; v < V; ++v) {
-        nv[v] = xv[v] * cv[v] / d;
-        part[v] = double(nv[v]);
+    for (int jj = 0; jj < 4; ++jj) {
+      w[t][jj] = __int_as_float(row_bcast_i(__float_as_int(wv), 4 * t + jj));
+      const int c = row_bcast_i(cv, 4 * t + jj);
+      if (k0 + 16 * t + 4 * slot < en) {
+        load_frames<4>(base + int64_t(c) * 64, g[t][jj]);
+      } else {
+        g[t][jj][0] = g[t][jj][1] = g[t][jj][2] = g[t][jj][3] = 0.0f;
       }
-      store_frames<V>(xx + o, nv);
     }
   }
-  if (last) {
-    // per-frame block sums: lanes with slot 0 of each wave hold frames
-    // V*q .. V*q+V-1
-    __shared__ double red[256 / kWave][64];
-    if (slot == 0) {
 #pragma unroll
-      for (int v = 0; v < V; ++v) red[w][V * q + v] = part[v];
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int jj = 0; jj < 4; ++jj)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc[v] = fmaf(w[t][jj], g[t][jj][v], acc[v]);
+}
+
+template <int VT>
+__global__ __launch_bounds__(256) void k_fwd64(int64_t m, const int64_t* __restrict__ rp,
+                                               const int* __restrict__ col,
+                                               const typename Val<VT>::T* __restrict__ val,
+                                               const float* __restrict__ rscale,
+                                               const float* __restrict__ x,
+                                               const float* __restrict__ y,
+                                               float* __restrict__ e) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int q = lane & 15, slot = lane >> 4;
+  const int64_t i = xcd_block(blockIdx.x, gridDim.x) * (256 / kWave) + threadIdx.x / kWave;
+  if (i >= m) return;
+  const int64_t b = rp[i], en = rp[i + 1];
+  float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  const float* xq = x + 4 * q;
+  for (int64_t k0 = b; k0 < en; k0 += 64) chunk64<VT>(val, col, k0, en, slot, q, xq, acc);
+#pragma unroll
+  for (int off = 32; off >= 16; off >>= 1)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) acc[v] += __shfl_xor(acc[v], off, kWave);
+  if (slot == 0) {
+    const float sc = VT == kF32 ? 1.0f : rscale[i];
+    float yy[4], r[4];
+    load_frames<4>(y + i * 64 + 4 * q, yy);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      float yt = sc * acc[v];
+      float t = (yt != 0.0f && yy[v] != 0.0f) ? yy[v] / yt : 0.0f;
+      r[v] = VT == kF32 ? t : sc * t;
     }
-    __syncthreads();
-    if (threadIdx.x < FR) {
-      double t = 0.0;
-      for (int ww = 0; ww < 256 / kWave; ++ww) t += red[ww][threadIdx.x];
-      partials[int64_t(lb) * FR + threadIdx.x] = t;
-    }
+    store_frames<4>(e + i * 64 + 4 * q, r);
   }
+}
+
+template <int VT>
+__global__ __launch_bounds__(256) void k_bwd64(int n, const int64_t* __restrict__ ptr, int nbl,
+                                               const int* __restrict__ brow,
+                                               const typename Val<VT>::T* __restrict__ val,
+                                               const float* __restrict__ e,
+                                               float* __restrict__ c, int first, int last,
+                                               const float* __restrict__ x,
+                                               const float* __restrict__ s,
+                                               float* __restrict__ xx,
+                                               double* __restrict__ partials) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int q = lane & 15, slot = lane >> 4;
+  const int w = threadIdx.x / kWave;
+  const int lb = int(xcd_block(blockIdx.x, gridDim.x));
+  const int j = lb * (256 / kWave) + w;
+  float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  int64_t pb = 0, pe = 0;
+  if (j < n && lane < nbl) {
+    pb = ptr[int64_t(lane) * (n + 1) + j];
+    pe = ptr[int64_t(lane) * (n + 1) + j + 1];
+  }
+  const float* eq = e + 4 * q;
+  for (int bb = 0; bb < nbl && j < n; ++bb) {
+    const int64_t b = __shfl(pb, bb, kWave), en = __shfl(pe, bb, kWave);
+    for (int64_t k0 = b; k0 < en; k0 += 64) chunk64<VT>(val, brow, k0, en, slot, q, eq, acc);
+  }
+  if (j < n) {
+#pragma unroll
+    for (int off = 32; off >= 16; off >>= 1)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc[v] += __shfl_xor(acc[v], off, kWave);
+  }
+  bwd_finish<64>(acc, j, n, q, slot, w, lb, first, last, c, x, s, xx, partials);
 }
 
 // Single-frame backward staged through LDS: each workgroup owns a
@@ -631,6 +767,17 @@ int launch_fwd_any(const raptr_recon* R, int F, const float* x, const float* y, 
     return fail(RAPTR_ERR_INVALID_ARG, "forward shape lanes %d unroll %d unsupported", L, U);
   }
   unsigned g = unsigned((m + 3) / 4);
+  if (F == 64 && R->bcast64) {
+#define RAPTR_F64(VV)                                                                        \
+    if (R->vt == VV) {                                                                       \
+      hipLaunchKernelGGL((k_fwd64<VV>), dim3(g), dim3(256), 0, st, m, R->frp, R->fcol,        \
+                         (const typename Val<VV>::T*)R->fval, R->rscale, x, y, e);            \
+      RAPTR_LAUNCH_CHECK();                                                                  \
+      return RAPTR_OK;                                                                       \
+    }
+    RAPTR_F64(kF32) RAPTR_F64(kF16) RAPTR_F64(kBF16) RAPTR_F64(kU16)
+#undef RAPTR_F64
+  }
 #define RAPTR_FF(VV, FF, UU)                                                               \
   if (R->vt == VV && F == FF && U == UU) {                                                 \
     hipLaunchKernelGGL((k_fwdF<VV, FF, UU>), dim3(g), dim3(256), 0, st, m, R->frp, R->fcol, \
@@ -657,6 +804,18 @@ int launch_bwd_any(const raptr_recon* R, int F, int blk, int nbl, const float* x
   if (nb_out) *nb_out = int(g);
   const int64_t* ptr = R->bptr + int64_t(blk) * (n + 1);
   int first = blk == 0, last = blk + nbl == R->nblk;
+  if (F == 64 && R->bcast64) {
+#define RAPTR_B64(VV)                                                                        \
+    if (R->vt == VV) {                                                                       \
+      hipLaunchKernelGGL((k_bwd64<VV>), dim3(g), dim3(256), 0, st, n, ptr, nbl, R->brow,      \
+                         (const typename Val<VV>::T*)R->bval, R->e, R->c, first, last, x,     \
+                         R->s, R->xx, R->partials);                                          \
+      RAPTR_LAUNCH_CHECK();                                                                  \
+      return RAPTR_OK;                                                                       \
+    }
+    RAPTR_B64(kF32) RAPTR_B64(kF16) RAPTR_B64(kBF16) RAPTR_B64(kU16)
+#undef RAPTR_B64
+  }
   int U = R->bwd_unroll;
 #define RAPTR_B(VV, FF, UU)                                                                 \
   if (R->vt == VV && F == FF && U == UU) {                                                  \
@@ -1075,6 +1234,21 @@ int raptr_recon_set_blocks_per_launch(raptr_recon* R, int blocks) {
 int raptr_recon_get_blocks_per_launch(const raptr_recon* R) { return R ? R->bwd_bpl : -1; }
 
 int64_t raptr_recon_transpose_nnz(const raptr_recon* R) { return R ? R->bnnz : -1; }
+
+int raptr_recon_set_broadcast(raptr_recon* R, int on) {
+  if (!R) return fail(RAPTR_ERR_INVALID_ARG, "null plan");
+  if (R->bcast64 != (on ? 1 : 0)) {
+    for (auto& kv : R->graphs) {
+      if (kv.second.second) (void)hipGraphExecDestroy(kv.second.second);
+      if (kv.second.first) (void)hipGraphDestroy(kv.second.first);
+    }
+    R->graphs.clear();
+  }
+  R->bcast64 = on ? 1 : 0;
+  return RAPTR_OK;
+}
+
+int raptr_recon_get_broadcast(const raptr_recon* R) { return R ? R->bcast64 : -1; }
 
 int raptr_recon_set_tiled(raptr_recon* R, int fwd_lanes, int fwd_unroll, int bwd_lanes,
                           int bwd_unroll) {

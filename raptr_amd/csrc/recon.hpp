@@ -108,6 +108,7 @@ struct raptr_recon {
   int bwd_lpc = 16;    // lanes per column of the LDS-staged backward
   int bwd_wgs = 512;   // workgroups of the LDS-staged backward
   int bwd_bpl = 1;     // row blocks per backward launch (blocked layout)
+  int bcast64 = 0;     // 64-frame kernels with in-row broadcast of values/indices
   // column-tiled single-frame layout (RAPTR_LAYOUT_TILED): tile q holds
   // columns [q*tile_cols, (q+1)*tile_cols) of every row, rows padded to
   // multiples of 4 entries, 16-bit column offsets within the tile

@@ -58,6 +58,9 @@ class Reconstructor:
                                  "tiled_bwd_lanes", "tiled_bwd_unroll"), [v.value for v in t])))
         return out
 
+    def set_broadcast(self, on):
+        capi.call("raptr_recon_set_broadcast", self.handle, 1 if on else 0)
+
     def set_blocks_per_launch(self, blocks):
         capi.call("raptr_recon_set_blocks_per_launch", self.handle, int(blocks))
 
@@ -171,9 +174,19 @@ class Reconstructor:
             self.set_backward(st, lpc, wgs)
         self.set_config(fl, fu, u)
         self.set_blocks_per_launch(bpl)
+        bc = 0
+        if nframes == 64:
+            # the 64-frame kernels with in-row broadcast of values and indices
+            base = timed()
+            self.set_broadcast(True)
+            if timed() < base:
+                bc = 1
+            else:
+                self.set_broadcast(False)
         capi.call("raptr_memcpy_d2d", x.ptr, x_save.ptr, x.nbytes, stream)
         cfg = {"fwd_lanes": fl, "fwd_unroll": fu, "bwd_unroll": u, "bwd_staged": st,
-               "bwd_lanes_per_column": lpc, "bwd_workgroups": wgs, "bwd_blocks_per_launch": bpl}
+               "bwd_lanes_per_column": lpc, "bwd_workgroups": wgs, "bwd_blocks_per_launch": bpl,
+               "broadcast64": bc}
         return cfg, best[1]
 
     def close(self):

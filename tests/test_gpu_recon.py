@@ -149,6 +149,12 @@ def test_recon_batched_frames_match_single(gpu, oracle, small_system, F):
         assert _rel(X[:, f], x1) <= 1e-6, f
         if f in (0, F - 1):
             assert _rel(X[:, f], _oracle_mlem(oracle, S, Y[:, f].astype(np.float64), 5)) <= 1e-4
+    # 64 frames: the kernels with in-row broadcast of values and indices
+    if F == 64:
+        Rb.set_broadcast(True)
+        Xc = Rb.mlem(Y, X0, 5)
+        assert _rel(Xc, X) <= 1e-6
+        Rb.set_broadcast(False)
     # fewer frames than the plan's maximum, in the same plan
     if F >= 4:
         Xh = Rb.mlem(Y[:, :F // 2].copy(), X0[:, :F // 2].copy(), 5)
