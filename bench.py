@@ -165,7 +165,7 @@ def main():
         # sizes (the first batched mode picks, the others reuse its choice)
         # one frame: both plan layouts (column tiles, row blocks) are tried
         if nf == 1:
-            rows_try = [("tiled", 0), ("blocked", 0)]
+            rows_try = [("tiled", 0), ("blocked", 0), ("blocked", 12288)]
         elif best_rows is not None:
             rows_try = [("blocked", best_rows)]
         else:
@@ -279,6 +279,13 @@ def main():
         bytes_f = 8 * info["padded_nnz"] + 8 * (m + 1) + 4 * m + 4 * m + 4 * n
     if info["layout"] == "tiled":
         pass  # set above
+    elif info["bwd_staged"] == 2:
+        # one launch: matrix (value + packed index), two segment bounds per
+        # wave and block, the ratio vector (counted once; each workgroup
+        # reads it from L2 into LDS), x/s read and x' written once
+        G = -(-n // 256)
+        bytes_b = (8 * info["transpose_nnz"] + 16 * nb * (4 * G) + 4 * m + 12 * n + 8 * G)
+        bwd_name = f"k_bwd_flat (backward + update, one launch over {nb} LDS row slices)"
     elif info["bwd_staged"]:
         # one launch; the ratio vector read once per workgroup from L2 (counted
         # once), column sums in LDS, x/s read and x' written once

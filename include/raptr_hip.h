@@ -341,12 +341,16 @@ int raptr_recon_info(const raptr_recon* plan, int64_t* device_bytes,
  * are padded to multiples of 4 entries (value 0) for 16-byte loads. */
 int raptr_recon_set_config(raptr_recon* plan, int fwd_lanes, int fwd_unroll,
                            int bwd_unroll);
-/* Single-frame backward pass staged through LDS (staged != 0; needs
- * block_rows <= 24576 and a multiple of 4): each of `workgroups` workgroups owns a contiguous
- * column range, loads each row block's ratio slice into LDS and gathers
- * from there with lanes_per_column (8, 16, 32) lanes per column; column sums
- * carried in LDS across blocks.  Single-frame plans created with such
- * block_rows start with it on; others start with the block-by-block pass. */
+/* Single-frame backward pass staged through LDS (needs block_rows <= 24576
+ * and a multiple of 4).  staged = 1: each of `workgroups` workgroups owns a
+ * contiguous column range, loads each row block's ratio slice into LDS and
+ * walks its columns with lanes_per_column (8, 16, 32) lanes per column.
+ * staged = 2 (flat; the default of such plans): workgroups own 256-column
+ * tiles and stream each block's entries of a wave's 64 columns as one range
+ * (packed column-in-tile/row-in-block indices), adding runs of equal column
+ * with a wave segmented scan; lanes_per_column and workgroups are unused.
+ * Column sums stay in LDS across blocks in a fixed order.  staged = 0: the
+ * block-by-block pass. */
 int raptr_recon_set_backward(raptr_recon* plan, int staged, int lanes_per_column,
                              int workgroups);
 int raptr_recon_get_backward(const raptr_recon* plan, int* staged, int* lanes_per_column,

@@ -689,6 +689,121 @@ __global__ __launch_bounds__(256) void k_bwd_lds(int n, int nblk, int blk_rows, 
   }
 }
 
+// Single-frame backward, flat LDS-staged form.  A workgroup owns a tile of
+// 256 columns (64 per wave) and walks the row blocks in order: block b's
+// slice of the ratio vector is loaded into LDS, then each wave streams the
+// block's entries of its 64 columns as one contiguous range (16-byte value
+// loads and 16-byte loads of packed (column in tile, row in block) indices,
+// 4 entries of one column per lane), multiplies by the LDS ratio and adds
+// runs of equal column with a wave segmented scan.  The run's last lane adds
+// to the column sum in LDS; a column belongs to one wave, so there are no
+// atomics and the summation order is fixed.  After the last block the
+// update x' = x*c/(s + (s==0)) and the workgroup's FP64 partial sum follow.
+template <int VT, int U>
+__global__ __launch_bounds__(256) void k_bwd_flat(int n, int nblk, int blk_rows, int64_t m,
+                                                  const int64_t* __restrict__ ptr,
+                                                  const uint32_t* __restrict__ pidx,
+                                                  const typename Val<VT>::T* __restrict__ val,
+                                                  const float* __restrict__ e,
+                                                  const float* __restrict__ x,
+                                                  const float* __restrict__ s,
+                                                  float* __restrict__ xx,
+                                                  double* __restrict__ partials) {
+  extern __shared__ float lds[];
+  float* es = lds;                // blk_rows
+  float* cacc = lds + blk_rows;   // 256 column sums
+  const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
+  const int tile = int(xcd_block(blockIdx.x, gridDim.x));
+  const int c0 = tile * 256;
+  const int cw0 = min(n, c0 + 64 * w), cw1 = min(n, c0 + 64 * w + 64);
+  cacc[t] = 0.0f;
+  for (int b = 0; b < nblk; ++b) {
+    const int64_t r0 = int64_t(b) * blk_rows;
+    const int rows = int(min(int64_t(blk_rows), m - r0));
+    __syncthreads();
+    for (int k = 4 * t; k < rows; k += 4 * 256) {
+      if (k + 3 < rows) {
+        vf4 v = *reinterpret_cast<const vf4*>(e + r0 + k);
+        es[k] = v.x, es[k + 1] = v.y, es[k + 2] = v.z, es[k + 3] = v.w;
+      } else {
+        for (int q2 = k; q2 < rows; ++q2) es[q2] = e[r0 + q2];
+      }
+    }
+    __syncthreads();
+    const int64_t* pb = ptr + int64_t(b) * (n + 1);
+    const int64_t K0 = pb[cw0], K1 = pb[cw1];  // multiples of 4
+    for (int64_t k = K0 + 4 * lane; k - 4 * lane < K1; k += int64_t(256) * U) {
+      float v[U][4];
+      vi4 pk[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int64_t kk = k + 256 * u;
+        if (kk < K1) {
+          load4<VT>(val, kk, v[u][0], v[u][1], v[u][2], v[u][3]);
+          pk[u] = __builtin_nontemporal_load(reinterpret_cast<const vi4*>(pidx + kk));
+        } else {
+          v[u][0] = v[u][1] = v[u][2] = v[u][3] = 0.0f;
+          pk[u] = vi4{-1, -1, -1, -1};
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const bool valid = k + 256 * u < K1;
+        const int col = valid ? int(uint32_t(pk[u].x) >> 24) : -1;
+        float p = 0.0f;
+        if (valid) {
+          p = v[u][0] * es[pk[u].x & 0xffffff];
+          p = fmaf(v[u][1], es[pk[u].y & 0xffffff], p);
+          p = fmaf(v[u][2], es[pk[u].z & 0xffffff], p);
+          p = fmaf(v[u][3], es[pk[u].w & 0xffffff], p);
+        }
+        // inclusive segmented scan over lanes of equal column
+#pragma unroll
+        for (int off = 1; off < kWave; off <<= 1) {
+          const float pv = __shfl_up(p, off, kWave);
+          const int pc = __shfl_up(col, off, kWave);
+          if (lane >= off && pc == col) p += pv;
+        }
+        const int nc = __shfl_down(col, 1, kWave);
+        if (valid && (lane == kWave - 1 || nc != col)) cacc[col] += p;
+      }
+    }
+  }
+  __syncthreads();
+  const int j = c0 + t;
+  double part = 0.0;
+  if (j < n) {
+    float d = s[j];
+    d += float(d == 0.0f);
+    const float v = x[j] * cacc[t] / d;
+    xx[j] = v;
+    part = double(v);
+  }
+  __shared__ double red[256 / kWave];
+  part = wave_reduce_sum(part);
+  if (lane == 0) red[w] = part;
+  __syncthreads();
+  if (t == 0) {
+    double tot = 0.0;
+    for (int q = 0; q < 256 / kWave; ++q) tot += red[q];
+    partials[tile] = tot;
+  }
+}
+
+// packed index of the flat staged backward: (column in its 256-column tile)
+// << 24 | (row in its row block), per entry of the padded transpose
+__global__ void k_bt_pack(const int64_t* pptr, int64_t total, int n, int blk_rows,
+                          const int* brow, uint32_t* pidx) {
+  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < total;
+       t += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t b = t / (n + 1), j = t - b * (n + 1);
+    if (j >= n) continue;
+    const uint32_t hi = uint32_t(j & 255) << 24;
+    const int64_t r0 = b * blk_rows;
+    for (int64_t p = pptr[t]; p < pptr[t + 1]; ++p) pidx[p] = hi | uint32_t(brow[p] - r0);
+  }
+}
+
 // scale_f = 1/sum_b partials[b][f] (FP64, fixed order); 0 for an empty frame
 __global__ void k_norms(const double* partials, int nb, int FR, float* scale, double* norm) {
   __shared__ double red[256];
@@ -842,7 +957,39 @@ int launch_bwd_any(const raptr_recon* R, int F, int blk, int nbl, const float* x
 // LDS capacity for the staged backward (one e slice + column sums)
 constexpr int kLdsRowsMax = 24576;
 
+int launch_bwd_flat(const raptr_recon* R, const float* x, hipStream_t st, int* nb_out) {
+  const int n = R->n;
+  const int G = (n + 255) / 256;
+  if (nb_out) *nb_out = G;
+  if (!R->pidx) return fail(RAPTR_ERR_INVALID_ARG, "plan has no packed staged indices");
+  const size_t lds = size_t(R->blk_rows + 256) * sizeof(float);
+  const int U = R->bwd_unroll;
+#define RAPTR_BF(VV, UU)                                                                    \
+  if (R->vt == VV && U == UU) {                                                             \
+    static bool attr = false;                                                               \
+    if (!attr) {                                                                            \
+      if (hipFuncSetAttribute((const void*)k_bwd_flat<VV, UU>,                              \
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) !=    \
+          hipSuccess)                                                                       \
+        (void)hipGetLastError();                                                            \
+      attr = true;                                                                          \
+    }                                                                                       \
+    hipLaunchKernelGGL((k_bwd_flat<VV, UU>), dim3(G), dim3(256), lds, st, n, R->nblk,        \
+                       R->blk_rows, int64_t(R->m), (const int64_t*)R->bptr,                 \
+                       (const uint32_t*)R->pidx, (const typename Val<VV>::T*)R->bval,       \
+                       (const float*)R->e, x, (const float*)R->s, R->xx, R->partials);      \
+    RAPTR_LAUNCH_CHECK();                                                                   \
+    return RAPTR_OK;                                                                        \
+  }
+#define RAPTR_BFV(VV) RAPTR_BF(VV, 1) RAPTR_BF(VV, 2) RAPTR_BF(VV, 4)
+  RAPTR_BFV(kF32) RAPTR_BFV(kF16) RAPTR_BFV(kBF16) RAPTR_BFV(kU16)
+#undef RAPTR_BFV
+#undef RAPTR_BF
+  return fail(RAPTR_ERR_INVALID_ARG, "flat staged backward shape unsupported");
+}
+
 int launch_bwd_lds(const raptr_recon* R, const float* x, hipStream_t st, int* nb_out) {
+  if (R->bwd_lds == 2) return launch_bwd_flat(R, x, st, nb_out);
   const int n = R->n;
   int G = std::min(R->bwd_wgs, std::max(1, (n + 63) / 64));
   const int cpw = (n + G - 1) / G;
@@ -1109,6 +1256,13 @@ int raptr_recon_create_ex(raptr_recon** out, int m, int n, int64_t nnz, const in
         hipLaunchKernelGGL(k_bt_pad, dim3(grid_for(T, 256, 1 << 16)), dim3(256), 0, st,
                            (const int64_t*)uptr, (const int64_t*)R->bptr, T, n, block_rows,
                            R->brow, R->bval, value_type);
+        // single frame with LDS-sized row blocks: packed indices for the
+        // flat staged backward
+        if (max_frames == 1 && block_rows <= kLdsRowsMax && block_rows % 4 == 0 &&
+            !(rc = ralloc(R, &R->pidx, size_t(h) + 4)))
+          hipLaunchKernelGGL(k_bt_pack, dim3(grid_for(T, 256, 1 << 16)), dim3(256), 0, st,
+                             (const int64_t*)R->bptr, T, n, block_rows, (const int*)R->brow,
+                             R->pidx);
         hipError_t le = hipGetLastError();
         if (le != hipSuccess) rc = fail(RAPTR_ERR_DEVICE, "transpose: %s", hipGetErrorString(le));
       }
@@ -1139,8 +1293,7 @@ int raptr_recon_create_ex(raptr_recon** out, int m, int n, int64_t nnz, const in
   if (se != hipSuccess) return bail(fail(RAPTR_ERR_DEVICE, "create: %s", hipGetErrorString(se)));
   R->fwd_lanes = double(nnz) / std::max(m, 1) >= 192 ? 32 : 16;
   R->fwd_unroll = max_frames == 1 ? 1 : 2;
-  R->bwd_lds = (layout == RAPTR_LAYOUT_BLOCKED && max_frames == 1 && block_rows <= kLdsRowsMax &&
-                block_rows % 4 == 0) ? 1 : 0;
+  R->bwd_lds = R->pidx ? 2 : 0;
   *out = R;
   return RAPTR_OK;
 }
@@ -1193,8 +1346,10 @@ int raptr_recon_set_config(raptr_recon* R, int fwd_lanes, int fwd_unroll, int bw
 
 int raptr_recon_set_backward(raptr_recon* R, int staged, int lanes_per_column, int workgroups) {
   if (!R) return fail(RAPTR_ERR_INVALID_ARG, "null plan");
+  if (staged < 0 || staged > 2)
+    return fail(RAPTR_ERR_INVALID_ARG, "staged must be 0 (off), 1 (per column) or 2 (flat)");
   if (staged && (R->layout != RAPTR_LAYOUT_BLOCKED || R->fmax != 1 ||
-                 R->blk_rows > kLdsRowsMax || R->blk_rows % 4))
+                 R->blk_rows > kLdsRowsMax || R->blk_rows % 4 || !R->pidx))
     return fail(RAPTR_ERR_INVALID_ARG,
                 "staged backward needs a single-frame plan with block_rows <= %d, "
                 "a multiple of 4", kLdsRowsMax);
@@ -1208,7 +1363,7 @@ int raptr_recon_set_backward(raptr_recon* R, int staged, int lanes_per_column, i
     }
     R->graphs.clear();
   }
-  R->bwd_lds = staged ? 1 : 0;
+  R->bwd_lds = staged;
   R->bwd_lpc = lanes_per_column;
   R->bwd_wgs = workgroups;
   return RAPTR_OK;

@@ -104,7 +104,8 @@ struct raptr_recon {
   int fwd_lanes = 64;
   int fwd_unroll = 1;  // groups of 4 entries in flight per lane / entry slot
   int bwd_unroll = 2;
-  int bwd_lds = 0;     // single-frame backward staged through LDS (0: off)
+  int bwd_lds = 0;     // single-frame backward through LDS: 0 off, 1 per column, 2 flat
+  uint32_t* pidx = nullptr;  // flat staged backward: (column in tile) << 24 | row in block
   int bwd_lpc = 16;    // lanes per column of the LDS-staged backward
   int bwd_wgs = 512;   // workgroups of the LDS-staged backward
   int bwd_bpl = 1;     // row blocks per backward launch (blocked layout)

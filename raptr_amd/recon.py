@@ -73,10 +73,12 @@ class Reconstructor:
                   int(bwd_unroll))
 
     def set_backward(self, staged, lanes_per_column=16, workgroups=512):
-        """Single-frame backward through LDS slices of the ratio vector
-        (``staged``) or block by block through L2."""
-        capi.call("raptr_recon_set_backward", self.handle, 1 if staged else 0,
-                  int(lanes_per_column), int(workgroups))
+        """Single-frame backward through LDS slices of the ratio vector:
+        staged 1 (per column) or 2 (flat, segmented scan; True means 2), or
+        block by block through L2 (0 / False)."""
+        mode = (2 if staged else 0) if isinstance(staged, bool) else int(staged)
+        capi.call("raptr_recon_set_backward", self.handle, mode, int(lanes_per_column),
+                  int(workgroups))
 
     def run(self, y, x, niter, nframes=1, use_graph=True, stream=None):
         """In-place MLEM on device arrays y ([m][F]) and x ([n][F])."""
@@ -160,6 +162,7 @@ class Reconstructor:
         if staged_ok:
             bshapes += [(1, lpc, wgs, u, 1) for lpc in (8, 16, 32) for wgs in (256, 512, 1024)
                         for u in (2, 4)]
+            bshapes += [(2, 16, 512, u, 1) for u in (1, 2, 4)]
         best = None
         for st, lpc, wgs, u, bpl in bshapes:
             if staged_ok:

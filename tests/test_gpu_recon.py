@@ -77,14 +77,22 @@ def test_recon_staged_backward(gpu, oracle, small_system, lpc, wgs, unroll):
     x0 = np.full(n, 1.0 / n, np.float32)
     x64 = _oracle_mlem(oracle, S, S["y"], 5)
     R = Reconstructor(S["A"], S["sens"], "f32", 1, block_rows=800, layout="blocked")
-    assert R.info()["bwd_staged"] == 1 and R.info()["row_blocks"] > 8
+    assert R.info()["bwd_staged"] == 2 and R.info()["row_blocks"] > 8
     R.set_config(16, 1, unroll)
-    R.set_backward(True, lpc, wgs)
+    R.set_backward(1, lpc, wgs)
     a = R.mlem(S["y"], x0, 5)
     R.set_backward(False)
     b = R.mlem(S["y"], x0, 5)
     assert _rel(a, x64) <= 1e-4
     assert _rel(a, b) <= 1e-6
+    # flat staged form (segmented scans, fixed order: replays are identical)
+    R.set_config(16, 1, min(unroll, 4))
+    R.set_backward(2)
+    c1 = R.mlem(S["y"], x0, 5)
+    c2 = R.mlem(S["y"], x0, 5)
+    np.testing.assert_array_equal(c1, c2)
+    assert _rel(c1, x64) <= 1e-4
+    assert _rel(c1, b) <= 1e-6
     with pytest.raises(Exception):
         R.set_backward(True, 12, 512)
 
