@@ -34,10 +34,33 @@ __global__ void k_padded_len(const int64_t* rp, int m, int64_t* plen) {
 __global__ void k_set_last(int64_t* p, int m, const int64_t* total) { p[m] = *total; }
 
 // One wave per row: row maximum -> scale (mixed), copy with zero padding.
+// interleave chunks: forward rows for 32 lanes x 4 entries per step, column
+// segments for 64 lanes x 4 entries per step
+constexpr int kIlvFwd = 128;
+constexpr int kIlvBwd = 256;
+
+// Single-frame plans store every padded row (forward) and column segment
+// (transpose) interleaved in chunks of C entries: the chunk's L entries
+// (L = C, or the remainder, a multiple of 4) are laid out as L/4 groups of
+// 4 with group g holding sorted entries g, g + L/4, g + 2L/4, g + 3L/4.  A
+// wave that vector-loads one group per lane then gathers sorted-consecutive
+// entries in each of its 4 gather instructions; consecutive entries of a
+// row (voxels along z) or of a column (neighbouring channels) share cache
+// lines, so the gathers touch about a quarter of the lines.  C = 0: no
+// interleave.  Returns the position of sorted entry o in a segment of
+// padded length P.
+__device__ __forceinline__ int64_t ilv(int64_t o, int64_t P, int C) {
+  if (C == 0) return o;
+  const int64_t c = o / C, base = c * C;
+  const int64_t L = P - base < C ? P - base : C;
+  const int64_t q = L / 4, r = o - base;
+  return base + 4 * (r % q) + r / q;
+}
+
 __global__ __launch_bounds__(256) void k_pad_copy(const int64_t* rp, const int* col,
                                                   const float* val, int m, int vt,
                                                   const int64_t* prp, int* pcol, void* pval,
-                                                  float* scale) {
+                                                  float* scale, int C) {
   int64_t wid = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
   int lane = threadIdx.x & (kWave - 1);
   int64_t nw = int64_t(gridDim.x) * blockDim.x / kWave;
@@ -55,11 +78,12 @@ __global__ __launch_bounds__(256) void k_pad_copy(const int64_t* rp, const int* 
       bool real = b + k < e;
       int c = real ? col[b + k] : lastc;
       float v = real ? val[b + k] : 0.0f;
-      pcol[pb + k] = c;
+      const int64_t d = pb + ilv(k, pe - pb, C);
+      pcol[d] = c;
       if (vt == kF32)
-        static_cast<float*>(pval)[pb + k] = v;
+        static_cast<float*>(pval)[d] = v;
       else
-        static_cast<uint16_t*>(pval)[pb + k] = f2h(v / sc, vt);
+        static_cast<uint16_t*>(pval)[d] = f2h(v / sc, vt);
     }
   }
 }
@@ -102,13 +126,13 @@ __global__ void k_bt_padlen(const int64_t* uptr, int64_t total, int n, int64_t* 
 __global__ void k_bt_scatter(const uint64_t* keys, const uint32_t* idx, int64_t nnz, int n,
                              int colbits, const int64_t* uptr, const int64_t* pptr,
                              const uint32_t* rowid, const float* val, const float* scale,
-                             int vt, int* brow, void* bval) {
+                             int vt, int* brow, void* bval, int C) {
   const uint64_t cmask = (uint64_t(1) << colbits) - 1;
   for (int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; j < nnz;
        j += int64_t(gridDim.x) * blockDim.x) {
     const uint64_t key = keys[j];
     const int64_t t = int64_t(key >> colbits) * (n + 1) + int64_t(key & cmask);
-    const int64_t d = pptr[t] + (j - uptr[t]);
+    const int64_t d = pptr[t] + ilv(j - uptr[t], pptr[t + 1] - pptr[t], C);
     const uint32_t k = idx[j];
     const uint32_t r = rowid[k];
     brow[d] = int(r);
@@ -123,14 +147,15 @@ __global__ void k_bt_scatter(const uint64_t* keys, const uint32_t* idx, int64_t 
 
 // padding entries: value 0, row = the block's first row (inside the block)
 __global__ void k_bt_pad(const int64_t* uptr, const int64_t* pptr, int64_t total, int n,
-                         int blk_rows, int* brow, void* bval, int vt) {
+                         int blk_rows, int* brow, void* bval, int vt, int C) {
   for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < total;
        t += int64_t(gridDim.x) * blockDim.x) {
     const int64_t b = t / (n + 1), j = t - b * (n + 1);
     if (j >= n) continue;
     const int64_t len = uptr[t + 1] - uptr[t];
     const int64_t plen = (len + 3) & ~int64_t(3);
-    for (int64_t p = pptr[t] + len; p < pptr[t] + plen; ++p) {
+    for (int64_t o = len; o < plen; ++o) {
+      const int64_t p = pptr[t] + ilv(o, plen, C);
       brow[p] = int(b * blk_rows);
       if (vt == kF32)
         static_cast<float*>(bval)[p] = 0.0f;
@@ -1173,7 +1198,7 @@ int raptr_recon_create_ex(raptr_recon** out, int m, int n, int64_t nnz, const in
     if (m > 0) {
       hipLaunchKernelGGL(k_pad_copy, dim3(grid_for(int64_t(m) * kWave, 256, 1 << 16)), dim3(256),
                          0, st, d_rp, d_col, d_val, m, value_type, (const int64_t*)R->frp,
-                         R->fcol, R->fval, R->rscale);
+                         R->fcol, R->fval, R->rscale, max_frames == 1 ? kIlvFwd : 0);
       hipError_t le = hipGetLastError();
       if (le != hipSuccess) return bail(fail(RAPTR_ERR_DEVICE, "pad: %s", hipGetErrorString(le)));
     }
@@ -1252,10 +1277,10 @@ int raptr_recon_create_ex(raptr_recon** out, int m, int n, int64_t nnz, const in
                            (const uint64_t*)keys, (const uint32_t*)idx, nnz, n, colbits,
                            (const int64_t*)uptr, (const int64_t*)R->bptr,
                            (const uint32_t*)rowid, d_val, (const float*)R->rscale, value_type,
-                           R->brow, R->bval);
+                           R->brow, R->bval, max_frames == 1 ? kIlvBwd : 0);
         hipLaunchKernelGGL(k_bt_pad, dim3(grid_for(T, 256, 1 << 16)), dim3(256), 0, st,
                            (const int64_t*)uptr, (const int64_t*)R->bptr, T, n, block_rows,
-                           R->brow, R->bval, value_type);
+                           R->brow, R->bval, value_type, max_frames == 1 ? kIlvBwd : 0);
         // single frame with LDS-sized row blocks: packed indices for the
         // flat staged backward
         if (max_frames == 1 && block_rows <= kLdsRowsMax && block_rows % 4 == 0 &&
