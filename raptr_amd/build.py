@@ -70,8 +70,46 @@ def build(verbose=False, jobs=None):
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    if build_h5():
+        build_cpp_io_test()
     build_cpp_tests()
     return LIB
+
+
+HDF5_PREFIX = os.environ.get("RAPTR_HDF5_PREFIX", "/opt/conda")
+H5_LIB = os.path.join(LIBDIR, "libraptr_h5.so")
+
+
+def build_h5():
+    """Host library for the HDF5 measurement/density files
+    (csrc/h5io.cpp against the image's HDF5 C library); skipped when HDF5
+    is absent, in which case raptr_amd.h5io raises on use."""
+    inc = os.path.join(HDF5_PREFIX, "include")
+    lib = os.path.join(HDF5_PREFIX, "lib")
+    if not os.path.exists(os.path.join(inc, "hdf5.h")):
+        return None
+    os.makedirs(LIBDIR, exist_ok=True)
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-static-libstdc++",
+           "-static-libgcc", f"-I{os.path.join(ROOT, 'include')}", f"-I{inc}",
+           os.path.join(CSRC, "h5io.cpp"), "-o", H5_LIB + ".tmp", f"-L{lib}", "-lhdf5",
+           f"-Wl,-rpath,{lib}"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"g++ failed for h5io.cpp:\n{r.stdout}\n{r.stderr}")
+    os.replace(H5_LIB + ".tmp", H5_LIB)
+    return H5_LIB
+
+
+def build_cpp_io_test():
+    """C++ client of include/raptr/io.hpp (host only, links libraptr_h5)."""
+    src = os.path.join(ROOT, "tests", "cpp", "test_io.cpp")
+    out = os.path.join(LIBDIR, "test_io")
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", f"-I{os.path.join(ROOT, 'include')}", src,
+           "-o", out + ".tmp", f"-L{LIBDIR}", "-lraptr_h5", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"g++ failed for {src}:\n{r.stdout}\n{r.stderr}")
+    os.replace(out + ".tmp", out)
 
 
 CPP_TESTS = [os.path.join(ROOT, "tests", "cpp", "test_reference_api.cpp")]

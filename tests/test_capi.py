@@ -1,5 +1,7 @@
-"""CPU checks of the C-ABI library: it loads, exports every function that
-include/raptr_hip.h declares, and the Python declarations cover them."""
+"""CPU checks of the C-ABI libraries: each loads and exports every function
+its header declares (include/raptr_hip.h -> libraptr_hip.so,
+include/raptr_h5.h -> libraptr_h5.so), and the Python declarations cover
+them."""
 import ctypes as C
 import os
 import re
@@ -9,16 +11,10 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared_functions():
-    names = set()
-    hdr_dir = os.path.join(ROOT, "include")
-    for f in os.listdir(hdr_dir):
-        if not f.endswith(".h"):
-            continue
-        text = open(os.path.join(hdr_dir, f)).read()
-        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-        names |= set(re.findall(r"\b(raptr_[a-z0-9_]+)\s*\(", text))
-    return sorted(names)
+def declared_functions(header="raptr_hip.h"):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(raptr_[a-z0-9_]+)\s*\(", text)))
 
 
 def test_library_exports_every_declared_symbol():
@@ -26,6 +22,17 @@ def test_library_exports_every_declared_symbol():
     L = C.CDLL(_capi.LIB_PATH)
     names = declared_functions()
     assert len(names) > 30
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_h5_library_exports_every_declared_symbol():
+    from raptr_amd import h5io
+    if not os.path.exists(h5io._LIB):
+        pytest.skip("HDF5 library not built")
+    L = C.CDLL(h5io._LIB)
+    names = declared_functions("raptr_h5.h")
+    assert len(names) >= 6
     missing = [n for n in names if not hasattr(L, n)]
     assert not missing, missing
 
