@@ -46,10 +46,10 @@ ERR_ITERS = 30  # iterations of the mixed-precision error figures
 # reads back) for the kernel shapes the autotuner picks on this workload;
 # other shapes report no traffic figure.
 MEASURED_TRAFFIC = {
-    ("k_fwd1", 32, 2): 2 * 7342510 * 1024 + 17622 * 1024,
+    ("k_fwd1", 32, 2, "list"): 2 * 7342510 * 1024 + 17622 * 1024,
     # backward: unroll 4, 9 row blocks, 1 block per launch, not LDS-staged
-    # (average over the 9 launches of one pass)
-    ("k_bwd", 4, 9, 1, 0): 2 * 734223 * 1024 + 1427 * 1024,
+    # (average over the 9 launches of one pass), rows in list order
+    ("k_bwd", 4, 9, 1, 0, "list"): 2 * 734223 * 1024 + 1427 * 1024,
 }
 
 
@@ -73,6 +73,9 @@ def parse():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serving-requests", type=int, default=120)
     ap.add_argument("--serving-rate", type=float, default=20.0, help="arrivals per second")
+    ap.add_argument("--row-order", default="locality", choices=("locality", "list", "both"),
+                    help="plan row order: locality (angle, midpoint, slope), the measurement "
+                         "list's, or try both")
     ap.add_argument("--block-slices-mb", default="8,32",
                     help="ratio-slice sizes (MB) of the batched plans' row blocks to try")
     return ap.parse_args()
@@ -152,8 +155,10 @@ def main():
     modes = {}
     plans = {}
     wanted = args.modes.split(",")
+    orders = ["locality", "list"] if args.row_order == "both" else [args.row_order]
+    d_order = sysmat.locality_order(setup, d_list, stream=st) if "locality" in orders else None
     slices_mb = [float(v) for v in args.block_slices_mb.split(",")]
-    best_rows = None
+    best_rows = best_order = None
     for name, vt, nf in (("plain", "f32", 1), ("batched", "f32", F), ("mixed_u16", "u16", F),
                          ("mixed_f16", "f16", F), ("mixed_bf16", "bf16", F)):
         if name not in wanted:
@@ -172,25 +177,29 @@ def main():
         else:
             rows_try = [("blocked", max(4, int(mb * 2**20 / (4 * nf)) // 4 * 4))
                         for mb in slices_mb]
+        rows_try = [(lay, rows, o) for lay, rows in rows_try
+                    for o in (orders if nf == 1 or best_order is None else [best_order])]
         R, t_plan, cfg, ms_it = None, 0.0, None, None
-        for layout, rows in rows_try:
+        for layout, rows, order in rows_try:
             t = time.perf_counter()
-            Rc = Reconstructor(A, sens, vt, nf, block_rows=rows, stream=st, layout=layout)
+            Rc = Reconstructor(A, sens, vt, nf, block_rows=rows, stream=st, layout=layout,
+                               row_order=d_order if order == "locality" else None)
             tp = time.perf_counter() - t
             cc, mc = Rc.autotune(dy, dx, nf, reps=2, stream=st)
             if len(rows_try) > 1:
                 log(f"{name}: {layout} layout, block_rows {rows} "
-                    f"({Rc.info()['row_blocks']} blocks) -> {mc:.2f} ms/iteration")
+                    f"({Rc.info()['row_blocks']} blocks), {order} row order "
+                    f"-> {mc:.2f} ms/iteration")
             if ms_it is None or mc < ms_it:
                 if R is not None:
                     R.close()
                 R, t_plan, cfg, ms_it = Rc, tp, cc, mc
-                chosen_rows = rows
-                cfg = dict(cfg, layout=layout)
+                chosen_rows, chosen_order = rows, order
+                cfg = dict(cfg, layout=layout, row_order=order)
             else:
                 Rc.close()
         if nf > 1:
-            best_rows = chosen_rows
+            best_rows, best_order = chosen_rows, chosen_order
         plans[name] = (R, dy, dx, xx0, nf)
         for _ in range(args.warmup):
             dx.copy_from(xx0, stream=st)
@@ -310,9 +319,9 @@ def main():
     else:
         nl_f = 1
         nl_b = 1 if info["bwd_staged"] else -(-nb // cfg_p["bwd_blocks_per_launch"])
-        key_f = ("k_fwd1", cfg_p["fwd_lanes"], cfg_p["fwd_unroll"])
+        key_f = ("k_fwd1", cfg_p["fwd_lanes"], cfg_p["fwd_unroll"], cfg_p["row_order"])
         key_b = ("k_bwd", cfg_p["bwd_unroll"], nb, cfg_p["bwd_blocks_per_launch"],
-                 cfg_p["bwd_staged"])
+                 cfg_p["bwd_staged"], cfg_p["row_order"])
     dom = {"kernel": fwd_name, "ms": ms_f, "bytes": bytes_f, "gbs": gbs_f, "launches": nl_f,
            "key": key_f}
     if ms_b > ms_f:

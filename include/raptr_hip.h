@@ -324,6 +324,28 @@ int raptr_recon_create_ex(raptr_recon** plan, int m, int n, int64_t nnz,
                           const float* d_val, const double* d_sens, int value_type,
                           int max_frames, int block_rows, int layout,
                           raptr_stream_t stream);
+/* A plan with a row order of its own: plan row p holds the caller's row
+ * d_row_order[p] (device; a permutation of 0..m-1, checked; null: the
+ * caller's order).  The interface keeps the caller's row order: the plan
+ * permutes y on entry to raptr_recon_mlem / raptr_recon_forward and e on
+ * the way out, and no row's sum changes.  An order that puts the rows
+ * through one voxel next to each other (raptr_locality_order) lets the
+ * backward's ratio gather touch fewer cache lines.  No counterpart in the
+ * reference, which keeps the measurement list order (reco.cu:230-303). */
+int raptr_recon_create_ordered(raptr_recon** plan, int m, int n, int64_t nnz,
+                               const int64_t* d_row_ptr, const int* d_col,
+                               const float* d_val, const double* d_sens, int value_type,
+                               int max_frames, int block_rows, int layout,
+                               const int* d_row_order, raptr_stream_t stream);
+/* 1 if the plan keeps its own row order. */
+int raptr_recon_row_ordered(const raptr_recon* plan);
+/* Row order for a two-head setup: the channels d_channels[0..m) sorted
+ * (stably) by angle, then the line's midpoint between the heads
+ * (id0z + id1z, id0y + id1y), then its slope (id1z - id0z, id1y - id0y);
+ * d_order[p] = position in d_channels.  Allocates its own scratch (about 26
+ * bytes per channel). */
+int raptr_locality_order(const raptr_setup_t* setup, const int* d_channels, int m,
+                         int* d_order, raptr_stream_t stream);
 int raptr_recon_layout(const raptr_recon* plan);
 /* Tiled layout kernel shape: lanes per row segment (4, 8, 16) and unroll
  * (1, 2, 4) of the forward and the backward. */

@@ -1099,6 +1099,99 @@ __global__ void k_frame_gather(const float* a, int64_t rows, int F, int slot, fl
     out[i] = a[i * F + slot];
 }
 
+// ---------------------------------------------------- plan row order
+// A plan may keep its rows in an order of its own (plan row p holds the
+// caller's row order[p]).  Row order changes which rows share cache lines in
+// the backward's ratio gather; it changes no row's sum.
+
+// seen[] counts how often each caller row occurs; *bad = 1 on a miss
+__global__ void k_perm_check(const int* __restrict__ order, int m, int* __restrict__ seen,
+                             int* __restrict__ bad) {
+  for (int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; p < m;
+       p += int64_t(gridDim.x) * blockDim.x) {
+    const int o = order[p];
+    if (o < 0 || o >= m || atomicAdd(&seen[o], 1) != 0) bad[0] = 1;
+  }
+}
+
+__global__ void k_perm_len(const int64_t* __restrict__ rp, const int* __restrict__ order,
+                           int m, int64_t* __restrict__ plen) {
+  for (int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; p < m;
+       p += int64_t(gridDim.x) * blockDim.x) {
+    const int o = order[p];
+    plen[p] = rp[o + 1] - rp[o];
+  }
+}
+
+// one wave per plan row copies the caller row's entries in their order
+__global__ __launch_bounds__(256) void k_perm_copy(const int64_t* __restrict__ rp,
+                                                   const int* __restrict__ col,
+                                                   const float* __restrict__ val,
+                                                   const int* __restrict__ order, int m,
+                                                   const int64_t* __restrict__ prp,
+                                                   int* __restrict__ pcol,
+                                                   float* __restrict__ pval) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t waves = int64_t(gridDim.x) * (blockDim.x / kWave);
+  for (int64_t p = int64_t(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave; p < m;
+       p += waves) {
+    const int o = order[p];
+    const int64_t b = rp[o], len = rp[o + 1] - b, d = prp[p];
+    for (int64_t k = lane; k < len; k += kWave) {
+      pcol[d + k] = col[b + k];
+      pval[d + k] = val[b + k];
+    }
+  }
+}
+
+// dst[p][f] = src[order[p]][f]  (caller rows -> plan rows)
+__global__ void k_rows_gather(const float* __restrict__ src, float* __restrict__ dst,
+                              const int* __restrict__ order, int64_t m, int F) {
+  const int64_t total = m * F;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t p = i / F;
+    dst[i] = src[int64_t(order[p]) * F + (i - p * F)];
+  }
+}
+
+// dst[order[p]][f] = src[p][f]  (plan rows -> caller rows)
+__global__ void k_rows_scatter(const float* __restrict__ src, float* __restrict__ dst,
+                               const int* __restrict__ order, int64_t m, int F) {
+  const int64_t total = m * F;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t p = i / F;
+    dst[int64_t(order[p]) * F + (i - p * F)] = src[i];
+  }
+}
+
+// Locality key of a two-head channel: angle, then the line's midpoint
+// between the heads (z0 + z1, y0 + y1), then its slope (z1 - z0, y1 - y0).
+// Lines through one voxel then sit in a few runs of consecutive keys.
+__global__ void k_locality_keys(raptr_setup_t s, const int* __restrict__ cnl, int m,
+                                uint64_t* __restrict__ keys, uint32_t* __restrict__ idx) {
+  const uint64_t zs = uint64_t(s.n0z + s.n1z - 1), ys = uint64_t(s.n0y + s.n1y - 1);
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < m;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    int ida, id0z, id0y, id1z, id1y;
+    cnl_split(s, cnl[i], ida, id0z, id0y, id1z, id1y);
+    uint64_t k = uint64_t(ida);
+    k = k * zs + uint64_t(id0z + id1z);
+    k = k * ys + uint64_t(id0y + id1y);
+    k = k * zs + uint64_t(id1z - id0z + s.n0z - 1);
+    k = k * ys + uint64_t(id1y - id0y + s.n0y - 1);
+    keys[i] = k;
+    idx[i] = uint32_t(i);
+  }
+}
+
+__global__ void k_u32_to_i32(const uint32_t* __restrict__ a, int m, int* __restrict__ b) {
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < m;
+       i += int64_t(gridDim.x) * blockDim.x)
+    b[i] = int(a[i]);
+}
+
 // frame vectors are loaded with 4*min(F,4)-byte vector accesses
 bool frames_aligned(const void* p, int F) {
   const uintptr_t a = uintptr_t(4) * uintptr_t(F >= 4 ? 4 : F);
@@ -1323,6 +1416,150 @@ int raptr_recon_create_ex(raptr_recon** out, int m, int n, int64_t nnz, const in
   return RAPTR_OK;
 }
 
+int raptr_recon_create_ordered(raptr_recon** out, int m, int n, int64_t nnz,
+                               const int64_t* d_rp, const int* d_col, const float* d_val,
+                               const double* d_sens, int value_type, int max_frames,
+                               int block_rows, int layout, const int* d_row_order,
+                               raptr_stream_t stream) {
+  if (!d_row_order)
+    return raptr_recon_create_ex(out, m, n, nnz, d_rp, d_col, d_val, d_sens, value_type,
+                                 max_frames, block_rows, layout, stream);
+  if (!out) return fail(RAPTR_ERR_INVALID_ARG, "null output");
+  *out = nullptr;
+  if (m < 0 || n <= 0 || nnz < 0) return fail(RAPTR_ERR_INVALID_ARG, "bad sizes");
+  if (!d_rp || (nnz > 0 && (!d_col || !d_val))) return fail(RAPTR_ERR_INVALID_ARG, "null buffer");
+  hipStream_t st = S(stream);
+  // scratch: seen[m] | bad | permuted row pointers | lengths | scan total |
+  // scan scratch | permuted columns | permuted values
+  Sizer z;
+  z.take<int>(size_t(m) + 1);
+  z.take<int>(4);
+  z.take<int64_t>(size_t(m) + 1);
+  z.take<int64_t>(size_t(m) + 1);
+  z.take<int64_t>(4);
+  const size_t sb = scan_workspace_bytes(m > 0 ? m : 1);
+  z.take<char>(sb);
+  z.take<int>(size_t(nnz) + 1);
+  z.take<float>(size_t(nnz) + 1);
+  void* tmp = nullptr;
+  hipError_t ae = hipMalloc(&tmp, z.bytes());
+  if (ae != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(RAPTR_ERR_OUT_OF_MEMORY, "row order scratch (%zu B): %s", z.bytes(),
+                hipGetErrorString(ae));
+  }
+  Carver cv(tmp, z.bytes());
+  int* seen = cv.take<int>(size_t(m) + 1);
+  int* bad = cv.take<int>(4);
+  int64_t* prp = cv.take<int64_t>(size_t(m) + 1);
+  int64_t* plen = cv.take<int64_t>(size_t(m) + 1);
+  int64_t* tot = cv.take<int64_t>(4);
+  char* sws = cv.take<char>(sb);
+  int* pcol = cv.take<int>(size_t(nnz) + 1);
+  float* pval = cv.take<float>(size_t(nnz) + 1);
+  int rc = RAPTR_OK;
+  int hbad = 0;
+  (void)hipMemsetAsync(seen, 0, (size_t(m) + 1) * sizeof(int), st);
+  (void)hipMemsetAsync(bad, 0, 4 * sizeof(int), st);
+  (void)hipMemsetAsync(prp, 0, sizeof(int64_t), st);
+  if (m > 0) {
+    hipLaunchKernelGGL(k_perm_check, dim3(grid_for(m, 256, 8192)), dim3(256), 0, st,
+                       d_row_order, m, seen, bad);
+  }
+  hipError_t e = hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) {
+    rc = fail(RAPTR_ERR_DEVICE, "row order check: %s", hipGetErrorString(e));
+  } else if (hbad) {
+    rc = fail(RAPTR_ERR_INVALID_ARG, "row order is not a permutation of 0..%d", m - 1);
+  } else if (m > 0) {
+    hipLaunchKernelGGL(k_perm_len, dim3(grid_for(m, 256, 8192)), dim3(256), 0, st, d_rp,
+                       d_row_order, m, plen);
+    rc = scan_exclusive_i64(plen, prp, m, tot, sws, sb, st);
+    if (!rc) {
+      hipLaunchKernelGGL(k_set_last, dim3(1), dim3(1), 0, st, prp, m, (const int64_t*)tot);
+      hipLaunchKernelGGL(k_perm_copy, dim3(grid_for(int64_t(m) * kWave, 256, 1 << 16)),
+                         dim3(256), 0, st, d_rp, d_col, d_val, d_row_order, m,
+                         (const int64_t*)prp, pcol, pval);
+      hipError_t le = hipGetLastError();
+      if (le != hipSuccess) rc = fail(RAPTR_ERR_DEVICE, "row order: %s", hipGetErrorString(le));
+    }
+  }
+  if (!rc)
+    rc = raptr_recon_create_ex(out, m, n, nnz, prp, pcol, pval, d_sens, value_type, max_frames,
+                               block_rows, layout, stream);
+  (void)hipStreamSynchronize(st);
+  (void)hipFree(tmp);
+  if (rc) return rc;
+  raptr_recon* R = *out;
+  if ((rc = ralloc(R, &R->rperm, size_t(m) + 1)) ||
+      (rc = ralloc(R, &R->yp, size_t(m) * size_t(R->fmax) + 4))) {
+    raptr_recon_destroy(R);
+    *out = nullptr;
+    return rc;
+  }
+  (void)hipMemcpyAsync(R->rperm, d_row_order, size_t(m) * sizeof(int), hipMemcpyDeviceToDevice,
+                       st);
+  e = hipStreamSynchronize(st);
+  if (e != hipSuccess) {
+    raptr_recon_destroy(R);
+    *out = nullptr;
+    return fail(RAPTR_ERR_DEVICE, "row order copy: %s", hipGetErrorString(e));
+  }
+  return RAPTR_OK;
+}
+
+int raptr_recon_row_ordered(const raptr_recon* R) { return R && R->rperm ? 1 : 0; }
+
+int raptr_locality_order(const raptr_setup_t* setup, const int* d_channels, int m,
+                         int* d_order, raptr_stream_t stream) {
+  if (!setup || m < 0 || (m > 0 && (!d_channels || !d_order)))
+    return fail(RAPTR_ERR_INVALID_ARG, "bad locality order arguments");
+  if (m == 0) return RAPTR_OK;
+  const raptr_setup_t s = *setup;
+  if (s.na < 1 || s.n0z < 1 || s.n0y < 1 || s.n1z < 1 || s.n1y < 1)
+    return fail(RAPTR_ERR_INVALID_ARG, "bad setup");
+  const double kmax = double(s.na) * (s.n0z + s.n1z - 1) * (s.n0y + s.n1y - 1) *
+                      (s.n0z + s.n1z - 1) * (s.n0y + s.n1y - 1);
+  if (kmax >= 1.8e19) return fail(RAPTR_ERR_INVALID_ARG, "setup too large for 64-bit keys");
+  int bits = 1;
+  while (bits < 64 && double(uint64_t(1) << bits) < kmax) ++bits;
+  hipStream_t st = S(stream);
+  const int64_t ntiles = (int64_t(m) + 2047) / 2048;
+  Sizer z;
+  z.take<uint64_t>(size_t(m));
+  z.take<uint64_t>(size_t(m));
+  z.take<uint32_t>(size_t(m));
+  z.take<uint32_t>(size_t(m));
+  z.take<uint32_t>(size_t(ntiles) * 256);
+  z.take<uint32_t>(size_t(ntiles) * 256);
+  z.take<char>(scan_workspace_bytes(ntiles * 256));
+  void* tmp = nullptr;
+  hipError_t ae = hipMalloc(&tmp, z.bytes() + 4096);
+  if (ae != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(RAPTR_ERR_OUT_OF_MEMORY, "locality order scratch: %s", hipGetErrorString(ae));
+  }
+  Carver cv(tmp, z.bytes() + 4096);
+  uint64_t* k0 = cv.take<uint64_t>(size_t(m));
+  uint64_t* k1 = cv.take<uint64_t>(size_t(m));
+  uint32_t* v0 = cv.take<uint32_t>(size_t(m));
+  uint32_t* v1 = cv.take<uint32_t>(size_t(m));
+  hipLaunchKernelGGL(k_locality_keys, dim3(grid_for(m, 256, 8192)), dim3(256), 0, st, s,
+                     d_channels, m, k0, v0);
+  int res = 0;
+  int rc = radix_sort_pairs(k0, k1, v0, v1, m, bits, st, &res, cv);
+  if (!rc) {
+    hipLaunchKernelGGL(k_u32_to_i32, dim3(grid_for(m, 256, 8192)), dim3(256), 0, st,
+                       (const uint32_t*)(res ? v1 : v0), m, d_order);
+    hipError_t le = hipGetLastError();
+    if (le != hipSuccess) rc = fail(RAPTR_ERR_DEVICE, "locality order: %s", hipGetErrorString(le));
+  }
+  (void)hipStreamSynchronize(st);
+  (void)hipFree(tmp);
+  return rc;
+}
+
 int raptr_recon_destroy(raptr_recon* R) {
   if (!R) return RAPTR_OK;
   for (auto& kv : R->graphs) {
@@ -1485,6 +1722,14 @@ int raptr_recon_mlem(raptr_recon* R, int nframes, const float* d_y, float* d_x, 
     return fail(RAPTR_ERR_INVALID_ARG, "y and x must be aligned to %d bytes for %d frames",
                 4 * (nframes >= 4 ? 4 : nframes), nframes);
   hipStream_t st = S(stream);
+  if (R->rperm && R->m > 0) {
+    // counts into plan row order once per call (outside any captured graph)
+    hipLaunchKernelGGL(k_rows_gather, dim3(grid_for(int64_t(R->m) * nframes, 256, 1 << 15)),
+                       dim3(256), 0, st, d_y, R->yp, (const int*)R->rperm, int64_t(R->m),
+                       nframes);
+    RAPTR_LAUNCH_CHECK();
+    d_y = R->yp;
+  }
   if (!use_graph || niter < 2 || !st) {
     for (int it = 0; it < niter; ++it) RAPTR_TRY(iteration(R, nframes, d_y, d_x, st));
     return RAPTR_OK;
@@ -1525,8 +1770,23 @@ int raptr_recon_forward(raptr_recon* R, int nframes, const float* d_x, const flo
   if (!frames_aligned(d_x, nframes) || !frames_aligned(d_y, nframes) ||
       !frames_aligned(d_e, nframes))
     return fail(RAPTR_ERR_INVALID_ARG, "x, y and e must be aligned to the frame vector");
-  if (R->layout == RAPTR_LAYOUT_TILED) return tiled_forward(R, d_x, d_y, d_e, S(stream));
-  return launch_fwd_any(R, nframes, d_x, d_y, d_e, S(stream));
+  hipStream_t st = S(stream);
+  if (R->rperm && R->m > 0) {
+    // caller rows -> plan rows, the pass into the plan's ratio buffer, back
+    const unsigned g = grid_for(int64_t(R->m) * nframes, 256, 1 << 15);
+    hipLaunchKernelGGL(k_rows_gather, dim3(g), dim3(256), 0, st, d_y, R->yp,
+                       (const int*)R->rperm, int64_t(R->m), nframes);
+    RAPTR_LAUNCH_CHECK();
+    RAPTR_TRY(R->layout == RAPTR_LAYOUT_TILED
+                  ? tiled_forward(R, d_x, R->yp, R->e, st)
+                  : launch_fwd_any(R, nframes, d_x, R->yp, R->e, st));
+    hipLaunchKernelGGL(k_rows_scatter, dim3(g), dim3(256), 0, st, (const float*)R->e, d_e,
+                       (const int*)R->rperm, int64_t(R->m), nframes);
+    RAPTR_LAUNCH_CHECK();
+    return RAPTR_OK;
+  }
+  if (R->layout == RAPTR_LAYOUT_TILED) return tiled_forward(R, d_x, d_y, d_e, st);
+  return launch_fwd_any(R, nframes, d_x, d_y, d_e, st);
 }
 
 int raptr_recon_backward(raptr_recon* R, int nframes, const float* d_x, raptr_stream_t stream) {

@@ -123,6 +123,10 @@ struct raptr_recon {
   float* tpart = nullptr;      // ntile*m partial row sums of the forward
   float* cpart = nullptr;      // ntile*tile_wgs*tile_cols partial column sums
   int t_fwd_lanes = 8, t_fwd_unroll = 2, t_bwd_lanes = 8, t_bwd_unroll = 2;
+  // plan row order: plan row p holds the caller's row rperm[p] (null:
+  // identity); yp holds the counts in plan order
+  int* rperm = nullptr;
+  float* yp = nullptr;
   size_t bytes = 0;
   std::vector<void*> allocs;
   std::map<long long, std::pair<hipGraph_t, hipGraphExec_t>> graphs;

@@ -16,6 +16,20 @@ VALUE_TYPES = {"f32": 0, "f16": 1, "bf16": 2, "u16": 3}
 LAYOUTS = {"blocked": 0, "tiled": 1}
 
 
+def _row_order(row_order, A, stream):
+    if row_order is None:
+        return None
+    if isinstance(row_order, DeviceArray):
+        return row_order
+    if isinstance(row_order, tuple):
+        from . import sysmat
+        return sysmat.locality_order(row_order[0], row_order[1], stream=stream)
+    if hasattr(row_order, "nchannels"):
+        from . import sysmat
+        return sysmat.locality_order(row_order, A.channels, stream=stream)
+    return DeviceArray.from_numpy(np.ascontiguousarray(row_order, np.int32), stream=stream)
+
+
 class Reconstructor:
     """layout: "blocked" (row-padded CSR + row-blocked transpose, any frame
     count; the faster one), "tiled" (column tiles through LDS, one frame,
@@ -24,18 +38,27 @@ class Reconstructor:
     width."""
 
     def __init__(self, A, sens, value_type="f32", max_frames=1, block_rows=0, stream=None,
-                 layout="auto"):
+                 layout="auto", row_order=None):
+        """row_order: None (the CSR's row order), a device or host int32
+        permutation (plan row p = CSR row row_order[p]), or a
+        ``MeasurementSetup`` with the CSR's channel ids given as
+        ``A.channels`` / a (setup, channels) pair, from which the locality
+        order (sysmat.locality_order) is computed.  Counts, images and the
+        forward's ratios keep the CSR's row order at the interface."""
         self.m, self.n, self.nnz = A.m, A.n, A.nnz
         self.value_type = value_type
         self.max_frames = max_frames
         if layout == "auto":
             layout = "blocked"
         self.layout = layout
+        order = _row_order(row_order, A, stream)
         p = C.c_void_p()
-        capi.call("raptr_recon_create_ex", C.byref(p), A.m, A.n, A.nnz, A.row_ptr.ptr,
+        capi.call("raptr_recon_create_ordered", C.byref(p), A.m, A.n, A.nnz, A.row_ptr.ptr,
                   A.col.ptr, A.val.ptr, sens.ptr, VALUE_TYPES[value_type], int(max_frames),
-                  int(block_rows), LAYOUTS[layout], stream)
+                  int(block_rows), LAYOUTS[layout], order.ptr if order is not None else None,
+                  stream)
         self.handle = p.value
+        self.row_ordered = bool(capi.lib().raptr_recon_row_ordered(self.handle))
 
     def info(self):
         b, p = C.c_int64(), C.c_int64()
@@ -50,7 +73,8 @@ class Reconstructor:
                "bwd_unroll": bu.value, "bwd_staged": st.value,
                "bwd_lanes_per_column": lpc.value, "bwd_workgroups": wgs.value,
                "bwd_blocks_per_launch": capi.lib().raptr_recon_get_blocks_per_launch(self.handle),
-               "transpose_nnz": capi.lib().raptr_recon_transpose_nnz(self.handle)}
+               "transpose_nnz": capi.lib().raptr_recon_transpose_nnz(self.handle),
+               "row_ordered": bool(capi.lib().raptr_recon_row_ordered(self.handle))}
         if self.layout == "tiled":
             t = [C.c_int() for _ in range(6)]
             capi.call("raptr_recon_get_tiled", self.handle, *[C.byref(v) for v in t])

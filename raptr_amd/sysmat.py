@@ -97,6 +97,7 @@ class DeviceCSR:
     row_ptr: DeviceArray
     col: DeviceArray
     val: DeviceArray
+    channels: object = None  # device list of the rows' channel ids, if known
 
     def to_host(self):
         return (self.row_ptr.numpy(self.m + 1), self.col.numpy(self.nnz),
@@ -143,7 +144,7 @@ def build_csr(setup, grid, cnl_list, nrays, seed=1, raygen=capi.RAYGEN_RAND, str
                                       int(seed), int(raygen), rp.ptr, col.ptr, val.ptr,
                                       int(capacity), C.byref(nnz), ws.ptr, ws.nbytes, stream)
             if st == capi.RAPTR_OK:
-                return DeviceCSR(m, grid.nvox, nnz.value, rp, col, val)
+                return DeviceCSR(m, grid.nvox, nnz.value, rp, col, val, d_list)
             if st != capi.RAPTR_ERR_WORKSPACE:
                 capi.check(st, "raptr_sm_build_csr")
             del col, val
@@ -158,7 +159,22 @@ def build_csr(setup, grid, cnl_list, nrays, seed=1, raygen=capi.RAYGEN_RAND, str
     capi.call("raptr_sm_build_csr", C.byref(s), C.byref(g), d_list.ptr, m, int(nrays),
               int(seed), int(raygen), rp.ptr, col.ptr, val.ptr, int(capacity), C.byref(nnz),
               ws.ptr, ws.nbytes, stream)
-    return DeviceCSR(m, grid.nvox, nnz.value, rp, col, val)
+    return DeviceCSR(m, grid.nvox, nnz.value, rp, col, val, d_list)
+
+
+def locality_order(setup, channels, stream=None):
+    """Plan row order that puts lines through the same voxels next to each
+    other (raptr_locality_order): positions into ``channels`` sorted by
+    angle, midpoint and slope.  Returns a device int32 array."""
+    if isinstance(channels, DeviceArray):
+        d_ch, m = channels, channels.count
+    else:
+        lst = np.ascontiguousarray(channels, np.int32)
+        d_ch, m = DeviceArray.from_numpy(lst, stream=stream), len(lst)
+    out = DeviceArray(max(m, 1), np.int32)
+    s = setup.struct()
+    capi.call("raptr_locality_order", C.byref(s), d_ch.ptr, int(m), out.ptr, stream)
+    return out
 
 
 def sensitivity(setup, grid, nrays, seed=1, raygen=capi.RAYGEN_RAND, stream=None):

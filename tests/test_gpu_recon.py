@@ -248,3 +248,68 @@ def test_recon_rejects_bad_arguments(gpu, small_system):
     R = Reconstructor(S["A"], S["sens"], "f32", 2)
     with pytest.raises(RaptrError):
         R.mlem(np.zeros((S["A"].m, 4), np.float32), np.zeros((S["grid"].nvox, 4), np.float32), 1)
+
+
+def _locality_keys(setup, ch):
+    ida, z0, y0, z1, y1 = (np.asarray(v, np.int64) for v in setup.split(ch))
+    zs, ys = setup.n0z + setup.n1z - 1, setup.n0y + setup.n1y - 1
+    k = ida
+    for v, r in ((z0 + z1, zs), (y0 + y1, ys), (z1 - z0 + setup.n0z - 1, zs),
+                 (y1 - y0 + setup.n0y - 1, ys)):
+        k = k * r + v
+    return k
+
+
+def test_locality_order_is_a_stable_sort(gpu, small_system):
+    """raptr_locality_order: a permutation of the rows, sorted by (angle,
+    midpoint, slope) keys computed independently here, ties in list order."""
+    from raptr_amd import sysmat
+    S = small_system
+    order = sysmat.locality_order(S["setup"], S["meas"]).numpy()
+    assert np.array_equal(np.sort(order), np.arange(len(S["meas"])))
+    want = np.argsort(_locality_keys(S["setup"], S["meas"]), kind="stable")
+    np.testing.assert_array_equal(order, want)
+
+
+@pytest.mark.parametrize("layout,F", [("blocked", 1), ("blocked", 4), ("tiled", 1)])
+def test_recon_row_ordered_plan(gpu, oracle, small_system, layout, F):
+    """A plan in locality row order: the forward's ratios come back in the
+    caller's row order bit for bit (no row's sum changes), and MLEM matches
+    the caller-ordered plan to FP32 rounding and the FP64 oracle to 1e-4."""
+    from raptr_amd import _capi as capi
+    from raptr_amd.device import DeviceArray
+    from raptr_amd.recon import Reconstructor
+    S = small_system
+    A, n, m = S["A"], S["grid"].nvox, S["A"].m
+    R0 = Reconstructor(A, S["sens"], "f32", F, layout=layout)
+    R1 = Reconstructor(A, S["sens"], "f32", F, layout=layout, row_order=S["setup"])
+    assert R1.row_ordered and not R0.row_ordered
+    rng = np.random.default_rng(5)
+    x = rng.random((n, F)).astype(np.float32)
+    y = np.repeat(S["y"].astype(np.float32)[:, None], F, 1) * rng.random((1, F)).astype(np.float32)
+    dx, dy = DeviceArray.from_numpy(x), DeviceArray.from_numpy(y)
+    es = []
+    for R in (R0, R1):
+        de = DeviceArray(m * F, np.float32)
+        capi.call("raptr_recon_forward", R.handle, F, dx.ptr, dy.ptr, de.ptr, None)
+        es.append(de.numpy())
+    np.testing.assert_array_equal(es[1], es[0])
+    x0 = np.full((n, F), 1.0 / n, np.float32)
+    a = R0.mlem(y if F > 1 else y[:, 0], x0 if F > 1 else x0[:, 0], 5)
+    b = R1.mlem(y if F > 1 else y[:, 0], x0 if F > 1 else x0[:, 0], 5)
+    assert _rel(b, a) <= 1e-6
+    x64 = _oracle_mlem(oracle, S, y[:, 0].astype(np.float64), 5)
+    assert _rel((b if F == 1 else b[:, 0]).astype(np.float64), x64) <= 1e-4
+
+
+def test_recon_row_order_must_be_a_permutation(gpu, small_system):
+    from raptr_amd import RaptrError
+    from raptr_amd.recon import Reconstructor
+    S = small_system
+    m = S["A"].m
+    bad = np.arange(m, dtype=np.int32)
+    bad[1] = 0
+    with pytest.raises(RaptrError, match="permutation"):
+        Reconstructor(S["A"], S["sens"], "f32", 1, row_order=bad)
+    with pytest.raises(RaptrError, match="permutation"):
+        Reconstructor(S["A"], S["sens"], "f32", 1, row_order=np.arange(1, m + 1, dtype=np.int32))
