@@ -76,6 +76,8 @@ def parse():
     ap.add_argument("--row-order", default="locality", choices=("locality", "list", "both"),
                     help="plan row order: locality (angle, midpoint, slope), the measurement "
                          "list's, or try both")
+    ap.add_argument("--group", type=int, default=16, choices=(4, 8, 16),
+                    help="rows / voxels per group of the 64-frame grouped layouts")
     ap.add_argument("--block-slices-mb", default="8,32",
                     help="ratio-slice sizes (MB) of the batched plans' row blocks to try")
     return ap.parse_args()
@@ -183,7 +185,8 @@ def main():
         for layout, rows, order in rows_try:
             t = time.perf_counter()
             Rc = Reconstructor(A, sens, vt, nf, block_rows=rows, stream=st, layout=layout,
-                               row_order=d_order if order == "locality" else None)
+                               row_order=d_order if order == "locality" else None,
+                               grid=grid, group=args.group)
             tp = time.perf_counter() - t
             cc, mc = Rc.autotune(dy, dx, nf, reps=2, stream=st)
             if len(rows_try) > 1:

@@ -324,14 +324,30 @@ int raptr_recon_create_ex(raptr_recon** plan, int m, int n, int64_t nnz,
                           const float* d_val, const double* d_sens, int value_type,
                           int max_frames, int block_rows, int layout,
                           raptr_stream_t stream);
-/* A plan with a row order of its own: plan row p holds the caller's row
+/* Plan options.  d_row_order: plan row p holds the caller's row
  * d_row_order[p] (device; a permutation of 0..m-1, checked; null: the
  * caller's order).  The interface keeps the caller's row order: the plan
  * permutes y on entry to raptr_recon_mlem / raptr_recon_forward and e on
  * the way out, and no row's sum changes.  An order that puts the rows
  * through one voxel next to each other (raptr_locality_order) lets the
- * backward's ratio gather touch fewer cache lines.  No counterpart in the
- * reference, which keeps the measurement list order (reco.cu:230-303). */
+ * gathers of both passes touch fewer cache lines.  grid: the voxel grid of
+ * the columns (n = nx*ny*nz), so that the backward groups of the 64-frame
+ * grouped layouts are bricks of voxels (null: runs of consecutive column
+ * ids).  group: members per group of those layouts (4, 8, 16; 0 = 16).
+ * No counterpart in the reference, which keeps the measurement list order
+ * and computes per chunk (reco.cu:230-303). */
+typedef struct raptr_recon_options {
+  int layout;
+  const int* d_row_order;
+  const raptr_grid_t* grid;
+  int group;
+} raptr_recon_options;
+int raptr_recon_create_opts(raptr_recon** plan, int m, int n, int64_t nnz,
+                            const int64_t* d_row_ptr, const int* d_col, const float* d_val,
+                            const double* d_sens, int value_type, int max_frames,
+                            int block_rows, const raptr_recon_options* options,
+                            raptr_stream_t stream);
+/* raptr_recon_create_opts with a row order only. */
 int raptr_recon_create_ordered(raptr_recon** plan, int m, int n, int64_t nnz,
                                const int64_t* d_row_ptr, const int* d_col,
                                const float* d_val, const double* d_sens, int value_type,
@@ -387,6 +403,24 @@ int raptr_recon_get_blocks_per_launch(const raptr_recon* plan);
  * of an entry slot (on != 0), or the general kernels (0). */
 int raptr_recon_set_broadcast(raptr_recon* plan, int on);
 int raptr_recon_get_broadcast(const raptr_recon* plan);
+/* 64-frame plans (max_frames = 64) also hold grouped layouts: groups of G
+ * consecutive plan rows (forward) and of G voxels (a brick of the options'
+ * grid, else consecutive columns) per row block (backward), each stored as
+ * one ascending union list with G values per union entry (0 where a member
+ * has no entry).  One gathered 256-byte frame vector then serves G rows or
+ * G voxels.  on: use them for 64-frame passes (the default when present);
+ * unroll: union entries per step of a wave; kind: 0 = vector FMAs with the
+ * keys and values in scalar registers, 1 = the same with the next step's
+ * loads issued behind the current step's gathers (two register sets), 2 =
+ * matrix cores (G = 16 only: v_mfma_f32_16x16x4f32 over 4 union entries x
+ * 16 frames, keys and values from coalesced vector loads).  Shapes per G:
+ * 4: unroll 8 kind 1 or 16 kind 0; 8: 4 kind 1 or 8 kind 0; 16: 2 kind 1,
+ * 4 kind 0, 8 or 16 kind 2 (the default).  get: group = G (0: no grouped
+ * layouts), fwd/bwd_entries = union entries of each layout (lists padded to
+ * multiples of 16). */
+int raptr_recon_set_grouped(raptr_recon* plan, int on, int unroll, int kind);
+int raptr_recon_get_grouped(const raptr_recon* plan, int* group, int* on, int* unroll,
+                            int* kind, int64_t* fwd_entries, int64_t* bwd_entries);
 /* Entries of the padded row-blocked transpose (0 for the tiled layout). */
 int64_t raptr_recon_transpose_nnz(const raptr_recon* plan);
 /* niter MLEM iterations (reco.cu:230-303) of nframes frames at once

@@ -36,6 +36,11 @@ class raptr_grid_t(C.Structure):
     _fields_ = [("o", C.c_float * 3), ("d", C.c_float * 3), ("n", C.c_int * 3)]
 
 
+class raptr_recon_options(C.Structure):
+    _fields_ = [("layout", C.c_int), ("d_row_order", C.c_void_p),
+                ("grid", C.POINTER(raptr_grid_t)), ("group", C.c_int)]
+
+
 class RaptrError(RuntimeError):
     def __init__(self, status, where, message):
         super().__init__(f"{where}: status {status}: {message}")
@@ -125,6 +130,9 @@ _SIGS = {
     "raptr_recon_transpose_nnz": (i64, [vp]),
     "raptr_recon_set_broadcast": (i32, [vp, i32]),
     "raptr_recon_get_broadcast": (i32, [vp]),
+    "raptr_recon_set_grouped": (i32, [vp, i32, i32, i32]),
+    "raptr_recon_get_grouped": (i32, [vp, vp, vp, vp, vp, vp, vp]),
+    "raptr_recon_create_opts": (i32, [vp, i32, i32, i64, vp, vp, vp, vp, i32, i32, i32, vp, vp]),
     "raptr_recon_set_tiled": (i32, [vp, i32, i32, i32, i32]),
     "raptr_recon_get_tiled": (i32, [vp, vp, vp, vp, vp, vp, vp]),
     "raptr_recon_get_backward": (i32, [vp, vp, vp, vp]),

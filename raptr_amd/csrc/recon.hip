@@ -18,6 +18,7 @@
 // per index ([index][frame]); one launch handles all frames, so the matrix
 // stream is shared by the batch.  The normalisation sums are FP64.
 #include <algorithm>
+#include <climits>
 
 #include "recon.hpp"
 
@@ -630,6 +631,577 @@ __global__ __launch_bounds__(256) void k_bwd64(int n, const int64_t* __restrict_
   bwd_finish<64>(acc, j, n, q, slot, w, lb, first, last, c, x, s, xx, partials);
 }
 
+// ------------------------------------------- grouped 64-frame layouts
+// A 64-frame pass gathers a 256-byte frame vector per matrix entry, and
+// those gathers (the cache lines they pull from L2), not the matrix bytes,
+// bound it.  Neighbouring plan rows (lines with one midpoint and
+// neighbouring slopes) and neighbouring voxels share most of their entries,
+// so the grouped layouts store groups of G rows (forward) or of G voxels of
+// a brick (backward) as one ascending union list with G values per union
+// entry (0 where a member has no entry).  One gathered frame vector then
+// serves all G members.  Every member's sum still runs over its own entries
+// in ascending order, with exact zero terms between them.
+constexpr int kGrpPad = 64;    // union entries of tail padding (>= 2 x the largest unroll)
+constexpr int kGrpAlign = 16;  // union lists padded to multiples of this (value 0, last key)
+
+// forward groups: list s of group g is CSR row G*g + s
+template <int G>
+struct RowLists {
+  const int64_t* rp;
+  const int* col;
+  const float* val;
+  int64_t m;
+  __device__ void bounds(int64_t g, int64_t (&b)[G], int64_t (&e)[G]) const {
+#pragma unroll
+    for (int s = 0; s < G; ++s) {
+      const int64_t r = int64_t(G) * g + s;
+      b[s] = r < m ? rp[r] : 0;
+      e[s] = r < m ? rp[r + 1] : 0;
+    }
+  }
+  __device__ int key(int64_t k) const { return col[k]; }
+  __device__ float value(int64_t k) const { return val[k]; }
+  __device__ int64_t row(int64_t g, int s, int) const { return int64_t(G) * g + s; }
+};
+
+// backward groups: list s of group t = (block b, brick h) is the block's
+// sorted transpose segment (rows ascending) of column cols[h*G + s]
+// (-1: the brick has no such member)
+template <int G>
+struct ColLists {
+  const int64_t* uptr;
+  const uint32_t* idx;
+  const uint32_t* rowid;
+  const float* val;
+  const int* cols;
+  int n, ngc;
+  __device__ void bounds(int64_t t, int64_t (&b)[G], int64_t (&e)[G]) const {
+    const int64_t blk = t / ngc, h = t - blk * ngc;
+#pragma unroll
+    for (int s = 0; s < G; ++s) {
+      const int j = cols[h * G + s];
+      const int64_t o = blk * (n + 1) + j;
+      b[s] = j >= 0 ? uptr[o] : 0;
+      e[s] = j >= 0 ? uptr[o + 1] : 0;
+    }
+  }
+  __device__ int key(int64_t k) const { return int(rowid[idx[k]]); }
+  __device__ float value(int64_t k) const { return val[idx[k]]; }
+  __device__ int64_t row(int64_t, int, int key) const { return key; }
+};
+
+// Merge of the G ascending lists of a group: emit(key, at) once per
+// distinct key in ascending order, at[s] = position in list s or -1.
+template <int G, class L, class Emit>
+__device__ __forceinline__ void merge_lists(const L& lists, int64_t g, Emit emit) {
+  int64_t p[G], e[G];
+  lists.bounds(g, p, e);
+  int h[G];
+#pragma unroll
+  for (int s = 0; s < G; ++s) h[s] = p[s] < e[s] ? lists.key(p[s]) : INT_MAX;
+  for (;;) {
+    int mn = INT_MAX;
+#pragma unroll
+    for (int s = 0; s < G; ++s) mn = min(mn, h[s]);
+    if (mn == INT_MAX) break;
+    int64_t at[G];
+#pragma unroll
+    for (int s = 0; s < G; ++s) {
+      if (h[s] == mn) {
+        at[s] = p[s]++;
+        h[s] = p[s] < e[s] ? lists.key(p[s]) : INT_MAX;
+      } else {
+        at[s] = -1;
+      }
+    }
+    emit(mn, at);
+  }
+}
+
+template <int G, class L>
+__global__ void k_grp_count(L lists, int64_t ng, int64_t* __restrict__ cnt) {
+  for (int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; g < ng;
+       g += int64_t(gridDim.x) * blockDim.x) {
+    int64_t c = 0;
+    merge_lists<G>(lists, g, [&](int, const int64_t (&)[G]) { ++c; });
+    cnt[g] = (c + kGrpAlign - 1) / kGrpAlign * kGrpAlign;
+  }
+}
+
+// union keys and G values per union entry; 16-bit values are quantized
+// against the member row's scale as in the transpose (k_bt_scatter)
+template <int G, class L>
+__global__ void k_grp_fill(L lists, int64_t ng, const int64_t* __restrict__ gptr,
+                           const float* __restrict__ scale, int vt, int* __restrict__ ukey,
+                           void* __restrict__ uval) {
+  for (int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; g < ng;
+       g += int64_t(gridDim.x) * blockDim.x) {
+    int64_t o = gptr[g];
+    int last = 0;
+    merge_lists<G>(lists, g, [&](int key, const int64_t (&at)[G]) {
+      ukey[o] = key;
+      last = key;
+#pragma unroll
+      for (int s = 0; s < G; ++s) {
+        const float v = at[s] >= 0 ? lists.value(at[s]) : 0.0f;
+        if (vt == kF32) {
+          static_cast<float*>(uval)[G * o + s] = v;
+        } else {
+          const int64_t r = lists.row(g, s, key);
+          static_cast<uint16_t*>(uval)[G * o + s] =
+              at[s] >= 0 ? f2h(vt == kU16 ? v / (scale[r] * 65535.0f) : v / scale[r], vt)
+                         : uint16_t(0);
+        }
+      }
+      ++o;
+    });
+    // alignment padding: the list's last key (a valid index), values 0
+    for (const int64_t e = gptr[g + 1]; o < e; ++o) {
+      ukey[o] = last;
+#pragma unroll
+      for (int s = 0; s < G; ++s) {
+        if (vt == kF32)
+          static_cast<float*>(uval)[G * o + s] = 0.0f;
+        else
+          static_cast<uint16_t*>(uval)[G * o + s] = 0;
+      }
+    }
+  }
+}
+
+// the G member values of union entry k
+template <int VT, int G>
+__device__ __forceinline__ void load_vals(const typename Val<VT>::T* __restrict__ v, int64_t k,
+                                          float (&o)[G]) {
+  if constexpr (VT == kF32) {
+    const vf4* p = reinterpret_cast<const vf4*>(v + int64_t(G) * k);
+#pragma unroll
+    for (int i = 0; i < G / 4; ++i) {
+      const vf4 t = p[i];
+      o[4 * i] = t.x, o[4 * i + 1] = t.y, o[4 * i + 2] = t.z, o[4 * i + 3] = t.w;
+    }
+  } else {
+    const vu2* p = reinterpret_cast<const vu2*>(v + int64_t(G) * k);
+#pragma unroll
+    for (int i = 0; i < G / 4; ++i) {
+      const vu2 t = p[i];
+      o[4 * i] = h2f(uint16_t(t.x & 0xffff), VT), o[4 * i + 1] = h2f(uint16_t(t.x >> 16), VT);
+      o[4 * i + 2] = h2f(uint16_t(t.y & 0xffff), VT), o[4 * i + 3] = h2f(uint16_t(t.y >> 16), VT);
+    }
+  }
+}
+
+// the wave's index inside its workgroup, as a wave-uniform value (so that
+// per-wave pointers and loop bounds live in scalar registers)
+__device__ __forceinline__ int wave_id() {
+  return __builtin_amdgcn_readfirstlane(int(threadIdx.x) / kWave);
+}
+
+// U union entries of a grouped list: their keys and member values
+// (wave-uniform, so they sit in scalar registers)
+template <int VT, int G, int U>
+struct GSet {
+  int c[U];
+  float v[U][G];
+};
+
+template <int VT, int G, int U>
+__device__ __forceinline__ void gload(GSet<VT, G, U>& s, const int* __restrict__ key,
+                                      const typename Val<VT>::T* __restrict__ val, int64_t k) {
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    s.c[u] = key[k + u];
+    load_vals<VT, G>(val, k + u, s.v[u]);
+  }
+}
+
+// the lane's frame of the U frame vectors base[c][0..63]
+template <int U>
+__device__ __forceinline__ void ggather(const int (&c)[U], const float* __restrict__ base,
+                                        float (&g)[U]) {
+#pragma unroll
+  for (int u = 0; u < U; ++u) g[u] = base[int64_t(c[u]) * 64];
+}
+
+template <int G, int U>
+__device__ __forceinline__ void gfma(const float (&v)[U][G], const float (&g)[U],
+                                     float (&acc)[G]) {
+#pragma unroll
+  for (int u = 0; u < U; ++u)
+#pragma unroll
+    for (int q = 0; q < G; ++q) acc[q] = fmaf(v[u][q], g[u], acc[q]);
+}
+
+// acc[q] += sum over the union entries [b, en) of v[q] * base[key][lane].
+// PIPE: two register sets; the next set's scalar loads are issued after the
+// current set's gathers (scheduling barriers keep that order), so their
+// latency overlaps the gathers' instead of adding to it.
+template <int VT, int G, int U, bool PIPE>
+__device__ __forceinline__ void grouped_sum(const int* __restrict__ key,
+                                            const typename Val<VT>::T* __restrict__ val,
+                                            int64_t b, int64_t en,
+                                            const float* __restrict__ base, float (&acc)[G]) {
+  int64_t k = b;
+  const int64_t nfull = (en - b) / U;
+  if (PIPE) {
+    // pairs of sets without a branch between them; the next set is loaded
+    // unconditionally (up to 2U entries past the pairs, inside the list or
+    // the layout's tail padding, values unused)
+    if (nfull & 1) {
+      GSet<VT, G, U> A;
+      float g[U];
+      gload<VT, G, U>(A, key, val, k);
+      ggather<U>(A.c, base, g);
+      gfma<G, U>(A.v, g, acc);
+      k += U;
+    }
+    const int64_t npairs = nfull >> 1;
+    if (npairs > 0) {
+      GSet<VT, G, U> A, B;
+      gload<VT, G, U>(A, key, val, k);
+      for (int64_t p = 0; p < npairs; ++p) {
+        float g[U], h[U];
+        ggather<U>(A.c, base, g);
+        __builtin_amdgcn_sched_barrier(0);
+        gload<VT, G, U>(B, key, val, k + U);
+        __builtin_amdgcn_sched_barrier(0);
+        gfma<G, U>(A.v, g, acc);
+        ggather<U>(B.c, base, h);
+        __builtin_amdgcn_sched_barrier(0);
+        gload<VT, G, U>(A, key, val, k + 2 * U);
+        __builtin_amdgcn_sched_barrier(0);
+        gfma<G, U>(B.v, h, acc);
+        k += 2 * U;
+      }
+    }
+  } else {
+    for (int64_t it = 0; it < nfull; ++it, k += U) {
+      GSet<VT, G, U> A;
+      float g[U];
+      gload<VT, G, U>(A, key, val, k);
+      ggather<U>(A.c, base, g);
+      gfma<G, U>(A.v, g, acc);
+    }
+  }
+  for (; k < en; ++k) {
+    GSet<VT, G, 1> A;
+    float g[1];
+    gload<VT, G, 1>(A, key, val, k);
+    ggather<1>(A.c, base, g);
+    gfma<G, 1>(A.v, g, acc);
+  }
+}
+
+// Grouped forward + ratio, 64 frames: one wave per group of G rows, lane =
+// frame.  Per union entry the column and the G values are wave-uniform
+// loads; the frame vector x[col][0..63] is one 256-byte gather shared by
+// the G rows.
+template <int VT, int G, int U, bool PIPE>
+__global__ __launch_bounds__(256) void k_fwdg64(int64_t ng, int64_t m,
+                                                const int64_t* __restrict__ gp,
+                                                const int* __restrict__ gcol,
+                                                const typename Val<VT>::T* __restrict__ gval,
+                                                const float* __restrict__ rscale,
+                                                const float* __restrict__ x,
+                                                const float* __restrict__ y,
+                                                float* __restrict__ e) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t g = xcd_block(blockIdx.x, gridDim.x) * (256 / kWave) + wave_id();
+  if (g >= ng) return;
+  float acc[G];
+#pragma unroll
+  for (int s = 0; s < G; ++s) acc[s] = 0.0f;
+  grouped_sum<VT, G, U, PIPE>(gcol, gval, gp[g], gp[g + 1], x + lane, acc);
+#pragma unroll
+  for (int s = 0; s < G; ++s) {
+    const int64_t r = int64_t(G) * g + s;
+    if (r >= m) break;
+    const float sc = VT == kF32 ? 1.0f : rscale[r];
+    const float yy = y[r * 64 + lane];
+    const float yt = sc * acc[s];
+    const float t = (yt != 0.0f && yy != 0.0f) ? yy / yt : 0.0f;
+    e[r * 64 + lane] = VT == kF32 ? t : sc * t;  // ratio folded with the row scale
+  }
+}
+
+// Grouped backward, 64 frames: one wave per brick of G voxels, lane =
+// frame, over nbl consecutive row blocks; the ratio vector of each union
+// row is one 256-byte gather shared by the brick's voxels.  The last launch
+// applies x' = x*c/(s + (s==0)) and writes the workgroup's FP64 partial
+// sums per frame.
+template <int VT, int G, int U, bool PIPE>
+__global__ __launch_bounds__(256) void k_bwdg64(int ngc, const int64_t* __restrict__ gp,
+                                                int nbl, const int* __restrict__ grow,
+                                                const typename Val<VT>::T* __restrict__ gval,
+                                                const int* __restrict__ cols,
+                                                const float* __restrict__ e,
+                                                float* __restrict__ c, int first, int last,
+                                                const float* __restrict__ x,
+                                                const float* __restrict__ s,
+                                                float* __restrict__ xx,
+                                                double* __restrict__ partials) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int w = wave_id();
+  const int lb = int(xcd_block(blockIdx.x, gridDim.x));
+  const int h = lb * (256 / kWave) + w;
+  float acc[G];
+#pragma unroll
+  for (int q = 0; q < G; ++q) acc[q] = 0.0f;
+  if (h < ngc) {
+    for (int bb = 0; bb < nbl; ++bb)
+      grouped_sum<VT, G, U, PIPE>(grow, gval, gp[int64_t(bb) * ngc + h],
+                                  gp[int64_t(bb) * ngc + h + 1], e + lane, acc);
+  }
+  double part = 0.0;
+  if (h < ngc) {
+#pragma unroll
+    for (int q = 0; q < G; ++q) {
+      const int j = cols[int64_t(h) * G + q];
+      if (j < 0) continue;
+      const int64_t o = int64_t(j) * 64 + lane;
+      const float cv = first ? acc[q] : c[o] + acc[q];
+      if (!last) {
+        c[o] = cv;
+      } else {
+        float d = s[j];
+        d += float(d == 0.0f);
+        const float nv = x[o] * cv / d;
+        xx[o] = nv;
+        part += double(nv);
+      }
+    }
+  }
+  if (last) {
+    __shared__ double red[256 / kWave][64];
+    red[w][lane] = part;
+    __syncthreads();
+    if (threadIdx.x < 64) {
+      double t = 0.0;
+      for (int ww = 0; ww < 256 / kWave; ++ww) t += red[ww][threadIdx.x];
+      partials[int64_t(lb) * 64 + threadIdx.x] = t;
+    }
+  }
+}
+
+// ---- 16-member groups on the matrix cores
+// v_mfma_f32_16x16x4f32 computes D[16][16] += A[16][4] B[4][16]: lane l
+// supplies A[l%16][l/16] and B[l/16][l%16] and holds D[4*(l/16)+i][l%16]
+// in register i.  Here A = the members' values of 4 union entries (rows of
+// A = group members), B = 16 frames of those entries' gathered frame
+// vectors, D = the members' sums for 16 frames; 4 instructions per 4
+// entries cover the 64 frames.  The values and keys arrive with coalesced
+// vector loads (lane l loads member l%16 of entry l/16), so nothing streams
+// through the scalar cache, and the next step's loads are issued before the
+// current step's gathers (vector loads complete in order).
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+template <int VT>
+__device__ __forceinline__ float load_val1(const typename Val<VT>::T* __restrict__ v,
+                                           int64_t i) {
+  if constexpr (VT == kF32)
+    return v[i];
+  else
+    return h2f(v[i], VT);
+}
+
+// keys and values of one step (4*U union entries; lane l takes entry
+// 4u + l/16 and member l%16)
+template <int VT, int U>
+struct MSet {
+  int c[U];
+  float a[U];
+};
+
+template <int VT, int U>
+__device__ __forceinline__ void mload(MSet<VT, U>& t, const int* __restrict__ key,
+                                      const typename Val<VT>::T* __restrict__ val, int64_t k0,
+                                      int kq, int r16) {
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int64_t k = k0 + 4 * u + kq;
+    t.c[u] = key[k];
+    t.a[u] = load_val1<VT>(val, k * 16 + r16);
+  }
+}
+
+// one step: gathers of set S, then the loads of set T (two steps ahead),
+// then the matrix products of S.  Vector loads complete in order, so the
+// products wait for S's gathers but not for T's loads.
+template <int VT, int U>
+__device__ __forceinline__ void mstep(const MSet<VT, U>& S, MSet<VT, U>& T, bool load_t,
+                                      const int* __restrict__ key,
+                                      const typename Val<VT>::T* __restrict__ val, int64_t kt,
+                                      int kq, int r16, const float* __restrict__ base,
+                                      f4v (&acc)[4]) {
+  float g[U][4];
+#pragma unroll
+  for (int u = 0; u < U; ++u)
+#pragma unroll
+    for (int fb = 0; fb < 4; ++fb) g[u][fb] = base[int64_t(S.c[u]) * 64 + 16 * fb];
+  __builtin_amdgcn_sched_barrier(0);
+  if (load_t) mload<VT, U>(T, key, val, kt, kq, r16);
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int u = 0; u < U; ++u)
+#pragma unroll
+    for (int fb = 0; fb < 4; ++fb)
+      acc[fb] = __builtin_amdgcn_mfma_f32_16x16x4f32(S.a[u], g[u][fb], acc[fb], 0, 0, 0);
+}
+
+// acc[fb] += the group's sums over the union entries [b, en) (a multiple
+// of kGrpAlign) for frames 16*fb .. 16*fb+15, in steps of 4*U entries with
+// three register sets (loads two steps ahead); reads up to 8*U entries past
+// en (inside the next list or the tail padding)
+template <int VT, int U>
+__device__ __forceinline__ void mfma_sum(const int* __restrict__ key,
+                                         const typename Val<VT>::T* __restrict__ val,
+                                         int64_t b, int64_t en, const float* __restrict__ base,
+                                         int kq, int r16, f4v (&acc)[4]) {
+  constexpr int E = 4 * U;
+  const int64_t steps = (en - b) / E;
+  if (steps <= 0) return;
+  MSet<VT, U> A, B, C;
+  mload<VT, U>(A, key, val, b, kq, r16);
+  mload<VT, U>(B, key, val, b + E, kq, r16);
+  int64_t kt = b + 2 * E;
+  int64_t s = 0;
+  for (; s + 3 <= steps; s += 3) {
+    mstep<VT, U>(A, C, true, key, val, kt, kq, r16, base, acc);
+    mstep<VT, U>(B, A, true, key, val, kt + E, kq, r16, base, acc);
+    mstep<VT, U>(C, B, true, key, val, kt + 2 * E, kq, r16, base, acc);
+    kt += 3 * E;
+  }
+  if (s < steps) mstep<VT, U>(A, C, false, key, val, kt, kq, r16, base, acc);
+  if (s + 1 < steps) mstep<VT, U>(B, C, false, key, val, kt, kq, r16, base, acc);
+}
+
+template <int VT, int U>
+__global__ __launch_bounds__(256) void k_fwdm64(int64_t ng, int64_t m,
+                                                const int64_t* __restrict__ gp,
+                                                const int* __restrict__ gcol,
+                                                const typename Val<VT>::T* __restrict__ gval,
+                                                const float* __restrict__ rscale,
+                                                const float* __restrict__ x,
+                                                const float* __restrict__ y,
+                                                float* __restrict__ e) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int kq = lane >> 4, r16 = lane & 15;
+  const int64_t g = xcd_block(blockIdx.x, gridDim.x) * (256 / kWave) + wave_id();
+  if (g >= ng) return;
+  f4v acc[4];
+#pragma unroll
+  for (int fb = 0; fb < 4; ++fb) acc[fb] = f4v{0.0f, 0.0f, 0.0f, 0.0f};
+  mfma_sum<VT, U>(gcol, gval, gp[g], gp[g + 1], x + r16, kq, r16, acc);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int64_t r = 16 * g + 4 * kq + i;
+    if (r >= m) break;
+    const float sc = VT == kF32 ? 1.0f : rscale[r];
+#pragma unroll
+    for (int fb = 0; fb < 4; ++fb) {
+      const int64_t o = r * 64 + 16 * fb + r16;
+      const float yy = y[o];
+      const float yt = sc * acc[fb][i];
+      const float t = (yt != 0.0f && yy != 0.0f) ? yy / yt : 0.0f;
+      e[o] = VT == kF32 ? t : sc * t;  // ratio folded with the row scale
+    }
+  }
+}
+
+template <int VT, int U>
+__global__ __launch_bounds__(256) void k_bwdm64(int ngc, const int64_t* __restrict__ gp,
+                                                int nbl, const int* __restrict__ grow,
+                                                const typename Val<VT>::T* __restrict__ gval,
+                                                const int* __restrict__ cols,
+                                                const float* __restrict__ e,
+                                                float* __restrict__ c, int first, int last,
+                                                const float* __restrict__ x,
+                                                const float* __restrict__ s,
+                                                float* __restrict__ xx,
+                                                double* __restrict__ partials) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int kq = lane >> 4, r16 = lane & 15;
+  const int w = wave_id();
+  const int lb = int(xcd_block(blockIdx.x, gridDim.x));
+  const int h = lb * (256 / kWave) + w;
+  f4v acc[4];
+#pragma unroll
+  for (int fb = 0; fb < 4; ++fb) acc[fb] = f4v{0.0f, 0.0f, 0.0f, 0.0f};
+  if (h < ngc) {
+    for (int bb = 0; bb < nbl; ++bb)
+      mfma_sum<VT, U>(grow, gval, gp[int64_t(bb) * ngc + h], gp[int64_t(bb) * ngc + h + 1],
+                      e + r16, kq, r16, acc);
+  }
+  double part[4] = {0.0, 0.0, 0.0, 0.0};
+  if (h < ngc) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int j = cols[int64_t(h) * 16 + 4 * kq + i];
+      if (j < 0) continue;
+      float d = 0.0f;
+      if (last) {
+        d = s[j];
+        d += float(d == 0.0f);
+      }
+#pragma unroll
+      for (int fb = 0; fb < 4; ++fb) {
+        const int64_t o = int64_t(j) * 64 + 16 * fb + r16;
+        const float cv = first ? acc[fb][i] : c[o] + acc[fb][i];
+        if (!last) {
+          c[o] = cv;
+        } else {
+          const float nv = x[o] * cv / d;
+          xx[o] = nv;
+          part[fb] += double(nv);
+        }
+      }
+    }
+  }
+  if (last) {
+    __shared__ double red[256 / kWave][4][64];
+#pragma unroll
+    for (int fb = 0; fb < 4; ++fb) red[w][kq][16 * fb + r16] = part[fb];
+    __syncthreads();
+    if (threadIdx.x < 64) {
+      double t = 0.0;
+      for (int ww = 0; ww < 256 / kWave; ++ww)
+        for (int q = 0; q < 4; ++q) t += red[ww][q][threadIdx.x];
+      partials[int64_t(lb) * 64 + threadIdx.x] = t;
+    }
+  }
+}
+
+// brick of the backward groups: G voxels (x, y, z extents)
+__host__ __device__ inline void brick_dims(int G, int& bx, int& by, int& bz) {
+  bx = G >= 8 ? 2 : 1;
+  by = 2;
+  bz = G >= 16 ? 4 : (G >= 8 ? 2 : 2);
+}
+
+// cols[h*G + s]: member s of brick h (bricks in x, y, z order, z fastest;
+// members x, y, z order, z fastest; -1 outside the grid)
+__global__ void k_brick_cols(int nx, int ny, int nz, int G, int64_t ngc, int* __restrict__ cols) {
+  int bx, by, bz;
+  brick_dims(G, bx, by, bz);
+  const int nbz = (nz + bz - 1) / bz, nby = (ny + by - 1) / by;
+  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < ngc * G;
+       t += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t h = t / G;
+    const int sidx = int(t - h * G);
+    const int hz = int(h % nbz), hy = int((h / nbz) % nby), hx = int(h / (int64_t(nbz) * nby));
+    const int mz = sidx % bz, my = (sidx / bz) % by, mx = sidx / (bz * by);
+    const int ix = hx * bx + mx, iy = hy * by + my, iz = hz * bz + mz;
+    cols[t] = (ix < nx && iy < ny && iz < nz) ? (ix * ny + iy) * nz + iz : -1;
+  }
+}
+
+// groups of G consecutive column ids (no grid known)
+__global__ void k_seq_cols(int n, int64_t total, int* __restrict__ cols) {
+  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < total;
+       t += int64_t(gridDim.x) * blockDim.x)
+    cols[t] = t < n ? int(t) : -1;
+}
+
 // Single-frame backward staged through LDS: each workgroup owns a
 // contiguous range of columns (voxels) and walks the row blocks in order;
 // block b's slice of the ratio vector is loaded into LDS once per workgroup
@@ -881,6 +1453,85 @@ namespace raptr {
 namespace {
 
 
+// ------------------------------------------------ grouped layout builds
+// union lists of ng groups: count pass, scan, fill pass
+template <int G, class L>
+int build_groups(raptr_recon* R, const L& lists, int64_t ng, int64_t** gptr, int** gkey,
+                 void** gval, int64_t* gnnz, hipStream_t st) {
+  int rc;
+  if ((rc = ralloc(R, gptr, size_t(ng) + 1))) return rc;
+  const size_t sb = scan_workspace_bytes(ng > 0 ? ng : 1);
+  Sizer z;
+  z.take<int64_t>(size_t(ng) + 1);
+  z.take<int64_t>(4);
+  z.take<char>(sb);
+  void* tmp = nullptr;
+  hipError_t ae = hipMalloc(&tmp, z.bytes());
+  if (ae != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(RAPTR_ERR_OUT_OF_MEMORY, "grouped layout scratch: %s", hipGetErrorString(ae));
+  }
+  Carver cv(tmp, z.bytes());
+  int64_t* cnt = cv.take<int64_t>(size_t(ng) + 1);
+  int64_t* tot = cv.take<int64_t>(4);
+  char* sws = cv.take<char>(sb);
+  int64_t h = 0;
+  if (ng > 0) {
+    hipLaunchKernelGGL((k_grp_count<G, L>), dim3(grid_for(ng, 256, 1 << 16)), dim3(256), 0, st,
+                       lists, ng, cnt);
+    rc = scan_exclusive_i64(cnt, *gptr, ng, tot, sws, sb, st);
+    if (!rc) {
+      hipLaunchKernelGGL(k_set_last, dim3(1), dim3(1), 0, st, *gptr, int(ng),
+                         (const int64_t*)tot);
+      (void)hipMemcpyAsync(&h, tot, 8, hipMemcpyDeviceToHost, st);
+    }
+  } else {
+    (void)hipMemsetAsync(*gptr, 0, 8, st);
+  }
+  hipError_t se = hipStreamSynchronize(st);
+  (void)hipFree(tmp);
+  if (rc) return rc;
+  if (se != hipSuccess) return fail(RAPTR_ERR_DEVICE, "grouped layout: %s", hipGetErrorString(se));
+  *gnnz = h;
+  // tail padding: pipelined kernels read up to kGrpPad union entries past
+  // a list's end (values unused)
+  if ((rc = ralloc(R, gkey, size_t(h) + kGrpPad))) return rc;
+  if (R->vt == kF32) {
+    float* p;
+    if ((rc = ralloc(R, &p, G * (size_t(h) + kGrpPad)))) return rc;
+    *gval = p;
+  } else {
+    uint16_t* p;
+    if ((rc = ralloc(R, &p, G * (size_t(h) + kGrpPad)))) return rc;
+    *gval = p;
+  }
+  // keep the padding defined (column / row 0, value 0)
+  const size_t vb = R->vt == kF32 ? 4 : 2;
+  (void)hipMemsetAsync(*gkey + h, 0, kGrpPad * sizeof(int), st);
+  (void)hipMemsetAsync(static_cast<char*>(*gval) + G * size_t(h) * vb, 0, G * kGrpPad * vb, st);
+  if (ng > 0) {
+    hipLaunchKernelGGL((k_grp_fill<G, L>), dim3(grid_for(ng, 256, 1 << 16)), dim3(256), 0, st,
+                       lists, ng, (const int64_t*)*gptr, (const float*)R->rscale, R->vt, *gkey,
+                       *gval);
+    hipError_t le = hipGetLastError();
+    if (le != hipSuccess) return fail(RAPTR_ERR_DEVICE, "grouped fill: %s", hipGetErrorString(le));
+  }
+  return RAPTR_OK;
+}
+
+// grouped kernel shapes compiled per group size: (unroll, pipelined)
+// kind: 0 plain, 1 pipelined (scalar loads, vector FMAs), 2 matrix cores
+bool grp_shape_ok(int G, int U, int kind) {
+  switch (G) {
+    case 4: return (U == 8 && kind == 1) || (U == 16 && kind == 0);
+    case 8: return (U == 4 && kind == 1) || (U == 8 && kind == 0);
+    case 16:
+      return (U == 2 && kind == 1) || (U == 4 && kind == 0) ||
+             ((U == 8 || U == 16) && kind == 2);
+  }
+  return false;
+}
+
 // ---------------------------------------------------------- launches
 int launch_fwd_any(const raptr_recon* R, int F, const float* x, const float* y, float* e,
                    hipStream_t st) {
@@ -907,6 +1558,42 @@ int launch_fwd_any(const raptr_recon* R, int F, const float* x, const float* y, 
     return fail(RAPTR_ERR_INVALID_ARG, "forward shape lanes %d unroll %d unsupported", L, U);
   }
   unsigned g = unsigned((m + 3) / 4);
+  if (F == 64 && R->grp64) {
+    const unsigned gg = unsigned((R->ngf + 3) / 4);
+    const int GU = R->grp_unroll, GG = R->grp_g;
+    if (R->grp_pipe == 2) {
+#define RAPTR_M64(VV, UU)                                                                    \
+      if (R->vt == VV && GG == 16 && GU == 4 * UU) {                                         \
+        hipLaunchKernelGGL((k_fwdm64<VV, UU>), dim3(gg), dim3(256), 0, st, R->ngf, m, R->gfp,\
+                           R->gfcol, (const typename Val<VV>::T*)R->gfval, R->rscale, x, y,  \
+                           e);                                                               \
+        RAPTR_LAUNCH_CHECK();                                                                \
+        return RAPTR_OK;                                                                     \
+      }
+#define RAPTR_M64V(VV) RAPTR_M64(VV, 2) RAPTR_M64(VV, 4)
+      RAPTR_M64V(kF32) RAPTR_M64V(kF16) RAPTR_M64V(kBF16) RAPTR_M64V(kU16)
+#undef RAPTR_M64V
+#undef RAPTR_M64
+      return fail(RAPTR_ERR_INVALID_ARG, "matrix-core forward shape unsupported");
+    }
+    const bool GP = R->grp_pipe != 0;
+#define RAPTR_G64(VV, G_, UU, PP)                                                            \
+    if (R->vt == VV && GG == G_ && GU == UU && GP == PP) {                                   \
+      hipLaunchKernelGGL((k_fwdg64<VV, G_, UU, PP>), dim3(gg), dim3(256), 0, st, R->ngf, m,   \
+                         R->gfp, R->gfcol, (const typename Val<VV>::T*)R->gfval, R->rscale,  \
+                         x, y, e);                                                           \
+      RAPTR_LAUNCH_CHECK();                                                                  \
+      return RAPTR_OK;                                                                       \
+    }
+#define RAPTR_G64V(VV)                                                                       \
+  RAPTR_G64(VV, 4, 8, true) RAPTR_G64(VV, 4, 16, false) RAPTR_G64(VV, 8, 4, true)            \
+  RAPTR_G64(VV, 8, 8, false) RAPTR_G64(VV, 16, 2, true) RAPTR_G64(VV, 16, 4, false)
+    RAPTR_G64V(kF32) RAPTR_G64V(kF16) RAPTR_G64V(kBF16) RAPTR_G64V(kU16)
+#undef RAPTR_G64V
+#undef RAPTR_G64
+    return fail(RAPTR_ERR_INVALID_ARG, "grouped forward shape G %d unroll %d unsupported", GG,
+                GU);
+  }
   if (F == 64 && R->bcast64) {
 #define RAPTR_F64(VV)                                                                        \
     if (R->vt == VV) {                                                                       \
@@ -944,6 +1631,46 @@ int launch_bwd_any(const raptr_recon* R, int F, int blk, int nbl, const float* x
   if (nb_out) *nb_out = int(g);
   const int64_t* ptr = R->bptr + int64_t(blk) * (n + 1);
   int first = blk == 0, last = blk + nbl == R->nblk;
+  if (F == 64 && R->grp64) {
+    const unsigned gg = unsigned((R->ngc + 3) / 4);
+    if (nb_out) *nb_out = int(gg);
+    const int64_t* gp = R->gbp + int64_t(blk) * R->ngc;
+    const int GU = R->grp_unroll, GG = R->grp_g;
+    if (R->grp_pipe == 2) {
+#define RAPTR_MB64(VV, UU)                                                                   \
+      if (R->vt == VV && GG == 16 && GU == 4 * UU) {                                         \
+        hipLaunchKernelGGL((k_bwdm64<VV, UU>), dim3(gg), dim3(256), 0, st, int(R->ngc), gp,   \
+                           nbl, R->gbrow, (const typename Val<VV>::T*)R->gbval,              \
+                           (const int*)R->gcols, R->e, R->c, first, last, x, R->s, R->xx,    \
+                           R->partials);                                                     \
+        RAPTR_LAUNCH_CHECK();                                                                \
+        return RAPTR_OK;                                                                     \
+      }
+#define RAPTR_MB64V(VV) RAPTR_MB64(VV, 2) RAPTR_MB64(VV, 4)
+      RAPTR_MB64V(kF32) RAPTR_MB64V(kF16) RAPTR_MB64V(kBF16) RAPTR_MB64V(kU16)
+#undef RAPTR_MB64V
+#undef RAPTR_MB64
+      return fail(RAPTR_ERR_INVALID_ARG, "matrix-core backward shape unsupported");
+    }
+    const bool GP = R->grp_pipe != 0;
+#define RAPTR_GB64(VV, G_, UU, PP)                                                           \
+    if (R->vt == VV && GG == G_ && GU == UU && GP == PP) {                                   \
+      hipLaunchKernelGGL((k_bwdg64<VV, G_, UU, PP>), dim3(gg), dim3(256), 0, st,              \
+                         int(R->ngc), gp, nbl, R->gbrow,                                     \
+                         (const typename Val<VV>::T*)R->gbval, (const int*)R->gcols, R->e,   \
+                         R->c, first, last, x, R->s, R->xx, R->partials);                    \
+      RAPTR_LAUNCH_CHECK();                                                                  \
+      return RAPTR_OK;                                                                       \
+    }
+#define RAPTR_GB64V(VV)                                                                      \
+  RAPTR_GB64(VV, 4, 8, true) RAPTR_GB64(VV, 4, 16, false) RAPTR_GB64(VV, 8, 4, true)         \
+  RAPTR_GB64(VV, 8, 8, false) RAPTR_GB64(VV, 16, 2, true) RAPTR_GB64(VV, 16, 4, false)
+    RAPTR_GB64V(kF32) RAPTR_GB64V(kF16) RAPTR_GB64V(kBF16) RAPTR_GB64V(kU16)
+#undef RAPTR_GB64V
+#undef RAPTR_GB64
+    return fail(RAPTR_ERR_INVALID_ARG, "grouped backward shape G %d unroll %d unsupported",
+                GG, GU);
+  }
   if (F == 64 && R->bcast64) {
 #define RAPTR_B64(VV)                                                                        \
     if (R->vt == VV) {                                                                       \
@@ -1201,6 +1928,15 @@ bool frames_aligned(const void* p, int F) {
 }  // namespace
 }  // namespace raptr
 
+namespace raptr {
+namespace {
+int create_plan(raptr_recon** out, int m, int n, int64_t nnz, const int64_t* d_rp,
+                const int* d_col, const float* d_val, const double* d_sens, int value_type,
+                int max_frames, int block_rows, int layout, const int* gdims, int group,
+                raptr_stream_t stream);
+}  // namespace
+}  // namespace raptr
+
 extern "C" {
 
 int raptr_recon_create(raptr_recon** out, int m, int n, int64_t nnz, const int64_t* d_rp,
@@ -1215,7 +1951,27 @@ int raptr_recon_create_ex(raptr_recon** out, int m, int n, int64_t nnz, const in
                           const int* d_col, const float* d_val, const double* d_sens,
                           int value_type, int max_frames, int block_rows, int layout,
                           raptr_stream_t stream) {
+  return create_plan(out, m, n, nnz, d_rp, d_col, d_val, d_sens, value_type, max_frames,
+                     block_rows, layout, nullptr, 0, stream);
+}
+}  // extern "C"
+
+namespace raptr {
+namespace {
+// gdims: the columns' voxel grid (nx, ny, nz) for brick-shaped backward
+// groups, or null; group: members per group of the 64-frame grouped
+// layouts (4, 8, 16; 0 = 16)
+int create_plan(raptr_recon** out, int m, int n, int64_t nnz, const int64_t* d_rp,
+                const int* d_col, const float* d_val, const double* d_sens, int value_type,
+                int max_frames, int block_rows, int layout, const int* gdims, int group,
+                raptr_stream_t stream) {
   if (!out) return fail(RAPTR_ERR_INVALID_ARG, "null output");
+  if (group == 0) group = 16;
+  if (group != 4 && group != 8 && group != 16)
+    return fail(RAPTR_ERR_INVALID_ARG, "group must be 4, 8 or 16");
+  if (gdims && int64_t(gdims[0]) * gdims[1] * gdims[2] != n)
+    return fail(RAPTR_ERR_INVALID_ARG, "grid %d x %d x %d does not hold %d voxels", gdims[0],
+                gdims[1], gdims[2], n);
   *out = nullptr;
   if (m < 0 || n <= 0 || nnz < 0) return fail(RAPTR_ERR_INVALID_ARG, "bad sizes");
   if (value_type < kF32 || value_type > kU16)
@@ -1294,6 +2050,24 @@ int raptr_recon_create_ex(raptr_recon** out, int m, int n, int64_t nnz, const in
                          R->fcol, R->fval, R->rscale, max_frames == 1 ? kIlvFwd : 0);
       hipError_t le = hipGetLastError();
       if (le != hipSuccess) return bail(fail(RAPTR_ERR_DEVICE, "pad: %s", hipGetErrorString(le)));
+    }
+    if (max_frames == 64) {
+      R->grp_g = group;
+      R->ngf = (int64_t(m) + group - 1) / group;
+      switch (group) {
+#define RAPTR_RG(G_)                                                                       \
+  case G_: {                                                                               \
+    RowLists<G_> L{d_rp, d_col, d_val, int64_t(m)};                                        \
+    rc = build_groups<G_>(R, L, R->ngf, &R->gfp, &R->gfcol, &R->gfval, &R->gfnnz, st);      \
+    break;                                                                                 \
+  }
+        RAPTR_RG(4) RAPTR_RG(8) RAPTR_RG(16)
+#undef RAPTR_RG
+      }
+      if (rc) return bail(rc);
+      // default shape: matrix cores for 16-member groups, else pipelined
+      R->grp_unroll = group == 4 ? 8 : (group == 8 ? 4 : 8);
+      R->grp_pipe = group == 16 ? 2 : 1;
     }
     // ---- backward: row-blocked transpose via a stable radix sort, every
     // (block, column) segment padded to a multiple of 4 entries
@@ -1384,6 +2158,41 @@ int raptr_recon_create_ex(raptr_recon** out, int m, int n, int64_t nnz, const in
         hipError_t le = hipGetLastError();
         if (le != hipSuccess) rc = fail(RAPTR_ERR_DEVICE, "transpose: %s", hipGetErrorString(le));
       }
+      if (!rc && max_frames == 64) {
+        // member columns of every group: bricks of the grid, or runs of
+        // consecutive ids
+        const int G = group;
+        if (gdims) {
+          int bx, by, bz;
+          brick_dims(G, bx, by, bz);
+          R->ngc = int64_t((gdims[0] + bx - 1) / bx) * ((gdims[1] + by - 1) / by) *
+                   ((gdims[2] + bz - 1) / bz);
+        } else {
+          R->ngc = (int64_t(n) + G - 1) / G;
+        }
+        if (!(rc = ralloc(R, &R->gcols, size_t(R->ngc) * G))) {
+          const int64_t tot = R->ngc * G;
+          if (gdims)
+            hipLaunchKernelGGL(k_brick_cols, dim3(grid_for(tot, 256, 1 << 16)), dim3(256), 0, st,
+                               gdims[0], gdims[1], gdims[2], G, R->ngc, R->gcols);
+          else
+            hipLaunchKernelGGL(k_seq_cols, dim3(grid_for(tot, 256, 1 << 16)), dim3(256), 0, st,
+                               n, tot, R->gcols);
+          switch (G) {
+#define RAPTR_CG(G_)                                                                       \
+  case G_: {                                                                               \
+    ColLists<G_> L{(const int64_t*)uptr, (const uint32_t*)idx, (const uint32_t*)rowid,     \
+                   d_val, (const int*)R->gcols, n, int(R->ngc)};                           \
+    rc = build_groups<G_>(R, L, int64_t(R->nblk) * R->ngc, &R->gbp, &R->gbrow, &R->gbval,  \
+                          &R->gbnnz, st);                                                  \
+    break;                                                                                 \
+  }
+            RAPTR_CG(4) RAPTR_CG(8) RAPTR_CG(16)
+#undef RAPTR_CG
+          }
+        }
+        if (!rc) R->grp64 = 1;
+      }
       (void)hipStreamSynchronize(st);
       (void)hipFree(tmp);
       if (rc) return bail(rc);
@@ -1415,15 +2224,36 @@ int raptr_recon_create_ex(raptr_recon** out, int m, int n, int64_t nnz, const in
   *out = R;
   return RAPTR_OK;
 }
+}  // namespace
+}  // namespace raptr
+
+extern "C" {
 
 int raptr_recon_create_ordered(raptr_recon** out, int m, int n, int64_t nnz,
                                const int64_t* d_rp, const int* d_col, const float* d_val,
                                const double* d_sens, int value_type, int max_frames,
                                int block_rows, int layout, const int* d_row_order,
                                raptr_stream_t stream) {
+  raptr_recon_options o = {layout, d_row_order, nullptr, 0};
+  return raptr_recon_create_opts(out, m, n, nnz, d_rp, d_col, d_val, d_sens, value_type,
+                                 max_frames, block_rows, &o, stream);
+}
+
+int raptr_recon_create_opts(raptr_recon** out, int m, int n, int64_t nnz, const int64_t* d_rp,
+                            const int* d_col, const float* d_val, const double* d_sens,
+                            int value_type, int max_frames, int block_rows,
+                            const raptr_recon_options* opts, raptr_stream_t stream) {
+  const raptr_recon_options o = opts ? *opts : raptr_recon_options{RAPTR_LAYOUT_BLOCKED, nullptr,
+                                                                   nullptr, 0};
+  const int* d_row_order = o.d_row_order;
+  int gd[3] = {0, 0, 0};
+  if (o.grid) {
+    for (int k = 0; k < 3; ++k) gd[k] = o.grid->n[k];
+  }
+  const int* gdims = o.grid ? gd : nullptr;
   if (!d_row_order)
-    return raptr_recon_create_ex(out, m, n, nnz, d_rp, d_col, d_val, d_sens, value_type,
-                                 max_frames, block_rows, layout, stream);
+    return create_plan(out, m, n, nnz, d_rp, d_col, d_val, d_sens, value_type, max_frames,
+                       block_rows, o.layout, gdims, o.group, stream);
   if (!out) return fail(RAPTR_ERR_INVALID_ARG, "null output");
   *out = nullptr;
   if (m < 0 || n <= 0 || nnz < 0) return fail(RAPTR_ERR_INVALID_ARG, "bad sizes");
@@ -1486,8 +2316,8 @@ int raptr_recon_create_ordered(raptr_recon** out, int m, int n, int64_t nnz,
     }
   }
   if (!rc)
-    rc = raptr_recon_create_ex(out, m, n, nnz, prp, pcol, pval, d_sens, value_type, max_frames,
-                               block_rows, layout, stream);
+    rc = create_plan(out, m, n, nnz, prp, pcol, pval, d_sens, value_type, max_frames,
+                     block_rows, o.layout, gdims, o.group, stream);
   (void)hipStreamSynchronize(st);
   (void)hipFree(tmp);
   if (rc) return rc;
@@ -1666,6 +2496,40 @@ int raptr_recon_set_broadcast(raptr_recon* R, int on) {
 }
 
 int raptr_recon_get_broadcast(const raptr_recon* R) { return R ? R->bcast64 : -1; }
+
+int raptr_recon_set_grouped(raptr_recon* R, int on, int unroll, int kind) {
+  if (!R) return fail(RAPTR_ERR_INVALID_ARG, "null plan");
+  if (on && !R->gbp) return fail(RAPTR_ERR_INVALID_ARG, "plan has no grouped layouts");
+  if (R->gbp && !grp_shape_ok(R->grp_g, unroll, kind))
+    return fail(RAPTR_ERR_INVALID_ARG,
+                "groups of %d take unroll %d (kind 1) or %d (kind 0)%s; got unroll %d kind %d",
+                R->grp_g, R->grp_g == 4 ? 8 : (R->grp_g == 8 ? 4 : 2),
+                R->grp_g == 4 ? 16 : (R->grp_g == 8 ? 8 : 4),
+                R->grp_g == 16 ? " or 8, 16 (kind 2)" : "", unroll, kind);
+  if (R->grp64 != (on ? 1 : 0) || R->grp_unroll != unroll || R->grp_pipe != kind) {
+    for (auto& kv : R->graphs) {
+      if (kv.second.second) (void)hipGraphExecDestroy(kv.second.second);
+      if (kv.second.first) (void)hipGraphDestroy(kv.second.first);
+    }
+    R->graphs.clear();
+  }
+  R->grp64 = on ? 1 : 0;
+  R->grp_unroll = unroll;
+  R->grp_pipe = kind;
+  return RAPTR_OK;
+}
+
+int raptr_recon_get_grouped(const raptr_recon* R, int* group, int* on, int* unroll,
+                            int* kind, int64_t* fwd_entries, int64_t* bwd_entries) {
+  if (!R) return fail(RAPTR_ERR_INVALID_ARG, "null plan");
+  if (group) *group = R->gbp ? R->grp_g : 0;
+  if (on) *on = R->grp64;
+  if (unroll) *unroll = R->grp_unroll;
+  if (kind) *kind = R->grp_pipe;
+  if (fwd_entries) *fwd_entries = R->gbp ? R->gfnnz : 0;
+  if (bwd_entries) *bwd_entries = R->gbp ? R->gbnnz : 0;
+  return RAPTR_OK;
+}
 
 int raptr_recon_set_tiled(raptr_recon* R, int fwd_lanes, int fwd_unroll, int bwd_lanes,
                           int bwd_unroll) {

@@ -123,6 +123,21 @@ struct raptr_recon {
   float* tpart = nullptr;      // ntile*m partial row sums of the forward
   float* cpart = nullptr;      // ntile*tile_wgs*tile_cols partial column sums
   int t_fwd_lanes = 8, t_fwd_unroll = 2, t_bwd_lanes = 8, t_bwd_unroll = 2;
+  // grouped 64-frame layouts (max_frames = 64): forward groups of 4 plan
+  // rows and backward groups of 4 consecutive columns per row block, each
+  // an ascending union list with 4 values per union entry
+  int grp64 = 0;       // use the grouped kernels for 64-frame passes
+  int grp_g = 16;      // members per group (4, 8, 16)
+  int* gcols = nullptr;  // ngc * grp_g member columns of the backward groups (-1: none)
+  int grp_unroll = 8;  // union entries in flight per wave (4, 8, 16)
+  int grp_pipe = 1;    // kernel kind: 0 plain, 1 pipelined scalar loads, 2 matrix cores
+  int64_t ngf = 0, ngc = 0, gfnnz = 0, gbnnz = 0;
+  int64_t* gfp = nullptr;  // ngf + 1
+  int* gfcol = nullptr;
+  void* gfval = nullptr;   // 4 values per union entry
+  int64_t* gbp = nullptr;  // nblk * ngc + 1, (block, column group) order
+  int* gbrow = nullptr;
+  void* gbval = nullptr;
   // plan row order: plan row p holds the caller's row rperm[p] (null:
   // identity); yp holds the counts in plan order
   int* rperm = nullptr;

@@ -38,13 +38,15 @@ class Reconstructor:
     width."""
 
     def __init__(self, A, sens, value_type="f32", max_frames=1, block_rows=0, stream=None,
-                 layout="auto", row_order=None):
+                 layout="auto", row_order=None, grid=None, group=0):
         """row_order: None (the CSR's row order), a device or host int32
         permutation (plan row p = CSR row row_order[p]), or a
         ``MeasurementSetup`` with the CSR's channel ids given as
         ``A.channels`` / a (setup, channels) pair, from which the locality
         order (sysmat.locality_order) is computed.  Counts, images and the
-        forward's ratios keep the CSR's row order at the interface."""
+        forward's ratios keep the CSR's row order at the interface.
+        grid (VoxelGrid) / group (4, 8, 16; 0 = 16): the 64-frame grouped
+        layouts' group size, with backward groups as bricks of the grid."""
         self.m, self.n, self.nnz = A.m, A.n, A.nnz
         self.value_type = value_type
         self.max_frames = max_frames
@@ -53,10 +55,13 @@ class Reconstructor:
         self.layout = layout
         order = _row_order(row_order, A, stream)
         p = C.c_void_p()
-        capi.call("raptr_recon_create_ordered", C.byref(p), A.m, A.n, A.nnz, A.row_ptr.ptr,
+        self._grid = grid.struct() if grid is not None else None
+        opts = capi.raptr_recon_options(LAYOUTS[layout], order.ptr if order is not None else None,
+                                        C.pointer(self._grid) if self._grid is not None else None,
+                                        int(group))
+        capi.call("raptr_recon_create_opts", C.byref(p), A.m, A.n, A.nnz, A.row_ptr.ptr,
                   A.col.ptr, A.val.ptr, sens.ptr, VALUE_TYPES[value_type], int(max_frames),
-                  int(block_rows), LAYOUTS[layout], order.ptr if order is not None else None,
-                  stream)
+                  int(block_rows), C.byref(opts), stream)
         self.handle = p.value
         self.row_ordered = bool(capi.lib().raptr_recon_row_ordered(self.handle))
 
@@ -75,6 +80,12 @@ class Reconstructor:
                "bwd_blocks_per_launch": capi.lib().raptr_recon_get_blocks_per_launch(self.handle),
                "transpose_nnz": capi.lib().raptr_recon_transpose_nnz(self.handle),
                "row_ordered": bool(capi.lib().raptr_recon_row_ordered(self.handle))}
+        gi = self.grouped()
+        if gi["group"]:
+            out.update({"grouped64": gi["on"], "group": gi["group"],
+                        "grouped_unroll": gi["unroll"], "grouped_kind": gi["kind"],
+                        "grouped_fwd_entries": gi["fwd_entries"],
+                        "grouped_bwd_entries": gi["bwd_entries"]})
         if self.layout == "tiled":
             t = [C.c_int() for _ in range(6)]
             capi.call("raptr_recon_get_tiled", self.handle, *[C.byref(v) for v in t])
@@ -84,6 +95,28 @@ class Reconstructor:
 
     def set_broadcast(self, on):
         capi.call("raptr_recon_set_broadcast", self.handle, 1 if on else 0)
+
+    # grouped kernel shapes per group size: (unroll, kind); kind 0 vector
+    # FMAs, 1 the same pipelined, 2 matrix cores
+    GROUP_SHAPES = {4: ((8, 1), (16, 0)), 8: ((4, 1), (8, 0)),
+                    16: ((8, 2), (16, 2), (2, 1), (4, 0))}
+
+    def grouped(self):
+        """dict of the grouped 64-frame layouts: group size (0: none), on,
+        unroll, kind, forward / backward union entries."""
+        v = [C.c_int() for _ in range(4)] + [C.c_int64(), C.c_int64()]
+        capi.call("raptr_recon_get_grouped", self.handle, *[C.byref(t) for t in v])
+        return dict(zip(("group", "on", "unroll", "kind", "fwd_entries", "bwd_entries"),
+                        [t.value for t in v]))
+
+    def set_grouped(self, on, unroll=None, kind=None):
+        """unroll / kind None: the group size's first shape."""
+        g = self.grouped()["group"]
+        shapes = self.GROUP_SHAPES.get(g, ((8, 1), (16, 0)))
+        if unroll is None or kind is None:
+            unroll, kind = [(u, k) for u, k in shapes
+                            if (unroll is None or u == unroll) and (kind is None or k == kind)][0]
+        capi.call("raptr_recon_set_grouped", self.handle, 1 if on else 0, int(unroll), int(kind))
 
     def set_blocks_per_launch(self, blocks):
         capi.call("raptr_recon_set_blocks_per_launch", self.handle, int(blocks))
@@ -164,6 +197,9 @@ class Reconstructor:
             return ({"layout": "tiled", "fwd_lanes": fl, "fwd_unroll": fu, "bwd_lanes": bl,
                      "bwd_unroll": bu_}, best[1])
         bu = inf["bwd_unroll"]
+        grp = nframes == 64 and inf.get("group", 0) > 0
+        if grp:  # tune the general kernels first
+            self.set_grouped(False)
         if nframes == 1:
             fshapes = [(fl, fu) for fl in (16, 32, 64) for fu in (1, 2, 4)]
         else:
@@ -210,11 +246,34 @@ class Reconstructor:
                 bc = 1
             else:
                 self.set_broadcast(False)
+        ms_best = timed()
+        g_on, g_u, g_p = 0, 0, 0
+        if grp:
+            # grouped layouts: union entries per step (plain or pipelined) x
+            # row blocks per launch
+            best_g = None
+            for gu, gp in self.GROUP_SHAPES[inf["group"]]:
+                for gb in (1, 4, 16):
+                    if gb > 1 and gb >= inf["row_blocks"]:
+                        continue
+                    self.set_grouped(True, gu, gp)
+                    self.set_blocks_per_launch(gb)
+                    ms = timed()
+                    if best_g is None or ms < best_g[1]:
+                        best_g = ((gu, gp, gb), ms)
+            if best_g[1] < ms_best:
+                (g_u, g_p, bpl), ms_best = best_g
+                g_on = 1
+                self.set_grouped(True, g_u, g_p)
+            else:
+                self.set_grouped(False)
+            self.set_blocks_per_launch(bpl)
         capi.call("raptr_memcpy_d2d", x.ptr, x_save.ptr, x.nbytes, stream)
         cfg = {"fwd_lanes": fl, "fwd_unroll": fu, "bwd_unroll": u, "bwd_staged": st,
                "bwd_lanes_per_column": lpc, "bwd_workgroups": wgs, "bwd_blocks_per_launch": bpl,
-               "broadcast64": bc}
-        return cfg, best[1]
+               "broadcast64": bc, "grouped64": g_on, "grouped_unroll": g_u,
+               "grouped_kind": g_p}
+        return cfg, ms_best
 
     def close(self):
         if getattr(self, "handle", None):

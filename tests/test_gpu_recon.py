@@ -157,12 +157,16 @@ def test_recon_batched_frames_match_single(gpu, oracle, small_system, F):
         assert _rel(X[:, f], x1) <= 1e-6, f
         if f in (0, F - 1):
             assert _rel(X[:, f], _oracle_mlem(oracle, S, Y[:, f].astype(np.float64), 5)) <= 1e-4
-    # 64 frames: the kernels with in-row broadcast of values and indices
+    # 64 frames: the grouped layouts are the default; the general kernels
+    # with and without in-row broadcast of values and indices agree
     if F == 64:
-        Rb.set_broadcast(True)
-        Xc = Rb.mlem(Y, X0, 5)
-        assert _rel(Xc, X) <= 1e-6
-        Rb.set_broadcast(False)
+        assert Rb.grouped()["on"] == 1
+        Rb.set_grouped(False)
+        for bc in (True, False):
+            Rb.set_broadcast(bc)
+            Xc = Rb.mlem(Y, X0, 5)
+            assert _rel(Xc, X) <= 1e-6
+        Rb.set_grouped(True)
     # fewer frames than the plan's maximum, in the same plan
     if F >= 4:
         Xh = Rb.mlem(Y[:, :F // 2].copy(), X0[:, :F // 2].copy(), 5)
@@ -313,3 +317,73 @@ def test_recon_row_order_must_be_a_permutation(gpu, small_system):
         Reconstructor(S["A"], S["sens"], "f32", 1, row_order=bad)
     with pytest.raises(RaptrError, match="permutation"):
         Reconstructor(S["A"], S["sens"], "f32", 1, row_order=np.arange(1, m + 1, dtype=np.int32))
+
+
+@pytest.mark.parametrize("vt,tol", [("f32", 1e-6), ("f16", 1e-5), ("bf16", 1e-5), ("u16", 1e-4)])
+@pytest.mark.parametrize("group,bricks", [(4, False), (8, True), (16, True), (16, False)])
+def test_recon_grouped64(gpu, oracle, small_system, vt, tol, group, bricks):
+    """64-frame grouped layouts (G rows / G voxels per union list; voxel
+    groups as bricks of the grid or runs of column ids) against the
+    general 64-frame kernels of the same plan, for every value type and
+    kernel shape, with several row blocks per launch, in the caller's and
+    in the locality row order; graph replays equal eager runs bit for
+    bit."""
+    from raptr_amd import synthetic
+    from raptr_amd.device import DeviceArray, Stream
+    from raptr_amd.recon import Reconstructor
+    S = small_system
+    n, m = S["grid"].nvox, S["A"].m
+    ybar = S["ybar"][S["meas"]]
+    Y = np.stack([synthetic.poisson_counts(ybar, tot, seed=40 + f)
+                  for f, tot in enumerate(np.geomspace(1e4, 2e6, 64))], axis=1).astype(np.float32)
+    X0 = np.full((n, 64), 1.0 / n, np.float32)
+    for order in (None, S["setup"]):
+        R = Reconstructor(S["A"], S["sens"], vt, 64, block_rows=1500, row_order=order,
+                          grid=S["grid"] if bricks else None, group=group)
+        gi = R.grouped()
+        assert gi["on"] == 1 and gi["group"] == group
+        # union lists are no longer than the member lists together
+        assert 0 < gi["fwd_entries"] <= S["A"].nnz and 0 < gi["bwd_entries"] <= S["A"].nnz
+        R.set_grouped(False)
+        ref = R.mlem(Y, X0, 5)
+        for gu, gp in Reconstructor.GROUP_SHAPES[group]:
+            for bpl in (1, 3):
+                R.set_grouped(True, gu, gp)
+                R.set_blocks_per_launch(bpl)
+                X = R.mlem(Y, X0, 5)
+                rel = _rel(X, ref)
+                print(f"{vt} G={group} bricks={bricks} order={order is not None} unroll {gu} "
+                      f"kind {gp} bpl {bpl}: rel {rel:.2e} (union fwd {gi['fwd_entries']} "
+                      f"bwd {gi['bwd_entries']} of {S['A'].nnz})")
+                assert rel <= tol
+        if vt == "f32":
+            x64 = _oracle_mlem(oracle, S, Y[:, 7].astype(np.float64), 5)
+            assert _rel(X[:, 7].astype(np.float64), x64) <= 1e-4
+            st = Stream()
+            dy, dx = DeviceArray.from_numpy(Y.reshape(-1)), DeviceArray.from_numpy(X0.reshape(-1))
+            R.run(dy, dx, 3, 64, use_graph=False, stream=st.handle)
+            eager = dx.numpy(stream=st.handle)
+            for _ in range(3):
+                dx.copy_from(X0.reshape(-1), stream=st.handle)
+                R.run(dy, dx, 3, 64, use_graph=True, stream=st.handle)
+                st.sync()
+                np.testing.assert_array_equal(dx.numpy(stream=st.handle), eager)
+                R.mlem(Y[:, :1].copy(), X0[:, :1].copy(), 1, use_graph=False)
+        R.close()
+
+
+def test_recon_grouped64_rejects_bad_shapes(gpu, small_system):
+    from raptr_amd import RaptrError
+    from raptr_amd.recon import Reconstructor
+    S = small_system
+    with pytest.raises(RaptrError, match="group"):
+        Reconstructor(S["A"], S["sens"], "f32", 64, group=6)
+    R = Reconstructor(S["A"], S["sens"], "f32", 64, grid=S["grid"], group=8)
+    with pytest.raises(RaptrError, match="unroll"):
+        R.set_grouped(True, 2, 1)
+    with pytest.raises(RaptrError, match="unroll"):
+        R.set_grouped(True, 8, 2)  # matrix cores need groups of 16
+    R1 = Reconstructor(S["A"], S["sens"], "f32", 16)
+    assert R1.grouped()["group"] == 0
+    with pytest.raises(RaptrError, match="no grouped"):
+        R1.set_grouped(True, 8, 1)
