@@ -46,10 +46,14 @@ ERR_ITERS = 30  # iterations of the mixed-precision error figures
 # reads back) for the kernel shapes the autotuner picks on this workload;
 # other shapes report no traffic figure.
 MEASURED_TRAFFIC = {
+    # forward (lanes, unroll, row order); one launch per pass
     ("k_fwd1", 32, 2, "list"): 2 * 7342510 * 1024 + 17622 * 1024,
-    # backward: unroll 4, 9 row blocks, 1 block per launch, not LDS-staged
-    # (average over the 9 launches of one pass), rows in list order
+    ("k_fwd1", 32, 1, "locality"): 2 * 7624320 * 1024 + 17622 * 1024,
+    ("k_fwd1", 16, 2, "locality"): 2 * 8096520 * 1024 + 17634 * 1024,
+    # backward (unroll, row blocks, blocks per launch, staged, row order);
+    # average over the launches of one pass
     ("k_bwd", 4, 9, 1, 0, "list"): 2 * 734223 * 1024 + 1427 * 1024,
+    ("k_bwd", 4, 9, 4, 0, "locality"): 2 * 2544819 * 1024 + 1603 * 1024,
 }
 
 
