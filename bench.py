@@ -82,7 +82,7 @@ def parse():
                          "list's, or try both")
     ap.add_argument("--group", type=int, default=16, choices=(4, 8, 16),
                     help="rows / voxels per group of the 64-frame grouped layouts")
-    ap.add_argument("--block-slices-mb", default="8,32",
+    ap.add_argument("--block-slices-mb", default="2,8,32",
                     help="ratio-slice sizes (MB) of the batched plans' row blocks to try")
     return ap.parse_args()
 

@@ -415,9 +415,13 @@ int raptr_recon_get_broadcast(const raptr_recon* plan);
  * matrix cores (G = 16 only: v_mfma_f32_16x16x4f32 over 4 union entries x
  * 16 frames, keys and values from coalesced vector loads).  Shapes per G:
  * 4: unroll 8 kind 1 or 16 kind 0; 8: 4 kind 1 or 8 kind 0; 16: 2 kind 1,
- * 4 kind 0, 8 or 16 kind 2 (the default).  get: group = G (0: no grouped
- * layouts), fwd/bwd_entries = union entries of each layout (lists padded to
- * multiples of 16). */
+ * 4 kind 0, 8 or 16 kind 2 (the default) or 3.  Kind 3 = kind 2 with a
+ * backward in which each XCD takes one eighth of the row blocks and writes
+ * its own partial image (8 x n x 64 floats, allocated with the plan); a
+ * fixed-order reduce adds them.  unroll applies to the forward (backward
+ * steps are 8 entries).  get: group = G (0: no grouped layouts), fwd/
+ * bwd_entries = union entries of each layout (lists padded to multiples of
+ * 16 / 8). */
 int raptr_recon_set_grouped(raptr_recon* plan, int on, int unroll, int kind);
 int raptr_recon_get_grouped(const raptr_recon* plan, int* group, int* on, int* unroll,
                             int* kind, int64_t* fwd_entries, int64_t* bwd_entries);

@@ -642,7 +642,11 @@ __global__ __launch_bounds__(256) void k_bwd64(int n, const int64_t* __restrict_
 // serves all G members.  Every member's sum still runs over its own entries
 // in ascending order, with exact zero terms between them.
 constexpr int kGrpPad = 64;    // union entries of tail padding (>= 2 x the largest unroll)
-constexpr int kGrpAlign = 16;  // union lists padded to multiples of this (value 0, last key)
+// union lists are padded (value 0, the list's last key) to multiples of
+// kAlignFwd entries (row groups) / kAlignBwd entries (block x brick lists,
+// which are short)
+constexpr int kAlignFwd = 16;
+constexpr int kAlignBwd = 8;
 
 // forward groups: list s of group g is CSR row G*g + s
 template <int G>
@@ -719,12 +723,12 @@ __device__ __forceinline__ void merge_lists(const L& lists, int64_t g, Emit emit
 }
 
 template <int G, class L>
-__global__ void k_grp_count(L lists, int64_t ng, int64_t* __restrict__ cnt) {
+__global__ void k_grp_count(L lists, int64_t ng, int align, int64_t* __restrict__ cnt) {
   for (int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; g < ng;
        g += int64_t(gridDim.x) * blockDim.x) {
     int64_t c = 0;
     merge_lists<G>(lists, g, [&](int, const int64_t (&)[G]) { ++c; });
-    cnt[g] = (c + kGrpAlign - 1) / kGrpAlign * kGrpAlign;
+    cnt[g] = (c + align - 1) / align * align;
   }
 }
 
@@ -1049,7 +1053,7 @@ __device__ __forceinline__ void mstep(const MSet<VT, U>& S, MSet<VT, U>& T, bool
 }
 
 // acc[fb] += the group's sums over the union entries [b, en) (a multiple
-// of kGrpAlign) for frames 16*fb .. 16*fb+15, in steps of 4*U entries with
+// of 4*U) for frames 16*fb .. 16*fb+15, in steps of 4*U entries with
 // three register sets (loads two steps ahead); reads up to 8*U entries past
 // en (inside the next list or the tail padding)
 template <int VT, int U>
@@ -1129,7 +1133,7 @@ __global__ __launch_bounds__(256) void k_bwdm64(int ngc, const int64_t* __restri
   for (int fb = 0; fb < 4; ++fb) acc[fb] = f4v{0.0f, 0.0f, 0.0f, 0.0f};
   if (h < ngc) {
     for (int bb = 0; bb < nbl; ++bb)
-      mfma_sum<VT, U>(grow, gval, gp[int64_t(bb) * ngc + h], gp[int64_t(bb) * ngc + h + 1],
+      mfma_sum<VT, 2>(grow, gval, gp[int64_t(bb) * ngc + h], gp[int64_t(bb) * ngc + h + 1],
                       e + r16, kq, r16, acc);
   }
   double part[4] = {0.0, 0.0, 0.0, 0.0};
@@ -1169,6 +1173,72 @@ __global__ __launch_bounds__(256) void k_bwdm64(int ngc, const int64_t* __restri
       partials[int64_t(lb) * 64 + threadIdx.x] = t;
     }
   }
+}
+
+// Backward with per-XCD row ranges (64 frames, bricks of 16): workgroup i
+// runs on XCD i % 8 (the dispatcher's round-robin), which takes row blocks
+// [nblk*x/8, nblk*(x+1)/8).  Each wave walks its brick through all of its
+// XCD's blocks; the bricks resident on one XCD at a time are consecutive
+// (neighbouring voxels) and walk the same small ratio slices together, so
+// their ratio gathers hit that XCD's L2.  Each XCD writes its partial image
+// sums to its own slice of c8; k_bwd_xreduce adds the 8 slices in a fixed
+// order and applies the update.  On a device that assigns workgroups
+// differently the result is the same (only the cache behaviour changes).
+template <int VT>
+__global__ __launch_bounds__(256) void k_bwdm64x(int ngc, int nblk, int n,
+                                                 const int64_t* __restrict__ gp,
+                                                 const int* __restrict__ grow,
+                                                 const typename Val<VT>::T* __restrict__ gval,
+                                                 const int* __restrict__ cols,
+                                                 const float* __restrict__ e,
+                                                 float* __restrict__ c8) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int kq = lane >> 4, r16 = lane & 15;
+  const int xcd = int(blockIdx.x % 8);
+  const int h = int(blockIdx.x / 8) * (256 / kWave) + wave_id();
+  if (h >= ngc) return;
+  const int b0 = int(int64_t(nblk) * xcd / 8), b1 = int(int64_t(nblk) * (xcd + 1) / 8);
+  f4v acc[4];
+#pragma unroll
+  for (int fb = 0; fb < 4; ++fb) acc[fb] = f4v{0.0f, 0.0f, 0.0f, 0.0f};
+  for (int b = b0; b < b1; ++b)
+    mfma_sum<VT, 2>(grow, gval, gp[int64_t(b) * ngc + h], gp[int64_t(b) * ngc + h + 1], e + r16,
+                    kq, r16, acc);
+  float* out = c8 + int64_t(xcd) * n * 64;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int j = cols[int64_t(h) * 16 + 4 * kq + i];
+    if (j < 0) continue;
+#pragma unroll
+    for (int fb = 0; fb < 4; ++fb) out[int64_t(j) * 64 + 16 * fb + r16] = acc[fb][i];
+  }
+}
+
+// c = sum of the 8 partial images (XCD order), x' = x*c/(s + (s==0)), and
+// FP64 partial sums per workgroup (4 voxels) and frame
+__global__ __launch_bounds__(256) void k_bwd_xreduce(int n, const float* __restrict__ c8,
+                                                     const float* __restrict__ x,
+                                                     const float* __restrict__ s,
+                                                     float* __restrict__ xx,
+                                                     double* __restrict__ partials) {
+  const int f = threadIdx.x & 63, jj = threadIdx.x >> 6;
+  const int64_t j = int64_t(blockIdx.x) * 4 + jj;
+  double part = 0.0;
+  if (j < n) {
+    float cv = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) cv += c8[(int64_t(q) * n + j) * 64 + f];
+    float d = s[j];
+    d += float(d == 0.0f);
+    const float nv = x[j * 64 + f] * cv / d;
+    xx[j * 64 + f] = nv;
+    part = double(nv);
+  }
+  __shared__ double red[4][64];
+  red[jj][f] = part;
+  __syncthreads();
+  if (threadIdx.x < 64)
+    partials[int64_t(blockIdx.x) * 64 + f] = red[0][f] + red[1][f] + red[2][f] + red[3][f];
 }
 
 // brick of the backward groups: G voxels (x, y, z extents)
@@ -1456,8 +1526,8 @@ namespace {
 // ------------------------------------------------ grouped layout builds
 // union lists of ng groups: count pass, scan, fill pass
 template <int G, class L>
-int build_groups(raptr_recon* R, const L& lists, int64_t ng, int64_t** gptr, int** gkey,
-                 void** gval, int64_t* gnnz, hipStream_t st) {
+int build_groups(raptr_recon* R, const L& lists, int64_t ng, int align, int64_t** gptr,
+                 int** gkey, void** gval, int64_t* gnnz, hipStream_t st) {
   int rc;
   if ((rc = ralloc(R, gptr, size_t(ng) + 1))) return rc;
   const size_t sb = scan_workspace_bytes(ng > 0 ? ng : 1);
@@ -1478,7 +1548,7 @@ int build_groups(raptr_recon* R, const L& lists, int64_t ng, int64_t** gptr, int
   int64_t h = 0;
   if (ng > 0) {
     hipLaunchKernelGGL((k_grp_count<G, L>), dim3(grid_for(ng, 256, 1 << 16)), dim3(256), 0, st,
-                       lists, ng, cnt);
+                       lists, ng, align, cnt);
     rc = scan_exclusive_i64(cnt, *gptr, ng, tot, sws, sb, st);
     if (!rc) {
       hipLaunchKernelGGL(k_set_last, dim3(1), dim3(1), 0, st, *gptr, int(ng),
@@ -1527,7 +1597,7 @@ bool grp_shape_ok(int G, int U, int kind) {
     case 8: return (U == 4 && kind == 1) || (U == 8 && kind == 0);
     case 16:
       return (U == 2 && kind == 1) || (U == 4 && kind == 0) ||
-             ((U == 8 || U == 16) && kind == 2);
+             ((U == 8 || U == 16) && (kind == 2 || kind == 3));
   }
   return false;
 }
@@ -1561,7 +1631,7 @@ int launch_fwd_any(const raptr_recon* R, int F, const float* x, const float* y, 
   if (F == 64 && R->grp64) {
     const unsigned gg = unsigned((R->ngf + 3) / 4);
     const int GU = R->grp_unroll, GG = R->grp_g;
-    if (R->grp_pipe == 2) {
+    if (R->grp_pipe >= 2) {
 #define RAPTR_M64(VV, UU)                                                                    \
       if (R->vt == VV && GG == 16 && GU == 4 * UU) {                                         \
         hipLaunchKernelGGL((k_fwdm64<VV, UU>), dim3(gg), dim3(256), 0, st, R->ngf, m, R->gfp,\
@@ -1636,9 +1706,10 @@ int launch_bwd_any(const raptr_recon* R, int F, int blk, int nbl, const float* x
     if (nb_out) *nb_out = int(gg);
     const int64_t* gp = R->gbp + int64_t(blk) * R->ngc;
     const int GU = R->grp_unroll, GG = R->grp_g;
-    if (R->grp_pipe == 2) {
+    if (R->grp_pipe >= 2) {
+      // backward lists are aligned to 8 entries: steps of 8 (U = 2)
 #define RAPTR_MB64(VV, UU)                                                                   \
-      if (R->vt == VV && GG == 16 && GU == 4 * UU) {                                         \
+      if (R->vt == VV && GG == 16 && UU == 2) {                                              \
         hipLaunchKernelGGL((k_bwdm64<VV, UU>), dim3(gg), dim3(256), 0, st, int(R->ngc), gp,   \
                            nbl, R->gbrow, (const typename Val<VV>::T*)R->gbval,              \
                            (const int*)R->gcols, R->e, R->c, first, last, x, R->s, R->xx,    \
@@ -1646,7 +1717,7 @@ int launch_bwd_any(const raptr_recon* R, int F, int blk, int nbl, const float* x
         RAPTR_LAUNCH_CHECK();                                                                \
         return RAPTR_OK;                                                                     \
       }
-#define RAPTR_MB64V(VV) RAPTR_MB64(VV, 2) RAPTR_MB64(VV, 4)
+#define RAPTR_MB64V(VV) RAPTR_MB64(VV, 2)
       RAPTR_MB64V(kF32) RAPTR_MB64V(kF16) RAPTR_MB64V(kBF16) RAPTR_MB64V(kU16)
 #undef RAPTR_MB64V
 #undef RAPTR_MB64
@@ -1777,6 +1848,29 @@ int launch_bwd_lds(const raptr_recon* R, const float* x, hipStream_t st, int* nb
   return fail(RAPTR_ERR_INVALID_ARG, "staged backward shape unsupported");
 }
 
+// 64-frame backward with per-XCD row ranges and partial images, then the
+// reduce + update (grouped layouts of 16, kind 3)
+int launch_bwd_xcd(const raptr_recon* R, const float* x, hipStream_t st, int* nb_out) {
+  if (!R->c8 || R->grp_g != 16)
+    return fail(RAPTR_ERR_INVALID_ARG, "plan has no per-XCD partial images");
+  const unsigned gg = unsigned(8 * ((R->ngc + 3) / 4));
+#define RAPTR_X64(VV)                                                                        \
+  if (R->vt == VV)                                                                           \
+    hipLaunchKernelGGL((k_bwdm64x<VV>), dim3(gg), dim3(256), 0, st, int(R->ngc), R->nblk,     \
+                       R->n, (const int64_t*)R->gbp, (const int*)R->gbrow,                   \
+                       (const typename Val<VV>::T*)R->gbval, (const int*)R->gcols,           \
+                       (const float*)R->e, R->c8);
+  RAPTR_X64(kF32) RAPTR_X64(kF16) RAPTR_X64(kBF16) RAPTR_X64(kU16)
+#undef RAPTR_X64
+  RAPTR_LAUNCH_CHECK();
+  const unsigned nr = unsigned((int64_t(R->n) + 3) / 4);
+  hipLaunchKernelGGL(k_bwd_xreduce, dim3(nr), dim3(256), 0, st, R->n, (const float*)R->c8, x,
+                     (const float*)R->s, R->xx, R->partials);
+  RAPTR_LAUNCH_CHECK();
+  if (nb_out) *nb_out = int(nr);
+  return RAPTR_OK;
+}
+
 int iteration(raptr_recon* R, int F, const float* y, float* x, hipStream_t st) {
   int nb = 0;
   if (R->layout == RAPTR_LAYOUT_TILED) {
@@ -1785,6 +1879,9 @@ int iteration(raptr_recon* R, int F, const float* y, float* x, hipStream_t st) {
   } else if (F == 1 && R->bwd_lds) {
     RAPTR_TRY(launch_fwd_any(R, F, x, y, R->e, st));
     RAPTR_TRY(launch_bwd_lds(R, x, st, &nb));
+  } else if (F == 64 && R->grp64 && R->grp_pipe == 3) {
+    RAPTR_TRY(launch_fwd_any(R, F, x, y, R->e, st));
+    RAPTR_TRY(launch_bwd_xcd(R, x, st, &nb));
   } else {
     RAPTR_TRY(launch_fwd_any(R, F, x, y, R->e, st));
     for (int b = 0; b < R->nblk; b += R->bwd_bpl)
@@ -2058,7 +2155,8 @@ int create_plan(raptr_recon** out, int m, int n, int64_t nnz, const int64_t* d_r
 #define RAPTR_RG(G_)                                                                       \
   case G_: {                                                                               \
     RowLists<G_> L{d_rp, d_col, d_val, int64_t(m)};                                        \
-    rc = build_groups<G_>(R, L, R->ngf, &R->gfp, &R->gfcol, &R->gfval, &R->gfnnz, st);      \
+    rc = build_groups<G_>(R, L, R->ngf, kAlignFwd, &R->gfp, &R->gfcol, &R->gfval, &R->gfnnz, \
+                          st);                                                             \
     break;                                                                                 \
   }
         RAPTR_RG(4) RAPTR_RG(8) RAPTR_RG(16)
@@ -2183,14 +2281,16 @@ int create_plan(raptr_recon** out, int m, int n, int64_t nnz, const int64_t* d_r
   case G_: {                                                                               \
     ColLists<G_> L{(const int64_t*)uptr, (const uint32_t*)idx, (const uint32_t*)rowid,     \
                    d_val, (const int*)R->gcols, n, int(R->ngc)};                           \
-    rc = build_groups<G_>(R, L, int64_t(R->nblk) * R->ngc, &R->gbp, &R->gbrow, &R->gbval,  \
-                          &R->gbnnz, st);                                                  \
+    rc = build_groups<G_>(R, L, int64_t(R->nblk) * R->ngc, kAlignBwd, &R->gbp, &R->gbrow,  \
+                          &R->gbval, &R->gbnnz, st);                                       \
     break;                                                                                 \
   }
             RAPTR_CG(4) RAPTR_CG(8) RAPTR_CG(16)
 #undef RAPTR_CG
           }
         }
+        // per-XCD partial images of the kind-3 backward (groups of 16)
+        if (!rc && G == 16) rc = ralloc(R, &R->c8, size_t(8) * n * 64);
         if (!rc) R->grp64 = 1;
       }
       (void)hipStreamSynchronize(st);
@@ -2660,6 +2760,7 @@ int raptr_recon_backward(raptr_recon* R, int nframes, const float* d_x, raptr_st
     return fail(RAPTR_ERR_INVALID_ARG, "x must be aligned to the frame vector");
   if (R->layout == RAPTR_LAYOUT_TILED) return tiled_backward(R, d_x, R->e, S(stream), nullptr);
   if (nframes == 1 && R->bwd_lds) return launch_bwd_lds(R, d_x, S(stream), nullptr);
+  if (nframes == 64 && R->grp64 && R->grp_pipe == 3) return launch_bwd_xcd(R, d_x, S(stream), nullptr);
   for (int b = 0; b < R->nblk; b += R->bwd_bpl)
     RAPTR_TRY(launch_bwd_any(R, nframes, b, std::min(R->bwd_bpl, R->nblk - b), d_x, S(stream),
                              nullptr));

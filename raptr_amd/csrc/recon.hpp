@@ -130,7 +130,9 @@ struct raptr_recon {
   int grp_g = 16;      // members per group (4, 8, 16)
   int* gcols = nullptr;  // ngc * grp_g member columns of the backward groups (-1: none)
   int grp_unroll = 8;  // union entries in flight per wave (4, 8, 16)
-  int grp_pipe = 1;    // kernel kind: 0 plain, 1 pipelined scalar loads, 2 matrix cores
+  int grp_pipe = 1;    // kernel kind: 0 plain, 1 pipelined scalar loads, 2 matrix cores,
+                       // 3 matrix cores with per-XCD row ranges in the backward
+  float* c8 = nullptr;  // kind 3: 8 partial images [8][n][64]
   int64_t ngf = 0, ngc = 0, gfnnz = 0, gbnnz = 0;
   int64_t* gfp = nullptr;  // ngf + 1
   int* gfcol = nullptr;
