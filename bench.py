@@ -77,6 +77,8 @@ def parse():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serving-requests", type=int, default=120)
     ap.add_argument("--serving-rate", type=float, default=20.0, help="arrivals per second")
+    ap.add_argument("--serving-slots", type=int, default=64,
+                    help="frame slots per geometry lane (64: grouped layouts on the matrix cores)")
     ap.add_argument("--row-order", default="locality", choices=("locality", "list", "both"),
                     help="plan row order: locality (angle, midpoint, slope), the measurement "
                          "list's, or try both")
@@ -122,7 +124,7 @@ def main():
         print(json.dumps({"partial": True, "copy": copy, "device": dev}), flush=True)
         return
     if args.modes == "serving":
-        serve = bench_serving(log, args.serving_requests, args.serving_rate)
+        serve = bench_serving(log, args.serving_requests, args.serving_rate, args.serving_slots)
         print(json.dumps({"partial": True, "serving": serve, "device": dev}), flush=True)
         return
 
@@ -399,7 +401,7 @@ def main():
         del plans, R, dy, dx, de
         res.pop("A")  # free the full-size matrix first
         A = None
-        serve = bench_serving(log, args.serving_requests, args.serving_rate)
+        serve = bench_serving(log, args.serving_requests, args.serving_rate, args.serving_slots)
 
     out = {
         "metric": "frames/s batched vs plain MLEM at <=1e-4 rel err; p50/p99 latency; HBM GB/s",
@@ -464,18 +466,18 @@ def copy_peak(timed, st, nbytes=2 << 30, reps=5):
     return {"bytes": nbytes, "ms": round(ms, 4), "GBps": round(2 * nbytes / (ms * 1e-3) / 1e9, 1)}
 
 
-def bench_serving(log, nreq, rate):
+def bench_serving(log, nreq, rate, slots=64):
     """Poisson arrivals of reconstruction requests over three geometries
     (8x8 heads / 32^3 / 180 angles at 1 deg, 13x13 / 52^3 / 120 at 1.5 deg,
     16x16 / 96^3 / 60 at 3 deg; 2 mm voxels, 20 rays), 1e4-5e6 events and
-    10-50 iterations per request, served by continuous batching (16 slots per
-    geometry) from a pre-built device matrix cache.  Latency = arrival to
-    image on the host."""
+    10-50 iterations per request, served by continuous batching (`slots` frame
+    slots per geometry; 64: grouped-only plans on the matrix cores) from a
+    pre-built device matrix cache.  Latency = arrival to image on the host."""
     from raptr_amd import geometry as G
     from raptr_amd import serving, synthetic, sysmat
     geoms = [serving.Geometry(G.two_head_setup(p, na, 180.0 / na), G.centred_grid(v, 2.0), 20)
              for p, v, na in ((8, 32, 180), (13, 52, 120), (16, 96, 60))]
-    cache = serving.MatrixCache(200 << 30, frames=16)
+    cache = serving.MatrixCache(200 << 30, frames=slots)
     t = time.perf_counter()
     builds = {}
     for g in geoms:
@@ -495,7 +497,8 @@ def bench_serving(log, nreq, rate):
     out = stats.summary()
     out.update({"workload": f"{nreq} requests, Poisson arrivals at {rate}/s, geometries "
                             "8x8/32^3/180 angles, 13x13/52^3/120, 16x16/96^3/60 (2 mm voxels, "
-                            "20 rays), 1e4-5e6 events, 10-50 iterations, 16 slots per geometry",
+                            f"20 rays), 1e4-5e6 events, 10-50 iterations, {slots} slots per "
+                            "geometry",
                 "lanes": builds, "prep_s": round(t_prep, 1)})
     log(f"serving: p50 {out['p50_ms']:.0f} ms, p99 {out['p99_ms']:.0f} ms, "
         f"{out['requests_per_s']:.1f} req/s over {out['wall_s']:.1f} s")

@@ -334,6 +334,10 @@ int raptr_recon_create_ex(raptr_recon** plan, int m, int n, int64_t nnz,
  * the columns (n = nx*ny*nz), so that the backward groups of the 64-frame
  * grouped layouts are bricks of voxels (null: runs of consecutive column
  * ids).  group: members per group of those layouts (4, 8, 16; 0 = 16).
+ * grouped_only (max_frames = 64): build only the grouped layouts (and the
+ * row scales), not the padded CSR and padded transpose; such a plan runs
+ * 64-frame passes only, and needs about 40 % less device memory (matrices
+ * of many channels, e.g. serving lanes over every channel of a geometry).
  * No counterpart in the reference, which keeps the measurement list order
  * and computes per chunk (reco.cu:230-303). */
 typedef struct raptr_recon_options {
@@ -341,6 +345,7 @@ typedef struct raptr_recon_options {
   const int* d_row_order;
   const raptr_grid_t* grid;
   int group;
+  int grouped_only;
 } raptr_recon_options;
 int raptr_recon_create_opts(raptr_recon** plan, int m, int n, int64_t nnz,
                             const int64_t* d_row_ptr, const int* d_col, const float* d_val,
@@ -415,7 +420,7 @@ int raptr_recon_get_broadcast(const raptr_recon* plan);
  * matrix cores (G = 16 only: v_mfma_f32_16x16x4f32 over 4 union entries x
  * 16 frames, keys and values from coalesced vector loads).  Shapes per G:
  * 4: unroll 8 kind 1 or 16 kind 0; 8: 4 kind 1 or 8 kind 0; 16: 2 kind 1,
- * 4 kind 0, 8 or 16 kind 2 (the default) or 3.  Kind 3 = kind 2 with a
+ * 4 kind 0, 8 or 16 kind 2 or 3 (the default).  Kind 3 = kind 2 with a
  * backward in which each XCD takes one eighth of the row blocks and writes
  * its own partial image (8 x n x 64 floats, allocated with the plan); a
  * fixed-order reduce adds them.  unroll applies to the forward (backward

@@ -38,7 +38,8 @@ class raptr_grid_t(C.Structure):
 
 class raptr_recon_options(C.Structure):
     _fields_ = [("layout", C.c_int), ("d_row_order", C.c_void_p),
-                ("grid", C.POINTER(raptr_grid_t)), ("group", C.c_int)]
+                ("grid", C.POINTER(raptr_grid_t)), ("group", C.c_int),
+                ("grouped_only", C.c_int)]
 
 
 class RaptrError(RuntimeError):

@@ -89,6 +89,22 @@ __global__ __launch_bounds__(256) void k_pad_copy(const int64_t* rp, const int* 
   }
 }
 
+// per-row scales of 16-bit plans (as k_pad_copy computes them) without the
+// padded copy: one wave per row
+__global__ __launch_bounds__(256) void k_row_scales(const int64_t* rp, const float* val, int m,
+                                                    int vt, float* scale) {
+  const int64_t wid = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t nw = int64_t(gridDim.x) * blockDim.x / kWave;
+  for (int64_t i = wid; i < m; i += nw) {
+    float mx = 0.0f;
+    for (int64_t k = rp[i] + lane; k < rp[i + 1]; k += kWave) mx = fmaxf(mx, fabsf(val[k]));
+    for (int off = kWave / 2; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, kWave));
+    const float sc = mx > 0.0f ? mx : 1.0f;
+    if (lane == 0) scale[i] = vt == kU16 ? sc * (1.0f / 65535.0f) : sc;
+  }
+}
+
 // transpose keys: (row block << colbits) | col, payload = entry index
 __global__ void k_bt_keys(const int64_t* rp, const int* col, int m, int blk_rows, int colbits,
                           uint64_t* keys, uint32_t* idx) {
@@ -1472,23 +1488,24 @@ __global__ void k_bt_pack(const int64_t* pptr, int64_t total, int n, int blk_row
 }
 
 // scale_f = 1/sum_b partials[b][f] (FP64, fixed order); 0 for an empty frame
-__global__ void k_norms(const double* partials, int nb, int FR, float* scale, double* norm) {
+// one workgroup per frame f = blockIdx.x (each thread sums blocks t, t+256,
+// ... in order, then a fixed tree): the same order for every launch shape
+__global__ __launch_bounds__(256) void k_norms(const double* partials, int nb, int FR,
+                                               float* scale, double* norm) {
   __shared__ double red[256];
-  for (int f = 0; f < FR; ++f) {
-    double t = 0.0;
-    for (int b = threadIdx.x; b < nb; b += blockDim.x) t += partials[int64_t(b) * FR + f];
-    red[threadIdx.x] = t;
+  const int f = blockIdx.x;
+  double t = 0.0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) t += partials[int64_t(b) * FR + f];
+  red[threadIdx.x] = t;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
     __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-      if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-      double tot = red[0];
-      norm[f] = tot;
-      scale[f] = tot != 0.0 ? float(1.0 / tot) : 0.0f;
-    }
-    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double tot = red[0];
+    norm[f] = tot;
+    scale[f] = tot != 0.0 ? float(1.0 / tot) : 0.0f;
   }
 }
 
@@ -1606,6 +1623,8 @@ bool grp_shape_ok(int G, int U, int kind) {
 int launch_fwd_any(const raptr_recon* R, int F, const float* x, const float* y, float* e,
                    hipStream_t st) {
   if (R->m == 0) return RAPTR_OK;
+  if (R->grouped_only && !(F == 64 && R->grp64))
+    return fail(RAPTR_ERR_INVALID_ARG, "a grouped-only plan runs 64-frame grouped passes only");
   const int64_t m = R->m;
   const int U = R->fwd_unroll;
   if (F == 1) {
@@ -1696,6 +1715,8 @@ int launch_fwd_any(const raptr_recon* R, int F, const float* x, const float* y, 
 // one launch over row blocks [blk, blk + nbl)
 int launch_bwd_any(const raptr_recon* R, int F, int blk, int nbl, const float* x,
                    hipStream_t st, int* nb_out) {
+  if (R->grouped_only && !(F == 64 && R->grp64))
+    return fail(RAPTR_ERR_INVALID_ARG, "a grouped-only plan runs 64-frame grouped passes only");
   const int n = R->n;
   unsigned g = unsigned((n + 3) / 4);
   if (nb_out) *nb_out = int(g);
@@ -1887,7 +1908,7 @@ int iteration(raptr_recon* R, int F, const float* y, float* x, hipStream_t st) {
     for (int b = 0; b < R->nblk; b += R->bwd_bpl)
       RAPTR_TRY(launch_bwd_any(R, F, b, std::min(R->bwd_bpl, R->nblk - b), x, st, &nb));
   }
-  hipLaunchKernelGGL(k_norms, dim3(1), dim3(256), 0, st, (const double*)R->partials, nb, F,
+  hipLaunchKernelGGL(k_norms, dim3(F), dim3(256), 0, st, (const double*)R->partials, nb, F,
                      R->scale, R->norm);
   RAPTR_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_scale_frames, dim3(grid_for(int64_t(R->n) * F, 256, 8192)), dim3(256),
@@ -2030,7 +2051,7 @@ namespace {
 int create_plan(raptr_recon** out, int m, int n, int64_t nnz, const int64_t* d_rp,
                 const int* d_col, const float* d_val, const double* d_sens, int value_type,
                 int max_frames, int block_rows, int layout, const int* gdims, int group,
-                raptr_stream_t stream);
+                int grouped_only, raptr_stream_t stream);
 }  // namespace
 }  // namespace raptr
 
@@ -2049,7 +2070,7 @@ int raptr_recon_create_ex(raptr_recon** out, int m, int n, int64_t nnz, const in
                           int value_type, int max_frames, int block_rows, int layout,
                           raptr_stream_t stream) {
   return create_plan(out, m, n, nnz, d_rp, d_col, d_val, d_sens, value_type, max_frames,
-                     block_rows, layout, nullptr, 0, stream);
+                     block_rows, layout, nullptr, 0, 0, stream);
 }
 }  // extern "C"
 
@@ -2061,9 +2082,14 @@ namespace {
 int create_plan(raptr_recon** out, int m, int n, int64_t nnz, const int64_t* d_rp,
                 const int* d_col, const float* d_val, const double* d_sens, int value_type,
                 int max_frames, int block_rows, int layout, const int* gdims, int group,
-                raptr_stream_t stream) {
+                int grouped_only, raptr_stream_t stream) {
   if (!out) return fail(RAPTR_ERR_INVALID_ARG, "null output");
   if (group == 0) group = 16;
+  const bool go = grouped_only != 0;
+  if (go && (max_frames != 64 || layout != RAPTR_LAYOUT_BLOCKED || nnz <= 0 || m <= 0))
+    return fail(RAPTR_ERR_INVALID_ARG,
+                "a grouped-only plan needs max_frames 64, the blocked layout and a non-empty "
+                "matrix");
   if (group != 4 && group != 8 && group != 16)
     return fail(RAPTR_ERR_INVALID_ARG, "group must be 4, 8 or 16");
   if (gdims && int64_t(gdims[0]) * gdims[1] * gdims[2] != n)
@@ -2103,11 +2129,18 @@ int create_plan(raptr_recon** out, int m, int n, int64_t nnz, const int64_t* d_r
   if (layout == RAPTR_LAYOUT_TILED) {
     if ((rc = tiled_create(R, d_rp, d_col, d_val, st))) return bail(rc);
   } else {
-    // ---- forward: padded CSR
+    // ---- forward: padded CSR (grouped-only plans: the row scales only)
+    R->grouped_only = go ? 1 : 0;
     int64_t* plen = nullptr;
     int64_t* tot = nullptr;
-    if ((rc = ralloc(R, &R->frp, size_t(m) + 1))) return bail(rc);
-    {
+    if (go) {
+      if ((rc = ralloc(R, &R->rscale, size_t(m) + 1))) return bail(rc);
+      if (value_type != kF32)
+        hipLaunchKernelGGL(k_row_scales, dim3(grid_for(int64_t(m) * kWave, 256, 1 << 16)),
+                           dim3(256), 0, st, d_rp, d_val, m, value_type, R->rscale);
+    }
+    if (!go && (rc = ralloc(R, &R->frp, size_t(m) + 1))) return bail(rc);
+    if (!go) {
       void* tmp = nullptr;
       size_t sb = scan_workspace_bytes(m);
       RAPTR_HIP_TRY(hipMalloc(&tmp, size_t(m) * 8 + 512 + sb));
@@ -2130,8 +2163,9 @@ int create_plan(raptr_recon** out, int m, int n, int64_t nnz, const int64_t* d_r
       if (rc) return bail(rc);
       R->pnnz = h;
     }
-    if ((rc = ralloc(R, &R->fcol, size_t(R->pnnz) + 4))) return bail(rc);
-    if (value_type == kF32) {
+    if (!go && (rc = ralloc(R, &R->fcol, size_t(R->pnnz) + 4))) return bail(rc);
+    if (go) {
+    } else if (value_type == kF32) {
       float* p;
       if ((rc = ralloc(R, &p, size_t(R->pnnz) + 4))) return bail(rc);
       R->fval = p;
@@ -2140,8 +2174,8 @@ int create_plan(raptr_recon** out, int m, int n, int64_t nnz, const int64_t* d_r
       if ((rc = ralloc(R, &p, size_t(R->pnnz) + 8))) return bail(rc);
       R->fval = p;
     }
-    if ((rc = ralloc(R, &R->rscale, size_t(m) + 1))) return bail(rc);
-    if (m > 0) {
+    if (!go && (rc = ralloc(R, &R->rscale, size_t(m) + 1))) return bail(rc);
+    if (!go && m > 0) {
       hipLaunchKernelGGL(k_pad_copy, dim3(grid_for(int64_t(m) * kWave, 256, 1 << 16)), dim3(256),
                          0, st, d_rp, d_col, d_val, m, value_type, (const int64_t*)R->frp,
                          R->fcol, R->fval, R->rscale, max_frames == 1 ? kIlvFwd : 0);
@@ -2163,14 +2197,15 @@ int create_plan(raptr_recon** out, int m, int n, int64_t nnz, const int64_t* d_r
 #undef RAPTR_RG
       }
       if (rc) return bail(rc);
-      // default shape: matrix cores for 16-member groups, else pipelined
+      // default shape: matrix cores with per-XCD row ranges for 16-member
+      // groups, else pipelined
       R->grp_unroll = group == 4 ? 8 : (group == 8 ? 4 : 8);
-      R->grp_pipe = group == 16 ? 2 : 1;
+      R->grp_pipe = group == 16 ? 3 : 1;
     }
     // ---- backward: row-blocked transpose via a stable radix sort, every
     // (block, column) segment padded to a multiple of 4 entries
     const int64_t T = int64_t(R->nblk) * (n + 1);
-    if ((rc = ralloc(R, &R->bptr, size_t(T) + 1))) return bail(rc);
+    if (!go && (rc = ralloc(R, &R->bptr, size_t(T) + 1))) return bail(rc);
     if (nnz > 0) {
       int colbits = 1;
       while ((int64_t(1) << colbits) <= n) ++colbits;
@@ -2213,11 +2248,13 @@ int create_plan(raptr_recon** out, int m, int n, int64_t nnz, const int64_t* d_r
           idx = res ? v1 : v0;
           hipLaunchKernelGGL(k_bt_ptr, dim3(grid_for(T, 256, 1 << 16)), dim3(256), 0, st,
                              (const uint64_t*)keys, nnz, R->nblk, n, colbits, uptr);
-          hipLaunchKernelGGL(k_bt_padlen, dim3(grid_for(T, 256, 1 << 16)), dim3(256), 0, st,
-                             (const int64_t*)uptr, T, n, plen);
-          rc = scan_exclusive_i64(plen, R->bptr, T, tot, sws, sb, st);
+          if (!go) {
+            hipLaunchKernelGGL(k_bt_padlen, dim3(grid_for(T, 256, 1 << 16)), dim3(256), 0, st,
+                               (const int64_t*)uptr, T, n, plen);
+            rc = scan_exclusive_i64(plen, R->bptr, T, tot, sws, sb, st);
+          }
         }
-        if (!rc) {
+        if (!rc && !go) {
           hipLaunchKernelGGL(k_set_last, dim3(1), dim3(1), 0, st, R->bptr, int(T),
                              (const int64_t*)tot);
           (void)hipMemcpyAsync(&h, tot, 8, hipMemcpyDeviceToHost, st);
@@ -2225,7 +2262,7 @@ int create_plan(raptr_recon** out, int m, int n, int64_t nnz, const int64_t* d_r
           if (se != hipSuccess) rc = fail(RAPTR_ERR_DEVICE, "transpose: %s", hipGetErrorString(se));
         }
       }
-      if (!rc) {
+      if (!rc && !go) {
         R->bnnz = h;
         if (!(rc = ralloc(R, &R->brow, size_t(h) + 4))) {
           if (value_type == kF32) {
@@ -2237,7 +2274,7 @@ int create_plan(raptr_recon** out, int m, int n, int64_t nnz, const int64_t* d_r
           }
         }
       }
-      if (!rc) {
+      if (!rc && !go) {
         hipLaunchKernelGGL(k_bt_scatter, dim3(grid_for(nnz, 256, 1 << 16)), dim3(256), 0, st,
                            (const uint64_t*)keys, (const uint32_t*)idx, nnz, n, colbits,
                            (const int64_t*)uptr, (const int64_t*)R->bptr,
@@ -2334,7 +2371,7 @@ int raptr_recon_create_ordered(raptr_recon** out, int m, int n, int64_t nnz,
                                const double* d_sens, int value_type, int max_frames,
                                int block_rows, int layout, const int* d_row_order,
                                raptr_stream_t stream) {
-  raptr_recon_options o = {layout, d_row_order, nullptr, 0};
+  raptr_recon_options o = {layout, d_row_order, nullptr, 0, 0};
   return raptr_recon_create_opts(out, m, n, nnz, d_rp, d_col, d_val, d_sens, value_type,
                                  max_frames, block_rows, &o, stream);
 }
@@ -2344,7 +2381,7 @@ int raptr_recon_create_opts(raptr_recon** out, int m, int n, int64_t nnz, const 
                             int value_type, int max_frames, int block_rows,
                             const raptr_recon_options* opts, raptr_stream_t stream) {
   const raptr_recon_options o = opts ? *opts : raptr_recon_options{RAPTR_LAYOUT_BLOCKED, nullptr,
-                                                                   nullptr, 0};
+                                                                   nullptr, 0, 0};
   const int* d_row_order = o.d_row_order;
   int gd[3] = {0, 0, 0};
   if (o.grid) {
@@ -2353,7 +2390,7 @@ int raptr_recon_create_opts(raptr_recon** out, int m, int n, int64_t nnz, const 
   const int* gdims = o.grid ? gd : nullptr;
   if (!d_row_order)
     return create_plan(out, m, n, nnz, d_rp, d_col, d_val, d_sens, value_type, max_frames,
-                       block_rows, o.layout, gdims, o.group, stream);
+                       block_rows, o.layout, gdims, o.group, o.grouped_only, stream);
   if (!out) return fail(RAPTR_ERR_INVALID_ARG, "null output");
   *out = nullptr;
   if (m < 0 || n <= 0 || nnz < 0) return fail(RAPTR_ERR_INVALID_ARG, "bad sizes");
@@ -2417,7 +2454,7 @@ int raptr_recon_create_opts(raptr_recon** out, int m, int n, int64_t nnz, const 
   }
   if (!rc)
     rc = create_plan(out, m, n, nnz, prp, pcol, pval, d_sens, value_type, max_frames,
-                     block_rows, o.layout, gdims, o.group, stream);
+                     block_rows, o.layout, gdims, o.group, o.grouped_only, stream);
   (void)hipStreamSynchronize(st);
   (void)hipFree(tmp);
   if (rc) return rc;
@@ -2599,6 +2636,8 @@ int raptr_recon_get_broadcast(const raptr_recon* R) { return R ? R->bcast64 : -1
 
 int raptr_recon_set_grouped(raptr_recon* R, int on, int unroll, int kind) {
   if (!R) return fail(RAPTR_ERR_INVALID_ARG, "null plan");
+  if (!on && R->grouped_only)
+    return fail(RAPTR_ERR_INVALID_ARG, "a grouped-only plan has no other 64-frame layout");
   if (on && !R->gbp) return fail(RAPTR_ERR_INVALID_ARG, "plan has no grouped layouts");
   if (R->gbp && !grp_shape_ok(R->grp_g, unroll, kind))
     return fail(RAPTR_ERR_INVALID_ARG,

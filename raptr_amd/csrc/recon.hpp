@@ -133,6 +133,7 @@ struct raptr_recon {
   int grp_pipe = 1;    // kernel kind: 0 plain, 1 pipelined scalar loads, 2 matrix cores,
                        // 3 matrix cores with per-XCD row ranges in the backward
   float* c8 = nullptr;  // kind 3: 8 partial images [8][n][64]
+  int grouped_only = 0;  // no padded CSR / padded transpose (64-frame grouped passes only)
   int64_t ngf = 0, ngc = 0, gfnnz = 0, gbnnz = 0;
   int64_t* gfp = nullptr;  // ngf + 1
   int* gfcol = nullptr;

@@ -38,7 +38,7 @@ class Reconstructor:
     width."""
 
     def __init__(self, A, sens, value_type="f32", max_frames=1, block_rows=0, stream=None,
-                 layout="auto", row_order=None, grid=None, group=0):
+                 layout="auto", row_order=None, grid=None, group=0, grouped_only=False):
         """row_order: None (the CSR's row order), a device or host int32
         permutation (plan row p = CSR row row_order[p]), or a
         ``MeasurementSetup`` with the CSR's channel ids given as
@@ -46,7 +46,9 @@ class Reconstructor:
         order (sysmat.locality_order) is computed.  Counts, images and the
         forward's ratios keep the CSR's row order at the interface.
         grid (VoxelGrid) / group (4, 8, 16; 0 = 16): the 64-frame grouped
-        layouts' group size, with backward groups as bricks of the grid."""
+        layouts' group size, with backward groups as bricks of the grid.
+        grouped_only (max_frames 64): keep only the grouped layouts (64-frame
+        passes only, ~40 % less device memory)."""
         self.m, self.n, self.nnz = A.m, A.n, A.nnz
         self.value_type = value_type
         self.max_frames = max_frames
@@ -58,7 +60,8 @@ class Reconstructor:
         self._grid = grid.struct() if grid is not None else None
         opts = capi.raptr_recon_options(LAYOUTS[layout], order.ptr if order is not None else None,
                                         C.pointer(self._grid) if self._grid is not None else None,
-                                        int(group))
+                                        int(group), 1 if grouped_only else 0)
+        self.grouped_only = bool(grouped_only)
         capi.call("raptr_recon_create_opts", C.byref(p), A.m, A.n, A.nnz, A.row_ptr.ptr,
                   A.col.ptr, A.val.ptr, sens.ptr, VALUE_TYPES[value_type], int(max_frames),
                   int(block_rows), C.byref(opts), stream)
@@ -199,6 +202,23 @@ class Reconstructor:
                      "bwd_unroll": bu_}, best[1])
         bu = inf["bwd_unroll"]
         grp = nframes == 64 and inf.get("group", 0) > 0
+        if self.grouped_only:
+            best_g = None
+            for gu, gp in self.GROUP_SHAPES[inf["group"]]:
+                for gb in ((1,) if gp == 3 else (1, 4, 16)):
+                    if gb > 1 and gb >= inf["row_blocks"]:
+                        continue
+                    self.set_grouped(True, gu, gp)
+                    self.set_blocks_per_launch(gb)
+                    ms = timed()
+                    if best_g is None or ms < best_g[1]:
+                        best_g = ((gu, gp, gb), ms)
+            (g_u, g_p, bpl), ms_best = best_g
+            self.set_grouped(True, g_u, g_p)
+            self.set_blocks_per_launch(bpl)
+            capi.call("raptr_memcpy_d2d", x.ptr, x_save.ptr, x.nbytes, stream)
+            return ({"grouped64": 1, "grouped_unroll": g_u, "grouped_kind": g_p,
+                     "bwd_blocks_per_launch": bpl, "grouped_only": 1}, ms_best)
         if grp:  # tune the general kernels first
             self.set_grouped(False)
         if nframes == 1:
@@ -254,7 +274,7 @@ class Reconstructor:
             # row blocks per launch
             best_g = None
             for gu, gp in self.GROUP_SHAPES[inf["group"]]:
-                for gb in (1, 4, 16):
+                for gb in ((1,) if gp == 3 else (1, 4, 16)):
                     if gb > 1 and gb >= inf["row_blocks"]:
                         continue
                     self.set_grouped(True, gu, gp)

@@ -40,7 +40,12 @@ from .recon import Reconstructor
 # device bytes per matrix entry of a plan: forward value + column, transposed
 # value + row (FP32), and the build's CSR while the plan is made from it
 _PLAN_BYTES_PER_NNZ = {"f32": 16, "u16": 12, "f16": 12, "bf16": 12}
+# 64-slot lanes keep only the grouped layouts (16 rows / 2x2x4 voxel bricks
+# per union entry; ~0.16 and ~0.26 union entries per matrix entry, each a
+# 4-byte key and 16 values)
+_GROUPED_BYTES_PER_NNZ = {"f32": 30, "u16": 16, "f16": 16, "bf16": 16}
 _BUILD_BYTES_PER_NNZ = 8
+GROUPED_FRAMES = 64
 
 
 @dataclass(frozen=True)
@@ -122,9 +127,14 @@ def estimate_nnz(geometry, sample_every=64, stream=None):
 
 def estimate_lane_bytes(geometry, frames, nnz):
     g = geometry
-    per = _PLAN_BYTES_PER_NNZ[g.value_type] + _BUILD_BYTES_PER_NNZ
-    return int(per * nnz + 8 * 3 * (g.nchannels + 1) + 4 * frames * (2 * g.nchannels + 4 * g.nvox)
-               + 16 * g.nvox)
+    if frames == GROUPED_FRAMES:
+        per = _GROUPED_BYTES_PER_NNZ[g.value_type] + _BUILD_BYTES_PER_NNZ
+        extra = 4 * 8 * frames * g.nvox  # per-XCD partial images
+    else:
+        per = _PLAN_BYTES_PER_NNZ[g.value_type] + _BUILD_BYTES_PER_NNZ
+        extra = 0
+    return int(per * nnz + 8 * 3 * (g.nchannels + 1) + 4 * frames * (3 * g.nchannels + 4 * g.nvox)
+               + 16 * g.nvox + extra)
 
 
 class Lane:
@@ -142,7 +152,14 @@ class Lane:
                              stream=st, capacity="auto")
         sens = sysmat.sensitivity(g.setup, g.grid, g.nrays, seed=g.seed, raygen=g.raygen,
                                   stream=st)
-        self.plan = Reconstructor(A, sens, g.value_type, frames, stream=st)
+        if frames == GROUPED_FRAMES:
+            # 64 slots: rows in locality order, grouped layouts only (16 rows
+            # / 2x2x4 voxel bricks on the matrix cores)
+            self.plan = Reconstructor(A, sens, g.value_type, frames, stream=st,
+                                      row_order=g.setup, grid=g.grid, group=16,
+                                      grouped_only=True)
+        else:
+            self.plan = Reconstructor(A, sens, g.value_type, frames, stream=st)
         self.nnz = A.nnz
         del A, sens
         self.m, self.n = g.nchannels, g.nvox

@@ -387,3 +387,33 @@ def test_recon_grouped64_rejects_bad_shapes(gpu, small_system):
     assert R1.grouped()["group"] == 0
     with pytest.raises(RaptrError, match="no grouped"):
         R1.set_grouped(True, 8, 1)
+
+
+@pytest.mark.parametrize("vt", ["f32", "f16"])
+def test_recon_grouped_only_plan(gpu, small_system, vt):
+    """A grouped-only plan (no padded CSR / transpose) gives the images of
+    the same grouped kernels in a full plan bit for bit, uses less device
+    memory, and refuses passes that need the general layouts."""
+    from raptr_amd import RaptrError, synthetic
+    from raptr_amd.recon import Reconstructor
+    S = small_system
+    n = S["grid"].nvox
+    ybar = S["ybar"][S["meas"]]
+    Y = np.stack([synthetic.poisson_counts(ybar, tot, seed=70 + f)
+                  for f, tot in enumerate(np.geomspace(1e4, 2e6, 64))], axis=1).astype(np.float32)
+    X0 = np.full((n, 64), 1.0 / n, np.float32)
+    kw = dict(block_rows=1500, row_order=S["setup"], grid=S["grid"], group=16)
+    full = Reconstructor(S["A"], S["sens"], vt, 64, **kw)
+    only = Reconstructor(S["A"], S["sens"], vt, 64, grouped_only=True, **kw)
+    assert only.info()["device_bytes"] < 0.8 * full.info()["device_bytes"]
+    assert only.info()["padded_nnz"] == 0 and only.info()["transpose_nnz"] == 0
+    for gu, kind in ((16, 3), (8, 2), (2, 1)):
+        full.set_grouped(True, gu, kind)
+        only.set_grouped(True, gu, kind)
+        np.testing.assert_array_equal(only.mlem(Y, X0, 3), full.mlem(Y, X0, 3))
+    with pytest.raises(RaptrError, match="grouped-only"):
+        only.mlem(Y[:, :16].copy(), X0[:, :16].copy(), 1)
+    with pytest.raises(RaptrError, match="grouped-only"):
+        only.set_grouped(False)
+    with pytest.raises(RaptrError, match="grouped-only"):
+        Reconstructor(S["A"], S["sens"], vt, 16, grouped_only=True)

@@ -184,10 +184,12 @@ def test_realtime_arrivals_wait_for_their_time(fake):
 
 # ------------------------------------------------------------------- GPU
 @pytest.mark.gpu
-def test_gpu_alone_equals_mixed_and_matches_oracle(gpu, oracle):
+@pytest.mark.parametrize("frames", [4, 64])
+def test_gpu_alone_equals_mixed_and_matches_oracle(gpu, oracle, frames):
     """Requests on the 20,480-channel geometry: each image from a mixed,
     continuously batched run equals the image of the request served alone
-    (bit for bit), and one matches the FP64 oracle (1e-4)."""
+    (bit for bit), and one matches the FP64 oracle (1e-4).  64 slots: lanes
+    with grouped layouts only, on the matrix cores."""
     from conftest import oracle_geom
     setup, grid, nrays = G.config_small()
     geo = serving.Geometry(setup, grid, nrays)
@@ -200,7 +202,7 @@ def test_gpu_alone_equals_mixed_and_matches_oracle(gpu, oracle):
                             np.zeros(len(allc)))
     reqs = serving.synthetic_requests([geo], 100.0, 7, seed=4, events=(1e4, 1e6),
                                       iters=(3, 12), ybar={geo: ybar})
-    cache = serving.MatrixCache(40 << 30, frames=4)
+    cache = serving.MatrixCache(40 << 30, frames=frames)
     srv = serving.Server(cache, max_chunk=4)
     mixed = {r.rid: r for r in srv.serve(reqs, realtime=False).results}
     assert all(r.error is None for r in mixed.values())
