@@ -82,6 +82,8 @@ def parse():
     ap.add_argument("--row-order", default="locality", choices=("locality", "list", "both"),
                     help="plan row order: locality (angle, midpoint, slope), the measurement "
                          "list's, or try both")
+    ap.add_argument("--locality-tile", type=int, default=8,
+                    help="midpoint tile of the locality row order")
     ap.add_argument("--group", type=int, default=16, choices=(4, 8, 16),
                     help="rows / voxels per group of the 64-frame grouped layouts")
     ap.add_argument("--block-slices-mb", default="2,8,32",
@@ -164,7 +166,8 @@ def main():
     plans = {}
     wanted = args.modes.split(",")
     orders = ["locality", "list"] if args.row_order == "both" else [args.row_order]
-    d_order = sysmat.locality_order(setup, d_list, stream=st) if "locality" in orders else None
+    d_order = (sysmat.locality_order(setup, d_list, stream=st, tile=args.locality_tile)
+               if "locality" in orders else None)
     slices_mb = [float(v) for v in args.block_slices_mb.split(",")]
     best_rows = best_order = None
     for name, vt, nf in (("plain", "f32", 1), ("batched", "f32", F), ("mixed_u16", "u16", F),

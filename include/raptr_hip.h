@@ -361,11 +361,12 @@ int raptr_recon_create_ordered(raptr_recon** plan, int m, int n, int64_t nnz,
 /* 1 if the plan keeps its own row order. */
 int raptr_recon_row_ordered(const raptr_recon* plan);
 /* Row order for a two-head setup: the channels d_channels[0..m) sorted
- * (stably) by angle, then the line's midpoint between the heads
- * (id0z + id1z, id0y + id1y), then its slope (id1z - id0z, id1y - id0y);
- * d_order[p] = position in d_channels.  Allocates its own scratch (about 26
- * bytes per channel). */
-int raptr_locality_order(const raptr_setup_t* setup, const int* d_channels, int m,
+ * (stably) by angle, then by the line's midpoint between the heads
+ * (id0z + id1z, id0y + id1y) in tiles of tile x tile midpoints (the tile,
+ * then the midpoint inside it; tile = 1: midpoints in (z, y) order), then
+ * by its slope (id1z - id0z, id1y - id0y); d_order[p] = position in
+ * d_channels.  Allocates its own scratch (about 26 bytes per channel). */
+int raptr_locality_order(const raptr_setup_t* setup, const int* d_channels, int m, int tile,
                          int* d_order, raptr_stream_t stream);
 int raptr_recon_layout(const raptr_recon* plan);
 /* Tiled layout kernel shape: lanes per row segment (4, 8, 16) and unroll
@@ -420,13 +421,13 @@ int raptr_recon_get_broadcast(const raptr_recon* plan);
  * matrix cores (G = 16 only: v_mfma_f32_16x16x4f32 over 4 union entries x
  * 16 frames, keys and values from coalesced vector loads).  Shapes per G:
  * 4: unroll 8 kind 1 or 16 kind 0; 8: 4 kind 1 or 8 kind 0; 16: 2 kind 1,
- * 4 kind 0, 8 or 16 kind 2 or 3 (the default).  Kind 3 = kind 2 with a
+ * 4 kind 0, 8, 16 or 32 kind 2 or 3 (the default).  Kind 3 = kind 2 with a
  * backward in which each XCD takes one eighth of the row blocks and writes
  * its own partial image (8 x n x 64 floats, allocated with the plan); a
  * fixed-order reduce adds them.  unroll applies to the forward (backward
  * steps are 8 entries).  get: group = G (0: no grouped layouts), fwd/
  * bwd_entries = union entries of each layout (lists padded to multiples of
- * 16 / 8). */
+ * 32 / 8). */
 int raptr_recon_set_grouped(raptr_recon* plan, int on, int unroll, int kind);
 int raptr_recon_get_grouped(const raptr_recon* plan, int* group, int* on, int* unroll,
                             int* kind, int64_t* fwd_entries, int64_t* bwd_entries);

@@ -661,7 +661,7 @@ constexpr int kGrpPad = 64;    // union entries of tail padding (>= 2 x the larg
 // union lists are padded (value 0, the list's last key) to multiples of
 // kAlignFwd entries (row groups) / kAlignBwd entries (block x brick lists,
 // which are short)
-constexpr int kAlignFwd = 16;
+constexpr int kAlignFwd = 32;
 constexpr int kAlignBwd = 8;
 
 // forward groups: list s of group g is CSR row G*g + s
@@ -1614,7 +1614,7 @@ bool grp_shape_ok(int G, int U, int kind) {
     case 8: return (U == 4 && kind == 1) || (U == 8 && kind == 0);
     case 16:
       return (U == 2 && kind == 1) || (U == 4 && kind == 0) ||
-             ((U == 8 || U == 16) && (kind == 2 || kind == 3));
+             ((U == 8 || U == 16 || U == 32) && (kind == 2 || kind == 3));
   }
   return false;
 }
@@ -1659,7 +1659,7 @@ int launch_fwd_any(const raptr_recon* R, int F, const float* x, const float* y, 
         RAPTR_LAUNCH_CHECK();                                                                \
         return RAPTR_OK;                                                                     \
       }
-#define RAPTR_M64V(VV) RAPTR_M64(VV, 2) RAPTR_M64(VV, 4)
+#define RAPTR_M64V(VV) RAPTR_M64(VV, 2) RAPTR_M64(VV, 4) RAPTR_M64(VV, 8)
       RAPTR_M64V(kF32) RAPTR_M64V(kF16) RAPTR_M64V(kBF16) RAPTR_M64V(kU16)
 #undef RAPTR_M64V
 #undef RAPTR_M64
@@ -2012,18 +2012,24 @@ __global__ void k_rows_scatter(const float* __restrict__ src, float* __restrict_
 }
 
 // Locality key of a two-head channel: angle, then the line's midpoint
-// between the heads (z0 + z1, y0 + y1), then its slope (z1 - z0, y1 - y0).
-// Lines through one voxel then sit in a few runs of consecutive keys.
-__global__ void k_locality_keys(raptr_setup_t s, const int* __restrict__ cnl, int m,
+// between the heads (z0 + z1, y0 + y1) in tiles of T x T midpoints (tile
+// first, then the midpoint inside it), then its slope (z1 - z0, y1 - y0).
+// Lines through one voxel then sit in a few runs of consecutive keys, and a
+// run of consecutive rows covers a compact bundle of lines (T > 1).
+__global__ void k_locality_keys(raptr_setup_t s, const int* __restrict__ cnl, int m, int T,
                                 uint64_t* __restrict__ keys, uint32_t* __restrict__ idx) {
   const uint64_t zs = uint64_t(s.n0z + s.n1z - 1), ys = uint64_t(s.n0y + s.n1y - 1);
+  const uint64_t nzt = (zs + T - 1) / T, nyt = (ys + T - 1) / T;
   for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < m;
        i += int64_t(gridDim.x) * blockDim.x) {
     int ida, id0z, id0y, id1z, id1y;
     cnl_split(s, cnl[i], ida, id0z, id0y, id1z, id1y);
+    const uint64_t zm = uint64_t(id0z + id1z), ym = uint64_t(id0y + id1y);
     uint64_t k = uint64_t(ida);
-    k = k * zs + uint64_t(id0z + id1z);
-    k = k * ys + uint64_t(id0y + id1y);
+    k = k * nzt + zm / T;
+    k = k * nyt + ym / T;
+    k = k * T + zm % T;
+    k = k * T + ym % T;
     k = k * zs + uint64_t(id1z - id0z + s.n0z - 1);
     k = k * ys + uint64_t(id1y - id0y + s.n0y - 1);
     keys[i] = k;
@@ -2478,16 +2484,17 @@ int raptr_recon_create_opts(raptr_recon** out, int m, int n, int64_t nnz, const 
 
 int raptr_recon_row_ordered(const raptr_recon* R) { return R && R->rperm ? 1 : 0; }
 
-int raptr_locality_order(const raptr_setup_t* setup, const int* d_channels, int m,
+int raptr_locality_order(const raptr_setup_t* setup, const int* d_channels, int m, int tile,
                          int* d_order, raptr_stream_t stream) {
-  if (!setup || m < 0 || (m > 0 && (!d_channels || !d_order)))
+  if (!setup || m < 0 || tile < 1 || tile > 64 || (m > 0 && (!d_channels || !d_order)))
     return fail(RAPTR_ERR_INVALID_ARG, "bad locality order arguments");
   if (m == 0) return RAPTR_OK;
   const raptr_setup_t s = *setup;
   if (s.na < 1 || s.n0z < 1 || s.n0y < 1 || s.n1z < 1 || s.n1y < 1)
     return fail(RAPTR_ERR_INVALID_ARG, "bad setup");
-  const double kmax = double(s.na) * (s.n0z + s.n1z - 1) * (s.n0y + s.n1y - 1) *
-                      (s.n0z + s.n1z - 1) * (s.n0y + s.n1y - 1);
+  const int zs = s.n0z + s.n1z - 1, ys = s.n0y + s.n1y - 1;
+  const double kmax = double(s.na) * ((zs + tile - 1) / tile) * ((ys + tile - 1) / tile) *
+                      tile * tile * double(zs) * ys;
   if (kmax >= 1.8e19) return fail(RAPTR_ERR_INVALID_ARG, "setup too large for 64-bit keys");
   int bits = 1;
   while (bits < 64 && double(uint64_t(1) << bits) < kmax) ++bits;
@@ -2513,7 +2520,7 @@ int raptr_locality_order(const raptr_setup_t* setup, const int* d_channels, int 
   uint32_t* v0 = cv.take<uint32_t>(size_t(m));
   uint32_t* v1 = cv.take<uint32_t>(size_t(m));
   hipLaunchKernelGGL(k_locality_keys, dim3(grid_for(m, 256, 8192)), dim3(256), 0, st, s,
-                     d_channels, m, k0, v0);
+                     d_channels, m, tile, k0, v0);
   int res = 0;
   int rc = radix_sort_pairs(k0, k1, v0, v1, m, bits, st, &res, cv);
   if (!rc) {

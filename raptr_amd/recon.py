@@ -103,7 +103,7 @@ class Reconstructor:
     # FMAs, 1 the same pipelined, 2 matrix cores, 3 matrix cores with per-XCD
     # row ranges in the backward
     GROUP_SHAPES = {4: ((8, 1), (16, 0)), 8: ((4, 1), (8, 0)),
-                    16: ((8, 2), (16, 2), (8, 3), (16, 3), (2, 1), (4, 0))}
+                    16: ((8, 2), (16, 2), (8, 3), (16, 3), (32, 3), (2, 1), (4, 0))}
 
     def grouped(self):
         """dict of the grouped 64-frame layouts: group size (0: none), on,

@@ -162,10 +162,15 @@ def build_csr(setup, grid, cnl_list, nrays, seed=1, raygen=capi.RAYGEN_RAND, str
     return DeviceCSR(m, grid.nvox, nnz.value, rp, col, val, d_list)
 
 
-def locality_order(setup, channels, stream=None):
+# midpoint tile of the default locality order
+LOCALITY_TILE = 8
+
+
+def locality_order(setup, channels, stream=None, tile=LOCALITY_TILE):
     """Plan row order that puts lines through the same voxels next to each
     other (raptr_locality_order): positions into ``channels`` sorted by
-    angle, midpoint and slope.  Returns a device int32 array."""
+    angle, midpoint (in tile x tile tiles) and slope.  Returns a device
+    int32 array."""
     if isinstance(channels, DeviceArray):
         d_ch, m = channels, channels.count
     else:
@@ -173,7 +178,7 @@ def locality_order(setup, channels, stream=None):
         d_ch, m = DeviceArray.from_numpy(lst, stream=stream), len(lst)
     out = DeviceArray(max(m, 1), np.int32)
     s = setup.struct()
-    capi.call("raptr_locality_order", C.byref(s), d_ch.ptr, int(m), out.ptr, stream)
+    capi.call("raptr_locality_order", C.byref(s), d_ch.ptr, int(m), int(tile), out.ptr, stream)
     return out
 
 

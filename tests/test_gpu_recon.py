@@ -254,24 +254,27 @@ def test_recon_rejects_bad_arguments(gpu, small_system):
         R.mlem(np.zeros((S["A"].m, 4), np.float32), np.zeros((S["grid"].nvox, 4), np.float32), 1)
 
 
-def _locality_keys(setup, ch):
+def _locality_keys(setup, ch, T):
     ida, z0, y0, z1, y1 = (np.asarray(v, np.int64) for v in setup.split(ch))
     zs, ys = setup.n0z + setup.n1z - 1, setup.n0y + setup.n1y - 1
+    zm, ym = z0 + z1, y0 + y1
     k = ida
-    for v, r in ((z0 + z1, zs), (y0 + y1, ys), (z1 - z0 + setup.n0z - 1, zs),
-                 (y1 - y0 + setup.n0y - 1, ys)):
+    for v, r in ((zm // T, -(-zs // T)), (ym // T, -(-ys // T)), (zm % T, T), (ym % T, T),
+                 (z1 - z0 + setup.n0z - 1, zs), (y1 - y0 + setup.n0y - 1, ys)):
         k = k * r + v
     return k
 
 
-def test_locality_order_is_a_stable_sort(gpu, small_system):
+@pytest.mark.parametrize("tile", [1, 3, 8])
+def test_locality_order_is_a_stable_sort(gpu, small_system, tile):
     """raptr_locality_order: a permutation of the rows, sorted by (angle,
-    midpoint, slope) keys computed independently here, ties in list order."""
+    midpoint tile, midpoint, slope) keys computed independently here, ties
+    in list order."""
     from raptr_amd import sysmat
     S = small_system
-    order = sysmat.locality_order(S["setup"], S["meas"]).numpy()
+    order = sysmat.locality_order(S["setup"], S["meas"], tile=tile).numpy()
     assert np.array_equal(np.sort(order), np.arange(len(S["meas"])))
-    want = np.argsort(_locality_keys(S["setup"], S["meas"]), kind="stable")
+    want = np.argsort(_locality_keys(S["setup"], S["meas"], tile), kind="stable")
     np.testing.assert_array_equal(order, want)
 
 
