@@ -46,15 +46,24 @@ ERR_ITERS = 30  # iterations of the mixed-precision error figures
 # reads back) for the kernel shapes the autotuner picks on this workload;
 # other shapes report no traffic figure.
 MEASURED_TRAFFIC = {
-    # forward (lanes, unroll, row order); one launch per pass
+    # forward (lanes, unroll, row order); one launch per pass; locality order
+    # with 8 x 8 midpoint tiles
     ("k_fwd1", 32, 2, "list"): 2 * 7342510 * 1024 + 17622 * 1024,
-    ("k_fwd1", 32, 1, "locality"): 2 * 7624320 * 1024 + 17622 * 1024,
-    ("k_fwd1", 16, 2, "locality"): 2 * 8096520 * 1024 + 17634 * 1024,
+    ("k_fwd1", 16, 1, "locality"): 2 * 8723487 * 1024 + 17632 * 1024,
+    ("k_fwd1", 16, 2, "locality"): 2 * 8101933 * 1024 + 17633 * 1024,
+    ("k_fwd1", 16, 4, "locality"): 2 * 7922277 * 1024 + 17629 * 1024,
+    ("k_fwd1", 32, 1, "locality"): 2 * 7624225 * 1024 + 17622 * 1024,
+    ("k_fwd1", 32, 2, "locality"): 2 * 7375797 * 1024 + 17622 * 1024,
+    ("k_fwd1", 32, 4, "locality"): 2 * 8005890 * 1024 + 17622 * 1024,
+    ("k_fwd1", 64, 1, "locality"): 2 * 7136907 * 1024 + 35241 * 1024,
+    ("k_fwd1", 64, 2, "locality"): 2 * 7236914 * 1024 + 35241 * 1024,
+    ("k_fwd1", 64, 4, "locality"): 2 * 7134732 * 1024 + 35241 * 1024,
     # backward (unroll, row blocks, blocks per launch, staged, row order);
     # average over the launches of one pass
     ("k_bwd", 4, 9, 1, 0, "list"): 2 * 734223 * 1024 + 1427 * 1024,
-    ("k_bwd", 4, 9, 4, 0, "locality"): 2 * 2544819 * 1024 + 1603 * 1024,
+    ("k_bwd", 4, 9, 4, 0, "locality"): 2 * 2562847 * 1024 + 1613 * 1024,
 }
+
 
 
 def log(*a):
