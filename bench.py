@@ -406,6 +406,12 @@ def main():
                             0 if args.no_cpu_baseline else
                             int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
 
+    phases = None
+    if not args.no_small:
+        phases = bench_build_phases(log, timed, stream, st,
+                                    0 if args.no_cpu_baseline else
+                                    int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+
     # ------------------------------------------------------------ serving
     serve = None
     nnz_full = A.nnz
@@ -453,6 +459,7 @@ def main():
         "build": {"matrix_ms": round(ms_build, 1), "sensitivity_ms": round(ms_sens, 1),
                   "data_s": round(t_data, 2)},
         "batched_small": small,
+        "build_phases_small": phases,
         "serving": serve,
         "device_copy": copy,
         "cpu_baseline": cpu,
@@ -516,6 +523,98 @@ def bench_serving(log, nreq, rate, slots=64):
         f"{out['requests_per_s']:.1f} req/s over {out['wall_s']:.1f} s")
     for g in list(cache.lanes):
         cache.lanes.pop(g).close()
+    return out
+
+
+def bench_build_phases(log, timed, stream, st, cpu_threads=0):
+    """The matrix build on the small geometry (20,480 channels x 32^3 voxels,
+    10 rays): the workqueue (cylinder cull + emit of (channel, voxel)
+    pairs), MC ray sampling of the workqueue's entries, and the fused
+    cull + sample + compaction into CSR that plans are built from.  GPU:
+    all channels, device buffers, HIP events.  CPU: the C oracle on the
+    first 2,048 channels (one tenth), scaled by 10: the workqueue serially
+    (the reference's getWorkqueue is a serial cursor walk), the sampling
+    and the fused build on cpu_threads threads."""
+    import ctypes as C
+    from raptr_amd import geometry as G
+    from raptr_amd import _capi as capi
+    from raptr_amd import sysmat
+    from raptr_amd.device import DeviceArray, Workspace
+    setup, grid, nrays = G.config_small()
+    m, n = setup.nchannels, grid.nvox
+    allc = np.arange(m, dtype=np.int32)
+    s, g = setup.struct(), grid.struct()
+    L = capi.lib()
+    d_list = DeviceArray.from_numpy(allc)
+    cnt = DeviceArray(m, np.int64)
+    ms_cnt = timed(lambda: capi.call("raptr_workqueue_counts", C.byref(s), C.byref(g),
+                                     d_list.ptr, m, cnt.ptr, st))
+    nwq = int(cnt.numpy().sum())
+    ws = Workspace(L.raptr_workqueue_workspace_size(C.byref(g), m))
+    d_cnl, d_vxl = DeviceArray(nwq, np.int32), DeviceArray(nwq, np.int32)
+    li, vi, found = C.c_int(0), C.c_int(0), C.c_int64()
+
+    def wq():
+        li.value, vi.value = 0, 0
+        capi.call("raptr_workqueue", C.byref(s), C.byref(g), d_list.ptr, m, C.byref(li),
+                  C.byref(vi), -1, d_cnl.ptr, d_vxl.ptr, nwq, C.byref(found), ws.ptr,
+                  ws.nbytes, st)
+    wq()
+    ms_wq = timed(wq)
+    assert found.value == nwq
+    d_val = DeviceArray(nwq, np.float32)
+    sample = lambda: capi.call("raptr_sm_from_wq", C.byref(s), C.byref(g), d_cnl.ptr,
+                               d_vxl.ptr, d_val.ptr, nwq, nrays, 1, capi.RAYGEN_RAND, st)
+    sample()
+    ms_sm = timed(sample)
+    res = {}
+    build = lambda: res.__setitem__("A", sysmat.build_csr(setup, grid, allc, nrays, stream=st,
+                                                          d_list=d_list))
+    build()
+    ms_csr = timed(build)
+    nnz = res["A"].nnz
+    out = {"workload": "20,480-channel heads x 5 angles, 32^3 @ 2.5 mm, 10 rays (all channels)",
+           "candidate_tests": m * n, "workqueue_entries": nwq, "nnz": nnz,
+           "gpu_ms": {"workqueue_counts": round(ms_cnt, 3), "workqueue": round(ms_wq, 3),
+                      "sm_from_wq": round(ms_sm, 3), "csr_build_fused": round(ms_csr, 3)},
+           "gpu_rates": {"cull_tests_per_s": round(m * n / (ms_wq * 1e-3) / 1e9, 2),
+                         "workqueue_GBps_written": round(8 * nwq / (ms_wq * 1e-3) / 1e9, 1),
+                         "ray_evaluations_per_s": round(nwq * nrays / (ms_sm * 1e-3) / 1e9, 2),
+                         "unit": "G/s (tests, evaluations), GB/s (bytes)"}}
+    log(f"build phases (small): workqueue {ms_wq:.2f} ms ({nwq} entries), sampling "
+        f"{ms_sm:.2f} ms, fused CSR build {ms_csr:.2f} ms (nnz {nnz})")
+    del d_cnl, d_vxl, d_val, res
+    if cpu_threads:
+        from oracle import pyoracle as O
+        os_, og = O.make_setup(*[getattr(setup, f) for f in (
+            "pos0x", "pos1x", "na", "n0z", "n0y", "n1z", "n1y", "da", "segx", "segy", "segz")]), \
+            O.make_grid(grid.o, grid.d, grid.n)
+        part = allc[:m // 10]
+        O.lib().ro_set_threads(cpu_threads)
+        t = time.perf_counter()
+        ccnt = O.workqueue_counts(os_, og, part)
+        t_cnt = time.perf_counter() - t
+        t = time.perf_counter()
+        ccnl, cvxl, _, _ = O.workqueue(os_, og, part, capacity=int(ccnt.sum()))
+        t_wq = time.perf_counter() - t
+        t = time.perf_counter()
+        O.sm_from_wq(os_, og, ccnl, cvxl, nrays)
+        t_sm = time.perf_counter() - t
+        t = time.perf_counter()
+        O.build_csr(os_, og, part, nrays)
+        t_csr = time.perf_counter() - t
+        scale = m / len(part)
+        out["cpu_ms"] = {"workqueue_counts": round(1e3 * t_cnt * scale, 1),
+                         "workqueue": round(1e3 * t_wq * scale, 1),
+                         "sm_from_wq": round(1e3 * t_sm * scale, 1),
+                         "csr_build_fused": round(1e3 * t_csr * scale, 1)}
+        out["cpu"] = {"kind": "port", "threads": {"workqueue": 1, "workqueue_counts": cpu_threads,
+                                                  "sm_from_wq": cpu_threads,
+                                                  "csr_build_fused": cpu_threads},
+                      "sample": f"first {len(part)} channels, scaled by {scale:.0f}"}
+        log(f"build phases (small), CPU oracle: workqueue {out['cpu_ms']['workqueue']:.0f} ms "
+            f"(1 thread), sampling {out['cpu_ms']['sm_from_wq']:.0f} ms, fused build "
+            f"{out['cpu_ms']['csr_build_fused']:.0f} ms ({cpu_threads} threads)")
     return out
 
 
