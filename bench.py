@@ -93,7 +93,7 @@ def parse():
                          "list's, or try both")
     ap.add_argument("--locality-tile", type=int, default=8,
                     help="midpoint tile of the locality row order")
-    ap.add_argument("--group", type=int, default=16, choices=(4, 8, 16),
+    ap.add_argument("--group", type=int, default=16, choices=(4, 8, 16, 32),
                     help="rows / voxels per group of the 64-frame grouped layouts")
     ap.add_argument("--block-slices-mb", default="2,8,32",
                     help="ratio-slice sizes (MB) of the batched plans' row blocks to try")

@@ -333,7 +333,8 @@ int raptr_recon_create_ex(raptr_recon** plan, int m, int n, int64_t nnz,
  * gathers of both passes touch fewer cache lines.  grid: the voxel grid of
  * the columns (n = nx*ny*nz), so that the backward groups of the 64-frame
  * grouped layouts are bricks of voxels (null: runs of consecutive column
- * ids).  group: members per group of those layouts (4, 8, 16; 0 = 16).
+ * ids).  group: members per group of those layouts (4, 8, 16, 32;
+ * 0 = 16).
  * grouped_only (max_frames = 64): build only the grouped layouts (and the
  * row scales), not the padded CSR and padded transpose; such a plan runs
  * 64-frame passes only, and needs about 40 % less device memory (matrices
@@ -421,7 +422,9 @@ int raptr_recon_get_broadcast(const raptr_recon* plan);
  * matrix cores (G = 16 only: v_mfma_f32_16x16x4f32 over 4 union entries x
  * 16 frames, keys and values from coalesced vector loads).  Shapes per G:
  * 4: unroll 8 kind 1 or 16 kind 0; 8: 4 kind 1 or 8 kind 0; 16: 2 kind 1,
- * 4 kind 0, 8, 16 or 32 kind 2 or 3 (the default).  Kind 3 = kind 2 with a
+ * 4 kind 0, 8, 16 or 32 kind 2 or 3 (the default); 32: 8 or 16 kind 3
+ * (v_mfma_f32_32x32x2f32, 2 entries x 32 frames, whole 128-byte lines per
+ * gathered frame block; bricks of 2x4x4 voxels).  Kind 3 = kind 2 with a
  * backward in which each XCD takes one eighth of the row blocks and writes
  * its own partial image (8 x n x 64 floats, allocated with the plan); a
  * fixed-order reduce adds them.  unroll applies to the forward (backward

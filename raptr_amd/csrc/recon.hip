@@ -739,7 +739,8 @@ __device__ __forceinline__ void merge_lists(const L& lists, int64_t g, Emit emit
 }
 
 template <int G, class L>
-__global__ void k_grp_count(L lists, int64_t ng, int align, int64_t* __restrict__ cnt) {
+__global__ __launch_bounds__(256) void k_grp_count(L lists, int64_t ng, int align,
+                                                   int64_t* __restrict__ cnt) {
   for (int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; g < ng;
        g += int64_t(gridDim.x) * blockDim.x) {
     int64_t c = 0;
@@ -751,7 +752,8 @@ __global__ void k_grp_count(L lists, int64_t ng, int align, int64_t* __restrict_
 // union keys and G values per union entry; 16-bit values are quantized
 // against the member row's scale as in the transpose (k_bt_scatter)
 template <int G, class L>
-__global__ void k_grp_fill(L lists, int64_t ng, const int64_t* __restrict__ gptr,
+__global__ __launch_bounds__(256) void k_grp_fill(L lists, int64_t ng,
+                                                  const int64_t* __restrict__ gptr,
                            const float* __restrict__ scale, int vt, int* __restrict__ ukey,
                            void* __restrict__ uval) {
   for (int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; g < ng;
@@ -1003,17 +1005,41 @@ __global__ __launch_bounds__(256) void k_bwdg64(int ngc, const int64_t* __restri
   }
 }
 
-// ---- 16-member groups on the matrix cores
+// ---- 16- and 32-member groups on the matrix cores
 // v_mfma_f32_16x16x4f32 computes D[16][16] += A[16][4] B[4][16]: lane l
-// supplies A[l%16][l/16] and B[l/16][l%16] and holds D[4*(l/16)+i][l%16]
-// in register i.  Here A = the members' values of 4 union entries (rows of
-// A = group members), B = 16 frames of those entries' gathered frame
-// vectors, D = the members' sums for 16 frames; 4 instructions per 4
-// entries cover the 64 frames.  The values and keys arrive with coalesced
-// vector loads (lane l loads member l%16 of entry l/16), so nothing streams
-// through the scalar cache, and the next step's loads are issued before the
-// current step's gathers (vector loads complete in order).
+// supplies A[l%16][l/16] and B[l/16][l%16] and holds D[4*(l/16)+r][l%16]
+// in register r.  v_mfma_f32_32x32x2f32 computes D[32][32] += A[32][2]
+// B[2][32]: lane l supplies A[l%32][l/32] and B[l/32][l%32] and holds
+// D[8*(r/4)+4*(l/32)+r%4][l%32] in register r (16 registers).  Here A = the
+// members' values of K union entries (rows of A = group members), B = a
+// block of frames of those entries' gathered frame vectors (16 or 32
+// frames: 64 or 128 contiguous bytes per entry), D = the members' sums for
+// that frame block; 64/W instructions per K entries cover the 64 frames.
+// The values and keys arrive with coalesced vector loads (lane l loads
+// member l%W of entry l/W), so nothing streams through the scalar cache.
 typedef float f4v __attribute__((ext_vector_type(4)));
+typedef float f16v __attribute__((ext_vector_type(16)));
+
+template <int G>
+struct MF;
+template <>
+struct MF<16> {
+  static constexpr int W = 16, K = 4, NR = 4, FB = 4;  // width, entries, registers, frame blocks
+  using acc_t = f4v;
+  __device__ static acc_t op(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  __device__ static int member(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+template <>
+struct MF<32> {
+  static constexpr int W = 32, K = 2, NR = 16, FB = 2;
+  using acc_t = f16v;
+  __device__ static acc_t op(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
+  }
+  __device__ static int member(int lane, int r) { return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3); }
+};
 
 template <int VT>
 __device__ __forceinline__ float load_val1(const typename Val<VT>::T* __restrict__ v,
@@ -1024,78 +1050,87 @@ __device__ __forceinline__ float load_val1(const typename Val<VT>::T* __restrict
     return h2f(v[i], VT);
 }
 
-// keys and values of one step (4*U union entries; lane l takes entry
-// 4u + l/16 and member l%16)
-template <int VT, int U>
+// keys and values of one step (K*U union entries; lane l takes entry
+// K*u + l/W and member l%W)
+template <int U>
 struct MSet {
   int c[U];
   float a[U];
 };
 
-template <int VT, int U>
-__device__ __forceinline__ void mload(MSet<VT, U>& t, const int* __restrict__ key,
+template <int VT, int G, int U>
+__device__ __forceinline__ void mload(MSet<U>& t, const int* __restrict__ key,
                                       const typename Val<VT>::T* __restrict__ val, int64_t k0,
-                                      int kq, int r16) {
+                                      int kq, int rw) {
 #pragma unroll
   for (int u = 0; u < U; ++u) {
-    const int64_t k = k0 + 4 * u + kq;
+    const int64_t k = k0 + MF<G>::K * u + kq;
     t.c[u] = key[k];
-    t.a[u] = load_val1<VT>(val, k * 16 + r16);
+    t.a[u] = load_val1<VT>(val, k * G + rw);
   }
 }
 
 // one step: gathers of set S, then the loads of set T (two steps ahead),
 // then the matrix products of S.  Vector loads complete in order, so the
 // products wait for S's gathers but not for T's loads.
-template <int VT, int U>
-__device__ __forceinline__ void mstep(const MSet<VT, U>& S, MSet<VT, U>& T, bool load_t,
+template <int VT, int G, int U>
+__device__ __forceinline__ void mstep(const MSet<U>& S, MSet<U>& T, bool load_t,
                                       const int* __restrict__ key,
                                       const typename Val<VT>::T* __restrict__ val, int64_t kt,
-                                      int kq, int r16, const float* __restrict__ base,
-                                      f4v (&acc)[4]) {
-  float g[U][4];
+                                      int kq, int rw, const float* __restrict__ base,
+                                      typename MF<G>::acc_t (&acc)[MF<G>::FB]) {
+  using M = MF<G>;
+  float g[U][M::FB];
 #pragma unroll
   for (int u = 0; u < U; ++u)
 #pragma unroll
-    for (int fb = 0; fb < 4; ++fb) g[u][fb] = base[int64_t(S.c[u]) * 64 + 16 * fb];
+    for (int fb = 0; fb < M::FB; ++fb) g[u][fb] = base[int64_t(S.c[u]) * 64 + M::W * fb];
   __builtin_amdgcn_sched_barrier(0);
-  if (load_t) mload<VT, U>(T, key, val, kt, kq, r16);
+  if (load_t) mload<VT, G, U>(T, key, val, kt, kq, rw);
   __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
   for (int u = 0; u < U; ++u)
 #pragma unroll
-    for (int fb = 0; fb < 4; ++fb)
-      acc[fb] = __builtin_amdgcn_mfma_f32_16x16x4f32(S.a[u], g[u][fb], acc[fb], 0, 0, 0);
+    for (int fb = 0; fb < M::FB; ++fb) acc[fb] = M::op(S.a[u], g[u][fb], acc[fb]);
 }
 
 // acc[fb] += the group's sums over the union entries [b, en) (a multiple
-// of 4*U) for frames 16*fb .. 16*fb+15, in steps of 4*U entries with
-// three register sets (loads two steps ahead); reads up to 8*U entries past
-// en (inside the next list or the tail padding)
-template <int VT, int U>
+// of K*U) for frames W*fb .. W*fb+W-1, in steps of K*U entries with three
+// register sets (loads two steps ahead); reads up to 2*K*U entries past en
+// (inside the next list or the tail padding)
+template <int VT, int G, int U>
 __device__ __forceinline__ void mfma_sum(const int* __restrict__ key,
                                          const typename Val<VT>::T* __restrict__ val,
                                          int64_t b, int64_t en, const float* __restrict__ base,
-                                         int kq, int r16, f4v (&acc)[4]) {
-  constexpr int E = 4 * U;
+                                         int kq, int rw,
+                                         typename MF<G>::acc_t (&acc)[MF<G>::FB]) {
+  constexpr int E = MF<G>::K * U;
   const int64_t steps = (en - b) / E;
   if (steps <= 0) return;
-  MSet<VT, U> A, B, C;
-  mload<VT, U>(A, key, val, b, kq, r16);
-  mload<VT, U>(B, key, val, b + E, kq, r16);
+  MSet<U> A, B, C;
+  mload<VT, G, U>(A, key, val, b, kq, rw);
+  mload<VT, G, U>(B, key, val, b + E, kq, rw);
   int64_t kt = b + 2 * E;
   int64_t s = 0;
   for (; s + 3 <= steps; s += 3) {
-    mstep<VT, U>(A, C, true, key, val, kt, kq, r16, base, acc);
-    mstep<VT, U>(B, A, true, key, val, kt + E, kq, r16, base, acc);
-    mstep<VT, U>(C, B, true, key, val, kt + 2 * E, kq, r16, base, acc);
+    mstep<VT, G, U>(A, C, true, key, val, kt, kq, rw, base, acc);
+    mstep<VT, G, U>(B, A, true, key, val, kt + E, kq, rw, base, acc);
+    mstep<VT, G, U>(C, B, true, key, val, kt + 2 * E, kq, rw, base, acc);
     kt += 3 * E;
   }
-  if (s < steps) mstep<VT, U>(A, C, false, key, val, kt, kq, r16, base, acc);
-  if (s + 1 < steps) mstep<VT, U>(B, C, false, key, val, kt, kq, r16, base, acc);
+  if (s < steps) mstep<VT, G, U>(A, C, false, key, val, kt, kq, rw, base, acc);
+  if (s + 1 < steps) mstep<VT, G, U>(B, C, false, key, val, kt, kq, rw, base, acc);
 }
 
-template <int VT, int U>
+template <int G>
+__device__ __forceinline__ void acc_zero(typename MF<G>::acc_t (&acc)[MF<G>::FB]) {
+#pragma unroll
+  for (int fb = 0; fb < MF<G>::FB; ++fb)
+#pragma unroll
+    for (int r = 0; r < MF<G>::NR; ++r) acc[fb][r] = 0.0f;
+}
+
+template <int VT, int G, int U>
 __global__ __launch_bounds__(256) void k_fwdm64(int64_t ng, int64_t m,
                                                 const int64_t* __restrict__ gp,
                                                 const int* __restrict__ gcol,
@@ -1104,24 +1139,24 @@ __global__ __launch_bounds__(256) void k_fwdm64(int64_t ng, int64_t m,
                                                 const float* __restrict__ x,
                                                 const float* __restrict__ y,
                                                 float* __restrict__ e) {
+  using M = MF<G>;
   const int lane = threadIdx.x & (kWave - 1);
-  const int kq = lane >> 4, r16 = lane & 15;
+  const int kq = lane / M::W, rw = lane % M::W;
   const int64_t g = xcd_block(blockIdx.x, gridDim.x) * (256 / kWave) + wave_id();
   if (g >= ng) return;
-  f4v acc[4];
+  typename M::acc_t acc[M::FB];
+  acc_zero<G>(acc);
+  mfma_sum<VT, G, U>(gcol, gval, gp[g], gp[g + 1], x + rw, kq, rw, acc);
 #pragma unroll
-  for (int fb = 0; fb < 4; ++fb) acc[fb] = f4v{0.0f, 0.0f, 0.0f, 0.0f};
-  mfma_sum<VT, U>(gcol, gval, gp[g], gp[g + 1], x + r16, kq, r16, acc);
+  for (int r = 0; r < M::NR; ++r) {
+    const int64_t row = int64_t(G) * g + M::member(lane, r);
+    if (row >= m) continue;
+    const float sc = VT == kF32 ? 1.0f : rscale[row];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int64_t r = 16 * g + 4 * kq + i;
-    if (r >= m) break;
-    const float sc = VT == kF32 ? 1.0f : rscale[r];
-#pragma unroll
-    for (int fb = 0; fb < 4; ++fb) {
-      const int64_t o = r * 64 + 16 * fb + r16;
+    for (int fb = 0; fb < M::FB; ++fb) {
+      const int64_t o = row * 64 + M::W * fb + rw;
       const float yy = y[o];
-      const float yt = sc * acc[fb][i];
+      const float yt = sc * acc[fb][r];
       const float t = (yt != 0.0f && yy != 0.0f) ? yy / yt : 0.0f;
       e[o] = VT == kF32 ? t : sc * t;  // ratio folded with the row scale
     }
@@ -1139,18 +1174,18 @@ __global__ __launch_bounds__(256) void k_bwdm64(int ngc, const int64_t* __restri
                                                 const float* __restrict__ s,
                                                 float* __restrict__ xx,
                                                 double* __restrict__ partials) {
+  using M = MF<16>;
   const int lane = threadIdx.x & (kWave - 1);
   const int kq = lane >> 4, r16 = lane & 15;
   const int w = wave_id();
   const int lb = int(xcd_block(blockIdx.x, gridDim.x));
   const int h = lb * (256 / kWave) + w;
-  f4v acc[4];
-#pragma unroll
-  for (int fb = 0; fb < 4; ++fb) acc[fb] = f4v{0.0f, 0.0f, 0.0f, 0.0f};
+  M::acc_t acc[4];
+  acc_zero<16>(acc);
   if (h < ngc) {
     for (int bb = 0; bb < nbl; ++bb)
-      mfma_sum<VT, 2>(grow, gval, gp[int64_t(bb) * ngc + h], gp[int64_t(bb) * ngc + h + 1],
-                      e + r16, kq, r16, acc);
+      mfma_sum<VT, 16, 2>(grow, gval, gp[int64_t(bb) * ngc + h], gp[int64_t(bb) * ngc + h + 1],
+                          e + r16, kq, r16, acc);
   }
   double part[4] = {0.0, 0.0, 0.0, 0.0};
   if (h < ngc) {
@@ -1191,16 +1226,17 @@ __global__ __launch_bounds__(256) void k_bwdm64(int ngc, const int64_t* __restri
   }
 }
 
-// Backward with per-XCD row ranges (64 frames, bricks of 16): workgroup i
-// runs on XCD i % 8 (the dispatcher's round-robin), which takes row blocks
-// [nblk*x/8, nblk*(x+1)/8).  Each wave walks its brick through all of its
-// XCD's blocks; the bricks resident on one XCD at a time are consecutive
-// (neighbouring voxels) and walk the same small ratio slices together, so
-// their ratio gathers hit that XCD's L2.  Each XCD writes its partial image
-// sums to its own slice of c8; k_bwd_xreduce adds the 8 slices in a fixed
-// order and applies the update.  On a device that assigns workgroups
-// differently the result is the same (only the cache behaviour changes).
-template <int VT>
+// Backward with per-XCD row ranges (64 frames, bricks of G = 16 or 32):
+// workgroup i runs on XCD i % 8 (the dispatcher's round-robin), which takes
+// row blocks [nblk*x/8, nblk*(x+1)/8).  Each wave walks its brick through
+// all of its XCD's blocks; the bricks resident on one XCD at a time are
+// consecutive (neighbouring voxels) and walk the same small ratio slices
+// together, so their ratio gathers hit that XCD's L2.  Each XCD writes its
+// partial image sums to its own slice of c8; k_bwd_xreduce adds the 8
+// slices in a fixed order and applies the update.  On a device that assigns
+// workgroups differently the result is the same (only the cache behaviour
+// changes).
+template <int VT, int G>
 __global__ __launch_bounds__(256) void k_bwdm64x(int ngc, int nblk, int n,
                                                  const int64_t* __restrict__ gp,
                                                  const int* __restrict__ grow,
@@ -1208,25 +1244,26 @@ __global__ __launch_bounds__(256) void k_bwdm64x(int ngc, int nblk, int n,
                                                  const int* __restrict__ cols,
                                                  const float* __restrict__ e,
                                                  float* __restrict__ c8) {
+  using M = MF<G>;
+  constexpr int U = 8 / M::K;  // 8-entry steps (backward lists are aligned to 8)
   const int lane = threadIdx.x & (kWave - 1);
-  const int kq = lane >> 4, r16 = lane & 15;
+  const int kq = lane / M::W, rw = lane % M::W;
   const int xcd = int(blockIdx.x % 8);
   const int h = int(blockIdx.x / 8) * (256 / kWave) + wave_id();
   if (h >= ngc) return;
   const int b0 = int(int64_t(nblk) * xcd / 8), b1 = int(int64_t(nblk) * (xcd + 1) / 8);
-  f4v acc[4];
-#pragma unroll
-  for (int fb = 0; fb < 4; ++fb) acc[fb] = f4v{0.0f, 0.0f, 0.0f, 0.0f};
+  typename M::acc_t acc[M::FB];
+  acc_zero<G>(acc);
   for (int b = b0; b < b1; ++b)
-    mfma_sum<VT, 2>(grow, gval, gp[int64_t(b) * ngc + h], gp[int64_t(b) * ngc + h + 1], e + r16,
-                    kq, r16, acc);
+    mfma_sum<VT, G, U>(grow, gval, gp[int64_t(b) * ngc + h], gp[int64_t(b) * ngc + h + 1],
+                       e + rw, kq, rw, acc);
   float* out = c8 + int64_t(xcd) * n * 64;
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int j = cols[int64_t(h) * 16 + 4 * kq + i];
+  for (int r = 0; r < M::NR; ++r) {
+    const int j = cols[int64_t(h) * G + M::member(lane, r)];
     if (j < 0) continue;
 #pragma unroll
-    for (int fb = 0; fb < 4; ++fb) out[int64_t(j) * 64 + 16 * fb + r16] = acc[fb][i];
+    for (int fb = 0; fb < M::FB; ++fb) out[int64_t(j) * 64 + M::W * fb + rw] = acc[fb][r];
   }
 }
 
@@ -1260,8 +1297,8 @@ __global__ __launch_bounds__(256) void k_bwd_xreduce(int n, const float* __restr
 // brick of the backward groups: G voxels (x, y, z extents)
 __host__ __device__ inline void brick_dims(int G, int& bx, int& by, int& bz) {
   bx = G >= 8 ? 2 : 1;
-  by = 2;
-  bz = G >= 16 ? 4 : (G >= 8 ? 2 : 2);
+  by = G >= 32 ? 4 : 2;
+  bz = G >= 16 ? 4 : 2;
 }
 
 // cols[h*G + s]: member s of brick h (bricks in x, y, z order, z fastest;
@@ -1615,6 +1652,7 @@ bool grp_shape_ok(int G, int U, int kind) {
     case 16:
       return (U == 2 && kind == 1) || (U == 4 && kind == 0) ||
              ((U == 8 || U == 16 || U == 32) && (kind == 2 || kind == 3));
+    case 32: return (U == 8 || U == 16) && kind == 3;
   }
   return false;
 }
@@ -1651,15 +1689,17 @@ int launch_fwd_any(const raptr_recon* R, int F, const float* x, const float* y, 
     const unsigned gg = unsigned((R->ngf + 3) / 4);
     const int GU = R->grp_unroll, GG = R->grp_g;
     if (R->grp_pipe >= 2) {
-#define RAPTR_M64(VV, UU)                                                                    \
-      if (R->vt == VV && GG == 16 && GU == 4 * UU) {                                         \
-        hipLaunchKernelGGL((k_fwdm64<VV, UU>), dim3(gg), dim3(256), 0, st, R->ngf, m, R->gfp,\
-                           R->gfcol, (const typename Val<VV>::T*)R->gfval, R->rscale, x, y,  \
-                           e);                                                               \
+#define RAPTR_M64(VV, G_, UU)                                                                \
+      if (R->vt == VV && GG == G_ && GU == MF<G_>::K * UU) {                                 \
+        hipLaunchKernelGGL((k_fwdm64<VV, G_, UU>), dim3(gg), dim3(256), 0, st, R->ngf, m,     \
+                           R->gfp, R->gfcol, (const typename Val<VV>::T*)R->gfval,          \
+                           R->rscale, x, y, e);                                              \
         RAPTR_LAUNCH_CHECK();                                                                \
         return RAPTR_OK;                                                                     \
       }
-#define RAPTR_M64V(VV) RAPTR_M64(VV, 2) RAPTR_M64(VV, 4) RAPTR_M64(VV, 8)
+#define RAPTR_M64V(VV)                                                                       \
+  RAPTR_M64(VV, 16, 2) RAPTR_M64(VV, 16, 4) RAPTR_M64(VV, 16, 8) RAPTR_M64(VV, 32, 4)        \
+  RAPTR_M64(VV, 32, 8)
       RAPTR_M64V(kF32) RAPTR_M64V(kF16) RAPTR_M64V(kBF16) RAPTR_M64V(kU16)
 #undef RAPTR_M64V
 #undef RAPTR_M64
@@ -1872,16 +1912,17 @@ int launch_bwd_lds(const raptr_recon* R, const float* x, hipStream_t st, int* nb
 // 64-frame backward with per-XCD row ranges and partial images, then the
 // reduce + update (grouped layouts of 16, kind 3)
 int launch_bwd_xcd(const raptr_recon* R, const float* x, hipStream_t st, int* nb_out) {
-  if (!R->c8 || R->grp_g != 16)
+  if (!R->c8 || R->grp_g < 16)
     return fail(RAPTR_ERR_INVALID_ARG, "plan has no per-XCD partial images");
   const unsigned gg = unsigned(8 * ((R->ngc + 3) / 4));
-#define RAPTR_X64(VV)                                                                        \
-  if (R->vt == VV)                                                                           \
-    hipLaunchKernelGGL((k_bwdm64x<VV>), dim3(gg), dim3(256), 0, st, int(R->ngc), R->nblk,     \
-                       R->n, (const int64_t*)R->gbp, (const int*)R->gbrow,                   \
+#define RAPTR_X64(VV, G_)                                                                    \
+  if (R->vt == VV && R->grp_g == G_)                                                         \
+    hipLaunchKernelGGL((k_bwdm64x<VV, G_>), dim3(gg), dim3(256), 0, st, int(R->ngc),          \
+                       R->nblk, R->n, (const int64_t*)R->gbp, (const int*)R->gbrow,          \
                        (const typename Val<VV>::T*)R->gbval, (const int*)R->gcols,           \
                        (const float*)R->e, R->c8);
-  RAPTR_X64(kF32) RAPTR_X64(kF16) RAPTR_X64(kBF16) RAPTR_X64(kU16)
+  RAPTR_X64(kF32, 16) RAPTR_X64(kF16, 16) RAPTR_X64(kBF16, 16) RAPTR_X64(kU16, 16)
+  RAPTR_X64(kF32, 32) RAPTR_X64(kF16, 32) RAPTR_X64(kBF16, 32) RAPTR_X64(kU16, 32)
 #undef RAPTR_X64
   RAPTR_LAUNCH_CHECK();
   const unsigned nr = unsigned((int64_t(R->n) + 3) / 4);
@@ -2096,8 +2137,8 @@ int create_plan(raptr_recon** out, int m, int n, int64_t nnz, const int64_t* d_r
     return fail(RAPTR_ERR_INVALID_ARG,
                 "a grouped-only plan needs max_frames 64, the blocked layout and a non-empty "
                 "matrix");
-  if (group != 4 && group != 8 && group != 16)
-    return fail(RAPTR_ERR_INVALID_ARG, "group must be 4, 8 or 16");
+  if (group != 4 && group != 8 && group != 16 && group != 32)
+    return fail(RAPTR_ERR_INVALID_ARG, "group must be 4, 8, 16 or 32");
   if (gdims && int64_t(gdims[0]) * gdims[1] * gdims[2] != n)
     return fail(RAPTR_ERR_INVALID_ARG, "grid %d x %d x %d does not hold %d voxels", gdims[0],
                 gdims[1], gdims[2], n);
@@ -2199,14 +2240,14 @@ int create_plan(raptr_recon** out, int m, int n, int64_t nnz, const int64_t* d_r
                           st);                                                             \
     break;                                                                                 \
   }
-        RAPTR_RG(4) RAPTR_RG(8) RAPTR_RG(16)
+        RAPTR_RG(4) RAPTR_RG(8) RAPTR_RG(16) RAPTR_RG(32)
 #undef RAPTR_RG
       }
       if (rc) return bail(rc);
       // default shape: matrix cores with per-XCD row ranges for 16-member
       // groups, else pipelined
-      R->grp_unroll = group == 4 ? 8 : (group == 8 ? 4 : 8);
-      R->grp_pipe = group == 16 ? 3 : 1;
+      R->grp_unroll = group == 4 ? 8 : (group == 8 ? 4 : (group == 16 ? 8 : 16));
+      R->grp_pipe = group >= 16 ? 3 : 1;
     }
     // ---- backward: row-blocked transpose via a stable radix sort, every
     // (block, column) segment padded to a multiple of 4 entries
@@ -2328,12 +2369,12 @@ int create_plan(raptr_recon** out, int m, int n, int64_t nnz, const int64_t* d_r
                           &R->gbval, &R->gbnnz, st);                                       \
     break;                                                                                 \
   }
-            RAPTR_CG(4) RAPTR_CG(8) RAPTR_CG(16)
+            RAPTR_CG(4) RAPTR_CG(8) RAPTR_CG(16) RAPTR_CG(32)
 #undef RAPTR_CG
           }
         }
         // per-XCD partial images of the kind-3 backward (groups of 16)
-        if (!rc && G == 16) rc = ralloc(R, &R->c8, size_t(8) * n * 64);
+        if (!rc && G >= 16) rc = ralloc(R, &R->c8, size_t(8) * n * 64);
         if (!rc) R->grp64 = 1;
       }
       (void)hipStreamSynchronize(st);

@@ -45,7 +45,7 @@ class Reconstructor:
         ``A.channels`` / a (setup, channels) pair, from which the locality
         order (sysmat.locality_order) is computed.  Counts, images and the
         forward's ratios keep the CSR's row order at the interface.
-        grid (VoxelGrid) / group (4, 8, 16; 0 = 16): the 64-frame grouped
+        grid (VoxelGrid) / group (4, 8, 16, 32; 0 = 16): the 64-frame grouped
         layouts' group size, with backward groups as bricks of the grid.
         grouped_only (max_frames 64): keep only the grouped layouts (64-frame
         passes only, ~40 % less device memory)."""
@@ -103,7 +103,8 @@ class Reconstructor:
     # FMAs, 1 the same pipelined, 2 matrix cores, 3 matrix cores with per-XCD
     # row ranges in the backward
     GROUP_SHAPES = {4: ((8, 1), (16, 0)), 8: ((4, 1), (8, 0)),
-                    16: ((8, 2), (16, 2), (8, 3), (16, 3), (32, 3), (2, 1), (4, 0))}
+                    16: ((8, 2), (16, 2), (8, 3), (16, 3), (32, 3), (2, 1), (4, 0)),
+                    32: ((16, 3), (8, 3))}
 
     def grouped(self):
         """dict of the grouped 64-frame layouts: group size (0: none), on,

@@ -323,7 +323,8 @@ def test_recon_row_order_must_be_a_permutation(gpu, small_system):
 
 
 @pytest.mark.parametrize("vt,tol", [("f32", 1e-6), ("f16", 1e-5), ("bf16", 1e-5), ("u16", 1e-4)])
-@pytest.mark.parametrize("group,bricks", [(4, False), (8, True), (16, True), (16, False)])
+@pytest.mark.parametrize("group,bricks", [(4, False), (8, True), (16, True), (16, False),
+                                          (32, True)])
 def test_recon_grouped64(gpu, oracle, small_system, vt, tol, group, bricks):
     """64-frame grouped layouts (G rows / G voxels per union list; voxel
     groups as bricks of the grid or runs of column ids) against the
