@@ -1041,13 +1041,15 @@ struct MF<32> {
   __device__ static int member(int lane, int r) { return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3); }
 };
 
+// streamed once per pass: non-temporal, so they do not push the gathered
+// frame vectors out of L2
 template <int VT>
 __device__ __forceinline__ float load_val1(const typename Val<VT>::T* __restrict__ v,
                                            int64_t i) {
   if constexpr (VT == kF32)
-    return v[i];
+    return __builtin_nontemporal_load(v + i);
   else
-    return h2f(v[i], VT);
+    return h2f(__builtin_nontemporal_load(v + i), VT);
 }
 
 // keys and values of one step (K*U union entries; lane l takes entry
@@ -1065,7 +1067,7 @@ __device__ __forceinline__ void mload(MSet<U>& t, const int* __restrict__ key,
 #pragma unroll
   for (int u = 0; u < U; ++u) {
     const int64_t k = k0 + MF<G>::K * u + kq;
-    t.c[u] = key[k];
+    t.c[u] = __builtin_nontemporal_load(key + k);
     t.a[u] = load_val1<VT>(val, k * G + rw);
   }
 }
