@@ -86,6 +86,8 @@ def parse():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serving-requests", type=int, default=120)
     ap.add_argument("--serving-rate", type=float, default=20.0, help="arrivals per second")
+    ap.add_argument("--serving-chunk", type=int, default=2,
+                    help="most iterations a lane runs before admitting new requests")
     ap.add_argument("--serving-slots", type=int, default=64,
                     help="frame slots per geometry lane (64: grouped layouts on the matrix cores)")
     ap.add_argument("--row-order", default="locality", choices=("locality", "list", "both"),
@@ -135,7 +137,8 @@ def main():
         print(json.dumps({"partial": True, "copy": copy, "device": dev}), flush=True)
         return
     if args.modes == "serving":
-        serve = bench_serving(log, args.serving_requests, args.serving_rate, args.serving_slots)
+        serve = bench_serving(log, args.serving_requests, args.serving_rate, args.serving_slots,
+                              args.serving_chunk)
         print(json.dumps({"partial": True, "serving": serve, "device": dev}), flush=True)
         return
 
@@ -191,7 +194,7 @@ def main():
         # sizes (the first batched mode picks, the others reuse its choice)
         # one frame: both plan layouts (column tiles, row blocks) are tried
         if nf == 1:
-            rows_try = [("tiled", 0), ("blocked", 0), ("blocked", 12288)]
+            rows_try = [("tiled", 0), ("blocked", 0), ("blocked", 65536), ("blocked", 12288)]
         elif best_rows is not None:
             rows_try = [("blocked", best_rows)]
         else:
@@ -310,6 +313,14 @@ def main():
         bytes_f = 8 * info["padded_nnz"] + 8 * (m + 1) + 4 * m + 4 * m + 4 * n
     if info["layout"] == "tiled":
         pass  # set above
+    elif info["bwd_staged"] == 3:
+        # one pass: matrix (value + row), two segment bounds per column and
+        # block, the ratio vector (counted once; each XCD reads its eighth),
+        # the 8 per-XCD partial sums written and read, x/s read, x' written
+        bytes_b = (8 * info["transpose_nnz"] + 8 * nb * (n + 1) + 4 * m + 2 * 8 * 4 * n
+                   + 12 * n)
+        bwd_name = (f"k_bwd1x + k_bwd1_xreduce (backward + update, one pass over {nb} row "
+                    "blocks, per-XCD row ranges)")
     elif info["bwd_staged"] == 2:
         # one launch: matrix (value + packed index), two segment bounds per
         # wave and block, the ratio vector (counted once; each workgroup
@@ -343,6 +354,8 @@ def main():
         key_f = ("k_fwd1", cfg_p["fwd_lanes"], cfg_p["fwd_unroll"], cfg_p["row_order"])
         key_b = ("k_bwd", cfg_p["bwd_unroll"], nb, cfg_p["bwd_blocks_per_launch"],
                  cfg_p["bwd_staged"], cfg_p["row_order"])
+        if info["bwd_staged"] == 3:
+            key_b = ("k_bwd1x", cfg_p["bwd_unroll"], nb, cfg_p["row_order"])
     dom = {"kernel": fwd_name, "ms": ms_f, "bytes": bytes_f, "gbs": gbs_f, "launches": nl_f,
            "key": key_f}
     if ms_b > ms_f:
@@ -419,7 +432,8 @@ def main():
         del plans, R, dy, dx, de
         res.pop("A")  # free the full-size matrix first
         A = None
-        serve = bench_serving(log, args.serving_requests, args.serving_rate, args.serving_slots)
+        serve = bench_serving(log, args.serving_requests, args.serving_rate, args.serving_slots,
+                              args.serving_chunk)
 
     out = {
         "metric": "frames/s batched vs plain MLEM at <=1e-4 rel err; p50/p99 latency; HBM GB/s",
@@ -485,7 +499,7 @@ def copy_peak(timed, st, nbytes=2 << 30, reps=5):
     return {"bytes": nbytes, "ms": round(ms, 4), "GBps": round(2 * nbytes / (ms * 1e-3) / 1e9, 1)}
 
 
-def bench_serving(log, nreq, rate, slots=64):
+def bench_serving(log, nreq, rate, slots=64, chunk=2):
     """Poisson arrivals of reconstruction requests over three geometries
     (8x8 heads / 32^3 / 180 angles at 1 deg, 13x13 / 52^3 / 120 at 1.5 deg,
     16x16 / 96^3 / 60 at 3 deg; 2 mm voxels, 20 rays), 1e4-5e6 events and
@@ -512,12 +526,12 @@ def bench_serving(log, nreq, rate, slots=64):
         ybar[g] = sysmat.forward_project(g.setup, g.grid, img.astype(np.float32), g.nrays)
     reqs = serving.synthetic_requests(geoms, rate, nreq, seed=11, ybar=ybar)
     t_prep = time.perf_counter() - t
-    stats = serving.Server(cache, max_chunk=10).serve(reqs, realtime=True)
+    stats = serving.Server(cache, max_chunk=chunk).serve(reqs, realtime=True)
     out = stats.summary()
     out.update({"workload": f"{nreq} requests, Poisson arrivals at {rate}/s, geometries "
                             "8x8/32^3/180 angles, 13x13/52^3/120, 16x16/96^3/60 (2 mm voxels, "
                             f"20 rays), 1e4-5e6 events, 10-50 iterations, {slots} slots per "
-                            "geometry",
+                            f"geometry, chunks of at most {chunk} iterations",
                 "lanes": builds, "prep_s": round(t_prep, 1)})
     log(f"serving: p50 {out['p50_ms']:.0f} ms, p99 {out['p99_ms']:.0f} ms, "
         f"{out['requests_per_s']:.1f} req/s over {out['wall_s']:.1f} s")

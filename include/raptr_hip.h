@@ -394,8 +394,11 @@ int raptr_recon_set_config(raptr_recon* plan, int fwd_lanes, int fwd_unroll,
  * tiles and stream each block's entries of a wave's 64 columns as one range
  * (packed column-in-tile/row-in-block indices), adding runs of equal column
  * with a wave segmented scan; lanes_per_column and workgroups are unused.
- * Column sums stay in LDS across blocks in a fixed order.  staged = 0: the
- * block-by-block pass. */
+ * Column sums stay in LDS across blocks in a fixed order.  staged = 3 (any
+ * block size): each XCD takes one eighth of the row blocks (workgroup i
+ * runs on XCD i % 8), so its ratio slices stay in its own L2; per-XCD
+ * partial column sums (8 x n floats) are added in a fixed order.
+ * staged = 0: the block-by-block pass. */
 int raptr_recon_set_backward(raptr_recon* plan, int staged, int lanes_per_column,
                              int workgroups);
 int raptr_recon_get_backward(const raptr_recon* plan, int* staged, int* lanes_per_column,

@@ -1296,6 +1296,93 @@ __global__ __launch_bounds__(256) void k_bwd_xreduce(int n, const float* __restr
     partials[int64_t(blockIdx.x) * 64 + f] = red[0][f] + red[1][f] + red[2][f] + red[3][f];
 }
 
+// Single-frame backward with per-XCD row ranges: workgroup i runs on XCD
+// i % 8 and takes row blocks [nblk*x/8, nblk*(x+1)/8), so each XCD's ratio
+// slices stay in its own L2.  A wave walks one column through those blocks
+// (4 entries per lane and step, UNROLL steps in flight, as k_bwd) and writes
+// the XCD's partial column sum to c8[x][j]; k_bwd1_xreduce adds the 8 in a
+// fixed order and applies the update.
+template <int VT, int UNROLL>
+__global__ __launch_bounds__(256) void k_bwd1x(int n, int nblk, const int64_t* __restrict__ ptr,
+                                               const int* __restrict__ brow,
+                                               const typename Val<VT>::T* __restrict__ val,
+                                               const float* __restrict__ e,
+                                               float* __restrict__ c8) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int xcd = int(blockIdx.x % 8);
+  const int j = int(blockIdx.x / 8) * (256 / kWave) + wave_id();
+  if (j >= n) return;
+  const int b0 = int(int64_t(nblk) * xcd / 8), b1 = int(int64_t(nblk) * (xcd + 1) / 8);
+  constexpr int64_t STEP = 4 * kWave;
+  float acc = 0.0f;
+  for (int b = b0; b < b1; ++b) {
+    const int64_t bb = ptr[int64_t(b) * (n + 1) + j], en = ptr[int64_t(b) * (n + 1) + j + 1];
+    int64_t k = bb + 4 * lane;
+    for (; k + STEP * (UNROLL - 1) < en; k += STEP * UNROLL) {
+      float wv[UNROLL][4];
+      vi4 rr[UNROLL];
+#pragma unroll
+      for (int u = 0; u < UNROLL; ++u) {
+        load4<VT>(val, k + STEP * u, wv[u][0], wv[u][1], wv[u][2], wv[u][3]);
+        rr[u] = __builtin_nontemporal_load(reinterpret_cast<const vi4*>(brow + k + STEP * u));
+      }
+      float g[UNROLL][4];
+#pragma unroll
+      for (int u = 0; u < UNROLL; ++u) {
+        g[u][0] = e[rr[u].x];
+        g[u][1] = e[rr[u].y];
+        g[u][2] = e[rr[u].z];
+        g[u][3] = e[rr[u].w];
+      }
+#pragma unroll
+      for (int u = 0; u < UNROLL; ++u)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc = fmaf(wv[u][q], g[u][q], acc);
+    }
+    for (; k < en; k += STEP) {
+      float wv[4];
+      load4<VT>(val, k, wv[0], wv[1], wv[2], wv[3]);
+      const vi4 rr = __builtin_nontemporal_load(reinterpret_cast<const vi4*>(brow + k));
+      acc = fmaf(wv[0], e[rr.x], acc);
+      acc = fmaf(wv[1], e[rr.y], acc);
+      acc = fmaf(wv[2], e[rr.z], acc);
+      acc = fmaf(wv[3], e[rr.w], acc);
+    }
+  }
+#pragma unroll
+  for (int off = kWave / 2; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, kWave);
+  if (lane == 0) c8[int64_t(xcd) * n + j] = acc;
+}
+
+// c = sum of the 8 partial column sums (XCD order), x' = x*c/(s + (s==0)),
+// FP64 partial sum per workgroup of 256 voxels
+__global__ __launch_bounds__(256) void k_bwd1_xreduce(int n, const float* __restrict__ c8,
+                                                      const float* __restrict__ x,
+                                                      const float* __restrict__ s,
+                                                      float* __restrict__ xx,
+                                                      double* __restrict__ partials) {
+  const int64_t j = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  double part = 0.0;
+  if (j < n) {
+    float cv = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) cv += c8[int64_t(q) * n + j];
+    float d = s[j];
+    d += float(d == 0.0f);
+    const float nv = x[j] * cv / d;
+    xx[j] = nv;
+    part = double(nv);
+  }
+  __shared__ double red[256];
+  red[threadIdx.x] = part;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
+}
+
 // brick of the backward groups: G voxels (x, y, z extents)
 __host__ __device__ inline void brick_dims(int G, int& bx, int& by, int& bz) {
   bx = G >= 8 ? 2 : 1;
@@ -1874,7 +1961,34 @@ int launch_bwd_flat(const raptr_recon* R, const float* x, hipStream_t st, int* n
   return fail(RAPTR_ERR_INVALID_ARG, "flat staged backward shape unsupported");
 }
 
+// single frame, per-XCD row ranges (backward mode 3)
+int launch_bwd1_xcd(const raptr_recon* R, const float* x, hipStream_t st, int* nb_out) {
+  if (!R->c8 || !R->bptr) return fail(RAPTR_ERR_INVALID_ARG, "plan has no per-XCD partial sums");
+  const int n = R->n;
+  const unsigned gg = unsigned(8 * ((int64_t(n) + 3) / 4));
+  const int U = R->bwd_unroll;
+#define RAPTR_B1X(VV, UU)                                                                   \
+  if (R->vt == VV && U == UU) {                                                             \
+    hipLaunchKernelGGL((k_bwd1x<VV, UU>), dim3(gg), dim3(256), 0, st, n, R->nblk,            \
+                       (const int64_t*)R->bptr, (const int*)R->brow,                        \
+                       (const typename Val<VV>::T*)R->bval, (const float*)R->e, R->c8);     \
+    RAPTR_LAUNCH_CHECK();                                                                   \
+  } else
+  RAPTR_B1X(kF32, 1) RAPTR_B1X(kF32, 2) RAPTR_B1X(kF32, 4) RAPTR_B1X(kF16, 1) RAPTR_B1X(kF16, 2)
+  RAPTR_B1X(kF16, 4) RAPTR_B1X(kBF16, 1) RAPTR_B1X(kBF16, 2) RAPTR_B1X(kBF16, 4)
+  RAPTR_B1X(kU16, 1) RAPTR_B1X(kU16, 2) RAPTR_B1X(kU16, 4)
+  return fail(RAPTR_ERR_INVALID_ARG, "per-XCD backward unroll %d unsupported", U);
+#undef RAPTR_B1X
+  const unsigned nr = unsigned((int64_t(n) + 255) / 256);
+  hipLaunchKernelGGL(k_bwd1_xreduce, dim3(nr), dim3(256), 0, st, n, (const float*)R->c8, x,
+                     (const float*)R->s, R->xx, R->partials);
+  RAPTR_LAUNCH_CHECK();
+  if (nb_out) *nb_out = int(nr);
+  return RAPTR_OK;
+}
+
 int launch_bwd_lds(const raptr_recon* R, const float* x, hipStream_t st, int* nb_out) {
+  if (R->bwd_lds == 3) return launch_bwd1_xcd(R, x, st, nb_out);
   if (R->bwd_lds == 2) return launch_bwd_flat(R, x, st, nb_out);
   const int n = R->n;
   int G = std::min(R->bwd_wgs, std::max(1, (n + 63) / 64));
@@ -2407,6 +2521,10 @@ int create_plan(raptr_recon** out, int m, int n, int64_t nnz, const int64_t* d_r
   R->fwd_lanes = double(nnz) / std::max(m, 1) >= 192 ? 32 : 16;
   R->fwd_unroll = max_frames == 1 ? 1 : 2;
   R->bwd_lds = R->pidx ? 2 : 0;
+  // single-frame blocked plans: per-XCD partial column sums (backward mode 3)
+  if (max_frames == 1 && layout == RAPTR_LAYOUT_BLOCKED && R->bptr &&
+      (rc = ralloc(R, &R->c8, size_t(8) * n)))
+    return bail(rc);
   *out = R;
   return RAPTR_OK;
 }
@@ -2625,10 +2743,15 @@ int raptr_recon_set_config(raptr_recon* R, int fwd_lanes, int fwd_unroll, int bw
 
 int raptr_recon_set_backward(raptr_recon* R, int staged, int lanes_per_column, int workgroups) {
   if (!R) return fail(RAPTR_ERR_INVALID_ARG, "null plan");
-  if (staged < 0 || staged > 2)
-    return fail(RAPTR_ERR_INVALID_ARG, "staged must be 0 (off), 1 (per column) or 2 (flat)");
-  if (staged && (R->layout != RAPTR_LAYOUT_BLOCKED || R->fmax != 1 ||
-                 R->blk_rows > kLdsRowsMax || R->blk_rows % 4 || !R->pidx))
+  if (staged < 0 || staged > 3)
+    return fail(RAPTR_ERR_INVALID_ARG,
+                "staged must be 0 (off), 1 (per column), 2 (flat) or 3 (per-XCD row ranges)");
+  if (staged == 3 && (R->layout != RAPTR_LAYOUT_BLOCKED || R->fmax != 1 || !R->c8))
+    return fail(RAPTR_ERR_INVALID_ARG,
+                "the per-XCD backward needs a single-frame plan of the blocked layout");
+  if ((staged == 1 || staged == 2) &&
+      (R->layout != RAPTR_LAYOUT_BLOCKED || R->fmax != 1 || R->blk_rows > kLdsRowsMax ||
+       R->blk_rows % 4 || !R->pidx))
     return fail(RAPTR_ERR_INVALID_ARG,
                 "staged backward needs a single-frame plan with block_rows <= %d, "
                 "a multiple of 4", kLdsRowsMax);

@@ -135,9 +135,9 @@ class Reconstructor:
                   int(bwd_unroll))
 
     def set_backward(self, staged, lanes_per_column=16, workgroups=512):
-        """Single-frame backward through LDS slices of the ratio vector:
-        staged 1 (per column) or 2 (flat, segmented scan; True means 2), or
-        block by block through L2 (0 / False)."""
+        """Single-frame backward: staged 1 (LDS slices, per column), 2 (LDS
+        slices, flat segmented scan; True means 2), 3 (per-XCD row ranges,
+        any block size), or block by block through L2 (0 / False)."""
         mode = (2 if staged else 0) if isinstance(staged, bool) else int(staged)
         capi.call("raptr_recon_set_backward", self.handle, mode, int(lanes_per_column),
                   int(workgroups))
@@ -236,6 +236,7 @@ class Reconstructor:
         bshapes = [(0, 16, 512, u, bpl) for u in (1, 2, 4) for bpl in (1, 4, 16)
                    if bpl == 1 or bpl < inf["row_blocks"]]
         staged_ok = nframes == 1 and self.max_frames == 1
+        xcd_ok = staged_ok
         if staged_ok:
             try:  # the plan's row blocks may be too large for LDS slices
                 self.set_backward(True)
@@ -245,6 +246,9 @@ class Reconstructor:
             bshapes += [(1, lpc, wgs, u, 1) for lpc in (8, 16, 32) for wgs in (256, 512, 1024)
                         for u in (2, 4)]
             bshapes += [(2, 16, 512, u, 1) for u in (1, 2, 4)]
+        if xcd_ok:  # per-XCD row ranges (any block size)
+            bshapes += [(3, 16, 512, u, 1) for u in (1, 2, 4)]
+        staged_ok = staged_ok or xcd_ok
         best = None
         for st, lpc, wgs, u, bpl in bshapes:
             if staged_ok:

@@ -97,6 +97,34 @@ def test_recon_staged_backward(gpu, oracle, small_system, lpc, wgs, unroll):
         R.set_backward(True, 12, 512)
 
 
+@pytest.mark.parametrize("block_rows", [0, 800, 5000])
+def test_recon_backward_per_xcd_row_ranges(gpu, oracle, small_system, block_rows):
+    """Single-frame backward mode 3 (each XCD takes an eighth of the row
+    blocks, per-XCD partial column sums added in a fixed order), with fewer
+    blocks than XCDs (block_rows 0 and 5000) and many (800): against the FP64
+    oracle, the block-by-block backward (1e-6) and itself (bit for bit)."""
+    from raptr_amd import RaptrError
+    from raptr_amd.recon import Reconstructor
+    S = small_system
+    n = S["grid"].nvox
+    x0 = np.full(n, 1.0 / n, np.float32)
+    x64 = _oracle_mlem(oracle, S, S["y"], 5)
+    R = Reconstructor(S["A"], S["sens"], "f32", 1, block_rows=block_rows, layout="blocked",
+                      row_order=S["setup"])
+    R.set_backward(0)
+    b = R.mlem(S["y"], x0, 5)
+    for unroll in (1, 2, 4):
+        R.set_config(16, 1, unroll)
+        R.set_backward(3)
+        a1 = R.mlem(S["y"], x0, 5)
+        a2 = R.mlem(S["y"], x0, 5, use_graph=False)
+        np.testing.assert_array_equal(a1, a2)
+        assert _rel(a1, x64) <= 1e-4
+        assert _rel(a1, b) <= 1e-6
+    with pytest.raises(RaptrError):
+        Reconstructor(S["A"], S["sens"], "f32", 2).set_backward(3)
+
+
 def test_recon_staged_backward_needs_small_slices(gpu, small_system):
     from raptr_amd import RaptrError
     from raptr_amd.recon import Reconstructor
