@@ -62,6 +62,9 @@ MEASURED_TRAFFIC = {
     # average over the launches of one pass
     ("k_bwd", 4, 9, 1, 0, "list"): 2 * 734223 * 1024 + 1427 * 1024,
     ("k_bwd", 4, 9, 4, 0, "locality"): 2 * 2562847 * 1024 + 1613 * 1024,
+    # single-frame backward with per-XCD row ranges (unroll, row blocks, row
+    # order); one pass (k_bwd1x; its reduce moves 0.04 GB more)
+    ("k_bwd1x", 4, 9, "locality"): 2 * 6605601 * 1024 + 4775 * 1024,
 }
 
 
