@@ -113,12 +113,15 @@ def test_recon_backward_per_xcd_row_ranges(gpu, oracle, small_system, block_rows
                       row_order=S["setup"])
     R.set_backward(0)
     b = R.mlem(S["y"], x0, 5)
+    from raptr_amd.device import Stream
+    st = Stream()
     for unroll in (1, 2, 4):
         R.set_config(16, 1, unroll)
         R.set_backward(3)
-        a1 = R.mlem(S["y"], x0, 5)
-        a2 = R.mlem(S["y"], x0, 5, use_graph=False)
-        np.testing.assert_array_equal(a1, a2)
+        a1 = R.mlem(S["y"], x0, 5, use_graph=False)
+        for _ in range(2):  # graph replays (eager work between them)
+            a2 = R.mlem(S["y"], x0, 5, use_graph=True, stream=st.handle)
+            np.testing.assert_array_equal(a2, a1)
         assert _rel(a1, x64) <= 1e-4
         assert _rel(a1, b) <= 1e-6
     with pytest.raises(RaptrError):
@@ -391,16 +394,21 @@ def test_recon_grouped64(gpu, oracle, small_system, vt, tol, group, bricks):
         if vt == "f32":
             x64 = _oracle_mlem(oracle, S, Y[:, 7].astype(np.float64), 5)
             assert _rel(X[:, 7].astype(np.float64), x64) <= 1e-4
+            # graph replays of every shape equal its eager run bit for bit
             st = Stream()
             dy, dx = DeviceArray.from_numpy(Y.reshape(-1)), DeviceArray.from_numpy(X0.reshape(-1))
-            R.run(dy, dx, 3, 64, use_graph=False, stream=st.handle)
-            eager = dx.numpy(stream=st.handle)
-            for _ in range(3):
+            for gu, gp in Reconstructor.GROUP_SHAPES[group]:
+                R.set_grouped(True, gu, gp)
+                R.set_blocks_per_launch(2)
                 dx.copy_from(X0.reshape(-1), stream=st.handle)
-                R.run(dy, dx, 3, 64, use_graph=True, stream=st.handle)
-                st.sync()
-                np.testing.assert_array_equal(dx.numpy(stream=st.handle), eager)
-                R.mlem(Y[:, :1].copy(), X0[:, :1].copy(), 1, use_graph=False)
+                R.run(dy, dx, 3, 64, use_graph=False, stream=st.handle)
+                eager = dx.numpy(stream=st.handle)
+                for _ in range(2):
+                    dx.copy_from(X0.reshape(-1), stream=st.handle)
+                    R.run(dy, dx, 3, 64, use_graph=True, stream=st.handle)
+                    st.sync()
+                    np.testing.assert_array_equal(dx.numpy(stream=st.handle), eager)
+                    R.mlem(Y[:, :1].copy(), X0[:, :1].copy(), 1, use_graph=False)
         R.close()
 
 
