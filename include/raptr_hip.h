@@ -403,6 +403,15 @@ int raptr_recon_set_backward(raptr_recon* plan, int staged, int lanes_per_column
                              int workgroups);
 int raptr_recon_get_backward(const raptr_recon* plan, int* staged, int* lanes_per_column,
                              int* workgroups);
+/* 16-bit indices of single-frame blocked plans (6 bytes per FP32 entry
+ * instead of 8): with block_rows <= 65536 the transpose's rows as offsets in
+ * their row block (read by the per-XCD backward, staged = 3); with <= 262,144
+ * voxels (and rows of <= 65,535 entries) the forward's columns as their low
+ * 16 bits plus, per row, the sorted positions where the high part steps.
+ * on (the default when present) switches both.  get: bit 0 = rows on, bit
+ * 1 = columns on; -1: none present. */
+int raptr_recon_set_compact_rows(raptr_recon* plan, int on);
+int raptr_recon_get_compact_rows(const raptr_recon* plan);
 /* Row blocks per backward launch of the blocked layout (1..64): a wave
  * walks its column through that many consecutive blocks, keeping the sum in
  * registers (longer waves, fewer launches). */

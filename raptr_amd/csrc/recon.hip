@@ -25,6 +25,7 @@
 namespace raptr {
 namespace {
 
+typedef unsigned short vus4 __attribute__((ext_vector_type(4)));
 
 // ------------------------------------------------- layout construction
 __global__ void k_padded_len(const int64_t* rp, int m, int64_t* plen) {
@@ -246,6 +247,75 @@ __global__ __launch_bounds__(256) void k_fwd1(int64_t m, const int64_t* __restri
     acc = fmaf(v1, x[c.y], acc);
     acc = fmaf(v2, x[c.z], acc);
     acc = fmaf(v3, x[c.w], acc);
+  }
+#pragma unroll
+  for (int off = LANES / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LANES);
+  if (g == 0) {
+    float sc = VT == kF32 ? 1.0f : rscale[i];
+    float yt = sc * acc;
+    float r = yt != 0.0f ? y[i] / yt : 0.0f;
+    e[i] = VT == kF32 ? r : sc * r;  // ratio folded with the row scale
+  }
+}
+
+// Single frame with 16-bit column low parts: as k_fwd1, with the 4 columns
+// of a lane's 4 stored entries rebuilt from their low parts and the row's
+// split points.  Stored entry d of a padded row of length P sits in chunk
+// cb = d - d % C; with L = min(C, P - cb) and q = L/4 its sorted position
+// is cb + (d % 4)*q + (d - cb)/4 (the inverse of the interleave).
+template <int VT, int LANES, int U>
+__global__ __launch_bounds__(256) void k_fwd1c(int64_t m, const int64_t* __restrict__ rp,
+                                               const uint16_t* __restrict__ lo,
+                                               const uint64_t* __restrict__ split,
+                                               const typename Val<VT>::T* __restrict__ val,
+                                               const float* __restrict__ rscale,
+                                               const float* __restrict__ x,
+                                               const float* __restrict__ y,
+                                               float* __restrict__ e) {
+  const int g = threadIdx.x % LANES;
+  const int64_t i = int64_t(blockIdx.x) * (256 / LANES) + threadIdx.x / LANES;
+  if (i >= m) return;
+  const int64_t b = rp[i], en = rp[i + 1];  // multiples of 4
+  const int P = int(en - b);
+  const uint64_t sp = split[i];
+  const int s1 = int(sp & 0xffff), s2 = int((sp >> 16) & 0xffff), s3 = int((sp >> 32) & 0xffff);
+  float acc = 0.0f;
+  auto cols4 = [&](int64_t k, int (&c)[4]) {
+    const vus4 h = __builtin_nontemporal_load(reinterpret_cast<const vus4*>(lo + k));
+    const int d = int(k - b);
+    const int cb = d - d % kIlvFwd;
+    const int L = P - cb < kIlvFwd ? P - cb : kIlvFwd;
+    const int q = L >> 2;
+    const int o0 = cb + ((d - cb) >> 2);
+    const unsigned short hl[4] = {h.x, h.y, h.z, h.w};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int o = o0 + j * q;
+      const int hi = int(o >= s1) + int(o >= s2) + int(o >= s3);
+      c[j] = (hi << 16) | int(hl[j]);
+    }
+  };
+  int64_t k = b + 4 * g;
+  for (; k + int64_t(4) * LANES * (U - 1) < en; k += int64_t(4) * LANES * U) {
+    int c[U][4];
+    float v[U][4];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      cols4(k + 4 * LANES * u, c[u]);
+      load4<VT>(val, k + 4 * LANES * u, v[u][0], v[u][1], v[u][2], v[u][3]);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc = fmaf(v[u][j], x[c[u][j]], acc);
+  }
+  for (; k < en; k += int64_t(4) * LANES) {
+    int c[4];
+    float v[4];
+    cols4(k, c);
+    load4<VT>(val, k, v[0], v[1], v[2], v[3]);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc = fmaf(v[j], x[c[j]], acc);
   }
 #pragma unroll
   for (int off = LANES / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LANES);
@@ -1302,9 +1372,22 @@ __global__ __launch_bounds__(256) void k_bwd_xreduce(int n, const float* __restr
 // (4 entries per lane and step, UNROLL steps in flight, as k_bwd) and writes
 // the XCD's partial column sum to c8[x][j]; k_bwd1_xreduce adds the 8 in a
 // fixed order and applies the update.
-template <int VT, int UNROLL>
-__global__ __launch_bounds__(256) void k_bwd1x(int n, int nblk, const int64_t* __restrict__ ptr,
-                                               const int* __restrict__ brow,
+// the rows of 4 stored entries: 32-bit rows, or 16-bit offsets in the block
+template <bool R16>
+__device__ __forceinline__ vi4 rows4(const void* __restrict__ brow, int64_t k, int r0) {
+  if constexpr (R16) {
+    const vus4 h =
+        __builtin_nontemporal_load(reinterpret_cast<const vus4*>(static_cast<const uint16_t*>(brow) + k));
+    return vi4{r0 + int(h.x), r0 + int(h.y), r0 + int(h.z), r0 + int(h.w)};
+  } else {
+    return __builtin_nontemporal_load(reinterpret_cast<const vi4*>(static_cast<const int*>(brow) + k));
+  }
+}
+
+template <int VT, int UNROLL, bool R16>
+__global__ __launch_bounds__(256) void k_bwd1x(int n, int nblk, int blk_rows,
+                                               const int64_t* __restrict__ ptr,
+                                               const void* __restrict__ brow,
                                                const typename Val<VT>::T* __restrict__ val,
                                                const float* __restrict__ e,
                                                float* __restrict__ c8) {
@@ -1317,6 +1400,7 @@ __global__ __launch_bounds__(256) void k_bwd1x(int n, int nblk, const int64_t* _
   float acc = 0.0f;
   for (int b = b0; b < b1; ++b) {
     const int64_t bb = ptr[int64_t(b) * (n + 1) + j], en = ptr[int64_t(b) * (n + 1) + j + 1];
+    const int r0 = b * blk_rows;
     int64_t k = bb + 4 * lane;
     for (; k + STEP * (UNROLL - 1) < en; k += STEP * UNROLL) {
       float wv[UNROLL][4];
@@ -1324,7 +1408,7 @@ __global__ __launch_bounds__(256) void k_bwd1x(int n, int nblk, const int64_t* _
 #pragma unroll
       for (int u = 0; u < UNROLL; ++u) {
         load4<VT>(val, k + STEP * u, wv[u][0], wv[u][1], wv[u][2], wv[u][3]);
-        rr[u] = __builtin_nontemporal_load(reinterpret_cast<const vi4*>(brow + k + STEP * u));
+        rr[u] = rows4<R16>(brow, k + STEP * u, r0);
       }
       float g[UNROLL][4];
 #pragma unroll
@@ -1342,7 +1426,7 @@ __global__ __launch_bounds__(256) void k_bwd1x(int n, int nblk, const int64_t* _
     for (; k < en; k += STEP) {
       float wv[4];
       load4<VT>(val, k, wv[0], wv[1], wv[2], wv[3]);
-      const vi4 rr = __builtin_nontemporal_load(reinterpret_cast<const vi4*>(brow + k));
+      const vi4 rr = rows4<R16>(brow, k, r0);
       acc = fmaf(wv[0], e[rr.x], acc);
       acc = fmaf(wv[1], e[rr.y], acc);
       acc = fmaf(wv[2], e[rr.z], acc);
@@ -1613,6 +1697,67 @@ __global__ void k_bt_pack(const int64_t* pptr, int64_t total, int n, int blk_row
   }
 }
 
+// rows of the transpose as 16-bit offsets in their row block (blocks of at
+// most 65,536 rows): 6 bytes per FP32 entry instead of 8
+__global__ void k_bt_rows16(int64_t nb, int64_t bnnz, const int64_t* pptr, int n, int blk_rows,
+                            const int* brow, uint16_t* brow16) {
+  // one thread per (block, column) segment
+  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < nb * (n + 1);
+       t += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t b = t / (n + 1);
+    const int64_t r0 = b * blk_rows;
+    for (int64_t p = pptr[t]; p < pptr[t + 1]; ++p) brow16[p] = uint16_t(brow[p] - r0);
+  }
+}
+
+// forward columns as 16-bit low parts (grids of <= 4 x 65,536 voxels): one
+// wave per row; split[i] packs the row's first positions with a column >=
+// 65,536*k (k = 1, 2, 3; 0xffff: none) so col = 65,536*hi + lo with hi =
+// number of splits <= the position.  Padding entries repeat the row's last
+// column, so they decode to it.  *bad = 1 if a padded row is longer than
+// 65,535 entries (positions must stay below the 0xffff "none") or a row's
+// high parts decrease (columns not ascending in 65,536-blocks).
+__global__ __launch_bounds__(256) void k_fwd_cols16(const int64_t* __restrict__ rp,
+                                                    const int* __restrict__ col, int m,
+                                                    const int64_t* __restrict__ prp,
+                                                    const int* __restrict__ pcol,
+                                                    uint16_t* __restrict__ lo,
+                                                    uint64_t* __restrict__ split,
+                                                    int* __restrict__ bad) {
+  const int64_t wid = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t nw = int64_t(gridDim.x) * blockDim.x / kWave;
+  for (int64_t i = wid; i < m; i += nw) {
+    const int64_t b = rp[i], len = rp[i + 1] - b;
+    if (prp[i + 1] - prp[i] > 65535) {
+      if (lane == 0) bad[0] = 1;
+      continue;
+    }
+    int c1 = 0, c2 = 0, c3 = 0;
+    bool down = false;
+    for (int64_t k = lane; k < len; k += kWave) {
+      const int c = col[b + k];
+      c1 += c < 65536;
+      c2 += c < 2 * 65536;
+      c3 += c < 3 * 65536;
+      down |= k > 0 && (col[b + k - 1] >> 16) > (c >> 16);
+    }
+    if (down) bad[0] = 1;
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+      c1 += __shfl_xor(c1, off, kWave);
+      c2 += __shfl_xor(c2, off, kWave);
+      c3 += __shfl_xor(c3, off, kWave);
+    }
+    if (lane == 0) {
+      const uint64_t s1 = c1 < len ? uint64_t(c1) : 0xffffu;
+      const uint64_t s2 = c2 < len ? uint64_t(c2) : 0xffffu;
+      const uint64_t s3 = c3 < len ? uint64_t(c3) : 0xffffu;
+      split[i] = s1 | (s2 << 16) | (s3 << 32);
+    }
+    for (int64_t d = prp[i] + lane; d < prp[i + 1]; d += kWave) lo[d] = uint16_t(pcol[d] & 0xffff);
+  }
+}
+
 // scale_f = 1/sum_b partials[b][f] (FP64, fixed order); 0 for an empty frame
 // one workgroup per frame f = blockIdx.x (each thread sums blocks t, t+256,
 // ... in order, then a fixed tree): the same order for every launch shape
@@ -1754,6 +1899,26 @@ int launch_fwd_any(const raptr_recon* R, int F, const float* x, const float* y, 
     return fail(RAPTR_ERR_INVALID_ARG, "a grouped-only plan runs 64-frame grouped passes only");
   const int64_t m = R->m;
   const int U = R->fwd_unroll;
+  if (F == 1 && R->fcol16 && R->fwd_c16) {
+    const int L = R->fwd_lanes;
+    const unsigned g = unsigned((m + (256 / L) - 1) / (256 / L));
+#define RAPTR_F1C(VV, LL, UU)                                                                \
+  if (R->vt == VV && L == LL && U == UU) {                                                   \
+    hipLaunchKernelGGL((k_fwd1c<VV, LL, UU>), dim3(g), dim3(256), 0, st, m, R->frp,           \
+                       (const uint16_t*)R->fcol16, (const uint64_t*)R->fsplit,               \
+                       (const typename Val<VV>::T*)R->fval, R->rscale, x, y, e);             \
+    RAPTR_LAUNCH_CHECK();                                                                    \
+    return RAPTR_OK;                                                                         \
+  }
+#define RAPTR_F1CV(VV)                                                                       \
+  RAPTR_F1C(VV, 16, 1) RAPTR_F1C(VV, 32, 1) RAPTR_F1C(VV, 64, 1) RAPTR_F1C(VV, 16, 2)        \
+  RAPTR_F1C(VV, 32, 2) RAPTR_F1C(VV, 64, 2) RAPTR_F1C(VV, 16, 4) RAPTR_F1C(VV, 32, 4)        \
+  RAPTR_F1C(VV, 64, 4)
+    RAPTR_F1CV(kF32) RAPTR_F1CV(kF16) RAPTR_F1CV(kBF16) RAPTR_F1CV(kU16)
+#undef RAPTR_F1CV
+#undef RAPTR_F1C
+    return fail(RAPTR_ERR_INVALID_ARG, "forward shape lanes %d unroll %d unsupported", L, U);
+  }
   if (F == 1) {
     const int L = R->fwd_lanes;
     unsigned g = unsigned((m + (256 / L) - 1) / (256 / L));
@@ -1967,11 +2132,17 @@ int launch_bwd1_xcd(const raptr_recon* R, const float* x, hipStream_t st, int* n
   const int n = R->n;
   const unsigned gg = unsigned(8 * ((int64_t(n) + 3) / 4));
   const int U = R->bwd_unroll;
+  const bool r16 = R->brow16 && R->bwd_r16;
 #define RAPTR_B1X(VV, UU)                                                                   \
   if (R->vt == VV && U == UU) {                                                             \
-    hipLaunchKernelGGL((k_bwd1x<VV, UU>), dim3(gg), dim3(256), 0, st, n, R->nblk,            \
-                       (const int64_t*)R->bptr, (const int*)R->brow,                        \
-                       (const typename Val<VV>::T*)R->bval, (const float*)R->e, R->c8);     \
+    if (r16)                                                                                \
+      hipLaunchKernelGGL((k_bwd1x<VV, UU, true>), dim3(gg), dim3(256), 0, st, n, R->nblk,    \
+                         R->blk_rows, (const int64_t*)R->bptr, (const void*)R->brow16,      \
+                         (const typename Val<VV>::T*)R->bval, (const float*)R->e, R->c8);   \
+    else                                                                                    \
+      hipLaunchKernelGGL((k_bwd1x<VV, UU, false>), dim3(gg), dim3(256), 0, st, n, R->nblk,   \
+                         R->blk_rows, (const int64_t*)R->bptr, (const void*)R->brow,        \
+                         (const typename Val<VV>::T*)R->bval, (const float*)R->e, R->c8);   \
     RAPTR_LAUNCH_CHECK();                                                                   \
   } else
   RAPTR_B1X(kF32, 1) RAPTR_B1X(kF32, 2) RAPTR_B1X(kF32, 4) RAPTR_B1X(kF16, 1) RAPTR_B1X(kF16, 2)
@@ -2525,6 +2696,42 @@ int create_plan(raptr_recon** out, int m, int n, int64_t nnz, const int64_t* d_r
   if (max_frames == 1 && layout == RAPTR_LAYOUT_BLOCKED && R->bptr &&
       (rc = ralloc(R, &R->c8, size_t(8) * n)))
     return bail(rc);
+  // ... and the transpose's rows as 16-bit offsets in blocks of <= 65,536 rows
+  if (max_frames == 1 && layout == RAPTR_LAYOUT_BLOCKED && R->bptr && R->brow &&
+      block_rows <= 65536 && R->bnnz > 0) {
+    if ((rc = ralloc(R, &R->brow16, size_t(R->bnnz) + 8))) return bail(rc);
+    hipLaunchKernelGGL(k_bt_rows16, dim3(grid_for(int64_t(R->nblk) * (n + 1), 256, 1 << 16)),
+                       dim3(256), 0, st, int64_t(R->nblk), R->bnnz, (const int64_t*)R->bptr, n,
+                       block_rows, (const int*)R->brow, R->brow16);
+    hipError_t le = hipGetLastError();
+    if (le == hipSuccess) le = hipStreamSynchronize(st);
+    if (le != hipSuccess) return bail(fail(RAPTR_ERR_DEVICE, "rows16: %s", hipGetErrorString(le)));
+    R->bwd_r16 = 1;
+  }
+  // ... and the forward's columns as 16-bit low parts (grids of <= 4 x 65,536
+  // voxels, rows of <= 65,535 entries)
+  if (max_frames == 1 && layout == RAPTR_LAYOUT_BLOCKED && R->fcol && int64_t(n) <= 4 * 65536 &&
+      m > 0 && R->pnnz > 0) {
+    int* bad = nullptr;
+    if ((rc = ralloc(R, &R->fcol16, size_t(R->pnnz) + 8)) ||
+        (rc = ralloc(R, &R->fsplit, size_t(m) + 1)) || (rc = ralloc(R, &bad, 4)))
+      return bail(rc);
+    (void)hipMemsetAsync(bad, 0, sizeof(int), st);
+    hipLaunchKernelGGL(k_fwd_cols16, dim3(grid_for(int64_t(m) * kWave, 256, 1 << 16)), dim3(256),
+                       0, st, d_rp, d_col, m, (const int64_t*)R->frp, (const int*)R->fcol,
+                       R->fcol16, R->fsplit, bad);
+    int hbad = 0;
+    hipError_t le = hipGetLastError();
+    if (le == hipSuccess) le = hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, st);
+    if (le == hipSuccess) le = hipStreamSynchronize(st);
+    if (le != hipSuccess) return bail(fail(RAPTR_ERR_DEVICE, "cols16: %s", hipGetErrorString(le)));
+    if (hbad) {
+      R->fcol16 = nullptr;  // rows too long: the plan keeps 32-bit columns only
+      R->fsplit = nullptr;
+    } else {
+      R->fwd_c16 = 1;
+    }
+  }
   *out = R;
   return RAPTR_OK;
 }
@@ -2769,6 +2976,30 @@ int raptr_recon_set_backward(raptr_recon* R, int staged, int lanes_per_column, i
   R->bwd_lpc = lanes_per_column;
   R->bwd_wgs = workgroups;
   return RAPTR_OK;
+}
+
+int raptr_recon_set_compact_rows(raptr_recon* R, int on) {
+  if (!R) return fail(RAPTR_ERR_INVALID_ARG, "null plan");
+  if (on && !R->brow16 && !R->fcol16)
+    return fail(RAPTR_ERR_INVALID_ARG,
+                "no 16-bit indices (single-frame blocked plans: transpose rows with block_rows "
+                "<= 65536, forward columns with <= 262144 voxels)");
+  if (R->bwd_r16 != (on && R->brow16 ? 1 : 0) || R->fwd_c16 != (on && R->fcol16 ? 1 : 0)) {
+    for (auto& kv : R->graphs) {
+      if (kv.second.second) (void)hipGraphExecDestroy(kv.second.second);
+      if (kv.second.first) (void)hipGraphDestroy(kv.second.first);
+    }
+    R->graphs.clear();
+  }
+  R->bwd_r16 = on && R->brow16 ? 1 : 0;
+  R->fwd_c16 = on && R->fcol16 ? 1 : 0;
+  return RAPTR_OK;
+}
+
+// bit 0: transpose rows, bit 1: forward columns (16-bit and on); -1: none
+int raptr_recon_get_compact_rows(const raptr_recon* R) {
+  if (!R || (!R->brow16 && !R->fcol16)) return -1;
+  return (R->brow16 && R->bwd_r16 ? 1 : 0) | (R->fcol16 && R->fwd_c16 ? 2 : 0);
 }
 
 int raptr_recon_layout(const raptr_recon* R) { return R ? R->layout : -1; }

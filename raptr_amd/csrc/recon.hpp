@@ -134,6 +134,11 @@ struct raptr_recon {
                        // 3 matrix cores with per-XCD row ranges in the backward
   float* c8 = nullptr;  // kind 3: 8 partial images [8][n][64]
   int grouped_only = 0;  // no padded CSR / padded transpose (64-frame grouped passes only)
+  uint16_t* brow16 = nullptr;  // transpose rows as offsets in their block (blocks <= 65,536 rows)
+  int bwd_r16 = 0;             // single-frame per-XCD backward reads brow16
+  uint16_t* fcol16 = nullptr;  // forward columns' low 16 bits (grids of <= 4 x 65,536 voxels)
+  uint64_t* fsplit = nullptr;  // per row: sorted positions where the high part steps
+  int fwd_c16 = 0;             // single-frame forward reads fcol16
   int64_t ngf = 0, ngc = 0, gfnnz = 0, gbnnz = 0;
   int64_t* gfp = nullptr;  // ngf + 1
   int* gfcol = nullptr;

@@ -82,7 +82,8 @@ class Reconstructor:
                "bwd_lanes_per_column": lpc.value, "bwd_workgroups": wgs.value,
                "bwd_blocks_per_launch": capi.lib().raptr_recon_get_blocks_per_launch(self.handle),
                "transpose_nnz": capi.lib().raptr_recon_transpose_nnz(self.handle),
-               "row_ordered": bool(capi.lib().raptr_recon_row_ordered(self.handle))}
+               "row_ordered": bool(capi.lib().raptr_recon_row_ordered(self.handle)),
+               "compact_rows": capi.lib().raptr_recon_get_compact_rows(self.handle)}
         gi = self.grouped()
         if gi["group"]:
             out.update({"grouped64": gi["on"], "group": gi["group"],
@@ -122,6 +123,10 @@ class Reconstructor:
             unroll, kind = [(u, k) for u, k in shapes
                             if (unroll is None or u == unroll) and (kind is None or k == kind)][0]
         capi.call("raptr_recon_set_grouped", self.handle, 1 if on else 0, int(unroll), int(kind))
+
+    def set_compact_rows(self, on):
+        """16-bit transpose rows in the per-XCD single-frame backward."""
+        capi.call("raptr_recon_set_compact_rows", self.handle, 1 if on else 0)
 
     def set_blocks_per_launch(self, blocks):
         capi.call("raptr_recon_set_blocks_per_launch", self.handle, int(blocks))

@@ -124,6 +124,16 @@ def test_recon_backward_per_xcd_row_ranges(gpu, oracle, small_system, block_rows
             np.testing.assert_array_equal(a2, a1)
         assert _rel(a1, x64) <= 1e-4
         assert _rel(a1, b) <= 1e-6
+        # 16-bit transpose rows / forward columns: the same sums
+        if R.info()["compact_rows"] >= 0:
+            R.set_compact_rows(False)
+            a3 = R.mlem(S["y"], x0, 5, use_graph=False)
+            R.set_compact_rows(True)
+            np.testing.assert_array_equal(a3, a1)
+    # 16-bit rows exist for blocks of <= 65,536 rows (the default block is
+    # 2 MB of ratios: 524,288 rows); 16-bit forward columns for <= 262,144
+    # voxels (32^3 here)
+    assert R.info()["compact_rows"] == ((1 if 0 < block_rows <= 65536 else 0) | 2)
     with pytest.raises(RaptrError):
         Reconstructor(S["A"], S["sens"], "f32", 2).set_backward(3)
 
@@ -457,3 +467,96 @@ def test_recon_grouped_only_plan(gpu, small_system, vt):
         only.set_grouped(False)
     with pytest.raises(RaptrError, match="grouped-only"):
         Reconstructor(S["A"], S["sens"], vt, 16, grouped_only=True)
+
+
+def _wide_csr(n, m=2500, seed=5, unsorted_row=False):
+    """Host CSR over n voxels whose rows span several 65,536-voxel blocks:
+    lengths 0..700 plus a few rows of 3,000-9,000 entries (several 128-entry
+    interleave chunks and a ragged last chunk), ascending columns."""
+    rng = np.random.default_rng(seed)
+    lens = rng.integers(0, 700, m)
+    lens[rng.choice(m, 6, replace=False)] = rng.integers(3000, 9000, 6)
+    lens[:3] = 0
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    cols = []
+    for L in lens:
+        c0 = int(rng.integers(0, n))
+        span = int(rng.integers(L + 1, min(n, 40 * L + 2) + 1))
+        c = np.sort(rng.choice(span, L, replace=False)) + c0
+        cols.append(np.sort(c % n))
+    col = np.concatenate(cols).astype(np.int32)
+    if unsorted_row:  # one long row with descending columns over the whole grid
+        i = int(np.argmax(lens))
+        a, b = rp[i], rp[i + 1]
+        col[a:b] = np.sort(rng.choice(n, b - a, replace=False))[::-1]
+    val = rng.random(len(col)).astype(np.float32) + 0.01
+    return rp, col, val
+
+
+def _device_csr(rp, col, val, n):
+    from raptr_amd.device import DeviceArray
+    from raptr_amd.sysmat import DeviceCSR
+    return DeviceCSR(len(rp) - 1, n, len(col), DeviceArray.from_numpy(rp),
+                     DeviceArray.from_numpy(col), DeviceArray.from_numpy(val))
+
+
+@pytest.mark.parametrize("vt", ["f32", "f16", "bf16", "u16"])
+def test_recon_forward_compact_columns(gpu, vt):
+    """16-bit forward columns (low parts + per-row split points) on a
+    262,139-voxel matrix whose rows cross the 65,536-voxel blocks: the same
+    ratios, bit for bit, as the 32-bit columns for every lanes/unroll shape;
+    FP32 against an FP64 host product."""
+    from raptr_amd import _capi as capi
+    from raptr_amd.device import DeviceArray
+    from raptr_amd.recon import Reconstructor
+    n = 4 * 65536 - 5
+    rp, col, val = _wide_csr(n)
+    m = len(rp) - 1
+    assert (col >> 16).max() == 3
+    A = _device_csr(rp, col, val, n)
+    sens = DeviceArray.from_numpy(np.ones(n))
+    rng = np.random.default_rng(7)
+    x = rng.random(n).astype(np.float32)
+    y = rng.random(m).astype(np.float32) + 0.5
+    R = Reconstructor(A, sens, vt, 1, layout="blocked")
+    assert R.info()["compact_rows"] & 2
+    dx, dy, de = DeviceArray.from_numpy(x), DeviceArray.from_numpy(y), DeviceArray(m, np.float32)
+    for lanes, unroll in [(16, 1), (32, 2), (64, 4), (16, 4), (64, 1)]:
+        R.set_config(lanes, unroll, 1)
+        out = []
+        for on in (True, False):
+            R.set_compact_rows(on)
+            assert bool(R.info()["compact_rows"] & 2) == on
+            capi.call("raptr_recon_forward", R.handle, 1, dx.ptr, dy.ptr, de.ptr, None)
+            out.append(de.numpy())
+        np.testing.assert_array_equal(out[0], out[1])
+        if vt == "f32":
+            yt = np.array([np.dot(val[rp[i]:rp[i + 1]].astype(np.float64),
+                                  x[col[rp[i]:rp[i + 1]]]) for i in range(m)])
+            ref = np.where(yt != 0, y / np.where(yt != 0, yt, 1), 0)
+            np.testing.assert_allclose(out[0], ref, rtol=1e-5)
+
+
+@pytest.mark.parametrize("case", ["descending", "too_many_voxels"])
+def test_recon_forward_compact_columns_absent(gpu, case):
+    """No 16-bit columns when a row's columns step down across 65,536-voxel
+    blocks or the grid has more than 262,144 voxels: the plan keeps its
+    32-bit columns and the forward still matches the host product."""
+    from raptr_amd import _capi as capi
+    from raptr_amd.device import DeviceArray
+    from raptr_amd.recon import Reconstructor
+    n = 4 * 65536 - 5 if case == "descending" else 4 * 65536 + 1
+    rp, col, val = _wide_csr(n, m=800, unsorted_row=case == "descending")
+    m = len(rp) - 1
+    A = _device_csr(rp, col, val, n)
+    R = Reconstructor(A, DeviceArray.from_numpy(np.ones(n)), "f32", 1, layout="blocked",
+                      block_rows=512)
+    assert R.info()["compact_rows"] & 2 == 0
+    x = np.random.default_rng(2).random(n).astype(np.float32)
+    y = np.ones(m, np.float32)
+    dx, dy, de = DeviceArray.from_numpy(x), DeviceArray.from_numpy(y), DeviceArray(m, np.float32)
+    capi.call("raptr_recon_forward", R.handle, 1, dx.ptr, dy.ptr, de.ptr, None)
+    yt = np.array([np.dot(val[rp[i]:rp[i + 1]].astype(np.float64), x[col[rp[i]:rp[i + 1]]])
+                   for i in range(m)])
+    ref = np.where(yt != 0, y / np.where(yt != 0, yt, 1), 0)
+    np.testing.assert_allclose(de.numpy(), ref, rtol=1e-5)
