@@ -58,13 +58,40 @@ MEASURED_TRAFFIC = {
     ("k_fwd1", 64, 1, "locality"): 2 * 7136907 * 1024 + 35241 * 1024,
     ("k_fwd1", 64, 2, "locality"): 2 * 7236914 * 1024 + 35241 * 1024,
     ("k_fwd1", 64, 4, "locality"): 2 * 7134732 * 1024 + 35241 * 1024,
+    # 16-bit columns (plain k_fwd1c, software-pipelined k_fwd1cp), 69 row
+    # blocks of 65,536 rows
+    ("k_fwd1c", 16, 1, "locality"): 2 * 7147423 * 1024 + 17626 * 1024,
+    ("k_fwd1c", 16, 2, "locality"): 2 * 6470240 * 1024 + 17623 * 1024,
+    ("k_fwd1c", 16, 4, "locality"): 2 * 6375603 * 1024 + 17624 * 1024,
+    ("k_fwd1c", 32, 1, "locality"): 2 * 6081126 * 1024 + 17621 * 1024,
+    ("k_fwd1c", 32, 2, "locality"): 2 * 5853171 * 1024 + 17621 * 1024,
+    ("k_fwd1c", 32, 4, "locality"): 2 * 6434950 * 1024 + 17621 * 1024,
+    ("k_fwd1c", 64, 1, "locality"): 2 * 5598289 * 1024 + 35241 * 1024,
+    ("k_fwd1c", 64, 2, "locality"): 2 * 5762812 * 1024 + 35241 * 1024,
+    ("k_fwd1c", 64, 4, "locality"): 2 * 5601200 * 1024 + 35241 * 1024,
+    ("k_fwd1cp", 16, 1, "locality"): 2 * 6969534 * 1024 + 17623 * 1024,
+    ("k_fwd1cp", 16, 2, "locality"): 2 * 6170956 * 1024 + 17622 * 1024,
+    ("k_fwd1cp", 16, 4, "locality"): 2 * 5740675 * 1024 + 17621 * 1024,
+    ("k_fwd1cp", 32, 1, "locality"): 2 * 5969195 * 1024 + 17621 * 1024,
+    ("k_fwd1cp", 32, 2, "locality"): 2 * 5565429 * 1024 + 17621 * 1024,
+    ("k_fwd1cp", 32, 4, "locality"): 2 * 5035376 * 1024 + 17620 * 1024,
+    ("k_fwd1cp", 64, 1, "locality"): 2 * 5518387 * 1024 + 35241 * 1024,
+    ("k_fwd1cp", 64, 2, "locality"): 2 * 5156511 * 1024 + 35241 * 1024,
+    ("k_fwd1cp", 64, 4, "locality"): 2 * 5163556 * 1024 + 35241 * 1024,
     # backward (unroll, row blocks, blocks per launch, staged, row order);
     # average over the launches of one pass
     ("k_bwd", 4, 9, 1, 0, "list"): 2 * 734223 * 1024 + 1427 * 1024,
     ("k_bwd", 4, 9, 4, 0, "locality"): 2 * 2562847 * 1024 + 1613 * 1024,
     # single-frame backward with per-XCD row ranges (unroll, row blocks, row
-    # order); one pass (k_bwd1x; its reduce moves 0.04 GB more)
-    ("k_bwd1x", 4, 9, "locality"): 2 * 6605601 * 1024 + 4775 * 1024,
+    # order, 32/16-bit rows); one pass (k_bwd1x; its reduce moves 0.04 GB more)
+    ("k_bwd1x", 4, 9, "locality", "r32"): 2 * 6605601 * 1024 + 4775 * 1024,
+    # 69 row blocks of 65,536 rows, 16-bit and 32-bit rows
+    ("k_bwd1x", 1, 69, "locality", "r16"): 2 * 5733034 * 1024 + 6807 * 1024,
+    ("k_bwd1x", 1, 69, "locality", "r32"): 2 * 6954772 * 1024 + 12515 * 1024,
+    ("k_bwd1x", 2, 69, "locality", "r16"): 2 * 5391567 * 1024 + 9850 * 1024,
+    ("k_bwd1x", 2, 69, "locality", "r32"): 2 * 6709416 * 1024 + 12085 * 1024,
+    ("k_bwd1x", 4, 69, "locality", "r16"): 2 * 5622596 * 1024 + 6761 * 1024,
+    ("k_bwd1x", 4, 69, "locality", "r32"): 2 * 6605579 * 1024 + 9487 * 1024,
 }
 
 
@@ -298,7 +325,16 @@ def main():
     ms_b = timed(lambda: [bwd() for _ in range(reps)]) / reps
     info = R.info()
     nb = info["row_blocks"]
-    fwd_name = "k_fwd1 (forward + ratio, one launch)"
+    cmp_ = max(info["compact_rows"], 0)
+    cols16, rows16 = bool(cmp_ & 2), bool(cmp_ & 1)
+    fwd_kernel = ("k_fwd1cp" if info["fwd_kind"] == 1 and cols16 else
+                  "k_fwd1c" if cols16 else "k_fwd1")
+    fwd_name = f"{fwd_kernel} (forward + ratio, one launch" + (
+        ", 16-bit columns)" if cols16 else ")")
+    # bytes as stored: 16-bit indices move 2 bytes less per entry than the
+    # 4-byte index of the algorithmic count (plus 8 bytes of split points per
+    # row for the columns)
+    stored_f = stored_b = None
     if info["layout"] == "tiled":
         # bytes each pass must move: padded matrix (4-byte value + 2-byte
         # column offset), segment starts, and the vectors (y, the per-tile
@@ -314,6 +350,8 @@ def main():
     else:
         # algorithmic bytes: matrix (value + column index), row pointers, vectors
         bytes_f = 8 * info["padded_nnz"] + 8 * (m + 1) + 4 * m + 4 * m + 4 * n
+        if cols16:
+            stored_f = bytes_f - 2 * info["padded_nnz"] + 8 * m
     if info["layout"] == "tiled":
         pass  # set above
     elif info["bwd_staged"] == 3:
@@ -322,8 +360,10 @@ def main():
         # the 8 per-XCD partial sums written and read, x/s read, x' written
         bytes_b = (8 * info["transpose_nnz"] + 8 * nb * (n + 1) + 4 * m + 2 * 8 * 4 * n
                    + 12 * n)
+        if rows16:
+            stored_b = bytes_b - 2 * info["transpose_nnz"]
         bwd_name = (f"k_bwd1x + k_bwd1_xreduce (backward + update, one pass over {nb} row "
-                    "blocks, per-XCD row ranges)")
+                    "blocks, per-XCD row ranges" + (", 16-bit rows)" if rows16 else ")"))
     elif info["bwd_staged"] == 2:
         # one launch: matrix (value + packed index), two segment bounds per
         # wave and block, the ratio vector (counted once; each workgroup
@@ -354,16 +394,17 @@ def main():
     else:
         nl_f = 1
         nl_b = 1 if info["bwd_staged"] else -(-nb // cfg_p["bwd_blocks_per_launch"])
-        key_f = ("k_fwd1", cfg_p["fwd_lanes"], cfg_p["fwd_unroll"], cfg_p["row_order"])
+        key_f = (fwd_kernel, cfg_p["fwd_lanes"], cfg_p["fwd_unroll"], cfg_p["row_order"])
         key_b = ("k_bwd", cfg_p["bwd_unroll"], nb, cfg_p["bwd_blocks_per_launch"],
                  cfg_p["bwd_staged"], cfg_p["row_order"])
         if info["bwd_staged"] == 3:
-            key_b = ("k_bwd1x", cfg_p["bwd_unroll"], nb, cfg_p["row_order"])
+            key_b = ("k_bwd1x", cfg_p["bwd_unroll"], nb, cfg_p["row_order"],
+                     "r16" if rows16 else "r32")
     dom = {"kernel": fwd_name, "ms": ms_f, "bytes": bytes_f, "gbs": gbs_f, "launches": nl_f,
-           "key": key_f}
+           "key": key_f, "stored": stored_f}
     if ms_b > ms_f:
         dom = {"kernel": bwd_name, "ms": ms_b, "bytes": bytes_b, "gbs": gbs_b, "launches": nl_b,
-               "key": key_b}
+               "key": key_b, "stored": stored_b}
     traffic = MEASURED_TRAFFIC.get(dom["key"]) if dom["key"] else None
 
     # ------------------------ FP64 host recurrence: errors + CPU baseline
@@ -466,6 +507,11 @@ def main():
                      "traffic_source": ("rocprofv3 FETCH_SIZE x2.00 + WRITE_SIZE x1.00 per launch, "
                                         "factors from a 2 GiB streaming copy") if traffic else None,
                      "algorithmic_bytes": int(dom["bytes"] / dom["launches"]),
+                     "bytes_basis": "4-byte value + 4-byte index per stored entry",
+                     "stored_bytes": (int(dom["stored"] / dom["launches"])
+                                      if dom["stored"] else None),
+                     "stored_GBps": (round(dom["stored"] / (dom["ms"] * 1e-3) / 1e9, 1)
+                                     if dom["stored"] else None),
                      "avg_launch_ms": round(dom["ms"] / dom["launches"], 4),
                      "launches_per_pass": dom["launches"], "pass_ms": round(dom["ms"], 4),
                      "copy_GBps_measured": copy["GBps"]},

@@ -412,6 +412,16 @@ int raptr_recon_get_backward(const raptr_recon* plan, int* staged, int* lanes_pe
  * 1 = columns on; -1: none present. */
 int raptr_recon_set_compact_rows(raptr_recon* plan, int on);
 int raptr_recon_get_compact_rows(const raptr_recon* plan);
+/* Kernel of the single-frame forward pass:
+ *   0  one matrix entry per lane and gather (32-bit or, when on, 16-bit
+ *      columns);
+ *   1  as 0 over 16-bit columns, software-pipelined (each lane loads its
+ *      next step while the gathers of the current one are in flight);
+ *      bit-identical to 0.
+ * available: bit k set if kind k can run on this plan.  Null outputs are
+ * skipped. */
+int raptr_recon_set_forward_kind(raptr_recon* plan, int kind);
+int raptr_recon_get_forward_kind(const raptr_recon* plan, int* kind, int* available);
 /* Row blocks per backward launch of the blocked layout (1..64): a wave
  * walks its column through that many consecutive blocks, keeping the sum in
  * registers (longer waves, fewer launches). */

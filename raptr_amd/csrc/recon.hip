@@ -19,6 +19,7 @@
 // stream is shared by the batch.  The normalisation sums are FP64.
 #include <algorithm>
 #include <climits>
+#include <type_traits>
 
 #include "recon.hpp"
 
@@ -316,6 +317,120 @@ __global__ __launch_bounds__(256) void k_fwd1c(int64_t m, const int64_t* __restr
     load4<VT>(val, k, v[0], v[1], v[2], v[3]);
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc = fmaf(v[j], x[c[j]], acc);
+  }
+#pragma unroll
+  for (int off = LANES / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LANES);
+  if (g == 0) {
+    float sc = VT == kF32 ? 1.0f : rscale[i];
+    float yt = sc * acc;
+    float r = yt != 0.0f ? y[i] / yt : 0.0f;
+    e[i] = VT == kF32 ? r : sc * r;  // ratio folded with the row scale
+  }
+}
+
+// k_fwd1c software-pipelined: a lane issues the gathers of its current
+// step, then the stream loads (16-bit columns, values) of its next step,
+// then waits for the gathers only (loads complete in issue order), so the
+// HBM latency of step s+1 overlaps the gathers of step s.  The loads are
+// unconditional (a divergent branch around them would make the wait count
+// cover them too): a chunk past the row end loads entries 0..3 of the
+// arrays instead, one cache line that every such lane shares, gathers x[0]
+// and is not accumulated.  Chunks are accumulated in the order k_fwd1c uses: the sums
+// are bit-identical.
+template <int VT>
+struct RawVals {
+  using T = typename std::conditional<VT == kF32, vf4, vu2>::type;
+  __device__ static T load(const typename Val<VT>::T* __restrict__ val, int64_t k) {
+    return __builtin_nontemporal_load(
+        reinterpret_cast<const T*>(reinterpret_cast<const char*>(val) + k * sizeof(typename Val<VT>::T)));
+  }
+  __device__ static void get(const T& r, float (&v)[4]) {
+    if constexpr (VT == kF32) {
+      v[0] = r.x, v[1] = r.y, v[2] = r.z, v[3] = r.w;
+    } else {
+      v[0] = h2f(uint16_t(r.x & 0xffff), VT), v[1] = h2f(uint16_t(r.x >> 16), VT);
+      v[2] = h2f(uint16_t(r.y & 0xffff), VT), v[3] = h2f(uint16_t(r.y >> 16), VT);
+    }
+  }
+};
+
+template <int VT, int LANES, int U>
+__global__ __launch_bounds__(256) void k_fwd1cp(int64_t m, const int64_t* __restrict__ rp,
+                                                const uint16_t* __restrict__ lo,
+                                                const uint64_t* __restrict__ split,
+                                                const typename Val<VT>::T* __restrict__ val,
+                                                const float* __restrict__ rscale,
+                                                const float* __restrict__ x,
+                                                const float* __restrict__ y,
+                                                float* __restrict__ e) {
+  using RV = RawVals<VT>;
+  const int g = threadIdx.x % LANES;
+  const int64_t i = int64_t(blockIdx.x) * (256 / LANES) + threadIdx.x / LANES;
+  if (i >= m) return;
+  const int64_t b = rp[i], en = rp[i + 1];  // multiples of 4
+  const int P = int(en - b);
+  const uint64_t sp = split[i];
+  const int s1 = int(sp & 0xffff), s2 = int((sp >> 16) & 0xffff), s3 = int((sp >> 32) & 0xffff);
+  constexpr int64_t STEP = int64_t(4) * LANES * U;
+  float acc = 0.0f;
+  int64_t k = b + 4 * g;
+  if (k < en) {
+    vus4 h[U];
+    typename RV::T w[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t q = k + 4 * LANES * u < en ? k + 4 * LANES * u : 0;
+      h[u] = __builtin_nontemporal_load(reinterpret_cast<const vus4*>(lo + q));
+      w[u] = RV::load(val, q);
+    }
+    for (;;) {
+      float xg[U][4];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int d = int(k - b) + 4 * LANES * u;
+        const bool ok = d < P;  // a chunk past the row end gathers x[0]
+        const int dd = ok ? d : int(k - b);
+        const int cb = dd - dd % kIlvFwd;
+        const int L = P - cb < kIlvFwd ? P - cb : kIlvFwd;
+        const int q = L >> 2;
+        const int o0 = cb + ((dd - cb) >> 2);
+        const unsigned short hl[4] = {h[u].x, h[u].y, h[u].z, h[u].w};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int o = o0 + j * q;
+          const int hi = int(o >= s1) + int(o >= s2) + int(o >= s3);
+          xg[u][j] = x[ok ? (hi << 16) | int(hl[j]) : 0];
+        }
+      }
+      typename RV::T wc[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) wc[u] = w[u];
+      const int64_t kn = k + STEP;
+      const bool more = kn < en;
+      __builtin_amdgcn_sched_barrier(0);
+      // columns first: the next step's gathers wait for them only
+      int64_t qn[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        qn[u] = kn + 4 * LANES * u < en ? kn + 4 * LANES * u : 0;
+        h[u] = __builtin_nontemporal_load(reinterpret_cast<const vus4*>(lo + qn[u]));
+      }
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int u = 0; u < U; ++u) w[u] = RV::load(val, qn[u]);
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        float v[4];
+        RV::get(wc[u], v);
+        if (k + 4 * LANES * u < en) {
+#pragma unroll
+          for (int j = 0; j < 4; ++j) acc = fmaf(v[j], xg[u][j], acc);
+        }
+      }
+      if (!more) break;
+      k = kn;
+    }
   }
 #pragma unroll
   for (int off = LANES / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LANES);
@@ -1904,7 +2019,12 @@ int launch_fwd_any(const raptr_recon* R, int F, const float* x, const float* y, 
     const unsigned g = unsigned((m + (256 / L) - 1) / (256 / L));
 #define RAPTR_F1C(VV, LL, UU)                                                                \
   if (R->vt == VV && L == LL && U == UU) {                                                   \
-    hipLaunchKernelGGL((k_fwd1c<VV, LL, UU>), dim3(g), dim3(256), 0, st, m, R->frp,           \
+    if (R->fwd_kind == 1)                                                                    \
+      hipLaunchKernelGGL((k_fwd1cp<VV, LL, UU>), dim3(g), dim3(256), 0, st, m, R->frp,        \
+                         (const uint16_t*)R->fcol16, (const uint64_t*)R->fsplit,             \
+                         (const typename Val<VV>::T*)R->fval, R->rscale, x, y, e);           \
+    else                                                                                     \
+      hipLaunchKernelGGL((k_fwd1c<VV, LL, UU>), dim3(g), dim3(256), 0, st, m, R->frp,         \
                        (const uint16_t*)R->fcol16, (const uint64_t*)R->fsplit,               \
                        (const typename Val<VV>::T*)R->fval, R->rscale, x, y, e);             \
     RAPTR_LAUNCH_CHECK();                                                                    \
@@ -2993,6 +3113,29 @@ int raptr_recon_set_compact_rows(raptr_recon* R, int on) {
   }
   R->bwd_r16 = on && R->brow16 ? 1 : 0;
   R->fwd_c16 = on && R->fcol16 ? 1 : 0;
+  return RAPTR_OK;
+}
+
+int raptr_recon_set_forward_kind(raptr_recon* R, int kind) {
+  if (!R) return fail(RAPTR_ERR_INVALID_ARG, "null plan");
+  if (kind < 0 || kind > 1) return fail(RAPTR_ERR_INVALID_ARG, "forward kind must be 0 or 1");
+  if (kind == 1 && !R->fcol16)
+    return fail(RAPTR_ERR_INVALID_ARG, "the pipelined forward reads 16-bit columns (none here)");
+  if (R->fwd_kind != kind) {
+    for (auto& kv : R->graphs) {
+      if (kv.second.second) (void)hipGraphExecDestroy(kv.second.second);
+      if (kv.second.first) (void)hipGraphDestroy(kv.second.first);
+    }
+    R->graphs.clear();
+  }
+  R->fwd_kind = kind;
+  return RAPTR_OK;
+}
+
+int raptr_recon_get_forward_kind(const raptr_recon* R, int* kind, int* available) {
+  if (!R) return fail(RAPTR_ERR_INVALID_ARG, "null plan");
+  if (kind) *kind = R->fwd_kind;
+  if (available) *available = 1 | (R->fcol16 ? 2 : 0);
   return RAPTR_OK;
 }
 

@@ -139,6 +139,7 @@ struct raptr_recon {
   uint16_t* fcol16 = nullptr;  // forward columns' low 16 bits (grids of <= 4 x 65,536 voxels)
   uint64_t* fsplit = nullptr;  // per row: sorted positions where the high part steps
   int fwd_c16 = 0;             // single-frame forward reads fcol16
+  int fwd_kind = 0;            // 0 plain, 1 software-pipelined (16-bit columns)
   int64_t ngf = 0, ngc = 0, gfnnz = 0, gbnnz = 0;
   int64_t* gfp = nullptr;  // ngf + 1
   int* gfcol = nullptr;

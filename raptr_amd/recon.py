@@ -83,7 +83,8 @@ class Reconstructor:
                "bwd_blocks_per_launch": capi.lib().raptr_recon_get_blocks_per_launch(self.handle),
                "transpose_nnz": capi.lib().raptr_recon_transpose_nnz(self.handle),
                "row_ordered": bool(capi.lib().raptr_recon_row_ordered(self.handle)),
-               "compact_rows": capi.lib().raptr_recon_get_compact_rows(self.handle)}
+               "compact_rows": capi.lib().raptr_recon_get_compact_rows(self.handle),
+               "fwd_kind": self.forward_kind()[0]}
         gi = self.grouped()
         if gi["group"]:
             out.update({"grouped64": gi["on"], "group": gi["group"],
@@ -125,8 +126,20 @@ class Reconstructor:
         capi.call("raptr_recon_set_grouped", self.handle, 1 if on else 0, int(unroll), int(kind))
 
     def set_compact_rows(self, on):
-        """16-bit transpose rows in the per-XCD single-frame backward."""
+        """16-bit transpose rows (per-XCD single-frame backward) and forward
+        columns, where the plan has them."""
         capi.call("raptr_recon_set_compact_rows", self.handle, 1 if on else 0)
+
+    def set_forward_kind(self, kind):
+        """Single-frame forward kernel: 0 one entry per lane, 1 the same
+        software-pipelined over 16-bit columns (bit-identical)."""
+        capi.call("raptr_recon_set_forward_kind", self.handle, int(kind))
+
+    def forward_kind(self):
+        """(kind, available kinds)."""
+        k, a = C.c_int(), C.c_int()
+        capi.call("raptr_recon_get_forward_kind", self.handle, C.byref(k), C.byref(a))
+        return k.value, [j for j in range(2) if a.value >> j & 1]
 
     def set_blocks_per_launch(self, blocks):
         capi.call("raptr_recon_set_blocks_per_launch", self.handle, int(blocks))
@@ -231,13 +244,20 @@ class Reconstructor:
             fshapes = [(fl, fu) for fl in (16, 32, 64) for fu in (1, 2, 4)]
         else:
             fshapes = [(inf["fwd_lanes"], fu) for fu in (1, 2, 4)]
+        # single frame: every forward kind the plan has
+        kinds = self.forward_kind()[1] if nframes == 1 else [None]
         best_f = None
-        for fl, fu in fshapes:
-            self.set_config(fl, fu, bu)
-            ms = timed()
-            if best_f is None or ms < best_f[1]:
-                best_f = ((fl, fu), ms)
-        fl, fu = best_f[0]
+        for fk in kinds:
+            if fk is not None:
+                self.set_forward_kind(fk)
+            for fl, fu in fshapes:
+                self.set_config(fl, fu, bu)
+                ms = timed()
+                if best_f is None or ms < best_f[1]:
+                    best_f = ((fl, fu, fk), ms)
+        fl, fu, fp = best_f[0]
+        if fp is not None:
+            self.set_forward_kind(fp)
         bshapes = [(0, 16, 512, u, bpl) for u in (1, 2, 4) for bpl in (1, 4, 16)
                    if bpl == 1 or bpl < inf["row_blocks"]]
         staged_ok = nframes == 1 and self.max_frames == 1
@@ -300,7 +320,8 @@ class Reconstructor:
                 self.set_grouped(False)
             self.set_blocks_per_launch(bpl)
         capi.call("raptr_memcpy_d2d", x.ptr, x_save.ptr, x.nbytes, stream)
-        cfg = {"fwd_lanes": fl, "fwd_unroll": fu, "bwd_unroll": u, "bwd_staged": st,
+        cfg = {"fwd_lanes": fl, "fwd_unroll": fu, "fwd_kind": self.forward_kind()[0],
+               "bwd_unroll": u, "bwd_staged": st,
                "bwd_lanes_per_column": lpc, "bwd_workgroups": wgs, "bwd_blocks_per_launch": bpl,
                "broadcast64": bc, "grouped64": g_on, "grouped_unroll": g_u,
                "grouped_kind": g_p}

@@ -469,21 +469,43 @@ def test_recon_grouped_only_plan(gpu, small_system, vt):
         Reconstructor(S["A"], S["sens"], vt, 16, grouped_only=True)
 
 
+def _runs(rng, n, L):
+    """L ascending columns in runs of 1..9 consecutive voxels, the last run
+    ending at voxel n - 1."""
+    out, c = [], n
+    while len(out) < L:
+        r = int(rng.integers(1, 10))
+        c -= r + int(rng.integers(1, 60))
+        out.extend(range(max(c, 0), max(c, 0) + r))
+        if c <= 0:
+            break
+    c = np.unique(np.array(out[:L], np.int64))
+    if len(c):
+        c = c + (n - 1 - c.max())
+    return c
+
+
 def _wide_csr(n, m=2500, seed=5, unsorted_row=False):
     """Host CSR over n voxels whose rows span several 65,536-voxel blocks:
     lengths 0..700 plus a few rows of 3,000-9,000 entries (several 128-entry
-    interleave chunks and a ragged last chunk), ascending columns."""
+    interleave chunks and a ragged last chunk), ascending columns; every
+    third row made of runs of 1..9 consecutive voxels ending at voxel n - 1."""
     rng = np.random.default_rng(seed)
     lens = rng.integers(0, 700, m)
     lens[rng.choice(m, 6, replace=False)] = rng.integers(3000, 9000, 6)
     lens[:3] = 0
-    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
     cols = []
-    for L in lens:
+    for r, L in enumerate(lens):
+        if r % 3 == 1:
+            c = _runs(rng, n, L)
+            lens[r] = len(c)
+            cols.append(c)
+            continue
         c0 = int(rng.integers(0, n))
         span = int(rng.integers(L + 1, min(n, 40 * L + 2) + 1))
         c = np.sort(rng.choice(span, L, replace=False)) + c0
         cols.append(np.sort(c % n))
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
     col = np.concatenate(cols).astype(np.int32)
     if unsorted_row:  # one long row with descending columns over the whole grid
         i = int(np.argmax(lens))
@@ -502,17 +524,18 @@ def _device_csr(rp, col, val, n):
 
 @pytest.mark.parametrize("vt", ["f32", "f16", "bf16", "u16"])
 def test_recon_forward_compact_columns(gpu, vt):
-    """16-bit forward columns (low parts + per-row split points) on a
-    262,139-voxel matrix whose rows cross the 65,536-voxel blocks: the same
-    ratios, bit for bit, as the 32-bit columns for every lanes/unroll shape;
-    FP32 against an FP64 host product."""
+    """Single-frame forward on a 262,139-voxel matrix whose rows cross the
+    65,536-voxel blocks (and end at the last voxel): 16-bit columns (low
+    parts + per-row split points), plain and software-pipelined, give the
+    same ratios, bit for bit, as 32-bit columns for every lanes/unroll
+    shape; in FP32 they match an FP64 host product."""
     from raptr_amd import _capi as capi
     from raptr_amd.device import DeviceArray
     from raptr_amd.recon import Reconstructor
     n = 4 * 65536 - 5
     rp, col, val = _wide_csr(n)
     m = len(rp) - 1
-    assert (col >> 16).max() == 3
+    assert (col >> 16).max() == 3 and col.max() == n - 1
     A = _device_csr(rp, col, val, n)
     sens = DeviceArray.from_numpy(np.ones(n))
     rng = np.random.default_rng(7)
@@ -520,20 +543,27 @@ def test_recon_forward_compact_columns(gpu, vt):
     y = rng.random(m).astype(np.float32) + 0.5
     R = Reconstructor(A, sens, vt, 1, layout="blocked")
     assert R.info()["compact_rows"] & 2
+    assert R.forward_kind() == (0, [0, 1])
     dx, dy, de = DeviceArray.from_numpy(x), DeviceArray.from_numpy(y), DeviceArray(m, np.float32)
-    for lanes, unroll in [(16, 1), (32, 2), (64, 4), (16, 4), (64, 1)]:
+    yt = np.array([np.dot(val[rp[i]:rp[i + 1]].astype(np.float64), x[col[rp[i]:rp[i + 1]]])
+                   for i in range(m)])
+    ref = np.where(yt != 0, y / np.where(yt != 0, yt, 1), 0)
+
+    def fwd():
+        capi.call("raptr_recon_forward", R.handle, 1, dx.ptr, dy.ptr, de.ptr, None)
+        return de.numpy()
+
+    for lanes, unroll in [(16, 1), (32, 2), (64, 4), (16, 4), (64, 1), (32, 4)]:
         R.set_config(lanes, unroll, 1)
         out = []
-        for on in (True, False):
+        for on, k in ((True, 0), (True, 1), (False, 0)):
             R.set_compact_rows(on)
+            R.set_forward_kind(k)
             assert bool(R.info()["compact_rows"] & 2) == on
-            capi.call("raptr_recon_forward", R.handle, 1, dx.ptr, dy.ptr, de.ptr, None)
-            out.append(de.numpy())
-        np.testing.assert_array_equal(out[0], out[1])
+            out.append(fwd())
+        np.testing.assert_array_equal(out[0], out[2])
+        np.testing.assert_array_equal(out[1], out[2])
         if vt == "f32":
-            yt = np.array([np.dot(val[rp[i]:rp[i + 1]].astype(np.float64),
-                                  x[col[rp[i]:rp[i + 1]]]) for i in range(m)])
-            ref = np.where(yt != 0, y / np.where(yt != 0, yt, 1), 0)
             np.testing.assert_allclose(out[0], ref, rtol=1e-5)
 
 
@@ -542,6 +572,7 @@ def test_recon_forward_compact_columns_absent(gpu, case):
     """No 16-bit columns when a row's columns step down across 65,536-voxel
     blocks or the grid has more than 262,144 voxels: the plan keeps its
     32-bit columns and the forward still matches the host product."""
+    from raptr_amd import RaptrError
     from raptr_amd import _capi as capi
     from raptr_amd.device import DeviceArray
     from raptr_amd.recon import Reconstructor
@@ -552,6 +583,9 @@ def test_recon_forward_compact_columns_absent(gpu, case):
     R = Reconstructor(A, DeviceArray.from_numpy(np.ones(n)), "f32", 1, layout="blocked",
                       block_rows=512)
     assert R.info()["compact_rows"] & 2 == 0
+    assert R.forward_kind() == (0, [0])
+    with pytest.raises(RaptrError):
+        R.set_forward_kind(1)
     x = np.random.default_rng(2).random(n).astype(np.float32)
     y = np.ones(m, np.float32)
     dx, dy, de = DeviceArray.from_numpy(x), DeviceArray.from_numpy(y), DeviceArray(m, np.float32)
