@@ -412,12 +412,18 @@ int raptr_recon_get_backward(const raptr_recon* plan, int* staged, int* lanes_pe
  * 1 = columns on; -1: none present. */
 int raptr_recon_set_compact_rows(raptr_recon* plan, int on);
 int raptr_recon_get_compact_rows(const raptr_recon* plan);
-/* Kernel of the single-frame forward pass:
+/* Kernel of the single-frame forward pass (all three give the same sums,
+ * bit for bit, for the same lanes/unroll):
  *   0  one matrix entry per lane and gather (32-bit or, when on, 16-bit
  *      columns);
  *   1  as 0 over 16-bit columns, software-pipelined (each lane loads its
  *      next step while the gathers of the current one are in flight);
- *      bit-identical to 0.
+ *   2  x staged in LDS: each workgroup copies x over the sorted column
+ *      union of its 256 consecutive plan rows into LDS and gathers from
+ *      there through 16-bit ranks.  Built for single-frame blocked plans
+ *      of <= 258,048 voxels whose unions all hold <= 16,384 voxels (rows
+ *      in an order that keeps neighbouring lines together, e.g.
+ *      raptr_locality_order); the default there.
  * available: bit k set if kind k can run on this plan.  Null outputs are
  * skipped. */
 int raptr_recon_set_forward_kind(raptr_recon* plan, int kind);

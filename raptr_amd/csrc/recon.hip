@@ -20,6 +20,7 @@
 #include <algorithm>
 #include <climits>
 #include <type_traits>
+#include <vector>
 
 #include "recon.hpp"
 
@@ -439,6 +440,73 @@ __global__ __launch_bounds__(256) void k_fwd1cp(int64_t m, const int64_t* __rest
     float yt = sc * acc;
     float r = yt != 0.0f ? y[i] / yt : 0.0f;
     e[i] = VT == kF32 ? r : sc * r;  // ratio folded with the row scale
+  }
+}
+
+// Single frame with x staged in LDS: a workgroup takes kLocRows
+// consecutive plan rows (neighbouring lines through the same voxels in the
+// locality row order), copies x over the sorted union of their columns into
+// LDS, then runs k_fwd1c's loop with each entry's 16-bit index into that
+// union.  The matrix stream stays 6 bytes per FP32 entry; the gathers read
+// LDS instead of the L1 cache, whose tag lookups bound k_fwd1/k_fwd1c.
+constexpr int kLocRows = 256;
+constexpr int kLocMaxUnion = 16384;  // 64 KB of LDS per workgroup
+
+template <int VT, int LANES, int U>
+__global__ __launch_bounds__(256) void k_fwd1l(int64_t m, const int64_t* __restrict__ rp,
+                                               const uint16_t* __restrict__ lcol,
+                                               const int64_t* __restrict__ up,
+                                               const int* __restrict__ ucol,
+                                               const typename Val<VT>::T* __restrict__ val,
+                                               const float* __restrict__ rscale,
+                                               const float* __restrict__ x,
+                                               const float* __restrict__ y,
+                                               float* __restrict__ e) {
+  extern __shared__ float xs[];
+  const int64_t grp = blockIdx.x;
+  const int64_t ub = up[grp];
+  const int nu = int(up[grp + 1] - ub);
+  for (int j = threadIdx.x; j < nu; j += 256) xs[j] = x[ucol[ub + j]];
+  __syncthreads();
+  const int g = threadIdx.x % LANES;
+  const int64_t r0 = grp * kLocRows, r1 = r0 + kLocRows < m ? r0 + kLocRows : m;
+  for (int64_t i = r0 + threadIdx.x / LANES; i < r1; i += 256 / LANES) {
+    const int64_t b = rp[i], en = rp[i + 1];  // multiples of 4
+    float acc = 0.0f;
+    int64_t k = b + 4 * g;
+    for (; k + int64_t(4) * LANES * (U - 1) < en; k += int64_t(4) * LANES * U) {
+      vus4 c[U];
+      float v[U][4];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        c[u] = __builtin_nontemporal_load(reinterpret_cast<const vus4*>(lcol + k + 4 * LANES * u));
+        load4<VT>(val, k + 4 * LANES * u, v[u][0], v[u][1], v[u][2], v[u][3]);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        acc = fmaf(v[u][0], xs[c[u].x], acc);
+        acc = fmaf(v[u][1], xs[c[u].y], acc);
+        acc = fmaf(v[u][2], xs[c[u].z], acc);
+        acc = fmaf(v[u][3], xs[c[u].w], acc);
+      }
+    }
+    for (; k < en; k += int64_t(4) * LANES) {
+      const vus4 c = __builtin_nontemporal_load(reinterpret_cast<const vus4*>(lcol + k));
+      float v0, v1, v2, v3;
+      load4<VT>(val, k, v0, v1, v2, v3);
+      acc = fmaf(v0, xs[c.x], acc);
+      acc = fmaf(v1, xs[c.y], acc);
+      acc = fmaf(v2, xs[c.z], acc);
+      acc = fmaf(v3, xs[c.w], acc);
+    }
+#pragma unroll
+    for (int off = LANES / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LANES);
+    if (g == 0) {
+      float sc = VT == kF32 ? 1.0f : rscale[i];
+      float yt = sc * acc;
+      float r = yt != 0.0f ? y[i] / yt : 0.0f;
+      e[i] = VT == kF32 ? r : sc * r;  // ratio folded with the row scale
+    }
   }
 }
 
@@ -1873,6 +1941,75 @@ __global__ __launch_bounds__(256) void k_fwd_cols16(const int64_t* __restrict__ 
   }
 }
 
+// Column unions of the LDS-staged forward (k_fwd1l), one workgroup per
+// group of kLocRows plan rows: a bitmap of the group's columns in LDS (n <=
+// 258,048 voxels: <= 8,064 words) and the words' exclusive popcount prefix.
+// pass 0 writes the union size cnt[group]; pass 1 the sorted union
+// ucol[up[group] + r] and every padded entry's rank lcol[d] in it.
+__global__ __launch_bounds__(256) void k_loc_union(const int64_t* __restrict__ frp,
+                                                   const int* __restrict__ fcol, int64_t m,
+                                                   int n, int pass, int64_t* __restrict__ cnt,
+                                                   const int64_t* __restrict__ up,
+                                                   int* __restrict__ ucol,
+                                                   uint16_t* __restrict__ lcol) {
+  extern __shared__ float ldsf[];  // bitmap, prefix (nw words each), 256 partials
+  uint32_t* lw = reinterpret_cast<uint32_t*>(ldsf);
+  const int nw = (n + 31) >> 5;
+  uint32_t* bm = lw;
+  uint32_t* pre = lw + nw;
+  uint32_t* part = lw + 2 * nw;
+  const int64_t grp = blockIdx.x;
+  const int64_t r0 = grp * kLocRows, r1 = r0 + kLocRows < m ? r0 + kLocRows : m;
+  const int64_t d0 = frp[r0], d1 = frp[r1];
+  for (int w = threadIdx.x; w < nw; w += 256) bm[w] = 0;
+  __syncthreads();
+  for (int64_t d = d0 + threadIdx.x; d < d1; d += 256) {
+    const int c = fcol[d];
+    atomicOr(&bm[c >> 5], 1u << (c & 31));
+  }
+  __syncthreads();
+  // thread t sums words [t*per, (t+1)*per); partial sums scanned by thread 0
+  const int per = (nw + 255) / 256;
+  const int w0 = threadIdx.x * per, w1 = w0 + per < nw ? w0 + per : nw;
+  uint32_t t = 0;
+  for (int w = w0; w < w1; ++w) t += __popc(bm[w]);
+  part[threadIdx.x] = t;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t run = 0;
+    for (int j = 0; j < 256; ++j) {
+      const uint32_t v = part[j];
+      part[j] = run;
+      run += v;
+    }
+    if (pass == 0) cnt[grp] = int64_t(run);
+  }
+  __syncthreads();
+  if (pass == 0) return;
+  uint32_t run = part[threadIdx.x];
+  for (int w = w0; w < w1; ++w) {
+    pre[w] = run;
+    run += __popc(bm[w]);
+  }
+  __syncthreads();
+  const int64_t ub = up[grp];
+  for (int w = threadIdx.x; w < nw; w += 256) {
+    uint32_t bits = bm[w];
+    uint32_t r = pre[w];
+    while (bits) {
+      const int bb = __ffs(bits) - 1;
+      ucol[ub + r] = (w << 5) | bb;
+      ++r;
+      bits &= bits - 1;
+    }
+  }
+  for (int64_t d = d0 + threadIdx.x; d < d1; d += 256) {
+    const int c = fcol[d];
+    const uint32_t w = uint32_t(c) >> 5;
+    lcol[d] = uint16_t(pre[w] + __popc(bm[w] & ((1u << (c & 31)) - 1)));
+  }
+}
+
 // scale_f = 1/sum_b partials[b][f] (FP64, fixed order); 0 for an empty frame
 // one workgroup per frame f = blockIdx.x (each thread sums blocks t, t+256,
 // ... in order, then a fixed tree): the same order for every launch shape
@@ -2014,6 +2151,28 @@ int launch_fwd_any(const raptr_recon* R, int F, const float* x, const float* y, 
     return fail(RAPTR_ERR_INVALID_ARG, "a grouped-only plan runs 64-frame grouped passes only");
   const int64_t m = R->m;
   const int U = R->fwd_unroll;
+  if (F == 1 && R->fwd_kind == 2 && R->lcol) {
+    const int L = R->fwd_lanes;
+    const unsigned g = unsigned((int64_t(m) + kLocRows - 1) / kLocRows);
+    const size_t shm = size_t(R->lmax > 0 ? R->lmax : 1) * sizeof(float);
+#define RAPTR_FL(VV, LL, UU)                                                                 \
+  if (R->vt == VV && L == LL && U == UU) {                                                   \
+    hipLaunchKernelGGL((k_fwd1l<VV, LL, UU>), dim3(g), dim3(256), shm, st, m, R->frp,         \
+                       (const uint16_t*)R->lcol, (const int64_t*)R->lup,                     \
+                       (const int*)R->lucol, (const typename Val<VV>::T*)R->fval, R->rscale, \
+                       x, y, e);                                                             \
+    RAPTR_LAUNCH_CHECK();                                                                    \
+    return RAPTR_OK;                                                                         \
+  }
+#define RAPTR_FLV(VV)                                                                        \
+  RAPTR_FL(VV, 16, 1) RAPTR_FL(VV, 32, 1) RAPTR_FL(VV, 64, 1) RAPTR_FL(VV, 16, 2)            \
+  RAPTR_FL(VV, 32, 2) RAPTR_FL(VV, 64, 2) RAPTR_FL(VV, 16, 4) RAPTR_FL(VV, 32, 4)            \
+  RAPTR_FL(VV, 64, 4)
+    RAPTR_FLV(kF32) RAPTR_FLV(kF16) RAPTR_FLV(kBF16) RAPTR_FLV(kU16)
+#undef RAPTR_FLV
+#undef RAPTR_FL
+    return fail(RAPTR_ERR_INVALID_ARG, "forward shape lanes %d unroll %d unsupported", L, U);
+  }
   if (F == 1 && R->fcol16 && R->fwd_c16) {
     const int L = R->fwd_lanes;
     const unsigned g = unsigned((m + (256 / L) - 1) / (256 / L));
@@ -2502,6 +2661,55 @@ bool frames_aligned(const void* p, int F) {
 
 namespace raptr {
 namespace {
+// column unions of the LDS-staged forward (k_fwd1l): sizes, scan, unions
+// and local indices; none when a group's union exceeds kLocMaxUnion
+int build_local_columns(raptr_recon* R, hipStream_t st) {
+  const int64_t m = R->m;
+  const int n = R->n;
+  const int64_t ng = (m + kLocRows - 1) / kLocRows;
+  const size_t shm = (2 * size_t((n + 31) >> 5) + 256) * sizeof(uint32_t);
+  int rc;
+  int64_t* cnt = nullptr;
+  if ((rc = ralloc(R, &R->lup, size_t(ng) + 1)) || (rc = ralloc(R, &cnt, size_t(ng))))
+    return rc;
+  hipLaunchKernelGGL(k_loc_union, dim3(unsigned(ng)), dim3(256), shm, st,
+                     (const int64_t*)R->frp, (const int*)R->fcol, m, n, 0, cnt,
+                     (const int64_t*)nullptr, (int*)nullptr, (uint16_t*)nullptr);
+  void* tmp = nullptr;
+  const size_t sb = scan_workspace_bytes(ng);
+  RAPTR_HIP_TRY(hipMalloc(&tmp, 512 + sb));
+  int64_t* tot = static_cast<int64_t*>(tmp);
+  rc = scan_exclusive_i64(cnt, R->lup, ng, tot, static_cast<char*>(tmp) + 256, sb, st);
+  if (!rc) hipLaunchKernelGGL(k_set_last, dim3(1), dim3(1), 0, st, R->lup, int(ng),
+                              (const int64_t*)tot);
+  std::vector<int64_t> hc(static_cast<size_t>(ng));
+  int64_t h = 0;
+  hipError_t le = hipGetLastError();
+  if (le == hipSuccess) le = hipMemcpyAsync(&h, tot, 8, hipMemcpyDeviceToHost, st);
+  if (le == hipSuccess)
+    le = hipMemcpyAsync(hc.data(), cnt, size_t(ng) * 8, hipMemcpyDeviceToHost, st);
+  if (le == hipSuccess) le = hipStreamSynchronize(st);
+  (void)hipFree(tmp);
+  if (rc) return rc;
+  if (le != hipSuccess) return fail(RAPTR_ERR_DEVICE, "column unions: %s", hipGetErrorString(le));
+  const int64_t mx = hc.empty() ? 0 : *std::max_element(hc.begin(), hc.end());
+  if (mx > kLocMaxUnion) {  // too spread out for LDS: the plan keeps no unions
+    R->lup = nullptr;
+    return RAPTR_OK;
+  }
+  if ((rc = ralloc(R, &R->lucol, size_t(h) + 1)) || (rc = ralloc(R, &R->lcol, size_t(R->pnnz) + 8)))
+    return rc;
+  hipLaunchKernelGGL(k_loc_union, dim3(unsigned(ng)), dim3(256), shm, st,
+                     (const int64_t*)R->frp, (const int*)R->fcol, m, n, 1, cnt,
+                     (const int64_t*)R->lup, R->lucol, R->lcol);
+  le = hipGetLastError();
+  if (le == hipSuccess) le = hipStreamSynchronize(st);
+  if (le != hipSuccess) return fail(RAPTR_ERR_DEVICE, "column unions: %s", hipGetErrorString(le));
+  R->lmax = int(mx);
+  R->lnu = h;
+  return RAPTR_OK;
+}
+
 int create_plan(raptr_recon** out, int m, int n, int64_t nnz, const int64_t* d_rp,
                 const int* d_col, const float* d_val, const double* d_sens, int value_type,
                 int max_frames, int block_rows, int layout, const int* gdims, int group,
@@ -2852,6 +3060,13 @@ int create_plan(raptr_recon** out, int m, int n, int64_t nnz, const int64_t* d_r
       R->fwd_c16 = 1;
     }
   }
+  // ... and the column unions of the LDS-staged forward (grids of <= 258,048
+  // voxels: the union kernel's bitmap and prefix fit 64 KB of LDS)
+  if (max_frames == 1 && layout == RAPTR_LAYOUT_BLOCKED && R->fcol && n <= 8064 * 32 && m > 0 &&
+      R->pnnz > 0) {
+    if ((rc = build_local_columns(R, st))) return bail(rc);
+    if (R->lcol) R->fwd_kind = 2;
+  }
   *out = R;
   return RAPTR_OK;
 }
@@ -3118,9 +3333,11 @@ int raptr_recon_set_compact_rows(raptr_recon* R, int on) {
 
 int raptr_recon_set_forward_kind(raptr_recon* R, int kind) {
   if (!R) return fail(RAPTR_ERR_INVALID_ARG, "null plan");
-  if (kind < 0 || kind > 1) return fail(RAPTR_ERR_INVALID_ARG, "forward kind must be 0 or 1");
+  if (kind < 0 || kind > 2) return fail(RAPTR_ERR_INVALID_ARG, "forward kind must be 0, 1 or 2");
   if (kind == 1 && !R->fcol16)
     return fail(RAPTR_ERR_INVALID_ARG, "the pipelined forward reads 16-bit columns (none here)");
+  if (kind == 2 && !R->lcol)
+    return fail(RAPTR_ERR_INVALID_ARG, "no LDS column unions on this plan");
   if (R->fwd_kind != kind) {
     for (auto& kv : R->graphs) {
       if (kv.second.second) (void)hipGraphExecDestroy(kv.second.second);
@@ -3135,7 +3352,7 @@ int raptr_recon_set_forward_kind(raptr_recon* R, int kind) {
 int raptr_recon_get_forward_kind(const raptr_recon* R, int* kind, int* available) {
   if (!R) return fail(RAPTR_ERR_INVALID_ARG, "null plan");
   if (kind) *kind = R->fwd_kind;
-  if (available) *available = 1 | (R->fcol16 ? 2 : 0);
+  if (available) *available = 1 | (R->fcol16 ? 2 : 0) | (R->lcol ? 4 : 0);
   return RAPTR_OK;
 }
 

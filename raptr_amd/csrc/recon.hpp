@@ -139,7 +139,15 @@ struct raptr_recon {
   uint16_t* fcol16 = nullptr;  // forward columns' low 16 bits (grids of <= 4 x 65,536 voxels)
   uint64_t* fsplit = nullptr;  // per row: sorted positions where the high part steps
   int fwd_c16 = 0;             // single-frame forward reads fcol16
-  int fwd_kind = 0;            // 0 plain, 1 software-pipelined (16-bit columns)
+  int fwd_kind = 0;            // 0 plain, 1 software-pipelined, 2 LDS-staged x
+  // LDS-staged forward: per group of consecutive plan rows the sorted union
+  // of its columns (lup: group offsets, lucol: columns) and every padded
+  // entry's 16-bit rank in its group's union
+  int64_t* lup = nullptr;
+  int* lucol = nullptr;
+  uint16_t* lcol = nullptr;
+  int lmax = 0;                // largest union
+  int64_t lnu = 0;             // union entries of all groups
   int64_t ngf = 0, ngc = 0, gfnnz = 0, gbnnz = 0;
   int64_t* gfp = nullptr;  // ngf + 1
   int* gfcol = nullptr;

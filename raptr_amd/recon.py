@@ -132,14 +132,15 @@ class Reconstructor:
 
     def set_forward_kind(self, kind):
         """Single-frame forward kernel: 0 one entry per lane, 1 the same
-        software-pipelined over 16-bit columns (bit-identical)."""
+        software-pipelined over 16-bit columns, 2 x staged in LDS over the
+        column union of each group of 256 plan rows (all bit-identical)."""
         capi.call("raptr_recon_set_forward_kind", self.handle, int(kind))
 
     def forward_kind(self):
         """(kind, available kinds)."""
         k, a = C.c_int(), C.c_int()
         capi.call("raptr_recon_get_forward_kind", self.handle, C.byref(k), C.byref(a))
-        return k.value, [j for j in range(2) if a.value >> j & 1]
+        return k.value, [j for j in range(3) if a.value >> j & 1]
 
     def set_blocks_per_launch(self, blocks):
         capi.call("raptr_recon_set_blocks_per_launch", self.handle, int(blocks))

@@ -515,6 +515,24 @@ def _wide_csr(n, m=2500, seed=5, unsorted_row=False):
     return rp, col, val
 
 
+def _local_csr(n, m=1500, seed=9, window=8000):
+    """Host CSR whose groups of 256 consecutive rows draw their columns from
+    one window of `window` voxels placed anywhere in the grid (the column
+    unions of the LDS-staged forward fit): lengths 0..700, ascending."""
+    rng = np.random.default_rng(seed)
+    lens = rng.integers(0, 700, m)
+    lens[:2] = 0
+    cols = []
+    for r, L in enumerate(lens):
+        if r % 256 == 0:
+            w0 = int(rng.integers(0, n - window + 1)) if r < m - 256 else n - window
+        cols.append(np.sort(rng.choice(window, L, replace=False)) + w0)
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    col = np.concatenate(cols).astype(np.int32)
+    val = rng.random(len(col)).astype(np.float32) + 0.01
+    return rp, col, val
+
+
 def _device_csr(rp, col, val, n):
     from raptr_amd.device import DeviceArray
     from raptr_amd.sysmat import DeviceCSR
@@ -594,3 +612,78 @@ def test_recon_forward_compact_columns_absent(gpu, case):
                    for i in range(m)])
     ref = np.where(yt != 0, y / np.where(yt != 0, yt, 1), 0)
     np.testing.assert_allclose(de.numpy(), ref, rtol=1e-5)
+
+
+@pytest.mark.parametrize("vt", ["f32", "bf16", "u16"])
+def test_recon_forward_lds_staged(gpu, vt):
+    """LDS-staged forward (kind 2: x over each 256-row group's column union
+    in LDS, 16-bit ranks) on a 258,000-voxel matrix whose groups sit in
+    8,000-voxel windows across the grid: the same ratios, bit for bit, as
+    kinds 0 and 1 and the 32-bit columns for every lanes/unroll shape; FP32
+    against an FP64 host product."""
+    from raptr_amd import _capi as capi
+    from raptr_amd.device import DeviceArray
+    from raptr_amd.recon import Reconstructor
+    n = 258000
+    rp, col, val = _local_csr(n)
+    m = len(rp) - 1
+    assert col.max() == n - 1 and (col >> 16).max() == 3
+    A = _device_csr(rp, col, val, n)
+    R = Reconstructor(A, DeviceArray.from_numpy(np.ones(n)), vt, 1, layout="blocked")
+    assert R.forward_kind() == (2, [0, 1, 2])
+    rng = np.random.default_rng(4)
+    x = rng.random(n).astype(np.float32)
+    y = rng.random(m).astype(np.float32) + 0.5
+    dx, dy, de = DeviceArray.from_numpy(x), DeviceArray.from_numpy(y), DeviceArray(m, np.float32)
+
+    def fwd():
+        capi.call("raptr_recon_forward", R.handle, 1, dx.ptr, dy.ptr, de.ptr, None)
+        return de.numpy()
+
+    for lanes, unroll in [(16, 1), (32, 2), (64, 4), (16, 4), (64, 1)]:
+        R.set_config(lanes, unroll, 1)
+        out = []
+        for on, k in ((True, 2), (True, 0), (True, 1), (False, 0)):
+            R.set_compact_rows(on)
+            R.set_forward_kind(k)
+            out.append(fwd())
+        for o in out[1:]:
+            np.testing.assert_array_equal(out[0], o)
+    if vt == "f32":
+        yt = np.array([np.dot(val[rp[i]:rp[i + 1]].astype(np.float64), x[col[rp[i]:rp[i + 1]]])
+                       for i in range(m)])
+        np.testing.assert_allclose(out[0], np.where(yt != 0, y / np.where(yt != 0, yt, 1), 0),
+                                   rtol=1e-5)
+
+
+def test_recon_forward_lds_staged_absent(gpu):
+    """No column unions when a 256-row group touches more than 16,384
+    voxels (rows of random columns over the grid) or the grid has more than
+    258,048 voxels; kind 2 is then refused."""
+    from raptr_amd import RaptrError
+    from raptr_amd.device import DeviceArray
+    from raptr_amd.recon import Reconstructor
+    for n, (rp, col, val) in ((200000, _wide_csr(200000, m=600)),
+                              (258049, _local_csr(258049, m=600))):
+        R = Reconstructor(_device_csr(rp, col, val, n), DeviceArray.from_numpy(np.ones(n)), "f32",
+                          1, layout="blocked")
+        assert R.forward_kind() == (0, [0, 1])
+        with pytest.raises(RaptrError):
+            R.set_forward_kind(2)
+
+
+def test_recon_forward_lds_staged_small_system(gpu, oracle, small_system):
+    """The small workload in locality row order builds column unions (kind 2
+    is the default); MLEM with kinds 2 and 0 is identical bit for bit and
+    within 1e-4 of the FP64 oracle."""
+    from raptr_amd.recon import Reconstructor
+    S = small_system
+    n = S["grid"].nvox
+    x0 = np.full(n, 1.0 / n, np.float32)
+    R = Reconstructor(S["A"], S["sens"], "f32", 1, layout="blocked", row_order=S["setup"])
+    assert R.forward_kind() == (2, [0, 1, 2])
+    a = R.mlem(S["y"], x0, 5)
+    R.set_forward_kind(0)
+    b = R.mlem(S["y"], x0, 5)
+    np.testing.assert_array_equal(a, b)
+    assert _rel(a, _oracle_mlem(oracle, S, S["y"], 5)) <= 1e-4
