@@ -327,9 +327,11 @@ def main():
     nb = info["row_blocks"]
     cmp_ = max(info["compact_rows"], 0)
     cols16, rows16 = bool(cmp_ & 2), bool(cmp_ & 1)
-    fwd_kernel = ("k_fwd1cp" if info["fwd_kind"] == 1 and cols16 else
+    fwd_kernel = ("k_fwd1l" if info["fwd_kind"] == 2 else
+                  "k_fwd1cp" if info["fwd_kind"] == 1 and cols16 else
                   "k_fwd1c" if cols16 else "k_fwd1")
     fwd_name = f"{fwd_kernel} (forward + ratio, one launch" + (
+        ", x staged in LDS per 256 rows, 16-bit ranks)" if fwd_kernel == "k_fwd1l" else
         ", 16-bit columns)" if cols16 else ")")
     # bytes as stored: 16-bit indices move 2 bytes less per entry than the
     # 4-byte index of the algorithmic count (plus 8 bytes of split points per
@@ -350,7 +352,13 @@ def main():
     else:
         # algorithmic bytes: matrix (value + column index), row pointers, vectors
         bytes_f = 8 * info["padded_nnz"] + 8 * (m + 1) + 4 * m + 4 * m + 4 * n
-        if cols16:
+        if fwd_kernel == "k_fwd1l":
+            # 2-byte ranks; per union entry its column and x (4 + 4 bytes);
+            # union offsets per 256 rows; no x term (read through the unions)
+            nu = R.forward_union_entries()
+            stored_f = (bytes_f - 2 * info["padded_nnz"] - 4 * n + 8 * nu
+                        + 8 * (-(-m // 256) + 1))
+        elif cols16:
             stored_f = bytes_f - 2 * info["padded_nnz"] + 8 * m
     if info["layout"] == "tiled":
         pass  # set above

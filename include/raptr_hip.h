@@ -424,10 +424,11 @@ int raptr_recon_get_compact_rows(const raptr_recon* plan);
  *      of <= 258,048 voxels whose unions all hold <= 16,384 voxels (rows
  *      in an order that keeps neighbouring lines together, e.g.
  *      raptr_locality_order); the default there.
- * available: bit k set if kind k can run on this plan.  Null outputs are
- * skipped. */
+ * available: bit k set if kind k can run on this plan; union_entries: the
+ * column unions' total size (0: none).  Null outputs are skipped. */
 int raptr_recon_set_forward_kind(raptr_recon* plan, int kind);
-int raptr_recon_get_forward_kind(const raptr_recon* plan, int* kind, int* available);
+int raptr_recon_get_forward_kind(const raptr_recon* plan, int* kind, int* available,
+                                 int64_t* union_entries);
 /* Row blocks per backward launch of the blocked layout (1..64): a wave
  * walks its column through that many consecutive blocks, keeping the sum in
  * registers (longer waves, fewer launches). */

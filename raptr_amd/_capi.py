@@ -127,7 +127,7 @@ _SIGS = {
     "raptr_recon_set_compact_rows": (i32, [vp, i32]),
     "raptr_recon_get_compact_rows": (i32, [vp]),
     "raptr_recon_set_forward_kind": (i32, [vp, i32]),
-    "raptr_recon_get_forward_kind": (i32, [vp, P(i32), P(i32)]),
+    "raptr_recon_get_forward_kind": (i32, [vp, P(i32), P(i32), P(i64)]),
     "raptr_locality_order": (i32, [vp, vp, i32, i32, vp, vp]),
     "raptr_recon_layout": (i32, [vp]),
     "raptr_recon_set_blocks_per_launch": (i32, [vp, i32]),

@@ -3349,10 +3349,12 @@ int raptr_recon_set_forward_kind(raptr_recon* R, int kind) {
   return RAPTR_OK;
 }
 
-int raptr_recon_get_forward_kind(const raptr_recon* R, int* kind, int* available) {
+int raptr_recon_get_forward_kind(const raptr_recon* R, int* kind, int* available,
+                                 int64_t* union_entries) {
   if (!R) return fail(RAPTR_ERR_INVALID_ARG, "null plan");
   if (kind) *kind = R->fwd_kind;
   if (available) *available = 1 | (R->fcol16 ? 2 : 0) | (R->lcol ? 4 : 0);
+  if (union_entries) *union_entries = R->lcol ? R->lnu : 0;
   return RAPTR_OK;
 }
 

@@ -139,8 +139,14 @@ class Reconstructor:
     def forward_kind(self):
         """(kind, available kinds)."""
         k, a = C.c_int(), C.c_int()
-        capi.call("raptr_recon_get_forward_kind", self.handle, C.byref(k), C.byref(a))
+        capi.call("raptr_recon_get_forward_kind", self.handle, C.byref(k), C.byref(a), None)
         return k.value, [j for j in range(3) if a.value >> j & 1]
+
+    def forward_union_entries(self):
+        """Total size of the LDS-staged forward's column unions (0: none)."""
+        u = C.c_int64()
+        capi.call("raptr_recon_get_forward_kind", self.handle, None, None, C.byref(u))
+        return u.value
 
     def set_blocks_per_launch(self, blocks):
         capi.call("raptr_recon_set_blocks_per_launch", self.handle, int(blocks))
