@@ -78,6 +78,16 @@ MEASURED_TRAFFIC = {
     ("k_fwd1cp", 64, 1, "locality"): 2 * 5518387 * 1024 + 35241 * 1024,
     ("k_fwd1cp", 64, 2, "locality"): 2 * 5156511 * 1024 + 35241 * 1024,
     ("k_fwd1cp", 64, 4, "locality"): 2 * 5163556 * 1024 + 35241 * 1024,
+    # x staged in LDS per 256 plan rows (k_fwd1l)
+    ("k_fwd1l", 16, 1, "locality"): 2 * 6463811 * 1024 + 17826 * 1024,
+    ("k_fwd1l", 16, 2, "locality"): 2 * 6183189 * 1024 + 17832 * 1024,
+    ("k_fwd1l", 16, 4, "locality"): 2 * 5961202 * 1024 + 17810 * 1024,
+    ("k_fwd1l", 32, 1, "locality"): 2 * 5660076 * 1024 + 17644 * 1024,
+    ("k_fwd1l", 32, 2, "locality"): 2 * 5524521 * 1024 + 17679 * 1024,
+    ("k_fwd1l", 32, 4, "locality"): 2 * 6039795 * 1024 + 17649 * 1024,
+    ("k_fwd1l", 64, 1, "locality"): 2 * 5303629 * 1024 + 17622 * 1024,
+    ("k_fwd1l", 64, 2, "locality"): 2 * 5500546 * 1024 + 17622 * 1024,
+    ("k_fwd1l", 64, 4, "locality"): 2 * 5302338 * 1024 + 17622 * 1024,
     # backward (unroll, row blocks, blocks per launch, staged, row order);
     # average over the launches of one pass
     ("k_bwd", 4, 9, 1, 0, "list"): 2 * 734223 * 1024 + 1427 * 1024,
@@ -86,12 +96,12 @@ MEASURED_TRAFFIC = {
     # order, 32/16-bit rows); one pass (k_bwd1x; its reduce moves 0.04 GB more)
     ("k_bwd1x", 4, 9, "locality", "r32"): 2 * 6605601 * 1024 + 4775 * 1024,
     # 69 row blocks of 65,536 rows, 16-bit and 32-bit rows
-    ("k_bwd1x", 1, 69, "locality", "r16"): 2 * 5733034 * 1024 + 6807 * 1024,
-    ("k_bwd1x", 1, 69, "locality", "r32"): 2 * 6954772 * 1024 + 12515 * 1024,
-    ("k_bwd1x", 2, 69, "locality", "r16"): 2 * 5391567 * 1024 + 9850 * 1024,
-    ("k_bwd1x", 2, 69, "locality", "r32"): 2 * 6709416 * 1024 + 12085 * 1024,
-    ("k_bwd1x", 4, 69, "locality", "r16"): 2 * 5622596 * 1024 + 6761 * 1024,
-    ("k_bwd1x", 4, 69, "locality", "r32"): 2 * 6605579 * 1024 + 9487 * 1024,
+    ("k_bwd1x", 1, 69, "locality", "r16"): 2 * 5503292 * 1024 + 9544 * 1024,
+    ("k_bwd1x", 1, 69, "locality", "r32"): 2 * 6952794 * 1024 + 12440 * 1024,
+    ("k_bwd1x", 2, 69, "locality", "r16"): 2 * 5636490 * 1024 + 7285 * 1024,
+    ("k_bwd1x", 2, 69, "locality", "r32"): 2 * 6707921 * 1024 + 11976 * 1024,
+    ("k_bwd1x", 4, 69, "locality", "r16"): 2 * 5620918 * 1024 + 6455 * 1024,
+    ("k_bwd1x", 4, 69, "locality", "r32"): 2 * 6604423 * 1024 + 9368 * 1024,
 }
 
 
