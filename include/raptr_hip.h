@@ -397,7 +397,8 @@ int raptr_recon_set_config(raptr_recon* plan, int fwd_lanes, int fwd_unroll,
  * Column sums stay in LDS across blocks in a fixed order.  staged = 3 (any
  * block size): each XCD takes one eighth of the row blocks (workgroup i
  * runs on XCD i % 8), so its ratio slices stay in its own L2; per-XCD
- * partial column sums (8 x n floats) are added in a fixed order.
+ * partial column sums (8 x n floats) are added in a fixed order;
+ * lanes_per_column 16, 32 or 64 (8 runs as 64).
  * staged = 0: the block-by-block pass. */
 int raptr_recon_set_backward(raptr_recon* plan, int staged, int lanes_per_column,
                              int workgroups);

@@ -1567,19 +1567,20 @@ __device__ __forceinline__ vi4 rows4(const void* __restrict__ brow, int64_t k, i
   }
 }
 
-template <int VT, int UNROLL, bool R16>
+// LPC lanes per column (16, 32 or 64), 256 / LPC columns per workgroup.
+template <int VT, int UNROLL, bool R16, int LPC>
 __global__ __launch_bounds__(256) void k_bwd1x(int n, int nblk, int blk_rows,
                                                const int64_t* __restrict__ ptr,
                                                const void* __restrict__ brow,
                                                const typename Val<VT>::T* __restrict__ val,
                                                const float* __restrict__ e,
                                                float* __restrict__ c8) {
-  const int lane = threadIdx.x & (kWave - 1);
+  const int lane = threadIdx.x % LPC;
   const int xcd = int(blockIdx.x % 8);
-  const int j = int(blockIdx.x / 8) * (256 / kWave) + wave_id();
-  if (j >= n) return;
+  const int j = int(blockIdx.x / 8) * (256 / LPC) + int(threadIdx.x) / LPC;
+  if (j >= n) return;  // the column's LPC lanes leave together
   const int b0 = int(int64_t(nblk) * xcd / 8), b1 = int(int64_t(nblk) * (xcd + 1) / 8);
-  constexpr int64_t STEP = 4 * kWave;
+  constexpr int64_t STEP = 4 * LPC;
   float acc = 0.0f;
   for (int b = b0; b < b1; ++b) {
     const int64_t bb = ptr[int64_t(b) * (n + 1) + j], en = ptr[int64_t(b) * (n + 1) + j + 1];
@@ -1617,7 +1618,7 @@ __global__ __launch_bounds__(256) void k_bwd1x(int n, int nblk, int blk_rows,
     }
   }
 #pragma unroll
-  for (int off = kWave / 2; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, kWave);
+  for (int off = LPC / 2; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, LPC);
   if (lane == 0) c8[int64_t(xcd) * n + j] = acc;
 }
 
@@ -2409,25 +2410,32 @@ int launch_bwd_flat(const raptr_recon* R, const float* x, hipStream_t st, int* n
 int launch_bwd1_xcd(const raptr_recon* R, const float* x, hipStream_t st, int* nb_out) {
   if (!R->c8 || !R->bptr) return fail(RAPTR_ERR_INVALID_ARG, "plan has no per-XCD partial sums");
   const int n = R->n;
-  const unsigned gg = unsigned(8 * ((int64_t(n) + 3) / 4));
+  // lanes per column: 64 unless set to 16 or 32 (set_backward)
+  const int L = R->bwd_lpc == 16 || R->bwd_lpc == 32 ? R->bwd_lpc : 64;
+  const unsigned gg = unsigned(8 * ((int64_t(n) + (256 / L) - 1) / (256 / L)));
   const int U = R->bwd_unroll;
   const bool r16 = R->brow16 && R->bwd_r16;
-#define RAPTR_B1X(VV, UU)                                                                   \
-  if (R->vt == VV && U == UU) {                                                             \
+#define RAPTR_B1X(VV, UU, LL)                                                               \
+  if (R->vt == VV && U == UU && L == LL) {                                                  \
     if (r16)                                                                                \
-      hipLaunchKernelGGL((k_bwd1x<VV, UU, true>), dim3(gg), dim3(256), 0, st, n, R->nblk,    \
-                         R->blk_rows, (const int64_t*)R->bptr, (const void*)R->brow16,      \
-                         (const typename Val<VV>::T*)R->bval, (const float*)R->e, R->c8);   \
+      hipLaunchKernelGGL((k_bwd1x<VV, UU, true, LL>), dim3(gg), dim3(256), 0, st, n,         \
+                         R->nblk, R->blk_rows, (const int64_t*)R->bptr,                     \
+                         (const void*)R->brow16, (const typename Val<VV>::T*)R->bval,       \
+                         (const float*)R->e, R->c8);                                        \
     else                                                                                    \
-      hipLaunchKernelGGL((k_bwd1x<VV, UU, false>), dim3(gg), dim3(256), 0, st, n, R->nblk,   \
-                         R->blk_rows, (const int64_t*)R->bptr, (const void*)R->brow,        \
-                         (const typename Val<VV>::T*)R->bval, (const float*)R->e, R->c8);   \
+      hipLaunchKernelGGL((k_bwd1x<VV, UU, false, LL>), dim3(gg), dim3(256), 0, st, n,        \
+                         R->nblk, R->blk_rows, (const int64_t*)R->bptr,                     \
+                         (const void*)R->brow, (const typename Val<VV>::T*)R->bval,         \
+                         (const float*)R->e, R->c8);                                        \
     RAPTR_LAUNCH_CHECK();                                                                   \
   } else
-  RAPTR_B1X(kF32, 1) RAPTR_B1X(kF32, 2) RAPTR_B1X(kF32, 4) RAPTR_B1X(kF16, 1) RAPTR_B1X(kF16, 2)
-  RAPTR_B1X(kF16, 4) RAPTR_B1X(kBF16, 1) RAPTR_B1X(kBF16, 2) RAPTR_B1X(kBF16, 4)
-  RAPTR_B1X(kU16, 1) RAPTR_B1X(kU16, 2) RAPTR_B1X(kU16, 4)
-  return fail(RAPTR_ERR_INVALID_ARG, "per-XCD backward unroll %d unsupported", U);
+#define RAPTR_B1XV(VV)                                                                      \
+  RAPTR_B1X(VV, 1, 16) RAPTR_B1X(VV, 2, 16) RAPTR_B1X(VV, 4, 16) RAPTR_B1X(VV, 1, 32)       \
+  RAPTR_B1X(VV, 2, 32) RAPTR_B1X(VV, 4, 32) RAPTR_B1X(VV, 1, 64) RAPTR_B1X(VV, 2, 64)       \
+  RAPTR_B1X(VV, 4, 64)
+  RAPTR_B1XV(kF32) RAPTR_B1XV(kF16) RAPTR_B1XV(kBF16) RAPTR_B1XV(kU16)
+  return fail(RAPTR_ERR_INVALID_ARG, "per-XCD backward unroll %d lanes %d unsupported", U, L);
+#undef RAPTR_B1XV
 #undef RAPTR_B1X
   const unsigned nr = unsigned((int64_t(n) + 255) / 256);
   hipLaunchKernelGGL(k_bwd1_xreduce, dim3(nr), dim3(256), 0, st, n, (const float*)R->c8, x,
@@ -3297,8 +3305,10 @@ int raptr_recon_set_backward(raptr_recon* R, int staged, int lanes_per_column, i
     return fail(RAPTR_ERR_INVALID_ARG,
                 "staged backward needs a single-frame plan with block_rows <= %d, "
                 "a multiple of 4", kLdsRowsMax);
-  if (lanes_per_column != 8 && lanes_per_column != 16 && lanes_per_column != 32)
-    return fail(RAPTR_ERR_INVALID_ARG, "lanes per column must be 8, 16 or 32");
+  if (lanes_per_column != 8 && lanes_per_column != 16 && lanes_per_column != 32 &&
+      !(staged == 3 && lanes_per_column == 64))
+    return fail(RAPTR_ERR_INVALID_ARG,
+                "lanes per column must be 8, 16 or 32 (or 64 for the per-XCD backward)");
   if (workgroups < 1) return fail(RAPTR_ERR_INVALID_ARG, "workgroups must be positive");
   if (R->bwd_lds != staged || R->bwd_lpc != lanes_per_column || R->bwd_wgs != workgroups) {
     for (auto& kv : R->graphs) {

@@ -279,7 +279,7 @@ class Reconstructor:
                         for u in (2, 4)]
             bshapes += [(2, 16, 512, u, 1) for u in (1, 2, 4)]
         if xcd_ok:  # per-XCD row ranges (any block size)
-            bshapes += [(3, 16, 512, u, 1) for u in (1, 2, 4)]
+            bshapes += [(3, lpc, 512, u, 1) for lpc in (16, 32, 64) for u in (1, 2, 4)]
         staged_ok = staged_ok or xcd_ok
         best = None
         for st, lpc, wgs, u, bpl in bshapes:

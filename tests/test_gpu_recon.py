@@ -100,9 +100,10 @@ def test_recon_staged_backward(gpu, oracle, small_system, lpc, wgs, unroll):
 @pytest.mark.parametrize("block_rows", [0, 800, 5000])
 def test_recon_backward_per_xcd_row_ranges(gpu, oracle, small_system, block_rows):
     """Single-frame backward mode 3 (each XCD takes an eighth of the row
-    blocks, per-XCD partial column sums added in a fixed order), with fewer
-    blocks than XCDs (block_rows 0 and 5000) and many (800): against the FP64
-    oracle, the block-by-block backward (1e-6) and itself (bit for bit)."""
+    blocks, per-XCD partial column sums added in a fixed order; 16, 32 or 64
+    lanes per column), with fewer blocks than XCDs (block_rows 0 and 5000)
+    and many (800): against the FP64 oracle, the block-by-block backward
+    (1e-6) and itself (bit for bit)."""
     from raptr_amd import RaptrError
     from raptr_amd.recon import Reconstructor
     S = small_system
@@ -115,9 +116,9 @@ def test_recon_backward_per_xcd_row_ranges(gpu, oracle, small_system, block_rows
     b = R.mlem(S["y"], x0, 5)
     from raptr_amd.device import Stream
     st = Stream()
-    for unroll in (1, 2, 4):
+    for unroll, lpc in ((1, 64), (2, 64), (4, 64), (1, 16), (2, 32), (4, 16)):
         R.set_config(16, 1, unroll)
-        R.set_backward(3)
+        R.set_backward(3, lpc)
         a1 = R.mlem(S["y"], x0, 5, use_graph=False)
         for _ in range(2):  # graph replays (eager work between them)
             a2 = R.mlem(S["y"], x0, 5, use_graph=True, stream=st.handle)
