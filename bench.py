@@ -93,15 +93,28 @@ MEASURED_TRAFFIC = {
     ("k_bwd", 4, 9, 1, 0, "list"): 2 * 734223 * 1024 + 1427 * 1024,
     ("k_bwd", 4, 9, 4, 0, "locality"): 2 * 2562847 * 1024 + 1613 * 1024,
     # single-frame backward with per-XCD row ranges (unroll, row blocks, row
-    # order, 32/16-bit rows); one pass (k_bwd1x; its reduce moves 0.04 GB more)
-    ("k_bwd1x", 4, 9, "locality", "r32"): 2 * 6605601 * 1024 + 4775 * 1024,
-    # 69 row blocks of 65,536 rows, 16-bit and 32-bit rows
-    ("k_bwd1x", 1, 69, "locality", "r16"): 2 * 5503292 * 1024 + 9544 * 1024,
-    ("k_bwd1x", 1, 69, "locality", "r32"): 2 * 6952794 * 1024 + 12440 * 1024,
-    ("k_bwd1x", 2, 69, "locality", "r16"): 2 * 5636490 * 1024 + 7285 * 1024,
-    ("k_bwd1x", 2, 69, "locality", "r32"): 2 * 6707921 * 1024 + 11976 * 1024,
-    ("k_bwd1x", 4, 69, "locality", "r16"): 2 * 5620918 * 1024 + 6455 * 1024,
-    ("k_bwd1x", 4, 69, "locality", "r32"): 2 * 6604423 * 1024 + 9368 * 1024,
+    # order, 32/16-bit rows, lanes per column); one pass (k_bwd1x; its reduce
+    # moves 0.04 GB more)
+    ("k_bwd1x", 4, 9, "locality", "r32", 64): 2 * 6605601 * 1024 + 4775 * 1024,
+    # 69 row blocks of 65,536 rows
+    ("k_bwd1x", 1, 69, "locality", "r16", 16): 2 * 7317888 * 1024 + 9221 * 1024,
+    ("k_bwd1x", 1, 69, "locality", "r16", 32): 2 * 6109372 * 1024 + 10973 * 1024,
+    ("k_bwd1x", 1, 69, "locality", "r16", 64): 2 * 5560842 * 1024 + 6292 * 1024,
+    ("k_bwd1x", 1, 69, "locality", "r32", 16): 2 * 8709134 * 1024 + 10990 * 1024,
+    ("k_bwd1x", 1, 69, "locality", "r32", 32): 2 * 7534303 * 1024 + 13905 * 1024,
+    ("k_bwd1x", 1, 69, "locality", "r32", 64): 2 * 6944555 * 1024 + 12157 * 1024,
+    ("k_bwd1x", 2, 69, "locality", "r16", 16): 2 * 6657187 * 1024 + 9759 * 1024,
+    ("k_bwd1x", 2, 69, "locality", "r16", 32): 2 * 5656951 * 1024 + 12299 * 1024,
+    ("k_bwd1x", 2, 69, "locality", "r16", 64): 2 * 5542259 * 1024 + 6110 * 1024,
+    ("k_bwd1x", 2, 69, "locality", "r32", 16): 2 * 7591338 * 1024 + 12140 * 1024,
+    ("k_bwd1x", 2, 69, "locality", "r32", 32): 2 * 7006851 * 1024 + 13763 * 1024,
+    ("k_bwd1x", 2, 69, "locality", "r32", 64): 2 * 6702576 * 1024 + 11656 * 1024,
+    ("k_bwd1x", 4, 69, "locality", "r16", 16): 2 * 6399207 * 1024 + 10333 * 1024,
+    ("k_bwd1x", 4, 69, "locality", "r16", 32): 2 * 5913324 * 1024 + 10769 * 1024,
+    ("k_bwd1x", 4, 69, "locality", "r16", 64): 2 * 5549076 * 1024 + 6098 * 1024,
+    ("k_bwd1x", 4, 69, "locality", "r32", 16): 2 * 7214243 * 1024 + 10933 * 1024,
+    ("k_bwd1x", 4, 69, "locality", "r32", 32): 2 * 6826134 * 1024 + 11861 * 1024,
+    ("k_bwd1x", 4, 69, "locality", "r32", 64): 2 * 6625995 * 1024 + 8958 * 1024,
 }
 
 
@@ -416,8 +429,9 @@ def main():
         key_b = ("k_bwd", cfg_p["bwd_unroll"], nb, cfg_p["bwd_blocks_per_launch"],
                  cfg_p["bwd_staged"], cfg_p["row_order"])
         if info["bwd_staged"] == 3:
+            lpc = cfg_p["bwd_lanes_per_column"]
             key_b = ("k_bwd1x", cfg_p["bwd_unroll"], nb, cfg_p["row_order"],
-                     "r16" if rows16 else "r32")
+                     "r16" if rows16 else "r32", lpc if lpc in (16, 32) else 64)
     dom = {"kernel": fwd_name, "ms": ms_f, "bytes": bytes_f, "gbs": gbs_f, "launches": nl_f,
            "key": key_f, "stored": stored_f}
     if ms_b > ms_f:
