@@ -2702,7 +2702,8 @@ int build_local_columns(raptr_recon* R, hipStream_t st) {
   if (le != hipSuccess) return fail(RAPTR_ERR_DEVICE, "column unions: %s", hipGetErrorString(le));
   const int64_t mx = hc.empty() ? 0 : *std::max_element(hc.begin(), hc.end());
   if (mx > kLocMaxUnion) {  // too spread out for LDS: the plan keeps no unions
-    R->lup = nullptr;
+    rfree(R, R->lup);
+    rfree(R, cnt);
     return RAPTR_OK;
   }
   if ((rc = ralloc(R, &R->lucol, size_t(h) + 1)) || (rc = ralloc(R, &R->lcol, size_t(R->pnnz) + 8)))
@@ -2713,6 +2714,7 @@ int build_local_columns(raptr_recon* R, hipStream_t st) {
   le = hipGetLastError();
   if (le == hipSuccess) le = hipStreamSynchronize(st);
   if (le != hipSuccess) return fail(RAPTR_ERR_DEVICE, "column unions: %s", hipGetErrorString(le));
+  rfree(R, cnt);
   R->lmax = int(mx);
   R->lnu = h;
   return RAPTR_OK;
@@ -3061,9 +3063,10 @@ int create_plan(raptr_recon** out, int m, int n, int64_t nnz, const int64_t* d_r
     if (le == hipSuccess) le = hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, st);
     if (le == hipSuccess) le = hipStreamSynchronize(st);
     if (le != hipSuccess) return bail(fail(RAPTR_ERR_DEVICE, "cols16: %s", hipGetErrorString(le)));
-    if (hbad) {
-      R->fcol16 = nullptr;  // rows too long: the plan keeps 32-bit columns only
-      R->fsplit = nullptr;
+    rfree(R, bad);
+    if (hbad) {  // the plan keeps 32-bit columns only
+      rfree(R, R->fcol16);
+      rfree(R, R->fsplit);
     } else {
       R->fwd_c16 = 1;
     }

@@ -161,6 +161,7 @@ struct raptr_recon {
   float* yp = nullptr;
   size_t bytes = 0;
   std::vector<void*> allocs;
+  std::vector<size_t> asizes;  // bytes of allocs[i]
   std::map<long long, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
 };
 
@@ -177,9 +178,25 @@ int ralloc(raptr_recon* R, T** p, size_t count) {
   }
   RAPTR_HIP_TRY(e);
   R->allocs.push_back(q);
+  R->asizes.push_back(count * sizeof(T));
   R->bytes += count * sizeof(T);
   *p = static_cast<T*>(q);
   return RAPTR_OK;
+}
+
+// frees a buffer of the plan before the plan ends (a format the build
+// rejected); the device must be done with it
+template <class T>
+void rfree(raptr_recon* R, T*& p) {
+  for (size_t i = 0; p && i < R->allocs.size(); ++i) {
+    if (R->allocs[i] != static_cast<void*>(p)) continue;
+    (void)hipFree(R->allocs[i]);
+    R->bytes -= R->asizes[i];
+    R->allocs.erase(R->allocs.begin() + i);
+    R->asizes.erase(R->asizes.begin() + i);
+    break;
+  }
+  p = nullptr;
 }
 
 inline int vbytes(int vt) { return vt == kF32 ? 4 : 2; }
