@@ -1582,8 +1582,16 @@ __global__ __launch_bounds__(256) void k_bwd1x(int n, int nblk, int blk_rows,
   const int b0 = int(int64_t(nblk) * xcd / 8), b1 = int(int64_t(nblk) * (xcd + 1) / 8);
   constexpr int64_t STEP = 4 * LPC;
   float acc = 0.0f;
+  // the segment bounds of up to LPC/2 blocks in one load (lane 2t, 2t+1:
+  // block bc + t), handed to the block loop by shuffles
+  int64_t pv = 0;
   for (int b = b0; b < b1; ++b) {
-    const int64_t bb = ptr[int64_t(b) * (n + 1) + j], en = ptr[int64_t(b) * (n + 1) + j + 1];
+    const int t = (b - b0) % (LPC / 2);
+    if (t == 0) {
+      const int nbc = b1 - b < LPC / 2 ? b1 - b : LPC / 2;
+      pv = lane < 2 * nbc ? ptr[int64_t(b + lane / 2) * (n + 1) + j + (lane & 1)] : 0;
+    }
+    const int64_t bb = __shfl(pv, 2 * t, LPC), en = __shfl(pv, 2 * t + 1, LPC);
     const int r0 = b * blk_rows;
     int64_t k = bb + 4 * lane;
     for (; k + STEP * (UNROLL - 1) < en; k += STEP * UNROLL) {
