@@ -247,7 +247,8 @@ def main():
         # sizes (the first batched mode picks, the others reuse its choice)
         # one frame: both plan layouts (column tiles, row blocks) are tried
         if nf == 1:
-            rows_try = [("tiled", 0), ("blocked", 0), ("blocked", 65536), ("blocked", 12288)]
+            # (block_rows 0: the plan's default, 65,536 rows for one frame)
+            rows_try = [("tiled", 0), ("blocked", 0), ("blocked", 524288), ("blocked", 12288)]
         elif best_rows is not None:
             rows_try = [("blocked", best_rows)]
         else:

@@ -297,8 +297,9 @@ int raptr_mlem_partial_slots(int n);
  * (64-bit row pointers, ascending columns) and the FP64 sensitivity.  The
  * plan owns two derived layouts: a row-padded CSR (16-byte aligned rows,
  * vector loads) for the forward pass and the transpose cut into row blocks
- * of block_rows channels (0: sized so the block's ratio slice fits one
- * XCD's L2) for the atomic-free backward pass.  value_type: 0 = FP32,
+ * of block_rows channels (0: 65,536 for single-frame row-blocked plans,
+ * else sized so the block's 2 MB ratio slice fits one XCD's L2) for the
+ * atomic-free backward pass.  value_type: 0 = FP32,
  * 1 = FP16, 2 = BF16 (of value/row maximum), 3 = UNORM16 (fixed point
  * round(65535*value/row maximum)); 16-bit types carry a per-row FP32
  * scale and accumulate in FP32.  max_frames: batch capacity (power of two <= 64).  The

@@ -2801,8 +2801,14 @@ int create_plan(raptr_recon** out, int m, int n, int64_t nnz, const int64_t* d_r
     raptr_recon_destroy(R);
     return rc;
   };
-  // row block: ratio slice of <= 2 MiB per launch (one XCD's L2 holds it)
-  if (block_rows <= 0) block_rows = std::max(1, (2 << 20) / (4 * max_frames));
+  // row block: ratio slice of <= 2 MiB per launch (one XCD's L2 holds it);
+  // single-frame row-blocked plans: 65,536 rows (256 KB slices, 16-bit
+  // transpose rows; the per-XCD backward keeps an eighth of the blocks in
+  // each XCD's L2)
+  if (block_rows <= 0)
+    block_rows = max_frames == 1 && layout == RAPTR_LAYOUT_BLOCKED
+                     ? 65536
+                     : std::max(1, (2 << 20) / (4 * max_frames));
   R->blk_rows = block_rows;
   R->nblk = std::max(1, int((int64_t(m) + block_rows - 1) / block_rows));
   int rc;

@@ -131,10 +131,9 @@ def test_recon_backward_per_xcd_row_ranges(gpu, oracle, small_system, block_rows
             a3 = R.mlem(S["y"], x0, 5, use_graph=False)
             R.set_compact_rows(True)
             np.testing.assert_array_equal(a3, a1)
-    # 16-bit rows exist for blocks of <= 65,536 rows (the default block is
-    # 2 MB of ratios: 524,288 rows); 16-bit forward columns for <= 262,144
-    # voxels (32^3 here)
-    assert R.info()["compact_rows"] == ((1 if 0 < block_rows <= 65536 else 0) | 2)
+    # 16-bit rows exist for blocks of <= 65,536 rows (the single-frame
+    # default); 16-bit forward columns for <= 262,144 voxels (32^3 here)
+    assert R.info()["compact_rows"] == ((1 if block_rows <= 65536 else 0) | 2)
     with pytest.raises(RaptrError):
         Reconstructor(S["A"], S["sens"], "f32", 2).set_backward(3)
 
