@@ -391,7 +391,7 @@ int raptr_recon_set_config(raptr_recon* plan, int fwd_lanes, int fwd_unroll,
  * and a multiple of 4).  staged = 1: each of `workgroups` workgroups owns a
  * contiguous column range, loads each row block's ratio slice into LDS and
  * walks its columns with lanes_per_column (8, 16, 32) lanes per column.
- * staged = 2 (flat; the default of such plans): workgroups own 256-column
+ * staged = 2 (flat): workgroups own 256-column
  * tiles and stream each block's entries of a wave's 64 columns as one range
  * (packed column-in-tile/row-in-block indices), adding runs of equal column
  * with a wave segmented scan; lanes_per_column and workgroups are unused.
@@ -399,7 +399,9 @@ int raptr_recon_set_config(raptr_recon* plan, int fwd_lanes, int fwd_unroll,
  * block size): each XCD takes one eighth of the row blocks (workgroup i
  * runs on XCD i % 8), so its ratio slices stay in its own L2; per-XCD
  * partial column sums (8 x n floats) are added in a fixed order;
- * lanes_per_column 16, 32 or 64 (8 runs as 64).
+ * lanes_per_column 16, 32 or 64 (8 runs as 64); the default of
+ * single-frame row-blocked plans with >= 8 row blocks (32 lanes, unroll
+ * 2; with fewer blocks some XCDs would idle).
  * staged = 0: the block-by-block pass. */
 int raptr_recon_set_backward(raptr_recon* plan, int staged, int lanes_per_column,
                              int workgroups);

@@ -3090,7 +3090,19 @@ int create_plan(raptr_recon** out, int m, int n, int64_t nnz, const int64_t* d_r
   if (max_frames == 1 && layout == RAPTR_LAYOUT_BLOCKED && R->fcol && n <= 8064 * 32 && m > 0 &&
       R->pnnz > 0) {
     if ((rc = build_local_columns(R, st))) return bail(rc);
-    if (R->lcol) R->fwd_kind = 2;
+    if (R->lcol) {
+      R->fwd_kind = 2;
+      R->fwd_lanes = 16;  // the fastest shape of the full-size workload
+      R->fwd_unroll = 4;
+    }
+  }
+  // single-frame row-blocked plans of >= 8 row blocks (one or more per
+  // XCD): the per-XCD backward (32 lanes per column, 2 steps in flight) is
+  // the default
+  if (R->c8 && max_frames == 1 && layout == RAPTR_LAYOUT_BLOCKED && R->nblk >= 8) {
+    R->bwd_lds = 3;
+    R->bwd_lpc = 32;
+    R->bwd_unroll = 2;
   }
   *out = R;
   return RAPTR_OK;

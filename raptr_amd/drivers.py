@@ -64,7 +64,9 @@ def reco(measurement, output, setup, grid, nrays, niter, sensitivity=None, guess
             raise ValueError(f"guess {guess} has {x0.size} voxels, grid {grid.nvox}")
     else:
         x0 = np.full(grid.nvox, 1.0 / grid.nvox, np.float32)
-    R = Reconstructor(A, sens, value_type, 1, stream=st.handle)
+    # plan rows in locality order (the single-frame forward then stages x in
+    # LDS); the images do not depend on it beyond FP32 summation order
+    R = Reconstructor(A, sens, value_type, 1, stream=st.handle, row_order=setup)
     dy = DeviceArray.from_numpy(counts.astype(np.float32), stream=st.handle)
     dx = DeviceArray.from_numpy(x0, stream=st.handle)
     head, tail = os.path.split(output)

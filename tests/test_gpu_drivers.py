@@ -33,7 +33,7 @@ def test_reco_driver_equals_library(tiny_files):
     out = str(d / "out.h5")
     x = drivers.reco(fn, out, setup, grid, nrays, 4, every_iteration=False)
     A = sysmat.build_csr(setup, grid, meas, nrays)
-    R = Reconstructor(A, sysmat.sensitivity(setup, grid, nrays), "f32", 1)
+    R = Reconstructor(A, sysmat.sensitivity(setup, grid, nrays), "f32", 1, row_order=setup)
     want = R.mlem(y, np.full(grid.nvox, 1.0 / grid.nvox, np.float32), 4, use_graph=False)
     np.testing.assert_array_equal(x, want)
     back, g = h5io.read_density(out)
