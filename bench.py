@@ -79,15 +79,15 @@ MEASURED_TRAFFIC = {
     ("k_fwd1cp", 64, 2, "locality"): 2 * 5156511 * 1024 + 35241 * 1024,
     ("k_fwd1cp", 64, 4, "locality"): 2 * 5163556 * 1024 + 35241 * 1024,
     # x staged in LDS per 256 plan rows (k_fwd1l)
-    ("k_fwd1l", 16, 1, "locality"): 2 * 6463811 * 1024 + 17826 * 1024,
-    ("k_fwd1l", 16, 2, "locality"): 2 * 6183189 * 1024 + 17832 * 1024,
-    ("k_fwd1l", 16, 4, "locality"): 2 * 5961202 * 1024 + 17810 * 1024,
-    ("k_fwd1l", 32, 1, "locality"): 2 * 5660076 * 1024 + 17644 * 1024,
-    ("k_fwd1l", 32, 2, "locality"): 2 * 5524521 * 1024 + 17679 * 1024,
-    ("k_fwd1l", 32, 4, "locality"): 2 * 6039795 * 1024 + 17649 * 1024,
-    ("k_fwd1l", 64, 1, "locality"): 2 * 5303629 * 1024 + 17622 * 1024,
-    ("k_fwd1l", 64, 2, "locality"): 2 * 5500546 * 1024 + 17622 * 1024,
-    ("k_fwd1l", 64, 4, "locality"): 2 * 5302338 * 1024 + 17622 * 1024,
+    ("k_fwd1l", 16, 1, "locality"): 2 * 5017666 * 1024 + 25641 * 1024,
+    ("k_fwd1l", 16, 2, "locality"): 2 * 4997635 * 1024 + 27475 * 1024,
+    ("k_fwd1l", 16, 4, "locality"): 2 * 5010383 * 1024 + 28855 * 1024,
+    ("k_fwd1l", 32, 1, "locality"): 2 * 4960777 * 1024 + 23186 * 1024,
+    ("k_fwd1l", 32, 2, "locality"): 2 * 4975836 * 1024 + 23456 * 1024,
+    ("k_fwd1l", 32, 4, "locality"): 2 * 4970694 * 1024 + 24794 * 1024,
+    ("k_fwd1l", 64, 1, "locality"): 2 * 4955786 * 1024 + 18578 * 1024,
+    ("k_fwd1l", 64, 2, "locality"): 2 * 4960665 * 1024 + 18593 * 1024,
+    ("k_fwd1l", 64, 4, "locality"): 2 * 4953147 * 1024 + 18560 * 1024,
     # backward (unroll, row blocks, blocks per launch, staged, row order);
     # average over the launches of one pass
     ("k_bwd", 4, 9, 1, 0, "list"): 2 * 734223 * 1024 + 1427 * 1024,

@@ -448,7 +448,10 @@ __global__ __launch_bounds__(256) void k_fwd1cp(int64_t m, const int64_t* __rest
 // locality row order), copies x over the sorted union of their columns into
 // LDS, then runs k_fwd1c's loop with each entry's 16-bit index into that
 // union.  The matrix stream stays 6 bytes per FP32 entry; the gathers read
-// LDS instead of the L1 cache, whose tag lookups bound k_fwd1/k_fwd1c.
+// LDS instead of the L1 cache, whose tag lookups bound k_fwd1/k_fwd1c.  The
+// stream uses ordinary cached loads: rows share the cache lines at their
+// boundaries, which non-temporal loads fetched twice (12.2 -> 10.3 GB of
+// HBM reads per pass at full size; x lives in LDS, so nothing is evicted).
 constexpr int kLocRows = 256;
 constexpr int kLocMaxUnion = 16384;  // 64 KB of LDS per workgroup
 
@@ -479,8 +482,8 @@ __global__ __launch_bounds__(256) void k_fwd1l(int64_t m, const int64_t* __restr
       float v[U][4];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        c[u] = __builtin_nontemporal_load(reinterpret_cast<const vus4*>(lcol + k + 4 * LANES * u));
-        load4<VT>(val, k + 4 * LANES * u, v[u][0], v[u][1], v[u][2], v[u][3]);
+        c[u] = *reinterpret_cast<const vus4*>(lcol + k + 4 * LANES * u);
+        load4c<VT>(val, k + 4 * LANES * u, v[u][0], v[u][1], v[u][2], v[u][3]);
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
@@ -491,9 +494,9 @@ __global__ __launch_bounds__(256) void k_fwd1l(int64_t m, const int64_t* __restr
       }
     }
     for (; k < en; k += int64_t(4) * LANES) {
-      const vus4 c = __builtin_nontemporal_load(reinterpret_cast<const vus4*>(lcol + k));
+      const vus4 c = *reinterpret_cast<const vus4*>(lcol + k);
       float v0, v1, v2, v3;
-      load4<VT>(val, k, v0, v1, v2, v3);
+      load4c<VT>(val, k, v0, v1, v2, v3);
       acc = fmaf(v0, xs[c.x], acc);
       acc = fmaf(v1, xs[c.y], acc);
       acc = fmaf(v2, xs[c.z], acc);
@@ -1556,6 +1559,7 @@ __global__ __launch_bounds__(256) void k_bwd_xreduce(int n, const float* __restr
 // the XCD's partial column sum to c8[x][j]; k_bwd1_xreduce adds the 8 in a
 // fixed order and applies the update.
 // the rows of 4 stored entries: 32-bit rows, or 16-bit offsets in the block
+// (non-temporal: the stream must not evict the ratio slices from L2)
 template <bool R16>
 __device__ __forceinline__ vi4 rows4(const void* __restrict__ brow, int64_t k, int r0) {
   if constexpr (R16) {

@@ -77,6 +77,19 @@ __device__ __forceinline__ void load4(const typename Val<VT>::T* __restrict__ va
     v2 = h2f(uint16_t(h.y & 0xffff), VT), v3 = h2f(uint16_t(h.y >> 16), VT);
   }
 }
+// as load4 with ordinary (cached) loads
+template <int VT>
+__device__ __forceinline__ void load4c(const typename Val<VT>::T* __restrict__ val, int64_t k,
+                                       float& v0, float& v1, float& v2, float& v3) {
+  if (VT == kF32) {
+    const vf4 v = *reinterpret_cast<const vf4*>(reinterpret_cast<const float*>(val) + k);
+    v0 = v.x, v1 = v.y, v2 = v.z, v3 = v.w;
+  } else {
+    const vu2 h = *reinterpret_cast<const vu2*>(reinterpret_cast<const uint16_t*>(val) + k);
+    v0 = h2f(uint16_t(h.x & 0xffff), VT), v1 = h2f(uint16_t(h.x >> 16), VT);
+    v2 = h2f(uint16_t(h.y & 0xffff), VT), v3 = h2f(uint16_t(h.y >> 16), VT);
+  }
+}
 }  // namespace raptr
 
 struct raptr_recon {
