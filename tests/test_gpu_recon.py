@@ -689,3 +689,41 @@ def test_recon_forward_lds_staged_small_system(gpu, oracle, small_system):
     b = R.mlem(S["y"], x0, 5)
     np.testing.assert_array_equal(a, b)
     assert _rel(a, _oracle_mlem(oracle, S, S["y"], 5)) <= 1e-4
+
+
+@pytest.mark.parametrize("case", ["empty_group", "one_row"])
+def test_recon_forward_lds_staged_edges(gpu, case):
+    """LDS-staged forward edge cases: a whole 256-row group with no entries
+    and a ragged last group (m = 785), and a one-row plan: the same ratios,
+    bit for bit, as the per-entry forward, and the FP64 host product."""
+    from raptr_amd import _capi as capi
+    from raptr_amd.device import DeviceArray
+    from raptr_amd.recon import Reconstructor
+    n = 5000
+    rng = np.random.default_rng(11)
+    m = 785 if case == "empty_group" else 1
+    lens = rng.integers(1, 300, m)
+    if case == "empty_group":
+        lens[256:512] = 0
+    cols = [np.sort(rng.choice(n, L, replace=False)) for L in lens]
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    col = np.concatenate(cols).astype(np.int32)
+    val = rng.random(len(col)).astype(np.float32) + 0.01
+    R = Reconstructor(_device_csr(rp, col, val, n), DeviceArray.from_numpy(np.ones(n)), "f32", 1,
+                      layout="blocked")
+    assert R.forward_kind()[0] == 2
+    x = rng.random(n).astype(np.float32)
+    y = rng.random(m).astype(np.float32) + 0.5
+    dx, dy, de = DeviceArray.from_numpy(x), DeviceArray.from_numpy(y), DeviceArray(m, np.float32)
+    out = []
+    for k in (2, 0):
+        R.set_forward_kind(k)
+        capi.call("raptr_recon_forward", R.handle, 1, dx.ptr, dy.ptr, de.ptr, None)
+        out.append(de.numpy())
+    np.testing.assert_array_equal(out[0], out[1])
+    yt = np.array([np.dot(val[rp[i]:rp[i + 1]].astype(np.float64), x[col[rp[i]:rp[i + 1]]])
+                   for i in range(m)])
+    np.testing.assert_allclose(out[0], np.where(yt != 0, y / np.where(yt != 0, yt, 1), 0),
+                               rtol=1e-5)
+    if case == "empty_group":
+        assert np.all(out[0][256:512] == 0)
