@@ -78,16 +78,16 @@ MEASURED_TRAFFIC = {
     ("k_fwd1cp", 64, 1, "locality"): 2 * 5518387 * 1024 + 35241 * 1024,
     ("k_fwd1cp", 64, 2, "locality"): 2 * 5156511 * 1024 + 35241 * 1024,
     ("k_fwd1cp", 64, 4, "locality"): 2 * 5163556 * 1024 + 35241 * 1024,
-    # x staged in LDS per 256 plan rows (k_fwd1l)
-    ("k_fwd1l", 16, 1, "locality"): 2 * 5017666 * 1024 + 25641 * 1024,
-    ("k_fwd1l", 16, 2, "locality"): 2 * 4997635 * 1024 + 27475 * 1024,
-    ("k_fwd1l", 16, 4, "locality"): 2 * 5010383 * 1024 + 28855 * 1024,
-    ("k_fwd1l", 32, 1, "locality"): 2 * 4960777 * 1024 + 23186 * 1024,
-    ("k_fwd1l", 32, 2, "locality"): 2 * 4975836 * 1024 + 23456 * 1024,
-    ("k_fwd1l", 32, 4, "locality"): 2 * 4970694 * 1024 + 24794 * 1024,
-    ("k_fwd1l", 64, 1, "locality"): 2 * 4955786 * 1024 + 18578 * 1024,
-    ("k_fwd1l", 64, 2, "locality"): 2 * 4960665 * 1024 + 18593 * 1024,
-    ("k_fwd1l", 64, 4, "locality"): 2 * 4953147 * 1024 + 18560 * 1024,
+    # x staged in LDS per 64 plan rows (k_fwd1l, cached stream loads)
+    ("k_fwd1l", 16, 1, "locality"): 2 * 5381663 * 1024 + 21150 * 1024,
+    ("k_fwd1l", 16, 2, "locality"): 2 * 5212891 * 1024 + 23302 * 1024,
+    ("k_fwd1l", 16, 4, "locality"): 2 * 5194149 * 1024 + 26996 * 1024,
+    ("k_fwd1l", 32, 1, "locality"): 2 * 5134107 * 1024 + 34597 * 1024,
+    ("k_fwd1l", 32, 2, "locality"): 2 * 5132328 * 1024 + 33701 * 1024,
+    ("k_fwd1l", 32, 4, "locality"): 2 * 5154610 * 1024 + 35567 * 1024,
+    ("k_fwd1l", 64, 1, "locality"): 2 * 5120169 * 1024 + 28999 * 1024,
+    ("k_fwd1l", 64, 2, "locality"): 2 * 5122436 * 1024 + 27835 * 1024,
+    ("k_fwd1l", 64, 4, "locality"): 2 * 5118499 * 1024 + 29949 * 1024,
     # backward (unroll, row blocks, blocks per launch, staged, row order);
     # average over the launches of one pass
     ("k_bwd", 4, 9, 1, 0, "list"): 2 * 734223 * 1024 + 1427 * 1024,
@@ -98,19 +98,19 @@ MEASURED_TRAFFIC = {
     ("k_bwd1x", 4, 9, "locality", "r32", 64): 2 * 6605601 * 1024 + 4775 * 1024,
     # 69 row blocks of 65,536 rows
     ("k_bwd1x", 1, 69, "locality", "r16", 16): 2 * 7317888 * 1024 + 9221 * 1024,
-    ("k_bwd1x", 1, 69, "locality", "r16", 32): 2 * 6109372 * 1024 + 10973 * 1024,
+    ("k_bwd1x", 1, 69, "locality", "r16", 32): 2 * 6206522 * 1024 + 12241 * 1024,
     ("k_bwd1x", 1, 69, "locality", "r16", 64): 2 * 5560842 * 1024 + 6292 * 1024,
     ("k_bwd1x", 1, 69, "locality", "r32", 16): 2 * 8709134 * 1024 + 10990 * 1024,
     ("k_bwd1x", 1, 69, "locality", "r32", 32): 2 * 7534303 * 1024 + 13905 * 1024,
     ("k_bwd1x", 1, 69, "locality", "r32", 64): 2 * 6944555 * 1024 + 12157 * 1024,
     ("k_bwd1x", 2, 69, "locality", "r16", 16): 2 * 6657187 * 1024 + 9759 * 1024,
-    ("k_bwd1x", 2, 69, "locality", "r16", 32): 2 * 5656951 * 1024 + 12299 * 1024,
+    ("k_bwd1x", 2, 69, "locality", "r16", 32): 2 * 5894705 * 1024 + 12803 * 1024,
     ("k_bwd1x", 2, 69, "locality", "r16", 64): 2 * 5542259 * 1024 + 6110 * 1024,
     ("k_bwd1x", 2, 69, "locality", "r32", 16): 2 * 7591338 * 1024 + 12140 * 1024,
     ("k_bwd1x", 2, 69, "locality", "r32", 32): 2 * 7006851 * 1024 + 13763 * 1024,
     ("k_bwd1x", 2, 69, "locality", "r32", 64): 2 * 6702576 * 1024 + 11656 * 1024,
     ("k_bwd1x", 4, 69, "locality", "r16", 16): 2 * 6399207 * 1024 + 10333 * 1024,
-    ("k_bwd1x", 4, 69, "locality", "r16", 32): 2 * 5913324 * 1024 + 10769 * 1024,
+    ("k_bwd1x", 4, 69, "locality", "r16", 32): 2 * 5997025 * 1024 + 12419 * 1024,
     ("k_bwd1x", 4, 69, "locality", "r16", 64): 2 * 5549076 * 1024 + 6098 * 1024,
     ("k_bwd1x", 4, 69, "locality", "r32", 16): 2 * 7214243 * 1024 + 10933 * 1024,
     ("k_bwd1x", 4, 69, "locality", "r32", 32): 2 * 6826134 * 1024 + 11861 * 1024,
@@ -355,7 +355,7 @@ def main():
                   "k_fwd1cp" if info["fwd_kind"] == 1 and cols16 else
                   "k_fwd1c" if cols16 else "k_fwd1")
     fwd_name = f"{fwd_kernel} (forward + ratio, one launch" + (
-        ", x staged in LDS per 256 rows, 16-bit ranks)" if fwd_kernel == "k_fwd1l" else
+        ", x staged in LDS per 64 rows, 16-bit ranks)" if fwd_kernel == "k_fwd1l" else
         ", 16-bit columns)" if cols16 else ")")
     # bytes as stored: 16-bit indices move 2 bytes less per entry than the
     # 4-byte index of the algorithmic count (plus 8 bytes of split points per
@@ -378,10 +378,10 @@ def main():
         bytes_f = 8 * info["padded_nnz"] + 8 * (m + 1) + 4 * m + 4 * m + 4 * n
         if fwd_kernel == "k_fwd1l":
             # 2-byte ranks; per union entry its column and x (4 + 4 bytes);
-            # union offsets per 256 rows; no x term (read through the unions)
+            # union offsets per 64 rows; no x term (read through the unions)
             nu = R.forward_union_entries()
             stored_f = (bytes_f - 2 * info["padded_nnz"] - 4 * n + 8 * nu
-                        + 8 * (-(-m // 256) + 1))
+                        + 8 * (-(-m // 64) + 1))
         elif cols16:
             stored_f = bytes_f - 2 * info["padded_nnz"] + 8 * m
     if info["layout"] == "tiled":

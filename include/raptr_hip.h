@@ -423,7 +423,7 @@ int raptr_recon_get_compact_rows(const raptr_recon* plan);
  *   1  as 0 over 16-bit columns, software-pipelined (each lane loads its
  *      next step while the gathers of the current one are in flight);
  *   2  x staged in LDS: each workgroup copies x over the sorted column
- *      union of its 256 consecutive plan rows into LDS and gathers from
+ *      union of its 64 consecutive plan rows into LDS and gathers from
  *      there through 16-bit ranks.  Built for single-frame blocked plans
  *      of <= 258,048 voxels whose unions all hold <= 16,384 voxels (rows
  *      in an order that keeps neighbouring lines together, e.g.

@@ -452,7 +452,7 @@ __global__ __launch_bounds__(256) void k_fwd1cp(int64_t m, const int64_t* __rest
 // stream uses ordinary cached loads: rows share the cache lines at their
 // boundaries, which non-temporal loads fetched twice (12.2 -> 10.3 GB of
 // HBM reads per pass at full size; x lives in LDS, so nothing is evicted).
-constexpr int kLocRows = 256;
+constexpr int kLocRows = 64;
 constexpr int kLocMaxUnion = 16384;  // 64 KB of LDS per workgroup
 
 template <int VT, int LANES, int U>

@@ -133,7 +133,7 @@ class Reconstructor:
     def set_forward_kind(self, kind):
         """Single-frame forward kernel: 0 one entry per lane, 1 the same
         software-pipelined over 16-bit columns, 2 x staged in LDS over the
-        column union of each group of 256 plan rows (all bit-identical)."""
+        column union of each group of 64 plan rows (all bit-identical)."""
         capi.call("raptr_recon_set_forward_kind", self.handle, int(kind))
 
     def forward_kind(self):

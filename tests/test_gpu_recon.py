@@ -618,7 +618,7 @@ def test_recon_forward_compact_columns_absent(gpu, case):
 
 @pytest.mark.parametrize("vt", ["f32", "bf16", "u16"])
 def test_recon_forward_lds_staged(gpu, vt):
-    """LDS-staged forward (kind 2: x over each 256-row group's column union
+    """LDS-staged forward (kind 2: x over each 64-row group's column union
     in LDS, 16-bit ranks) on a 258,000-voxel matrix whose groups sit in
     8,000-voxel windows across the grid: the same ratios, bit for bit, as
     kinds 0 and 1 and the 32-bit columns for every lanes/unroll shape; FP32
@@ -659,7 +659,7 @@ def test_recon_forward_lds_staged(gpu, vt):
 
 
 def test_recon_forward_lds_staged_absent(gpu):
-    """No column unions when a 256-row group touches more than 16,384
+    """No column unions when a 64-row group touches more than 16,384
     voxels (rows of random columns over the grid) or the grid has more than
     258,048 voxels; kind 2 is then refused."""
     from raptr_amd import RaptrError
@@ -693,7 +693,7 @@ def test_recon_forward_lds_staged_small_system(gpu, oracle, small_system):
 
 @pytest.mark.parametrize("case", ["empty_group", "one_row"])
 def test_recon_forward_lds_staged_edges(gpu, case):
-    """LDS-staged forward edge cases: a whole 256-row group with no entries
+    """LDS-staged forward edge cases: 256 rows (4 groups) with no entries
     and a ragged last group (m = 785), and a one-row plan: the same ratios,
     bit for bit, as the per-entry forward, and the FP64 host product."""
     from raptr_amd import _capi as capi
