@@ -26,13 +26,13 @@ inline float angle_sin(int ida, float da) {
   float arg = 3.14159265358979323846f * float(ida);
   arg = arg * da;
   arg = arg / 180.0f;
-  return float(std::sin(double(arg)));
+  return ::sinf(arg);  // the host C library's, as the reference evaluates it
 }
 inline float angle_cos(int ida, float da) {
   float arg = 3.14159265358979323846f * float(ida);
   arg = arg * da;
   arg = arg / 180.0f;
-  return float(std::cos(double(arg)));
+  return ::cosf(arg);
 }
 }  // namespace raptr_detail
 

@@ -82,17 +82,20 @@ int ro_vxl_lin(const ro_grid* g, int ix, int iy, int iz) {
  * A4: angle table.  The reference evaluates sinf/cosf of
  * float(pi)*ida*da/180 inside every trafo call
  * (MeasurementSetupTrafo2CartCoord.hpp:110-111).  The argument keeps that
- * float evaluation order; the function value is the correctly rounded float
- * of the double-precision sin/cos of that float argument, tabulated per
- * angle on the host and shared by the oracle and every GPU kernel.
+ * float evaluation order and the function value is the C library's
+ * sinf/cosf of that float argument, i.e. the reference's host evaluation
+ * (glibc's sinf/cosf are not correctly rounded everywhere: at 8 of the 180
+ * one-degree angles they differ from the rounded double-precision value by
+ * one ulp).  Tabulated per angle on the host and shared by the oracle and
+ * every GPU kernel.
  * ==================================================================== */
 void ro_angle_table(const ro_setup* s, float* sn, float* cs) {
   for (int a = 0; a < s->na; ++a) {
     float arg = RO_PI_F * (float)a;
     arg = arg * s->da;
     arg = arg / 180.0f;
-    sn[a] = (float)sin((double)arg);
-    cs[a] = (float)cos((double)arg);
+    sn[a] = sinf(arg);
+    cs[a] = cosf(arg);
   }
 }
 

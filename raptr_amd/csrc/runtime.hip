@@ -204,8 +204,11 @@ int raptr_angle_table(const raptr_setup_t* setup, float* sn, float* cs) {
     float arg = pi_f * float(a);
     arg = arg * setup->da;
     arg = arg / 180.0f;
-    sn[a] = float(std::sin(double(arg)));
-    cs[a] = float(std::cos(double(arg)));
+    // the host C library's sinf/cosf of the float argument: the values the
+    // reference's host code uses (glibc's are not correctly rounded at every
+    // angle, so a double-precision evaluation would differ by an ulp there)
+    sn[a] = ::sinf(arg);
+    cs[a] = ::cosf(arg);
   }
   return RAPTR_OK;
 }

@@ -2,7 +2,7 @@
 
 Run in a container that holds the raptr sources (builds oracle/_ref first):
 
-    make -C oracle ref && python tests/golden/make_golden.py
+    make -C oracle ref && python tests/golden/make_golden.py [full_angles]
 
 Fixtures (numpy .npz, data only):
   wq_small.npz    workqueue of the 20,480-channel / 32^3 workload computed
@@ -18,6 +18,14 @@ Fixtures (numpy .npz, data only):
                   Coord.hpp, compiled) for all pixels/angles of the 4x4-head
                   setup of MeasurementSetupTrafo2CartCoordUT.cu:42-58 at 7
                   unit-cube points.
+  wq_full_angles.npz  the 13x13-head, 180-angle (1 deg), 52^3 @ 2 mm
+                  geometry of the full-size workload: the reference trafo of
+                  both heads at all 180 angles (4 pixels x 3 unit-cube
+                  points each), and the reference workqueue of one channel
+                  per angle over the whole grid (pixels varied with the
+                  angle), stored in full (voxel ids delta-coded per channel),
+                  with the near-boundary pairs of those channels and their
+                  in/out decision.
 """
 import hashlib
 import os
@@ -133,7 +141,58 @@ def main():
     np.savez_compressed(os.path.join(HERE, "trafo_ref.npz"), setup_f=sf, setup_i=si, pts=pts,
                         out=out)
     print("trafo_ref written")
+    full_angles()
+
+
+def full_angle_channels(setup):
+    """One channel per angle of the 180-angle setup, pixels varied with the
+    angle so that every head row and column occurs."""
+    a = np.arange(setup.na)
+    n = setup.n0z
+    return np.asarray(setup.lin_id(a % n, (5 * a + 3) % n, (7 * a + 1) % n, (11 * a + 6) % n, a),
+                      np.int32)
+
+
+def full_angles():
+    setup, grid, _ = G.config_full()
+    s, g = osetup(setup), ogrid(grid)
+    lst = full_angle_channels(setup)
+    c, v, li, vi = O.ref_workqueue(s, g, lst)
+    assert li == len(lst) and vi == 0
+    counts = np.bincount(np.searchsorted(lst, c) if np.all(np.diff(lst) > 0) else
+                         np.array([np.nonzero(lst == k)[0][0] for k in c]), minlength=len(lst))
+    starts = np.concatenate([[0], np.cumsum(counts)[:-1]])
+    dv = np.diff(v, prepend=0).astype(np.int32)
+    dv[starts[counts > 0]] = v[starts[counts > 0]]
+    nb = near_boundary(setup, grid, lst)
+    key_all = c.astype(np.int64) * grid.nvox + v
+    key_nb = nb[:, 0].astype(np.int64) * grid.nvox + nb[:, 1]
+    inq = np.isin(key_nb, key_all).astype(np.int8)
+    pix = np.array([[0, 0], [12, 12], [6, 6], [3, 9]], np.int32)
+    pts = np.array([[.5, .5, .5], [0, 0, 0], [.9, .3, .7]], np.float32)
+    sf = np.array([setup.pos0x, setup.pos1x, setup.da, setup.segx, setup.segy, setup.segz],
+                  np.float32)
+    si = np.array([setup.na, setup.n0z, setup.n0y, setup.n1z, setup.n1y], np.int32)
+    R = O.ref()
+    out = np.zeros((2, setup.na, len(pix), len(pts), 3), np.float32)
+    for det in range(2):
+        for a in range(setup.na):
+            for j, (iz, iy) in enumerate(pix):
+                for k, p in enumerate(pts):
+                    o = np.zeros(3, np.float32)
+                    R.ref_trafo(O._p(sf, O.C.c_float), O._p(si, O.C.c_int), det, int(iz),
+                                int(iy), a, O._p(p, O.C.c_float), O._p(o, O.C.c_float))
+                    out[det, a, j, k] = o
+    np.savez_compressed(os.path.join(HERE, "wq_full_angles.npz"), channels=lst,
+                        counts=counts.astype(np.int32), vxl_delta=dv,
+                        sha256=np.array(sha(c, v)), near=nb, near_in=inq, setup_f=sf,
+                        setup_i=si, trafo_pix=pix, trafo_pts=pts, trafo_out=out)
+    print("wq_full_angles:", len(c), "entries,", len(nb), "near-boundary pairs,", inq.sum(),
+          "inside")
 
 
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["full_angles"]:
+        full_angles()
+    else:
+        main()

@@ -37,6 +37,42 @@ def test_workqueue_small_matches_reference(gpu):
     np.testing.assert_array_equal(v[-4096:], gold["tail_vxl"])
 
 
+def test_workqueue_full_angles_matches_reference(gpu):
+    """The 180-angle, 52^3 geometry of the full-size workload: one channel
+    per angle, every (channel, voxel) pair equal to the reference's
+    getWorkqueue (its per-call float sinf/cosf against the shared table)."""
+    from raptr_amd import sysmat
+    from test_oracle import _full_angle_fixture
+    gold, cnl, vxl = _full_angle_fixture()
+    setup, grid, _ = G.config_full()
+    counts = sysmat.workqueue_counts(setup, grid, gold["channels"])
+    np.testing.assert_array_equal(counts, gold["counts"])
+    c, v, li, vi = sysmat.workqueue(setup, grid, gold["channels"])
+    assert (li, vi) == (len(gold["channels"]), 0)
+    assert sha(c, v) == str(gold["sha256"])
+    np.testing.assert_array_equal(v, vxl)
+
+
+def test_pixel_centres_full_angles_match_reference(gpu):
+    """Pixel-centre rays (REGU) of the 180-angle setup equal the reference
+    trafo at the unit-cube centre for both heads at every angle."""
+    from raptr_amd import sysmat, RAYGEN_REGU
+    gold = load_golden("wq_full_angles.npz")
+    setup, _, _ = G.config_full()
+    pix, out = gold["trafo_pix"], gold["trafo_out"]
+    assert np.array_equal(gold["trafo_pts"][0], [0.5, 0.5, 0.5])
+    chans, want = [], []
+    for a in range(setup.na):
+        for (iz0, iy0) in pix:
+            for (iz1, iy1) in pix:
+                chans.append(setup.lin_id(iz0, iy0, iz1, iy1, a))
+                j0 = [tuple(p) for p in pix].index((iz0, iy0))
+                j1 = [tuple(p) for p in pix].index((iz1, iy1))
+                want.append(np.concatenate([out[0, a, j0, 0], out[1, a, j1, 0]]))
+    r = sysmat.generate_rays(setup, np.asarray(chans, np.int32), 1, raygen=RAYGEN_REGU)
+    np.testing.assert_array_equal(r[:, 0], np.asarray(want, np.float32))
+
+
 def test_workqueue_bounded_cursor_matches_reference(gpu):
     from raptr_amd import sysmat
     gold = load_golden("wq_small.npz")

@@ -98,8 +98,15 @@ def test_trafo_closed_form(oracle):
 
 
 def test_angle_table_matches_libm(oracle):
-    """The shared sin/cos table equals float sinf/cosf of the reference's
-    float argument for every angle of the named workloads."""
+    """The shared sin/cos table (oracle and the HIP library's host code)
+    equals the C library's float sinf/cosf of the reference's float argument
+    for every angle of the named workloads; the reference-built trafo at all
+    180 angles (test_trafo_full_angles_matches_reference_build) pins that
+    this is the reference's evaluation."""
+    import ctypes as C
+    libm = C.CDLL("libm.so.6")
+    for f in (libm.sinf, libm.cosf):
+        f.restype, f.argtypes = C.c_float, [C.c_float]
     for setup, _, _ in (G.config_small(), G.config_full(), G.config_tiny()):
         s = oracle.make_setup(setup.pos0x, setup.pos1x, setup.na, setup.n0z, setup.n0y,
                               setup.n1z, setup.n1y, setup.da, setup.segx, setup.segy,
@@ -107,8 +114,61 @@ def test_angle_table_matches_libm(oracle):
         sn, cs = oracle.angle_table(s)
         arg = (np.float32(np.pi) * np.arange(setup.na, dtype=np.float32)) * np.float32(setup.da)
         arg = arg / np.float32(180.0)
-        np.testing.assert_array_equal(sn, np.sin(arg.astype(np.float64)).astype(np.float32))
-        np.testing.assert_array_equal(cs, np.cos(arg.astype(np.float64)).astype(np.float32))
+        np.testing.assert_array_equal(sn, np.array([libm.sinf(float(a)) for a in arg], np.float32))
+        np.testing.assert_array_equal(cs, np.array([libm.cosf(float(a)) for a in arg], np.float32))
+        from raptr_amd import _capi as capi
+        hs, hc = np.zeros(setup.na, np.float32), np.zeros(setup.na, np.float32)
+        capi.call("raptr_angle_table", C.byref(setup.struct()), hs.ctypes.data_as(C.POINTER(C.c_float)),
+                  hc.ctypes.data_as(C.POINTER(C.c_float)))
+        np.testing.assert_array_equal(hs, sn)
+        np.testing.assert_array_equal(hc, cs)
+
+
+def _full_angle_fixture():
+    gold = load_golden("wq_full_angles.npz")
+    counts = gold["counts"].astype(np.int64)
+    cnl = np.repeat(gold["channels"], counts)
+    starts = np.concatenate([[0], np.cumsum(counts)[:-1]])
+    vxl = np.empty(len(cnl), np.int64)
+    for s0, k in zip(starts, counts):
+        vxl[s0:s0 + k] = np.cumsum(gold["vxl_delta"][s0:s0 + k].astype(np.int64))
+    return gold, cnl.astype(np.int32), vxl.astype(np.int32)
+
+
+def test_trafo_full_angles_matches_reference_build(oracle):
+    """All 180 one-degree angles of the full-size setup: the shared sin/cos
+    table gives the reference trafo (per-call float sinf/cosf,
+    MeasurementSetupTrafo2CartCoord.hpp:110-111) bit for bit."""
+    gold = load_golden("wq_full_angles.npz")
+    setup, _, _ = G.config_full()
+    s = oracle.make_setup(setup.pos0x, setup.pos1x, setup.na, setup.n0z, setup.n0y,
+                          setup.n1z, setup.n1y, setup.da, setup.segx, setup.segy, setup.segz)
+    out = gold["trafo_out"]
+    assert out.shape[1] == 180
+    for det in range(2):
+        for a in range(setup.na):
+            for j, (iz, iy) in enumerate(gold["trafo_pix"]):
+                for k, p in enumerate(gold["trafo_pts"]):
+                    np.testing.assert_array_equal(oracle.trafo(s, det, a, int(iz), int(iy), p),
+                                                  out[det, a, j, k])
+
+
+def test_workqueue_full_angles_matches_reference(oracle):
+    """One channel per angle of the 180-angle, 52^3 geometry: every (channel,
+    voxel) decision equals the reference's getWorkqueue, including the pairs
+    within 1e-4 mm of the cylinder radius."""
+    gold, cnl, vxl = _full_angle_fixture()
+    assert sha(cnl, vxl) == str(gold["sha256"])
+    setup, grid, _ = G.config_full()
+    s, g = oracle_geom(oracle, setup, grid)
+    c, v, li, vi = oracle.workqueue(s, g, gold["channels"], capacity=len(cnl) + 16)
+    assert (li, vi) == (len(gold["channels"]), 0)
+    np.testing.assert_array_equal(c, cnl)
+    np.testing.assert_array_equal(v, vxl)
+    key = c.astype(np.int64) * grid.nvox + v
+    near = gold["near"]
+    got_in = np.isin(near[:, 0].astype(np.int64) * grid.nvox + near[:, 1], key)
+    np.testing.assert_array_equal(got_in.astype(np.int8), gold["near_in"])
 
 
 # ------------------------------------------------------------ distance (A5)
