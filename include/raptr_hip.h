@@ -35,7 +35,17 @@ enum {
   RAPTR_ERR_NOT_SUPPORTED = 6
 };
 
-enum { RAPTR_RAYGEN_RAND = 0, RAPTR_RAYGEN_REGU = 1 };
+/* Ray generators (RayGenerators.hpp:36-99).  RAND: Philox4x32-10 rays per
+ * channel, key (seed, 0), counter (channel, ray, 0|1, 0): every voxel of a
+ * channel sees the same rays (the per-channel seeding of
+ * getSystemMatrixFromWorkqueue.cu:209-210).  REGU: pixel-centre rays.
+ * RAND_ELEMENT: rays per (channel, voxel) element, counter (channel, ray,
+ * 0|1, voxel + 1): every matrix element draws its own independent rays, the
+ * semantics of the fused kernel's per-thread generators
+ * (getSystemMatrixDeviceOnly.cu:396), reproducible whatever the launch
+ * shape.  RAND_ELEMENT applies to matrix values; ray listings
+ * (raptr_generate_rays) take RAND or REGU. */
+enum { RAPTR_RAYGEN_RAND = 0, RAPTR_RAYGEN_REGU = 1, RAPTR_RAYGEN_RAND_ELEMENT = 2 };
 enum { RAPTR_OP_NON_TRANSPOSE = 0, RAPTR_OP_TRANSPOSE = 1 };
 enum { RAPTR_MAX_ANGLES = 360 };
 
@@ -160,6 +170,18 @@ int raptr_sm_build_csr(const raptr_setup_t* setup, const raptr_grid_t* grid,
                        int* d_col, float* d_val, int64_t capacity,
                        int64_t* nnz, void* d_workspace, size_t workspace_bytes,
                        raptr_stream_t stream);
+/* raptr_sm_build_csr with build flags.  RAPTR_SM_KEEP_ZEROS: store every
+ * candidate that passes the cylinder cull, zero-valued ones included, as
+ * the reference's fused kernel writes them (getSystemMatrixDeviceOnly.cu:
+ * 507-509); the matrix then holds exactly the workqueue's (channel, voxel)
+ * pairs (nnz = workqueue length).  flags = 0 is raptr_sm_build_csr. */
+enum { RAPTR_SM_KEEP_ZEROS = 1 };
+int raptr_sm_build_csr_ex(const raptr_setup_t* setup, const raptr_grid_t* grid,
+                          const int* d_list, int list_size, int nrays,
+                          uint32_t seed, int raygen, int flags, int64_t* d_row_ptr,
+                          int* d_col, float* d_val, int64_t capacity,
+                          int64_t* nnz, void* d_workspace, size_t workspace_bytes,
+                          raptr_stream_t stream);
 /* Sensitivity image s_j = sum over ALL channels of the setup of A_ij, in
  * FP64 (d_sens[nvox]); the rows are generated and reduced on the fly, never
  * stored.  Replaces the sensitivity file the reference reads

@@ -17,6 +17,8 @@ REF_PATH = os.path.join(HERE, "_ref", "libraptr_ref.so")
 
 RAYGEN_RAND = 0
 RAYGEN_REGU = 1
+RAYGEN_RAND_ELEMENT = 2
+SM_KEEP_ZEROS = 1
 
 
 class Setup(C.Structure):
@@ -98,6 +100,9 @@ def _bind(L):
         "ro_sm_from_wq_f64": (None, [S, G, P(i32), P(i32), P(f64), i64, i32, u32, i32]),
         "ro_get_chords": (i32, [P(f32), P(i32), i32, P(f32), P(f32), P(f32), P(i32)]),
         "ro_build_csr": (i64, [S, G, P(i32), i32, i32, u32, i32, P(i64), P(i32), P(f32), P(f64)]),
+        "ro_build_csr_ex": (i64, [S, G, P(i32), i32, i32, u32, i32, i32, P(i64), P(i32), P(f32),
+                                  P(f64)]),
+        "ro_ray_element": (None, [S, P(f32), P(f32), i32, u32, i32, i32, i32, P(f32)]),
         "ro_coo_sort": (None, [P(f32), P(i32), P(i32), i64]),
         "ro_coo2csr": (None, [P(i32), P(i32), i64, i32]),
         "ro_csr2ecsr": (None, [P(i32), P(i32), i32, P(i32), i32]),
@@ -211,6 +216,14 @@ def philox(ctr, key):
     return o
 
 
+def ray_element(s, cnl, element, r, seed=1, raygen=RAYGEN_RAND_ELEMENT):
+    sn, cs = angle_table(s)
+    out = np.zeros(6, np.float32)
+    lib().ro_ray_element(C.byref(s), _p(sn, C.c_float), _p(cs, C.c_float), raygen, seed, cnl,
+                         element, r, _p(out, C.c_float))
+    return out
+
+
 def ray(s, cnl, r, seed=1, raygen=RAYGEN_RAND):
     sn, cs = angle_table(s)
     o = np.zeros(6, np.float32)
@@ -273,20 +286,21 @@ def get_chords(r, g, cap=None):
     return ch[:k].copy(), ids[:3 * k].reshape(-1, 3).copy()
 
 
-def build_csr(s, g, lst, nrays, seed=1, raygen=RAYGEN_RAND, with_f64=False):
-    """Matrix of the listed rows, zeros dropped: (row_ptr i64, col, val f32, val f64|None)."""
+def build_csr(s, g, lst, nrays, seed=1, raygen=RAYGEN_RAND, with_f64=False, flags=0):
+    """Matrix of the listed rows, zeros dropped (flags SM_KEEP_ZEROS: kept):
+    (row_ptr i64, col, val f32, val f64|None)."""
     lst = np.ascontiguousarray(lst, np.int32)
     rp = np.zeros(len(lst) + 1, np.int64)
     L = lib()
-    L.ro_build_csr(C.byref(s), C.byref(g), _p(lst, C.c_int), len(lst), nrays, seed, raygen,
-                   _p(rp, C.c_int64), None, None, None)
+    L.ro_build_csr_ex(C.byref(s), C.byref(g), _p(lst, C.c_int), len(lst), nrays, seed, raygen,
+                      flags, _p(rp, C.c_int64), None, None, None)
     nnz = int(rp[-1])
     col = np.zeros(max(nnz, 1), np.int32)
     val = np.zeros(max(nnz, 1), np.float32)
     v64 = np.zeros(max(nnz, 1), np.float64) if with_f64 else None
-    L.ro_build_csr(C.byref(s), C.byref(g), _p(lst, C.c_int), len(lst), nrays, seed, raygen,
-                   _p(rp, C.c_int64), _p(col, C.c_int), _p(val, C.c_float),
-                   _p(v64, C.c_double) if with_f64 else None)
+    L.ro_build_csr_ex(C.byref(s), C.byref(g), _p(lst, C.c_int), len(lst), nrays, seed, raygen,
+                      flags, _p(rp, C.c_int64), _p(col, C.c_int), _p(val, C.c_float),
+                      _p(v64, C.c_double) if with_f64 else None)
     return rp, col[:nnz], val[:nnz], (v64[:nnz] if with_f64 else None)
 
 

@@ -305,15 +305,19 @@ float ro_u01(uint32_t x) { return (float)((x >> 8) + 1u) * (1.0f / 16777216.0f);
  * in pixel 0's volume by trafo0, the last three in pixel 1's by trafo1.
  * Counter = (channel, ray, block, 0), key = (seed, 0); block 0 gives u0..u3,
  * block 1 gives u4, u5.  ReguRayGen (RayGenerators.hpp:73-99): all six = 0.5. */
-void ro_ray(const ro_setup* s, const float* sn, const float* cs, int raygen,
-            uint32_t seed, int cnl, int ray, float out[6]) {
+void ro_ray_element(const ro_setup* s, const float* sn, const float* cs, int raygen,
+                    uint32_t seed, int cnl, int element, int ray, float out[6]) {
   float u[6];
   if (raygen == RO_RAYGEN_REGU) {
     for (int i = 0; i < 6; ++i) u[i] = 0.5f;
   } else {
+    /* per-element streams (RAND_ELEMENT): counter word 3 = element + 1, the
+     * per-thread generators of getSystemMatrixDeviceOnly.cu:396 restated as
+     * one independent stream per (channel, voxel) */
+    uint32_t w3 = raygen == RO_RAYGEN_RAND_ELEMENT ? (uint32_t)element + 1u : 0u;
     uint32_t key[2] = {seed, 0u};
-    uint32_t c0[4] = {(uint32_t)cnl, (uint32_t)ray, 0u, 0u};
-    uint32_t c1[4] = {(uint32_t)cnl, (uint32_t)ray, 1u, 0u};
+    uint32_t c0[4] = {(uint32_t)cnl, (uint32_t)ray, 0u, w3};
+    uint32_t c1[4] = {(uint32_t)cnl, (uint32_t)ray, 1u, w3};
     uint32_t r0[4], r1[4];
     ro_philox4x32_10(c0, key, r0);
     ro_philox4x32_10(c1, key, r1);
@@ -328,6 +332,11 @@ void ro_ray(const ro_setup* s, const float* sn, const float* cs, int raygen,
   ro_cnl_split(s, cnl, &ida, &id0z, &id0y, &id1z, &id1y);
   ro_trafo(s, 0, sn[ida], cs[ida], id0z, id0y, &u[0], &out[0]);
   ro_trafo(s, 1, sn[ida], cs[ida], id1z, id1y, &u[3], &out[3]);
+}
+
+void ro_ray(const ro_setup* s, const float* sn, const float* cs, int raygen,
+            uint32_t seed, int cnl, int ray, float out[6]) {
+  ro_ray_element(s, sn, cs, raygen, seed, cnl, -1, ray, out);
 }
 
 /* ======================================================================
@@ -426,7 +435,11 @@ void ro_sm_from_wq(const ro_setup* s, const ro_grid* g, const int* cnl,
     int cur = -1;
 #pragma omp for schedule(static)
     for (int64_t w = 0; w < wq_len; ++w) {
-      if (cnl[w] != cur) {
+      if (raygen == RO_RAYGEN_RAND_ELEMENT) {
+        for (int r = 0; r < nrays; ++r)
+          ro_ray_element(s, sn, cs, raygen, seed, cnl[w], vxl[w], r, &rays[6 * r]);
+        cur = -1;
+      } else if (cnl[w] != cur) {
         cur = cnl[w];
         for (int r = 0; r < nrays; ++r)
           ro_ray(s, sn, cs, raygen, seed, cur, r, &rays[6 * r]);
@@ -456,11 +469,11 @@ void ro_sm_from_wq_f64(const ro_setup* s, const ro_grid* g, const int* cnl,
     int cur = -1;
 #pragma omp for schedule(static)
     for (int64_t w = 0; w < wq_len; ++w) {
-      if (cnl[w] != cur) {
-        cur = cnl[w];
+      if (raygen == RO_RAYGEN_RAND_ELEMENT || cnl[w] != cur) {
+        cur = raygen == RO_RAYGEN_RAND_ELEMENT ? -1 : cnl[w];
         for (int r = 0; r < nrays; ++r) {
           float rf[6];
-          ro_ray(s, sn, cs, raygen, seed, cur, r, rf);
+          ro_ray_element(s, sn, cs, raygen, seed, cnl[w], vxl[w], r, rf);
           for (int k = 0; k < 6; ++k) rays[6 * r + k] = (double)rf[k];
         }
       }
@@ -574,10 +587,23 @@ int ro_get_chords(float* chords, int* voxel_ids, int cap, const float ray[6],
  * candidate set per row is A6's.  This is what the reference's
  * systemMatrixCalculation (wrappers.hpp:185-266) produces after cooSort,
  * coo2csr and csr2ecsr, minus the zero-valued candidates it stores.
+ * ro_build_csr_ex with RO_SM_KEEP_ZEROS keeps them too
+ * (getSystemMatrixDeviceOnly.cu:507-509): the matrix then holds exactly the
+ * workqueue's pairs.  RO_RAYGEN_RAND_ELEMENT samples each element with its
+ * own rays.
  * ==================================================================== */
 int64_t ro_build_csr(const ro_setup* s, const ro_grid* g, const int* list,
                      int list_size, int nrays, uint32_t seed, int raygen,
                      int64_t* row_ptr, int* col, float* val, double* val64) {
+  return ro_build_csr_ex(s, g, list, list_size, nrays, seed, raygen, 0, row_ptr, col, val,
+                         val64);
+}
+
+int64_t ro_build_csr_ex(const ro_setup* s, const ro_grid* g, const int* list,
+                        int list_size, int nrays, uint32_t seed, int raygen, int flags,
+                        int64_t* row_ptr, int* col, float* val, double* val64) {
+  const int keep_zeros = (flags & RO_SM_KEEP_ZEROS) != 0;
+  const int elem = raygen == RO_RAYGEN_RAND_ELEMENT;
   int nvox = g->n[0] * g->n[1] * g->n[2];
   float* sn = (float*)malloc(sizeof(float) * s->na);
   float* cs = (float*)malloc(sizeof(float) * s->na);
@@ -607,10 +633,15 @@ int64_t ro_build_csr(const ro_setup* s, const ro_grid* g, const int* list,
         if (!(ro_distance(cg.pix0, cg.pix1, vc) < rsum)) continue;
         float box[6];
         ro_voxel_box(g, v, box);
+        if (elem)
+          for (int r = 0; r < nrays; ++r) {
+            ro_ray_element(s, sn, cs, raygen, seed, cnl, v, r, &rays[6 * r]);
+            for (int k = 0; k < 6; ++k) rays64[6 * r + k] = (double)rays[6 * r + k];
+          }
         float a = 0.0f;
         for (int r = 0; r < nrays; ++r) a += ro_intersection_length(box, &rays[6 * r]);
         a /= (float)nrays;
-        if (a == 0.0f) continue;
+        if (a == 0.0f && !keep_zeros) continue;
         if (fill) {
           col[pos + c] = v;
           val[pos + c] = a;

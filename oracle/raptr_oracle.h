@@ -56,7 +56,8 @@ typedef struct ro_grid {
   int n[3];
 } ro_grid;
 
-enum { RO_RAYGEN_RAND = 0, RO_RAYGEN_REGU = 1 };
+enum { RO_RAYGEN_RAND = 0, RO_RAYGEN_REGU = 1, RO_RAYGEN_RAND_ELEMENT = 2 };
+enum { RO_SM_KEEP_ZEROS = 1 };
 
 /* ---- A2/A3: index maps ---------------------------------------------- */
 int64_t ro_nchannels(const ro_setup* s);
@@ -93,6 +94,10 @@ void ro_set_threads(int n);
 int ro_get_threads(void);
 void ro_ray(const ro_setup* s, const float* sn, const float* cs, int raygen,
             uint32_t seed, int cnl, int ray, float out[6]);
+/* ray of one (channel, element) pair: element is the voxel for
+ * RO_RAYGEN_RAND_ELEMENT, ignored otherwise */
+void ro_ray_element(const ro_setup* s, const float* sn, const float* cs, int raygen,
+                    uint32_t seed, int cnl, int element, int ray, float out[6]);
 
 /* ---- A9: slab intersection length ------------------------------------ */
 float ro_intersection_length(const float vxl[6], const float ray[6]);
@@ -118,6 +123,9 @@ int ro_get_chords(float* chords, int* voxel_ids, int n_chords_cap,
 int64_t ro_build_csr(const ro_setup* s, const ro_grid* g, const int* list,
                      int list_size, int nrays, uint32_t seed, int raygen,
                      int64_t* row_ptr, int* col, float* val, double* val64);
+int64_t ro_build_csr_ex(const ro_setup* s, const ro_grid* g, const int* list,
+                        int list_size, int nrays, uint32_t seed, int raygen, int flags,
+                        int64_t* row_ptr, int* col, float* val, double* val64);
 
 /* ---- A14-A16: COO sort, COO->CSR, CSR->ECSR --------------------------- */
 void ro_coo_sort(float* val, int* row, int* col, int64_t n);

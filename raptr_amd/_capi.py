@@ -20,6 +20,8 @@ RAPTR_ERR_NOT_SUPPORTED = 6
 
 RAYGEN_RAND = 0
 RAYGEN_REGU = 1
+RAYGEN_RAND_ELEMENT = 2
+SM_KEEP_ZEROS = 1
 OP_NON_TRANSPOSE = 0
 OP_TRANSPOSE = 1
 MAX_ANGLES = 360
@@ -89,6 +91,8 @@ _SIGS = {
     "raptr_sm_build_workspace_size": (sz, [GP, i32, i64]),
     "raptr_sm_build_csr": (i32, [SP, GP, vp, i32, i32, u32, i32, vp, vp, vp, i64, P(i64), vp,
                                  sz, vp]),
+    "raptr_sm_build_csr_ex": (i32, [SP, GP, vp, i32, i32, u32, i32, i32, vp, vp, vp, i64,
+                                    P(i64), vp, sz, vp]),
     "raptr_sensitivity_f64": (i32, [SP, GP, i32, u32, i32, vp, vp]),
     "raptr_backproject_f64": (i32, [SP, GP, vp, vp, i32, i32, u32, i32, vp, vp]),
     "raptr_forward_project_f64": (i32, [SP, GP, vp, i32, i32, u32, i32, vp, vp, vp]),

@@ -216,16 +216,19 @@ __host__ __device__ inline float u01(uint32_t x) {
 
 // RandRayGen / ReguRayGen (RayGenerators.hpp:36-99): six uniforms placed in
 // the two pixel volumes by the trafos.
+// element: the voxel of a RAND_ELEMENT ray (counter word 3 = element + 1);
+// ignored by the other generators.
 __host__ __device__ inline void make_ray(const Geom& G, int raygen,
                                          uint32_t seed, int cnl, int ray,
-                                         float* out) {
+                                         float* out, int element = -1) {
   float u[6];
   if (raygen == RAPTR_RAYGEN_REGU) {
     for (int i = 0; i < 6; ++i) u[i] = 0.5f;
   } else {
+    const uint32_t w3 = raygen == RAPTR_RAYGEN_RAND_ELEMENT ? uint32_t(element) + 1u : 0u;
     uint32_t r0[4], r1[4];
-    philox4x32_10(uint32_t(cnl), uint32_t(ray), 0u, 0u, seed, 0u, r0);
-    philox4x32_10(uint32_t(cnl), uint32_t(ray), 1u, 0u, seed, 0u, r1);
+    philox4x32_10(uint32_t(cnl), uint32_t(ray), 0u, w3, seed, 0u, r0);
+    philox4x32_10(uint32_t(cnl), uint32_t(ray), 1u, w3, seed, 0u, r1);
     u[0] = u01(r0[0]);
     u[1] = u01(r0[1]);
     u[2] = u01(r0[2]);

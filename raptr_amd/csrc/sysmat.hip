@@ -48,7 +48,7 @@ __global__ __launch_bounds__(256) void k_sm_from_wq(Geom G, const int* cnl, cons
     float a = 0.0f;
     for (int r = 0; r < nrays; ++r) {
       float ray[6];
-      make_ray(G, raygen, seed, c, r, ray);
+      make_ray(G, raygen, seed, c, r, ray, vv);
       a += slab_length(box, ray);
     }
     a /= float(nrays);
@@ -90,6 +90,7 @@ struct BuildArgs {
   int nrays;
   uint32_t seed;
   int raygen;
+  int keep_zeros;   // store zero-valued candidates too (RAPTR_SM_KEEP_ZEROS)
   int pl;           // planes per ray (nx+1 + ny+1 + nz+1)
   // outputs
   int* row_nnz;     // [rows]
@@ -152,6 +153,22 @@ __device__ inline float sample_voxel(const float* rinfo, const float* alpha, int
     }
     float len = (amin < amax) ? (amax - amin) * ri[6] : 0.0f;
     a += len;
+  }
+  return a / float(nrays);
+}
+
+// RAPTR_RAYGEN_RAND_ELEMENT: the element's own nrays rays, slab method on
+// the voxel box (the arithmetic of k_sm_from_wq, so both paths agree bit
+// for bit).
+__device__ inline float sample_element(const Geom& G, uint32_t seed, int nrays, int cnl,
+                                       int v, int ix, int iy, int iz) {
+  float box[6] = {plane(G.g, 0, ix),     plane(G.g, 1, iy),     plane(G.g, 2, iz),
+                  plane(G.g, 0, ix + 1), plane(G.g, 1, iy + 1), plane(G.g, 2, iz + 1)};
+  float a = 0.0f;
+  for (int r = 0; r < nrays; ++r) {
+    float ray[6];
+    make_ray(G, RAPTR_RAYGEN_RAND_ELEMENT, seed, cnl, r, ray, v);
+    a += slab_length(box, ray);
   }
   return a / float(nrays);
 }
@@ -224,8 +241,10 @@ __global__ __launch_bounds__(kSmBlock) void k_sm_build(BuildArgs A) {
         uint32_t qx = fdiv(uint32_t(v), A.div_yz);
         uint32_t rem = uint32_t(v) - qx * A.div_yz.d;
         uint32_t qy = fdiv(rem, A.div_z);
-        cval[c] = sample_voxel(rinfo, alpha, nr, pl, int(qx), int(qy),
-                               int(rem - qy * A.div_z.d), off1, off2);
+        const int iz = int(rem - qy * A.div_z.d);
+        cval[c] = A.raygen == RAPTR_RAYGEN_RAND_ELEMENT
+                      ? sample_element(A.G, A.seed, nr, cnl, v, int(qx), int(qy), iz)
+                      : sample_voxel(rinfo, alpha, nr, pl, int(qx), int(qy), iz, off1, off2);
       }
       __syncthreads();
       if (MODE == kModeColsum) {
@@ -278,7 +297,7 @@ __global__ __launch_bounds__(kSmBlock) void k_sm_build(BuildArgs A) {
       // compact non-zeros in candidate order
       for (int c0 = 0; c0 < cand_n; c0 += kSmBlock) {
         int c = c0 + t;
-        bool nz = c < cand_n && cval[c] != 0.0f;
+        bool nz = c < cand_n && (A.keep_zeros || cval[c] != 0.0f);
         uint64_t m = __ballot(nz);
         if (lane == 0) wcnt[w] = __popcll(m);
         __syncthreads();
@@ -464,7 +483,8 @@ int make_build_args(const raptr_setup_t* setup, const raptr_grid_t* grid, int nr
   std::memset(A, 0, sizeof(BuildArgs));
   RAPTR_TRY(make_geom(setup, grid, &A->G));
   if (nrays <= 0) return fail(RAPTR_ERR_INVALID_ARG, "nrays must be positive");
-  if (raygen != RAPTR_RAYGEN_RAND && raygen != RAPTR_RAYGEN_REGU)
+  if (raygen != RAPTR_RAYGEN_RAND && raygen != RAPTR_RAYGEN_REGU &&
+      raygen != RAPTR_RAYGEN_RAND_ELEMENT)
     return fail(RAPTR_ERR_INVALID_ARG, "unknown ray generator %d", raygen);
   A->div_yz = make_fastdiv(uint32_t(grid->n[1] * grid->n[2]));
   A->div_z = make_fastdiv(uint32_t(grid->n[2]));
@@ -543,6 +563,9 @@ int raptr_sm_from_wq(const raptr_setup_t* setup, const raptr_grid_t* grid, const
   RAPTR_TRY(make_geom(setup, grid, &G));
   if (nrays <= 0) return fail(RAPTR_ERR_INVALID_ARG, "nrays must be positive");
   if (wq_length < 0) return fail(RAPTR_ERR_INVALID_ARG, "negative length");
+  if (raygen != RAPTR_RAYGEN_RAND && raygen != RAPTR_RAYGEN_REGU &&
+      raygen != RAPTR_RAYGEN_RAND_ELEMENT)
+    return fail(RAPTR_ERR_INVALID_ARG, "unknown ray generator %d", raygen);
   if (wq_length == 0) return RAPTR_OK;
   if (!d_cnl || !d_vxl || !d_val) return fail(RAPTR_ERR_INVALID_ARG, "null buffer");
   hipLaunchKernelGGL(k_sm_from_wq, dim3(grid_for(wq_length, 256, 1 << 16)), dim3(256), 0,
@@ -556,6 +579,8 @@ int raptr_generate_rays(const raptr_setup_t* setup, const int* d_cnl, int n_cnl,
   raptr_grid_t g = {{0, 0, 0}, {1, 1, 1}, {1, 1, 1}};
   Geom G;
   RAPTR_TRY(make_geom(setup, &g, &G));
+  if (raygen != RAPTR_RAYGEN_RAND && raygen != RAPTR_RAYGEN_REGU)
+    return fail(RAPTR_ERR_INVALID_ARG, "ray listings take RAND or REGU (got %d)", raygen);
   if (n_cnl <= 0 || nrays <= 0) return RAPTR_OK;
   hipLaunchKernelGGL(k_generate_rays, dim3(grid_for(int64_t(n_cnl) * nrays, 256, 1 << 16)),
                      dim3(256), 0, S(stream), G, d_cnl, n_cnl, nrays, seed, raygen, d_rays);
@@ -576,13 +601,26 @@ int raptr_sm_build_csr(const raptr_setup_t* setup, const raptr_grid_t* grid,
                        int raygen, int64_t* d_row_ptr, int* d_col, float* d_val,
                        int64_t capacity, int64_t* nnz, void* d_ws, size_t ws_bytes,
                        raptr_stream_t stream) {
+  return raptr_sm_build_csr_ex(setup, grid, d_list, list_size, nrays, seed, raygen, 0,
+                               d_row_ptr, d_col, d_val, capacity, nnz, d_ws, ws_bytes,
+                               stream);
+}
+
+int raptr_sm_build_csr_ex(const raptr_setup_t* setup, const raptr_grid_t* grid,
+                          const int* d_list, int list_size, int nrays, uint32_t seed,
+                          int raygen, int flags, int64_t* d_row_ptr, int* d_col, float* d_val,
+                          int64_t capacity, int64_t* nnz, void* d_ws, size_t ws_bytes,
+                          raptr_stream_t stream) {
   if (!nnz || !d_row_ptr) return fail(RAPTR_ERR_INVALID_ARG, "null output");
   *nnz = 0;
   if (list_size < 0 || (list_size > 0 && !d_list))
     return fail(RAPTR_ERR_INVALID_ARG, "bad list");
+  if (flags & ~int(RAPTR_SM_KEEP_ZEROS))
+    return fail(RAPTR_ERR_INVALID_ARG, "unknown build flags 0x%x", flags);
   BuildArgs A;
   size_t lds = 0;
   RAPTR_TRY(make_build_args(setup, grid, nrays, seed, raygen, &A, &lds));
+  A.keep_zeros = (flags & RAPTR_SM_KEEP_ZEROS) ? 1 : 0;
   hipStream_t st = S(stream);
   const bool fill = d_col != nullptr;
   if (fill && !d_val) return fail(RAPTR_ERR_INVALID_ARG, "null values");

@@ -112,11 +112,14 @@ DEFAULT_STAGING = 64 << 20  # staging entries of the chunked build
 
 
 def build_csr(setup, grid, cnl_list, nrays, seed=1, raygen=capi.RAYGEN_RAND, stream=None,
-              staging=DEFAULT_STAGING, capacity=None, d_list=None):
-    """Matrix of the listed channels (rows in list order), zeros dropped.
+              staging=DEFAULT_STAGING, capacity=None, d_list=None, keep_zeros=False):
+    """Matrix of the listed channels (rows in list order), zeros dropped
+    (keep_zeros: every culled candidate stored, as the reference's fused
+    kernel does; raygen RAYGEN_RAND_ELEMENT: rays per element).
     capacity: None = exact count pass first; "auto" = estimate from a count
     pass over every 64th row (falls back to the exact pass if exceeded);
     an int = caller's bound."""
+    flags = capi.SM_KEEP_ZEROS if keep_zeros else 0
     lst = np.ascontiguousarray(cnl_list, np.int32)
     m = len(lst)
     if d_list is None:
@@ -133,31 +136,31 @@ def build_csr(setup, grid, cnl_list, nrays, seed=1, raygen=capi.RAYGEN_RAND, str
             sample = lst[::64].copy()
             d_s = DeviceArray.from_numpy(sample)
             rps = DeviceArray(len(sample) + 1, np.int64)
-            capi.call("raptr_sm_build_csr", C.byref(s), C.byref(g), d_s.ptr, len(sample),
-                      int(nrays), int(seed), int(raygen), rps.ptr, None, None, 0, C.byref(nnz),
+            capi.call("raptr_sm_build_csr_ex", C.byref(s), C.byref(g), d_s.ptr, len(sample),
+                      int(nrays), int(seed), int(raygen), flags, rps.ptr, None, None, 0, C.byref(nnz),
                       ws.ptr, ws.nbytes, stream)
             est = nnz.value * m / len(sample)
             capacity = int(est * 1.03) + 65536
             col = DeviceArray(capacity, np.int32)
             val = DeviceArray(capacity, np.float32)
-            st = L.raptr_sm_build_csr(C.byref(s), C.byref(g), d_list.ptr, m, int(nrays),
-                                      int(seed), int(raygen), rp.ptr, col.ptr, val.ptr,
+            st = L.raptr_sm_build_csr_ex(C.byref(s), C.byref(g), d_list.ptr, m, int(nrays),
+                                         int(seed), int(raygen), flags, rp.ptr, col.ptr, val.ptr,
                                       int(capacity), C.byref(nnz), ws.ptr, ws.nbytes, stream)
             if st == capi.RAPTR_OK:
                 return DeviceCSR(m, grid.nvox, nnz.value, rp, col, val, d_list)
             if st != capi.RAPTR_ERR_WORKSPACE:
-                capi.check(st, "raptr_sm_build_csr")
+                capi.check(st, "raptr_sm_build_csr_ex")
             del col, val
             capacity = None
     if capacity is None:
-        capi.call("raptr_sm_build_csr", C.byref(s), C.byref(g), d_list.ptr, m, int(nrays),
-                  int(seed), int(raygen), rp.ptr, None, None, 0, C.byref(nnz), ws.ptr,
+        capi.call("raptr_sm_build_csr_ex", C.byref(s), C.byref(g), d_list.ptr, m, int(nrays),
+                  int(seed), int(raygen), flags, rp.ptr, None, None, 0, C.byref(nnz), ws.ptr,
                   ws.nbytes, stream)
         capacity = nnz.value
     col = DeviceArray(max(capacity, 1), np.int32)
     val = DeviceArray(max(capacity, 1), np.float32)
-    capi.call("raptr_sm_build_csr", C.byref(s), C.byref(g), d_list.ptr, m, int(nrays),
-              int(seed), int(raygen), rp.ptr, col.ptr, val.ptr, int(capacity), C.byref(nnz),
+    capi.call("raptr_sm_build_csr_ex", C.byref(s), C.byref(g), d_list.ptr, m, int(nrays),
+              int(seed), int(raygen), flags, rp.ptr, col.ptr, val.ptr, int(capacity), C.byref(nnz),
               ws.ptr, ws.nbytes, stream)
     return DeviceCSR(m, grid.nvox, nnz.value, rp, col, val, d_list)
 

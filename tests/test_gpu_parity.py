@@ -231,6 +231,52 @@ def test_build_csr_bit_exact(gpu, oracle, cfg):
     np.testing.assert_array_equal(B.to_host()[2], val)
 
 
+@pytest.mark.parametrize("cfg", ["tiny", "small"])
+def test_build_csr_k2_modes_bit_exact(gpu, oracle, cfg):
+    """The fused kernel's options: zero-valued candidates kept
+    (getSystemMatrixDeviceOnly.cu:507-509) and per-element ray streams
+    (:396), alone and together, equal the oracle bit for bit; with zeros kept
+    the pairs are exactly the workqueue's."""
+    from raptr_amd import sysmat
+    setup, grid, nrays = G.config_tiny() if cfg == "tiny" else G.config_small()
+    s, g = oracle_geom(oracle, setup, grid)
+    rng = np.random.default_rng(5)
+    lst = np.sort(rng.choice(setup.nchannels, 300, replace=False)).astype(np.int32)
+    cw, vw, _, _ = oracle.workqueue(s, g, lst)
+    for raygen, keep in ((oracle.RAYGEN_RAND, True), (oracle.RAYGEN_RAND_ELEMENT, False),
+                         (oracle.RAYGEN_RAND_ELEMENT, True), (oracle.RAYGEN_REGU, True)):
+        rp, col, val, _ = oracle.build_csr(s, g, lst, nrays, raygen=raygen,
+                                           flags=oracle.SM_KEEP_ZEROS if keep else 0)
+        A = sysmat.build_csr(setup, grid, lst, nrays, raygen=raygen, keep_zeros=keep,
+                             staging=1 << 16)
+        grp, gcol, gval = A.to_host()
+        np.testing.assert_array_equal(grp, rp)
+        np.testing.assert_array_equal(gcol, col)
+        np.testing.assert_array_equal(gval, val)
+        if keep:
+            assert A.nnz == len(vw)
+            np.testing.assert_array_equal(gcol, vw)
+    # two-stage path in element mode
+    got = sysmat.sm_from_wq(setup, grid, cw, vw, nrays, raygen=oracle.RAYGEN_RAND_ELEMENT)
+    want = oracle.sm_from_wq(s, g, cw, vw, nrays, raygen=oracle.RAYGEN_RAND_ELEMENT)
+    np.testing.assert_array_equal(got, want)
+
+
+def test_build_keep_zeros_small_is_reference_workqueue(gpu):
+    """All 20,480 channels with zeros kept: nnz and the SHA-256 of the
+    (row channel, column) pairs equal the reference's getWorkqueue output
+    (57,119,676 pairs), i.e. the reference's nnz."""
+    from raptr_amd import sysmat
+    gold = load_golden("wq_small.npz")
+    setup, grid, nrays = G.config_small()
+    lst = np.arange(setup.nchannels, dtype=np.int32)
+    A = sysmat.build_csr(setup, grid, lst, nrays, keep_zeros=True)
+    assert A.nnz == int(gold["total"])
+    rp, col, _ = A.to_host()
+    np.testing.assert_array_equal(np.diff(rp), gold["counts"])
+    assert sha(np.repeat(lst, np.diff(rp)), col) == str(gold["sha256"])
+
+
 def test_build_csr_chunking_independent(gpu):
     """Small staging areas force many chunks and piece flushes; result equal."""
     from raptr_amd import sysmat

@@ -268,6 +268,93 @@ def test_workqueue_is_conservative(oracle):
         assert int(allc[k]) * grid.nvox + int(allv[k]) in cand
 
 
+# ------------------------------- fused-kernel semantics (K2): zeros, streams
+def test_keep_zeros_build_is_the_reference_workqueue(oracle):
+    """With zero-valued candidates kept (getSystemMatrixDeviceOnly.cu:507-509)
+    the matrix's (row, column) pairs are exactly the reference's getWorkqueue
+    output (the 768-channel golden list, all 369,968 pairs), so nnz equals the
+    workqueue length as in the reference; the non-zero values are those of the
+    zero-dropping build."""
+    setup, grid, nrays = G.config_tiny()
+    s, g = oracle_geom(oracle, setup, grid)
+    gold = load_golden("wq_tiny.npz")
+    lst = np.arange(setup.nchannels, dtype=np.int32)
+    rp, col, val, _ = oracle.build_csr(s, g, lst, nrays, flags=oracle.SM_KEEP_ZEROS)
+    assert rp[-1] == len(gold["cnl"])
+    np.testing.assert_array_equal(np.repeat(lst, np.diff(rp)), gold["cnl"])
+    np.testing.assert_array_equal(col, gold["vxl"])
+    rp0, col0, val0, _ = oracle.build_csr(s, g, lst, nrays)
+    nz = val != 0
+    np.testing.assert_array_equal(col[nz], col0)
+    np.testing.assert_array_equal(val[nz], val0)
+    assert np.count_nonzero(~nz) > 0
+
+
+def test_element_rays_are_independent_streams(oracle):
+    """RAND_ELEMENT: each (channel, voxel) element draws its own rays
+    (Philox counter word 3 = voxel + 1); word 3 = 0 is the per-channel stream,
+    and every element ray still lies in the channel's two pixel volumes."""
+    setup, grid, nrays = G.config_small()
+    s, _ = oracle_geom(oracle, setup, grid)
+    for cnl in (0, 777, 20479):
+        ida, id0z, id0y, id1z, id1y = (int(v) for v in setup.split(cnl))
+        for r in range(3):
+            np.testing.assert_array_equal(
+                oracle.ray_element(s, cnl, -1, r, raygen=oracle.RAYGEN_RAND_ELEMENT),
+                oracle.ray(s, cnl, r))
+            a = oracle.ray_element(s, cnl, 5, r)
+            b = oracle.ray_element(s, cnl, 6, r)
+            assert not np.array_equal(a, b)
+            # RAND with an element argument is the per-channel stream
+            np.testing.assert_array_equal(
+                oracle.ray_element(s, cnl, 5, r, raygen=oracle.RAYGEN_RAND),
+                oracle.ray(s, cnl, r))
+            # end points inside the pixel boxes: the inverse trafo maps them
+            # into the unit cube
+            for det, (iz, iy), p in ((0, (id0z, id0y), a[:3]), (1, (id1z, id1y), a[3:])):
+                o = oracle.trafo(s, det, ida, iz, iy, np.zeros(3, np.float32))
+                ex = oracle.trafo(s, det, ida, iz, iy, np.array([1, 0, 0], np.float32)) - o
+                ey = oracle.trafo(s, det, ida, iz, iy, np.array([0, 1, 0], np.float32)) - o
+                ez = oracle.trafo(s, det, ida, iz, iy, np.array([0, 0, 1], np.float32)) - o
+                u = np.linalg.solve(np.stack([ex, ey, ez], 1).astype(np.float64),
+                                    (p - o).astype(np.float64))
+                assert np.all(u > -1e-5) and np.all(u < 1 + 1e-5)
+
+
+def test_element_values_vs_fp64_and_expectation(oracle):
+    """RAND_ELEMENT matrix values: within 1e-5 of the FP64 restatement on
+    identical rays, equal to the slab mean of the element's own rays, and,
+    averaged over many rows, the same row sums as per-channel streams (both
+    estimate the mean in-grid path length; per-element sums scatter)."""
+    setup, grid, nrays = G.config_small()
+    s, g = oracle_geom(oracle, setup, grid)
+    chans = np.arange(0, setup.nchannels, 41, dtype=np.int32)
+    c, v, _, _ = oracle.workqueue(s, g, chans)
+    E = oracle.RAYGEN_RAND_ELEMENT
+    a32 = oracle.sm_from_wq(s, g, c, v, nrays, raygen=E)
+    a64 = oracle.sm_from_wq(s, g, c, v, nrays, raygen=E, f64=True)
+    assert np.all(np.abs(a32 - a64) <= 1e-5 * np.abs(a64) + 1e-5 * grid.d[0])
+    for w in (0, 1234, len(c) - 1):
+        box = oracle.voxel_box(g, int(v[w]))
+        want = np.float32(0)
+        for r in range(nrays):
+            want = np.float32(want + oracle.intersection_length(
+                box, oracle.ray_element(s, int(c[w]), int(v[w]), r)))
+        assert a32[w] == np.float32(want / np.float32(nrays))
+    a_ch = oracle.sm_from_wq(s, g, c, v, nrays)
+    rows_e = np.bincount(np.searchsorted(chans, c), weights=a32)
+    rows_c = np.bincount(np.searchsorted(chans, c), weights=a_ch)
+    assert not np.allclose(rows_e, rows_c, rtol=1e-3)
+    assert rows_e.sum() == pytest.approx(rows_c.sum(), rel=5e-3)
+    # the build in element mode equals the two-stage values, zeros dropped
+    lst = chans[:60]
+    rp, col, val, _ = oracle.build_csr(s, g, lst, nrays, raygen=E)
+    cc, vv, _, _ = oracle.workqueue(s, g, lst)
+    ae = oracle.sm_from_wq(s, g, cc, vv, nrays, raygen=E)
+    np.testing.assert_array_equal(col, vv[ae != 0])
+    np.testing.assert_array_equal(val, ae[ae != 0])
+
+
 # --------------------------------------------------- slab length (A9)
 def test_intersection_length_closed_forms(oracle):
     box = np.array([0, 0, 0, 2, 2, 2], np.float32)
