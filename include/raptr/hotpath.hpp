@@ -12,7 +12,12 @@
 //   convertCsr2Ecsr            convertCsr2Ecsr.hpp:51-64
 //   CSRmv<T>                   csrmv.hpp:37-99
 //   divides / dividesMultiplies / scales / sum   mlemOperations.hpp:44-169
-//   systemMatrixCalculation    wrappers.hpp:185-266
+//   systemMatrixCalculation    wrappers.hpp:185-266 (the reference's
+//                              template and parameter list, geometry from
+//                              grid_const / setup_const / nrays_const)
+//   RandRayGen / ReguRayGen    RayGenerators.hpp:36-99
+//   grid_const / setup_const / nrays_const   device_constant_memory.hpp:34-46
+//                              (host variables here)
 //   nInChunk / chunkPtr / nChunks   wrappers.hpp:429-464
 //
 // Error behaviour follows the reference: device errors throw
@@ -90,6 +95,133 @@ void require_default_geometry() {
                 "the gfx950 kernels implement DefaultVoxelGrid<float>");
 }
 
+}  // namespace raptr
+
+// ----------------------------------------------- device constant memory
+// device_constant_memory.hpp:34-46 keeps the voxel grid, the measurement
+// setup and the ray count in __constant__ memory; the drivers set them with
+// cudaMemcpyToSymbol (reco.cu:85-93) and getSystemMatrix and the ray
+// generators read them.  Here they are host variables with the same names,
+// read by the calls that take no geometry argument (systemMatrixCalculation
+// and the RayGen functors); a driver sets them by assignment or with
+// raptr::setConstant.  The geometry is copied into every launch's
+// arguments, so no device state exists and calls may run concurrently on
+// different streams as long as these variables are not changed meanwhile.
+inline DefaultVoxelGrid<float> grid_const;
+inline DefaultMeasurementSetup<float> setup_const;
+inline int nrays_const = 1;
+
+namespace raptr {
+// Generator key of systemMatrixCalculation's rays (the reference compiles
+// RANDOM_SEED = 1251 into its kernels, getSystemMatrixDeviceOnly.cu:43; the
+// workloads of this project use 1).
+inline uint32_t random_seed = 1;
+
+template <typename T>
+void setConstant(T& symbol, const T& value) {
+  symbol = value;
+}
+
+namespace detail {
+// Philox4x32-10 (Salmon et al., SC'11), the counter-based generator of every
+// ray of this project; u01 maps a 32-bit draw to (0, 1].
+inline void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
+                          uint32_t k1, uint32_t* out) {
+  for (int r = 0; r < 10; ++r) {
+    if (r) {
+      k0 += 0x9E3779B9u;
+      k1 += 0xBB67AE85u;
+    }
+    const uint64_t p0 = uint64_t(0xD2511F53u) * c0;
+    const uint64_t p1 = uint64_t(0xCD9E8D57u) * c2;
+    const uint32_t hi0 = uint32_t(p0 >> 32), lo0 = uint32_t(p0);
+    const uint32_t hi1 = uint32_t(p1 >> 32), lo1 = uint32_t(p1);
+    c0 = hi1 ^ c1 ^ k0;
+    c1 = lo1;
+    c2 = hi0 ^ c3 ^ k1;
+    c3 = lo0;
+  }
+  out[0] = c0;
+  out[1] = c1;
+  out[2] = c2;
+  out[3] = c3;
+}
+inline float u01(uint32_t x) { return float((x >> 8) + 1u) * (1.0f / 16777216.0f); }
+}  // namespace detail
+}  // namespace raptr
+
+// ------------------------------------------------------- ray generators
+// RandRayGen (RayGenerators.hpp:36-67): rays with end points uniform in the
+// two pixel volumes.  The reference draws six cuRAND XORWOW uniforms from
+// the stream curand_init(seed, setNr, 0); here the k-th call of a generator
+// constructed with (seed, setNr) draws them from Philox4x32-10 with key
+// (seed, 0) and counters (setNr, k, 0|1, element + 1), element = -1 unless
+// set with setElement.  With setNr = channel id the k-th ray is the k-th
+// RAPTR_RAYGEN_RAND ray of that channel (raptr_generate_rays,
+// raptr_sm_from_wq); with an element set, the RAPTR_RAYGEN_RAND_ELEMENT ray
+// of that (channel, voxel) pair.  The trafos read setup_const, as the
+// reference's do.
+template <typename T, typename MSTrafo0_inplace, typename MSTrafo1_inplace>
+struct RandRayGen {
+  RandRayGen(int const seed, int const setNr)
+      : key_(uint32_t(seed)), set_(uint32_t(setNr)), word3_(0u), k_(0u) {}
+
+  void setElement(int const element) { word3_ = uint32_t(element) + 1u; }
+
+  void operator()(T* const ray, int const& id0z, int const& id0y, int const& id1z,
+                  int const& id1y, int const& ida, MSTrafo0_inplace const& trafo0,
+                  MSTrafo1_inplace const& trafo1) {
+    uint32_t r0[4], r1[4];
+    raptr::detail::philox4x32_10(set_, k_, 0u, word3_, key_, 0u, r0);
+    raptr::detail::philox4x32_10(set_, k_, 1u, word3_, key_, 0u, r1);
+    ++k_;
+    ray[0] = T(raptr::detail::u01(r0[0]));
+    ray[1] = T(raptr::detail::u01(r0[1]));
+    ray[2] = T(raptr::detail::u01(r0[2]));
+    ray[3] = T(raptr::detail::u01(r0[3]));
+    ray[4] = T(raptr::detail::u01(r1[0]));
+    ray[5] = T(raptr::detail::u01(r1[1]));
+    trafo0(&ray[0], id0z, id0y, ida, &setup_const);
+    trafo1(&ray[3], id1z, id1y, ida, &setup_const);
+  }
+
+ private:
+  uint32_t key_, set_, word3_, k_;
+};
+
+// ReguRayGen (RayGenerators.hpp:73-99): pixel-centre rays (RAPTR_RAYGEN_REGU).
+template <typename T, typename MSTrafo0_inplace, typename MSTrafo1_inplace>
+struct ReguRayGen {
+  ReguRayGen(int const seed, int const setNr) {
+    (void)seed;
+    (void)setNr;
+  }
+
+  void operator()(T* const ray, int const& id0z, int const& id0y, int const& id1z,
+                  int const& id1y, int const& ida, MSTrafo0_inplace const& trafo0,
+                  MSTrafo1_inplace const& trafo1) const {
+    for (int i = 0; i < 6; ++i) ray[i] = T(0.5);
+    trafo0(&ray[0], id0z, id0y, ida, &setup_const);
+    trafo1(&ray[3], id1z, id1y, ida, &setup_const);
+  }
+};
+
+namespace raptr {
+// The device generator a RayGen functor type selects in
+// systemMatrixCalculation: RandRayGen -> per-element streams (the reference's
+// getSystemMatrix gives each thread its own generator,
+// getSystemMatrixDeviceOnly.cu:396, so every element draws its own rays),
+// ReguRayGen -> pixel centres.
+template <typename RayGen>
+struct raygen_kind;
+template <typename T, typename A, typename B>
+struct raygen_kind<RandRayGen<T, A, B> > {
+  static constexpr int value = RAPTR_RAYGEN_RAND_ELEMENT;
+};
+template <typename T, typename A, typename B>
+struct raygen_kind<ReguRayGen<T, A, B> > {
+  static constexpr int value = RAPTR_RAYGEN_REGU;
+};
 }  // namespace raptr
 
 // ------------------------------------------------------------ workqueue
@@ -324,11 +456,68 @@ ChunkGridSizeType nChunks(MemArrSizeType const maxNnz, MemArrSizeType const maxN
 }
 
 // ------------------------------------------- system matrix of a chunk
-// The matrix of the nY listed channels (device yRowId_devi) as ECSR:
-// aEcsrCnlPtr_devi[nY+1] (int32 row pointers), aVxlId_devi / aVal_devi
-// (capacity entries), *nnz_host receives the count.  Zero-valued candidates
-// are not stored (the reference stores them); the non-zero values are
-// identical.
+// The reference's call (wrappers.hpp:185-266, called at reco.cu:246-254,
+// backprojection.cu:145-152, pureSMCalculation.cu:149-155): the matrix of
+// the *nY_host channels yRowId_devi[0..nY) (ascending channel ids) with the
+// geometry of grid_const / setup_const / nrays_const and the generator of
+// the RayGen type (raygen_kind), every culled candidate stored (zero-valued
+// ones too, getSystemMatrixDeviceOnly.cu:507-509), so the count is the
+// workqueue's.  Entries go to aVxlId_devi / aVal_devi after the first
+// *nnz_devi entries (device in/out accumulator: it receives its initial
+// value plus the count, wrappers.hpp:172-174), rows in list order, columns
+// ascending (the cooSort order).  Also written, in the reference's int32
+// types: aEcsrCnlPtr_devi[0..nY] (ECSR row pointers), aCnlId_devi (COO row
+// ids of the new entries) and aCsrCnlPtr_devi[0..NCHANNELS] (row pointer
+// over the channel space).  The arrays must hold the entries, as in the
+// reference (LIMNNZ sizing, reco.cu:47-51).  Work goes on the handle's
+// stream and is complete on return.  Scratch (the 64-bit row pointers and
+// the build's staging) is allocated and freed here, as the reference
+// allocates its nY_devi temporary.
+template <typename T, typename ListSizeType, typename GridSizeType, typename MemArrSizeType,
+          typename RayGen>
+void systemMatrixCalculation(int* const aEcsrCnlPtr_devi, GridSizeType* const aVxlId_devi,
+                             T* const aVal_devi, MemArrSizeType* const nnz_devi,
+                             int* const aCnlId_devi, int* const aCsrCnlPtr_devi,
+                             int* const yRowId_devi, ListSizeType const* const& nY_host,
+                             raptr_sparse_handle_t const& handle) {
+  static_assert(std::is_same<T, float>::value, "matrix values are float");
+  static_assert(sizeof(GridSizeType) == sizeof(int) && std::is_integral<GridSizeType>::value,
+                "voxel ids are 32-bit integers");
+  static_assert(std::is_integral<MemArrSizeType>::value, "nnz counter is an integer");
+  const raptr_stream_t st = handle ? handle->stream : nullptr;
+  const int nY = int(*nY_host);
+  raptr_setup_t s = setup_const.to_c();
+  raptr_grid_t g = grid_const.to_c();
+  MemArrSizeType nnz0 = 0;
+  raptr::check(raptr_memcpy_d2h(&nnz0, nnz_devi, sizeof(MemArrSizeType), st), "d2h nnz");
+  raptr::check(raptr_stream_sync(st), "stream sync");
+  raptr::DeviceBuffer<int64_t> rp(size_t(nY) + 1);
+  const size_t wsb = raptr_sm_build_workspace_size(&g, nrays_const, int64_t(1) << 22);
+  raptr::DeviceBuffer<char> ws(wsb);
+  const int kind = raptr::raygen_kind<RayGen>::value;
+  int64_t count = 0;
+  raptr::check(raptr_sm_build_csr_ex(&s, &g, yRowId_devi, nY, nrays_const, raptr::random_seed,
+                                     kind, RAPTR_SM_KEEP_ZEROS, rp.get(), nullptr, nullptr, 0,
+                                     &count, ws.get(), wsb, st),
+               "raptr_sm_build_csr_ex (count)");
+  int64_t filled = 0;
+  raptr::check(raptr_sm_build_csr_ex(&s, &g, yRowId_devi, nY, nrays_const, raptr::random_seed,
+                                     kind, RAPTR_SM_KEEP_ZEROS, rp.get(),
+                                     reinterpret_cast<int*>(aVxlId_devi) + nnz0, aVal_devi + nnz0,
+                                     count, &filled, ws.get(), wsb, st),
+               "raptr_sm_build_csr_ex (fill)");
+  raptr::check(raptr_sm_index_arrays(rp.get(), yRowId_devi, nY, int64_t(nnz0),
+                                     setup_const.nChannels(), aEcsrCnlPtr_devi, aCnlId_devi,
+                                     aCsrCnlPtr_devi, st),
+               "raptr_sm_index_arrays");
+  const MemArrSizeType nnz1 = nnz0 + MemArrSizeType(filled);
+  raptr::check(raptr_memcpy_h2d(nnz_devi, &nnz1, sizeof(MemArrSizeType), st), "h2d nnz");
+  raptr::check(raptr_stream_sync(st), "stream sync");
+}
+
+// The same matrix with explicit geometry and exact zeros dropped (the
+// compact form the reconstruction plans are built from), int32 ECSR row
+// pointers, *nnz_host = count.
 inline void systemMatrixCalculation(int* aEcsrCnlPtr_devi, int* aVxlId_devi, float* aVal_devi,
                                     int64_t capacity, int64_t* nnz_host,
                                     const int* yRowId_devi, int nY,

@@ -223,6 +223,21 @@ int raptr_coo2csr(int* d_row_ptr, const int* d_row_id, int64_t nnz, int m,
  * ecsr[i] = csr[y_row_id[i]] (i < nnzr), ecsr[nnzr] = csr[m]. */
 int raptr_csr2ecsr(int* d_ecsr, const int* d_y_row_id, int nnzr,
                    const int* d_csr, int m, raptr_stream_t stream);
+/* The index arrays the reference's systemMatrixCalculation leaves
+ * (wrappers.hpp:185-266), in its int32 types, for a matrix of the listed
+ * channels d_y_row_id[0..m) (ascending) built by raptr_sm_build_csr[_ex]
+ * into arrays whose first `base` entries were already taken:
+ *   d_ecsr[i] = base + d_row_ptr[i], i <= m (the ECSR row pointers,
+ *     convertCsr2Ecsr.hpp:51-64);
+ *   d_coo_row[base + k] = channel of entry k (the COO row ids cooSort
+ *     leaves, cooSort.hpp:75-99);
+ *   d_csr[c] = base + entries of listed channels < c, c <= nchannels (the
+ *     CSR row pointer over the channel space, cusparseWrapper.hpp:53-60).
+ * Null outputs are skipped; RAPTR_ERR_INVALID_ARG if base + nnz exceeds
+ * int32. */
+int raptr_sm_index_arrays(const int64_t* d_row_ptr, const int* d_y_row_id, int m, int64_t base,
+                          int nchannels, int* d_ecsr, int* d_coo_row, int* d_csr,
+                          raptr_stream_t stream);
 /* CSR (64-bit row pointers) -> CSR of the transpose, stable: entries of
  * each transposed row keep ascending original-row order. */
 size_t raptr_csr_transpose_workspace_size(int64_t nnz, int m, int n);

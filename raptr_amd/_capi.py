@@ -100,6 +100,7 @@ _SIGS = {
     "raptr_coo_sort": (i32, [vp, vp, vp, i64, vp, sz, vp]),
     "raptr_coo2csr": (i32, [vp, vp, i64, i32, vp]),
     "raptr_csr2ecsr": (i32, [vp, vp, i32, vp, i32, vp]),
+    "raptr_sm_index_arrays": (i32, [vp, vp, i32, i64, i32, vp, vp, vp, vp]),
     "raptr_csr_transpose_workspace_size": (sz, [i64, i32, i32]),
     "raptr_csr_transpose": (i32, [i32, i32, i64, vp, vp, vp, vp, vp, vp, vp, sz, vp]),
     "raptr_csrmv_f32": (i32, [i32, i32, i32, i64, f32, vp, vp, vp, vp, f32, vp, vp]),

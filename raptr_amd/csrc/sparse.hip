@@ -196,6 +196,41 @@ __global__ void k_csr2ecsr(int* ecsr, const int* y, int nnzr, const int* csr, in
   if (i == nnzr) ecsr[nnzr] = csr[m];
 }
 
+// ----------------------- reference index arrays of a built chunk (int32)
+// ecsr[i] = base + row_ptr[i] (i <= m)
+__global__ void k_ecsr32(const int64_t* row_ptr, int m, int64_t base, int* ecsr) {
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i <= m;
+       i += int64_t(gridDim.x) * blockDim.x)
+    ecsr[i] = int(base + row_ptr[i]);
+}
+
+// coo[base + k] = y[i] for the entries k of row i (one wave per row)
+__global__ void k_coo_rows_of(const int64_t* row_ptr, const int* y, int m, int64_t base,
+                              int* coo) {
+  int64_t wid = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
+  int lane = threadIdx.x & (kWave - 1);
+  int64_t nw = int64_t(gridDim.x) * blockDim.x / kWave;
+  for (int64_t i = wid; i < m; i += nw) {
+    const int c = y[i];
+    for (int64_t k = row_ptr[i] + lane; k < row_ptr[i + 1]; k += kWave) coo[base + k] = c;
+  }
+}
+
+// csr[c] = base + row_ptr[#listed channels < c] (c <= nch): the row pointer
+// over the whole channel space that coo2csr gives for these entries
+__global__ void k_csr_full(const int64_t* row_ptr, const int* y, int m, int64_t base, int nch,
+                           int* csr) {
+  for (int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; c <= nch;
+       c += int64_t(gridDim.x) * blockDim.x) {
+    int lo = 0, hi = m;
+    while (lo < hi) {
+      const int mid = lo + ((hi - lo) >> 1);
+      if (int64_t(y[mid]) < c) lo = mid + 1; else hi = mid;
+    }
+    csr[c] = int(base + row_ptr[lo]);
+  }
+}
+
 // --------------------------------------------------------- transpose
 __global__ void k_t_keys(const int* col, int64_t nnz, uint64_t* keys, uint32_t* idx) {
   for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < nnz;
@@ -314,6 +349,37 @@ int raptr_csr2ecsr(int* d_ecsr, const int* d_y, int nnzr, const int* d_csr, int 
   hipLaunchKernelGGL(k_csr2ecsr, dim3(grid_for(int64_t(nnzr) + 1, 256)), dim3(256), 0,
                      S(stream), d_ecsr, d_y, nnzr, d_csr, m);
   RAPTR_LAUNCH_CHECK();
+  return RAPTR_OK;
+}
+
+int raptr_sm_index_arrays(const int64_t* d_row_ptr, const int* d_y_row_id, int m, int64_t base,
+                          int nchannels, int* d_ecsr, int* d_coo_row, int* d_csr,
+                          raptr_stream_t stream) {
+  if (m < 0 || base < 0 || nchannels < 0) return fail(RAPTR_ERR_INVALID_ARG, "negative size");
+  if (!d_row_ptr || (m > 0 && !d_y_row_id)) return fail(RAPTR_ERR_INVALID_ARG, "null buffer");
+  int64_t last = 0;
+  RAPTR_HIP_TRY(hipMemcpyAsync(&last, d_row_ptr + m, sizeof(int64_t), hipMemcpyDeviceToHost,
+                               S(stream)));
+  RAPTR_HIP_TRY(hipStreamSynchronize(S(stream)));
+  if (base + last > INT32_MAX)
+    return fail(RAPTR_ERR_INVALID_ARG, "%lld entries exceed int32 row pointers",
+                (long long)(base + last));
+  if (d_ecsr) {
+    hipLaunchKernelGGL(k_ecsr32, dim3(grid_for(int64_t(m) + 1, 256)), dim3(256), 0, S(stream),
+                       d_row_ptr, m, base, d_ecsr);
+    RAPTR_LAUNCH_CHECK();
+  }
+  if (d_coo_row && m > 0) {
+    hipLaunchKernelGGL(k_coo_rows_of, dim3(grid_for(int64_t(m) * kWave, 256, 1 << 16)),
+                       dim3(256), 0, S(stream), d_row_ptr, d_y_row_id, m, base, d_coo_row);
+    RAPTR_LAUNCH_CHECK();
+  }
+  if (d_csr) {
+    hipLaunchKernelGGL(k_csr_full, dim3(grid_for(int64_t(nchannels) + 1, 256, 1 << 16)),
+                       dim3(256), 0, S(stream), d_row_ptr, d_y_row_id, m, base, nchannels,
+                       d_csr);
+    RAPTR_LAUNCH_CHECK();
+  }
   return RAPTR_OK;
 }
 

@@ -9,6 +9,11 @@
 //   test_getWorkqueue.cu:124-219     bounded queue = prefix of unbounded
 //   test_MeasurementSetupLinIndex.cu:57-66, test_VoxelGridLinIndex.cu:57-64
 //   MeasurementSetupTrafo2CartCoordUT.cu:91-137, 309-320
+//   RayGenerators.hpp:36-99          RandRayGen / ReguRayGen against the
+//                                    device's rays of the same streams
+//   reco.cu:238-259                  systemMatrixCalculation called with
+//                                    the reference's template list and
+//                                    arguments, then CSRmv on its ECSR
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -18,6 +23,8 @@
 #include <vector>
 
 #include "raptr/hotpath.hpp"
+
+using raptr::check;
 
 static int g_fail = 0;
 #define EXPECT(cond)                                                             \
@@ -42,6 +49,10 @@ typedef DefaultMeasurementSetupId1y<MS> Id1y;
 typedef DefaultMeasurementSetupIda<MS> Ida;
 typedef DefaultMeasurementSetupTrafo2CartCoordFirstPixel<val_t, MS> Trafo0;
 typedef DefaultMeasurementSetupTrafo2CartCoordSecndPixel<val_t, MS> Trafo1;
+typedef DefaultMeasurementSetupTrafo2CartCoordFirstPixel_inplace<val_t, MS> Trafo0_inplace;
+typedef DefaultMeasurementSetupTrafo2CartCoordSecndPixel_inplace<val_t, MS> Trafo1_inplace;
+typedef uint32_t ListSizeType;                   // typedefs_array_sizes.hpp:44
+typedef unsigned long long int MemArrSizeType;   // typedefs_array_sizes.hpp:36
 
 // ---------------------------------------------------------------- host
 static void test_index_round_trips() {
@@ -242,6 +253,176 @@ static void test_workqueue_prefix() {
   EXPECT(nz > 0);
 }
 
+// RayGenerators.hpp:42-63 usage: a generator per (seed, setNr), one call per
+// ray; with setNr = channel its rays are the device's RAND rays of that
+// channel, bit for bit; ReguRayGen gives the pixel centres.
+static void test_ray_generators() {
+  MS ms(-230, 230, 5, 8, 8, 8, 8, 36, 20, 4, 4);
+  raptr::setConstant(setup_const, ms);
+  const int nrays = 10;
+  const std::vector<int> chans = {0, 1, 777, 12345, ms.nChannels() - 1};
+  raptr::DeviceBuffer<int> dc(chans.size());
+  dc.upload(chans.data(), chans.size());
+  raptr::DeviceBuffer<val_t> dr(chans.size() * nrays * 6);
+  raptr_setup_t s = ms.to_c();
+  std::vector<val_t> dev(chans.size() * nrays * 6);
+  Trafo0_inplace trafo0;
+  Trafo1_inplace trafo1;
+  for (int kind : {RAPTR_RAYGEN_RAND, RAPTR_RAYGEN_REGU}) {
+    raptr::check(raptr_generate_rays(&s, dc.get(), int(chans.size()), nrays, 1, kind, dr.get(),
+                                     nullptr),
+                 "raptr_generate_rays");
+    dr.download(dev.data(), dev.size());
+    for (size_t i = 0; i < chans.size(); ++i) {
+      const int c = chans[i];
+      const int ida = Ida()(c, &ms), id0z = Id0z()(c, &ms), id0y = Id0y()(c, &ms),
+                id1z = Id1z()(c, &ms), id1y = Id1y()(c, &ms);
+      RandRayGen<val_t, Trafo0_inplace, Trafo1_inplace> rand(1, c);
+      ReguRayGen<val_t, Trafo0_inplace, Trafo1_inplace> regu(1, c);
+      for (int r = 0; r < nrays; ++r) {
+        val_t ray[6];
+        if (kind == RAPTR_RAYGEN_RAND)
+          rand(ray, id0z, id0y, id1z, id1y, ida, trafo0, trafo1);
+        else
+          regu(ray, id0z, id0y, id1z, id1y, ida, trafo0, trafo1);
+        for (int q = 0; q < 6; ++q) EXPECT(ray[q] == dev[(i * nrays + r) * 6 + q]);
+      }
+    }
+  }
+  // an element stream differs from the channel stream
+  RandRayGen<val_t, Trafo0_inplace, Trafo1_inplace> a(1, 777), b(1, 777);
+  b.setElement(42);
+  val_t ra[6], rb[6];
+  a(ra, 0, 1, 2, 3, 4, trafo0, trafo1);
+  b(rb, 0, 1, 2, 3, 4, trafo0, trafo1);
+  EXPECT(std::memcmp(ra, rb, sizeof(ra)) != 0);
+}
+
+// reco.cu:238-259: chunks of the measured channel list through
+// systemMatrixCalculation<val_t, ListSizeType, int, MemArrSizeType,
+// RandRayGen<...> > with the constants set as in reco.cu:84-93, then the
+// forward CSRmv on the ECSR.
+static void test_system_matrix_calculation() {
+  MS setup(-230, 230, 5, 8, 8, 8, 8, 36, 20, 4, 4);
+  VG grid(-40, -40, -40, 2.5, 2.5, 2.5, 32, 32, 32);
+  const int nrays = 10;
+  raptr::setConstant(nrays_const, nrays);
+  raptr::setConstant(setup_const, setup);
+  raptr::setConstant(grid_const, grid);
+  const int VGRIDSIZE = grid.nVoxels(), NCHANNELS = setup.nChannels();
+  // measured channels (ascending), chunks of LIMM rows
+  std::vector<int> yRowId_host;
+  for (int c = 3; c < NCHANNELS; c += 97) yRowId_host.push_back(c);
+  const ListSizeType effM = ListSizeType(yRowId_host.size()), LIMM = 64;
+  const MemArrSizeType LIMNNZ = MemArrSizeType(LIMM) * MemArrSizeType(VGRIDSIZE);
+  raptr::DeviceBuffer<int> yRowId(effM);
+  yRowId.upload(yRowId_host.data(), effM);
+  raptr::DeviceBuffer<int> ecsr(LIMM + 1), vxl(2 * LIMNNZ), cnlId(2 * LIMNNZ), csr(NCHANNELS + 1);
+  raptr::DeviceBuffer<val_t> val(2 * LIMNNZ), x(VGRIDSIZE), yTilde(LIMM);
+  raptr::DeviceBuffer<MemArrSizeType> nnz(1);
+  int* aEcsrCnlPtr_devi = ecsr.get();
+  int* aVxlId_devi = vxl.get();
+  int* aCnlId_devi = cnlId.get();
+  int* aCsrCnlPtr_devi = csr.get();
+  int* yRowId_devi = yRowId.get();
+  val_t* aVal_devi = val.get();
+  MemArrSizeType* nnz_devi = nnz.get();
+  raptr_sparse_handle_t handle;
+  raptr_sparse_create(&handle);
+  raptr_mat_descr_t A;
+  raptr_sparse_create_descr(&A);
+  customizeMatDescr(A, handle);
+  std::vector<val_t> x_host(VGRIDSIZE);
+  for (int j = 0; j < VGRIDSIZE; ++j) x_host[j] = val_t(1 + (j % 7));
+  x.upload(x_host.data(), VGRIDSIZE);
+  val_t* x_devi = x.get();
+  val_t* yTildeVal_devi = yTilde.get();
+  val_t one = 1, zero = 0;
+  raptr_setup_t s = setup.to_c();
+  raptr_grid_t g = grid.to_c();
+  const unsigned NChunks = nChunks<unsigned, MemArrSizeType>(MemArrSizeType(effM), LIMM);
+  for (unsigned chunkId = 0; chunkId < NChunks; ++chunkId) {
+    ListSizeType m = nInChunk(chunkId, effM, LIMM);
+    ListSizeType ptr = chunkPtr(chunkId, LIMM);
+    MemArrSizeType nnz_host[1] = {0};
+    check(raptr_memcpy_h2d(nnz_devi, nnz_host, sizeof(MemArrSizeType), nullptr), "h2d");
+    systemMatrixCalculation<val_t, ListSizeType, int, MemArrSizeType,
+                            RandRayGen<val_t, Trafo0_inplace, Trafo1_inplace> >(
+        aEcsrCnlPtr_devi, aVxlId_devi, aVal_devi, nnz_devi, aCnlId_devi, aCsrCnlPtr_devi,
+        &(yRowId_devi[ptr]), &m, handle);
+    check(raptr_memcpy_d2h(nnz_host, nnz_devi, sizeof(MemArrSizeType), nullptr), "d2h");
+    // the reference's nnz: the workqueue length of the chunk's channels
+    std::vector<int64_t> cnt(m);
+    raptr::DeviceBuffer<int64_t> dcnt(m);
+    check(raptr_workqueue_counts(&s, &g, yRowId_devi + ptr, int(m), dcnt.get(), nullptr),
+          "raptr_workqueue_counts");
+    dcnt.download(cnt.data(), m);
+    const int64_t wq = std::accumulate(cnt.begin(), cnt.end(), int64_t(0));
+    EXPECT(int64_t(nnz_host[0]) == wq);
+    const int64_t nnz_c = int64_t(nnz_host[0]);
+    std::vector<int> e(m + 1), col(nnz_c), rows(nnz_c), c(NCHANNELS + 1);
+    std::vector<val_t> v(nnz_c);
+    ecsr.download(e.data(), m + 1);
+    vxl.download(col.data(), nnz_c);
+    cnlId.download(rows.data(), nnz_c);
+    csr.download(c.data(), NCHANNELS + 1);
+    val.download(v.data(), nnz_c);
+    EXPECT(e[0] == 0 && e[m] == nnz_c);
+    for (ListSizeType i = 0; i < m; ++i) {
+      EXPECT(e[i + 1] - e[i] == cnt[i]);
+      const int ch = yRowId_host[ptr + i];
+      EXPECT(c[ch + 1] - c[ch] == cnt[i] && c[ch] == e[i]);
+      for (int k = e[i]; k < e[i + 1]; ++k) {
+        EXPECT(rows[k] == ch);
+        if (k > e[i]) EXPECT(col[k] > col[k - 1]);
+      }
+    }
+    EXPECT(c[0] == 0 && c[NCHANNELS] == nnz_c);
+    // values: the two-stage kernel on the same pairs with per-element rays
+    raptr::DeviceBuffer<val_t> v2(nnz_c);
+    check(raptr_sm_from_wq(&s, &g, aCnlId_devi, aVxlId_devi, v2.get(), nnz_c, nrays,
+                           raptr::random_seed, RAPTR_RAYGEN_RAND_ELEMENT, nullptr),
+          "raptr_sm_from_wq");
+    std::vector<val_t> w(nnz_c);
+    v2.download(w.data(), nnz_c);
+    EXPECT(std::memcmp(w.data(), v.data(), sizeof(val_t) * nnz_c) == 0);
+    // forward projection of the chunk (reco.cu:257-259)
+    EXPECT(CSRmv<val_t>()(handle, RAPTR_SPARSE_OPERATION_NON_TRANSPOSE, int(m), VGRIDSIZE,
+                          int(nnz_c), &one, A, aVal_devi, aEcsrCnlPtr_devi, aVxlId_devi, x_devi,
+                          &zero, yTildeVal_devi) == RAPTR_SPARSE_STATUS_SUCCESS);
+    std::vector<val_t> yt(m);
+    yTilde.download(yt.data(), m);
+    for (ListSizeType i = 0; i < m; ++i) {
+      double want = 0.0;
+      for (int k = e[i]; k < e[i + 1]; ++k) want += double(v[k]) * x_host[col[k]];
+      EXPECT(std::fabs(yt[i] - want) <= 1e-5 * std::fabs(want) + 1e-6);
+    }
+  }
+  // nnz_devi accumulates: a second call appends after the first's entries
+  {
+    ListSizeType m = 5;
+    MemArrSizeType nnz_host[1] = {0};
+    check(raptr_memcpy_h2d(nnz_devi, nnz_host, sizeof(MemArrSizeType), nullptr), "h2d");
+    systemMatrixCalculation<val_t, ListSizeType, int, MemArrSizeType,
+                            ReguRayGen<val_t, Trafo0_inplace, Trafo1_inplace> >(
+        aEcsrCnlPtr_devi, aVxlId_devi, aVal_devi, nnz_devi, aCnlId_devi, aCsrCnlPtr_devi,
+        yRowId_devi, &m, handle);
+    MemArrSizeType first = 0, second = 0;
+    check(raptr_memcpy_d2h(&first, nnz_devi, sizeof(first), nullptr), "d2h");
+    systemMatrixCalculation<val_t, ListSizeType, int, MemArrSizeType,
+                            ReguRayGen<val_t, Trafo0_inplace, Trafo1_inplace> >(
+        aEcsrCnlPtr_devi, aVxlId_devi, aVal_devi, nnz_devi, aCnlId_devi, aCsrCnlPtr_devi,
+        yRowId_devi + 5, &m, handle);
+    check(raptr_memcpy_d2h(&second, nnz_devi, sizeof(second), nullptr), "d2h");
+    std::vector<int> e(m + 1);
+    ecsr.download(e.data(), m + 1);
+    EXPECT(first > 0 && second > first && MemArrSizeType(e[0]) == first &&
+           MemArrSizeType(e[m]) == second);
+  }
+  raptr_sparse_destroy_descr(A);
+  raptr_sparse_destroy(handle);
+}
+
 int main(int argc, char** argv) {
   bool cpu_only = argc > 1 && std::strcmp(argv[1], "--cpu-only") == 0;
   test_index_round_trips();
@@ -253,6 +434,8 @@ int main(int argc, char** argv) {
       test_coo2csr_and_ecsr();
       test_mlem_operations();
       test_workqueue_prefix();
+      test_ray_generators();
+      test_system_matrix_calculation();
     } catch (const std::exception& e) {
       std::fprintf(stderr, "exception: %s\n", e.what());
       return 2;
