@@ -33,7 +33,7 @@ import numpy as np
 
 from . import _capi as capi
 from . import sysmat
-from .device import DeviceArray, Stream
+from .device import DeviceArray, Event, Stream
 from .geometry import MeasurementSetup, VoxelGrid
 from .recon import Reconstructor
 
@@ -294,6 +294,10 @@ class ServeStats:
     results: list = field(default_factory=list)
     chunks: int = 0
     wall_s: float = 0.0
+    # with Server(trace=True): (lane number, start ms, end ms) per chunk, from
+    # HIP events recorded on the lane's stream around the chunk's launch
+    # (times relative to the first chunk's start)
+    trace: list = field(default_factory=list)
 
     def summary(self):
         ok = [r for r in self.results if r.error is None]
@@ -312,9 +316,10 @@ class ServeStats:
 class Server:
     """Continuous-batching loop over a MatrixCache."""
 
-    def __init__(self, cache, max_chunk=10):
+    def __init__(self, cache, max_chunk=10, trace=False):
         self.cache = cache
         self.max_chunk = max_chunk
+        self.trace = trace
 
     def _admit_queued(self, lane, now):
         for s in lane.free_slots():
@@ -332,6 +337,8 @@ class Server:
         stats = ServeStats()
         t0 = clock()
         now = lambda: clock() - t0  # noqa: E731
+        events = []  # (lane number, start event, end event)
+        lane_no = {}
         while True:
             t = now() if realtime else float("inf")
             while pending and pending[0][0] <= t:
@@ -349,7 +356,13 @@ class Server:
                 self._admit_queued(lane, max(now(), 0.0))
                 k = lane.chunk(self.max_chunk)
                 if k > 0:
+                    if self.trace:
+                        e0, e1 = Event(), Event()
+                        e0.record(lane.stream.handle)
                     lane.launch(k)
+                    if self.trace:
+                        e1.record(lane.stream.handle)
+                        events.append((lane_no.setdefault(id(lane), len(lane_no)), e0, e1))
                     active.append((lane, k))
             for lane, k in active:
                 lane.stream.sync()
@@ -363,7 +376,21 @@ class Server:
                     if wait > 0:
                         time.sleep(min(wait, 0.005))
         stats.wall_s = now()
+        if events:
+            ref = events[0][1]
+            stats.trace = [(ln, ref.elapsed_ms(a), ref.elapsed_ms(b)) for ln, a, b in events]
         return stats
+
+
+def overlapping_chunks(trace):
+    """Pairs of chunks of different lanes whose device time intervals
+    overlap (ServeStats.trace)."""
+    n = 0
+    for i, (li, a0, a1) in enumerate(trace):
+        for lj, b0, b1 in trace[i + 1:]:
+            if li != lj and a0 < b1 and b0 < a1:
+                n += 1
+    return n
 
 
 def synthetic_requests(geometries, rate_hz, n, seed=0, events=(1e4, 5e6), iters=(10, 50),

@@ -220,6 +220,55 @@ def test_gpu_alone_equals_mixed_and_matches_oracle(gpu, oracle, frames):
 
 
 @pytest.mark.gpu
+def test_gpu_geometry_mix_alone_equals_mixed(gpu, oracle):
+    """The serving workload's three geometries (8x8 heads / 32^3, 13x13 /
+    52^3, 16x16 / 96^3; 2 mm voxels; angle counts cut to 12, 6 and 4 to keep
+    the test short), 64-slot lanes on the matrix cores, requests of 1e4-1e6
+    events and 3-12 iterations arriving together: every image equals the
+    image of the same request served alone, bit for bit, across geometries;
+    the chunks of different lanes overlap on the device (HIP event
+    timestamps on the lanes' streams); one request of the smallest geometry
+    matches the FP64 oracle (1e-4)."""
+    from conftest import oracle_geom
+    from raptr_amd import synthetic, sysmat
+    geos = [serving.Geometry(G.two_head_setup(p, na, 180.0 / na), G.centred_grid(v, 2.0), 10)
+            for p, v, na in ((8, 32, 12), (13, 52, 6), (16, 96, 4))]
+    ybar = {}
+    for g in geos:
+        img, _ = synthetic.random_ellipsoids(g.grid, n=4, seed=5)
+        ybar[g] = sysmat.forward_project(g.setup, g.grid, img.astype(np.float32), g.nrays)
+    reqs = serving.synthetic_requests(geos, 100.0, 9, seed=8, events=(1e4, 1e6), iters=(3, 12),
+                                      ybar=ybar)
+    assert len({r.geometry for r in reqs}) == 3
+    cache = serving.MatrixCache(200 << 30, frames=64)
+    srv = serving.Server(cache, max_chunk=3, trace=True)
+    st = srv.serve(reqs, realtime=False)
+    mixed = {r.rid: r for r in st.results}
+    assert all(r.error is None for r in mixed.values())
+    # overlap: some chunk of one lane starts before a chunk of another ends
+    ov = serving.overlapping_chunks(st.trace)
+    print(f"chunks {len(st.trace)}, overlapping pairs of different lanes {ov}")
+    assert ov > 0
+    for req in reqs:
+        alone = srv.serve([req], realtime=False).results[0]
+        np.testing.assert_array_equal(mixed[req.rid].image, alone.image)
+    # FP64 oracle for a request of the 8x8 / 32^3 geometry
+    req = next(r for r in reqs if r.geometry is geos[0])
+    setup, grid, nrays = geos[0].setup, geos[0].grid, geos[0].nrays
+    s, g = oracle_geom(oracle, setup, grid)
+    allc = np.arange(setup.nchannels, dtype=np.int32)
+    rp, col, val, v64 = oracle.build_csr(s, g, allc, nrays, with_f64=True)
+    y = np.zeros(len(allc))
+    y[req.channels] = req.counts
+    x64 = oracle.mlem_f64(rp, col, v64, y, oracle.sensitivity_f64(rp, col, v64, grid.nvox),
+                          np.full(grid.nvox, 1.0 / grid.nvox), req.niter)
+    rel = np.linalg.norm(mixed[req.rid].image - x64) / np.linalg.norm(x64)
+    assert rel <= 1e-4
+    for geo in list(cache.lanes):
+        cache.lanes.pop(geo).close()
+
+
+@pytest.mark.gpu
 def test_gpu_cache_budget_and_rejection(gpu):
     tiny = serving.Geometry(*G.config_tiny())
     small = serving.Geometry(*G.config_small())
