@@ -113,6 +113,7 @@ def _bind(L):
         "ro_scales": (None, [P(f32), f32, i32]),
         "ro_sum": (f32, [P(f32), i32]),
         "ro_mlem_f64": (None, [i32, i32, P(i64), P(i32), P(f64), P(f64), P(f64), P(f64), i32]),
+        "ro_mlem_f32": (None, [i32, i32, P(i64), P(i32), P(f32), P(f32), P(f64), P(f32), i32]),
         "ro_sensitivity_f64": (None, [i32, i32, P(i64), P(i32), P(f64), P(f64)]),
     }
     for name, (res, args) in sig.items():
@@ -124,6 +125,37 @@ def _bind(L):
 
 def ref_available():
     return os.path.exists(REF_PATH)
+
+
+H5REF_PATH = os.path.join(os.path.dirname(REF_PATH), "libraptr_h5ref.so")
+_h5ref = None
+
+
+def h5ref_available():
+    return os.path.exists(H5REF_PATH)
+
+
+def h5ref():
+    """The reference's HDF5 writer and readers (oracle/ref/h5_ref.cpp)."""
+    global _h5ref
+    if _h5ref is None:
+        L = C.CDLL(H5REF_PATH)
+        P = C.POINTER
+        L.ref_h5_write_density.restype = C.c_int
+        L.ref_h5_write_density.argtypes = [C.c_char_p, P(C.c_float), P(C.c_float),
+                                           P(C.c_float), P(C.c_int)]
+        L.ref_h5_read.restype = C.c_int64
+        L.ref_h5_read.argtypes = [C.c_char_p, C.c_int, P(C.c_float), C.c_int64]
+        _h5ref = L
+    return _h5ref
+
+
+def ref_h5_read(path, density, n):
+    out = np.zeros(n, np.float32)
+    got = h5ref().ref_h5_read(path.encode(), 1 if density else 0, _p(out, C.c_float), n)
+    if got != n:
+        raise RuntimeError(f"reference reader failed on {path}")
+    return out
 
 
 def ref():
@@ -395,4 +427,18 @@ def mlem_f64(rp, col, val64, y, sens, x0, niter):
     x = np.ascontiguousarray(x0, np.float64).copy()
     lib().ro_mlem_f64(len(rp) - 1, len(x), _p(rp, C.c_int64), _p(col, C.c_int), _p(v, C.c_double),
                       _p(y, C.c_double), _p(s, C.c_double), _p(x, C.c_double), niter)
+    return x
+
+
+def mlem_f32(rp, col, val, y, sens, x0, niter):
+    """The recurrence in FP32 (the reference's precision)."""
+    rp = np.ascontiguousarray(rp, np.int64)
+    col = np.ascontiguousarray(col, np.int32)
+    val = np.ascontiguousarray(val, np.float32)
+    y = np.ascontiguousarray(y, np.float32)
+    sens = np.ascontiguousarray(sens, np.float64)
+    x = np.ascontiguousarray(x0, np.float32).copy()
+    lib().ro_mlem_f32(len(rp) - 1, len(x), _p(rp, C.c_int64), _p(col, C.c_int),
+                      _p(val, C.c_float), _p(y, C.c_float), _p(sens, C.c_double),
+                      _p(x, C.c_float), niter)
     return x

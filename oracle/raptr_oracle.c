@@ -868,3 +868,71 @@ void ro_mlem_f64(int m, int nvox, const int64_t* rp, const int* col,
   free(c);
   free(e);
 }
+
+/* ======================================================================
+ * A21 in the reference's own precision: the MLEM recurrence of
+ * reco.cu:230-303 with FP32 matrix values, vectors and reductions, the way
+ * the reference runs it (cusparseScsrmv, thrust transform/reduce on float):
+ * s <- s/sum(s) (reco.cu:203-204), then per iteration yt = A x,
+ * e = y/yt (0 where yt == 0, the guard of the GPU path), c = A^T e,
+ * x <- x*c/(s + (s==0)), x <- x*(1/sum(x)) (mlemOperations.hpp:44-169).
+ * The CPU baseline of the benchmark; the FP64 recurrence above is the
+ * accuracy reference.
+ * ==================================================================== */
+void ro_mlem_f32(int m, int nvox, const int64_t* rp, const int* col,
+                 const float* val, const float* y, const double* sens,
+                 float* x, int niter) {
+  float* s = (float*)malloc(sizeof(float) * nvox);
+  float* c = (float*)malloc(sizeof(float) * nvox);
+  float* e = (float*)malloc(sizeof(float) * (m > 0 ? m : 1));
+  int nt = 1;
+#ifdef _OPENMP
+  nt = omp_get_max_threads();
+#endif
+  float* part = (float*)malloc(sizeof(float) * (size_t)nvox * (size_t)nt);
+  float ss = 0.0f;
+  for (int j = 0; j < nvox; ++j) ss += (float)sens[j];
+  for (int j = 0; j < nvox; ++j) s[j] = (float)sens[j] / ss;
+  int used = 1;
+  for (int it = 0; it < niter; ++it) {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < m; ++i) {
+      float acc = 0.0f;
+      for (int64_t k = rp[i]; k < rp[i + 1]; ++k) acc += val[k] * x[col[k]];
+      e[i] = acc != 0.0f ? y[i] / acc : 0.0f;
+    }
+#pragma omp parallel num_threads(nt)
+    {
+      int t = 0, T = 1;
+#ifdef _OPENMP
+      t = omp_get_thread_num();
+      T = omp_get_num_threads();
+#endif
+      if (t == 0) used = T;
+      float* p = part + (size_t)nvox * (size_t)t;
+      for (int j = 0; j < nvox; ++j) p[j] = 0.0f;
+      int64_t lo = (int64_t)m * t / T, hi = (int64_t)m * (t + 1) / T;
+      for (int64_t i = lo; i < hi; ++i)
+        for (int64_t k = rp[i]; k < rp[i + 1]; ++k) p[col[k]] += val[k] * e[i];
+    }
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < nvox; ++j) {
+      float acc = 0.0f;
+      for (int t = 0; t < used; ++t) acc += part[(size_t)nvox * (size_t)t + j];
+      c[j] = acc;
+    }
+    float norm = 0.0f;
+    for (int j = 0; j < nvox; ++j) {
+      float d = s[j];
+      d += (float)(d == 0.0f);
+      x[j] = x[j] * c[j] / d;
+      norm += x[j];
+    }
+    const float inv = 1.0f / norm;
+    for (int j = 0; j < nvox; ++j) x[j] = x[j] * inv;
+  }
+  free(part);
+  free(s);
+  free(c);
+  free(e);
+}

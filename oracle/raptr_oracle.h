@@ -155,6 +155,10 @@ float ro_sum(const float* x, int n);
 void ro_mlem_f64(int m, int nvox, const int64_t* row_ptr, const int* col,
                  const double* val, const double* y, const double* sens,
                  double* x, int niter);
+/* the same recurrence in FP32 (the reference's precision; CPU baseline) */
+void ro_mlem_f32(int m, int nvox, const int64_t* rp, const int* col,
+                 const float* val, const float* y, const double* sens,
+                 float* x, int niter);
 void ro_sensitivity_f64(int m, int nvox, const int64_t* row_ptr,
                         const int* col, const double* val, double* sens);
 

@@ -9,8 +9,9 @@ writing its HDF5 files.
   "<iteration>_<output>" (reco.cu:304-311) or only the last one.
 * ``backprojection``: summed back-projection A^T y normalised to sum 1
   (tests/backprojection.cu:144-185).
-* ``system_matrix``: the matrix of the measured channels
-  (tests/pureSMCalculation.cu:117-176).
+* ``sm`` / ``system_matrix``: the matrix of the measured channels only,
+  with the reference's per-chunk element counts
+  (tests/pureSMCalculation.cu:53-176).
 
 The geometry is an argument (the reference compiles it in from
 real_measurementsetup_defines.h / voxelgrid_defines.h); ``--config`` names
@@ -30,10 +31,22 @@ from .recon import Reconstructor
 CONFIGS = {"tiny": G.config_tiny, "small": G.config_small, "full": G.config_full}
 
 
-def system_matrix(measurement, setup, grid, nrays, seed=1, stream=None):
+def system_matrix(measurement, setup, grid, nrays, seed=1, stream=None, keep_zeros=False,
+                  raygen=sysmat.capi.RAYGEN_RAND):
+    """The matrix of the measured channels (rows in list order).  keep_zeros:
+    every culled candidate stored, as the reference's fused kernel does (nnz
+    = workqueue length); raygen RAYGEN_RAND_ELEMENT: rays per element."""
     channels, _ = h5io.read_measurement(measurement, setup)
     return channels, sysmat.build_csr(setup, grid, channels, nrays, seed=seed, stream=stream,
-                                      capacity="auto")
+                                      capacity="auto", keep_zeros=keep_zeros, raygen=raygen)
+
+
+def chunk_counts(row_ptr, eff_m, lim_m):
+    """Entries per chunk of lim_m measured channels (wrappers.hpp:429-464
+    chunk math over the measurement list), from a row pointer."""
+    n_chunks = (eff_m + lim_m - 1) // lim_m
+    return [int(row_ptr[min((k + 1) * lim_m, eff_m)] - row_ptr[k * lim_m])
+            for k in range(n_chunks)]
 
 
 def backprojection(measurement, output, setup, grid, nrays, seed=1):
@@ -100,13 +113,45 @@ def main(argv=None):
     b.add_argument("measurement")
     b.add_argument("output")
     b.add_argument("nrays", type=int)
-    for p in (r, b):
+    m = sub.add_parser("sm", help="system matrix only (pureSMCalculation.cu)")
+    m.add_argument("measurement")
+    m.add_argument("nrays", type=int)
+    m.add_argument("--drop-zeros", action="store_true",
+                   help="store exact non-zeros only (the reconstruction plans' form); "
+                        "default: every culled candidate, as the reference")
+    m.add_argument("--element-rays", action="store_true",
+                   help="rays per (channel, voxel) element (the reference's per-thread "
+                        "generators) instead of per channel")
+    m.add_argument("--lim-m", type=int, default=0,
+                   help="channels per reported chunk (default: the reference's LIMM, "
+                        "512 MiB of float values / voxels)")
+    m.add_argument("--output", help="write the matrix as .npz (row_ptr, col, val, channels)")
+    for p in (r, b, m):
         p.add_argument("--config", choices=sorted(CONFIGS), default="full")
     a = ap.parse_args(argv)
     setup, grid, _ = CONFIGS[a.config]()
     if a.cmd == "reco":
         reco(a.measurement, a.output, setup, grid, a.nrays, a.niter, a.sensitivity, a.guess,
              every_iteration=not a.last_only, log=lambda s: print(s, flush=True))
+    elif a.cmd == "sm":
+        import time
+        t = time.perf_counter()
+        raygen = sysmat.capi.RAYGEN_RAND_ELEMENT if a.element_rays else sysmat.capi.RAYGEN_RAND
+        channels, A = system_matrix(a.measurement, setup, grid, a.nrays,
+                                    keep_zeros=not a.drop_zeros, raygen=raygen)
+        secs = time.perf_counter() - t
+        rp = A.row_ptr.numpy(A.m + 1)
+        lim = a.lim_m or max(1, (512 * 1024 * 1024 // 4) // grid.nvox)
+        counts = chunk_counts(rp, len(channels), lim)
+        print(f"Calculate system matrix with #LOR = {len(channels)} and #voxels = {grid.nvox}. "
+              f"Partition into chunks of maximum #LOR = {lim}: {len(counts)} chunks.")
+        for k, c in enumerate(counts):
+            print(f"Finished chunk {k} of {len(counts)}, found {c} elements.")
+        print(f"Found: {A.nnz} elements.")
+        print(f"Time for SM calculation: {secs:.3f} s")
+        if a.output:
+            rp, col, val = A.to_host()
+            np.savez(a.output, row_ptr=rp, col=col, val=val, channels=channels)
     else:
         x = backprojection(a.measurement, a.output, setup, grid, a.nrays)
         print(f"Norm: {float(x.sum()):.6f}")

@@ -2,7 +2,7 @@
 
 Run in a container that holds the raptr sources (builds oracle/_ref first):
 
-    make -C oracle ref && python tests/golden/make_golden.py [full_angles]
+    make -C oracle ref h5ref && python tests/golden/make_golden.py [full_angles | h5]
 
 Fixtures (numpy .npz, data only):
   wq_small.npz    workqueue of the 20,480-channel / 32^3 workload computed
@@ -26,6 +26,12 @@ Fixtures (numpy .npz, data only):
                   angle), stored in full (voxel ids delta-coded per channel),
                   with the near-boundary pairs of those channels and their
                   in/out decision.
+  density_ref.h5  a density file written by the reference's
+                  H5DensityWriter (5 x 6 x 7 voxels, origin (-3, -1.5, 2),
+                  voxel (0.5, 0.25, 1.5)); h5_ref.npz holds its input, and
+                  the values the reference's H5Reader reads from the
+                  measurement file raptr_amd.h5io writes for a fixed sparse
+                  list on the 4x4-head setup.
 """
 import hashlib
 import os
@@ -142,6 +148,36 @@ def main():
                         out=out)
     print("trafo_ref written")
     full_angles()
+    h5_fixtures()
+
+
+def h5_fixtures():
+    """Reference-written density file, and the reference reader's view of a
+    measurement file written by raptr_amd.h5io (needs oracle/_ref/
+    libraptr_h5ref.so: `make -C oracle h5ref`)."""
+    import tempfile
+    from raptr_amd import h5io
+    if not O.h5ref_available():
+        raise SystemExit("oracle/_ref/libraptr_h5ref.so missing: run `make -C oracle h5ref`")
+    nv = np.array([5, 6, 7], np.int32)
+    org = np.array([-3.0, -1.5, 2.0], np.float32)
+    vox = np.array([0.5, 0.25, 1.5], np.float32)
+    dens = (np.arange(nv.prod(), dtype=np.float32) * np.float32(0.37) - 11).astype(np.float32)
+    path = os.path.join(HERE, "density_ref.h5")
+    assert O.h5ref().ref_h5_write_density(path.encode(), O._p(dens, O.C.c_float),
+                                          O._p(org, O.C.c_float), O._p(vox, O.C.c_float),
+                                          O._p(nv, O.C.c_int)) == 0
+    setup = G.config_tiny()[0]
+    rng = np.random.default_rng(9)
+    ch = np.sort(rng.choice(setup.nchannels, 200, replace=False)).astype(np.int32)
+    counts = rng.integers(1, 50, len(ch)).astype(np.float32)
+    with tempfile.TemporaryDirectory() as d:
+        mfn = os.path.join(d, "meas.h5")
+        h5io.write_measurement(mfn, setup, ch, counts)
+        read = O.ref_h5_read(mfn, False, setup.nchannels)
+    np.savez_compressed(os.path.join(HERE, "h5_ref.npz"), density=dens, origin=org, voxel=vox,
+                        nvox=nv, meas_channels=ch, meas_counts=counts, meas_ref_read=read)
+    print("density_ref.h5 and h5_ref.npz written")
 
 
 def full_angle_channels(setup):
@@ -194,5 +230,7 @@ def full_angles():
 if __name__ == "__main__":
     if sys.argv[1:] == ["full_angles"]:
         full_angles()
+    elif sys.argv[1:] == ["h5"]:
+        h5_fixtures()
     else:
         main()

@@ -75,3 +75,52 @@ def test_backprojection_driver(tiny_files):
     np.testing.assert_array_equal(x, want)
     back, _ = h5io.read_density(out)
     np.testing.assert_array_equal(back.reshape(-1), want.astype(np.float32))
+
+
+def test_sm_driver_vs_oracle(tiny_files, oracle, capsys):
+    """The system-matrix-only driver (pureSMCalculation.cu): its matrix equals
+    the oracle's build bit for bit in the reference's semantics (every culled
+    candidate kept: nnz = the workqueue length, per-element rays), with
+    exact zeros dropped, and per channel rays; the chunk counts add up."""
+    from conftest import oracle_geom
+    from raptr_amd import drivers
+    d, fn, setup, grid, nrays, meas, y = tiny_files
+    s, g = oracle_geom(oracle, setup, grid)
+    cw, vw, _, _ = oracle.workqueue(s, g, meas)
+    for extra, raygen, flags in (([], oracle.RAYGEN_RAND, oracle.SM_KEEP_ZEROS),
+                                 (["--element-rays"], oracle.RAYGEN_RAND_ELEMENT,
+                                  oracle.SM_KEEP_ZEROS),
+                                 (["--drop-zeros"], oracle.RAYGEN_RAND, 0)):
+        out = str(d / "sm.npz")
+        assert drivers.main(["sm", fn, str(nrays), "--config", "tiny", "--lim-m", "100",
+                             "--output", out] + extra) == 0
+        text = capsys.readouterr().out
+        got = np.load(out)
+        rp, col, val, _ = oracle.build_csr(s, g, meas, nrays, raygen=raygen, flags=flags)
+        np.testing.assert_array_equal(got["row_ptr"], rp)
+        np.testing.assert_array_equal(got["col"], col)
+        np.testing.assert_array_equal(got["val"], val)
+        np.testing.assert_array_equal(got["channels"], meas)
+        assert f"Found: {len(col)} elements." in text
+        if flags:
+            assert len(col) == len(vw)
+        chunks = [int(line.split("found ")[1].split()[0]) for line in text.splitlines()
+                  if line.startswith("Finished chunk")]
+        assert len(chunks) == -(-len(meas) // 100) and sum(chunks) == len(col)
+
+
+def test_reco_driver_vs_fp64_oracle(tiny_files, oracle):
+    """The reco driver's image against the FP64 host recurrence on the
+    oracle's matrix of the measured channels (1e-4 relative L2)."""
+    from conftest import oracle_geom
+    from raptr_amd import drivers
+    d, fn, setup, grid, nrays, meas, y = tiny_files
+    x = drivers.reco(fn, str(d / "o.h5"), setup, grid, nrays, 5, every_iteration=False)
+    s, g = oracle_geom(oracle, setup, grid)
+    allc = np.arange(setup.nchannels, dtype=np.int32)
+    rpa, cola, vala, _ = oracle.build_csr(s, g, allc, nrays)
+    sens = oracle.sensitivity_f64(rpa, cola, vala.astype(np.float64), grid.nvox)
+    rp, col, val, _ = oracle.build_csr(s, g, meas, nrays)
+    x64 = oracle.mlem_f64(rp, col, val.astype(np.float64), y.astype(np.float64), sens,
+                          np.full(grid.nvox, 1.0 / grid.nvox), 5)
+    assert np.linalg.norm(x - x64) / np.linalg.norm(x64) <= 1e-4

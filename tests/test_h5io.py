@@ -89,3 +89,66 @@ def test_cpp_mirror_client(tmp_path):
     r = subprocess.run([exe, str(tmp_path)], capture_output=True, text=True, timeout=60)
     assert r.returncode == 0, r.stdout + r.stderr
     assert r.stdout.strip().endswith("OK")
+
+
+# ------------------------------------- against the reference's own HDF5 code
+def _grid_attrs(fn):
+    import ctypes as C
+    lo, hi, n = np.zeros(3, np.float32), np.zeros(3, np.float32), np.zeros(3, np.int32)
+    assert h5io.lib().raptr_h5_density_grid(fn.encode(), h5io._p(lo, C.c_float),
+                                           h5io._p(hi, C.c_float), h5io._p(n, C.c_int)) == 0
+    return lo, hi, n
+
+
+def test_reads_density_written_by_reference():
+    """tests/golden/density_ref.h5 was written by the reference's
+    H5DensityWriter (compiled from its sources against HDF5 C++): values,
+    shape and every grid attribute read back exactly."""
+    from conftest import GOLDEN, load_golden
+    gold = load_golden("h5_ref.npz")
+    fn = os.path.join(GOLDEN, "density_ref.h5")
+    nv = tuple(int(v) for v in gold["nvox"])
+    assert h5io.dims(fn, "density") == nv
+    back, g = h5io.read_density(fn)
+    np.testing.assert_array_equal(back.reshape(-1), gold["density"])
+    lo, hi, n = _grid_attrs(fn)
+    np.testing.assert_array_equal(lo, gold["origin"])
+    np.testing.assert_array_equal(hi, gold["origin"] + gold["nvox"].astype(np.float32) *
+                                  gold["voxel"])
+    assert n.tolist() == list(nv)
+    # our writer gives a file with the same contents and attributes
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        mine = os.path.join(d, "d.h5")
+        h5io.write_density(mine, gold["density"].reshape(nv),
+                           G.VoxelGrid(tuple(gold["origin"]), tuple(gold["voxel"]), nv))
+        for a, b in zip(_grid_attrs(mine), _grid_attrs(fn)):
+            np.testing.assert_array_equal(a, b)
+        np.testing.assert_array_equal(h5io.read_density(mine)[0], back)
+
+
+def test_reference_reader_reads_our_measurement(oracle):
+    """The measurement file raptr_amd.h5io writes is read by the reference's
+    H5Reader ("messung", sizeOfFile, read) as the dense channel vector:
+    recorded when the fixture was made, and live where the reference
+    binding is built."""
+    from conftest import load_golden
+    import tempfile
+    gold = load_golden("h5_ref.npz")
+    setup = G.config_tiny()[0]
+    dense = np.zeros(setup.nchannels, np.float32)
+    dense[gold["meas_channels"]] = gold["meas_counts"]
+    np.testing.assert_array_equal(gold["meas_ref_read"], dense)
+    with tempfile.TemporaryDirectory() as d:
+        fn = os.path.join(d, "m.h5")
+        h5io.write_measurement(fn, setup, gold["meas_channels"], gold["meas_counts"])
+        np.testing.assert_array_equal(h5io.read_measurement_dense(fn).reshape(-1), dense)
+        if oracle.h5ref_available():
+            np.testing.assert_array_equal(oracle.ref_h5_read(fn, False, setup.nchannels), dense)
+            # and a density written here is read by the reference's
+            # H5DensityReader
+            grid = G.VoxelGrid((0.1, 0.2, 0.3), (1.0, 2.0, 3.0), (5, 6, 7))
+            img = np.arange(210, dtype=np.float32).reshape(5, 6, 7) * np.float32(0.5)
+            dfn = os.path.join(d, "d.h5")
+            h5io.write_density(dfn, img, grid)
+            np.testing.assert_array_equal(oracle.ref_h5_read(dfn, True, 210), img.reshape(-1))
