@@ -40,6 +40,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 ERR_BAR = 1e-4
 ERR_ITERS = 30  # iterations of the mixed-precision error figures
+CPU_ITERS = 5  # iterations per timed run of the CPU baseline
 
 # HBM bytes per launch measured with rocprofv3 (FETCH_SIZE scaled x2.00,
 # WRITE_SIZE x1.00: the factors a 2 GiB streaming copy of this library
@@ -134,7 +135,8 @@ def parse():
                     help="skip the FP64 host recurrence (no error figures either)")
     ap.add_argument("--no-small", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=10)
-    ap.add_argument("--modes", default="plain,batched,mixed_u16,mixed_f16,mixed_bf16",
+    ap.add_argument("--modes",
+                    default="plain,batched,mixed_u16,mixed_f16,mixed_bf16,plain_u16,plain_f16",
                     help="comma-separated subset of the modes to run")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serving-requests", type=int, default=120)
@@ -231,12 +233,18 @@ def main():
     plans = {}
     wanted = args.modes.split(",")
     orders = ["locality", "list"] if args.row_order == "both" else [args.row_order]
-    d_order = (sysmat.locality_order(setup, d_list, stream=st, tile=args.locality_tile)
-               if "locality" in orders else None)
+    # locality orders: y-major for the single-frame plans, slope-tiled for
+    # the batched ones (recon.default_order_kind)
+    from raptr_amd.recon import default_order_kind
+    d_orders = ({k: sysmat.locality_order(setup, d_list, stream=st, tile=args.locality_tile,
+                                          kind=k) for k in ("y", "slope")}
+                if "locality" in orders else {})
     slices_mb = [float(v) for v in args.block_slices_mb.split(",")]
     best_rows = best_order = None
+    plain_images = None  # every frame by the plain plan (per-frame errors)
     for name, vt, nf in (("plain", "f32", 1), ("batched", "f32", F), ("mixed_u16", "u16", F),
-                         ("mixed_f16", "f16", F), ("mixed_bf16", "bf16", F)):
+                         ("mixed_f16", "f16", F), ("mixed_bf16", "bf16", F),
+                         ("plain_u16", "u16", 1), ("plain_f16", "f16", 1)):
         if name not in wanted:
             continue
         yy = Y[:, 0].copy() if nf == 1 else Y
@@ -246,7 +254,9 @@ def main():
         # row blocks of the backward: for F frames try several ratio-slice
         # sizes (the first batched mode picks, the others reuse its choice)
         # one frame: both plan layouts (column tiles, row blocks) are tried
-        if nf == 1:
+        if nf == 1 and name != "plain":
+            rows_try = [("blocked", 0)]
+        elif nf == 1:
             # (block_rows 0: the plan's default, 65,536 rows for one frame)
             rows_try = [("tiled", 0), ("blocked", 0), ("blocked", 524288), ("blocked", 12288)]
         elif best_rows is not None:
@@ -260,7 +270,8 @@ def main():
         for layout, rows, order in rows_try:
             t = time.perf_counter()
             Rc = Reconstructor(A, sens, vt, nf, block_rows=rows, stream=st, layout=layout,
-                               row_order=d_order if order == "locality" else None,
+                               row_order=(d_orders[default_order_kind(nf)]
+                                          if order == "locality" else None),
                                grid=grid, group=args.group)
             tp = time.perf_counter() - t
             cc, mc = Rc.autotune(dy, dx, nf, reps=2, stream=st)
@@ -317,11 +328,27 @@ def main():
         log(f"{name}: {ms:.1f} ms/batch of {nf} -> {fps:.2f} frames/s "
             f"(cfg {cfg}, {info['row_blocks']} row blocks, plan {t_plan:.1f} s, "
             f"passes {modes[name]['passes_ms']})")
+        if name == "plain" and nf < F and F > 1:
+            # every frame of the batch by the plain plan: the per-frame
+            # reference of the batched modes (FP32, 1.2e-6 from FP64)
+            plain_images = np.empty((n, F), np.float32)
+            dyf = DeviceArray(m, np.float32)
+            for f in range(F):
+                dyf.copy_from(Y[:, f].copy(), stream=st)
+                dx.copy_from(x0, stream=st)
+                R.run(dyf, dx, niter, 1, use_graph=True, stream=st)
+                plain_images[:, f] = dx.numpy(stream=st)
+            del dyf
+        if nf > 1:
+            modes[name]["roofline"] = batched_roofline(
+                R, info, nf, m, n, A.nnz, vt, args.kernel_reps, timed, capi, dx, dy, st)
+            log(f"{name}: roofline {modes[name]['roofline']}")
+        if nf > 1 and plain_images is not None:
+            per = [rel_l2(xr[:, f], plain_images[:, f].astype(np.float64)) for f in range(nf)]
+            modes[name]["rel_l2_vs_plain_per_frame"] = [float(f"{v:.3e}") for v in per]
+            modes[name]["rel_l2_vs_plain_max"] = max(per)
+            modes[name]["image63"] = xr[:, nf - 1].copy()
         if name != "plain":
-            # each batched frame against the plain single-frame run of frame 0
-            if "plain" in modes:
-                modes[name]["rel_l2_vs_plain_frame0"] = rel_l2(xr[:, 0],
-                                                               modes["plain"]["image0"])
             R.close()
             del plans[name], R, dy, dx
 
@@ -330,6 +357,7 @@ def main():
         for name in modes:
             modes[name].pop("image0")
             modes[name].pop("image0_30")
+            modes[name].pop("image63", None)
         print(json.dumps({"partial": True, "modes": modes, "device": dev}), flush=True)
         return
 
@@ -461,8 +489,28 @@ def main():
         O.mlem_f64(rp, col, v64, y.astype(np.float64), sens.numpy(), x0.astype(np.float64), 3)
         cpu1_s = (time.perf_counter() - t) * niter / 3.0
         O.lib().ro_set_threads(threads)
-        del rp, col, v64
+        del v64
+        # the reference's FP32 recurrence restated on the host cores: one
+        # warm-up, then the median of 5 runs of CPU_ITERS iterations, scaled
+        # to niter
+        val32 = A.val.numpy(A.nnz)
+        sens64 = sens.numpy()
+        O.mlem_f32(rp, col, val32, y, sens64, x0, 1)
+        t32 = []
+        for _ in range(5):
+            t = time.perf_counter()
+            O.mlem_f32(rp, col, val32, y, sens64, x0, CPU_ITERS)
+            t32.append(time.perf_counter() - t)
+        cpu32_s = float(np.median(t32)) * niter / CPU_ITERS
+        del rp, col, val32
+        x64_63 = None
+        if plain_images is not None:
+            x64_63 = O.mlem_f64(rp, col, v64, Y[:, F - 1].astype(np.float64), sens.numpy(),
+                                x0.astype(np.float64), niter)
         for name in modes:
+            if x64_63 is not None and "image63" in modes[name]:
+                modes[name]["rel_l2_vs_fp64_last_frame"] = rel_l2(modes[name].pop("image63"),
+                                                                  x64_63)
             modes[name]["rel_l2_vs_fp64"] = rel_l2(modes[name]["image0"], x64)
             modes[name][f"rel_l2_vs_fp64_{ERR_ITERS}it"] = rel_l2(modes[name]["image0_30"], x64_30)
             modes[name][f"max_voxel_err_vs_fp64_{ERR_ITERS}it"] = float(
@@ -470,20 +518,30 @@ def main():
             modes[name]["max_voxel_err_vs_fp64"] = float(
                 np.max(np.abs(modes[name]["image0"] - x64)) / np.max(np.abs(x64)))
             log(f"{name}: rel L2 vs FP64 oracle {modes[name]['rel_l2_vs_fp64']:.2e}")
-        cpu = {"value": round(1.0 / cpu_s, 6), "unit": "frames/s", "cores": threads,
+        cpu = {"value": round(1.0 / cpu32_s, 6), "unit": "frames/s", "cores": threads,
                "kind": "port",
-               "sample": f"one full frame ({niter} MLEM iterations) of the same workload by the "
-                         f"FP64 C oracle (OpenMP) on the same {A.nnz}-entry matrix: "
-                         f"{cpu_s:.1f} s"}
+               "sample": f"the reference's FP32 MLEM recurrence restated in C (OpenMP) on the "
+                         f"same {A.nnz}-entry matrix: median of 5 runs of {CPU_ITERS} "
+                         f"iterations after one warm-up, scaled to {niter} iterations "
+                         f"({cpu32_s * 1e3 / niter:.1f} ms per iteration)",
+               "ms_per_iteration": round(cpu32_s * 1e3 / niter, 2)}
+        cpu["fp64"] = {"value": round(1.0 / cpu_s, 6), "unit": "frames/s", "cores": threads,
+                       "sample": f"one full frame ({niter} iterations) by the FP64 oracle: "
+                                 f"{cpu_s:.1f} s"}
         cpu["single_thread"] = {"value": round(1.0 / cpu1_s, 6), "unit": "frames/s", "cores": 1,
-                                "sample": f"3 iterations on one thread, scaled to {niter}"}
-        log(f"cpu baseline: {cpu_s:.1f} s/frame on {threads} threads, {cpu1_s:.1f} s/frame "
-            "on one thread")
+                                "sample": f"FP64, 3 iterations on one thread, scaled to {niter}"}
+        log(f"cpu baseline: FP32 {cpu32_s:.1f} s/frame, FP64 {cpu_s:.1f} s/frame on {threads} "
+            f"threads, FP64 {cpu1_s:.1f} s/frame on one thread")
 
     for name in modes:
         modes[name].pop("image0")
         modes[name].pop("image0_30")
-    ok = [k for k in modes if modes[k].get("rel_l2_vs_fp64", 0.0) <= ERR_BAR]
+        modes[name].pop("image63", None)
+    # a mode qualifies when frame 0 (and the batch's last frame) meet the bar
+    # against the FP64 oracle and every frame is within it of the plain path
+    ok = [k for k in modes if modes[k].get("rel_l2_vs_fp64", 0.0) <= ERR_BAR
+          and modes[k].get("rel_l2_vs_fp64_last_frame", 0.0) <= ERR_BAR
+          and modes[k].get("rel_l2_vs_plain_max", 0.0) <= ERR_BAR]
     best = max(ok, key=lambda k: modes[k]["frames_per_s"]) if ok else "plain"
     plain_fps = modes["plain"]["frames_per_s"]
     for k in modes:
@@ -512,6 +570,27 @@ def main():
         serve = bench_serving(log, args.serving_requests, args.serving_rate, args.serving_slots,
                               args.serving_chunk)
 
+    plain_roofline = {"bound": "hbm", "kernel": dom["kernel"], "achieved": round(dom["gbs"], 1),
+                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                 "frac": round(dom["gbs"] / HBM_PEAK_GBS, 4),
+                 "traffic": int(traffic) if traffic else None,
+                 "traffic_source": ("rocprofv3 FETCH_SIZE x2.00 + WRITE_SIZE x1.00 per launch, "
+                                    "factors from a 2 GiB streaming copy") if traffic else None,
+                 "algorithmic_bytes": int(dom["bytes"] / dom["launches"]),
+                 "bytes_basis": "4-byte value + 4-byte index per stored entry",
+                 "stored_bytes": (int(dom["stored"] / dom["launches"])
+                                  if dom["stored"] else None),
+                 "stored_GBps": (round(dom["stored"] / (dom["ms"] * 1e-3) / 1e9, 1)
+                                 if dom["stored"] else None),
+                 "avg_launch_ms": round(dom["ms"] / dom["launches"], 4),
+                 "launches_per_pass": dom["launches"], "pass_ms": round(dom["ms"], 4),
+                 "copy_GBps_measured": copy["GBps"]}
+    # the roofline line describes the headline mode's dominant kernel (the
+    # batched backward when a batched mode is the value); the plain path's is
+    # kept beside it
+    headline_roofline = dict(modes[best].get("roofline", plain_roofline))
+    headline_roofline["copy_GBps_measured"] = copy["GBps"]
+    headline_roofline["mode"] = best
     out = {
         "metric": "frames/s batched vs plain MLEM at <=1e-4 rel err; p50/p99 latency; HBM GB/s",
         "value": modes[best]["frames_per_s"],
@@ -533,21 +612,8 @@ def main():
         "modes": modes,
         "plain_frames_per_s": plain_fps,
         "speedup_vs_plain": modes[best]["speedup_vs_plain"],
-        "roofline": {"bound": "hbm", "kernel": dom["kernel"], "achieved": round(dom["gbs"], 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(dom["gbs"] / HBM_PEAK_GBS, 4),
-                     "traffic": int(traffic) if traffic else None,
-                     "traffic_source": ("rocprofv3 FETCH_SIZE x2.00 + WRITE_SIZE x1.00 per launch, "
-                                        "factors from a 2 GiB streaming copy") if traffic else None,
-                     "algorithmic_bytes": int(dom["bytes"] / dom["launches"]),
-                     "bytes_basis": "4-byte value + 4-byte index per stored entry",
-                     "stored_bytes": (int(dom["stored"] / dom["launches"])
-                                      if dom["stored"] else None),
-                     "stored_GBps": (round(dom["stored"] / (dom["ms"] * 1e-3) / 1e9, 1)
-                                     if dom["stored"] else None),
-                     "avg_launch_ms": round(dom["ms"] / dom["launches"], 4),
-                     "launches_per_pass": dom["launches"], "pass_ms": round(dom["ms"], 4),
-                     "copy_GBps_measured": copy["GBps"]},
+        "roofline": headline_roofline,
+        "roofline_plain": plain_roofline,
         "passes": {"forward_kernel": fwd_name,
                    "forward_ms": round(ms_f, 4), "forward_GBps": round(gbs_f, 1),
                    "backward_ms": round(ms_b, 4), "backward_GBps": round(gbs_b, 1),
@@ -562,6 +628,70 @@ def main():
         "device": dev,
     }
     print(json.dumps(out), flush=True)
+
+
+# HBM bytes per launch of the 64-frame grouped backward (rocprofv3
+# FETCH_SIZE x2.00 + WRITE_SIZE x1.00, the factors of the 2 GiB streaming
+# copy), full-size workload, locality row order, keyed by (kernel, group,
+# row blocks)
+MEASURED_TRAFFIC_BATCHED = {
+    ("k_bwdm64x", 16, 138): 2 * 46954025 * 1024 + 281403 * 1024,
+}
+
+
+def batched_roofline(R, info, nf, m, n, nnz, vt, reps, timed, capi, dx, dy, st):
+    """Roofline of an F-frame plan's passes: the backward (the slower pass
+    and the dominant kernel of the batched iteration) and the forward, timed
+    with HIP events on the work stream over `reps` launches each.
+    Algorithmic bytes per pass (matrix value + 4-byte index per stored
+    entry, 4-byte row pointers, F-frame vectors):
+      forward   (b_v + 4)*nnz + 4*(m+1) + F*4*n (x) + F*4*m (y) + F*4*m (e)
+      backward  (b_v + 4)*nnz + 4*(m+1) + F*4*m (e) + F*4*n (x) + 4*n (s)
+                + F*4*n (x')"""
+    from raptr_amd.device import DeviceArray
+    bv = 4 if vt == "f32" else 2
+    e = DeviceArray(m * nf, np.float32)
+    fwd = lambda: capi.call("raptr_recon_forward", R.handle, nf, dx.ptr, dy.ptr, e.ptr, st)  # noqa
+    bwd = lambda: capi.call("raptr_recon_backward", R.handle, nf, dx.ptr, st)  # noqa: E731
+    fwd(), bwd()
+    ms_f = timed(lambda: [fwd() for _ in range(reps)]) / reps
+    ms_b = timed(lambda: [bwd() for _ in range(reps)]) / reps
+    del e
+    bytes_f = (bv + 4) * nnz + 4 * (m + 1) + nf * 4 * n + nf * 8 * m
+    bytes_b = (bv + 4) * nnz + 4 * (m + 1) + nf * 4 * m + nf * 8 * n + 4 * n
+    g = info.get("group", 0)
+    kind = info.get("grouped_kind", 0)
+    grouped = bool(info.get("grouped64")) and nf == 64
+    key = None
+    if grouped and kind == 4:
+        kb = f"k_bwdm64b<{vt}, {g}> (per-XCD brick ranges, update fused; one launch per pass)"
+        key = ("k_bwdm64b", g, info["row_blocks"])
+    elif grouped and kind == 3:
+        kb = f"k_bwdm64x<{vt}, {g}> + k_bwd_xreduce (per-XCD row ranges)"
+        key = ("k_bwdm64x", g, info["row_blocks"])
+    else:
+        kb = "general F-frame backward"
+    stored_b = None
+    if grouped:
+        # union entries (key + G values), list pointers per (row block, group)
+        stored_b = (info["grouped_bwd_entries"] * (4 + g * bv) + 8 * info["row_blocks"] *
+                    (-(-n // g) + 1) + nf * 4 * m + nf * 8 * n + 4 * n)
+    traffic = MEASURED_TRAFFIC_BATCHED.get(key) if key else None
+    gb_b = bytes_b / (ms_b * 1e-3) / 1e9
+    gb_f = bytes_f / (ms_f * 1e-3) / 1e9
+    return {"bound": "hbm", "kernel": kb, "achieved": round(gb_b, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(gb_b / HBM_PEAK_GBS, 4),
+            "traffic": int(traffic) if traffic else None,
+            "traffic_source": ("rocprofv3 FETCH_SIZE x2.00 + WRITE_SIZE x1.00 per launch, "
+                               "factors from a 2 GiB streaming copy") if traffic else None,
+            "algorithmic_bytes": int(bytes_b), "bytes_basis":
+                "4-byte value (2 for 16-bit) + 4-byte index per matrix entry, F-frame vectors",
+            "stored_bytes": int(stored_b) if stored_b else None,
+            "stored_GBps": round(stored_b / (ms_b * 1e-3) / 1e9, 1) if stored_b else None,
+            "avg_launch_ms": round(ms_b, 4), "launches_per_pass": 1, "frames": nf,
+            "forward": {"kernel": f"k_fwdm64<{vt}, {g}>" if grouped else "general F-frame forward",
+                        "ms": round(ms_f, 4), "algorithmic_bytes": int(bytes_f),
+                        "achieved": round(gb_f, 1), "frac": round(gb_f / HBM_PEAK_GBS, 4)}}
 
 
 def copy_peak(timed, st, nbytes=2 << 30, reps=5):

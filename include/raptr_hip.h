@@ -185,8 +185,10 @@ int raptr_sm_build_csr_ex(const raptr_setup_t* setup, const raptr_grid_t* grid,
 /* Sensitivity image s_j = sum over ALL channels of the setup of A_ij, in
  * FP64 (d_sens[nvox]); the rows are generated and reduced on the fly, never
  * stored.  Replaces the sensitivity file the reference reads
- * (reco.cu:171-204).  Accumulation uses FP64 atomics (order-dependent in the
- * last bits of FP64 only). */
+ * (reco.cu:171-204).  The column sums are accumulated in 64-bit fixed
+ * point (at least 2^-37 resolution at the full-size workload, the float
+ * matrix values themselves exact from 2^-14 mm up) with integer atomics, so
+ * the result is bit-reproducible whatever the order of the additions. */
 int raptr_sensitivity_f64(const raptr_setup_t* setup, const raptr_grid_t* grid,
                           int nrays, uint32_t seed, int raygen, double* d_sens,
                           raptr_stream_t stream);
@@ -407,6 +409,20 @@ int raptr_recon_row_ordered(const raptr_recon* plan);
  * d_channels.  Allocates its own scratch (about 26 bytes per channel). */
 int raptr_locality_order(const raptr_setup_t* setup, const int* d_channels, int m, int tile,
                          int* d_order, raptr_stream_t stream);
+/* raptr_locality_order with the key order chosen: RAPTR_ORDER_ANGLE_MAJOR
+ * (the order above) or RAPTR_ORDER_Y_MAJOR: by the midpoint's y sum
+ * (id0y + id1y) first, then angle, midpoint z in tiles of `tile`, slope.
+ * The lines of a two-head setup rotated about y stay within a few voxels of
+ * their midpoint's y, so a range of consecutive y-major rows crosses only a
+ * band of the grid: the rows one XCD processes at a time touch a fraction
+ * of the image (batched forward) and the voxels that gather one row's
+ * ratios share that band (batched backward).  RAPTR_ORDER_SLOPE_TILES: the
+ * angle-major order with the slopes (id1z - id0z, id1y - id0y) in 4 x 4
+ * tiles, so 16 consecutive rows through one midpoint have nearly the same
+ * direction and cover nearly the same voxels. */
+enum { RAPTR_ORDER_ANGLE_MAJOR = 0, RAPTR_ORDER_Y_MAJOR = 1, RAPTR_ORDER_SLOPE_TILES = 2 };
+int raptr_locality_order_ex(const raptr_setup_t* setup, const int* d_channels, int m, int tile,
+                            int kind, int* d_order, raptr_stream_t stream);
 int raptr_recon_layout(const raptr_recon* plan);
 /* Tiled layout kernel shape: lanes per row segment (4, 8, 16) and unroll
  * (1, 2, 4) of the forward and the backward. */
@@ -512,9 +528,14 @@ int64_t raptr_recon_transpose_nnz(const raptr_recon* plan);
  * yt = A x, e = y/yt (0 where y or yt is 0), c = A^T e,
  * x = x*c/(s + (s==0)), x *= 1/sum(x) (FP64 sum).  With use_graph and a
  * non-null stream the iteration is captured once into a HIP graph (cached
- * per frame count and buffers) and replayed. */
+ * per frame count and buffers, raptr_recon_graph_count) and replayed. */
 int raptr_recon_mlem(raptr_recon* plan, int nframes, const float* d_y, float* d_x,
                      int niter, int use_graph, raptr_stream_t stream);
+/* Captured iterations the plan holds (raptr_recon_mlem with use_graph):
+ * one per distinct (nframes, d_y, d_x), at most 16, the least recently
+ * used destroyed first; all are destroyed when a kernel shape changes.
+ * -1 for a null plan. */
+int raptr_recon_graph_count(const raptr_recon* plan);
 /* The two passes alone (timing, tests): forward + ratio into d_e
  * ([m][nframes]); backward over all row blocks with the update into the
  * plan's internal image buffer. */

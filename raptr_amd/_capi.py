@@ -134,11 +134,13 @@ _SIGS = {
     "raptr_recon_set_forward_kind": (i32, [vp, i32]),
     "raptr_recon_get_forward_kind": (i32, [vp, P(i32), P(i32), P(i64)]),
     "raptr_locality_order": (i32, [vp, vp, i32, i32, vp, vp]),
+    "raptr_locality_order_ex": (i32, [vp, vp, i32, i32, i32, vp, vp]),
     "raptr_recon_layout": (i32, [vp]),
     "raptr_recon_set_blocks_per_launch": (i32, [vp, i32]),
     "raptr_recon_get_blocks_per_launch": (i32, [vp]),
     "raptr_recon_transpose_nnz": (i64, [vp]),
     "raptr_recon_set_broadcast": (i32, [vp, i32]),
+    "raptr_recon_graph_count": (i32, [vp]),
     "raptr_recon_get_broadcast": (i32, [vp]),
     "raptr_recon_set_grouped": (i32, [vp, i32, i32, i32]),
     "raptr_recon_get_grouped": (i32, [vp, vp, vp, vp, vp, vp, vp]),

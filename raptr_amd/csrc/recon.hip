@@ -1269,8 +1269,11 @@ __global__ __launch_bounds__(256) void k_bwdg64(int ngc, const int64_t* __restri
 // D[8*(r/4)+4*(l/32)+r%4][l%32] in register r (16 registers).  Here A = the
 // members' values of K union entries (rows of A = group members), B = a
 // block of frames of those entries' gathered frame vectors (16 or 32
-// frames: 64 or 128 contiguous bytes per entry), D = the members' sums for
-// that frame block; 64/W instructions per K entries cover the 64 frames.
+// frames), D = the members' sums for that frame block; 64/W instructions
+// per K entries cover the 64 frames.  Frame block fb, column c holds frame
+// FB*c + fb (FB = 64/W blocks): lane l gathers frames FB*(l%W) .. FB*(l%W) +
+// FB-1 of its entry with one FB-float vector load, so the W lanes of an
+// entry read its 256-byte frame vector with one instruction.
 // The values and keys arrive with coalesced vector loads (lane l loads
 // member l%W of entry l/W), so nothing streams through the scalar cache.
 typedef float f4v __attribute__((ext_vector_type(4)));
@@ -1338,11 +1341,10 @@ __device__ __forceinline__ void mstep(const MSet<U>& S, MSet<U>& T, bool load_t,
                                       int kq, int rw, const float* __restrict__ base,
                                       typename MF<G>::acc_t (&acc)[MF<G>::FB]) {
   using M = MF<G>;
-  float g[U][M::FB];
+  typedef float gvec __attribute__((ext_vector_type(M::FB)));
+  gvec g[U];
 #pragma unroll
-  for (int u = 0; u < U; ++u)
-#pragma unroll
-    for (int fb = 0; fb < M::FB; ++fb) g[u][fb] = base[int64_t(S.c[u]) * 64 + M::W * fb];
+  for (int u = 0; u < U; ++u) g[u] = *reinterpret_cast<const gvec*>(base + int64_t(S.c[u]) * 64);
   __builtin_amdgcn_sched_barrier(0);
   if (load_t) mload<VT, G, U>(T, key, val, kt, kq, rw);
   __builtin_amdgcn_sched_barrier(0);
@@ -1404,7 +1406,7 @@ __global__ __launch_bounds__(256) void k_fwdm64(int64_t ng, int64_t m,
   if (g >= ng) return;
   typename M::acc_t acc[M::FB];
   acc_zero<G>(acc);
-  mfma_sum<VT, G, U>(gcol, gval, gp[g], gp[g + 1], x + rw, kq, rw, acc);
+  mfma_sum<VT, G, U>(gcol, gval, gp[g], gp[g + 1], x + M::FB * rw, kq, rw, acc);
 #pragma unroll
   for (int r = 0; r < M::NR; ++r) {
     const int64_t row = int64_t(G) * g + M::member(lane, r);
@@ -1412,7 +1414,7 @@ __global__ __launch_bounds__(256) void k_fwdm64(int64_t ng, int64_t m,
     const float sc = VT == kF32 ? 1.0f : rscale[row];
 #pragma unroll
     for (int fb = 0; fb < M::FB; ++fb) {
-      const int64_t o = row * 64 + M::W * fb + rw;
+      const int64_t o = row * 64 + M::FB * rw + fb;
       const float yy = y[o];
       const float yt = sc * acc[fb][r];
       const float t = (yt != 0.0f && yy != 0.0f) ? yy / yt : 0.0f;
@@ -1443,7 +1445,7 @@ __global__ __launch_bounds__(256) void k_bwdm64(int ngc, const int64_t* __restri
   if (h < ngc) {
     for (int bb = 0; bb < nbl; ++bb)
       mfma_sum<VT, 16, 2>(grow, gval, gp[int64_t(bb) * ngc + h], gp[int64_t(bb) * ngc + h + 1],
-                          e + r16, kq, r16, acc);
+                          e + 4 * r16, kq, r16, acc);
   }
   double part[4] = {0.0, 0.0, 0.0, 0.0};
   if (h < ngc) {
@@ -1458,7 +1460,7 @@ __global__ __launch_bounds__(256) void k_bwdm64(int ngc, const int64_t* __restri
       }
 #pragma unroll
       for (int fb = 0; fb < 4; ++fb) {
-        const int64_t o = int64_t(j) * 64 + 16 * fb + r16;
+        const int64_t o = int64_t(j) * 64 + 4 * r16 + fb;
         const float cv = first ? acc[fb][i] : c[o] + acc[fb][i];
         if (!last) {
           c[o] = cv;
@@ -1473,7 +1475,7 @@ __global__ __launch_bounds__(256) void k_bwdm64(int ngc, const int64_t* __restri
   if (last) {
     __shared__ double red[256 / kWave][4][64];
 #pragma unroll
-    for (int fb = 0; fb < 4; ++fb) red[w][kq][16 * fb + r16] = part[fb];
+    for (int fb = 0; fb < 4; ++fb) red[w][kq][4 * r16 + fb] = part[fb];
     __syncthreads();
     if (threadIdx.x < 64) {
       double t = 0.0;
@@ -1514,14 +1516,14 @@ __global__ __launch_bounds__(256) void k_bwdm64x(int ngc, int nblk, int n,
   acc_zero<G>(acc);
   for (int b = b0; b < b1; ++b)
     mfma_sum<VT, G, U>(grow, gval, gp[int64_t(b) * ngc + h], gp[int64_t(b) * ngc + h + 1],
-                       e + rw, kq, rw, acc);
+                       e + M::FB * rw, kq, rw, acc);
   float* out = c8 + int64_t(xcd) * n * 64;
 #pragma unroll
   for (int r = 0; r < M::NR; ++r) {
     const int j = cols[int64_t(h) * G + M::member(lane, r)];
     if (j < 0) continue;
 #pragma unroll
-    for (int fb = 0; fb < M::FB; ++fb) out[int64_t(j) * 64 + M::W * fb + rw] = acc[fb][r];
+    for (int fb = 0; fb < M::FB; ++fb) out[int64_t(j) * 64 + M::FB * rw + fb] = acc[fb][r];
   }
 }
 
@@ -1670,17 +1672,21 @@ __host__ __device__ inline void brick_dims(int G, int& bx, int& by, int& bz) {
   bz = G >= 16 ? 4 : 2;
 }
 
-// cols[h*G + s]: member s of brick h (bricks in x, y, z order, z fastest;
-// members x, y, z order, z fastest; -1 outside the grid)
+// cols[h*G + s]: member s of brick h; -1 outside the grid.  Bricks in y,
+// x, z order (z fastest), so consecutive bricks fill x-z planes: the lines
+// of a two-head setup rotated about y run nearly parallel to those planes,
+// so the bricks one XCD holds at a time (consecutive ids) are crossed by the
+// same lines over many bricks, and the ratio rows they gather stay in its
+// L2.  Members in x, y, z order, z fastest.
 __global__ void k_brick_cols(int nx, int ny, int nz, int G, int64_t ngc, int* __restrict__ cols) {
   int bx, by, bz;
   brick_dims(G, bx, by, bz);
-  const int nbz = (nz + bz - 1) / bz, nby = (ny + by - 1) / by;
+  const int nbz = (nz + bz - 1) / bz, nbx = (nx + bx - 1) / bx;
   for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < ngc * G;
        t += int64_t(gridDim.x) * blockDim.x) {
     const int64_t h = t / G;
     const int sidx = int(t - h * G);
-    const int hz = int(h % nbz), hy = int((h / nbz) % nby), hx = int(h / (int64_t(nbz) * nby));
+    const int hz = int(h % nbz), hx = int((h / nbz) % nbx), hy = int(h / (int64_t(nbz) * nbx));
     const int mz = sidx % bz, my = (sidx / bz) % by, mx = sidx / (bz * by);
     const int ix = hx * bx + mx, iy = hy * by + my, iz = hz * bz + mz;
     cols[t] = (ix < nx && iy < ny && iz < nz) ? (ix * ny + iy) * nz + iz : -1;
@@ -2643,8 +2649,12 @@ __global__ void k_rows_scatter(const float* __restrict__ src, float* __restrict_
 // first, then the midpoint inside it), then its slope (z1 - z0, y1 - y0).
 // Lines through one voxel then sit in a few runs of consecutive keys, and a
 // run of consecutive rows covers a compact bundle of lines (T > 1).
+// kind 0: (angle, midpoint tile, midpoint in tile, slope); kind 1 (y-major):
+// (midpoint y, angle, midpoint z tile, midpoint z in tile, slope); kind 2:
+// as kind 0 with the slopes in 4 x 4 tiles (slope tile, slope in tile)
 __global__ void k_locality_keys(raptr_setup_t s, const int* __restrict__ cnl, int m, int T,
-                                uint64_t* __restrict__ keys, uint32_t* __restrict__ idx) {
+                                int kind, uint64_t* __restrict__ keys,
+                                uint32_t* __restrict__ idx) {
   const uint64_t zs = uint64_t(s.n0z + s.n1z - 1), ys = uint64_t(s.n0y + s.n1y - 1);
   const uint64_t nzt = (zs + T - 1) / T, nyt = (ys + T - 1) / T;
   for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < m;
@@ -2652,13 +2662,30 @@ __global__ void k_locality_keys(raptr_setup_t s, const int* __restrict__ cnl, in
     int ida, id0z, id0y, id1z, id1y;
     cnl_split(s, cnl[i], ida, id0z, id0y, id1z, id1y);
     const uint64_t zm = uint64_t(id0z + id1z), ym = uint64_t(id0y + id1y);
-    uint64_t k = uint64_t(ida);
-    k = k * nzt + zm / T;
-    k = k * nyt + ym / T;
-    k = k * T + zm % T;
-    k = k * T + ym % T;
-    k = k * zs + uint64_t(id1z - id0z + s.n0z - 1);
-    k = k * ys + uint64_t(id1y - id0y + s.n0y - 1);
+    uint64_t k;
+    if (kind == 1) {
+      k = ym;
+      k = k * uint64_t(s.na) + uint64_t(ida);
+      k = k * nzt + zm / T;
+      k = k * T + zm % T;
+    } else {
+      k = uint64_t(ida);
+      k = k * nzt + zm / T;
+      k = k * nyt + ym / T;
+      k = k * T + zm % T;
+      k = k * T + ym % T;
+    }
+    const uint64_t dz = uint64_t(id1z - id0z + s.n0z - 1), dy = uint64_t(id1y - id0y + s.n0y - 1);
+    if (kind == 2) {
+      constexpr uint64_t S = 4;
+      k = k * ((zs + S - 1) / S) + dz / S;
+      k = k * ((ys + S - 1) / S) + dy / S;
+      k = k * S + dz % S;
+      k = k * S + dy % S;
+    } else {
+      k = k * zs + dz;
+      k = k * ys + dy;
+    }
     keys[i] = k;
     idx[i] = uint32_t(i);
   }
@@ -3230,15 +3257,29 @@ int raptr_recon_row_ordered(const raptr_recon* R) { return R && R->rperm ? 1 : 0
 
 int raptr_locality_order(const raptr_setup_t* setup, const int* d_channels, int m, int tile,
                          int* d_order, raptr_stream_t stream) {
+  return raptr_locality_order_ex(setup, d_channels, m, tile, RAPTR_ORDER_ANGLE_MAJOR, d_order,
+                                 stream);
+}
+
+int raptr_locality_order_ex(const raptr_setup_t* setup, const int* d_channels, int m, int tile,
+                            int kind, int* d_order, raptr_stream_t stream) {
   if (!setup || m < 0 || tile < 1 || tile > 64 || (m > 0 && (!d_channels || !d_order)))
     return fail(RAPTR_ERR_INVALID_ARG, "bad locality order arguments");
+  if (kind != RAPTR_ORDER_ANGLE_MAJOR && kind != RAPTR_ORDER_Y_MAJOR &&
+      kind != RAPTR_ORDER_SLOPE_TILES)
+    return fail(RAPTR_ERR_INVALID_ARG, "unknown locality order kind %d", kind);
   if (m == 0) return RAPTR_OK;
   const raptr_setup_t s = *setup;
   if (s.na < 1 || s.n0z < 1 || s.n0y < 1 || s.n1z < 1 || s.n1y < 1)
     return fail(RAPTR_ERR_INVALID_ARG, "bad setup");
   const int zs = s.n0z + s.n1z - 1, ys = s.n0y + s.n1y - 1;
+  // largest key + 1 (kind 1 orders the same factors differently; kind 2
+  // pads the slope ranges to whole 4 x 4 tiles)
+  const double slopes = kind == RAPTR_ORDER_SLOPE_TILES
+                            ? double((zs + 3) / 4 * 4) * double((ys + 3) / 4 * 4)
+                            : double(zs) * ys;
   const double kmax = double(s.na) * ((zs + tile - 1) / tile) * ((ys + tile - 1) / tile) *
-                      tile * tile * double(zs) * ys;
+                      tile * tile * slopes;
   if (kmax >= 1.8e19) return fail(RAPTR_ERR_INVALID_ARG, "setup too large for 64-bit keys");
   int bits = 1;
   while (bits < 64 && double(uint64_t(1) << bits) < kmax) ++bits;
@@ -3264,7 +3305,7 @@ int raptr_locality_order(const raptr_setup_t* setup, const int* d_channels, int 
   uint32_t* v0 = cv.take<uint32_t>(size_t(m));
   uint32_t* v1 = cv.take<uint32_t>(size_t(m));
   hipLaunchKernelGGL(k_locality_keys, dim3(grid_for(m, 256, 8192)), dim3(256), 0, st, s,
-                     d_channels, m, tile, k0, v0);
+                     d_channels, m, tile, kind, k0, v0);
   int res = 0;
   int rc = radix_sort_pairs(k0, k1, v0, v1, m, bits, st, &res, cv);
   if (!rc) {
@@ -3280,10 +3321,7 @@ int raptr_locality_order(const raptr_setup_t* setup, const int* d_channels, int 
 
 int raptr_recon_destroy(raptr_recon* R) {
   if (!R) return RAPTR_OK;
-  for (auto& kv : R->graphs) {
-    if (kv.second.second) (void)hipGraphExecDestroy(kv.second.second);
-    if (kv.second.first) (void)hipGraphDestroy(kv.second.first);
-  }
+  clear_graphs(R);
   for (void* p : R->allocs) (void)hipFree(p);
   delete R;
   return RAPTR_OK;
@@ -3312,11 +3350,7 @@ int raptr_recon_set_config(raptr_recon* R, int fwd_lanes, int fwd_unroll, int bw
     return fail(RAPTR_ERR_INVALID_ARG, "backward unroll must be 1, 2 or 4");
   if (R->fwd_lanes != fwd_lanes || R->fwd_unroll != fwd_unroll ||
       R->bwd_unroll != bwd_unroll) {
-    for (auto& kv : R->graphs) {
-      if (kv.second.second) (void)hipGraphExecDestroy(kv.second.second);
-      if (kv.second.first) (void)hipGraphDestroy(kv.second.first);
-    }
-    R->graphs.clear();
+    clear_graphs(R);
   }
   R->fwd_lanes = fwd_lanes;
   R->fwd_unroll = fwd_unroll;
@@ -3344,11 +3378,7 @@ int raptr_recon_set_backward(raptr_recon* R, int staged, int lanes_per_column, i
                 "lanes per column must be 8, 16 or 32 (or 64 for the per-XCD backward)");
   if (workgroups < 1) return fail(RAPTR_ERR_INVALID_ARG, "workgroups must be positive");
   if (R->bwd_lds != staged || R->bwd_lpc != lanes_per_column || R->bwd_wgs != workgroups) {
-    for (auto& kv : R->graphs) {
-      if (kv.second.second) (void)hipGraphExecDestroy(kv.second.second);
-      if (kv.second.first) (void)hipGraphDestroy(kv.second.first);
-    }
-    R->graphs.clear();
+    clear_graphs(R);
   }
   R->bwd_lds = staged;
   R->bwd_lpc = lanes_per_column;
@@ -3363,11 +3393,7 @@ int raptr_recon_set_compact_rows(raptr_recon* R, int on) {
                 "no 16-bit indices (single-frame blocked plans: transpose rows with block_rows "
                 "<= 65536, forward columns with <= 262144 voxels)");
   if (R->bwd_r16 != (on && R->brow16 ? 1 : 0) || R->fwd_c16 != (on && R->fcol16 ? 1 : 0)) {
-    for (auto& kv : R->graphs) {
-      if (kv.second.second) (void)hipGraphExecDestroy(kv.second.second);
-      if (kv.second.first) (void)hipGraphDestroy(kv.second.first);
-    }
-    R->graphs.clear();
+    clear_graphs(R);
   }
   R->bwd_r16 = on && R->brow16 ? 1 : 0;
   R->fwd_c16 = on && R->fcol16 ? 1 : 0;
@@ -3382,11 +3408,7 @@ int raptr_recon_set_forward_kind(raptr_recon* R, int kind) {
   if (kind == 2 && !R->lcol)
     return fail(RAPTR_ERR_INVALID_ARG, "no LDS column unions on this plan");
   if (R->fwd_kind != kind) {
-    for (auto& kv : R->graphs) {
-      if (kv.second.second) (void)hipGraphExecDestroy(kv.second.second);
-      if (kv.second.first) (void)hipGraphDestroy(kv.second.first);
-    }
-    R->graphs.clear();
+    clear_graphs(R);
   }
   R->fwd_kind = kind;
   return RAPTR_OK;
@@ -3414,11 +3436,7 @@ int raptr_recon_set_blocks_per_launch(raptr_recon* R, int blocks) {
   if (blocks < 1 || blocks > 64)
     return fail(RAPTR_ERR_INVALID_ARG, "blocks per launch must be in [1, 64]");
   if (R->bwd_bpl != blocks) {
-    for (auto& kv : R->graphs) {
-      if (kv.second.second) (void)hipGraphExecDestroy(kv.second.second);
-      if (kv.second.first) (void)hipGraphDestroy(kv.second.first);
-    }
-    R->graphs.clear();
+    clear_graphs(R);
   }
   R->bwd_bpl = blocks;
   return RAPTR_OK;
@@ -3431,17 +3449,15 @@ int64_t raptr_recon_transpose_nnz(const raptr_recon* R) { return R ? R->bnnz : -
 int raptr_recon_set_broadcast(raptr_recon* R, int on) {
   if (!R) return fail(RAPTR_ERR_INVALID_ARG, "null plan");
   if (R->bcast64 != (on ? 1 : 0)) {
-    for (auto& kv : R->graphs) {
-      if (kv.second.second) (void)hipGraphExecDestroy(kv.second.second);
-      if (kv.second.first) (void)hipGraphDestroy(kv.second.first);
-    }
-    R->graphs.clear();
+    clear_graphs(R);
   }
   R->bcast64 = on ? 1 : 0;
   return RAPTR_OK;
 }
 
 int raptr_recon_get_broadcast(const raptr_recon* R) { return R ? R->bcast64 : -1; }
+
+int raptr_recon_graph_count(const raptr_recon* R) { return R ? int(R->graphs.size()) : -1; }
 
 int raptr_recon_set_grouped(raptr_recon* R, int on, int unroll, int kind) {
   if (!R) return fail(RAPTR_ERR_INVALID_ARG, "null plan");
@@ -3455,11 +3471,7 @@ int raptr_recon_set_grouped(raptr_recon* R, int on, int unroll, int kind) {
                 R->grp_g == 4 ? 16 : (R->grp_g == 8 ? 8 : 4),
                 R->grp_g == 16 ? " or 8, 16 (kind 2)" : "", unroll, kind);
   if (R->grp64 != (on ? 1 : 0) || R->grp_unroll != unroll || R->grp_pipe != kind) {
-    for (auto& kv : R->graphs) {
-      if (kv.second.second) (void)hipGraphExecDestroy(kv.second.second);
-      if (kv.second.first) (void)hipGraphDestroy(kv.second.first);
-    }
-    R->graphs.clear();
+    clear_graphs(R);
   }
   R->grp64 = on ? 1 : 0;
   R->grp_unroll = unroll;
@@ -3490,11 +3502,7 @@ int raptr_recon_set_tiled(raptr_recon* R, int fwd_lanes, int fwd_unroll, int bwd
     return fail(RAPTR_ERR_INVALID_ARG, "tiled lanes must be 4, 8 or 16 and unroll 1, 2 or 4");
   if (R->t_fwd_lanes != fwd_lanes || R->t_fwd_unroll != fwd_unroll ||
       R->t_bwd_lanes != bwd_lanes || R->t_bwd_unroll != bwd_unroll) {
-    for (auto& kv : R->graphs) {
-      if (kv.second.second) (void)hipGraphExecDestroy(kv.second.second);
-      if (kv.second.first) (void)hipGraphDestroy(kv.second.first);
-    }
-    R->graphs.clear();
+    clear_graphs(R);
   }
   R->t_fwd_lanes = fwd_lanes;
   R->t_fwd_unroll = fwd_unroll;
@@ -3546,11 +3554,19 @@ int raptr_recon_mlem(raptr_recon* R, int nframes, const float* d_y, float* d_x, 
     for (int it = 0; it < niter; ++it) RAPTR_TRY(iteration(R, nframes, d_y, d_x, st));
     return RAPTR_OK;
   }
-  long long key = (long long)nframes ^ ((long long)(uintptr_t)d_y << 7) ^
-                  ((long long)(uintptr_t)d_x << 19);
+  const auto key = std::make_tuple(nframes, uintptr_t(d_y), uintptr_t(d_x));
   auto it = R->graphs.find(key);
   hipGraphExec_t exec = nullptr;
   if (it == R->graphs.end()) {
+    if (R->graphs.size() >= kMaxGraphs) {
+      auto lru = R->graphs.begin();
+      for (auto jt = R->graphs.begin(); jt != R->graphs.end(); ++jt)
+        if (jt->second.used < lru->second.used) lru = jt;
+      // a replay of the evicted graph may still be queued on some stream
+      RAPTR_HIP_TRY(hipDeviceSynchronize());
+      destroy_graph(lru->second);
+      R->graphs.erase(lru);
+    }
     hipGraph_t graph = nullptr;
     RAPTR_HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     int irc = iteration(R, nframes, d_y, d_x, st);
@@ -3565,9 +3581,10 @@ int raptr_recon_mlem(raptr_recon* R, int nframes, const float* d_y, float* d_x, 
       (void)hipGraphDestroy(graph);
       return fail(RAPTR_ERR_DEVICE, "graph instantiate: %s", hipGetErrorString(ie));
     }
-    R->graphs[key] = {graph, exec};
+    R->graphs[key] = {graph, exec, ++R->graph_tick};
   } else {
-    exec = it->second.second;
+    exec = it->second.exec;
+    it->second.used = ++R->graph_tick;
   }
   for (int i = 0; i < niter; ++i) RAPTR_HIP_TRY(hipGraphLaunch(exec, st));
   return RAPTR_OK;

@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <map>
+#include <tuple>
 #include <vector>
 
 #include "primitives.hpp"
@@ -175,10 +176,37 @@ struct raptr_recon {
   size_t bytes = 0;
   std::vector<void*> allocs;
   std::vector<size_t> asizes;  // bytes of allocs[i]
-  std::map<long long, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
+  // captured iterations, keyed by (frames, y, x) exactly; at most
+  // kMaxGraphs, the least recently used destroyed first
+  struct CachedGraph {
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+    uint64_t used;
+  };
+  std::map<std::tuple<int, uintptr_t, uintptr_t>, CachedGraph> graphs;
+  uint64_t graph_tick = 0;
 };
 
 namespace raptr {
+
+constexpr size_t kMaxGraphs = 16;
+
+inline void destroy_graph(raptr_recon::CachedGraph& g) {
+  if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  if (g.graph) (void)hipGraphDestroy(g.graph);
+  g.exec = nullptr;
+  g.graph = nullptr;
+}
+
+// every captured iteration of the plan (after a change of kernel shape);
+// replays may still be queued on the caller's streams, so the device is
+// drained first
+inline void clear_graphs(raptr_recon* R) {
+  if (R->graphs.empty()) return;
+  (void)hipDeviceSynchronize();
+  for (auto& kv : R->graphs) destroy_graph(kv.second);
+  R->graphs.clear();
+}
 
 template <class T>
 int ralloc(raptr_recon* R, T** p, size_t count) {

@@ -19,11 +19,13 @@
 //         dropped with a second ballot compaction into an LDS row buffer;
 //      4. row buffers are flushed as pieces into a staging area and packed
 //         into a CSR with 64-bit row pointers per channel chunk.
+#include <cmath>
+
 #include "primitives.hpp"
 
 namespace raptr {
-namespace {
 
+namespace {
 constexpr int kSmBlock = 256;
 constexpr int kCandCap = 3072;  // >= 2048 + one cull round (1024)
 constexpr int kCandFlush = 2048;
@@ -103,6 +105,11 @@ struct BuildArgs {
   unsigned* piece_count;
   unsigned* flags;  // bit0 staging overflow, bit1 piece overflow
   double* colsum;   // sensitivity / back-projection mode
+  // sensitivity: exact fixed-point column sums (value * 2^fx_bits rounded to
+  // an integer, added with 64-bit integer atomics, so the result does not
+  // depend on the order in which the atomics land); null: FP64 atomics
+  unsigned long long* colsum_fx;
+  double fx_scale;
   const float* wgt; // back-projection mode: weight per list entry (nullptr: 1)
   const float* xin; // forward-projection mode: image
   double* fwd;      // forward-projection mode: per-row FP64 projection
@@ -251,7 +258,13 @@ __global__ __launch_bounds__(kSmBlock) void k_sm_build(BuildArgs A) {
         const double wt = A.wgt ? double(A.wgt[li]) : 1.0;
         for (int c = t; c < cand_n; c += kSmBlock) {
           float a = cval[c];
-          if (a != 0.0f && wt != 0.0) atomicAdd(&A.colsum[cand[c]], double(a) * wt);
+          if (A.colsum_fx) {
+            if (a != 0.0f)
+              atomicAdd(&A.colsum_fx[cand[c]],
+                        (unsigned long long)llrint(double(a) * A.fx_scale));
+          } else if (a != 0.0f && wt != 0.0) {
+            atomicAdd(&A.colsum[cand[c]], double(a) * wt);
+          }
         }
         __syncthreads();
         cand_n = 0;
@@ -448,6 +461,15 @@ __global__ __launch_bounds__(kSmBlock) void k_sm_build(BuildArgs A) {
     }
     if ((MODE == kModeCount || MODE == kModeFill) && t == 0) A.row_nnz[rl] = flushed;
     __syncthreads();
+  }
+}
+
+// fixed-point column sums -> FP64, in place
+__global__ void k_fx_to_f64(unsigned long long* v, int n, double inv_scale) {
+  for (int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; j < n;
+       j += int64_t(gridDim.x) * blockDim.x) {
+    const unsigned long long q = v[j];
+    reinterpret_cast<double*>(v)[j] = double(q) * inv_scale;
   }
 }
 
@@ -724,15 +746,35 @@ int raptr_sensitivity_f64(const raptr_setup_t* setup, const raptr_grid_t* grid, 
   hipStream_t st = S(stream);
   RAPTR_HIP_TRY(hipMemsetAsync(d_sens, 0, size_t(A.G.nvox) * sizeof(double), st));
   int64_t nch = int64_t(setup->na) * setup->n0z * setup->n0y * setup->n1z * setup->n1y;
+  // Fixed point with `bits` fraction bits: no column sum exceeds (channels)
+  // x (voxel diagonal, the longest mean chord), so 2^bits times that bound
+  // stays below 2^62.  A float value of at least 2^(23 - bits) is a whole
+  // multiple of 2^-bits (exact); smaller ones are rounded by at most
+  // 2^-(bits + 1).  The integer sums are the same whatever the order of the
+  // additions, so the sensitivity is reproducible bit for bit.
+  const double diag = std::sqrt(double(grid->d[0]) * grid->d[0] +
+                                double(grid->d[1]) * grid->d[1] +
+                                double(grid->d[2]) * grid->d[2]);
+  const double bound = double(nch > 0 ? nch : 1) * (diag > 0.0 ? diag : 1.0);
+  int bits = int(std::floor(62.0 - std::log2(bound)));
+  if (bits > 52) bits = 52;
+  if (bits < 8)
+    return fail(RAPTR_ERR_NOT_SUPPORTED, "sensitivity bound %.3g too large for fixed point",
+                bound);
   A.list = nullptr;
   A.wgt = nullptr;
   A.colsum = d_sens;
+  A.colsum_fx = reinterpret_cast<unsigned long long*>(d_sens);
+  A.fx_scale = std::ldexp(1.0, bits);
   const int64_t kRowsPerLaunch = 1 << 20;
   for (int64_t f = 0; f < nch; f += kRowsPerLaunch) {
     A.first = int(f);
     A.rows = int(nch - f < kRowsPerLaunch ? nch - f : kRowsPerLaunch);
     RAPTR_TRY(launch_build<kModeColsum>(A, lds, st));
   }
+  hipLaunchKernelGGL(k_fx_to_f64, dim3(grid_for(A.G.nvox, 256)), dim3(256), 0, st,
+                     A.colsum_fx, A.G.nvox, std::ldexp(1.0, -bits));
+  RAPTR_LAUNCH_CHECK();
   return RAPTR_OK;
 }
 

@@ -169,11 +169,14 @@ def build_csr(setup, grid, cnl_list, nrays, seed=1, raygen=capi.RAYGEN_RAND, str
 LOCALITY_TILE = 8
 
 
-def locality_order(setup, channels, stream=None, tile=LOCALITY_TILE):
+ORDERS = {"angle": 0, "y": 1, "slope": 2}
+
+
+def locality_order(setup, channels, stream=None, tile=LOCALITY_TILE, kind="angle"):
     """Plan row order that puts lines through the same voxels next to each
-    other (raptr_locality_order): positions into ``channels`` sorted by
-    angle, midpoint (in tile x tile tiles) and slope.  Returns a device
-    int32 array."""
+    other (raptr_locality_order_ex): positions into ``channels`` sorted by
+    angle, midpoint (in tile x tile tiles) and slope (kind "angle"), or by
+    the midpoint's y first (kind "y").  Returns a device int32 array."""
     if isinstance(channels, DeviceArray):
         d_ch, m = channels, channels.count
     else:
@@ -181,7 +184,8 @@ def locality_order(setup, channels, stream=None, tile=LOCALITY_TILE):
         d_ch, m = DeviceArray.from_numpy(lst, stream=stream), len(lst)
     out = DeviceArray(max(m, 1), np.int32)
     s = setup.struct()
-    capi.call("raptr_locality_order", C.byref(s), d_ch.ptr, int(m), int(tile), out.ptr, stream)
+    capi.call("raptr_locality_order_ex", C.byref(s), d_ch.ptr, int(m), int(tile),
+              ORDERS[kind], out.ptr, stream)
     return out
 
 

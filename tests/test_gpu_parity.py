@@ -303,6 +303,27 @@ def test_build_csr_matches_two_stage(gpu):
     np.testing.assert_array_equal(rp[:-1], np.searchsorted(pos, np.arange(len(lst))))
 
 
+def test_sensitivity_bit_reproducible_and_exact(gpu, oracle):
+    """The FP64 sensitivity of the 20,480-channel workload (all channels)
+    accumulates in fixed point: two computations are bit-identical, and both
+    equal the exactly summed column sums of the oracle's float matrix to
+    1e-13 (FP64 atomics would give order-dependent last bits)."""
+    import math
+    from raptr_amd import sysmat
+    setup, grid, nrays = G.config_small()
+    s, g = oracle_geom(oracle, setup, grid)
+    a = sysmat.sensitivity(setup, grid, nrays).numpy()
+    b = sysmat.sensitivity(setup, grid, nrays).numpy()
+    np.testing.assert_array_equal(a, b)
+    lst = np.arange(setup.nchannels, dtype=np.int32)
+    rp, col, val, _ = oracle.build_csr(s, g, lst, nrays)
+    order = np.argsort(col, kind="stable")
+    cs, vs = col[order], val[order].astype(np.float64)
+    starts = np.searchsorted(cs, np.arange(grid.nvox + 1))
+    exact = np.array([math.fsum(vs[starts[j]:starts[j + 1]]) for j in range(grid.nvox)])
+    np.testing.assert_allclose(a, exact, rtol=1e-13, atol=1e-13)
+
+
 def test_sensitivity_and_forward_projection(gpu, oracle):
     from raptr_amd import sysmat
     from raptr_amd.device import DeviceArray

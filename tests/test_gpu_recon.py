@@ -265,6 +265,41 @@ def test_recon_graph_replays_match_eager(gpu, small_system, layout):
         R.mlem(S["y"], x0, 1, use_graph=False)  # eager work between replays
 
 
+def test_recon_graph_cache_exact_keys_and_bound(gpu, small_system):
+    """Each distinct (frames, y, x) buffer pair gets its own captured graph
+    (exact keys, no aliasing); the cache holds at most 16, least recently
+    used evicted; every replay, including a recaptured evicted pair, equals
+    the eager result bit for bit."""
+    from raptr_amd import _capi as capi
+    from raptr_amd.device import DeviceArray, Stream
+    from raptr_amd.recon import Reconstructor
+    S = small_system
+    n = S["grid"].nvox
+    R = Reconstructor(S["A"], S["sens"], "f32", 1)
+    count = lambda: capi.lib().raptr_recon_graph_count(R.handle)  # noqa: E731
+    x0 = np.full(n, 1.0 / n, np.float32)
+    y = S["y"].astype(np.float32)
+    eager = R.mlem(y, x0, 3, use_graph=False)
+    st = Stream()
+    pairs = [(DeviceArray.from_numpy(y), DeviceArray.from_numpy(x0)) for _ in range(18)]
+    assert count() == 0
+    for k, (dy, dx) in enumerate(pairs):
+        R.run(dy, dx, 3, 1, use_graph=True, stream=st.handle)
+        st.sync()
+        assert count() == min(k + 1, 16)
+        np.testing.assert_array_equal(dx.numpy(), eager)
+    for k in (17, 0):  # cached (no new capture), then evicted (recaptured)
+        dy, dx = pairs[k]
+        dx.copy_from(x0, stream=st.handle)
+        R.run(dy, dx, 3, 1, use_graph=True, stream=st.handle)
+        st.sync()
+        assert count() == 16
+        np.testing.assert_array_equal(dx.numpy(), eager)
+    inf = R.info()  # a shape change drops every graph
+    R.set_config(inf["fwd_lanes"], 1 if inf["fwd_unroll"] != 1 else 2, inf["bwd_unroll"])
+    assert count() == 0
+
+
 @pytest.mark.parametrize("layout,cap", [("blocked", 0), ("tiled", 0), ("tiled", 3000)])
 def test_recon_forward_ratio(gpu, small_system, layout, cap):
     """The forward pass alone: e = y / (A x) against an FP64 host product."""
