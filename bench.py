@@ -483,6 +483,11 @@ def main():
         x64 = O.mlem_f64(rp, col, v64, y.astype(np.float64), sens.numpy(), x64_30,
                          niter - ERR_ITERS)
         cpu_s = time.perf_counter() - t
+        # the batch's last frame (the per-frame check of the batched modes)
+        x64_63 = None
+        if plain_images is not None:
+            x64_63 = O.mlem_f64(rp, col, v64, Y[:, F - 1].astype(np.float64), sens.numpy(),
+                                x0.astype(np.float64), niter)
         # one host thread on a bounded sample: 3 iterations, scaled to niter
         O.lib().ro_set_threads(1)
         t = time.perf_counter()
@@ -503,10 +508,6 @@ def main():
             t32.append(time.perf_counter() - t)
         cpu32_s = float(np.median(t32)) * niter / CPU_ITERS
         del rp, col, val32
-        x64_63 = None
-        if plain_images is not None:
-            x64_63 = O.mlem_f64(rp, col, v64, Y[:, F - 1].astype(np.float64), sens.numpy(),
-                                x0.astype(np.float64), niter)
         for name in modes:
             if x64_63 is not None and "image63" in modes[name]:
                 modes[name]["rel_l2_vs_fp64_last_frame"] = rel_l2(modes[name].pop("image63"),
