@@ -182,6 +182,22 @@ int raptr_sm_build_csr_ex(const raptr_setup_t* setup, const raptr_grid_t* grid,
                           int* d_col, float* d_val, int64_t capacity,
                           int64_t* nnz, void* d_workspace, size_t workspace_bytes,
                           raptr_stream_t stream);
+/* raptr_sm_build_csr_ex without a workspace argument and without any
+ * device allocation: no counterpart in the reference; for callers that
+ * cannot allocate device memory (HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The build's staging area comes from a
+ * 64 MiB array of device memory that the library holds on every device it
+ * runs on until the process ends; rows are built in chunks that fit it, and
+ * a row with more candidates than it stages (about 7.5 million) is rejected
+ * with RAPTR_ERR_WORKSPACE.  Results are bit-identical to
+ * raptr_sm_build_csr_ex.  Every call on one device shares that device's
+ * array: calls that overlap in time on one device (other host threads or
+ * streams) race and must be serialised by the caller. */
+int raptr_sm_build_csr_noalloc(const raptr_setup_t* setup, const raptr_grid_t* grid,
+                               const int* d_list, int list_size, int nrays, uint32_t seed,
+                               int raygen, int flags, int64_t* d_row_ptr, int* d_col,
+                               float* d_val, int64_t capacity, int64_t* nnz,
+                               raptr_stream_t stream);
 /* Sensitivity image s_j = sum over ALL channels of the setup of A_ij, in
  * FP64 (d_sens[nvox]); the rows are generated and reduced on the fly, never
  * stored.  Replaces the sensitivity file the reference reads

@@ -93,6 +93,8 @@ _SIGS = {
                                  sz, vp]),
     "raptr_sm_build_csr_ex": (i32, [SP, GP, vp, i32, i32, u32, i32, i32, vp, vp, vp, i64,
                                     P(i64), vp, sz, vp]),
+    "raptr_sm_build_csr_noalloc": (i32, [SP, GP, vp, i32, i32, u32, i32, i32, vp, vp, vp, i64,
+                                         P(i64), vp]),
     "raptr_sensitivity_f64": (i32, [SP, GP, i32, u32, i32, vp, vp]),
     "raptr_backproject_f64": (i32, [SP, GP, vp, vp, i32, i32, u32, i32, vp, vp]),
     "raptr_forward_project_f64": (i32, [SP, GP, vp, i32, i32, u32, i32, vp, vp, vp]),

@@ -20,12 +20,46 @@
 //      4. row buffers are flushed as pieces into a staging area and packed
 //         into a CSR with 64-bit row pointers per channel chunk.
 #include <cmath>
+#include <mutex>
 
 #include "primitives.hpp"
 
 namespace raptr {
 
+// Scratch of the allocation-free build (raptr_sm_build_csr_noalloc): the
+// staging area, counters, piece table and scan space of the fused build in
+// device memory that exists from module load to process end, one copy per
+// device.  `scratch` is a host-side stand-in in host code: the code below
+// takes the array's device address once per device (hipGetSymbolAddress,
+// with that device current) and uses only that address.
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
 namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxScratchDevices] = {};
+
+// the current device's address of `scratch` (looked up on its first use)
+int scratch_address(void** out) {
+  int dev = 0;
+  RAPTR_HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxScratchDevices)
+    return fail(RAPTR_ERR_NOT_SUPPORTED, "device %d beyond the %d scratch slots", dev,
+                kMaxScratchDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(RAPTR_ERR_DEVICE, "hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                  hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return RAPTR_OK;
+}
+
 constexpr int kSmBlock = 256;
 constexpr int kCandCap = 3072;  // >= 2048 + one cull round (1024)
 constexpr int kCandFlush = 2048;
@@ -735,6 +769,27 @@ int raptr_sm_build_csr_ex(const raptr_setup_t* setup, const raptr_grid_t* grid,
   RAPTR_HIP_TRY(hipStreamSynchronize(st));
   *nnz = total;
   return RAPTR_OK;
+}
+
+int raptr_sm_build_csr_noalloc(const raptr_setup_t* setup, const raptr_grid_t* grid,
+                               const int* d_list, int list_size, int nrays, uint32_t seed,
+                               int raygen, int flags, int64_t* d_row_ptr, int* d_col,
+                               float* d_val, int64_t capacity, int64_t* nnz,
+                               raptr_stream_t stream) {
+  void* ws = nullptr;
+  RAPTR_TRY(scratch_address(&ws));
+  const int st = raptr_sm_build_csr_ex(setup, grid, d_list, list_size, nrays, seed, raygen,
+                                       flags, d_row_ptr, d_col, d_val, capacity, nnz, ws,
+                                       kNoAllocScratchBytes, stream);
+  if (st == RAPTR_ERR_WORKSPACE && std::strstr(raptr_last_error(), "staging")) {
+    char why[256];
+    std::snprintf(why, sizeof(why), "%s", raptr_last_error());  // fail() rewrites it
+    return fail(RAPTR_ERR_WORKSPACE,
+                "raptr_sm_build_csr_noalloc: a row has more candidates than the %zu-byte "
+                "scratch stages (%s)",
+                kNoAllocScratchBytes, why);
+  }
+  return st;
 }
 
 int raptr_sensitivity_f64(const raptr_setup_t* setup, const raptr_grid_t* grid, int nrays,
