@@ -633,10 +633,11 @@ def main():
 
 # HBM bytes per launch of the 64-frame grouped backward (rocprofv3
 # FETCH_SIZE x2.00 + WRITE_SIZE x1.00, the factors of the 2 GiB streaming
-# copy), full-size workload, locality row order, keyed by (kernel, group,
-# row blocks)
+# copy), full-size workload, slope-tiled locality row order, keyed by
+# (kernel, group,
+# row blocks, row blocks per XCD and launch); per launch of k_bwdm64x
 MEASURED_TRAFFIC_BATCHED = {
-    ("k_bwdm64x", 16, 138): 2 * 46954025 * 1024 + 281403 * 1024,
+    ("k_bwdm64x", 16, 138, 2): 2 * 2629779 * 1024 + 321299 * 1024,
 }
 
 
@@ -664,12 +665,14 @@ def batched_roofline(R, info, nf, m, n, nnz, vt, reps, timed, capi, dx, dy, st):
     kind = info.get("grouped_kind", 0)
     grouped = bool(info.get("grouped64")) and nf == 64
     key = None
-    if grouped and kind == 4:
-        kb = f"k_bwdm64b<{vt}, {g}> (per-XCD brick ranges, update fused; one launch per pass)"
-        key = ("k_bwdm64b", g, info["row_blocks"])
-    elif grouped and kind == 3:
-        kb = f"k_bwdm64x<{vt}, {g}> + k_bwd_xreduce (per-XCD row ranges)"
-        key = ("k_bwdm64x", g, info["row_blocks"])
+    nl = 1
+    if grouped and kind == 3:
+        xb = capi.lib().raptr_recon_get_xcd_blocks(R.handle)
+        per_xcd = -(-info["row_blocks"] // 8)
+        nl = -(-per_xcd // xb) if xb > 0 else 1
+        kb = (f"k_bwdm64x<{vt}, {g}> ({nl} launches per pass, {xb or per_xcd} row blocks per "
+              "XCD each, per-XCD row ranges) + k_bwd_xreduce")
+        key = ("k_bwdm64x", g, info["row_blocks"], xb)
     else:
         kb = "general F-frame backward"
     stored_b = None
@@ -685,11 +688,14 @@ def batched_roofline(R, info, nf, m, n, nnz, vt, reps, timed, capi, dx, dy, st):
             "traffic": int(traffic) if traffic else None,
             "traffic_source": ("rocprofv3 FETCH_SIZE x2.00 + WRITE_SIZE x1.00 per launch, "
                                "factors from a 2 GiB streaming copy") if traffic else None,
-            "algorithmic_bytes": int(bytes_b), "bytes_basis":
-                "4-byte value (2 for 16-bit) + 4-byte index per matrix entry, F-frame vectors",
-            "stored_bytes": int(stored_b) if stored_b else None,
+            "algorithmic_bytes": int(bytes_b / nl), "bytes_basis":
+                "per launch of the pass's nl launches: 4-byte value (2 for 16-bit) + 4-byte "
+                "index per matrix entry, F-frame vectors",
+            "stored_bytes": int(stored_b / nl) if stored_b else None,
             "stored_GBps": round(stored_b / (ms_b * 1e-3) / 1e9, 1) if stored_b else None,
-            "avg_launch_ms": round(ms_b, 4), "launches_per_pass": 1, "frames": nf,
+            "avg_launch_ms": round(ms_b / nl, 4), "launches_per_pass": nl,
+            "pass_ms": round(ms_b, 4), "algorithmic_bytes_per_pass": int(bytes_b),
+            "frames": nf,
             "forward": {"kernel": f"k_fwdm64<{vt}, {g}>" if grouped else "general F-frame forward",
                         "ms": round(ms_f, 4), "algorithmic_bytes": int(bytes_f),
                         "achieved": round(gb_f, 1), "frac": round(gb_f / HBM_PEAK_GBS, 4)}}

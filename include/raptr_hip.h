@@ -534,6 +534,13 @@ int raptr_recon_get_broadcast(const raptr_recon* plan);
  * bwd_entries = union entries of each layout (lists padded to multiples of
  * 32 / 8). */
 int raptr_recon_set_grouped(raptr_recon* plan, int on, int unroll, int kind);
+/* Kind 3's backward in several launches: each XCD walks `blocks` of its
+ * row blocks per launch (0, the default: all in one launch), adding to its
+ * partial image between launches.  The bricks resident on an XCD then stay
+ * within a few blocks of each other, so the ratio rows they gather fit its
+ * L2; each extra launch re-reads and re-writes the partial images. */
+int raptr_recon_set_xcd_blocks(raptr_recon* plan, int blocks);
+int raptr_recon_get_xcd_blocks(const raptr_recon* plan);
 int raptr_recon_get_grouped(const raptr_recon* plan, int* group, int* on, int* unroll,
                             int* kind, int64_t* fwd_entries, int64_t* bwd_entries);
 /* Entries of the padded row-blocked transpose (0 for the tiled layout). */

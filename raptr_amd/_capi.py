@@ -143,6 +143,8 @@ _SIGS = {
     "raptr_recon_transpose_nnz": (i64, [vp]),
     "raptr_recon_set_broadcast": (i32, [vp, i32]),
     "raptr_recon_graph_count": (i32, [vp]),
+    "raptr_recon_set_xcd_blocks": (i32, [vp, i32]),
+    "raptr_recon_get_xcd_blocks": (i32, [vp]),
     "raptr_recon_get_broadcast": (i32, [vp]),
     "raptr_recon_set_grouped": (i32, [vp, i32, i32, i32]),
     "raptr_recon_get_grouped": (i32, [vp, vp, vp, vp, vp, vp, vp]),

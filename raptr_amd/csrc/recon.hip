@@ -1496,6 +1496,10 @@ __global__ __launch_bounds__(256) void k_bwdm64(int ngc, const int64_t* __restri
 // slices in a fixed order and applies the update.  On a device that assigns
 // workgroups differently the result is the same (only the cache behaviour
 // changes).
+// boff / bcnt: this launch covers row blocks b0 + boff .. b0 + boff + bcnt
+// - 1 of the XCD's range (several launches keep the bricks resident on an
+// XCD within a few blocks of each other, so the ratio rows they gather fit
+// its L2); accum: add to the partial images instead of writing them.
 template <int VT, int G>
 __global__ __launch_bounds__(256) void k_bwdm64x(int ngc, int nblk, int n,
                                                  const int64_t* __restrict__ gp,
@@ -1503,7 +1507,8 @@ __global__ __launch_bounds__(256) void k_bwdm64x(int ngc, int nblk, int n,
                                                  const typename Val<VT>::T* __restrict__ gval,
                                                  const int* __restrict__ cols,
                                                  const float* __restrict__ e,
-                                                 float* __restrict__ c8) {
+                                                 float* __restrict__ c8, int boff, int bcnt,
+                                                 int accum) {
   using M = MF<G>;
   constexpr int U = 8 / M::K;  // 8-entry steps (backward lists are aligned to 8)
   const int lane = threadIdx.x & (kWave - 1);
@@ -1512,9 +1517,11 @@ __global__ __launch_bounds__(256) void k_bwdm64x(int ngc, int nblk, int n,
   const int h = int(blockIdx.x / 8) * (256 / kWave) + wave_id();
   if (h >= ngc) return;
   const int b0 = int(int64_t(nblk) * xcd / 8), b1 = int(int64_t(nblk) * (xcd + 1) / 8);
+  const int s0 = b0 + boff < b1 ? b0 + boff : b1;
+  const int s1 = s0 + bcnt < b1 ? s0 + bcnt : b1;
   typename M::acc_t acc[M::FB];
   acc_zero<G>(acc);
-  for (int b = b0; b < b1; ++b)
+  for (int b = s0; b < s1; ++b)
     mfma_sum<VT, G, U>(grow, gval, gp[int64_t(b) * ngc + h], gp[int64_t(b) * ngc + h + 1],
                        e + M::FB * rw, kq, rw, acc);
   float* out = c8 + int64_t(xcd) * n * 64;
@@ -1523,7 +1530,10 @@ __global__ __launch_bounds__(256) void k_bwdm64x(int ngc, int nblk, int n,
     const int j = cols[int64_t(h) * G + M::member(lane, r)];
     if (j < 0) continue;
 #pragma unroll
-    for (int fb = 0; fb < M::FB; ++fb) out[int64_t(j) * 64 + M::FB * rw + fb] = acc[fb][r];
+    for (int fb = 0; fb < M::FB; ++fb) {
+      const int64_t o = int64_t(j) * 64 + M::FB * rw + fb;
+      out[o] = accum ? out[o] + acc[fb][r] : acc[fb][r];
+    }
   }
 }
 
@@ -2507,16 +2517,21 @@ int launch_bwd_xcd(const raptr_recon* R, const float* x, hipStream_t st, int* nb
   if (!R->c8 || R->grp_g < 16)
     return fail(RAPTR_ERR_INVALID_ARG, "plan has no per-XCD partial images");
   const unsigned gg = unsigned(8 * ((R->ngc + 3) / 4));
+  const int per_xcd = (R->nblk + 7) / 8;  // most blocks of any XCD's range
+  const int bpl = R->xcd_bpl > 0 ? R->xcd_bpl : per_xcd;
+  for (int boff = 0; boff < per_xcd; boff += bpl) {
+    const int accum = boff > 0;
 #define RAPTR_X64(VV, G_)                                                                    \
   if (R->vt == VV && R->grp_g == G_)                                                         \
     hipLaunchKernelGGL((k_bwdm64x<VV, G_>), dim3(gg), dim3(256), 0, st, int(R->ngc),          \
                        R->nblk, R->n, (const int64_t*)R->gbp, (const int*)R->gbrow,          \
                        (const typename Val<VV>::T*)R->gbval, (const int*)R->gcols,           \
-                       (const float*)R->e, R->c8);
-  RAPTR_X64(kF32, 16) RAPTR_X64(kF16, 16) RAPTR_X64(kBF16, 16) RAPTR_X64(kU16, 16)
-  RAPTR_X64(kF32, 32) RAPTR_X64(kF16, 32) RAPTR_X64(kBF16, 32) RAPTR_X64(kU16, 32)
+                       (const float*)R->e, R->c8, boff, bpl, accum);
+    RAPTR_X64(kF32, 16) RAPTR_X64(kF16, 16) RAPTR_X64(kBF16, 16) RAPTR_X64(kU16, 16)
+    RAPTR_X64(kF32, 32) RAPTR_X64(kF16, 32) RAPTR_X64(kBF16, 32) RAPTR_X64(kU16, 32)
 #undef RAPTR_X64
-  RAPTR_LAUNCH_CHECK();
+    RAPTR_LAUNCH_CHECK();
+  }
   const unsigned nr = unsigned((int64_t(R->n) + 3) / 4);
   hipLaunchKernelGGL(k_bwd_xreduce, dim3(nr), dim3(256), 0, st, R->n, (const float*)R->c8, x,
                      (const float*)R->s, R->xx, R->partials);
@@ -2918,6 +2933,10 @@ int create_plan(raptr_recon** out, int m, int n, int64_t nnz, const int64_t* d_r
       // groups, else pipelined
       R->grp_unroll = group == 4 ? 8 : (group == 8 ? 4 : (group == 16 ? 8 : 16));
       R->grp_pipe = group >= 16 ? 3 : 1;
+      // kind 3's backward walks windows of about 65,536 rows per XCD and
+      // launch (the best measured at full size for 8,192- to 32,768-row
+      // blocks)
+      R->xcd_bpl = block_rows > 0 && block_rows < 65536 ? 65536 / block_rows : 1;
     }
     // ---- backward: row-blocked transpose via a stable radix sort, every
     // (block, column) segment padded to a multiple of 4 entries
@@ -3443,6 +3462,16 @@ int raptr_recon_set_blocks_per_launch(raptr_recon* R, int blocks) {
 }
 
 int raptr_recon_get_blocks_per_launch(const raptr_recon* R) { return R ? R->bwd_bpl : -1; }
+
+int raptr_recon_set_xcd_blocks(raptr_recon* R, int blocks) {
+  if (!R) return fail(RAPTR_ERR_INVALID_ARG, "null plan");
+  if (blocks < 0) return fail(RAPTR_ERR_INVALID_ARG, "blocks per launch must be >= 0");
+  if (R->xcd_bpl != blocks) clear_graphs(R);
+  R->xcd_bpl = blocks;
+  return RAPTR_OK;
+}
+
+int raptr_recon_get_xcd_blocks(const raptr_recon* R) { return R ? R->xcd_bpl : -1; }
 
 int64_t raptr_recon_transpose_nnz(const raptr_recon* R) { return R ? R->bnnz : -1; }
 

@@ -123,6 +123,8 @@ struct raptr_recon {
   int bwd_lpc = 16;    // lanes per column of the LDS-staged backward
   int bwd_wgs = 512;   // workgroups of the LDS-staged backward
   int bwd_bpl = 1;     // row blocks per backward launch (blocked layout)
+  int xcd_bpl = 0;     // per-XCD 64-frame backward: row blocks each XCD walks
+                       // per launch (0: all of its blocks in one launch)
   int bcast64 = 0;     // 64-frame kernels with in-row broadcast of values/indices
   // column-tiled single-frame layout (RAPTR_LAYOUT_TILED): tile q holds
   // columns [q*tile_cols, (q+1)*tile_cols) of every row, rows padded to

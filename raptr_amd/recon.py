@@ -159,6 +159,10 @@ class Reconstructor:
         capi.call("raptr_recon_get_forward_kind", self.handle, None, None, C.byref(u))
         return u.value
 
+    def set_xcd_blocks(self, blocks):
+        """Kind 3 backward: row blocks each XCD walks per launch (0: all)."""
+        capi.call("raptr_recon_set_xcd_blocks", self.handle, int(blocks))
+
     def set_blocks_per_launch(self, blocks):
         capi.call("raptr_recon_set_blocks_per_launch", self.handle, int(blocks))
 
@@ -240,22 +244,11 @@ class Reconstructor:
         bu = inf["bwd_unroll"]
         grp = nframes == 64 and inf.get("group", 0) > 0
         if self.grouped_only:
-            best_g = None
-            for gu, gp in self.GROUP_SHAPES[inf["group"]]:
-                for gb in ((1,) if gp >= 3 else (1, 4, 16)):
-                    if gb > 1 and gb >= inf["row_blocks"]:
-                        continue
-                    self.set_grouped(True, gu, gp)
-                    self.set_blocks_per_launch(gb)
-                    ms = timed()
-                    if best_g is None or ms < best_g[1]:
-                        best_g = ((gu, gp, gb), ms)
-            (g_u, g_p, bpl), ms_best = best_g
-            self.set_grouped(True, g_u, g_p)
-            self.set_blocks_per_launch(bpl)
+            (g_u, g_p, bpl, xb), ms_best = self._tune_grouped(inf, timed)
             capi.call("raptr_memcpy_d2d", x.ptr, x_save.ptr, x.nbytes, stream)
             return ({"grouped64": 1, "grouped_unroll": g_u, "grouped_kind": g_p,
-                     "bwd_blocks_per_launch": bpl, "grouped_only": 1}, ms_best)
+                     "bwd_blocks_per_launch": bpl, "xcd_blocks": xb, "grouped_only": 1},
+                    ms_best)
         if grp:  # tune the general kernels first
             self.set_grouped(False)
         if nframes == 1:
@@ -316,34 +309,58 @@ class Reconstructor:
             else:
                 self.set_broadcast(False)
         ms_best = timed()
-        g_on, g_u, g_p = 0, 0, 0
+        g_on, g_u, g_p, xb = 0, 0, 0, 0
         if grp:
-            # grouped layouts: union entries per step (plain or pipelined) x
-            # row blocks per launch
-            best_g = None
-            for gu, gp in self.GROUP_SHAPES[inf["group"]]:
-                for gb in ((1,) if gp >= 3 else (1, 4, 16)):
-                    if gb > 1 and gb >= inf["row_blocks"]:
-                        continue
-                    self.set_grouped(True, gu, gp)
-                    self.set_blocks_per_launch(gb)
-                    ms = timed()
-                    if best_g is None or ms < best_g[1]:
-                        best_g = ((gu, gp, gb), ms)
+            best_g = self._tune_grouped(inf, timed)
             if best_g[1] < ms_best:
-                (g_u, g_p, bpl), ms_best = best_g
+                (g_u, g_p, bpl, xb), ms_best = best_g
                 g_on = 1
-                self.set_grouped(True, g_u, g_p)
             else:
                 self.set_grouped(False)
+                self.set_xcd_blocks(0)
             self.set_blocks_per_launch(bpl)
         capi.call("raptr_memcpy_d2d", x.ptr, x_save.ptr, x.nbytes, stream)
         cfg = {"fwd_lanes": fl, "fwd_unroll": fu, "fwd_kind": self.forward_kind()[0],
                "bwd_unroll": u, "bwd_staged": st,
                "bwd_lanes_per_column": lpc, "bwd_workgroups": wgs, "bwd_blocks_per_launch": bpl,
                "broadcast64": bc, "grouped64": g_on, "grouped_unroll": g_u,
-               "grouped_kind": g_p}
+               "grouped_kind": g_p, "xcd_blocks": xb}
         return cfg, ms_best
+
+    # rows each XCD walks per launch of kind 3's backward: windows of about
+    # 65,536 plan rows (16 MB of 64-frame ratios) measured best at full size
+    # with 8,192-, 16,384- and 32,768-row blocks; tried with half and twice
+    # that window and in one launch
+    XCD_WINDOW_ROWS = 65536
+
+    def _xcd_blocks_options(self, inf):
+        rows = max(1, -(-self.m // max(1, inf["row_blocks"])))
+        t = max(1, self.XCD_WINDOW_ROWS // rows)
+        return sorted({0, t, max(1, t // 2), 2 * t})
+
+    def _tune_grouped(self, inf, timed):
+        """Grouped 64-frame shapes: union entries per step and kind, with
+        row blocks per launch (kinds 0-2) or per XCD and launch (kind 3);
+        leaves the fastest set.  Returns ((unroll, kind, bpl, xcd_blocks),
+        ms)."""
+        best = None
+        for gu, gp in self.GROUP_SHAPES[inf["group"]]:
+            opts = ([(1, xb) for xb in self._xcd_blocks_options(inf)] if gp >= 3 else
+                    [(gb, 0) for gb in (1, 4, 16)])
+            for gb, xb in opts:
+                if gb > 1 and gb >= inf["row_blocks"]:
+                    continue
+                self.set_grouped(True, gu, gp)
+                self.set_blocks_per_launch(gb)
+                self.set_xcd_blocks(xb)
+                ms = timed()
+                if best is None or ms < best[1]:
+                    best = ((gu, gp, gb, xb), ms)
+        (gu, gp, gb, xb), _ = best
+        self.set_grouped(True, gu, gp)
+        self.set_blocks_per_launch(gb)
+        self.set_xcd_blocks(xb)
+        return best
 
     def close(self):
         if getattr(self, "handle", None):

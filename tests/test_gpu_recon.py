@@ -432,10 +432,14 @@ def test_recon_grouped64(gpu, oracle, small_system, vt, tol, group, bricks):
             for bpl in (1, 3):
                 R.set_grouped(True, gu, gp)
                 R.set_blocks_per_launch(bpl)
+                # kind 3: also its backward split over launches of 1 and 2
+                # row blocks per XCD (partial images added between launches)
+                R.set_xcd_blocks(bpl - 1 if gp == 3 else 0)
                 X = R.mlem(Y, X0, 5)
                 rel = _rel(X, ref)
                 print(f"{vt} G={group} bricks={bricks} order={order is not None} unroll {gu} "
-                      f"kind {gp} bpl {bpl}: rel {rel:.2e} (union fwd {gi['fwd_entries']} "
+                      f"kind {gp} bpl {bpl} xcd-blocks {bpl - 1 if gp == 3 else 0}: "
+                      f"rel {rel:.2e} (union fwd {gi['fwd_entries']} "
                       f"bwd {gi['bwd_entries']} of {S['A'].nnz})")
                 assert rel <= tol
         if vt == "f32":
