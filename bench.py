@@ -44,78 +44,16 @@ CPU_ITERS = 5  # iterations per timed run of the CPU baseline
 
 # HBM bytes per launch measured with rocprofv3 (FETCH_SIZE scaled x2.00,
 # WRITE_SIZE x1.00: the factors a 2 GiB streaming copy of this library
-# reads back) for the kernel shapes the autotuner picks on this workload;
-# other shapes report no traffic figure.
+# reads back) by `python -m raptr_amd.passes` on this workload, for the
+# kernel shapes the autotuner picks; other shapes report no traffic figure.
+# Row order: y-major locality order for one frame.
 MEASURED_TRAFFIC = {
-    # forward (lanes, unroll, row order); one launch per pass; locality order
-    # with 8 x 8 midpoint tiles
-    ("k_fwd1", 32, 2, "list"): 2 * 7342510 * 1024 + 17622 * 1024,
-    ("k_fwd1", 16, 1, "locality"): 2 * 8723487 * 1024 + 17632 * 1024,
-    ("k_fwd1", 16, 2, "locality"): 2 * 8101933 * 1024 + 17633 * 1024,
-    ("k_fwd1", 16, 4, "locality"): 2 * 7922277 * 1024 + 17629 * 1024,
-    ("k_fwd1", 32, 1, "locality"): 2 * 7624225 * 1024 + 17622 * 1024,
-    ("k_fwd1", 32, 2, "locality"): 2 * 7375797 * 1024 + 17622 * 1024,
-    ("k_fwd1", 32, 4, "locality"): 2 * 8005890 * 1024 + 17622 * 1024,
-    ("k_fwd1", 64, 1, "locality"): 2 * 7136907 * 1024 + 35241 * 1024,
-    ("k_fwd1", 64, 2, "locality"): 2 * 7236914 * 1024 + 35241 * 1024,
-    ("k_fwd1", 64, 4, "locality"): 2 * 7134732 * 1024 + 35241 * 1024,
-    # 16-bit columns (plain k_fwd1c, software-pipelined k_fwd1cp), 69 row
-    # blocks of 65,536 rows
-    ("k_fwd1c", 16, 1, "locality"): 2 * 7147423 * 1024 + 17626 * 1024,
-    ("k_fwd1c", 16, 2, "locality"): 2 * 6470240 * 1024 + 17623 * 1024,
-    ("k_fwd1c", 16, 4, "locality"): 2 * 6375603 * 1024 + 17624 * 1024,
-    ("k_fwd1c", 32, 1, "locality"): 2 * 6081126 * 1024 + 17621 * 1024,
-    ("k_fwd1c", 32, 2, "locality"): 2 * 5853171 * 1024 + 17621 * 1024,
-    ("k_fwd1c", 32, 4, "locality"): 2 * 6434950 * 1024 + 17621 * 1024,
-    ("k_fwd1c", 64, 1, "locality"): 2 * 5598289 * 1024 + 35241 * 1024,
-    ("k_fwd1c", 64, 2, "locality"): 2 * 5762812 * 1024 + 35241 * 1024,
-    ("k_fwd1c", 64, 4, "locality"): 2 * 5601200 * 1024 + 35241 * 1024,
-    ("k_fwd1cp", 16, 1, "locality"): 2 * 6969534 * 1024 + 17623 * 1024,
-    ("k_fwd1cp", 16, 2, "locality"): 2 * 6170956 * 1024 + 17622 * 1024,
-    ("k_fwd1cp", 16, 4, "locality"): 2 * 5740675 * 1024 + 17621 * 1024,
-    ("k_fwd1cp", 32, 1, "locality"): 2 * 5969195 * 1024 + 17621 * 1024,
-    ("k_fwd1cp", 32, 2, "locality"): 2 * 5565429 * 1024 + 17621 * 1024,
-    ("k_fwd1cp", 32, 4, "locality"): 2 * 5035376 * 1024 + 17620 * 1024,
-    ("k_fwd1cp", 64, 1, "locality"): 2 * 5518387 * 1024 + 35241 * 1024,
-    ("k_fwd1cp", 64, 2, "locality"): 2 * 5156511 * 1024 + 35241 * 1024,
-    ("k_fwd1cp", 64, 4, "locality"): 2 * 5163556 * 1024 + 35241 * 1024,
-    # x staged in LDS per 64 plan rows (k_fwd1l, cached stream loads)
-    ("k_fwd1l", 16, 1, "locality"): 2 * 5381663 * 1024 + 21150 * 1024,
-    ("k_fwd1l", 16, 2, "locality"): 2 * 5212891 * 1024 + 23302 * 1024,
-    ("k_fwd1l", 16, 4, "locality"): 2 * 5194149 * 1024 + 26996 * 1024,
-    ("k_fwd1l", 32, 1, "locality"): 2 * 5134107 * 1024 + 34597 * 1024,
-    ("k_fwd1l", 32, 2, "locality"): 2 * 5132328 * 1024 + 33701 * 1024,
-    ("k_fwd1l", 32, 4, "locality"): 2 * 5154610 * 1024 + 35567 * 1024,
-    ("k_fwd1l", 64, 1, "locality"): 2 * 5120169 * 1024 + 28999 * 1024,
-    ("k_fwd1l", 64, 2, "locality"): 2 * 5122436 * 1024 + 27835 * 1024,
-    ("k_fwd1l", 64, 4, "locality"): 2 * 5118499 * 1024 + 29949 * 1024,
-    # backward (unroll, row blocks, blocks per launch, staged, row order);
-    # average over the launches of one pass
-    ("k_bwd", 4, 9, 1, 0, "list"): 2 * 734223 * 1024 + 1427 * 1024,
-    ("k_bwd", 4, 9, 4, 0, "locality"): 2 * 2562847 * 1024 + 1613 * 1024,
-    # single-frame backward with per-XCD row ranges (unroll, row blocks, row
-    # order, 32/16-bit rows, lanes per column); one pass (k_bwd1x; its reduce
-    # moves 0.04 GB more)
-    ("k_bwd1x", 4, 9, "locality", "r32", 64): 2 * 6605601 * 1024 + 4775 * 1024,
-    # 69 row blocks of 65,536 rows
-    ("k_bwd1x", 1, 69, "locality", "r16", 16): 2 * 7317888 * 1024 + 9221 * 1024,
-    ("k_bwd1x", 1, 69, "locality", "r16", 32): 2 * 6206522 * 1024 + 12241 * 1024,
-    ("k_bwd1x", 1, 69, "locality", "r16", 64): 2 * 5560842 * 1024 + 6292 * 1024,
-    ("k_bwd1x", 1, 69, "locality", "r32", 16): 2 * 8709134 * 1024 + 10990 * 1024,
-    ("k_bwd1x", 1, 69, "locality", "r32", 32): 2 * 7534303 * 1024 + 13905 * 1024,
-    ("k_bwd1x", 1, 69, "locality", "r32", 64): 2 * 6944555 * 1024 + 12157 * 1024,
-    ("k_bwd1x", 2, 69, "locality", "r16", 16): 2 * 6657187 * 1024 + 9759 * 1024,
-    ("k_bwd1x", 2, 69, "locality", "r16", 32): 2 * 5894705 * 1024 + 12803 * 1024,
-    ("k_bwd1x", 2, 69, "locality", "r16", 64): 2 * 5542259 * 1024 + 6110 * 1024,
-    ("k_bwd1x", 2, 69, "locality", "r32", 16): 2 * 7591338 * 1024 + 12140 * 1024,
-    ("k_bwd1x", 2, 69, "locality", "r32", 32): 2 * 7006851 * 1024 + 13763 * 1024,
-    ("k_bwd1x", 2, 69, "locality", "r32", 64): 2 * 6702576 * 1024 + 11656 * 1024,
-    ("k_bwd1x", 4, 69, "locality", "r16", 16): 2 * 6399207 * 1024 + 10333 * 1024,
-    ("k_bwd1x", 4, 69, "locality", "r16", 32): 2 * 5997025 * 1024 + 12419 * 1024,
-    ("k_bwd1x", 4, 69, "locality", "r16", 64): 2 * 5549076 * 1024 + 6098 * 1024,
-    ("k_bwd1x", 4, 69, "locality", "r32", 16): 2 * 7214243 * 1024 + 10933 * 1024,
-    ("k_bwd1x", 4, 69, "locality", "r32", 32): 2 * 6826134 * 1024 + 11861 * 1024,
-    ("k_bwd1x", 4, 69, "locality", "r32", 64): 2 * 6625995 * 1024 + 8958 * 1024,
+    # forward (kernel, lanes, unroll, row order); one launch per pass
+    ("k_fwd1l", 16, 4, "y"): 2 * 5156007 * 1024 + 27546 * 1024,
+    # backward with per-XCD row ranges (unroll, row blocks, row order,
+    # 16-bit rows, lanes per column, row blocks per XCD and launch); bytes
+    # per launch (its reduce, k_bwd1_xreduce, moves 0.006 GB more per pass)
+    ("k_bwd1x", 2, 368, "y", "r16", 32, 0): 2 * 5855002 * 1024 + 8618 * 1024,
 }
 
 
@@ -150,7 +88,7 @@ def parse():
                          "list's, or try both")
     ap.add_argument("--locality-tile", type=int, default=8,
                     help="midpoint tile of the locality row order")
-    ap.add_argument("--group", type=int, default=16, choices=(4, 8, 16, 32),
+    ap.add_argument("--group", type=int, default=16, choices=(16, 32),
                     help="rows / voxels per group of the 64-frame grouped layouts")
     ap.add_argument("--block-slices-mb", default="2,8,32",
                     help="ratio-slice sizes (MB) of the batched plans' row blocks to try")
@@ -233,11 +171,12 @@ def main():
     plans = {}
     wanted = args.modes.split(",")
     orders = ["locality", "list"] if args.row_order == "both" else [args.row_order]
-    # locality orders: y-major for the single-frame plans, slope-tiled for
-    # the batched ones (recon.default_order_kind)
+    # locality orders: y-major for the single-frame plans, slope tiles over
+    # angle groups for the batched ones (recon.default_order_kind)
     from raptr_amd.recon import default_order_kind
     d_orders = ({k: sysmat.locality_order(setup, d_list, stream=st, tile=args.locality_tile,
-                                          kind=k) for k in ("y", "slope")}
+                                          kind=k)
+                 for k in {default_order_kind(1), default_order_kind(F)}}
                 if "locality" in orders else {})
     slices_mb = [float(v) for v in args.block_slices_mb.split(",")]
     best_rows = best_order = None
@@ -253,12 +192,12 @@ def main():
         dx = DeviceArray.from_numpy(xx0.reshape(-1))
         # row blocks of the backward: for F frames try several ratio-slice
         # sizes (the first batched mode picks, the others reuse its choice)
-        # one frame: both plan layouts (column tiles, row blocks) are tried
+        # one frame: a few row-block sizes are tried
         if nf == 1 and name != "plain":
             rows_try = [("blocked", 0)]
         elif nf == 1:
             # (block_rows 0: the plan's default, 65,536 rows for one frame)
-            rows_try = [("tiled", 0), ("blocked", 0), ("blocked", 524288), ("blocked", 12288)]
+            rows_try = [("blocked", 0), ("blocked", 12288), ("blocked", 8192)]
         elif best_rows is not None:
             rows_try = [("blocked", best_rows)]
         else:
@@ -284,7 +223,8 @@ def main():
                     R.close()
                 R, t_plan, cfg, ms_it = Rc, tp, cc, mc
                 chosen_rows, chosen_order = rows, order
-                cfg = dict(cfg, layout=layout, row_order=order)
+                cfg = dict(cfg, layout=layout,
+                           row_order=default_order_kind(nf) if order == "locality" else order)
             else:
                 Rc.close()
         if nf > 1:
@@ -380,7 +320,6 @@ def main():
     cmp_ = max(info["compact_rows"], 0)
     cols16, rows16 = bool(cmp_ & 2), bool(cmp_ & 1)
     fwd_kernel = ("k_fwd1l" if info["fwd_kind"] == 2 else
-                  "k_fwd1cp" if info["fwd_kind"] == 1 and cols16 else
                   "k_fwd1c" if cols16 else "k_fwd1")
     fwd_name = f"{fwd_kernel} (forward + ratio, one launch" + (
         ", x staged in LDS per 64 rows, 16-bit ranks)" if fwd_kernel == "k_fwd1l" else
@@ -389,32 +328,17 @@ def main():
     # 4-byte index of the algorithmic count (plus 8 bytes of split points per
     # row for the columns)
     stored_f = stored_b = None
-    if info["layout"] == "tiled":
-        # bytes each pass must move: padded matrix (4-byte value + 2-byte
-        # column offset), segment starts, and the vectors (y, the per-tile
-        # partial row sums written and re-read by the ratio kernel, e; the
-        # ratio read by every tile, the partial column sums, x, s, x')
-        T, TC = info["tiles"], info["tile_cols"]
-        W = -(-256 // T)  # workgroups per tile on a 256-CU device
-        tn = info["padded_nnz"]
-        bytes_f = 6 * tn + 8 * (T * m + 1) + 4 * m + 4 * T * m + (4 * T * m + 8 * m) + 4 * n
-        bytes_b = 6 * tn + 8 * (T * m + 1) + 4 * T * m + 2 * 4 * T * W * TC + 12 * n
-        fwd_name = f"k_tfwd + k_tratio (forward + ratio over {T} column tiles)"
-        bwd_name = f"k_tbwd + k_tupdate (backward + update over {T} column tiles)"
-    else:
-        # algorithmic bytes: matrix (value + column index), row pointers, vectors
-        bytes_f = 8 * info["padded_nnz"] + 8 * (m + 1) + 4 * m + 4 * m + 4 * n
-        if fwd_kernel == "k_fwd1l":
-            # 2-byte ranks; per union entry its column and x (4 + 4 bytes);
-            # union offsets per 64 rows; no x term (read through the unions)
-            nu = R.forward_union_entries()
-            stored_f = (bytes_f - 2 * info["padded_nnz"] - 4 * n + 8 * nu
-                        + 8 * (-(-m // 64) + 1))
-        elif cols16:
-            stored_f = bytes_f - 2 * info["padded_nnz"] + 8 * m
-    if info["layout"] == "tiled":
-        pass  # set above
-    elif info["bwd_staged"] == 3:
+    # algorithmic bytes: matrix (value + column index), row pointers, vectors
+    bytes_f = 8 * info["padded_nnz"] + 8 * (m + 1) + 4 * m + 4 * m + 4 * n
+    if fwd_kernel == "k_fwd1l":
+        # 2-byte ranks; per union entry its column and x (4 + 4 bytes);
+        # union offsets per 64 rows; no x term (read through the unions)
+        nu = R.forward_union_entries()
+        stored_f = (bytes_f - 2 * info["padded_nnz"] - 4 * n + 8 * nu
+                    + 8 * (-(-m // 64) + 1))
+    elif cols16:
+        stored_f = bytes_f - 2 * info["padded_nnz"] + 8 * m
+    if info["bwd_staged"] == 3:
         # one pass: matrix (value + row), two segment bounds per column and
         # block, the ratio vector (counted once; each XCD reads its eighth),
         # the 8 per-XCD partial sums written and read, x/s read, x' written
@@ -424,19 +348,6 @@ def main():
             stored_b = bytes_b - 2 * info["transpose_nnz"]
         bwd_name = (f"k_bwd1x + k_bwd1_xreduce (backward + update, one pass over {nb} row "
                     "blocks, per-XCD row ranges" + (", 16-bit rows)" if rows16 else ")"))
-    elif info["bwd_staged"] == 2:
-        # one launch: matrix (value + packed index), two segment bounds per
-        # wave and block, the ratio vector (counted once; each workgroup
-        # reads it from L2 into LDS), x/s read and x' written once
-        G = -(-n // 256)
-        bytes_b = (8 * info["transpose_nnz"] + 16 * nb * (4 * G) + 4 * m + 12 * n + 8 * G)
-        bwd_name = f"k_bwd_flat (backward + update, one launch over {nb} LDS row slices)"
-    elif info["bwd_staged"]:
-        # one launch; the ratio vector read once per workgroup from L2 (counted
-        # once), column sums in LDS, x/s read and x' written once
-        bytes_b = (8 * info["transpose_nnz"] + 8 * nb * (n + 1) + 4 * m + 12 * n
-                   + 8 * info["bwd_workgroups"])
-        bwd_name = f"k_bwd_lds (backward + update, one launch over {nb} LDS row slices)"
     else:
         bytes_b = (8 * info["transpose_nnz"] + 8 * nb * (n + 1) + 4 * m
                    + 4 * n * (2 * nb + 2) + 8 * (n // 4))
@@ -448,19 +359,18 @@ def main():
     # launches per pass: the blocked backward is one launch per group of
     # row blocks; the roofline is quoted per launch (bytes and time averaged)
     cfg_p = modes["plain"]["config"]
-    if info["layout"] == "tiled":
-        nl_f = nl_b = 1
-        key_f = key_b = None
-    else:
-        nl_f = 1
-        nl_b = 1 if info["bwd_staged"] else -(-nb // cfg_p["bwd_blocks_per_launch"])
-        key_f = (fwd_kernel, cfg_p["fwd_lanes"], cfg_p["fwd_unroll"], cfg_p["row_order"])
-        key_b = ("k_bwd", cfg_p["bwd_unroll"], nb, cfg_p["bwd_blocks_per_launch"],
-                 cfg_p["bwd_staged"], cfg_p["row_order"])
-        if info["bwd_staged"] == 3:
-            lpc = cfg_p["bwd_lanes_per_column"]
-            key_b = ("k_bwd1x", cfg_p["bwd_unroll"], nb, cfg_p["row_order"],
-                     "r16" if rows16 else "r32", lpc if lpc in (16, 32) else 64)
+    nl_f = 1
+    nl_b = 1 if info["bwd_staged"] else -(-nb // cfg_p["bwd_blocks_per_launch"])
+    key_f = (fwd_kernel, cfg_p["fwd_lanes"], cfg_p["fwd_unroll"], cfg_p["row_order"])
+    key_b = ("k_bwd", cfg_p["bwd_unroll"], nb, cfg_p["bwd_blocks_per_launch"],
+             cfg_p["bwd_staged"], cfg_p["row_order"])
+    if info["bwd_staged"] == 3:
+        lpc = cfg_p["bwd_lanes_per_column"]
+        xb = capi.lib().raptr_recon_get_xcd_blocks(R.handle)
+        # launches per pass: windows of xb row blocks per XCD (0: one)
+        nl_b = -(-(-(-nb // 8)) // xb) if xb > 0 else 1
+        key_b = ("k_bwd1x", cfg_p["bwd_unroll"], nb, cfg_p["row_order"],
+                 "r16" if rows16 else "r32", lpc if lpc in (16, 32) else 64, xb)
     dom = {"kernel": fwd_name, "ms": ms_f, "bytes": bytes_f, "gbs": gbs_f, "launches": nl_f,
            "key": key_f, "stored": stored_f}
     if ms_b > ms_f:
@@ -637,7 +547,11 @@ def main():
 # (kernel, group,
 # row blocks, row blocks per XCD and launch); per launch of k_bwdm64x
 MEASURED_TRAFFIC_BATCHED = {
-    ("k_bwdm64x", 16, 138, 2): 2 * 2629779 * 1024 + 321299 * 1024,
+    # (kernel, group, row blocks, row blocks per XCD and launch): bytes per
+    # launch of the 64-frame backward (9 launches per pass here)
+    ("k_bwdm64x", 16, 138, 2): 2 * 2293737 * 1024 + 289077 * 1024,
+    # (kernel, group, unroll): the forward, one launch per pass
+    ("k_fwdm64", 16, 8): 2 * 14024181 * 1024 + 1127705 * 1024,
 }
 
 
@@ -681,6 +595,8 @@ def batched_roofline(R, info, nf, m, n, nnz, vt, reps, timed, capi, dx, dy, st):
         stored_b = (info["grouped_bwd_entries"] * (4 + g * bv) + 8 * info["row_blocks"] *
                     (-(-n // g) + 1) + nf * 4 * m + nf * 8 * n + 4 * n)
     traffic = MEASURED_TRAFFIC_BATCHED.get(key) if key else None
+    traffic_f = (MEASURED_TRAFFIC_BATCHED.get(("k_fwdm64", g, info.get("grouped_unroll")))
+                 if grouped else None)
     gb_b = bytes_b / (ms_b * 1e-3) / 1e9
     gb_f = bytes_f / (ms_f * 1e-3) / 1e9
     return {"bound": "hbm", "kernel": kb, "achieved": round(gb_b, 1), "peak": HBM_PEAK_GBS,
@@ -698,7 +614,8 @@ def batched_roofline(R, info, nf, m, n, nnz, vt, reps, timed, capi, dx, dy, st):
             "frames": nf,
             "forward": {"kernel": f"k_fwdm64<{vt}, {g}>" if grouped else "general F-frame forward",
                         "ms": round(ms_f, 4), "algorithmic_bytes": int(bytes_f),
-                        "achieved": round(gb_f, 1), "frac": round(gb_f / HBM_PEAK_GBS, 4)}}
+                        "achieved": round(gb_f, 1), "frac": round(gb_f / HBM_PEAK_GBS, 4),
+                        "traffic": int(traffic_f) if traffic_f else None}}
 
 
 def copy_peak(timed, st, nbytes=2 << 30, reps=5):

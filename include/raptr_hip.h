@@ -365,16 +365,9 @@ int raptr_recon_create(raptr_recon** plan, int m, int n, int64_t nnz,
                        const int64_t* d_row_ptr, const int* d_col,
                        const float* d_val, const double* d_sens, int value_type,
                        int max_frames, int block_rows, raptr_stream_t stream);
-/* Layout choice.  BLOCKED: the two layouts above (any frame count).
- * TILED (max_frames = 1): the columns are cut into tiles of at most 36,864
- * voxels; each row's entries are stored per tile (rows padded to multiples
- * of 4, 16-bit column offsets within the tile).  A workgroup holds one
- * tile of x (forward) or of the column sums (backward) in LDS and streams
- * its rows' segments, so neither pass gathers from L2 and no transposed
- * copy is kept.  The backward accumulates with LDS atomics: its FP32 sums
- * are not bitwise reproducible from run to run.  For TILED, block_rows > 0
- * caps the tile width (columns) below the LDS limit. */
-enum { RAPTR_LAYOUT_BLOCKED = 0, RAPTR_LAYOUT_TILED = 1 };
+/* Layout choice: RAPTR_LAYOUT_BLOCKED, the two layouts above (any frame
+ * count), is the only layout; other values are rejected. */
+enum { RAPTR_LAYOUT_BLOCKED = 0 };
 int raptr_recon_create_ex(raptr_recon** plan, int m, int n, int64_t nnz,
                           const int64_t* d_row_ptr, const int* d_col,
                           const float* d_val, const double* d_sens, int value_type,
@@ -389,7 +382,7 @@ int raptr_recon_create_ex(raptr_recon** plan, int m, int n, int64_t nnz,
  * gathers of both passes touch fewer cache lines.  grid: the voxel grid of
  * the columns (n = nx*ny*nz), so that the backward groups of the 64-frame
  * grouped layouts are bricks of voxels (null: runs of consecutive column
- * ids).  group: members per group of those layouts (4, 8, 16, 32;
+ * ids).  group: members per group of those layouts (16 or 32;
  * 0 = 16).
  * grouped_only (max_frames = 64): build only the grouped layouts (and the
  * row scales), not the padded CSR and padded transpose; such a plan runs
@@ -435,17 +428,21 @@ int raptr_locality_order(const raptr_setup_t* setup, const int* d_channels, int 
  * ratios share that band (batched backward).  RAPTR_ORDER_SLOPE_TILES: the
  * angle-major order with the slopes (id1z - id0z, id1y - id0y) in 4 x 4
  * tiles, so 16 consecutive rows through one midpoint have nearly the same
- * direction and cover nearly the same voxels. */
-enum { RAPTR_ORDER_ANGLE_MAJOR = 0, RAPTR_ORDER_Y_MAJOR = 1, RAPTR_ORDER_SLOPE_TILES = 2 };
+ * direction and cover nearly the same voxels.  RAPTR_ORDER_ANGLE_SLOPE_TILES:
+ * the slope-tiled order over angles in groups of 4 (ida / 4 first, ida % 4
+ * last): one channel at neighbouring angles is nearly the same line inside
+ * the grid, so a run of 16 rows is 4 slopes x 4 angles of one midpoint (on
+ * the full-size geometry 7.5 instead of 5.9 of 16 rows per column of a
+ * 16-row union). */
+enum {
+  RAPTR_ORDER_ANGLE_MAJOR = 0,
+  RAPTR_ORDER_Y_MAJOR = 1,
+  RAPTR_ORDER_SLOPE_TILES = 2,
+  RAPTR_ORDER_ANGLE_SLOPE_TILES = 3
+};
 int raptr_locality_order_ex(const raptr_setup_t* setup, const int* d_channels, int m, int tile,
                             int kind, int* d_order, raptr_stream_t stream);
 int raptr_recon_layout(const raptr_recon* plan);
-/* Tiled layout kernel shape: lanes per row segment (4, 8, 16) and unroll
- * (1, 2, 4) of the forward and the backward. */
-int raptr_recon_set_tiled(raptr_recon* plan, int fwd_lanes, int fwd_unroll, int bwd_lanes,
-                          int bwd_unroll);
-int raptr_recon_get_tiled(const raptr_recon* plan, int* ntile, int* tile_cols,
-                          int* fwd_lanes, int* fwd_unroll, int* bwd_lanes, int* bwd_unroll);
 int raptr_recon_destroy(raptr_recon* plan);
 int raptr_recon_info(const raptr_recon* plan, int64_t* device_bytes,
                      int64_t* padded_nnz, int* row_blocks, int* fwd_lanes,
@@ -456,22 +453,14 @@ int raptr_recon_info(const raptr_recon* plan, int64_t* device_bytes,
  * are padded to multiples of 4 entries (value 0) for 16-byte loads. */
 int raptr_recon_set_config(raptr_recon* plan, int fwd_lanes, int fwd_unroll,
                            int bwd_unroll);
-/* Single-frame backward pass staged through LDS (needs block_rows <= 24576
- * and a multiple of 4).  staged = 1: each of `workgroups` workgroups owns a
- * contiguous column range, loads each row block's ratio slice into LDS and
- * walks its columns with lanes_per_column (8, 16, 32) lanes per column.
- * staged = 2 (flat): workgroups own 256-column
- * tiles and stream each block's entries of a wave's 64 columns as one range
- * (packed column-in-tile/row-in-block indices), adding runs of equal column
- * with a wave segmented scan; lanes_per_column and workgroups are unused.
- * Column sums stay in LDS across blocks in a fixed order.  staged = 3 (any
- * block size): each XCD takes one eighth of the row blocks (workgroup i
- * runs on XCD i % 8), so its ratio slices stay in its own L2; per-XCD
- * partial column sums (8 x n floats) are added in a fixed order;
- * lanes_per_column 16, 32 or 64 (8 runs as 64); the default of
- * single-frame row-blocked plans with >= 8 row blocks (32 lanes, unroll
- * 2; with fewer blocks some XCDs would idle).
- * staged = 0: the block-by-block pass. */
+/* Single-frame backward pass.  staged = 3 (per-XCD row ranges): each XCD
+ * takes one eighth of the row blocks (workgroup i runs on XCD i % 8), so
+ * its ratio slices stay in its own L2; per-XCD partial column sums (8 x n
+ * floats) are added in a fixed order; lanes_per_column 16, 32 or 64 (8
+ * runs as 64); the default of single-frame row-blocked plans with >= 8 row
+ * blocks (32 lanes, unroll 2; with fewer blocks some XCDs would idle).
+ * staged = 0: the block-by-block pass.  workgroups is kept for the
+ * signature and must be positive; no kernel reads it. */
 int raptr_recon_set_backward(raptr_recon* plan, int staged, int lanes_per_column,
                              int workgroups);
 int raptr_recon_get_backward(const raptr_recon* plan, int* staged, int* lanes_per_column,
@@ -485,12 +474,10 @@ int raptr_recon_get_backward(const raptr_recon* plan, int* staged, int* lanes_pe
  * 1 = columns on; -1: none present. */
 int raptr_recon_set_compact_rows(raptr_recon* plan, int on);
 int raptr_recon_get_compact_rows(const raptr_recon* plan);
-/* Kernel of the single-frame forward pass (all three give the same sums,
- * bit for bit, for the same lanes/unroll):
+/* Kernel of the single-frame forward pass (both give the same sums, bit
+ * for bit, for the same lanes/unroll):
  *   0  one matrix entry per lane and gather (32-bit or, when on, 16-bit
  *      columns);
- *   1  as 0 over 16-bit columns, software-pipelined (each lane loads its
- *      next step while the gathers of the current one are in flight);
  *   2  x staged in LDS: each workgroup copies x over the sorted column
  *      union of its 64 consecutive plan rows into LDS and gathers from
  *      there through 16-bit ranks.  Built for single-frame blocked plans
@@ -518,32 +505,37 @@ int raptr_recon_get_broadcast(const raptr_recon* plan);
  * one ascending union list with G values per union entry (0 where a member
  * has no entry).  One gathered 256-byte frame vector then serves G rows or
  * G voxels.  on: use them for 64-frame passes (the default when present);
- * unroll: union entries per step of a wave; kind: 0 = vector FMAs with the
- * keys and values in scalar registers, 1 = the same with the next step's
- * loads issued behind the current step's gathers (two register sets), 2 =
- * matrix cores (G = 16 only: v_mfma_f32_16x16x4f32 over 4 union entries x
- * 16 frames, keys and values from coalesced vector loads).  Shapes per G:
- * 4: unroll 8 kind 1 or 16 kind 0; 8: 4 kind 1 or 8 kind 0; 16: 2 kind 1,
- * 4 kind 0, 8, 16 or 32 kind 2 or 3 (the default); 32: 8 or 16 kind 3
+ * unroll: union entries per step of a wave; kind 2 = matrix cores
+ * (G = 16 only: v_mfma_f32_16x16x4f32 over 4 union entries x 16 frames,
+ * keys and values from coalesced vector loads; the backward adds each
+ * launch's row blocks into one shared image sum), kind 3 (the default) =
+ * kind 2 with a backward in which each XCD takes one eighth of the row
+ * blocks and writes its own partial image (8 x n x 64 floats, allocated
+ * with the plan); a fixed-order reduce adds them.  Shapes per G: 16: unroll
+ * 8, 16 or 32, kind 2 or 3; 32: unroll 8 or 16, kind 3
  * (v_mfma_f32_32x32x2f32, 2 entries x 32 frames, whole 128-byte lines per
- * gathered frame block; bricks of 2x4x4 voxels).  Kind 3 = kind 2 with a
- * backward in which each XCD takes one eighth of the row blocks and writes
- * its own partial image (8 x n x 64 floats, allocated with the plan); a
- * fixed-order reduce adds them.  unroll applies to the forward (backward
+ * gathered frame block).  Voxel bricks: 4x1x4 (G = 16), 4x2x4 (G = 32).
+ * Kinds 0 and 1 (vector FMAs with the keys and values in scalar
+ * registers, groups of 4 and 8) were removed: the scalar-cache stream
+ * capped them near 0.8 TB/s.  unroll applies to the forward (backward
  * steps are 8 entries).  get: group = G (0: no grouped layouts), fwd/
  * bwd_entries = union entries of each layout (lists padded to multiples of
  * 32 / 8). */
 int raptr_recon_set_grouped(raptr_recon* plan, int on, int unroll, int kind);
-/* Kind 3's backward in several launches: each XCD walks `blocks` of its
- * row blocks per launch (0, the default: all in one launch), adding to its
- * partial image between launches.  The bricks resident on an XCD then stay
+/* The per-XCD backwards (64-frame kind 3; single-frame mode 3) in several
+ * launches: each XCD walks `blocks` of its row blocks per launch (0: all in
+ * one launch, the single-frame default), adding to its partial image
+ * between launches.  The bricks or columns resident on an XCD then stay
  * within a few blocks of each other, so the ratio rows they gather fit its
- * L2; each extra launch re-reads and re-writes the partial images. */
+ * L2; each extra launch re-reads and re-writes the partial images (8 x n x
+ * F floats).  Same sums whatever the window for 64 frames; for one frame
+ * each window's lane sums are added to the partial sum in window order, a
+ * fixed order per window size. */
 int raptr_recon_set_xcd_blocks(raptr_recon* plan, int blocks);
 int raptr_recon_get_xcd_blocks(const raptr_recon* plan);
 int raptr_recon_get_grouped(const raptr_recon* plan, int* group, int* on, int* unroll,
                             int* kind, int64_t* fwd_entries, int64_t* bwd_entries);
-/* Entries of the padded row-blocked transpose (0 for the tiled layout). */
+/* Entries of the padded row-blocked transpose. */
 int64_t raptr_recon_transpose_nnz(const raptr_recon* plan);
 /* niter MLEM iterations (reco.cu:230-303) of nframes frames at once
  * (power of two <= max_frames).  d_y: counts [m][nframes]; d_x: images
@@ -554,8 +546,19 @@ int64_t raptr_recon_transpose_nnz(const raptr_recon* plan);
  * per frame count and buffers, raptr_recon_graph_count) and replayed. */
 int raptr_recon_mlem(raptr_recon* plan, int nframes, const float* d_y, float* d_x,
                      int niter, int use_graph, raptr_stream_t stream);
+/* raptr_recon_mlem over the first `active` frames (16 or 32) of a 64-frame
+ * pass: d_y and d_x keep their [rows][64] layout, frames active .. 63 of
+ * d_x are left as they are, and frames 0 .. active - 1 get exactly the
+ * images raptr_recon_mlem gives them (the same arithmetic per frame).
+ * The matrix-core kernels then gather and multiply active/64 of the frame
+ * vectors.  Needs the grouped layouts of 16 on, kind 3 (the default of a
+ * 64-frame plan with a grid).  active = nframes runs raptr_recon_mlem.
+ * Graphs are cached per (frames, active, buffers).  No counterpart in the
+ * reference; for serving lanes whose busy slots fit the low frames. */
+int raptr_recon_mlem_active(raptr_recon* plan, int nframes, int active, const float* d_y,
+                            float* d_x, int niter, int use_graph, raptr_stream_t stream);
 /* Captured iterations the plan holds (raptr_recon_mlem with use_graph):
- * one per distinct (nframes, d_y, d_x), at most 16, the least recently
+ * one per distinct (nframes, active frames, d_y, d_x), at most 16, the least recently
  * used destroyed first; all are destroyed when a kernel shape changes.
  * -1 for a null plan. */
 int raptr_recon_graph_count(const raptr_recon* plan);
@@ -572,13 +575,17 @@ int raptr_recon_backward(raptr_recon* plan, int nframes, const float* d_x,
  * occupies one frame slot of a batched plan's y ([m][F]) and x ([n][F]).
  * fill: column `slot` = value.  scatter: column `slot`[d_idx[i]] = d_val[i]
  * (rows outside [0, rows) are skipped and set *d_bad = 1).  gather:
- * d_out[i] = column `slot`[i]. */
+ * d_out[i] = column `slot`[i].  copy: column dst_slot = column src_slot
+ * (a request moved to a lower slot, so a lane's busy slots fit the first
+ * 16 or 32 frames of raptr_recon_mlem_active). */
 int raptr_frame_fill(float* d_a, int64_t rows, int nframes, int slot, float value,
                      raptr_stream_t stream);
 int raptr_frame_scatter(float* d_a, int64_t rows, int nframes, int slot, const int* d_idx,
                         const float* d_val, int64_t n, int* d_bad, raptr_stream_t stream);
 int raptr_frame_gather(const float* d_a, int64_t rows, int nframes, int slot, float* d_out,
                        raptr_stream_t stream);
+int raptr_frame_copy(float* d_a, int64_t rows, int nframes, int src_slot, int dst_slot,
+                     raptr_stream_t stream);
 
 #ifdef __cplusplus
 }

@@ -149,13 +149,13 @@ _SIGS = {
     "raptr_recon_set_grouped": (i32, [vp, i32, i32, i32]),
     "raptr_recon_get_grouped": (i32, [vp, vp, vp, vp, vp, vp, vp]),
     "raptr_recon_create_opts": (i32, [vp, i32, i32, i64, vp, vp, vp, vp, i32, i32, i32, vp, vp]),
-    "raptr_recon_set_tiled": (i32, [vp, i32, i32, i32, i32]),
-    "raptr_recon_get_tiled": (i32, [vp, vp, vp, vp, vp, vp, vp]),
     "raptr_recon_get_backward": (i32, [vp, vp, vp, vp]),
     "raptr_frame_fill": (i32, [vp, i64, i32, i32, f32, vp]),
     "raptr_frame_scatter": (i32, [vp, i64, i32, i32, vp, vp, i64, vp, vp]),
     "raptr_frame_gather": (i32, [vp, i64, i32, i32, vp, vp]),
+    "raptr_frame_copy": (i32, [vp, i64, i32, i32, i32, vp]),
     "raptr_recon_mlem": (i32, [vp, i32, vp, vp, i32, i32, vp]),
+    "raptr_recon_mlem_active": (i32, [vp, i32, i32, vp, vp, i32, i32, vp]),
     "raptr_recon_forward": (i32, [vp, i32, vp, vp, vp, vp]),
     "raptr_recon_backward": (i32, [vp, i32, vp, vp]),
 }

@@ -329,120 +329,6 @@ __global__ __launch_bounds__(256) void k_fwd1c(int64_t m, const int64_t* __restr
   }
 }
 
-// k_fwd1c software-pipelined: a lane issues the gathers of its current
-// step, then the stream loads (16-bit columns, values) of its next step,
-// then waits for the gathers only (loads complete in issue order), so the
-// HBM latency of step s+1 overlaps the gathers of step s.  The loads are
-// unconditional (a divergent branch around them would make the wait count
-// cover them too): a chunk past the row end loads entries 0..3 of the
-// arrays instead, one cache line that every such lane shares, gathers x[0]
-// and is not accumulated.  Chunks are accumulated in the order k_fwd1c uses: the sums
-// are bit-identical.
-template <int VT>
-struct RawVals {
-  using T = typename std::conditional<VT == kF32, vf4, vu2>::type;
-  __device__ static T load(const typename Val<VT>::T* __restrict__ val, int64_t k) {
-    return __builtin_nontemporal_load(
-        reinterpret_cast<const T*>(reinterpret_cast<const char*>(val) + k * sizeof(typename Val<VT>::T)));
-  }
-  __device__ static void get(const T& r, float (&v)[4]) {
-    if constexpr (VT == kF32) {
-      v[0] = r.x, v[1] = r.y, v[2] = r.z, v[3] = r.w;
-    } else {
-      v[0] = h2f(uint16_t(r.x & 0xffff), VT), v[1] = h2f(uint16_t(r.x >> 16), VT);
-      v[2] = h2f(uint16_t(r.y & 0xffff), VT), v[3] = h2f(uint16_t(r.y >> 16), VT);
-    }
-  }
-};
-
-template <int VT, int LANES, int U>
-__global__ __launch_bounds__(256) void k_fwd1cp(int64_t m, const int64_t* __restrict__ rp,
-                                                const uint16_t* __restrict__ lo,
-                                                const uint64_t* __restrict__ split,
-                                                const typename Val<VT>::T* __restrict__ val,
-                                                const float* __restrict__ rscale,
-                                                const float* __restrict__ x,
-                                                const float* __restrict__ y,
-                                                float* __restrict__ e) {
-  using RV = RawVals<VT>;
-  const int g = threadIdx.x % LANES;
-  const int64_t i = int64_t(blockIdx.x) * (256 / LANES) + threadIdx.x / LANES;
-  if (i >= m) return;
-  const int64_t b = rp[i], en = rp[i + 1];  // multiples of 4
-  const int P = int(en - b);
-  const uint64_t sp = split[i];
-  const int s1 = int(sp & 0xffff), s2 = int((sp >> 16) & 0xffff), s3 = int((sp >> 32) & 0xffff);
-  constexpr int64_t STEP = int64_t(4) * LANES * U;
-  float acc = 0.0f;
-  int64_t k = b + 4 * g;
-  if (k < en) {
-    vus4 h[U];
-    typename RV::T w[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int64_t q = k + 4 * LANES * u < en ? k + 4 * LANES * u : 0;
-      h[u] = __builtin_nontemporal_load(reinterpret_cast<const vus4*>(lo + q));
-      w[u] = RV::load(val, q);
-    }
-    for (;;) {
-      float xg[U][4];
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int d = int(k - b) + 4 * LANES * u;
-        const bool ok = d < P;  // a chunk past the row end gathers x[0]
-        const int dd = ok ? d : int(k - b);
-        const int cb = dd - dd % kIlvFwd;
-        const int L = P - cb < kIlvFwd ? P - cb : kIlvFwd;
-        const int q = L >> 2;
-        const int o0 = cb + ((dd - cb) >> 2);
-        const unsigned short hl[4] = {h[u].x, h[u].y, h[u].z, h[u].w};
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const int o = o0 + j * q;
-          const int hi = int(o >= s1) + int(o >= s2) + int(o >= s3);
-          xg[u][j] = x[ok ? (hi << 16) | int(hl[j]) : 0];
-        }
-      }
-      typename RV::T wc[U];
-#pragma unroll
-      for (int u = 0; u < U; ++u) wc[u] = w[u];
-      const int64_t kn = k + STEP;
-      const bool more = kn < en;
-      __builtin_amdgcn_sched_barrier(0);
-      // columns first: the next step's gathers wait for them only
-      int64_t qn[U];
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        qn[u] = kn + 4 * LANES * u < en ? kn + 4 * LANES * u : 0;
-        h[u] = __builtin_nontemporal_load(reinterpret_cast<const vus4*>(lo + qn[u]));
-      }
-      __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-      for (int u = 0; u < U; ++u) w[u] = RV::load(val, qn[u]);
-      __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        float v[4];
-        RV::get(wc[u], v);
-        if (k + 4 * LANES * u < en) {
-#pragma unroll
-          for (int j = 0; j < 4; ++j) acc = fmaf(v[j], xg[u][j], acc);
-        }
-      }
-      if (!more) break;
-      k = kn;
-    }
-  }
-#pragma unroll
-  for (int off = LANES / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LANES);
-  if (g == 0) {
-    float sc = VT == kF32 ? 1.0f : rscale[i];
-    float yt = sc * acc;
-    float r = yt != 0.0f ? y[i] / yt : 0.0f;
-    e[i] = VT == kF32 ? r : sc * r;  // ratio folded with the row scale
-  }
-}
-
 // Single frame with x staged in LDS: a workgroup takes kLocRows
 // consecutive plan rows (neighbouring lines through the same voxels in the
 // locality row order), copies x over the sorted union of their columns into
@@ -1047,218 +933,10 @@ __global__ __launch_bounds__(256) void k_grp_fill(L lists, int64_t ng,
   }
 }
 
-// the G member values of union entry k
-template <int VT, int G>
-__device__ __forceinline__ void load_vals(const typename Val<VT>::T* __restrict__ v, int64_t k,
-                                          float (&o)[G]) {
-  if constexpr (VT == kF32) {
-    const vf4* p = reinterpret_cast<const vf4*>(v + int64_t(G) * k);
-#pragma unroll
-    for (int i = 0; i < G / 4; ++i) {
-      const vf4 t = p[i];
-      o[4 * i] = t.x, o[4 * i + 1] = t.y, o[4 * i + 2] = t.z, o[4 * i + 3] = t.w;
-    }
-  } else {
-    const vu2* p = reinterpret_cast<const vu2*>(v + int64_t(G) * k);
-#pragma unroll
-    for (int i = 0; i < G / 4; ++i) {
-      const vu2 t = p[i];
-      o[4 * i] = h2f(uint16_t(t.x & 0xffff), VT), o[4 * i + 1] = h2f(uint16_t(t.x >> 16), VT);
-      o[4 * i + 2] = h2f(uint16_t(t.y & 0xffff), VT), o[4 * i + 3] = h2f(uint16_t(t.y >> 16), VT);
-    }
-  }
-}
-
 // the wave's index inside its workgroup, as a wave-uniform value (so that
 // per-wave pointers and loop bounds live in scalar registers)
 __device__ __forceinline__ int wave_id() {
   return __builtin_amdgcn_readfirstlane(int(threadIdx.x) / kWave);
-}
-
-// U union entries of a grouped list: their keys and member values
-// (wave-uniform, so they sit in scalar registers)
-template <int VT, int G, int U>
-struct GSet {
-  int c[U];
-  float v[U][G];
-};
-
-template <int VT, int G, int U>
-__device__ __forceinline__ void gload(GSet<VT, G, U>& s, const int* __restrict__ key,
-                                      const typename Val<VT>::T* __restrict__ val, int64_t k) {
-#pragma unroll
-  for (int u = 0; u < U; ++u) {
-    s.c[u] = key[k + u];
-    load_vals<VT, G>(val, k + u, s.v[u]);
-  }
-}
-
-// the lane's frame of the U frame vectors base[c][0..63]
-template <int U>
-__device__ __forceinline__ void ggather(const int (&c)[U], const float* __restrict__ base,
-                                        float (&g)[U]) {
-#pragma unroll
-  for (int u = 0; u < U; ++u) g[u] = base[int64_t(c[u]) * 64];
-}
-
-template <int G, int U>
-__device__ __forceinline__ void gfma(const float (&v)[U][G], const float (&g)[U],
-                                     float (&acc)[G]) {
-#pragma unroll
-  for (int u = 0; u < U; ++u)
-#pragma unroll
-    for (int q = 0; q < G; ++q) acc[q] = fmaf(v[u][q], g[u], acc[q]);
-}
-
-// acc[q] += sum over the union entries [b, en) of v[q] * base[key][lane].
-// PIPE: two register sets; the next set's scalar loads are issued after the
-// current set's gathers (scheduling barriers keep that order), so their
-// latency overlaps the gathers' instead of adding to it.
-template <int VT, int G, int U, bool PIPE>
-__device__ __forceinline__ void grouped_sum(const int* __restrict__ key,
-                                            const typename Val<VT>::T* __restrict__ val,
-                                            int64_t b, int64_t en,
-                                            const float* __restrict__ base, float (&acc)[G]) {
-  int64_t k = b;
-  const int64_t nfull = (en - b) / U;
-  if (PIPE) {
-    // pairs of sets without a branch between them; the next set is loaded
-    // unconditionally (up to 2U entries past the pairs, inside the list or
-    // the layout's tail padding, values unused)
-    if (nfull & 1) {
-      GSet<VT, G, U> A;
-      float g[U];
-      gload<VT, G, U>(A, key, val, k);
-      ggather<U>(A.c, base, g);
-      gfma<G, U>(A.v, g, acc);
-      k += U;
-    }
-    const int64_t npairs = nfull >> 1;
-    if (npairs > 0) {
-      GSet<VT, G, U> A, B;
-      gload<VT, G, U>(A, key, val, k);
-      for (int64_t p = 0; p < npairs; ++p) {
-        float g[U], h[U];
-        ggather<U>(A.c, base, g);
-        __builtin_amdgcn_sched_barrier(0);
-        gload<VT, G, U>(B, key, val, k + U);
-        __builtin_amdgcn_sched_barrier(0);
-        gfma<G, U>(A.v, g, acc);
-        ggather<U>(B.c, base, h);
-        __builtin_amdgcn_sched_barrier(0);
-        gload<VT, G, U>(A, key, val, k + 2 * U);
-        __builtin_amdgcn_sched_barrier(0);
-        gfma<G, U>(B.v, h, acc);
-        k += 2 * U;
-      }
-    }
-  } else {
-    for (int64_t it = 0; it < nfull; ++it, k += U) {
-      GSet<VT, G, U> A;
-      float g[U];
-      gload<VT, G, U>(A, key, val, k);
-      ggather<U>(A.c, base, g);
-      gfma<G, U>(A.v, g, acc);
-    }
-  }
-  for (; k < en; ++k) {
-    GSet<VT, G, 1> A;
-    float g[1];
-    gload<VT, G, 1>(A, key, val, k);
-    ggather<1>(A.c, base, g);
-    gfma<G, 1>(A.v, g, acc);
-  }
-}
-
-// Grouped forward + ratio, 64 frames: one wave per group of G rows, lane =
-// frame.  Per union entry the column and the G values are wave-uniform
-// loads; the frame vector x[col][0..63] is one 256-byte gather shared by
-// the G rows.
-template <int VT, int G, int U, bool PIPE>
-__global__ __launch_bounds__(256) void k_fwdg64(int64_t ng, int64_t m,
-                                                const int64_t* __restrict__ gp,
-                                                const int* __restrict__ gcol,
-                                                const typename Val<VT>::T* __restrict__ gval,
-                                                const float* __restrict__ rscale,
-                                                const float* __restrict__ x,
-                                                const float* __restrict__ y,
-                                                float* __restrict__ e) {
-  const int lane = threadIdx.x & (kWave - 1);
-  const int64_t g = xcd_block(blockIdx.x, gridDim.x) * (256 / kWave) + wave_id();
-  if (g >= ng) return;
-  float acc[G];
-#pragma unroll
-  for (int s = 0; s < G; ++s) acc[s] = 0.0f;
-  grouped_sum<VT, G, U, PIPE>(gcol, gval, gp[g], gp[g + 1], x + lane, acc);
-#pragma unroll
-  for (int s = 0; s < G; ++s) {
-    const int64_t r = int64_t(G) * g + s;
-    if (r >= m) break;
-    const float sc = VT == kF32 ? 1.0f : rscale[r];
-    const float yy = y[r * 64 + lane];
-    const float yt = sc * acc[s];
-    const float t = (yt != 0.0f && yy != 0.0f) ? yy / yt : 0.0f;
-    e[r * 64 + lane] = VT == kF32 ? t : sc * t;  // ratio folded with the row scale
-  }
-}
-
-// Grouped backward, 64 frames: one wave per brick of G voxels, lane =
-// frame, over nbl consecutive row blocks; the ratio vector of each union
-// row is one 256-byte gather shared by the brick's voxels.  The last launch
-// applies x' = x*c/(s + (s==0)) and writes the workgroup's FP64 partial
-// sums per frame.
-template <int VT, int G, int U, bool PIPE>
-__global__ __launch_bounds__(256) void k_bwdg64(int ngc, const int64_t* __restrict__ gp,
-                                                int nbl, const int* __restrict__ grow,
-                                                const typename Val<VT>::T* __restrict__ gval,
-                                                const int* __restrict__ cols,
-                                                const float* __restrict__ e,
-                                                float* __restrict__ c, int first, int last,
-                                                const float* __restrict__ x,
-                                                const float* __restrict__ s,
-                                                float* __restrict__ xx,
-                                                double* __restrict__ partials) {
-  const int lane = threadIdx.x & (kWave - 1);
-  const int w = wave_id();
-  const int lb = int(xcd_block(blockIdx.x, gridDim.x));
-  const int h = lb * (256 / kWave) + w;
-  float acc[G];
-#pragma unroll
-  for (int q = 0; q < G; ++q) acc[q] = 0.0f;
-  if (h < ngc) {
-    for (int bb = 0; bb < nbl; ++bb)
-      grouped_sum<VT, G, U, PIPE>(grow, gval, gp[int64_t(bb) * ngc + h],
-                                  gp[int64_t(bb) * ngc + h + 1], e + lane, acc);
-  }
-  double part = 0.0;
-  if (h < ngc) {
-#pragma unroll
-    for (int q = 0; q < G; ++q) {
-      const int j = cols[int64_t(h) * G + q];
-      if (j < 0) continue;
-      const int64_t o = int64_t(j) * 64 + lane;
-      const float cv = first ? acc[q] : c[o] + acc[q];
-      if (!last) {
-        c[o] = cv;
-      } else {
-        float d = s[j];
-        d += float(d == 0.0f);
-        const float nv = x[o] * cv / d;
-        xx[o] = nv;
-        part += double(nv);
-      }
-    }
-  }
-  if (last) {
-    __shared__ double red[256 / kWave][64];
-    red[w][lane] = part;
-    __syncthreads();
-    if (threadIdx.x < 64) {
-      double t = 0.0;
-      for (int ww = 0; ww < 256 / kWave; ++ww) t += red[ww][threadIdx.x];
-      partials[int64_t(lb) * 64 + threadIdx.x] = t;
-    }
-  }
 }
 
 // ---- 16- and 32-member groups on the matrix cores
@@ -1311,16 +989,32 @@ __device__ __forceinline__ float load_val1(const typename Val<VT>::T* __restrict
     return h2f(__builtin_nontemporal_load(v + i), VT);
 }
 
-// keys and values of one step (K*U union entries; lane l takes entry
-// K*u + l/W and member l%W)
-template <int U>
-struct MSet {
-  int c[U];
-  float a[U];
+// a lane's FB gathered frames: one FB-float vector load (FB = 1: a float)
+template <int FB>
+struct GV {
+  typedef float type __attribute__((ext_vector_type(FB)));
+  __device__ static float at(const type& v, int i) { return v[i]; }
+};
+template <>
+struct GV<1> {
+  typedef float type;
+  __device__ static float at(const type& v, int) { return v; }
 };
 
-template <int VT, int G, int U>
-__device__ __forceinline__ void mload(MSet<U>& t, const int* __restrict__ key,
+// keys, values and gathered frame vectors of one step (K*U union entries;
+// lane l takes entry K*u + l/W and member l%W).  FB frame blocks: the
+// first W*FB frames of each 64-frame vector (FB < MF<G>::FB: passes over
+// the first 16 or 32 frames only)
+template <int G, int U, int FB>
+struct MSet {
+  using gvec = typename GV<FB>::type;
+  int c[U];
+  float a[U];
+  gvec g[U];
+};
+
+template <int VT, int G, int U, int FB>
+__device__ __forceinline__ void mload(MSet<G, U, FB>& t, const int* __restrict__ key,
                                       const typename Val<VT>::T* __restrict__ val, int64_t k0,
                                       int kq, int rw) {
 #pragma unroll
@@ -1331,66 +1025,114 @@ __device__ __forceinline__ void mload(MSet<U>& t, const int* __restrict__ key,
   }
 }
 
-// one step: gathers of set S, then the loads of set T (two steps ahead),
-// then the matrix products of S.  Vector loads complete in order, so the
-// products wait for S's gathers but not for T's loads.
-template <int VT, int G, int U>
-__device__ __forceinline__ void mstep(const MSet<U>& S, MSet<U>& T, bool load_t,
+// the frame vectors of a loaded set (lane l: frames FB*(l%W) .. + FB-1)
+template <int G, int U, int FB>
+__device__ __forceinline__ void mgather(MSet<G, U, FB>& t, const float* __restrict__ base) {
+  using gvec = typename MSet<G, U, FB>::gvec;
+#pragma unroll
+  for (int u = 0; u < U; ++u) t.g[u] = *reinterpret_cast<const gvec*>(base + int64_t(t.c[u]) * 64);
+}
+
+// one step: the loads of T (two steps ahead), the gathers of N (one step
+// ahead), then the matrix products of S.  Vector loads complete in order,
+// so S's products wait for S's gathers (issued a step earlier) but neither
+// for N's gathers nor for T's loads: each gather has a whole step of other
+// waves' work to hide behind.
+template <int VT, int G, int U, int FB>
+__device__ __forceinline__ void mstep(const MSet<G, U, FB>& S, MSet<G, U, FB>& N,
+                                      MSet<G, U, FB>& T, bool load_t, bool gather_n,
                                       const int* __restrict__ key,
                                       const typename Val<VT>::T* __restrict__ val, int64_t kt,
                                       int kq, int rw, const float* __restrict__ base,
-                                      typename MF<G>::acc_t (&acc)[MF<G>::FB]) {
+                                      typename MF<G>::acc_t (&acc)[FB]) {
   using M = MF<G>;
-  typedef float gvec __attribute__((ext_vector_type(M::FB)));
-  gvec g[U];
-#pragma unroll
-  for (int u = 0; u < U; ++u) g[u] = *reinterpret_cast<const gvec*>(base + int64_t(S.c[u]) * 64);
+  if (load_t) mload<VT, G, U, FB>(T, key, val, kt, kq, rw);
   __builtin_amdgcn_sched_barrier(0);
-  if (load_t) mload<VT, G, U>(T, key, val, kt, kq, rw);
+  if (gather_n) mgather<G, U, FB>(N, base);
   __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
   for (int u = 0; u < U; ++u)
 #pragma unroll
-    for (int fb = 0; fb < M::FB; ++fb) acc[fb] = M::op(S.a[u], g[u][fb], acc[fb]);
+    for (int fb = 0; fb < FB; ++fb) acc[fb] = M::op(S.a[u], GV<FB>::at(S.g[u], fb), acc[fb]);
+}
+
+// one step with the gathers in it: the gathers of S, the loads of T (two
+// steps ahead), then the products of S (waiting for S's gathers only)
+template <int VT, int G, int U, int FB>
+__device__ __forceinline__ void mstep_in(MSet<G, U, FB>& S, MSet<G, U, FB>& T, bool load_t,
+                                         const int* __restrict__ key,
+                                         const typename Val<VT>::T* __restrict__ val,
+                                         int64_t kt, int kq, int rw,
+                                         const float* __restrict__ base,
+                                         typename MF<G>::acc_t (&acc)[FB]) {
+  using M = MF<G>;
+  mgather<G, U, FB>(S, base);
+  __builtin_amdgcn_sched_barrier(0);
+  if (load_t) mload<VT, G, U, FB>(T, key, val, kt, kq, rw);
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int u = 0; u < U; ++u)
+#pragma unroll
+    for (int fb = 0; fb < FB; ++fb) acc[fb] = M::op(S.a[u], GV<FB>::at(S.g[u], fb), acc[fb]);
 }
 
 // acc[fb] += the group's sums over the union entries [b, en) (a multiple
-// of K*U) for frames W*fb .. W*fb+W-1, in steps of K*U entries with three
-// register sets (loads two steps ahead); reads up to 2*K*U entries past en
-// (inside the next list or the tail padding)
-template <int VT, int G, int U>
+// of K*U) for every frame, in steps of K*U entries with three register sets
+// (keys and values two steps ahead; frame gathers in the step, or one step
+// ahead with AHEAD); reads keys and values up to 2*K*U entries past en and
+// gathers frame vectors of keys up to K*U entries past en (inside the next
+// list or the tail padding, whose keys are valid columns).  Gathering ahead
+// gives each gather a step of other waves' work to hide behind (the 64-frame
+// backward, 12.0 -> 11.6 ms at full size) but costs the forward, whose
+// higher occupancy already hides it (9.2 -> 9.9 ms).
+template <int VT, int G, int U, bool AHEAD, int FB>
 __device__ __forceinline__ void mfma_sum(const int* __restrict__ key,
                                          const typename Val<VT>::T* __restrict__ val,
                                          int64_t b, int64_t en, const float* __restrict__ base,
-                                         int kq, int rw,
-                                         typename MF<G>::acc_t (&acc)[MF<G>::FB]) {
+                                         int kq, int rw, typename MF<G>::acc_t (&acc)[FB]) {
   constexpr int E = MF<G>::K * U;
   const int64_t steps = (en - b) / E;
   if (steps <= 0) return;
-  MSet<U> A, B, C;
-  mload<VT, G, U>(A, key, val, b, kq, rw);
-  mload<VT, G, U>(B, key, val, b + E, kq, rw);
+  MSet<G, U, FB> A, B, C;
+  mload<VT, G, U, FB>(A, key, val, b, kq, rw);
+  mload<VT, G, U, FB>(B, key, val, b + E, kq, rw);
   int64_t kt = b + 2 * E;
   int64_t s = 0;
-  for (; s + 3 <= steps; s += 3) {
-    mstep<VT, G, U>(A, C, true, key, val, kt, kq, rw, base, acc);
-    mstep<VT, G, U>(B, A, true, key, val, kt + E, kq, rw, base, acc);
-    mstep<VT, G, U>(C, B, true, key, val, kt + 2 * E, kq, rw, base, acc);
-    kt += 3 * E;
+  if constexpr (AHEAD) {
+    mgather<G, U, FB>(A, base);
+    for (; s + 3 <= steps; s += 3) {
+      mstep<VT, G, U, FB>(A, B, C, true, true, key, val, kt, kq, rw, base, acc);
+      mstep<VT, G, U, FB>(B, C, A, true, true, key, val, kt + E, kq, rw, base, acc);
+      mstep<VT, G, U, FB>(C, A, B, true, true, key, val, kt + 2 * E, kq, rw, base, acc);
+      kt += 3 * E;
+    }
+    if (s < steps)
+      mstep<VT, G, U, FB>(A, B, C, false, s + 1 < steps, key, val, kt, kq, rw, base, acc);
+    if (s + 1 < steps)
+      mstep<VT, G, U, FB>(B, C, A, false, false, key, val, kt, kq, rw, base, acc);
+  } else {
+    for (; s + 3 <= steps; s += 3) {
+      mstep_in<VT, G, U, FB>(A, C, true, key, val, kt, kq, rw, base, acc);
+      mstep_in<VT, G, U, FB>(B, A, true, key, val, kt + E, kq, rw, base, acc);
+      mstep_in<VT, G, U, FB>(C, B, true, key, val, kt + 2 * E, kq, rw, base, acc);
+      kt += 3 * E;
+    }
+    if (s < steps) mstep_in<VT, G, U, FB>(A, C, false, key, val, kt, kq, rw, base, acc);
+    if (s + 1 < steps) mstep_in<VT, G, U, FB>(B, C, false, key, val, kt, kq, rw, base, acc);
   }
-  if (s < steps) mstep<VT, G, U>(A, C, false, key, val, kt, kq, rw, base, acc);
-  if (s + 1 < steps) mstep<VT, G, U>(B, C, false, key, val, kt, kq, rw, base, acc);
 }
 
-template <int G>
-__device__ __forceinline__ void acc_zero(typename MF<G>::acc_t (&acc)[MF<G>::FB]) {
+template <int G, int FB>
+__device__ __forceinline__ void acc_zero(typename MF<G>::acc_t (&acc)[FB]) {
 #pragma unroll
-  for (int fb = 0; fb < MF<G>::FB; ++fb)
+  for (int fb = 0; fb < FB; ++fb)
 #pragma unroll
     for (int r = 0; r < MF<G>::NR; ++r) acc[fb][r] = 0.0f;
 }
 
-template <int VT, int G, int U>
+// FB frame blocks: frames 0 .. W*FB - 1 of each 64-frame row (FB =
+// MF<G>::FB: all 64); ratios of the other frames are not written.
+template <int VT, int G, int U, int FB>
 __global__ __launch_bounds__(256) void k_fwdm64(int64_t ng, int64_t m,
                                                 const int64_t* __restrict__ gp,
                                                 const int* __restrict__ gcol,
@@ -1404,17 +1146,17 @@ __global__ __launch_bounds__(256) void k_fwdm64(int64_t ng, int64_t m,
   const int kq = lane / M::W, rw = lane % M::W;
   const int64_t g = xcd_block(blockIdx.x, gridDim.x) * (256 / kWave) + wave_id();
   if (g >= ng) return;
-  typename M::acc_t acc[M::FB];
-  acc_zero<G>(acc);
-  mfma_sum<VT, G, U>(gcol, gval, gp[g], gp[g + 1], x + M::FB * rw, kq, rw, acc);
+  typename M::acc_t acc[FB];
+  acc_zero<G, FB>(acc);
+  mfma_sum<VT, G, U, false, FB>(gcol, gval, gp[g], gp[g + 1], x + FB * rw, kq, rw, acc);
 #pragma unroll
   for (int r = 0; r < M::NR; ++r) {
     const int64_t row = int64_t(G) * g + M::member(lane, r);
     if (row >= m) continue;
     const float sc = VT == kF32 ? 1.0f : rscale[row];
 #pragma unroll
-    for (int fb = 0; fb < M::FB; ++fb) {
-      const int64_t o = row * 64 + M::FB * rw + fb;
+    for (int fb = 0; fb < FB; ++fb) {
+      const int64_t o = row * 64 + FB * rw + fb;
       const float yy = y[o];
       const float yt = sc * acc[fb][r];
       const float t = (yt != 0.0f && yy != 0.0f) ? yy / yt : 0.0f;
@@ -1441,10 +1183,11 @@ __global__ __launch_bounds__(256) void k_bwdm64(int ngc, const int64_t* __restri
   const int lb = int(xcd_block(blockIdx.x, gridDim.x));
   const int h = lb * (256 / kWave) + w;
   M::acc_t acc[4];
-  acc_zero<16>(acc);
+  acc_zero<16, 4>(acc);
   if (h < ngc) {
     for (int bb = 0; bb < nbl; ++bb)
-      mfma_sum<VT, 16, 2>(grow, gval, gp[int64_t(bb) * ngc + h], gp[int64_t(bb) * ngc + h + 1],
+      mfma_sum<VT, 16, 2, true, 4>(grow, gval, gp[int64_t(bb) * ngc + h],
+                                gp[int64_t(bb) * ngc + h + 1],
                           e + 4 * r16, kq, r16, acc);
   }
   double part[4] = {0.0, 0.0, 0.0, 0.0};
@@ -1500,7 +1243,9 @@ __global__ __launch_bounds__(256) void k_bwdm64(int ngc, const int64_t* __restri
 // - 1 of the XCD's range (several launches keep the bricks resident on an
 // XCD within a few blocks of each other, so the ratio rows they gather fit
 // its L2); accum: add to the partial images instead of writing them.
-template <int VT, int G>
+// FB frame blocks as in k_fwdm64: the partial images of frames W*FB .. 63
+// are not written.
+template <int VT, int G, int FB>
 __global__ __launch_bounds__(256) void k_bwdm64x(int ngc, int nblk, int n,
                                                  const int64_t* __restrict__ gp,
                                                  const int* __restrict__ grow,
@@ -1519,27 +1264,28 @@ __global__ __launch_bounds__(256) void k_bwdm64x(int ngc, int nblk, int n,
   const int b0 = int(int64_t(nblk) * xcd / 8), b1 = int(int64_t(nblk) * (xcd + 1) / 8);
   const int s0 = b0 + boff < b1 ? b0 + boff : b1;
   const int s1 = s0 + bcnt < b1 ? s0 + bcnt : b1;
-  typename M::acc_t acc[M::FB];
-  acc_zero<G>(acc);
+  typename M::acc_t acc[FB];
+  acc_zero<G, FB>(acc);
   for (int b = s0; b < s1; ++b)
-    mfma_sum<VT, G, U>(grow, gval, gp[int64_t(b) * ngc + h], gp[int64_t(b) * ngc + h + 1],
-                       e + M::FB * rw, kq, rw, acc);
+    mfma_sum<VT, G, U, true, FB>(grow, gval, gp[int64_t(b) * ngc + h],
+                                 gp[int64_t(b) * ngc + h + 1], e + FB * rw, kq, rw, acc);
   float* out = c8 + int64_t(xcd) * n * 64;
 #pragma unroll
   for (int r = 0; r < M::NR; ++r) {
     const int j = cols[int64_t(h) * G + M::member(lane, r)];
     if (j < 0) continue;
 #pragma unroll
-    for (int fb = 0; fb < M::FB; ++fb) {
-      const int64_t o = int64_t(j) * 64 + M::FB * rw + fb;
+    for (int fb = 0; fb < FB; ++fb) {
+      const int64_t o = int64_t(j) * 64 + FB * rw + fb;
       out[o] = accum ? out[o] + acc[fb][r] : acc[fb][r];
     }
   }
 }
 
 // c = sum of the 8 partial images (XCD order), x' = x*c/(s + (s==0)), and
-// FP64 partial sums per workgroup (4 voxels) and frame
-__global__ __launch_bounds__(256) void k_bwd_xreduce(int n, const float* __restrict__ c8,
+// FP64 partial sums per workgroup (4 voxels) and frame; frames fa .. 63 are
+// skipped (x' and their partial sums not written)
+__global__ __launch_bounds__(256) void k_bwd_xreduce(int n, int fa, const float* __restrict__ c8,
                                                      const float* __restrict__ x,
                                                      const float* __restrict__ s,
                                                      float* __restrict__ xx,
@@ -1547,7 +1293,7 @@ __global__ __launch_bounds__(256) void k_bwd_xreduce(int n, const float* __restr
   const int f = threadIdx.x & 63, jj = threadIdx.x >> 6;
   const int64_t j = int64_t(blockIdx.x) * 4 + jj;
   double part = 0.0;
-  if (j < n) {
+  if (j < n && f < fa) {
     float cv = 0.0f;
 #pragma unroll
     for (int q = 0; q < 8; ++q) cv += c8[(int64_t(q) * n + j) * 64 + f];
@@ -1560,7 +1306,7 @@ __global__ __launch_bounds__(256) void k_bwd_xreduce(int n, const float* __restr
   __shared__ double red[4][64];
   red[jj][f] = part;
   __syncthreads();
-  if (threadIdx.x < 64)
+  if (threadIdx.x < fa)
     partials[int64_t(blockIdx.x) * 64 + f] = red[0][f] + red[1][f] + red[2][f] + red[3][f];
 }
 
@@ -1584,8 +1330,13 @@ __device__ __forceinline__ vi4 rows4(const void* __restrict__ brow, int64_t k, i
 }
 
 // LPC lanes per column (16, 32 or 64), 256 / LPC columns per workgroup.
+// boff / bcnt: this launch covers row blocks b0 + boff .. b0 + boff + bcnt
+// - 1 of the XCD's range (several launches keep the ratio rows the XCD's
+// resident columns gather within a few blocks, so they stay in its L2
+// while the matrix streams through); accum: add to c8 instead of writing.
 template <int VT, int UNROLL, bool R16, int LPC>
-__global__ __launch_bounds__(256) void k_bwd1x(int n, int nblk, int blk_rows,
+__global__ __launch_bounds__(256) void k_bwd1x(int n, int nblk, int blk_rows, int boff,
+                                               int bcnt, int accum,
                                                const int64_t* __restrict__ ptr,
                                                const void* __restrict__ brow,
                                                const typename Val<VT>::T* __restrict__ val,
@@ -1595,7 +1346,9 @@ __global__ __launch_bounds__(256) void k_bwd1x(int n, int nblk, int blk_rows,
   const int xcd = int(blockIdx.x % 8);
   const int j = int(blockIdx.x / 8) * (256 / LPC) + int(threadIdx.x) / LPC;
   if (j >= n) return;  // the column's LPC lanes leave together
-  const int b0 = int(int64_t(nblk) * xcd / 8), b1 = int(int64_t(nblk) * (xcd + 1) / 8);
+  const int x0 = int(int64_t(nblk) * xcd / 8), x1 = int(int64_t(nblk) * (xcd + 1) / 8);
+  const int b0 = x0 + boff < x1 ? x0 + boff : x1;
+  const int b1 = b0 + bcnt < x1 ? b0 + bcnt : x1;
   constexpr int64_t STEP = 4 * LPC;
   float acc = 0.0f;
   // the segment bounds of up to LPC/2 blocks in one load (lane 2t, 2t+1:
@@ -1643,7 +1396,10 @@ __global__ __launch_bounds__(256) void k_bwd1x(int n, int nblk, int blk_rows,
   }
 #pragma unroll
   for (int off = LPC / 2; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, LPC);
-  if (lane == 0) c8[int64_t(xcd) * n + j] = acc;
+  if (lane == 0) {
+    float* o = c8 + int64_t(xcd) * n + j;
+    *o = accum ? *o + acc : acc;
+  }
 }
 
 // c = sum of the 8 partial column sums (XCD order), x' = x*c/(s + (s==0)),
@@ -1675,11 +1431,17 @@ __global__ __launch_bounds__(256) void k_bwd1_xreduce(int n, const float* __rest
   if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
 }
 
-// brick of the backward groups: G voxels (x, y, z extents)
+// brick of the backward groups: G voxels (x, y, z extents), wide in x and z
+// and thin in y.  The lines of a two-head setup rotated about y run nearly
+// parallel to the x-z planes, so a line that crosses a 4 x 1 x 4 brick meets
+// more of its voxels than one crossing a 2 x 2 x 4 brick: on the 52^3 grid
+// of 13 x 13-pixel heads 6.1 of 16 members per union entry against 5.4
+// (10 % fewer entries), and 8.6 of 32 for 4 x 2 x 4 against 6.8 for
+// 2 x 4 x 4.
 __host__ __device__ inline void brick_dims(int G, int& bx, int& by, int& bz) {
-  bx = G >= 8 ? 2 : 1;
-  by = G >= 32 ? 4 : 2;
-  bz = G >= 16 ? 4 : 2;
+  bx = 4;
+  by = G >= 32 ? 2 : 1;
+  bz = 4;
 }
 
 // cols[h*G + s]: member s of brick h; -1 outside the grid.  Bricks in y,
@@ -1708,205 +1470,6 @@ __global__ void k_seq_cols(int n, int64_t total, int* __restrict__ cols) {
   for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < total;
        t += int64_t(gridDim.x) * blockDim.x)
     cols[t] = t < n ? int(t) : -1;
-}
-
-// Single-frame backward staged through LDS: each workgroup owns a
-// contiguous range of columns (voxels) and walks the row blocks in order;
-// block b's slice of the ratio vector is loaded into LDS once per workgroup
-// and the column segments of that block gather from LDS instead of L2.
-// Column sums stay in LDS across blocks (fixed order, no atomics); the last
-// step applies the update and writes one FP64 partial sum per workgroup.
-template <int VT, int LPC, int U>
-__global__ __launch_bounds__(256) void k_bwd_lds(int n, int nblk, int blk_rows, int64_t m,
-                                                 int cols_per_wg,
-                                                 const int64_t* __restrict__ ptr,
-                                                 const int* __restrict__ brow,
-                                                 const typename Val<VT>::T* __restrict__ val,
-                                                 const float* __restrict__ e,
-                                                 const float* __restrict__ x,
-                                                 const float* __restrict__ s,
-                                                 float* __restrict__ xx,
-                                                 double* __restrict__ partials) {
-  extern __shared__ float lds[];
-  float* eslice = lds;               // blk_rows
-  float* cacc = lds + blk_rows;      // cols_per_wg
-  constexpr int GROUPS = 256 / LPC;
-  const int t = threadIdx.x, grp = t / LPC, g = t % LPC;
-  const int G = gridDim.x, bx = blockIdx.x;
-  const int xcd = bx % 8, q = G / 8, r8 = G % 8;
-  const int wg = xcd * q + (xcd < r8 ? xcd : r8) + bx / 8;
-  const int c0 = wg * cols_per_wg;
-  const int c1 = min(n, c0 + cols_per_wg);
-  for (int k = t; k < cols_per_wg; k += 256) cacc[k] = 0.0f;
-  for (int b = 0; b < nblk; ++b) {
-    const int64_t r0 = int64_t(b) * blk_rows;
-    const int rows = int(min(int64_t(blk_rows), m - r0));
-    __syncthreads();
-    for (int k = 4 * t; k < rows; k += 1024) {
-      if (k + 3 < rows) {
-        vf4 v = *reinterpret_cast<const vf4*>(e + r0 + k);
-        eslice[k] = v.x, eslice[k + 1] = v.y, eslice[k + 2] = v.z, eslice[k + 3] = v.w;
-      } else {
-        for (int q2 = k; q2 < rows; ++q2) eslice[q2] = e[r0 + q2];
-      }
-    }
-    __syncthreads();
-    const int64_t* pb = ptr + int64_t(b) * (n + 1);
-    for (int j = c0 + grp; j < c1; j += GROUPS) {
-      const int64_t k0 = pb[j], k1 = pb[j + 1];
-      float acc = 0.0f;
-      int64_t k = k0 + g;
-      for (; k + int64_t(LPC) * (U - 1) < k1; k += int64_t(LPC) * U) {
-        float v[U];
-        int r[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-          v[u] = Val<VT>::get(val, k + u * LPC);
-          r[u] = __builtin_nontemporal_load(brow + k + u * LPC);
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) acc = fmaf(v[u], eslice[r[u] - int(r0)], acc);
-      }
-      for (; k < k1; k += LPC) acc = fmaf(Val<VT>::get(val, k), eslice[brow[k] - int(r0)], acc);
-#pragma unroll
-      for (int off = LPC / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LPC);
-      if (g == 0) cacc[j - c0] += acc;
-    }
-  }
-  __syncthreads();
-  double part = 0.0;
-  for (int k = t; k < c1 - c0; k += 256) {
-    const int j = c0 + k;
-    float d = s[j];
-    d += float(d == 0.0f);
-    float v = x[j] * cacc[k] / d;
-    xx[j] = v;
-    part += double(v);
-  }
-  __shared__ double red[256 / kWave];
-  part = wave_reduce_sum(part);
-  if ((t & (kWave - 1)) == 0) red[t / kWave] = part;
-  __syncthreads();
-  if (t == 0) {
-    double tot = 0.0;
-    for (int w = 0; w < 256 / kWave; ++w) tot += red[w];
-    partials[wg] = tot;
-  }
-}
-
-// Single-frame backward, flat LDS-staged form.  A workgroup owns a tile of
-// 256 columns (64 per wave) and walks the row blocks in order: block b's
-// slice of the ratio vector is loaded into LDS, then each wave streams the
-// block's entries of its 64 columns as one contiguous range (16-byte value
-// loads and 16-byte loads of packed (column in tile, row in block) indices,
-// 4 entries of one column per lane), multiplies by the LDS ratio and adds
-// runs of equal column with a wave segmented scan.  The run's last lane adds
-// to the column sum in LDS; a column belongs to one wave, so there are no
-// atomics and the summation order is fixed.  After the last block the
-// update x' = x*c/(s + (s==0)) and the workgroup's FP64 partial sum follow.
-template <int VT, int U>
-__global__ __launch_bounds__(256) void k_bwd_flat(int n, int nblk, int blk_rows, int64_t m,
-                                                  const int64_t* __restrict__ ptr,
-                                                  const uint32_t* __restrict__ pidx,
-                                                  const typename Val<VT>::T* __restrict__ val,
-                                                  const float* __restrict__ e,
-                                                  const float* __restrict__ x,
-                                                  const float* __restrict__ s,
-                                                  float* __restrict__ xx,
-                                                  double* __restrict__ partials) {
-  extern __shared__ float lds[];
-  float* es = lds;                // blk_rows
-  float* cacc = lds + blk_rows;   // 256 column sums
-  const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
-  const int tile = int(xcd_block(blockIdx.x, gridDim.x));
-  const int c0 = tile * 256;
-  const int cw0 = min(n, c0 + 64 * w), cw1 = min(n, c0 + 64 * w + 64);
-  cacc[t] = 0.0f;
-  for (int b = 0; b < nblk; ++b) {
-    const int64_t r0 = int64_t(b) * blk_rows;
-    const int rows = int(min(int64_t(blk_rows), m - r0));
-    __syncthreads();
-    for (int k = 4 * t; k < rows; k += 4 * 256) {
-      if (k + 3 < rows) {
-        vf4 v = *reinterpret_cast<const vf4*>(e + r0 + k);
-        es[k] = v.x, es[k + 1] = v.y, es[k + 2] = v.z, es[k + 3] = v.w;
-      } else {
-        for (int q2 = k; q2 < rows; ++q2) es[q2] = e[r0 + q2];
-      }
-    }
-    __syncthreads();
-    const int64_t* pb = ptr + int64_t(b) * (n + 1);
-    const int64_t K0 = pb[cw0], K1 = pb[cw1];  // multiples of 4
-    for (int64_t k = K0 + 4 * lane; k - 4 * lane < K1; k += int64_t(256) * U) {
-      float v[U][4];
-      vi4 pk[U];
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int64_t kk = k + 256 * u;
-        if (kk < K1) {
-          load4<VT>(val, kk, v[u][0], v[u][1], v[u][2], v[u][3]);
-          pk[u] = __builtin_nontemporal_load(reinterpret_cast<const vi4*>(pidx + kk));
-        } else {
-          v[u][0] = v[u][1] = v[u][2] = v[u][3] = 0.0f;
-          pk[u] = vi4{-1, -1, -1, -1};
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const bool valid = k + 256 * u < K1;
-        const int col = valid ? int(uint32_t(pk[u].x) >> 24) : -1;
-        float p = 0.0f;
-        if (valid) {
-          p = v[u][0] * es[pk[u].x & 0xffffff];
-          p = fmaf(v[u][1], es[pk[u].y & 0xffffff], p);
-          p = fmaf(v[u][2], es[pk[u].z & 0xffffff], p);
-          p = fmaf(v[u][3], es[pk[u].w & 0xffffff], p);
-        }
-        // inclusive segmented scan over lanes of equal column
-#pragma unroll
-        for (int off = 1; off < kWave; off <<= 1) {
-          const float pv = __shfl_up(p, off, kWave);
-          const int pc = __shfl_up(col, off, kWave);
-          if (lane >= off && pc == col) p += pv;
-        }
-        const int nc = __shfl_down(col, 1, kWave);
-        if (valid && (lane == kWave - 1 || nc != col)) cacc[col] += p;
-      }
-    }
-  }
-  __syncthreads();
-  const int j = c0 + t;
-  double part = 0.0;
-  if (j < n) {
-    float d = s[j];
-    d += float(d == 0.0f);
-    const float v = x[j] * cacc[t] / d;
-    xx[j] = v;
-    part = double(v);
-  }
-  __shared__ double red[256 / kWave];
-  part = wave_reduce_sum(part);
-  if (lane == 0) red[w] = part;
-  __syncthreads();
-  if (t == 0) {
-    double tot = 0.0;
-    for (int q = 0; q < 256 / kWave; ++q) tot += red[q];
-    partials[tile] = tot;
-  }
-}
-
-// packed index of the flat staged backward: (column in its 256-column tile)
-// << 24 | (row in its row block), per entry of the padded transpose
-__global__ void k_bt_pack(const int64_t* pptr, int64_t total, int n, int blk_rows,
-                          const int* brow, uint32_t* pidx) {
-  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < total;
-       t += int64_t(gridDim.x) * blockDim.x) {
-    const int64_t b = t / (n + 1), j = t - b * (n + 1);
-    if (j >= n) continue;
-    const uint32_t hi = uint32_t(j & 255) << 24;
-    const int64_t r0 = b * blk_rows;
-    for (int64_t p = pptr[t]; p < pptr[t + 1]; ++p) pidx[p] = hi | uint32_t(brow[p] - r0);
-  }
 }
 
 // rows of the transpose as 16-bit offsets in their row block (blocks of at
@@ -1971,72 +1534,91 @@ __global__ __launch_bounds__(256) void k_fwd_cols16(const int64_t* __restrict__ 
 }
 
 // Column unions of the LDS-staged forward (k_fwd1l), one workgroup per
-// group of kLocRows plan rows: a bitmap of the group's columns in LDS (n <=
-// 258,048 voxels: <= 8,064 words) and the words' exclusive popcount prefix.
-// pass 0 writes the union size cnt[group]; pass 1 the sorted union
-// ucol[up[group] + r] and every padded entry's rank lcol[d] in it.
+// group of kLocRows plan rows.  The voxels are taken in windows of
+// kUnionWin (256,000: a window's bitmap and popcount prefix, 8,000 words
+// each, and 257 partials fit 64 KB of LDS); per window a bitmap of the group's columns in it
+// and the words' exclusive popcount prefix, offset by the union entries of
+// the windows before.  pass 0 writes the union size cnt[group]; pass 1 the
+// sorted union ucol[up[group] + r] and every padded entry's rank lcol[d] in
+// it.  Each window re-reads the group's entries (grids of more than one
+// window: a build-time cost only).
+constexpr int kUnionWinWords = 8000;
+constexpr int kUnionWin = kUnionWinWords * 32;
 __global__ __launch_bounds__(256) void k_loc_union(const int64_t* __restrict__ frp,
                                                    const int* __restrict__ fcol, int64_t m,
                                                    int n, int pass, int64_t* __restrict__ cnt,
                                                    const int64_t* __restrict__ up,
                                                    int* __restrict__ ucol,
                                                    uint16_t* __restrict__ lcol) {
-  extern __shared__ float ldsf[];  // bitmap, prefix (nw words each), 256 partials
+  // bitmap, prefix (nwmax words each), 256 partials and the window's total
+  extern __shared__ float ldsf[];
   uint32_t* lw = reinterpret_cast<uint32_t*>(ldsf);
-  const int nw = (n + 31) >> 5;
+  const int nwmax = (min(n, kUnionWin) + 31) >> 5;
   uint32_t* bm = lw;
-  uint32_t* pre = lw + nw;
-  uint32_t* part = lw + 2 * nw;
+  uint32_t* pre = lw + nwmax;
+  uint32_t* part = lw + 2 * nwmax;
+  uint32_t& win_total = part[256];
   const int64_t grp = blockIdx.x;
   const int64_t r0 = grp * kLocRows, r1 = r0 + kLocRows < m ? r0 + kLocRows : m;
   const int64_t d0 = frp[r0], d1 = frp[r1];
-  for (int w = threadIdx.x; w < nw; w += 256) bm[w] = 0;
-  __syncthreads();
-  for (int64_t d = d0 + threadIdx.x; d < d1; d += 256) {
-    const int c = fcol[d];
-    atomicOr(&bm[c >> 5], 1u << (c & 31));
-  }
-  __syncthreads();
-  // thread t sums words [t*per, (t+1)*per); partial sums scanned by thread 0
-  const int per = (nw + 255) / 256;
-  const int w0 = threadIdx.x * per, w1 = w0 + per < nw ? w0 + per : nw;
-  uint32_t t = 0;
-  for (int w = w0; w < w1; ++w) t += __popc(bm[w]);
-  part[threadIdx.x] = t;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t run = 0;
-    for (int j = 0; j < 256; ++j) {
-      const uint32_t v = part[j];
-      part[j] = run;
-      run += v;
+  const int64_t ub = pass ? up[grp] : 0;
+  uint32_t base = 0;  // union entries of the windows before (same in every thread)
+  for (int c0 = 0; c0 < n; c0 += kUnionWin) {
+    const int c1 = n - c0 < kUnionWin ? n : c0 + kUnionWin;
+    const int nw = (c1 - c0 + 31) >> 5;
+    for (int w = threadIdx.x; w < nw; w += 256) bm[w] = 0;
+    __syncthreads();
+    for (int64_t d = d0 + threadIdx.x; d < d1; d += 256) {
+      const int c = fcol[d];
+      if (c >= c0 && c < c1) atomicOr(&bm[(c - c0) >> 5], 1u << ((c - c0) & 31));
     }
-    if (pass == 0) cnt[grp] = int64_t(run);
-  }
-  __syncthreads();
-  if (pass == 0) return;
-  uint32_t run = part[threadIdx.x];
-  for (int w = w0; w < w1; ++w) {
-    pre[w] = run;
-    run += __popc(bm[w]);
-  }
-  __syncthreads();
-  const int64_t ub = up[grp];
-  for (int w = threadIdx.x; w < nw; w += 256) {
-    uint32_t bits = bm[w];
-    uint32_t r = pre[w];
-    while (bits) {
-      const int bb = __ffs(bits) - 1;
-      ucol[ub + r] = (w << 5) | bb;
-      ++r;
-      bits &= bits - 1;
+    __syncthreads();
+    // thread t sums words [t*per, (t+1)*per); partial sums scanned by thread 0
+    const int per = (nw + 255) / 256;
+    const int w0 = threadIdx.x * per, w1 = w0 + per < nw ? w0 + per : nw;
+    uint32_t t = 0;
+    for (int w = w0; w < w1; ++w) t += __popc(bm[w]);
+    part[threadIdx.x] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t run = 0;
+      for (int j = 0; j < 256; ++j) {
+        const uint32_t v = part[j];
+        part[j] = run;
+        run += v;
+      }
+      win_total = run;
     }
+    __syncthreads();
+    if (pass == 1) {
+      uint32_t run = base + part[threadIdx.x];
+      for (int w = w0; w < w1; ++w) {
+        pre[w] = run;
+        run += __popc(bm[w]);
+      }
+      __syncthreads();
+      for (int w = threadIdx.x; w < nw; w += 256) {
+        uint32_t bits = bm[w];
+        uint32_t r = pre[w];
+        while (bits) {
+          const int bb = __ffs(bits) - 1;
+          ucol[ub + r] = c0 + ((w << 5) | bb);
+          ++r;
+          bits &= bits - 1;
+        }
+      }
+      for (int64_t d = d0 + threadIdx.x; d < d1; d += 256) {
+        const int c = fcol[d];
+        if (c >= c0 && c < c1) {
+          const uint32_t w = uint32_t(c - c0) >> 5;
+          lcol[d] = uint16_t(pre[w] + __popc(bm[w] & ((1u << ((c - c0) & 31)) - 1)));
+        }
+      }
+    }
+    base += win_total;
+    __syncthreads();  // the next window clears the bitmap and rewrites win_total
   }
-  for (int64_t d = d0 + threadIdx.x; d < d1; d += 256) {
-    const int c = fcol[d];
-    const uint32_t w = uint32_t(c) >> 5;
-    lcol[d] = uint16_t(pre[w] + __popc(bm[w] & ((1u << (c & 31)) - 1)));
-  }
+  if (pass == 0 && threadIdx.x == 0) cnt[grp] = int64_t(base);
 }
 
 // scale_f = 1/sum_b partials[b][f] (FP64, fixed order); 0 for an empty frame
@@ -2061,11 +1643,13 @@ __global__ __launch_bounds__(256) void k_norms(const double* partials, int nb, i
   }
 }
 
-__global__ void k_scale_frames(int64_t n, int FR, const float* xx, const float* scale,
+// x = scale * x' for frames 0 .. fa - 1 of each FR-frame row (the others
+// keep their values)
+__global__ void k_scale_frames(int64_t n, int FR, int fa, const float* xx, const float* scale,
                                float* x) {
   for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < n * FR;
        t += int64_t(gridDim.x) * blockDim.x)
-    x[t] = scale[t % FR] * xx[t];
+    if (t % FR < fa) x[t] = scale[t % FR] * xx[t];
 }
 
 __global__ void k_sens_norm32(const double* sens, int n, float* s) {
@@ -2162,11 +1746,7 @@ int build_groups(raptr_recon* R, const L& lists, int64_t ng, int align, int64_t*
 // kind: 0 plain, 1 pipelined (scalar loads, vector FMAs), 2 matrix cores
 bool grp_shape_ok(int G, int U, int kind) {
   switch (G) {
-    case 4: return (U == 8 && kind == 1) || (U == 16 && kind == 0);
-    case 8: return (U == 4 && kind == 1) || (U == 8 && kind == 0);
-    case 16:
-      return (U == 2 && kind == 1) || (U == 4 && kind == 0) ||
-             ((U == 8 || U == 16 || U == 32) && (kind == 2 || kind == 3));
+    case 16: return (U == 8 || U == 16 || U == 32) && (kind == 2 || kind == 3);
     case 32: return (U == 8 || U == 16) && kind == 3;
   }
   return false;
@@ -2207,12 +1787,7 @@ int launch_fwd_any(const raptr_recon* R, int F, const float* x, const float* y, 
     const unsigned g = unsigned((m + (256 / L) - 1) / (256 / L));
 #define RAPTR_F1C(VV, LL, UU)                                                                \
   if (R->vt == VV && L == LL && U == UU) {                                                   \
-    if (R->fwd_kind == 1)                                                                    \
-      hipLaunchKernelGGL((k_fwd1cp<VV, LL, UU>), dim3(g), dim3(256), 0, st, m, R->frp,        \
-                         (const uint16_t*)R->fcol16, (const uint64_t*)R->fsplit,             \
-                         (const typename Val<VV>::T*)R->fval, R->rscale, x, y, e);           \
-    else                                                                                     \
-      hipLaunchKernelGGL((k_fwd1c<VV, LL, UU>), dim3(g), dim3(256), 0, st, m, R->frp,         \
+    hipLaunchKernelGGL((k_fwd1c<VV, LL, UU>), dim3(g), dim3(256), 0, st, m, R->frp,           \
                        (const uint16_t*)R->fcol16, (const uint64_t*)R->fsplit,               \
                        (const typename Val<VV>::T*)R->fval, R->rscale, x, y, e);             \
     RAPTR_LAUNCH_CHECK();                                                                    \
@@ -2248,42 +1823,28 @@ int launch_fwd_any(const raptr_recon* R, int F, const float* x, const float* y, 
   }
   unsigned g = unsigned((m + 3) / 4);
   if (F == 64 && R->grp64) {
+    // kinds 2 and 3 share the forward; frame blocks: the plan's active
+    // frames (R->fa) over the group's W columns
     const unsigned gg = unsigned((R->ngf + 3) / 4);
     const int GU = R->grp_unroll, GG = R->grp_g;
-    if (R->grp_pipe >= 2) {
-#define RAPTR_M64(VV, G_, UU)                                                                \
-      if (R->vt == VV && GG == G_ && GU == MF<G_>::K * UU) {                                 \
-        hipLaunchKernelGGL((k_fwdm64<VV, G_, UU>), dim3(gg), dim3(256), 0, st, R->ngf, m,     \
-                           R->gfp, R->gfcol, (const typename Val<VV>::T*)R->gfval,          \
-                           R->rscale, x, y, e);                                              \
-        RAPTR_LAUNCH_CHECK();                                                                \
-        return RAPTR_OK;                                                                     \
-      }
-#define RAPTR_M64V(VV)                                                                       \
-  RAPTR_M64(VV, 16, 2) RAPTR_M64(VV, 16, 4) RAPTR_M64(VV, 16, 8) RAPTR_M64(VV, 32, 4)        \
-  RAPTR_M64(VV, 32, 8)
-      RAPTR_M64V(kF32) RAPTR_M64V(kF16) RAPTR_M64V(kBF16) RAPTR_M64V(kU16)
-#undef RAPTR_M64V
-#undef RAPTR_M64
-      return fail(RAPTR_ERR_INVALID_ARG, "matrix-core forward shape unsupported");
-    }
-    const bool GP = R->grp_pipe != 0;
-#define RAPTR_G64(VV, G_, UU, PP)                                                            \
-    if (R->vt == VV && GG == G_ && GU == UU && GP == PP) {                                   \
-      hipLaunchKernelGGL((k_fwdg64<VV, G_, UU, PP>), dim3(gg), dim3(256), 0, st, R->ngf, m,   \
+    const int FBA = R->fa / GG;
+#define RAPTR_M64(VV, G_, UU, FB_)                                                           \
+    if (R->vt == VV && GG == G_ && GU == MF<G_>::K * UU && FBA == FB_) {                     \
+      hipLaunchKernelGGL((k_fwdm64<VV, G_, UU, FB_>), dim3(gg), dim3(256), 0, st, R->ngf, m,  \
                          R->gfp, R->gfcol, (const typename Val<VV>::T*)R->gfval, R->rscale,  \
                          x, y, e);                                                           \
       RAPTR_LAUNCH_CHECK();                                                                  \
       return RAPTR_OK;                                                                       \
     }
-#define RAPTR_G64V(VV)                                                                       \
-  RAPTR_G64(VV, 4, 8, true) RAPTR_G64(VV, 4, 16, false) RAPTR_G64(VV, 8, 4, true)            \
-  RAPTR_G64(VV, 8, 8, false) RAPTR_G64(VV, 16, 2, true) RAPTR_G64(VV, 16, 4, false)
-    RAPTR_G64V(kF32) RAPTR_G64V(kF16) RAPTR_G64V(kBF16) RAPTR_G64V(kU16)
-#undef RAPTR_G64V
-#undef RAPTR_G64
-    return fail(RAPTR_ERR_INVALID_ARG, "grouped forward shape G %d unroll %d unsupported", GG,
-                GU);
+#define RAPTR_M64V(VV)                                                                       \
+  RAPTR_M64(VV, 16, 2, 4) RAPTR_M64(VV, 16, 4, 4) RAPTR_M64(VV, 16, 8, 4)                     \
+  RAPTR_M64(VV, 16, 2, 2) RAPTR_M64(VV, 16, 4, 2) RAPTR_M64(VV, 16, 8, 2)                     \
+  RAPTR_M64(VV, 16, 2, 1) RAPTR_M64(VV, 16, 4, 1) RAPTR_M64(VV, 16, 8, 1)                     \
+  RAPTR_M64(VV, 32, 4, 2) RAPTR_M64(VV, 32, 8, 2)
+    RAPTR_M64V(kF32) RAPTR_M64V(kF16) RAPTR_M64V(kBF16) RAPTR_M64V(kU16)
+#undef RAPTR_M64V
+#undef RAPTR_M64
+    return fail(RAPTR_ERR_INVALID_ARG, "matrix-core forward shape unsupported");
   }
   if (F == 64 && R->bcast64) {
 #define RAPTR_F64(VV)                                                                        \
@@ -2325,45 +1886,26 @@ int launch_bwd_any(const raptr_recon* R, int F, int blk, int nbl, const float* x
   const int64_t* ptr = R->bptr + int64_t(blk) * (n + 1);
   int first = blk == 0, last = blk + nbl == R->nblk;
   if (F == 64 && R->grp64) {
+    // kind 2: one launch per group of row blocks into the shared image sums
+    // (kind 3 runs launch_bwd_xcd instead); all 64 frames
+    if (R->fa != 64)
+      return fail(RAPTR_ERR_INVALID_ARG, "passes over fewer than 64 frames need kind 3");
     const unsigned gg = unsigned((R->ngc + 3) / 4);
     if (nb_out) *nb_out = int(gg);
     const int64_t* gp = R->gbp + int64_t(blk) * R->ngc;
-    const int GU = R->grp_unroll, GG = R->grp_g;
-    if (R->grp_pipe >= 2) {
-      // backward lists are aligned to 8 entries: steps of 8 (U = 2)
-#define RAPTR_MB64(VV, UU)                                                                   \
-      if (R->vt == VV && GG == 16 && UU == 2) {                                              \
-        hipLaunchKernelGGL((k_bwdm64<VV, UU>), dim3(gg), dim3(256), 0, st, int(R->ngc), gp,   \
-                           nbl, R->gbrow, (const typename Val<VV>::T*)R->gbval,              \
-                           (const int*)R->gcols, R->e, R->c, first, last, x, R->s, R->xx,    \
-                           R->partials);                                                     \
-        RAPTR_LAUNCH_CHECK();                                                                \
-        return RAPTR_OK;                                                                     \
-      }
-#define RAPTR_MB64V(VV) RAPTR_MB64(VV, 2)
-      RAPTR_MB64V(kF32) RAPTR_MB64V(kF16) RAPTR_MB64V(kBF16) RAPTR_MB64V(kU16)
-#undef RAPTR_MB64V
-#undef RAPTR_MB64
-      return fail(RAPTR_ERR_INVALID_ARG, "matrix-core backward shape unsupported");
-    }
-    const bool GP = R->grp_pipe != 0;
-#define RAPTR_GB64(VV, G_, UU, PP)                                                           \
-    if (R->vt == VV && GG == G_ && GU == UU && GP == PP) {                                   \
-      hipLaunchKernelGGL((k_bwdg64<VV, G_, UU, PP>), dim3(gg), dim3(256), 0, st,              \
-                         int(R->ngc), gp, nbl, R->gbrow,                                     \
-                         (const typename Val<VV>::T*)R->gbval, (const int*)R->gcols, R->e,   \
-                         R->c, first, last, x, R->s, R->xx, R->partials);                    \
+    // backward lists are aligned to 8 entries: steps of 8 (U = 2)
+#define RAPTR_MB64(VV)                                                                       \
+    if (R->vt == VV && R->grp_g == 16) {                                                     \
+      hipLaunchKernelGGL((k_bwdm64<VV, 2>), dim3(gg), dim3(256), 0, st, int(R->ngc), gp, nbl, \
+                         R->gbrow, (const typename Val<VV>::T*)R->gbval,                     \
+                         (const int*)R->gcols, R->e, R->c, first, last, x, R->s, R->xx,      \
+                         R->partials);                                                       \
       RAPTR_LAUNCH_CHECK();                                                                  \
       return RAPTR_OK;                                                                       \
     }
-#define RAPTR_GB64V(VV)                                                                      \
-  RAPTR_GB64(VV, 4, 8, true) RAPTR_GB64(VV, 4, 16, false) RAPTR_GB64(VV, 8, 4, true)         \
-  RAPTR_GB64(VV, 8, 8, false) RAPTR_GB64(VV, 16, 2, true) RAPTR_GB64(VV, 16, 4, false)
-    RAPTR_GB64V(kF32) RAPTR_GB64V(kF16) RAPTR_GB64V(kBF16) RAPTR_GB64V(kU16)
-#undef RAPTR_GB64V
-#undef RAPTR_GB64
-    return fail(RAPTR_ERR_INVALID_ARG, "grouped backward shape G %d unroll %d unsupported",
-                GG, GU);
+    RAPTR_MB64(kF32) RAPTR_MB64(kF16) RAPTR_MB64(kBF16) RAPTR_MB64(kU16)
+#undef RAPTR_MB64
+    return fail(RAPTR_ERR_INVALID_ARG, "matrix-core backward shape unsupported");
   }
   if (F == 64 && R->bcast64) {
 #define RAPTR_B64(VV)                                                                        \
@@ -2400,40 +1942,6 @@ int launch_bwd_any(const raptr_recon* R, int F, int blk, int nbl, const float* x
   return fail(RAPTR_ERR_INVALID_ARG, "backward configuration unsupported");
 }
 
-// LDS capacity for the staged backward (one e slice + column sums)
-constexpr int kLdsRowsMax = 24576;
-
-int launch_bwd_flat(const raptr_recon* R, const float* x, hipStream_t st, int* nb_out) {
-  const int n = R->n;
-  const int G = (n + 255) / 256;
-  if (nb_out) *nb_out = G;
-  if (!R->pidx) return fail(RAPTR_ERR_INVALID_ARG, "plan has no packed staged indices");
-  const size_t lds = size_t(R->blk_rows + 256) * sizeof(float);
-  const int U = R->bwd_unroll;
-#define RAPTR_BF(VV, UU)                                                                    \
-  if (R->vt == VV && U == UU) {                                                             \
-    static bool attr = false;                                                               \
-    if (!attr) {                                                                            \
-      if (hipFuncSetAttribute((const void*)k_bwd_flat<VV, UU>,                              \
-                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) !=    \
-          hipSuccess)                                                                       \
-        (void)hipGetLastError();                                                            \
-      attr = true;                                                                          \
-    }                                                                                       \
-    hipLaunchKernelGGL((k_bwd_flat<VV, UU>), dim3(G), dim3(256), lds, st, n, R->nblk,        \
-                       R->blk_rows, int64_t(R->m), (const int64_t*)R->bptr,                 \
-                       (const uint32_t*)R->pidx, (const typename Val<VV>::T*)R->bval,       \
-                       (const float*)R->e, x, (const float*)R->s, R->xx, R->partials);      \
-    RAPTR_LAUNCH_CHECK();                                                                   \
-    return RAPTR_OK;                                                                        \
-  }
-#define RAPTR_BFV(VV) RAPTR_BF(VV, 1) RAPTR_BF(VV, 2) RAPTR_BF(VV, 4)
-  RAPTR_BFV(kF32) RAPTR_BFV(kF16) RAPTR_BFV(kBF16) RAPTR_BFV(kU16)
-#undef RAPTR_BFV
-#undef RAPTR_BF
-  return fail(RAPTR_ERR_INVALID_ARG, "flat staged backward shape unsupported");
-}
-
 // single frame, per-XCD row ranges (backward mode 3)
 int launch_bwd1_xcd(const raptr_recon* R, const float* x, hipStream_t st, int* nb_out) {
   if (!R->c8 || !R->bptr) return fail(RAPTR_ERR_INVALID_ARG, "plan has no per-XCD partial sums");
@@ -2443,72 +1951,43 @@ int launch_bwd1_xcd(const raptr_recon* R, const float* x, hipStream_t st, int* n
   const unsigned gg = unsigned(8 * ((int64_t(n) + (256 / L) - 1) / (256 / L)));
   const int U = R->bwd_unroll;
   const bool r16 = R->brow16 && R->bwd_r16;
+  // windows of xcd_bpl row blocks per XCD and launch (0: all in one)
+  const int per_xcd = (R->nblk + 7) / 8;
+  const int bpl = R->xcd_bpl > 0 ? R->xcd_bpl : per_xcd;
+  for (int boff = 0; boff < per_xcd; boff += bpl) {
+    const int accum = boff > 0;
+    bool launched = false;
 #define RAPTR_B1X(VV, UU, LL)                                                               \
   if (R->vt == VV && U == UU && L == LL) {                                                  \
     if (r16)                                                                                \
       hipLaunchKernelGGL((k_bwd1x<VV, UU, true, LL>), dim3(gg), dim3(256), 0, st, n,         \
-                         R->nblk, R->blk_rows, (const int64_t*)R->bptr,                     \
+                         R->nblk, R->blk_rows, boff, bpl, accum, (const int64_t*)R->bptr,   \
                          (const void*)R->brow16, (const typename Val<VV>::T*)R->bval,       \
                          (const float*)R->e, R->c8);                                        \
     else                                                                                    \
       hipLaunchKernelGGL((k_bwd1x<VV, UU, false, LL>), dim3(gg), dim3(256), 0, st, n,        \
-                         R->nblk, R->blk_rows, (const int64_t*)R->bptr,                     \
+                         R->nblk, R->blk_rows, boff, bpl, accum, (const int64_t*)R->bptr,   \
                          (const void*)R->brow, (const typename Val<VV>::T*)R->bval,         \
                          (const float*)R->e, R->c8);                                        \
-    RAPTR_LAUNCH_CHECK();                                                                   \
-  } else
+    launched = true;                                                                        \
+  }
 #define RAPTR_B1XV(VV)                                                                      \
   RAPTR_B1X(VV, 1, 16) RAPTR_B1X(VV, 2, 16) RAPTR_B1X(VV, 4, 16) RAPTR_B1X(VV, 1, 32)       \
   RAPTR_B1X(VV, 2, 32) RAPTR_B1X(VV, 4, 32) RAPTR_B1X(VV, 1, 64) RAPTR_B1X(VV, 2, 64)       \
   RAPTR_B1X(VV, 4, 64)
-  RAPTR_B1XV(kF32) RAPTR_B1XV(kF16) RAPTR_B1XV(kBF16) RAPTR_B1XV(kU16)
-  return fail(RAPTR_ERR_INVALID_ARG, "per-XCD backward unroll %d lanes %d unsupported", U, L);
+    RAPTR_B1XV(kF32) RAPTR_B1XV(kF16) RAPTR_B1XV(kBF16) RAPTR_B1XV(kU16)
 #undef RAPTR_B1XV
 #undef RAPTR_B1X
+    if (!launched)
+      return fail(RAPTR_ERR_INVALID_ARG, "per-XCD backward unroll %d lanes %d unsupported", U, L);
+    RAPTR_LAUNCH_CHECK();
+  }
   const unsigned nr = unsigned((int64_t(n) + 255) / 256);
   hipLaunchKernelGGL(k_bwd1_xreduce, dim3(nr), dim3(256), 0, st, n, (const float*)R->c8, x,
                      (const float*)R->s, R->xx, R->partials);
   RAPTR_LAUNCH_CHECK();
   if (nb_out) *nb_out = int(nr);
   return RAPTR_OK;
-}
-
-int launch_bwd_lds(const raptr_recon* R, const float* x, hipStream_t st, int* nb_out) {
-  if (R->bwd_lds == 3) return launch_bwd1_xcd(R, x, st, nb_out);
-  if (R->bwd_lds == 2) return launch_bwd_flat(R, x, st, nb_out);
-  const int n = R->n;
-  int G = std::min(R->bwd_wgs, std::max(1, (n + 63) / 64));
-  const int cpw = (n + G - 1) / G;
-  G = (n + cpw - 1) / cpw;
-  if (nb_out) *nb_out = G;
-  size_t lds = size_t(R->blk_rows + cpw) * sizeof(float);
-  if (lds > size_t(160) * 1024)
-    return fail(RAPTR_ERR_INVALID_ARG, "staged backward needs %zu B of LDS (> 160 KiB)", lds);
-  const int L = R->bwd_lpc, U = R->bwd_unroll > 4 ? 4 : R->bwd_unroll;
-#define RAPTR_BL(VV, LL, UU)                                                                \
-  if (R->vt == VV && L == LL && U == UU) {                                                  \
-    static bool attr = false;                                                               \
-    if (!attr) {                                                                            \
-      if (hipFuncSetAttribute((const void*)k_bwd_lds<VV, LL, UU>,                           \
-                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) !=    \
-          hipSuccess)                                                                       \
-        (void)hipGetLastError();                                                            \
-      attr = true;                                                                          \
-    }                                                                                       \
-    hipLaunchKernelGGL((k_bwd_lds<VV, LL, UU>), dim3(G), dim3(256), lds, st, n, R->nblk,     \
-                       R->blk_rows, int64_t(R->m), cpw, (const int64_t*)R->bptr,            \
-                       (const int*)R->brow, (const typename Val<VV>::T*)R->bval,            \
-                       (const float*)R->e, x, (const float*)R->s, R->xx, R->partials);      \
-    RAPTR_LAUNCH_CHECK();                                                                   \
-    return RAPTR_OK;                                                                        \
-  }
-#define RAPTR_BLV(VV)                                                                       \
-  RAPTR_BL(VV, 8, 2) RAPTR_BL(VV, 8, 4) RAPTR_BL(VV, 16, 2) RAPTR_BL(VV, 16, 4)              \
-  RAPTR_BL(VV, 32, 2) RAPTR_BL(VV, 32, 4)
-  RAPTR_BLV(kF32) RAPTR_BLV(kF16) RAPTR_BLV(kBF16) RAPTR_BLV(kU16)
-#undef RAPTR_BLV
-#undef RAPTR_BL
-  return fail(RAPTR_ERR_INVALID_ARG, "staged backward shape unsupported");
 }
 
 // 64-frame backward with per-XCD row ranges and partial images, then the
@@ -2519,22 +1998,29 @@ int launch_bwd_xcd(const raptr_recon* R, const float* x, hipStream_t st, int* nb
   const unsigned gg = unsigned(8 * ((R->ngc + 3) / 4));
   const int per_xcd = (R->nblk + 7) / 8;  // most blocks of any XCD's range
   const int bpl = R->xcd_bpl > 0 ? R->xcd_bpl : per_xcd;
+  const int FBA = R->fa / R->grp_g;
   for (int boff = 0; boff < per_xcd; boff += bpl) {
     const int accum = boff > 0;
-#define RAPTR_X64(VV, G_)                                                                    \
-  if (R->vt == VV && R->grp_g == G_)                                                         \
-    hipLaunchKernelGGL((k_bwdm64x<VV, G_>), dim3(gg), dim3(256), 0, st, int(R->ngc),          \
+    bool launched = false;
+#define RAPTR_X64(VV, G_, FB_)                                                               \
+  if (R->vt == VV && R->grp_g == G_ && FBA == FB_) {                                         \
+    hipLaunchKernelGGL((k_bwdm64x<VV, G_, FB_>), dim3(gg), dim3(256), 0, st, int(R->ngc),     \
                        R->nblk, R->n, (const int64_t*)R->gbp, (const int*)R->gbrow,          \
                        (const typename Val<VV>::T*)R->gbval, (const int*)R->gcols,           \
-                       (const float*)R->e, R->c8, boff, bpl, accum);
-    RAPTR_X64(kF32, 16) RAPTR_X64(kF16, 16) RAPTR_X64(kBF16, 16) RAPTR_X64(kU16, 16)
-    RAPTR_X64(kF32, 32) RAPTR_X64(kF16, 32) RAPTR_X64(kBF16, 32) RAPTR_X64(kU16, 32)
+                       (const float*)R->e, R->c8, boff, bpl, accum);                         \
+    launched = true;                                                                         \
+  }
+#define RAPTR_X64V(VV)                                                                       \
+  RAPTR_X64(VV, 16, 4) RAPTR_X64(VV, 16, 2) RAPTR_X64(VV, 16, 1) RAPTR_X64(VV, 32, 2)
+    RAPTR_X64V(kF32) RAPTR_X64V(kF16) RAPTR_X64V(kBF16) RAPTR_X64V(kU16)
+#undef RAPTR_X64V
 #undef RAPTR_X64
+    if (!launched) return fail(RAPTR_ERR_INVALID_ARG, "per-XCD backward shape unsupported");
     RAPTR_LAUNCH_CHECK();
   }
   const unsigned nr = unsigned((int64_t(R->n) + 3) / 4);
-  hipLaunchKernelGGL(k_bwd_xreduce, dim3(nr), dim3(256), 0, st, R->n, (const float*)R->c8, x,
-                     (const float*)R->s, R->xx, R->partials);
+  hipLaunchKernelGGL(k_bwd_xreduce, dim3(nr), dim3(256), 0, st, R->n, R->fa,
+                     (const float*)R->c8, x, (const float*)R->s, R->xx, R->partials);
   RAPTR_LAUNCH_CHECK();
   if (nb_out) *nb_out = int(nr);
   return RAPTR_OK;
@@ -2542,12 +2028,9 @@ int launch_bwd_xcd(const raptr_recon* R, const float* x, hipStream_t st, int* nb
 
 int iteration(raptr_recon* R, int F, const float* y, float* x, hipStream_t st) {
   int nb = 0;
-  if (R->layout == RAPTR_LAYOUT_TILED) {
-    RAPTR_TRY(tiled_forward(R, x, y, R->e, st));
-    RAPTR_TRY(tiled_backward(R, x, R->e, st, &nb));
-  } else if (F == 1 && R->bwd_lds) {
+  if (F == 1 && R->bwd_lds == 3) {
     RAPTR_TRY(launch_fwd_any(R, F, x, y, R->e, st));
-    RAPTR_TRY(launch_bwd_lds(R, x, st, &nb));
+    RAPTR_TRY(launch_bwd1_xcd(R, x, st, &nb));
   } else if (F == 64 && R->grp64 && R->grp_pipe == 3) {
     RAPTR_TRY(launch_fwd_any(R, F, x, y, R->e, st));
     RAPTR_TRY(launch_bwd_xcd(R, x, st, &nb));
@@ -2556,11 +2039,14 @@ int iteration(raptr_recon* R, int F, const float* y, float* x, hipStream_t st) {
     for (int b = 0; b < R->nblk; b += R->bwd_bpl)
       RAPTR_TRY(launch_bwd_any(R, F, b, std::min(R->bwd_bpl, R->nblk - b), x, st, &nb));
   }
-  hipLaunchKernelGGL(k_norms, dim3(F), dim3(256), 0, st, (const double*)R->partials, nb, F,
+  // active frames: all F, or the plan's fa of a 64-frame grouped pass
+  const int FA = F == 64 && R->grp64 ? R->fa : F;
+  hipLaunchKernelGGL(k_norms, dim3(FA), dim3(256), 0, st, (const double*)R->partials, nb, F,
                      R->scale, R->norm);
   RAPTR_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_scale_frames, dim3(grid_for(int64_t(R->n) * F, 256, 8192)), dim3(256),
-                     0, st, int64_t(R->n), F, (const float*)R->xx, (const float*)R->scale, x);
+                     0, st, int64_t(R->n), F, FA, (const float*)R->xx,
+                     (const float*)R->scale, x);
   RAPTR_LAUNCH_CHECK();
   return RAPTR_OK;
 }
@@ -2590,6 +2076,12 @@ __global__ void k_frame_gather(const float* a, int64_t rows, int F, int slot, fl
   for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < rows;
        i += int64_t(gridDim.x) * blockDim.x)
     out[i] = a[i * F + slot];
+}
+
+__global__ void k_frame_copy(float* a, int64_t rows, int F, int src, int dst) {
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < rows;
+       i += int64_t(gridDim.x) * blockDim.x)
+    a[i * F + dst] = a[i * F + src];
 }
 
 // ---------------------------------------------------- plan row order
@@ -2666,7 +2158,8 @@ __global__ void k_rows_scatter(const float* __restrict__ src, float* __restrict_
 // run of consecutive rows covers a compact bundle of lines (T > 1).
 // kind 0: (angle, midpoint tile, midpoint in tile, slope); kind 1 (y-major):
 // (midpoint y, angle, midpoint z tile, midpoint z in tile, slope); kind 2:
-// as kind 0 with the slopes in 4 x 4 tiles (slope tile, slope in tile)
+// as kind 0 with the slopes in 4 x 4 tiles (slope tile, slope in tile);
+// kind 3: as kind 2 with angle / 4 in place of the angle and angle % 4 last
 __global__ void k_locality_keys(raptr_setup_t s, const int* __restrict__ cnl, int m, int T,
                                 int kind, uint64_t* __restrict__ keys,
                                 uint32_t* __restrict__ idx) {
@@ -2684,19 +2177,20 @@ __global__ void k_locality_keys(raptr_setup_t s, const int* __restrict__ cnl, in
       k = k * nzt + zm / T;
       k = k * T + zm % T;
     } else {
-      k = uint64_t(ida);
+      k = uint64_t(kind == 3 ? ida / 4 : ida);
       k = k * nzt + zm / T;
       k = k * nyt + ym / T;
       k = k * T + zm % T;
       k = k * T + ym % T;
     }
     const uint64_t dz = uint64_t(id1z - id0z + s.n0z - 1), dy = uint64_t(id1y - id0y + s.n0y - 1);
-    if (kind == 2) {
+    if (kind >= 2) {
       constexpr uint64_t S = 4;
       k = k * ((zs + S - 1) / S) + dz / S;
       k = k * ((ys + S - 1) / S) + dy / S;
       k = k * S + dz % S;
       k = k * S + dy % S;
+      if (kind == 3) k = k * 4 + uint64_t(ida % 4);
     } else {
       k = k * zs + dz;
       k = k * ys + dy;
@@ -2729,7 +2223,7 @@ int build_local_columns(raptr_recon* R, hipStream_t st) {
   const int64_t m = R->m;
   const int n = R->n;
   const int64_t ng = (m + kLocRows - 1) / kLocRows;
-  const size_t shm = (2 * size_t((n + 31) >> 5) + 256) * sizeof(uint32_t);
+  const size_t shm = (2 * size_t((std::min(n, kUnionWin) + 31) >> 5) + 257) * sizeof(uint32_t);
   int rc;
   int64_t* cnt = nullptr;
   if ((rc = ralloc(R, &R->lup, size_t(ng) + 1)) || (rc = ralloc(R, &cnt, size_t(ng))))
@@ -2804,7 +2298,7 @@ namespace raptr {
 namespace {
 // gdims: the columns' voxel grid (nx, ny, nz) for brick-shaped backward
 // groups, or null; group: members per group of the 64-frame grouped
-// layouts (4, 8, 16; 0 = 16)
+// layouts (16 or 32; 0 = 16)
 int create_plan(raptr_recon** out, int m, int n, int64_t nnz, const int64_t* d_rp,
                 const int* d_col, const float* d_val, const double* d_sens, int value_type,
                 int max_frames, int block_rows, int layout, const int* gdims, int group,
@@ -2816,8 +2310,8 @@ int create_plan(raptr_recon** out, int m, int n, int64_t nnz, const int64_t* d_r
     return fail(RAPTR_ERR_INVALID_ARG,
                 "a grouped-only plan needs max_frames 64, the blocked layout and a non-empty "
                 "matrix");
-  if (group != 4 && group != 8 && group != 16 && group != 32)
-    return fail(RAPTR_ERR_INVALID_ARG, "group must be 4, 8, 16 or 32");
+  if (group != 16 && group != 32)
+    return fail(RAPTR_ERR_INVALID_ARG, "group must be 16 or 32");
   if (gdims && int64_t(gdims[0]) * gdims[1] * gdims[2] != n)
     return fail(RAPTR_ERR_INVALID_ARG, "grid %d x %d x %d does not hold %d voxels", gdims[0],
                 gdims[1], gdims[2], n);
@@ -2831,13 +2325,10 @@ int create_plan(raptr_recon** out, int m, int n, int64_t nnz, const int64_t* d_r
     return fail(RAPTR_ERR_INVALID_ARG, "null buffer");
   if (nnz > int64_t(UINT32_MAX) - 16)
     return fail(RAPTR_ERR_INVALID_ARG, "nnz exceeds 2^32");
-  if (layout != RAPTR_LAYOUT_BLOCKED && layout != RAPTR_LAYOUT_TILED)
-    return fail(RAPTR_ERR_INVALID_ARG, "unknown layout %d", layout);
-  if (layout == RAPTR_LAYOUT_TILED && max_frames != 1)
-    return fail(RAPTR_ERR_INVALID_ARG, "the tiled layout is single-frame (max_frames = 1)");
+  if (layout != RAPTR_LAYOUT_BLOCKED)
+    return fail(RAPTR_ERR_INVALID_ARG, "unknown layout %d (only RAPTR_LAYOUT_BLOCKED)", layout);
   hipStream_t st = S(stream);
   raptr_recon* R = new raptr_recon();
-  R->layout = layout;
   R->m = m;
   R->n = n;
   R->nnz = nnz;
@@ -2852,15 +2343,11 @@ int create_plan(raptr_recon** out, int m, int n, int64_t nnz, const int64_t* d_r
   // transpose rows; the per-XCD backward keeps an eighth of the blocks in
   // each XCD's L2)
   if (block_rows <= 0)
-    block_rows = max_frames == 1 && layout == RAPTR_LAYOUT_BLOCKED
-                     ? 65536
-                     : std::max(1, (2 << 20) / (4 * max_frames));
+    block_rows = max_frames == 1 ? 65536 : std::max(1, (2 << 20) / (4 * max_frames));
   R->blk_rows = block_rows;
   R->nblk = std::max(1, int((int64_t(m) + block_rows - 1) / block_rows));
   int rc;
-  if (layout == RAPTR_LAYOUT_TILED) {
-    if ((rc = tiled_create(R, d_rp, d_col, d_val, st))) return bail(rc);
-  } else {
+  {
     // ---- forward: padded CSR (grouped-only plans: the row scales only)
     R->grouped_only = go ? 1 : 0;
     int64_t* plen = nullptr;
@@ -2925,18 +2412,17 @@ int create_plan(raptr_recon** out, int m, int n, int64_t nnz, const int64_t* d_r
                           st);                                                             \
     break;                                                                                 \
   }
-        RAPTR_RG(4) RAPTR_RG(8) RAPTR_RG(16) RAPTR_RG(32)
+        RAPTR_RG(16) RAPTR_RG(32)
 #undef RAPTR_RG
       }
       if (rc) return bail(rc);
-      // default shape: matrix cores with per-XCD row ranges for 16-member
-      // groups, else pipelined
-      R->grp_unroll = group == 4 ? 8 : (group == 8 ? 4 : (group == 16 ? 8 : 16));
-      R->grp_pipe = group >= 16 ? 3 : 1;
-      // kind 3's backward walks windows of about 65,536 rows per XCD and
-      // launch (the best measured at full size for 8,192- to 32,768-row
+      // default shape: matrix cores with per-XCD row ranges
+      R->grp_unroll = group == 16 ? 8 : 16;
+      R->grp_pipe = 3;
+      // kind 3's backward walks windows of about 131,072 rows per XCD and
+      // launch (the best measured at full size for 16,384- and 32,768-row
       // blocks)
-      R->xcd_bpl = block_rows > 0 && block_rows < 65536 ? 65536 / block_rows : 1;
+      R->xcd_bpl = block_rows > 0 && block_rows < 131072 ? 131072 / block_rows : 1;
     }
     // ---- backward: row-blocked transpose via a stable radix sort, every
     // (block, column) segment padded to a multiple of 4 entries
@@ -3019,13 +2505,6 @@ int create_plan(raptr_recon** out, int m, int n, int64_t nnz, const int64_t* d_r
         hipLaunchKernelGGL(k_bt_pad, dim3(grid_for(T, 256, 1 << 16)), dim3(256), 0, st,
                            (const int64_t*)uptr, (const int64_t*)R->bptr, T, n, block_rows,
                            R->brow, R->bval, value_type, max_frames == 1 ? kIlvBwd : 0);
-        // single frame with LDS-sized row blocks: packed indices for the
-        // flat staged backward
-        if (max_frames == 1 && block_rows <= kLdsRowsMax && block_rows % 4 == 0 &&
-            !(rc = ralloc(R, &R->pidx, size_t(h) + 4)))
-          hipLaunchKernelGGL(k_bt_pack, dim3(grid_for(T, 256, 1 << 16)), dim3(256), 0, st,
-                             (const int64_t*)R->bptr, T, n, block_rows, (const int*)R->brow,
-                             R->pidx);
         hipError_t le = hipGetLastError();
         if (le != hipSuccess) rc = fail(RAPTR_ERR_DEVICE, "transpose: %s", hipGetErrorString(le));
       }
@@ -3058,12 +2537,12 @@ int create_plan(raptr_recon** out, int m, int n, int64_t nnz, const int64_t* d_r
                           &R->gbval, &R->gbnnz, st);                                       \
     break;                                                                                 \
   }
-            RAPTR_CG(4) RAPTR_CG(8) RAPTR_CG(16) RAPTR_CG(32)
+            RAPTR_CG(16) RAPTR_CG(32)
 #undef RAPTR_CG
           }
         }
-        // per-XCD partial images of the kind-3 backward (groups of 16)
-        if (!rc && G >= 16) rc = ralloc(R, &R->c8, size_t(8) * n * 64);
+        // per-XCD partial images of the kind-3 backward
+        if (!rc) rc = ralloc(R, &R->c8, size_t(8) * n * 64);
         if (!rc) R->grp64 = 1;
       }
       (void)hipStreamSynchronize(st);
@@ -3093,7 +2572,7 @@ int create_plan(raptr_recon** out, int m, int n, int64_t nnz, const int64_t* d_r
   if (se != hipSuccess) return bail(fail(RAPTR_ERR_DEVICE, "create: %s", hipGetErrorString(se)));
   R->fwd_lanes = double(nnz) / std::max(m, 1) >= 192 ? 32 : 16;
   R->fwd_unroll = max_frames == 1 ? 1 : 2;
-  R->bwd_lds = R->pidx ? 2 : 0;
+  R->bwd_lds = 0;
   // single-frame blocked plans: per-XCD partial column sums (backward mode 3)
   if (max_frames == 1 && layout == RAPTR_LAYOUT_BLOCKED && R->bptr &&
       (rc = ralloc(R, &R->c8, size_t(8) * n)))
@@ -3135,10 +2614,8 @@ int create_plan(raptr_recon** out, int m, int n, int64_t nnz, const int64_t* d_r
       R->fwd_c16 = 1;
     }
   }
-  // ... and the column unions of the LDS-staged forward (grids of <= 258,048
-  // voxels: the union kernel's bitmap and prefix fit 64 KB of LDS)
-  if (max_frames == 1 && layout == RAPTR_LAYOUT_BLOCKED && R->fcol && n <= 8064 * 32 && m > 0 &&
-      R->pnnz > 0) {
+  // ... and the column unions of the LDS-staged forward
+  if (max_frames == 1 && layout == RAPTR_LAYOUT_BLOCKED && R->fcol && m > 0 && R->pnnz > 0) {
     if ((rc = build_local_columns(R, st))) return bail(rc);
     if (R->lcol) {
       R->fwd_kind = 2;
@@ -3285,20 +2762,23 @@ int raptr_locality_order_ex(const raptr_setup_t* setup, const int* d_channels, i
   if (!setup || m < 0 || tile < 1 || tile > 64 || (m > 0 && (!d_channels || !d_order)))
     return fail(RAPTR_ERR_INVALID_ARG, "bad locality order arguments");
   if (kind != RAPTR_ORDER_ANGLE_MAJOR && kind != RAPTR_ORDER_Y_MAJOR &&
-      kind != RAPTR_ORDER_SLOPE_TILES)
+      kind != RAPTR_ORDER_SLOPE_TILES && kind != RAPTR_ORDER_ANGLE_SLOPE_TILES)
     return fail(RAPTR_ERR_INVALID_ARG, "unknown locality order kind %d", kind);
   if (m == 0) return RAPTR_OK;
   const raptr_setup_t s = *setup;
   if (s.na < 1 || s.n0z < 1 || s.n0y < 1 || s.n1z < 1 || s.n1y < 1)
     return fail(RAPTR_ERR_INVALID_ARG, "bad setup");
   const int zs = s.n0z + s.n1z - 1, ys = s.n0y + s.n1y - 1;
-  // largest key + 1 (kind 1 orders the same factors differently; kind 2
-  // pads the slope ranges to whole 4 x 4 tiles)
-  const double slopes = kind == RAPTR_ORDER_SLOPE_TILES
-                            ? double((zs + 3) / 4 * 4) * double((ys + 3) / 4 * 4)
-                            : double(zs) * ys;
-  const double kmax = double(s.na) * ((zs + tile - 1) / tile) * ((ys + tile - 1) / tile) *
-                      tile * tile * slopes;
+  // largest key + 1 (kind 1 orders the same factors differently; kinds 2
+  // and 3 pad the slope ranges to whole 4 x 4 tiles; kind 3 pads the angles
+  // to a multiple of 4)
+  const bool stiles = kind == RAPTR_ORDER_SLOPE_TILES || kind == RAPTR_ORDER_ANGLE_SLOPE_TILES;
+  const double slopes =
+      stiles ? double((zs + 3) / 4 * 4) * double((ys + 3) / 4 * 4) : double(zs) * ys;
+  const double angles = kind == RAPTR_ORDER_ANGLE_SLOPE_TILES ? double((s.na + 3) / 4 * 4)
+                                                              : double(s.na);
+  const double kmax = angles * ((zs + tile - 1) / tile) * ((ys + tile - 1) / tile) * tile *
+                      tile * slopes;
   if (kmax >= 1.8e19) return fail(RAPTR_ERR_INVALID_ARG, "setup too large for 64-bit keys");
   int bits = 1;
   while (bits < 64 && double(uint64_t(1) << bits) < kmax) ++bits;
@@ -3379,22 +2859,14 @@ int raptr_recon_set_config(raptr_recon* R, int fwd_lanes, int fwd_unroll, int bw
 
 int raptr_recon_set_backward(raptr_recon* R, int staged, int lanes_per_column, int workgroups) {
   if (!R) return fail(RAPTR_ERR_INVALID_ARG, "null plan");
-  if (staged < 0 || staged > 3)
+  if (staged != 0 && staged != 3)
+    return fail(RAPTR_ERR_INVALID_ARG, "staged must be 0 (off) or 3 (per-XCD row ranges)");
+  if (staged == 3 && (R->fmax != 1 || !R->c8))
     return fail(RAPTR_ERR_INVALID_ARG,
-                "staged must be 0 (off), 1 (per column), 2 (flat) or 3 (per-XCD row ranges)");
-  if (staged == 3 && (R->layout != RAPTR_LAYOUT_BLOCKED || R->fmax != 1 || !R->c8))
-    return fail(RAPTR_ERR_INVALID_ARG,
-                "the per-XCD backward needs a single-frame plan of the blocked layout");
-  if ((staged == 1 || staged == 2) &&
-      (R->layout != RAPTR_LAYOUT_BLOCKED || R->fmax != 1 || R->blk_rows > kLdsRowsMax ||
-       R->blk_rows % 4 || !R->pidx))
-    return fail(RAPTR_ERR_INVALID_ARG,
-                "staged backward needs a single-frame plan with block_rows <= %d, "
-                "a multiple of 4", kLdsRowsMax);
+                "the per-XCD backward needs a single-frame plan with partial sums");
   if (lanes_per_column != 8 && lanes_per_column != 16 && lanes_per_column != 32 &&
-      !(staged == 3 && lanes_per_column == 64))
-    return fail(RAPTR_ERR_INVALID_ARG,
-                "lanes per column must be 8, 16 or 32 (or 64 for the per-XCD backward)");
+      lanes_per_column != 64)
+    return fail(RAPTR_ERR_INVALID_ARG, "lanes per column must be 8, 16, 32 or 64");
   if (workgroups < 1) return fail(RAPTR_ERR_INVALID_ARG, "workgroups must be positive");
   if (R->bwd_lds != staged || R->bwd_lpc != lanes_per_column || R->bwd_wgs != workgroups) {
     clear_graphs(R);
@@ -3421,9 +2893,8 @@ int raptr_recon_set_compact_rows(raptr_recon* R, int on) {
 
 int raptr_recon_set_forward_kind(raptr_recon* R, int kind) {
   if (!R) return fail(RAPTR_ERR_INVALID_ARG, "null plan");
-  if (kind < 0 || kind > 2) return fail(RAPTR_ERR_INVALID_ARG, "forward kind must be 0, 1 or 2");
-  if (kind == 1 && !R->fcol16)
-    return fail(RAPTR_ERR_INVALID_ARG, "the pipelined forward reads 16-bit columns (none here)");
+  if (kind != 0 && kind != 2)
+    return fail(RAPTR_ERR_INVALID_ARG, "forward kind must be 0 (rows) or 2 (LDS-staged x)");
   if (kind == 2 && !R->lcol)
     return fail(RAPTR_ERR_INVALID_ARG, "no LDS column unions on this plan");
   if (R->fwd_kind != kind) {
@@ -3437,7 +2908,7 @@ int raptr_recon_get_forward_kind(const raptr_recon* R, int* kind, int* available
                                  int64_t* union_entries) {
   if (!R) return fail(RAPTR_ERR_INVALID_ARG, "null plan");
   if (kind) *kind = R->fwd_kind;
-  if (available) *available = 1 | (R->fcol16 ? 2 : 0) | (R->lcol ? 4 : 0);
+  if (available) *available = 1 | (R->lcol ? 4 : 0);
   if (union_entries) *union_entries = R->lcol ? R->lnu : 0;
   return RAPTR_OK;
 }
@@ -3495,10 +2966,9 @@ int raptr_recon_set_grouped(raptr_recon* R, int on, int unroll, int kind) {
   if (on && !R->gbp) return fail(RAPTR_ERR_INVALID_ARG, "plan has no grouped layouts");
   if (R->gbp && !grp_shape_ok(R->grp_g, unroll, kind))
     return fail(RAPTR_ERR_INVALID_ARG,
-                "groups of %d take unroll %d (kind 1) or %d (kind 0)%s; got unroll %d kind %d",
-                R->grp_g, R->grp_g == 4 ? 8 : (R->grp_g == 8 ? 4 : 2),
-                R->grp_g == 4 ? 16 : (R->grp_g == 8 ? 8 : 4),
-                R->grp_g == 16 ? " or 8, 16 (kind 2)" : "", unroll, kind);
+                "groups of %d take unroll %s with kind %s; got unroll %d kind %d", R->grp_g,
+                R->grp_g == 16 ? "8, 16 or 32" : "8 or 16", R->grp_g == 16 ? "2 or 3" : "3",
+                unroll, kind);
   if (R->grp64 != (on ? 1 : 0) || R->grp_unroll != unroll || R->grp_pipe != kind) {
     clear_graphs(R);
   }
@@ -3520,37 +2990,6 @@ int raptr_recon_get_grouped(const raptr_recon* R, int* group, int* on, int* unro
   return RAPTR_OK;
 }
 
-int raptr_recon_set_tiled(raptr_recon* R, int fwd_lanes, int fwd_unroll, int bwd_lanes,
-                          int bwd_unroll) {
-  if (!R) return fail(RAPTR_ERR_INVALID_ARG, "null plan");
-  if (R->layout != RAPTR_LAYOUT_TILED) return fail(RAPTR_ERR_INVALID_ARG, "plan is not tiled");
-  auto lanes_ok = [](int l) { return l == 4 || l == 8 || l == 16; };
-  auto unroll_ok = [](int u) { return u == 1 || u == 2 || u == 4; };
-  if (!lanes_ok(fwd_lanes) || !lanes_ok(bwd_lanes) || !unroll_ok(fwd_unroll) ||
-      !unroll_ok(bwd_unroll))
-    return fail(RAPTR_ERR_INVALID_ARG, "tiled lanes must be 4, 8 or 16 and unroll 1, 2 or 4");
-  if (R->t_fwd_lanes != fwd_lanes || R->t_fwd_unroll != fwd_unroll ||
-      R->t_bwd_lanes != bwd_lanes || R->t_bwd_unroll != bwd_unroll) {
-    clear_graphs(R);
-  }
-  R->t_fwd_lanes = fwd_lanes;
-  R->t_fwd_unroll = fwd_unroll;
-  R->t_bwd_lanes = bwd_lanes;
-  R->t_bwd_unroll = bwd_unroll;
-  return RAPTR_OK;
-}
-
-int raptr_recon_get_tiled(const raptr_recon* R, int* ntile, int* tile_cols, int* fwd_lanes,
-                          int* fwd_unroll, int* bwd_lanes, int* bwd_unroll) {
-  if (!R) return fail(RAPTR_ERR_INVALID_ARG, "null plan");
-  if (ntile) *ntile = R->ntile;
-  if (tile_cols) *tile_cols = R->tile_cols;
-  if (fwd_lanes) *fwd_lanes = R->t_fwd_lanes;
-  if (fwd_unroll) *fwd_unroll = R->t_fwd_unroll;
-  if (bwd_lanes) *bwd_lanes = R->t_bwd_lanes;
-  if (bwd_unroll) *bwd_unroll = R->t_bwd_unroll;
-  return RAPTR_OK;
-}
 
 int raptr_recon_get_backward(const raptr_recon* R, int* staged, int* lanes_per_column,
                              int* workgroups) {
@@ -3561,16 +3000,41 @@ int raptr_recon_get_backward(const raptr_recon* R, int* staged, int* lanes_per_c
   return RAPTR_OK;
 }
 
+static int mlem_impl(raptr_recon* R, int nframes, const float* d_y, float* d_x, int niter,
+                     int use_graph, hipStream_t st);
+
 int raptr_recon_mlem(raptr_recon* R, int nframes, const float* d_y, float* d_x, int niter,
                      int use_graph, raptr_stream_t stream) {
   if (!R) return fail(RAPTR_ERR_INVALID_ARG, "null plan");
+  R->fa = 64;
+  return mlem_impl(R, nframes, d_y, d_x, niter, use_graph, S(stream));
+}
+
+int raptr_recon_mlem_active(raptr_recon* R, int nframes, int active, const float* d_y,
+                            float* d_x, int niter, int use_graph, raptr_stream_t stream) {
+  if (!R) return fail(RAPTR_ERR_INVALID_ARG, "null plan");
+  if (active == nframes) return raptr_recon_mlem(R, nframes, d_y, d_x, niter, use_graph, stream);
+  if (nframes != 64 || (active != 16 && active != 32))
+    return fail(RAPTR_ERR_INVALID_ARG,
+                "active frames must be 16, 32 or 64 of a 64-frame pass (got %d of %d)", active,
+                nframes);
+  if (!(R->fmax == 64 && R->grp64 && R->grp_pipe == 3 && R->grp_g == 16))
+    return fail(RAPTR_ERR_INVALID_ARG,
+                "passes over fewer than 64 frames need the grouped layouts of 16 on, kind 3");
+  R->fa = active;
+  const int rc = mlem_impl(R, nframes, d_y, d_x, niter, use_graph, S(stream));
+  R->fa = 64;
+  return rc;
+}
+
+static int mlem_impl(raptr_recon* R, int nframes, const float* d_y, float* d_x, int niter,
+                     int use_graph, hipStream_t st) {
   if (!valid_frames(nframes) || nframes > R->fmax)
     return fail(RAPTR_ERR_INVALID_ARG, "nframes %d not a power of two <= %d", nframes, R->fmax);
   if (niter < 0 || !d_y || !d_x) return fail(RAPTR_ERR_INVALID_ARG, "bad arguments");
   if (!frames_aligned(d_y, nframes) || !frames_aligned(d_x, nframes))
     return fail(RAPTR_ERR_INVALID_ARG, "y and x must be aligned to %d bytes for %d frames",
                 4 * (nframes >= 4 ? 4 : nframes), nframes);
-  hipStream_t st = S(stream);
   if (R->rperm && R->m > 0) {
     // counts into plan row order once per call (outside any captured graph)
     hipLaunchKernelGGL(k_rows_gather, dim3(grid_for(int64_t(R->m) * nframes, 256, 1 << 15)),
@@ -3583,7 +3047,7 @@ int raptr_recon_mlem(raptr_recon* R, int nframes, const float* d_y, float* d_x, 
     for (int it = 0; it < niter; ++it) RAPTR_TRY(iteration(R, nframes, d_y, d_x, st));
     return RAPTR_OK;
   }
-  const auto key = std::make_tuple(nframes, uintptr_t(d_y), uintptr_t(d_x));
+  const auto key = std::make_tuple(nframes, R->fa, uintptr_t(d_y), uintptr_t(d_x));
   auto it = R->graphs.find(key);
   hipGraphExec_t exec = nullptr;
   if (it == R->graphs.end()) {
@@ -3635,15 +3099,12 @@ int raptr_recon_forward(raptr_recon* R, int nframes, const float* d_x, const flo
     hipLaunchKernelGGL(k_rows_gather, dim3(g), dim3(256), 0, st, d_y, R->yp,
                        (const int*)R->rperm, int64_t(R->m), nframes);
     RAPTR_LAUNCH_CHECK();
-    RAPTR_TRY(R->layout == RAPTR_LAYOUT_TILED
-                  ? tiled_forward(R, d_x, R->yp, R->e, st)
-                  : launch_fwd_any(R, nframes, d_x, R->yp, R->e, st));
+    RAPTR_TRY(launch_fwd_any(R, nframes, d_x, R->yp, R->e, st));
     hipLaunchKernelGGL(k_rows_scatter, dim3(g), dim3(256), 0, st, (const float*)R->e, d_e,
                        (const int*)R->rperm, int64_t(R->m), nframes);
     RAPTR_LAUNCH_CHECK();
     return RAPTR_OK;
   }
-  if (R->layout == RAPTR_LAYOUT_TILED) return tiled_forward(R, d_x, d_y, d_e, st);
   return launch_fwd_any(R, nframes, d_x, d_y, d_e, st);
 }
 
@@ -3652,8 +3113,7 @@ int raptr_recon_backward(raptr_recon* R, int nframes, const float* d_x, raptr_st
     return fail(RAPTR_ERR_INVALID_ARG, "bad plan or frame count");
   if (!frames_aligned(d_x, nframes))
     return fail(RAPTR_ERR_INVALID_ARG, "x must be aligned to the frame vector");
-  if (R->layout == RAPTR_LAYOUT_TILED) return tiled_backward(R, d_x, R->e, S(stream), nullptr);
-  if (nframes == 1 && R->bwd_lds) return launch_bwd_lds(R, d_x, S(stream), nullptr);
+  if (nframes == 1 && R->bwd_lds == 3) return launch_bwd1_xcd(R, d_x, S(stream), nullptr);
   if (nframes == 64 && R->grp64 && R->grp_pipe == 3) return launch_bwd_xcd(R, d_x, S(stream), nullptr);
   for (int b = 0; b < R->nblk; b += R->bwd_bpl)
     RAPTR_TRY(launch_bwd_any(R, nframes, b, std::min(R->bwd_bpl, R->nblk - b), d_x, S(stream),
@@ -3692,6 +3152,18 @@ int raptr_frame_gather(const float* d_a, int64_t rows, int nframes, int slot, fl
   if (rows == 0) return RAPTR_OK;
   hipLaunchKernelGGL(k_frame_gather, dim3(grid_for(rows, 256, 8192)), dim3(256), 0, S(stream),
                      d_a, rows, nframes, slot, d_out);
+  RAPTR_LAUNCH_CHECK();
+  return RAPTR_OK;
+}
+
+int raptr_frame_copy(float* d_a, int64_t rows, int nframes, int src_slot, int dst_slot,
+                     raptr_stream_t stream) {
+  if (!d_a || rows < 0 || nframes < 1 || src_slot < 0 || src_slot >= nframes || dst_slot < 0 ||
+      dst_slot >= nframes)
+    return fail(RAPTR_ERR_INVALID_ARG, "bad frame copy arguments");
+  if (rows == 0 || src_slot == dst_slot) return RAPTR_OK;
+  hipLaunchKernelGGL(k_frame_copy, dim3(grid_for(rows, 256, 8192)), dim3(256), 0, S(stream),
+                     d_a, rows, nframes, src_slot, dst_slot);
   RAPTR_LAUNCH_CHECK();
   return RAPTR_OK;
 }

@@ -1,6 +1,5 @@
-// recon.hpp -- the reconstruction plan shared by its translation units
-// (recon.hip: plan, blocked layouts, batched kernels; recon_tiled.hip:
-// column-tiled single-frame layout).
+// recon.hpp -- the reconstruction plan (recon.hip: plan, row-blocked
+// layouts, single-frame and batched kernels).
 #pragma once
 
 #include <hip/hip_bf16.h>
@@ -95,6 +94,9 @@ __device__ __forceinline__ void load4c(const typename Val<VT>::T* __restrict__ v
 
 struct raptr_recon {
   int m = 0, n = 0, vt = raptr::kF32, fmax = 1;
+  // frames a 64-frame grouped pass computes (16, 32 or 64: the first fa of
+  // each 64-frame row); 64 except inside raptr_recon_mlem_active
+  int fa = 64;
   int64_t nnz = 0, pnnz = 0;
   int blk_rows = 0, nblk = 0;
   // forward (padded CSR)
@@ -118,32 +120,19 @@ struct raptr_recon {
   int fwd_lanes = 64;
   int fwd_unroll = 1;  // groups of 4 entries in flight per lane / entry slot
   int bwd_unroll = 2;
-  int bwd_lds = 0;     // single-frame backward through LDS: 0 off, 1 per column, 2 flat
-  uint32_t* pidx = nullptr;  // flat staged backward: (column in tile) << 24 | row in block
-  int bwd_lpc = 16;    // lanes per column of the LDS-staged backward
-  int bwd_wgs = 512;   // workgroups of the LDS-staged backward
+  int bwd_lds = 0;     // single-frame backward: 0 block by block, 3 per-XCD row ranges
+  int bwd_lpc = 16;    // lanes per column of the per-XCD backward (8 runs as 64)
+  int bwd_wgs = 512;   // kept for the get/set API (unused)
   int bwd_bpl = 1;     // row blocks per backward launch (blocked layout)
-  int xcd_bpl = 0;     // per-XCD 64-frame backward: row blocks each XCD walks
-                       // per launch (0: all of its blocks in one launch)
+  int xcd_bpl = 0;     // per-XCD backwards (64-frame kind 3, single-frame mode 3):
+                       // row blocks each XCD walks per launch (0: all in one)
   int bcast64 = 0;     // 64-frame kernels with in-row broadcast of values/indices
-  // column-tiled single-frame layout (RAPTR_LAYOUT_TILED): tile q holds
-  // columns [q*tile_cols, (q+1)*tile_cols) of every row, rows padded to
-  // multiples of 4 entries, 16-bit column offsets within the tile
   int layout = RAPTR_LAYOUT_BLOCKED;
-  int ntile = 0, tile_cols = 0, tile_wgs = 0;
-  int64_t tnnz = 0;            // padded entries over all tiles
-  int64_t* tptr = nullptr;     // ntile*m + 1 segment starts, (tile, row) order
-  uint16_t* tidx = nullptr;    // column offset within the tile
-  void* tval = nullptr;        // float or 16-bit
-  int* trow = nullptr;         // ntile*(tile_wgs+1) row bounds per workgroup
-  float* tpart = nullptr;      // ntile*m partial row sums of the forward
-  float* cpart = nullptr;      // ntile*tile_wgs*tile_cols partial column sums
-  int t_fwd_lanes = 8, t_fwd_unroll = 2, t_bwd_lanes = 8, t_bwd_unroll = 2;
   // grouped 64-frame layouts (max_frames = 64): forward groups of 4 plan
   // rows and backward groups of 4 consecutive columns per row block, each
   // an ascending union list with 4 values per union entry
   int grp64 = 0;       // use the grouped kernels for 64-frame passes
-  int grp_g = 16;      // members per group (4, 8, 16)
+  int grp_g = 16;      // members per group (16, 32)
   int* gcols = nullptr;  // ngc * grp_g member columns of the backward groups (-1: none)
   int grp_unroll = 8;  // union entries in flight per wave (4, 8, 16)
   int grp_pipe = 1;    // kernel kind: 0 plain, 1 pipelined scalar loads, 2 matrix cores,
@@ -155,7 +144,7 @@ struct raptr_recon {
   uint16_t* fcol16 = nullptr;  // forward columns' low 16 bits (grids of <= 4 x 65,536 voxels)
   uint64_t* fsplit = nullptr;  // per row: sorted positions where the high part steps
   int fwd_c16 = 0;             // single-frame forward reads fcol16
-  int fwd_kind = 0;            // 0 plain, 1 software-pipelined, 2 LDS-staged x
+  int fwd_kind = 0;            // 0 rows per lane group, 2 LDS-staged x
   // LDS-staged forward: per group of consecutive plan rows the sorted union
   // of its columns (lup: group offsets, lucol: columns) and every padded
   // entry's 16-bit rank in its group's union
@@ -185,7 +174,8 @@ struct raptr_recon {
     hipGraphExec_t exec;
     uint64_t used;
   };
-  std::map<std::tuple<int, uintptr_t, uintptr_t>, CachedGraph> graphs;
+  // keyed by (frames, active frames, y, x)
+  std::map<std::tuple<int, int, uintptr_t, uintptr_t>, CachedGraph> graphs;
   uint64_t graph_tick = 0;
 };
 
@@ -243,17 +233,5 @@ void rfree(raptr_recon* R, T*& p) {
 }
 
 inline int vbytes(int vt) { return vt == kF32 ? 4 : 2; }
-
-// column-tiled single-frame layout (recon_tiled.hip)
-int tiled_create(raptr_recon* R, const int64_t* d_rp, const int* d_col, const float* d_val,
-                 hipStream_t st);
-// forward + ratio into e (length m): e = y / (A x) per row, 0 where y or
-// A x is 0, folded with the row scale for 16-bit values
-int tiled_forward(const raptr_recon* R, const float* x, const float* y, float* e,
-                  hipStream_t st);
-// backward with update: xx = x * (A^T e) / (s + (s==0)) and FP64 partial
-// sums of xx (count in *nb) for the normalisation
-int tiled_backward(const raptr_recon* R, const float* x, const float* e, hipStream_t st,
-                   int* nb);
 
 }  // namespace raptr

@@ -1,7 +1,7 @@
 """Fixed-shape pass loop on the full-size workload, for profilers.
 
     python -m raptr_amd.passes [--frames F] [--value-type f32] [--block-rows R]
-        [--order y|slope|angle] [--xcd-blocks B] [--reps N]
+        [--order y|slope|angle|angle_slope] [--xcd-blocks B] [--active A] [--reps N]
 
 Builds the benchmark's full-size workload (13x13-pixel heads, 180 angles,
 52^3 grid at 2 mm, 30 rays, six-ellipsoid phantom, 2e7 Poisson events), the
@@ -20,7 +20,7 @@ from . import _capi as capi
 from . import geometry as G
 from . import synthetic, sysmat
 from .device import DeviceArray, Stream
-from .recon import Reconstructor
+from .recon import Reconstructor, default_order_kind
 
 
 def log(*a):
@@ -32,10 +32,12 @@ def main(argv=None):
     ap.add_argument("--frames", type=int, default=64)
     ap.add_argument("--value-type", default="f32", choices=("f32", "f16", "bf16", "u16"))
     ap.add_argument("--block-rows", type=int, default=0, help="0: the plan's default")
-    ap.add_argument("--order", default=None, choices=("y", "slope", "angle"),
-                    help="locality row order (default: y for one frame, slope for batches)")
+    ap.add_argument("--order", default=None, choices=("y", "slope", "angle", "angle_slope"),
+                    help="locality row order (default: the plan's, recon.default_order_kind)")
     ap.add_argument("--xcd-blocks", type=int, default=-1,
                     help="64 frames: row blocks per XCD and launch (-1: the plan's default)")
+    ap.add_argument("--active", type=int, default=64, choices=(16, 32, 64),
+                    help="64 frames: frames each pass computes (an MLEM iteration per rep)")
     ap.add_argument("--reps", type=int, default=3)
     a = ap.parse_args(argv)
     st_obj = Stream()
@@ -53,7 +55,7 @@ def main(argv=None):
     d_list = DeviceArray.from_numpy(meas)
     A = sysmat.build_csr(setup, grid, meas, nrays, d_list=d_list, stream=st, capacity="auto")
     sens = sysmat.sensitivity(setup, grid, nrays, stream=st)
-    kind = a.order or ("y" if F == 1 else "slope")
+    kind = a.order or default_order_kind(F)
     order = sysmat.locality_order(setup, d_list, stream=st, kind=kind)
     R = Reconstructor(A, sens, a.value_type, F, block_rows=a.block_rows, stream=st,
                       row_order=order, grid=grid if F == 64 else None, group=16)
@@ -65,8 +67,11 @@ def main(argv=None):
     dx = DeviceArray.from_numpy(np.full(n * F, 1.0 / n, np.float32))
     de = DeviceArray(m * F, np.float32)
     for r in range(a.reps):
-        capi.call("raptr_recon_forward", R.handle, F, dx.ptr, dy.ptr, de.ptr, st)
-        capi.call("raptr_recon_backward", R.handle, F, dx.ptr, st)
+        if a.active < 64:
+            R.run(dy, dx, 1, F, use_graph=False, stream=st, active=a.active)
+        else:
+            capi.call("raptr_recon_forward", R.handle, F, dx.ptr, dy.ptr, de.ptr, st)
+            capi.call("raptr_recon_backward", R.handle, F, dx.ptr, st)
         st_obj.sync()
         log(f"passes {r + 1}/{a.reps}")
     R.close()

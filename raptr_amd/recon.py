@@ -13,15 +13,17 @@ from . import _capi as capi
 from .device import DeviceArray
 
 VALUE_TYPES = {"f32": 0, "f16": 1, "bf16": 2, "u16": 3}
-LAYOUTS = {"blocked": 0, "tiled": 1}
+LAYOUTS = {"blocked": 0}
 
 
 def default_order_kind(max_frames):
     """Locality order of a plan computed from a setup: y-major for one frame
     (the plain backward's ratio gathers touch fewer lines: 2.72 -> 2.56 ms
-    at full size), angle-major with slope tiles for batches (the 64-frame
-    forward's image gathers stay in L2 more often: 10.5 -> 9.2 ms)."""
-    return "y" if max_frames == 1 else "slope"
+    at full size); for batches slope tiles over angles in groups of 4 (16
+    consecutive rows = 4 slopes x 4 angles of one midpoint: the 64-frame
+    forward's 16-row unions hold 19 % fewer entries than with the
+    angle-major slope tiles, 9.1 -> 7.9 ms at full size)."""
+    return "y" if max_frames == 1 else "angle_slope"
 
 
 def _row_order(row_order, A, stream, kind):
@@ -40,10 +42,7 @@ def _row_order(row_order, A, stream, kind):
 
 class Reconstructor:
     """layout: "blocked" (row-padded CSR + row-blocked transpose, any frame
-    count; the faster one), "tiled" (column tiles through LDS, one frame,
-    ~40% of the device memory) or "auto" (= blocked).  block_rows: row-block
-    height of the blocked layout; for the tiled layout a cap on the tile
-    width."""
+    count) or "auto" (= blocked).  block_rows: row-block height."""
 
     def __init__(self, A, sens, value_type="f32", max_frames=1, block_rows=0, stream=None,
                  layout="auto", row_order=None, grid=None, group=0, grouped_only=False,
@@ -54,7 +53,7 @@ class Reconstructor:
         ``A.channels`` / a (setup, channels) pair, from which the locality
         order (sysmat.locality_order) is computed.  Counts, images and the
         forward's ratios keep the CSR's row order at the interface.
-        grid (VoxelGrid) / group (4, 8, 16, 32; 0 = 16): the 64-frame grouped
+        grid (VoxelGrid) / group (16, 32; 0 = 16): the 64-frame grouped
         layouts' group size, with backward groups as bricks of the grid.
         grouped_only (max_frames 64): keep only the grouped layouts (64-frame
         passes only, ~40 % less device memory).  order_kind: the locality
@@ -102,21 +101,14 @@ class Reconstructor:
                         "grouped_unroll": gi["unroll"], "grouped_kind": gi["kind"],
                         "grouped_fwd_entries": gi["fwd_entries"],
                         "grouped_bwd_entries": gi["bwd_entries"]})
-        if self.layout == "tiled":
-            t = [C.c_int() for _ in range(6)]
-            capi.call("raptr_recon_get_tiled", self.handle, *[C.byref(v) for v in t])
-            out.update(dict(zip(("tiles", "tile_cols", "tiled_fwd_lanes", "tiled_fwd_unroll",
-                                 "tiled_bwd_lanes", "tiled_bwd_unroll"), [v.value for v in t])))
         return out
 
     def set_broadcast(self, on):
         capi.call("raptr_recon_set_broadcast", self.handle, 1 if on else 0)
 
-    # grouped kernel shapes per group size: (unroll, kind); kind 0 vector
-    # FMAs, 1 the same pipelined, 2 matrix cores, 3 matrix cores with per-XCD
-    # row ranges in the backward
-    GROUP_SHAPES = {4: ((8, 1), (16, 0)), 8: ((4, 1), (8, 0)),
-                    16: ((8, 2), (16, 2), (8, 3), (16, 3), (32, 3), (2, 1), (4, 0)),
+    # grouped kernel shapes per group size: (unroll, kind); kind 2 matrix
+    # cores, 3 matrix cores with per-XCD row ranges in the backward
+    GROUP_SHAPES = {16: ((8, 3), (16, 3), (32, 3), (8, 2), (16, 2)),
                     32: ((16, 3), (8, 3))}
 
     def grouped(self):
@@ -130,7 +122,7 @@ class Reconstructor:
     def set_grouped(self, on, unroll=None, kind=None):
         """unroll / kind None: the group size's first shape."""
         g = self.grouped()["group"]
-        shapes = self.GROUP_SHAPES.get(g, ((8, 1), (16, 0)))
+        shapes = self.GROUP_SHAPES.get(g, ((8, 3),))
         if unroll is None or kind is None:
             unroll, kind = [(u, k) for u, k in shapes
                             if (unroll is None or u == unroll) and (kind is None or k == kind)][0]
@@ -142,9 +134,9 @@ class Reconstructor:
         capi.call("raptr_recon_set_compact_rows", self.handle, 1 if on else 0)
 
     def set_forward_kind(self, kind):
-        """Single-frame forward kernel: 0 one entry per lane, 1 the same
-        software-pipelined over 16-bit columns, 2 x staged in LDS over the
-        column union of each group of 64 plan rows (all bit-identical)."""
+        """Single-frame forward kernel: 0 one entry per lane, 2 x staged in
+        LDS over the column union of each group of 64 plan rows (both
+        bit-identical)."""
         capi.call("raptr_recon_set_forward_kind", self.handle, int(kind))
 
     def forward_kind(self):
@@ -166,26 +158,27 @@ class Reconstructor:
     def set_blocks_per_launch(self, blocks):
         capi.call("raptr_recon_set_blocks_per_launch", self.handle, int(blocks))
 
-    def set_tiled(self, fwd_lanes, fwd_unroll, bwd_lanes, bwd_unroll):
-        capi.call("raptr_recon_set_tiled", self.handle, int(fwd_lanes), int(fwd_unroll),
-                  int(bwd_lanes), int(bwd_unroll))
-
     def set_config(self, fwd_lanes, fwd_unroll, bwd_unroll):
         capi.call("raptr_recon_set_config", self.handle, int(fwd_lanes), int(fwd_unroll),
                   int(bwd_unroll))
 
     def set_backward(self, staged, lanes_per_column=16, workgroups=512):
-        """Single-frame backward: staged 1 (LDS slices, per column), 2 (LDS
-        slices, flat segmented scan; True means 2), 3 (per-XCD row ranges,
-        any block size), or block by block through L2 (0 / False)."""
-        mode = (2 if staged else 0) if isinstance(staged, bool) else int(staged)
+        """Single-frame backward: 3 (per-XCD row ranges; True means 3) or
+        block by block through L2 (0 / False)."""
+        mode = (3 if staged else 0) if isinstance(staged, bool) else int(staged)
         capi.call("raptr_recon_set_backward", self.handle, mode, int(lanes_per_column),
                   int(workgroups))
 
-    def run(self, y, x, niter, nframes=1, use_graph=True, stream=None):
-        """In-place MLEM on device arrays y ([m][F]) and x ([n][F])."""
-        capi.call("raptr_recon_mlem", self.handle, int(nframes), y.ptr, x.ptr, int(niter),
-                  1 if use_graph else 0, stream)
+    def run(self, y, x, niter, nframes=1, use_graph=True, stream=None, active=None):
+        """In-place MLEM on device arrays y ([m][F]) and x ([n][F]).
+        active (64-frame grouped plans: 16 or 32): iterate only the first
+        `active` frames of each 64-frame row (raptr_recon_mlem_active)."""
+        if active is None or active == nframes:
+            capi.call("raptr_recon_mlem", self.handle, int(nframes), y.ptr, x.ptr, int(niter),
+                      1 if use_graph else 0, stream)
+        else:
+            capi.call("raptr_recon_mlem_active", self.handle, int(nframes), int(active), y.ptr,
+                      x.ptr, int(niter), 1 if use_graph else 0, stream)
 
     def mlem(self, y, x0, niter, use_graph=True, stream=None):
         """Host convenience: y [m] or [m, F]; x0 [n] or [n, F]; returns x."""
@@ -204,7 +197,7 @@ class Reconstructor:
         """Time the kernel shapes on this plan with HIP events and keep the
         fastest: forward shapes first (lanes per row, unroll) with the
         current backward, then backward shapes (unroll; for one frame also
-        the LDS-staged variant's lanes per column and workgroup count).
+        the per-XCD pass's lanes per column).
         Returns (config dict, ms per iteration)."""
         from .device import Event
         e0, e1 = Event(), Event()
@@ -219,28 +212,6 @@ class Reconstructor:
             return e0.elapsed_ms(e1) / reps
 
         inf = self.info()
-        if self.layout == "tiled":
-            shapes = [(l, u) for l in (4, 8, 16) for u in (1, 2, 4)]
-            fl, fu, bl, bu_ = (inf["tiled_fwd_lanes"], inf["tiled_fwd_unroll"],
-                               inf["tiled_bwd_lanes"], inf["tiled_bwd_unroll"])
-            best = None
-            for l, u in shapes:
-                self.set_tiled(l, u, bl, bu_)
-                ms = timed()
-                if best is None or ms < best[1]:
-                    best = ((l, u), ms)
-            fl, fu = best[0]
-            best = None
-            for l, u in shapes:
-                self.set_tiled(fl, fu, l, u)
-                ms = timed()
-                if best is None or ms < best[1]:
-                    best = ((l, u), ms)
-            bl, bu_ = best[0]
-            self.set_tiled(fl, fu, bl, bu_)
-            capi.call("raptr_memcpy_d2d", x.ptr, x_save.ptr, x.nbytes, stream)
-            return ({"layout": "tiled", "fwd_lanes": fl, "fwd_unroll": fu, "bwd_lanes": bl,
-                     "bwd_unroll": bu_}, best[1])
         bu = inf["bwd_unroll"]
         grp = nframes == 64 and inf.get("group", 0) > 0
         if self.grouped_only:
@@ -272,31 +243,32 @@ class Reconstructor:
         bshapes = [(0, 16, 512, u, bpl) for u in (1, 2, 4) for bpl in (1, 4, 16)
                    if bpl == 1 or bpl < inf["row_blocks"]]
         staged_ok = nframes == 1 and self.max_frames == 1
-        xcd_ok = staged_ok
         if staged_ok:
-            try:  # the plan's row blocks may be too large for LDS slices
-                self.set_backward(True)
+            try:  # per-XCD row ranges need the plan's partial column sums
+                self.set_backward(3, 32)
             except capi.RaptrError:
                 staged_ok = False
         if staged_ok:
-            bshapes += [(1, lpc, wgs, u, 1) for lpc in (8, 16, 32) for wgs in (256, 512, 1024)
-                        for u in (2, 4)]
-            bshapes += [(2, 16, 512, u, 1) for u in (1, 2, 4)]
-        if xcd_ok:  # per-XCD row ranges (any block size)
+            # per-XCD row ranges, in one launch or in windows of 1-4 row
+            # blocks per XCD and launch
+            per_xcd = -(-inf["row_blocks"] // 8)
             bshapes += [(3, lpc, 512, u, 1) for lpc in (16, 32, 64) for u in (1, 2, 4)]
-        staged_ok = staged_ok or xcd_ok
+            xbs = [0] + [w for w in (1, 2, 4) if w < per_xcd]
         best = None
-        for st, lpc, wgs, u, bpl in bshapes:
+        for (st, lpc, wgs, u, bpl), xb in [(b, w) for b in bshapes
+                                           for w in (xbs if b[0] == 3 else [0])]:
             if staged_ok:
                 self.set_backward(st, lpc, wgs)
+                self.set_xcd_blocks(xb)
             self.set_config(fl, fu, u)
             self.set_blocks_per_launch(bpl)
             ms = timed()
             if best is None or ms < best[1]:
-                best = ((st, lpc, wgs, u, bpl), ms)
-        st, lpc, wgs, u, bpl = best[0]
+                best = ((st, lpc, wgs, u, bpl, xb), ms)
+        st, lpc, wgs, u, bpl, xb1 = best[0]
         if staged_ok:
             self.set_backward(st, lpc, wgs)
+            self.set_xcd_blocks(xb1)
         self.set_config(fl, fu, u)
         self.set_blocks_per_launch(bpl)
         bc = 0
@@ -309,7 +281,7 @@ class Reconstructor:
             else:
                 self.set_broadcast(False)
         ms_best = timed()
-        g_on, g_u, g_p, xb = 0, 0, 0, 0
+        g_on, g_u, g_p, xb = 0, 0, 0, xb1
         if grp:
             best_g = self._tune_grouped(inf, timed)
             if best_g[1] < ms_best:
@@ -328,10 +300,10 @@ class Reconstructor:
         return cfg, ms_best
 
     # rows each XCD walks per launch of kind 3's backward: windows of about
-    # 65,536 plan rows (16 MB of 64-frame ratios) measured best at full size
-    # with 8,192-, 16,384- and 32,768-row blocks; tried with half and twice
-    # that window and in one launch
-    XCD_WINDOW_ROWS = 65536
+    # 131,072 plan rows (32 MB of 64-frame ratios) measured best at full size
+    # with 16,384- and 32,768-row blocks and 4 x 1 x 4 voxel bricks; tried
+    # with half and twice that window and in one launch
+    XCD_WINDOW_ROWS = 131072
 
     def _xcd_blocks_options(self, inf):
         rows = max(1, -(-self.m // max(1, inf["row_blocks"])))

@@ -15,7 +15,12 @@ the same arithmetic whatever the other slots hold, so a request gives the
 same image alone or in a mixed batch.  A chunk never runs past the fewest
 remaining iterations of the lane's requests, so each request gets exactly
 its iteration count.  Requests join free slots at chunk boundaries
-(continuous batching) and leave when done.
+(continuous batching) and leave when done.  A 64-slot lane keeps its busy
+slots in the lowest frames (a request moves down when a lower slot frees
+and that lets the lane shrink) and runs each chunk over the first 16, 32
+or 64 frames only, the fewest that hold every busy slot
+(raptr_recon_mlem_active): a pass costs about in proportion to the frames
+it gathers, so a lightly loaded lane iterates faster.
 
 The cache admits a lane only if its estimated device bytes fit the budget.
 It evicts idle lanes (no request in a slot or queued) least recently used
@@ -40,7 +45,7 @@ from .recon import Reconstructor
 # device bytes per matrix entry of a plan: forward value + column, transposed
 # value + row (FP32), and the build's CSR while the plan is made from it
 _PLAN_BYTES_PER_NNZ = {"f32": 16, "u16": 12, "f16": 12, "bf16": 12}
-# 64-slot lanes keep only the grouped layouts (16 rows / 2x2x4 voxel bricks
+# 64-slot lanes keep only the grouped layouts (16 rows / 4x1x4 voxel bricks
 # per union entry; ~0.16 and ~0.26 union entries per matrix entry, each a
 # 4-byte key and 16 values)
 _GROUPED_BYTES_PER_NNZ = {"f32": 30, "u16": 16, "f16": 16, "bf16": 16}
@@ -154,7 +159,7 @@ class Lane:
                                   stream=st)
         if frames == GROUPED_FRAMES:
             # 64 slots: rows in locality order, grouped layouts only (16 rows
-            # / 2x2x4 voxel bricks on the matrix cores)
+            # / 4x1x4 voxel bricks on the matrix cores)
             self.plan = Reconstructor(A, sens, g.value_type, frames, stream=st,
                                       row_order=g.setup, grid=g.grid, group=16,
                                       grouped_only=True)
@@ -175,6 +180,9 @@ class Lane:
         self.stream.sync()
         self.build_s = time.perf_counter() - t
         self.bytes = self.plan.info()["device_bytes"] + 4 * frames * (self.m + self.n)
+        gi = self.plan.grouped() if frames == GROUPED_FRAMES else {}
+        # passes over 16 / 32 of the 64 frames: grouped layouts of 16, kind 3
+        self.partial = bool(gi) and gi["on"] == 1 and gi["group"] == 16 and gi["kind"] == 3
 
     # ---------------------------------------------------------- slots
     def free_slots(self):
@@ -182,6 +190,39 @@ class Lane:
 
     def busy(self):
         return any(r is not None for r in self.slots) or bool(self.queue)
+
+    def active(self):
+        """Frames the next chunk computes: the fewest of 16, 32, 64 holding
+        every busy slot (all frames on lanes without partial passes)."""
+        if not self.partial:
+            return self.frames
+        hi = max((s + 1 for s, r in enumerate(self.slots) if r is not None), default=0)
+        return 16 if hi <= 16 else (32 if hi <= 32 else 64)
+
+    def compact(self):
+        """Move requests from the highest busy slots into free lower ones
+        until the busy slots fit the fewest active frames their count
+        allows (a column copy of y and x; each request's frame computes the
+        same whichever slot holds it)."""
+        if not self.partial:
+            return 0
+        busy = [s for s, r in enumerate(self.slots) if r is not None]
+        target = 16 if len(busy) <= 16 else (32 if len(busy) <= 32 else 64)
+        high = [s for s in reversed(busy) if s >= target]
+        free = [s for s in range(target) if self.slots[s] is None]
+        for src, dst in zip(high, free):
+            self._move(src, dst)
+            self.slots[dst], self.slots[src] = self.slots[src], None
+            self.remaining[dst], self.remaining[src] = self.remaining[src], 0
+            self.admitted[dst] = self.admitted[src]
+        return min(len(high), len(free))
+
+    def _move(self, src, dst):
+        """Columns src of y and x to dst; src cleared."""
+        st = self.stream.handle
+        for arr, rows in ((self.y, self.m), (self.x, self.n)):
+            capi.call("raptr_frame_copy", arr.ptr, rows, self.frames, src, dst, st)
+            capi.call("raptr_frame_fill", arr.ptr, rows, self.frames, src, 0.0, st)
 
     def load(self, slot, req, now):
         st = self.stream.handle
@@ -216,7 +257,11 @@ class Lane:
         return min(min(rem), max_chunk) if rem else 0
 
     def launch(self, k):
-        self.plan.run(self.y, self.x, k, self.frames, use_graph=True, stream=self.stream.handle)
+        """k iterations over the active frames; returns their count."""
+        a = self.active()
+        self.plan.run(self.y, self.x, k, self.frames, use_graph=True, stream=self.stream.handle,
+                      active=a)
+        return a
 
     def retire(self, k, now):
         """After a chunk of k iterations: results of the slots that are done."""
@@ -298,6 +343,9 @@ class ServeStats:
     # HIP events recorded on the lane's stream around the chunk's launch
     # (times relative to the first chunk's start)
     trace: list = field(default_factory=list)
+    # chunks per active frame count (16 / 32 / 64-slot passes; F otherwise)
+    active_frames: dict = field(default_factory=dict)
+    moves: int = 0
 
     def summary(self):
         ok = [r for r in self.results if r.error is None]
@@ -310,7 +358,9 @@ class ServeStats:
                 "mean_ms": float(lat.mean()) * 1e3 if len(lat) else None,
                 "requests_per_s": len(ok) / self.wall_s if self.wall_s else None,
                 "frame_iterations_per_s": frames_it / self.wall_s if self.wall_s else None,
-                "chunks": self.chunks, "wall_s": self.wall_s}
+                "chunks": self.chunks, "wall_s": self.wall_s,
+                "chunks_by_active_frames": dict(sorted(self.active_frames.items())),
+                "slot_moves": self.moves}
 
 
 class Server:
@@ -354,12 +404,14 @@ class Server:
             active = []
             for lane in list(self.cache.lanes.values()):
                 self._admit_queued(lane, max(now(), 0.0))
+                stats.moves += lane.compact()
                 k = lane.chunk(self.max_chunk)
                 if k > 0:
                     if self.trace:
                         e0, e1 = Event(), Event()
                         e0.record(lane.stream.handle)
-                    lane.launch(k)
+                    a = lane.launch(k)
+                    stats.active_frames[a] = stats.active_frames.get(a, 0) + 1
                     if self.trace:
                         e1.record(lane.stream.handle)
                         events.append((lane_no.setdefault(id(lane), len(lane_no)), e0, e1))

@@ -169,14 +169,16 @@ def build_csr(setup, grid, cnl_list, nrays, seed=1, raygen=capi.RAYGEN_RAND, str
 LOCALITY_TILE = 8
 
 
-ORDERS = {"angle": 0, "y": 1, "slope": 2}
+ORDERS = {"angle": 0, "y": 1, "slope": 2, "angle_slope": 3}
 
 
 def locality_order(setup, channels, stream=None, tile=LOCALITY_TILE, kind="angle"):
     """Plan row order that puts lines through the same voxels next to each
     other (raptr_locality_order_ex): positions into ``channels`` sorted by
-    angle, midpoint (in tile x tile tiles) and slope (kind "angle"), or by
-    the midpoint's y first (kind "y").  Returns a device int32 array."""
+    angle, midpoint (in tile x tile tiles) and slope (kind "angle"), by
+    the midpoint's y first (kind "y"), with the slopes in 4 x 4 tiles (kind
+    "slope"), or that over angles in groups of 4, the angle within its group
+    last (kind "angle_slope").  Returns a device int32 array."""
     if isinstance(channels, DeviceArray):
         d_ch, m = channels, channels.count
     else:

@@ -43,59 +43,24 @@ def _rel(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
-@pytest.mark.parametrize("layout,block_rows", [("blocked", 0), ("blocked", 1777), ("tiled", 0),
-                                               ("tiled", 5000)])
-def test_recon_f32_vs_fp64_oracle(gpu, oracle, small_system, layout, block_rows):
-    """Plain FP32 MLEM (both layouts; several row blocks, several column
-    tiles) against the FP64 host recurrence."""
+@pytest.mark.parametrize("block_rows", [0, 1777])
+def test_recon_f32_vs_fp64_oracle(gpu, oracle, small_system, block_rows):
+    """Plain FP32 MLEM (one and several row blocks) against the FP64 host
+    recurrence."""
     from raptr_amd.device import Stream
     from raptr_amd.recon import Reconstructor
     S = small_system
     n = S["grid"].nvox
     x64 = _oracle_mlem(oracle, S, S["y"], 5)
-    R = Reconstructor(S["A"], S["sens"], "f32", 1, block_rows=block_rows, layout=layout)
-    if block_rows and layout == "blocked":
+    R = Reconstructor(S["A"], S["sens"], "f32", 1, block_rows=block_rows)
+    if block_rows:
         assert R.info()["row_blocks"] > 5
-    if block_rows and layout == "tiled":
-        assert R.info()["tiles"] == 7
     x0 = np.full(n, 1.0 / n, np.float32)
     st = Stream()
     for graph in (False, True):
         x = R.mlem(S["y"], x0, 5, use_graph=graph, stream=st.handle if graph else None)
         assert _rel(x, x64) <= 1e-4
         assert abs(float(x.astype(np.float64).sum()) - 1.0) < 1e-5
-
-
-@pytest.mark.parametrize("lpc,wgs,unroll", [(8, 512, 2), (16, 64, 4), (32, 1024, 4)])
-def test_recon_staged_backward(gpu, oracle, small_system, lpc, wgs, unroll):
-    """The LDS-staged single-frame backward (column ranges per workgroup,
-    ratio slices through LDS) against the FP64 oracle and against the
-    block-by-block backward (different summation order: 1e-6)."""
-    from raptr_amd.recon import Reconstructor
-    S = small_system
-    n = S["grid"].nvox
-    x0 = np.full(n, 1.0 / n, np.float32)
-    x64 = _oracle_mlem(oracle, S, S["y"], 5)
-    R = Reconstructor(S["A"], S["sens"], "f32", 1, block_rows=800, layout="blocked")
-    # the single-frame default is the per-XCD backward (mode 3)
-    assert R.info()["bwd_staged"] == 3 and R.info()["row_blocks"] > 8
-    R.set_config(16, 1, unroll)
-    R.set_backward(1, lpc, wgs)
-    a = R.mlem(S["y"], x0, 5)
-    R.set_backward(False)
-    b = R.mlem(S["y"], x0, 5)
-    assert _rel(a, x64) <= 1e-4
-    assert _rel(a, b) <= 1e-6
-    # flat staged form (segmented scans, fixed order: replays are identical)
-    R.set_config(16, 1, min(unroll, 4))
-    R.set_backward(2)
-    c1 = R.mlem(S["y"], x0, 5)
-    c2 = R.mlem(S["y"], x0, 5)
-    np.testing.assert_array_equal(c1, c2)
-    assert _rel(c1, x64) <= 1e-4
-    assert _rel(c1, b) <= 1e-6
-    with pytest.raises(Exception):
-        R.set_backward(True, 12, 512)
 
 
 @pytest.mark.parametrize("block_rows", [0, 800, 5000])
@@ -132,6 +97,17 @@ def test_recon_backward_per_xcd_row_ranges(gpu, oracle, small_system, block_rows
             a3 = R.mlem(S["y"], x0, 5, use_graph=False)
             R.set_compact_rows(True)
             np.testing.assert_array_equal(a3, a1)
+    # in windows of 1 and 3 row blocks per XCD and launch: the same images
+    # to FP32 rounding, replays bit for bit
+    R.set_config(16, 1, 2)
+    R.set_backward(3, 32)
+    for xb in (1, 3):
+        R.set_xcd_blocks(xb)
+        w1 = R.mlem(S["y"], x0, 5, use_graph=False)
+        w2 = R.mlem(S["y"], x0, 5, use_graph=True, stream=st.handle)
+        np.testing.assert_array_equal(w2, w1)
+        assert _rel(w1, b) <= 1e-6
+    R.set_xcd_blocks(0)
     # 16-bit rows exist for blocks of <= 65,536 rows (the single-frame
     # default); 16-bit forward columns for <= 262,144 voxels (32^3 here)
     assert R.info()["compact_rows"] == ((1 if block_rows <= 65536 else 0) | 2)
@@ -139,20 +115,24 @@ def test_recon_backward_per_xcd_row_ranges(gpu, oracle, small_system, block_rows
         Reconstructor(S["A"], S["sens"], "f32", 2).set_backward(3)
 
 
-def test_recon_staged_backward_needs_small_slices(gpu, small_system):
+def test_recon_backward_modes(gpu, small_system):
+    """Fewer than 8 row blocks: the block-by-block backward is the default;
+    the per-XCD pass is still allowed there.  Only modes 0 and 3 exist, the
+    per-XCD pass is single-frame only, and lanes per column are 8-64."""
     from raptr_amd import RaptrError
     from raptr_amd.recon import Reconstructor
     S = small_system
-    R = Reconstructor(S["A"], S["sens"], "f32", 1, block_rows=30000, layout="blocked")
+    R = Reconstructor(S["A"], S["sens"], "f32", 1, block_rows=30000)
     assert R.info()["row_blocks"] < 8
     assert R.info()["bwd_staged"] == 0  # per-XCD row ranges need >= 8 blocks
-    with pytest.raises(RaptrError):
-        R.set_backward(True)
+    R.set_backward(True)
+    assert R.info()["bwd_staged"] == 3
+    for bad in ((1, 16), (2, 16), (4, 16), (3, 12)):
+        with pytest.raises(RaptrError):
+            R.set_backward(*bad)
     Rb = Reconstructor(S["A"], S["sens"], "f32", 2)
     with pytest.raises(RaptrError):
         Rb.set_backward(True)
-    with pytest.raises(RaptrError):
-        Reconstructor(S["A"], S["sens"], "f32", 1, layout="tiled").set_backward(True)
 
 
 def test_recon_row_block_cut_is_immaterial(gpu, small_system):
@@ -171,11 +151,6 @@ def test_recon_row_block_cut_is_immaterial(gpu, small_system):
     for bpl in (3, 64):
         Rb.set_blocks_per_launch(bpl)
         assert _rel(Rb.mlem(S["y"], x0, 5), a) <= 1e-6
-    # the column-tiled layout (one and several tiles) gives the same images
-    for cap in (0, 3000):
-        c = Reconstructor(S["A"], S["sens"], "f32", 1, block_rows=cap, layout="tiled").mlem(
-            S["y"], x0, 5)
-        assert _rel(c, a) <= 1e-6
 
 
 @pytest.mark.parametrize("F", [2, 8, 16, 64])
@@ -220,9 +195,8 @@ def test_recon_batched_frames_match_single(gpu, oracle, small_system, F):
     assert np.all(np.isfinite(X))
 
 
-@pytest.mark.parametrize("layout", ["blocked", "tiled"])
 @pytest.mark.parametrize("vt,tol", [("u16", 1e-4), ("f16", 4e-4), ("bf16", 5e-3)])
-def test_recon_mixed_precision_error(gpu, oracle, small_system, vt, tol, layout):
+def test_recon_mixed_precision_error(gpu, oracle, small_system, vt, tol):
     """16-bit matrix values with per-row FP32 scale, FP32 accumulation:
     relative L2 against the FP64 oracle after 5 iterations.  Stated bounds:
     UNORM16 1e-4 (measured ~1e-5), FP16 4e-4 (measured 2.0e-4), BF16 5e-3
@@ -231,7 +205,7 @@ def test_recon_mixed_precision_error(gpu, oracle, small_system, vt, tol, layout)
     S = small_system
     n = S["grid"].nvox
     x64 = _oracle_mlem(oracle, S, S["y"], 5)
-    R = Reconstructor(S["A"], S["sens"], vt, 1, layout=layout)
+    R = Reconstructor(S["A"], S["sens"], vt, 1)
     x = R.mlem(S["y"], np.full(n, 1.0 / n, np.float32), 5)
     rel = _rel(x, x64)
     mx = float(np.max(np.abs(x - x64)) / np.max(np.abs(x64)))
@@ -239,16 +213,14 @@ def test_recon_mixed_precision_error(gpu, oracle, small_system, vt, tol, layout)
     assert rel <= tol
 
 
-@pytest.mark.parametrize("layout", ["blocked", "tiled"])
-def test_recon_graph_replays_match_eager(gpu, small_system, layout):
+def test_recon_graph_replays_match_eager(gpu, small_system):
     """Graph replays (with eager work and syncs between them) equal the eager
-    result every time: bit for bit for the blocked layout, within 1e-6 for
-    the tiled one (its backward adds with LDS atomics, in no fixed order)."""
+    result bit for bit every time."""
     from raptr_amd.device import DeviceArray, Stream
     from raptr_amd.recon import Reconstructor
     S = small_system
     n = S["grid"].nvox
-    R = Reconstructor(S["A"], S["sens"], "f32", 1, layout=layout)
+    R = Reconstructor(S["A"], S["sens"], "f32", 1)
     x0 = np.full(n, 1.0 / n, np.float32)
     eager = R.mlem(S["y"], x0, 4, use_graph=False)
     st = Stream()
@@ -258,10 +230,7 @@ def test_recon_graph_replays_match_eager(gpu, small_system, layout):
         dx.copy_from(x0, stream=st.handle)
         R.run(dy, dx, 4, 1, use_graph=True, stream=st.handle)
         st.sync()
-        if layout == "blocked":
-            np.testing.assert_array_equal(dx.numpy(), eager)
-        else:
-            assert _rel(dx.numpy(), eager) <= 1e-6
+        np.testing.assert_array_equal(dx.numpy(), eager)
         R.mlem(S["y"], x0, 1, use_graph=False)  # eager work between replays
 
 
@@ -300,8 +269,8 @@ def test_recon_graph_cache_exact_keys_and_bound(gpu, small_system):
     assert count() == 0
 
 
-@pytest.mark.parametrize("layout,cap", [("blocked", 0), ("tiled", 0), ("tiled", 3000)])
-def test_recon_forward_ratio(gpu, small_system, layout, cap):
+@pytest.mark.parametrize("cap", [0, 3000])
+def test_recon_forward_ratio(gpu, small_system, cap):
     """The forward pass alone: e = y / (A x) against an FP64 host product."""
     from raptr_amd import _capi as capi
     from raptr_amd.device import DeviceArray
@@ -312,7 +281,7 @@ def test_recon_forward_ratio(gpu, small_system, layout, cap):
     x = np.random.default_rng(3).random(n).astype(np.float32)
     yt = np.array([np.dot(val[rp[i]:rp[i + 1]].astype(np.float64), x[col[rp[i]:rp[i + 1]]])
                    for i in range(A.m)])
-    R = Reconstructor(A, S["sens"], "f32", 1, block_rows=cap, layout=layout)
+    R = Reconstructor(A, S["sens"], "f32", 1, block_rows=cap)
     dx, dy = DeviceArray.from_numpy(x), DeviceArray.from_numpy(S["y"].astype(np.float32))
     de = DeviceArray(A.m, np.float32)
     capi.call("raptr_recon_forward", R.handle, 1, dx.ptr, dy.ptr, de.ptr, None)
@@ -325,39 +294,47 @@ def test_recon_rejects_bad_arguments(gpu, small_system):
     S = small_system
     with pytest.raises(RaptrError):
         Reconstructor(S["A"], S["sens"], "f32", 3)
-    with pytest.raises(RaptrError):
+    with pytest.raises(KeyError):
         Reconstructor(S["A"], S["sens"], "f32", 2, layout="tiled")
     R = Reconstructor(S["A"], S["sens"], "f32", 2)
     with pytest.raises(RaptrError):
         R.mlem(np.zeros((S["A"].m, 4), np.float32), np.zeros((S["grid"].nvox, 4), np.float32), 1)
 
 
-def _locality_keys(setup, ch, T):
+def _locality_order(setup, ch, T, kind):
+    """The locality orders as a stable lexicographic sort of independently
+    computed key columns (most significant first)."""
     ida, z0, y0, z1, y1 = (np.asarray(v, np.int64) for v in setup.split(ch))
-    zs, ys = setup.n0z + setup.n1z - 1, setup.n0y + setup.n1y - 1
     zm, ym = z0 + z1, y0 + y1
-    k = ida
-    for v, r in ((zm // T, -(-zs // T)), (ym // T, -(-ys // T)), (zm % T, T), (ym % T, T),
-                 (z1 - z0 + setup.n0z - 1, zs), (y1 - y0 + setup.n0y - 1, ys)):
-        k = k * r + v
-    return k
+    dz, dy = z1 - z0, y1 - y0
+    if kind == "y":
+        cols = [ym, ida, zm // T, zm % T, dz, dy]
+    elif kind == "angle":
+        cols = [ida, zm // T, ym // T, zm % T, ym % T, dz, dy]
+    else:
+        # slopes in 4 x 4 tiles counted from the smallest slope
+        sz, sy = dz + setup.n0z - 1, dy + setup.n0y - 1
+        head = [ida // 4] if kind == "angle_slope" else [ida]
+        tail = [ida % 4] if kind == "angle_slope" else []
+        cols = head + [zm // T, ym // T, zm % T, ym % T, sz // 4, sy // 4, sz % 4,
+                       sy % 4] + tail
+    return np.lexsort(cols[::-1])
 
 
-@pytest.mark.parametrize("tile", [1, 3, 8])
-def test_locality_order_is_a_stable_sort(gpu, small_system, tile):
-    """raptr_locality_order: a permutation of the rows, sorted by (angle,
-    midpoint tile, midpoint, slope) keys computed independently here, ties
-    in list order."""
+@pytest.mark.parametrize("kind,tile", [("angle", 1), ("angle", 3), ("angle", 8), ("y", 8),
+                                       ("slope", 8), ("angle_slope", 8), ("angle_slope", 3)])
+def test_locality_order_is_a_stable_sort(gpu, small_system, kind, tile):
+    """raptr_locality_order_ex: a permutation of the rows, sorted by the
+    kind's keys computed independently here, ties in list order."""
     from raptr_amd import sysmat
     S = small_system
-    order = sysmat.locality_order(S["setup"], S["meas"], tile=tile).numpy()
+    order = sysmat.locality_order(S["setup"], S["meas"], tile=tile, kind=kind).numpy()
     assert np.array_equal(np.sort(order), np.arange(len(S["meas"])))
-    want = np.argsort(_locality_keys(S["setup"], S["meas"], tile), kind="stable")
-    np.testing.assert_array_equal(order, want)
+    np.testing.assert_array_equal(order, _locality_order(S["setup"], S["meas"], tile, kind))
 
 
-@pytest.mark.parametrize("layout,F", [("blocked", 1), ("blocked", 4), ("tiled", 1)])
-def test_recon_row_ordered_plan(gpu, oracle, small_system, layout, F):
+@pytest.mark.parametrize("F", [1, 4])
+def test_recon_row_ordered_plan(gpu, oracle, small_system, F):
     """A plan in locality row order: the forward's ratios come back in the
     caller's row order bit for bit (no row's sum changes), and MLEM matches
     the caller-ordered plan to FP32 rounding and the FP64 oracle to 1e-4."""
@@ -366,8 +343,8 @@ def test_recon_row_ordered_plan(gpu, oracle, small_system, layout, F):
     from raptr_amd.recon import Reconstructor
     S = small_system
     A, n, m = S["A"], S["grid"].nvox, S["A"].m
-    R0 = Reconstructor(A, S["sens"], "f32", F, layout=layout)
-    R1 = Reconstructor(A, S["sens"], "f32", F, layout=layout, row_order=S["setup"])
+    R0 = Reconstructor(A, S["sens"], "f32", F)
+    R1 = Reconstructor(A, S["sens"], "f32", F, row_order=S["setup"])
     assert R1.row_ordered and not R0.row_ordered
     rng = np.random.default_rng(5)
     x = rng.random((n, F)).astype(np.float32)
@@ -401,8 +378,7 @@ def test_recon_row_order_must_be_a_permutation(gpu, small_system):
 
 
 @pytest.mark.parametrize("vt,tol", [("f32", 1e-6), ("f16", 1e-5), ("bf16", 1e-5), ("u16", 1e-4)])
-@pytest.mark.parametrize("group,bricks", [(4, False), (8, True), (16, True), (16, False),
-                                          (32, True)])
+@pytest.mark.parametrize("group,bricks", [(16, True), (16, False), (32, True)])
 def test_recon_grouped64(gpu, oracle, small_system, vt, tol, group, bricks):
     """64-frame grouped layouts (G rows / G voxels per union list; voxel
     groups as bricks of the grid or runs of column ids) against the
@@ -467,17 +443,62 @@ def test_recon_grouped64_rejects_bad_shapes(gpu, small_system):
     from raptr_amd import RaptrError
     from raptr_amd.recon import Reconstructor
     S = small_system
-    with pytest.raises(RaptrError, match="group"):
-        Reconstructor(S["A"], S["sens"], "f32", 64, group=6)
-    R = Reconstructor(S["A"], S["sens"], "f32", 64, grid=S["grid"], group=8)
+    for g in (6, 4, 8):  # groups of 4 and 8 ran the removed scalar-register kernels
+        with pytest.raises(RaptrError, match="group"):
+            Reconstructor(S["A"], S["sens"], "f32", 64, group=g)
+    R = Reconstructor(S["A"], S["sens"], "f32", 64, grid=S["grid"], group=32)
     with pytest.raises(RaptrError, match="unroll"):
-        R.set_grouped(True, 2, 1)
-    with pytest.raises(RaptrError, match="unroll"):
-        R.set_grouped(True, 8, 2)  # matrix cores need groups of 16
+        R.set_grouped(True, 8, 2)  # kind 2 needs groups of 16
+    R16 = Reconstructor(S["A"], S["sens"], "f32", 64, grid=S["grid"], group=16)
+    for gu, kind in ((2, 1), (4, 0), (4, 3), (8, 4)):
+        with pytest.raises(RaptrError, match="unroll"):
+            R16.set_grouped(True, gu, kind)
     R1 = Reconstructor(S["A"], S["sens"], "f32", 16)
     assert R1.grouped()["group"] == 0
     with pytest.raises(RaptrError, match="no grouped"):
-        R1.set_grouped(True, 8, 1)
+        R1.set_grouped(True, 8, 3)
+
+
+@pytest.mark.parametrize("vt,active", [("f32", 16), ("f32", 32), ("u16", 16)])
+def test_recon_mlem_active_frames(gpu, small_system, vt, active):
+    """raptr_recon_mlem_active: a 64-frame grouped pass over the first 16 or
+    32 frames gives those frames the images of the full 64-frame pass bit
+    for bit (eager and graph replays) and leaves frames active .. 63 as they
+    were; kind 2 plans, other counts and non-grouped plans are refused."""
+    from raptr_amd import RaptrError, synthetic
+    from raptr_amd.device import DeviceArray, Stream
+    from raptr_amd.recon import Reconstructor
+    S = small_system
+    n = S["grid"].nvox
+    ybar = S["ybar"][S["meas"]]
+    Y = np.stack([synthetic.poisson_counts(ybar, tot, seed=90 + f)
+                  for f, tot in enumerate(np.geomspace(1e4, 2e6, 64))], axis=1).astype(np.float32)
+    rng = np.random.default_rng(1)
+    X0 = (rng.random((n, 64)) + 0.5).astype(np.float32) / n
+    R = Reconstructor(S["A"], S["sens"], vt, 64, block_rows=1500, row_order=S["setup"],
+                      grid=S["grid"], group=16, grouped_only=True)
+    assert R.grouped()["kind"] == 3
+    st = Stream()
+    dy = DeviceArray.from_numpy(Y.reshape(-1))
+    dx = DeviceArray.from_numpy(X0.reshape(-1))
+    R.run(dy, dx, 4, 64, use_graph=False, stream=st.handle)
+    full = dx.numpy(stream=st.handle).reshape(n, 64)
+    for graph in (False, True, True):
+        dx.copy_from(X0.reshape(-1), stream=st.handle)
+        R.run(dy, dx, 4, 64, use_graph=graph, stream=st.handle, active=active)
+        got = dx.numpy(stream=st.handle).reshape(n, 64)
+        np.testing.assert_array_equal(got[:, :active], full[:, :active])
+        np.testing.assert_array_equal(got[:, active:], X0[:, active:])
+    with pytest.raises(RaptrError):
+        R.run(dy, dx, 1, 64, use_graph=False, active=8)
+    R.set_grouped(True, 8, 2)
+    with pytest.raises(RaptrError, match="kind 3"):
+        R.run(dy, dx, 1, 64, use_graph=False, active=active)
+    R2 = Reconstructor(S["A"], S["sens"], vt, 16)
+    with pytest.raises(RaptrError):
+        R2.run(DeviceArray.from_numpy(Y[:, :16].copy().reshape(-1)),
+               DeviceArray.from_numpy(X0[:, :16].copy().reshape(-1)), 1, 16, use_graph=False,
+               active=8)
 
 
 @pytest.mark.parametrize("vt", ["f32", "f16"])
@@ -496,9 +517,14 @@ def test_recon_grouped_only_plan(gpu, small_system, vt):
     kw = dict(block_rows=1500, row_order=S["setup"], grid=S["grid"], group=16)
     full = Reconstructor(S["A"], S["sens"], vt, 64, **kw)
     only = Reconstructor(S["A"], S["sens"], vt, 64, grouped_only=True, **kw)
-    assert only.info()["device_bytes"] < 0.8 * full.info()["device_bytes"]
+    # the general layouts it drops: value + 4-byte index per padded CSR and
+    # padded transpose entry (the grouped lists and per-XCD images remain)
+    fi = full.info()
+    bv = 4 if vt == "f32" else 2
+    assert (fi["device_bytes"] - only.info()["device_bytes"] >=
+            (bv + 4) * (fi["padded_nnz"] + fi["transpose_nnz"]))
     assert only.info()["padded_nnz"] == 0 and only.info()["transpose_nnz"] == 0
-    for gu, kind in ((16, 3), (8, 2), (2, 1)):
+    for gu, kind in ((16, 3), (8, 2), (32, 3)):
         full.set_grouped(True, gu, kind)
         only.set_grouped(True, gu, kind)
         np.testing.assert_array_equal(only.mlem(Y, X0, 3), full.mlem(Y, X0, 3))
@@ -602,7 +628,7 @@ def test_recon_forward_compact_columns(gpu, vt):
     y = rng.random(m).astype(np.float32) + 0.5
     R = Reconstructor(A, sens, vt, 1, layout="blocked")
     assert R.info()["compact_rows"] & 2
-    assert R.forward_kind() == (0, [0, 1])
+    assert R.forward_kind() == (0, [0])
     dx, dy, de = DeviceArray.from_numpy(x), DeviceArray.from_numpy(y), DeviceArray(m, np.float32)
     yt = np.array([np.dot(val[rp[i]:rp[i + 1]].astype(np.float64), x[col[rp[i]:rp[i + 1]]])
                    for i in range(m)])
@@ -615,13 +641,11 @@ def test_recon_forward_compact_columns(gpu, vt):
     for lanes, unroll in [(16, 1), (32, 2), (64, 4), (16, 4), (64, 1), (32, 4)]:
         R.set_config(lanes, unroll, 1)
         out = []
-        for on, k in ((True, 0), (True, 1), (False, 0)):
+        for on in (True, False):
             R.set_compact_rows(on)
-            R.set_forward_kind(k)
             assert bool(R.info()["compact_rows"] & 2) == on
             out.append(fwd())
-        np.testing.assert_array_equal(out[0], out[2])
-        np.testing.assert_array_equal(out[1], out[2])
+        np.testing.assert_array_equal(out[0], out[1])
         if vt == "f32":
             np.testing.assert_allclose(out[0], ref, rtol=1e-5)
 
@@ -644,7 +668,7 @@ def test_recon_forward_compact_columns_absent(gpu, case):
     assert R.info()["compact_rows"] & 2 == 0
     assert R.forward_kind() == (0, [0])
     with pytest.raises(RaptrError):
-        R.set_forward_kind(1)
+        R.set_forward_kind(2)
     x = np.random.default_rng(2).random(n).astype(np.float32)
     y = np.ones(m, np.float32)
     dx, dy, de = DeviceArray.from_numpy(x), DeviceArray.from_numpy(y), DeviceArray(m, np.float32)
@@ -660,7 +684,7 @@ def test_recon_forward_lds_staged(gpu, vt):
     """LDS-staged forward (kind 2: x over each 64-row group's column union
     in LDS, 16-bit ranks) on a 258,000-voxel matrix whose groups sit in
     8,000-voxel windows across the grid: the same ratios, bit for bit, as
-    kinds 0 and 1 and the 32-bit columns for every lanes/unroll shape; FP32
+    kind 0 over 16-bit and 32-bit columns for every lanes/unroll shape; FP32
     against an FP64 host product."""
     from raptr_amd import _capi as capi
     from raptr_amd.device import DeviceArray
@@ -671,7 +695,7 @@ def test_recon_forward_lds_staged(gpu, vt):
     assert col.max() == n - 1 and (col >> 16).max() == 3
     A = _device_csr(rp, col, val, n)
     R = Reconstructor(A, DeviceArray.from_numpy(np.ones(n)), vt, 1, layout="blocked")
-    assert R.forward_kind() == (2, [0, 1, 2])
+    assert R.forward_kind() == (2, [0, 2])
     rng = np.random.default_rng(4)
     x = rng.random(n).astype(np.float32)
     y = rng.random(m).astype(np.float32) + 0.5
@@ -684,7 +708,7 @@ def test_recon_forward_lds_staged(gpu, vt):
     for lanes, unroll in [(16, 1), (32, 2), (64, 4), (16, 4), (64, 1)]:
         R.set_config(lanes, unroll, 1)
         out = []
-        for on, k in ((True, 2), (True, 0), (True, 1), (False, 0)):
+        for on, k in ((True, 2), (True, 0), (False, 0)):
             R.set_compact_rows(on)
             R.set_forward_kind(k)
             out.append(fwd())
@@ -697,18 +721,80 @@ def test_recon_forward_lds_staged(gpu, vt):
                                    rtol=1e-5)
 
 
+def _straddling_csr(n, m=1200, seed=13, window=6000, win=256000):
+    """Host CSR whose groups of 64 consecutive rows draw their columns from
+    one window of `window` voxels; every other group's window straddles a
+    multiple of `win` (the union builder's voxel windows), the last group's
+    ends at voxel n - 1.  Lengths 0..500, ascending."""
+    rng = np.random.default_rng(seed)
+    lens = rng.integers(0, 500, m)
+    cols = []
+    nb = (n - 1) // win
+    for r, L in enumerate(lens):
+        if r % 64 == 0:
+            g = r // 64
+            if r >= m - 64:
+                w0 = n - window
+            elif g % 2 == 0 and nb > 0:
+                w0 = win * int(rng.integers(1, nb + 1)) - window // 2
+            else:
+                w0 = int(rng.integers(0, n - window + 1))
+        c = rng.choice(window, L, replace=False)
+        if r == m - 1:  # the last row ends at voxel n - 1
+            c = np.union1d(c, [window - 1])
+            lens[r] = len(c)
+        cols.append(np.sort(c) + w0)
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    col = np.concatenate(cols).astype(np.int32)
+    val = rng.random(len(col)).astype(np.float32) + 0.01
+    return rp, col, val
+
+
+@pytest.mark.parametrize("n", [600001, 884736])
+def test_recon_forward_lds_staged_large_grid(gpu, n):
+    """Grids of more than one 256,000-voxel window of the union builder
+    (600,001 voxels; 96^3): the LDS-staged forward is built, groups whose
+    columns straddle a window edge included, and gives the ratios of the
+    32-bit per-entry forward bit for bit and of an FP64 host product."""
+    from raptr_amd import _capi as capi
+    from raptr_amd.device import DeviceArray
+    from raptr_amd.recon import Reconstructor
+    rp, col, val = _straddling_csr(n)
+    m = len(rp) - 1
+    assert col.max() == n - 1
+    R = Reconstructor(_device_csr(rp, col, val, n), DeviceArray.from_numpy(np.ones(n)), "f32", 1)
+    assert R.forward_kind() == (2, [0, 2])
+    assert R.info()["compact_rows"] & 2 == 0  # no 16-bit columns beyond 262,144 voxels
+    rng = np.random.default_rng(6)
+    x = rng.random(n).astype(np.float32)
+    y = rng.random(m).astype(np.float32) + 0.5
+    dx, dy, de = DeviceArray.from_numpy(x), DeviceArray.from_numpy(y), DeviceArray(m, np.float32)
+    for lanes, unroll in [(16, 4), (32, 2), (64, 1)]:
+        R.set_config(lanes, unroll, 1)
+        out = []
+        for k in (2, 0):
+            R.set_forward_kind(k)
+            capi.call("raptr_recon_forward", R.handle, 1, dx.ptr, dy.ptr, de.ptr, None)
+            out.append(de.numpy())
+        np.testing.assert_array_equal(out[0], out[1])
+    yt = np.array([np.dot(val[rp[i]:rp[i + 1]].astype(np.float64), x[col[rp[i]:rp[i + 1]]])
+                   for i in range(m)])
+    np.testing.assert_allclose(out[0], np.where(yt != 0, y / np.where(yt != 0, yt, 1), 0),
+                               rtol=1e-5)
+
+
 def test_recon_forward_lds_staged_absent(gpu):
     """No column unions when a 64-row group touches more than 16,384
-    voxels (rows of random columns over the grid) or the grid has more than
-    258,048 voxels; kind 2 is then refused."""
+    voxels (rows of random columns over the grid, one window of the union
+    builder or several); kind 2 is then refused."""
     from raptr_amd import RaptrError
     from raptr_amd.device import DeviceArray
     from raptr_amd.recon import Reconstructor
     for n, (rp, col, val) in ((200000, _wide_csr(200000, m=600)),
-                              (258049, _local_csr(258049, m=600))):
+                              (700000, _wide_csr(700000, m=600))):
         R = Reconstructor(_device_csr(rp, col, val, n), DeviceArray.from_numpy(np.ones(n)), "f32",
                           1, layout="blocked")
-        assert R.forward_kind() == (0, [0, 1])
+        assert R.forward_kind() == (0, [0])
         with pytest.raises(RaptrError):
             R.set_forward_kind(2)
 
@@ -722,7 +808,7 @@ def test_recon_forward_lds_staged_small_system(gpu, oracle, small_system):
     n = S["grid"].nvox
     x0 = np.full(n, 1.0 / n, np.float32)
     R = Reconstructor(S["A"], S["sens"], "f32", 1, layout="blocked", row_order=S["setup"])
-    assert R.forward_kind() == (2, [0, 1, 2])
+    assert R.forward_kind() == (2, [0, 2])
     a = R.mlem(S["y"], x0, 5)
     R.set_forward_kind(0)
     b = R.mlem(S["y"], x0, 5)

@@ -44,6 +44,8 @@ class FakeLane(serving.Lane):
         self.queue = deque()
         self.bytes = serving.estimate_lane_bytes(geometry, frames, 1000)
         self.closed = False
+        self.partial = frames == serving.GROUPED_FRAMES
+        self.active_log = []
         FakeLane.log.append(("build", geometry))
 
     def load(self, slot, req, now):
@@ -57,11 +59,22 @@ class FakeLane(serving.Lane):
         self.admitted[slot] = now
 
     def launch(self, k):
+        a = self.active()
+        # every busy slot lies in the frames the pass computes
+        assert all(r is None for r in self.slots[a:])
+        self.active_log.append(a)
         for _ in range(k):
-            for f in range(self.frames):
+            for f in range(a):
                 self.X[:, f] = mlem_step(self.A, self.s, self.Y[:, f], self.X[:, f])
                 if self.slots[f] is not None:
                     self.iters[f] += 1
+        return a
+
+    def _move(self, src, dst):
+        for arr in (self.Y, self.X):
+            arr[:, dst] = arr[:, src]
+            arr[:, src] = 0
+        self.iters[dst], self.iters[src] = self.iters[src], 0
 
     def retire(self, k, now):
         out = []
@@ -142,6 +155,24 @@ def test_each_request_gets_its_iterations_alone_or_mixed(fake):
     assert s["completed"] == 23 and s["p99_ms"] >= s["p50_ms"]
 
 
+def test_64_slot_lanes_shrink_to_16_or_32_frames(fake):
+    """64-slot lanes: 40 requests at once run 64-frame passes, then, as
+    requests finish, the remaining ones move into the lowest slots and the
+    lane runs 32- and 16-frame passes; every request still gets exactly its
+    iterations and the image it gets alone."""
+    g = _geometry(3, 2, 4)
+    reqs = _requests([g], 40, seed=3, iters=(1, 12))
+    srv = serving.Server(serving.MatrixCache(1 << 40, 64), max_chunk=2)
+    st = srv.serve(reqs, realtime=False)
+    mixed = {r.rid: r for r in st.results}
+    log = srv.cache.get(g).active_log
+    assert log[0] == 64 and 32 in log and log[-1] == 16
+    assert st.moves > 0 and st.summary()["chunks_by_active_frames"][16] > 0
+    for req in reqs[:6]:
+        alone = srv.serve([req], realtime=False).results[0]
+        np.testing.assert_allclose(mixed[req.rid].image, alone.image, rtol=1e-12)
+
+
 def test_invalid_requests_are_rejected(fake):
     g = _geometry(2, 2, 3)
     good = _requests([g], 2)
@@ -217,6 +248,31 @@ def test_gpu_alone_equals_mixed_and_matches_oracle(gpu, oracle, frames):
                           req.niter)
     rel = np.linalg.norm(mixed[req.rid].image - x64) / np.linalg.norm(x64)
     assert rel <= 1e-4
+
+
+@pytest.mark.gpu
+def test_gpu_lane_shrinks_to_active_frames(gpu):
+    """A 64-slot lane on the 20,480-channel geometry with 40 requests at
+    once: it runs 64-, 32- and 16-frame passes as requests finish and the
+    rest move into the lowest slots, and every image equals the image of
+    the request served alone (a 16-frame pass) bit for bit."""
+    from raptr_amd import synthetic, sysmat
+    setup, grid, nrays = G.config_small()
+    geo = serving.Geometry(setup, grid, nrays)
+    img, _ = synthetic.random_ellipsoids(grid, n=4, seed=3)
+    ybar = sysmat.forward_project(setup, grid, img.astype(np.float32), nrays)
+    reqs = serving.synthetic_requests([geo], 1000.0, 40, seed=6, events=(1e4, 1e6),
+                                      iters=(2, 14), ybar={geo: ybar})
+    srv = serving.Server(serving.MatrixCache(40 << 30, frames=64), max_chunk=2)
+    st = srv.serve(reqs, realtime=False)
+    summ = st.summary()
+    print(summ["chunks_by_active_frames"], "moves", summ["slot_moves"])
+    assert set(summ["chunks_by_active_frames"]) == {16, 32, 64} and summ["slot_moves"] > 0
+    mixed = {r.rid: r for r in st.results}
+    assert all(r.error is None for r in mixed.values())
+    for req in reqs[::5]:
+        alone = srv.serve([req], realtime=False).results[0]
+        np.testing.assert_array_equal(mixed[req.rid].image, alone.image)
 
 
 @pytest.mark.gpu
