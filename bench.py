@@ -49,11 +49,12 @@ CPU_ITERS = 5  # iterations per timed run of the CPU baseline
 # Row order: y-major locality order for one frame.
 MEASURED_TRAFFIC = {
     # forward (kernel, lanes, unroll, row order); one launch per pass
-    ("k_fwd1l", 16, 4, "y"): 2 * 5156007 * 1024 + 27546 * 1024,
+    ("k_fwd1l", 16, 4, "y"): 2 * 5156424 * 1024 + 27546 * 1024,
     # backward with per-XCD row ranges (unroll, row blocks, row order,
-    # 16-bit rows, lanes per column, row blocks per XCD and launch); bytes
-    # per launch (its reduce, k_bwd1_xreduce, moves 0.006 GB more per pass)
-    ("k_bwd1x", 2, 368, "y", "r16", 32, 0): 2 * 5855002 * 1024 + 8618 * 1024,
+    # 16-bit rows, lanes per column); one launch per pass (its reduce,
+    # k_bwd1_xreduce, moves 0.006 GB more)
+    ("k_bwd1x", 2, 69, "y", "r16", 32): 2 * 5467831 * 1024 + 10202 * 1024,
+    ("k_bwd1x", 2, 368, "y", "r16", 32): 2 * 5855002 * 1024 + 8618 * 1024,
 }
 
 
@@ -366,11 +367,8 @@ def main():
              cfg_p["bwd_staged"], cfg_p["row_order"])
     if info["bwd_staged"] == 3:
         lpc = cfg_p["bwd_lanes_per_column"]
-        xb = capi.lib().raptr_recon_get_xcd_blocks(R.handle)
-        # launches per pass: windows of xb row blocks per XCD (0: one)
-        nl_b = -(-(-(-nb // 8)) // xb) if xb > 0 else 1
         key_b = ("k_bwd1x", cfg_p["bwd_unroll"], nb, cfg_p["row_order"],
-                 "r16" if rows16 else "r32", lpc if lpc in (16, 32) else 64, xb)
+                 "r16" if rows16 else "r32", lpc if lpc in (16, 32) else 64)
     dom = {"kernel": fwd_name, "ms": ms_f, "bytes": bytes_f, "gbs": gbs_f, "launches": nl_f,
            "key": key_f, "stored": stored_f}
     if ms_b > ms_f:
@@ -548,10 +546,10 @@ def main():
 # row blocks, row blocks per XCD and launch); per launch of k_bwdm64x
 MEASURED_TRAFFIC_BATCHED = {
     # (kernel, group, row blocks, row blocks per XCD and launch): bytes per
-    # launch of the 64-frame backward (9 launches per pass here)
-    ("k_bwdm64x", 16, 138, 2): 2 * 2293737 * 1024 + 289077 * 1024,
+    # launch of the 64-frame backward (5 launches per pass here)
+    ("k_bwdm64x", 16, 138, 4): 2 * 5455964 * 1024 + 291307 * 1024,
     # (kernel, group, unroll): the forward, one launch per pass
-    ("k_fwdm64", 16, 8): 2 * 14024181 * 1024 + 1127705 * 1024,
+    ("k_fwdm64", 16, 8): 2 * 9235221 * 1024 + 1127705 * 1024,
 }
 
 

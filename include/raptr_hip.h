@@ -522,15 +522,13 @@ int raptr_recon_get_broadcast(const raptr_recon* plan);
  * bwd_entries = union entries of each layout (lists padded to multiples of
  * 32 / 8). */
 int raptr_recon_set_grouped(raptr_recon* plan, int on, int unroll, int kind);
-/* The per-XCD backwards (64-frame kind 3; single-frame mode 3) in several
- * launches: each XCD walks `blocks` of its row blocks per launch (0: all in
- * one launch, the single-frame default), adding to its partial image
- * between launches.  The bricks or columns resident on an XCD then stay
- * within a few blocks of each other, so the ratio rows they gather fit its
- * L2; each extra launch re-reads and re-writes the partial images (8 x n x
- * F floats).  Same sums whatever the window for 64 frames; for one frame
- * each window's lane sums are added to the partial sum in window order, a
- * fixed order per window size. */
+/* Kind 3's backward in several launches: each XCD walks `blocks` of its
+ * row blocks per launch (0: all in one launch), adding to its partial
+ * image between launches.  The bricks resident on an XCD then stay within
+ * a few blocks of each other, so the ratio rows they gather fit its L2;
+ * each extra launch re-reads and re-writes the partial images.  (The
+ * single-frame per-XCD backward always runs in one launch: windows there
+ * were 2-40 % slower at full size.) */
 int raptr_recon_set_xcd_blocks(raptr_recon* plan, int blocks);
 int raptr_recon_get_xcd_blocks(const raptr_recon* plan);
 int raptr_recon_get_grouped(const raptr_recon* plan, int* group, int* on, int* unroll,

@@ -1330,13 +1330,8 @@ __device__ __forceinline__ vi4 rows4(const void* __restrict__ brow, int64_t k, i
 }
 
 // LPC lanes per column (16, 32 or 64), 256 / LPC columns per workgroup.
-// boff / bcnt: this launch covers row blocks b0 + boff .. b0 + boff + bcnt
-// - 1 of the XCD's range (several launches keep the ratio rows the XCD's
-// resident columns gather within a few blocks, so they stay in its L2
-// while the matrix streams through); accum: add to c8 instead of writing.
 template <int VT, int UNROLL, bool R16, int LPC>
-__global__ __launch_bounds__(256) void k_bwd1x(int n, int nblk, int blk_rows, int boff,
-                                               int bcnt, int accum,
+__global__ __launch_bounds__(256) void k_bwd1x(int n, int nblk, int blk_rows,
                                                const int64_t* __restrict__ ptr,
                                                const void* __restrict__ brow,
                                                const typename Val<VT>::T* __restrict__ val,
@@ -1346,9 +1341,7 @@ __global__ __launch_bounds__(256) void k_bwd1x(int n, int nblk, int blk_rows, in
   const int xcd = int(blockIdx.x % 8);
   const int j = int(blockIdx.x / 8) * (256 / LPC) + int(threadIdx.x) / LPC;
   if (j >= n) return;  // the column's LPC lanes leave together
-  const int x0 = int(int64_t(nblk) * xcd / 8), x1 = int(int64_t(nblk) * (xcd + 1) / 8);
-  const int b0 = x0 + boff < x1 ? x0 + boff : x1;
-  const int b1 = b0 + bcnt < x1 ? b0 + bcnt : x1;
+  const int b0 = int(int64_t(nblk) * xcd / 8), b1 = int(int64_t(nblk) * (xcd + 1) / 8);
   constexpr int64_t STEP = 4 * LPC;
   float acc = 0.0f;
   // the segment bounds of up to LPC/2 blocks in one load (lane 2t, 2t+1:
@@ -1396,10 +1389,7 @@ __global__ __launch_bounds__(256) void k_bwd1x(int n, int nblk, int blk_rows, in
   }
 #pragma unroll
   for (int off = LPC / 2; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, LPC);
-  if (lane == 0) {
-    float* o = c8 + int64_t(xcd) * n + j;
-    *o = accum ? *o + acc : acc;
-  }
+  if (lane == 0) c8[int64_t(xcd) * n + j] = acc;
 }
 
 // c = sum of the 8 partial column sums (XCD order), x' = x*c/(s + (s==0)),
@@ -1951,22 +1941,17 @@ int launch_bwd1_xcd(const raptr_recon* R, const float* x, hipStream_t st, int* n
   const unsigned gg = unsigned(8 * ((int64_t(n) + (256 / L) - 1) / (256 / L)));
   const int U = R->bwd_unroll;
   const bool r16 = R->brow16 && R->bwd_r16;
-  // windows of xcd_bpl row blocks per XCD and launch (0: all in one)
-  const int per_xcd = (R->nblk + 7) / 8;
-  const int bpl = R->xcd_bpl > 0 ? R->xcd_bpl : per_xcd;
-  for (int boff = 0; boff < per_xcd; boff += bpl) {
-    const int accum = boff > 0;
-    bool launched = false;
+  bool launched = false;
 #define RAPTR_B1X(VV, UU, LL)                                                               \
   if (R->vt == VV && U == UU && L == LL) {                                                  \
     if (r16)                                                                                \
       hipLaunchKernelGGL((k_bwd1x<VV, UU, true, LL>), dim3(gg), dim3(256), 0, st, n,         \
-                         R->nblk, R->blk_rows, boff, bpl, accum, (const int64_t*)R->bptr,   \
+                         R->nblk, R->blk_rows, (const int64_t*)R->bptr,                     \
                          (const void*)R->brow16, (const typename Val<VV>::T*)R->bval,       \
                          (const float*)R->e, R->c8);                                        \
     else                                                                                    \
       hipLaunchKernelGGL((k_bwd1x<VV, UU, false, LL>), dim3(gg), dim3(256), 0, st, n,        \
-                         R->nblk, R->blk_rows, boff, bpl, accum, (const int64_t*)R->bptr,   \
+                         R->nblk, R->blk_rows, (const int64_t*)R->bptr,                     \
                          (const void*)R->brow, (const typename Val<VV>::T*)R->bval,         \
                          (const float*)R->e, R->c8);                                        \
     launched = true;                                                                        \
@@ -1975,13 +1960,12 @@ int launch_bwd1_xcd(const raptr_recon* R, const float* x, hipStream_t st, int* n
   RAPTR_B1X(VV, 1, 16) RAPTR_B1X(VV, 2, 16) RAPTR_B1X(VV, 4, 16) RAPTR_B1X(VV, 1, 32)       \
   RAPTR_B1X(VV, 2, 32) RAPTR_B1X(VV, 4, 32) RAPTR_B1X(VV, 1, 64) RAPTR_B1X(VV, 2, 64)       \
   RAPTR_B1X(VV, 4, 64)
-    RAPTR_B1XV(kF32) RAPTR_B1XV(kF16) RAPTR_B1XV(kBF16) RAPTR_B1XV(kU16)
+  RAPTR_B1XV(kF32) RAPTR_B1XV(kF16) RAPTR_B1XV(kBF16) RAPTR_B1XV(kU16)
 #undef RAPTR_B1XV
 #undef RAPTR_B1X
-    if (!launched)
-      return fail(RAPTR_ERR_INVALID_ARG, "per-XCD backward unroll %d lanes %d unsupported", U, L);
-    RAPTR_LAUNCH_CHECK();
-  }
+  if (!launched)
+    return fail(RAPTR_ERR_INVALID_ARG, "per-XCD backward unroll %d lanes %d unsupported", U, L);
+  RAPTR_LAUNCH_CHECK();
   const unsigned nr = unsigned((int64_t(n) + 255) / 256);
   hipLaunchKernelGGL(k_bwd1_xreduce, dim3(nr), dim3(256), 0, st, n, (const float*)R->c8, x,
                      (const float*)R->s, R->xx, R->partials);

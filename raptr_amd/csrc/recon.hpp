@@ -124,8 +124,8 @@ struct raptr_recon {
   int bwd_lpc = 16;    // lanes per column of the per-XCD backward (8 runs as 64)
   int bwd_wgs = 512;   // kept for the get/set API (unused)
   int bwd_bpl = 1;     // row blocks per backward launch (blocked layout)
-  int xcd_bpl = 0;     // per-XCD backwards (64-frame kind 3, single-frame mode 3):
-                       // row blocks each XCD walks per launch (0: all in one)
+  int xcd_bpl = 0;     // per-XCD 64-frame backward (kind 3): row blocks each
+                       // XCD walks per launch (0: all of its blocks in one)
   int bcast64 = 0;     // 64-frame kernels with in-row broadcast of values/indices
   int layout = RAPTR_LAYOUT_BLOCKED;
   // grouped 64-frame layouts (max_frames = 64): forward groups of 4 plan

@@ -249,26 +249,19 @@ class Reconstructor:
             except capi.RaptrError:
                 staged_ok = False
         if staged_ok:
-            # per-XCD row ranges, in one launch or in windows of 1-4 row
-            # blocks per XCD and launch
-            per_xcd = -(-inf["row_blocks"] // 8)
             bshapes += [(3, lpc, 512, u, 1) for lpc in (16, 32, 64) for u in (1, 2, 4)]
-            xbs = [0] + [w for w in (1, 2, 4) if w < per_xcd]
         best = None
-        for (st, lpc, wgs, u, bpl), xb in [(b, w) for b in bshapes
-                                           for w in (xbs if b[0] == 3 else [0])]:
+        for st, lpc, wgs, u, bpl in bshapes:
             if staged_ok:
                 self.set_backward(st, lpc, wgs)
-                self.set_xcd_blocks(xb)
             self.set_config(fl, fu, u)
             self.set_blocks_per_launch(bpl)
             ms = timed()
             if best is None or ms < best[1]:
-                best = ((st, lpc, wgs, u, bpl, xb), ms)
-        st, lpc, wgs, u, bpl, xb1 = best[0]
+                best = ((st, lpc, wgs, u, bpl), ms)
+        st, lpc, wgs, u, bpl = best[0]
         if staged_ok:
             self.set_backward(st, lpc, wgs)
-            self.set_xcd_blocks(xb1)
         self.set_config(fl, fu, u)
         self.set_blocks_per_launch(bpl)
         bc = 0
@@ -281,7 +274,7 @@ class Reconstructor:
             else:
                 self.set_broadcast(False)
         ms_best = timed()
-        g_on, g_u, g_p, xb = 0, 0, 0, xb1
+        g_on, g_u, g_p, xb = 0, 0, 0, 0
         if grp:
             best_g = self._tune_grouped(inf, timed)
             if best_g[1] < ms_best:
