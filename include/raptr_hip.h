@@ -489,6 +489,31 @@ int raptr_recon_get_compact_rows(const raptr_recon* plan);
 int raptr_recon_set_forward_kind(raptr_recon* plan, int kind);
 int raptr_recon_get_forward_kind(const raptr_recon* plan, int* kind, int* available,
                                  int64_t* union_entries);
+/* Fused single-frame iteration.  Replaces the pair of products
+ * cusparseScsrmv(NON_TRANSPOSE) / cusparseScsrmv(TRANSPOSE) of one MLEM
+ * iteration (reco.cu:257-270, csrmv.hpp:72-83) by one pass over the
+ * matrix: per group of 64 consecutive plan rows, x over the group's column
+ * union in LDS, each row's ratio e_i = y_i / (A x)_i, and A_ij e_i added
+ * into column sums over the union in LDS, written as the group's partial
+ * sums; a second kernel adds each column's partials in group order and
+ * applies the update.  Bit-reproducible run to run (fixed orders, no
+ * atomics).  Built with the LDS-staged forward's unions where no row
+ * repeats a column; adds 8 bytes per union entry and 8 per voxel to the
+ * plan.
+ *   on: 0 the separate forward and backward passes; 1 fused, one column-sum
+ *       region per wave (520 + 4 x (largest union + waves x (largest union
+ *       + 1)) bytes of LDS must fit 160 KiB); 2 fused, one region per
+ *       workgroup that the waves add to in turn (520 + 4 x (2 x largest
+ *       union + 1) bytes);
+ *   waves: waves per row group (2 or 4; 8 with on = 2);
+ *   lanes_per_row: lanes per matrix row (16, 32 or 64);
+ *   lanes_per_column: lanes per column of the reduce (16, 32 or 64).
+ * get: available = 1 if the plan has the fused pass; null outputs are
+ * skipped. */
+int raptr_recon_set_fused(raptr_recon* plan, int on, int waves, int lanes_per_row,
+                          int lanes_per_column);
+int raptr_recon_get_fused(const raptr_recon* plan, int* available, int* on, int* waves,
+                          int* lanes_per_row, int* lanes_per_column);
 /* Row blocks per backward launch of the blocked layout (1..64): a wave
  * walks its column through that many consecutive blocks, keeping the sum in
  * registers (longer waves, fewer launches). */

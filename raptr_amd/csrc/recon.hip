@@ -1421,6 +1421,348 @@ __global__ __launch_bounds__(256) void k_bwd1_xreduce(int n, const float* __rest
   if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
 }
 
+// ------------------------------------------------------------------------
+// Single frame, both products in one pass over the matrix.  A workgroup of
+// W waves takes one group of kLocRows consecutive plan rows (the groups of
+// the LDS-staged forward) and copies x over the group's column union into
+// LDS.  The group's rows go out in batches: in each, every wave takes
+// 64 / LANES rows, LANES lanes per row, with S steps of each row's entries
+// held in registers and the next batch's already in flight:
+//   yt_i = sum_j A_ij x_j (x from LDS), e_i = y_i / yt_i,
+// then A_ij e_i is added into the wave's own column sums over the union,
+// also in LDS (waves never share a sum, so no two waves' adds interleave;
+// inside one wave the adds follow its instruction order).  At the end the
+// W partial column sums are added in wave order and written over the
+// union (cpart[up[g] + rank]); k_fb1_reduce adds each column's group
+// partials in group order.  The matrix is streamed once per iteration
+// instead of once per product.
+template <int CTRL>
+__device__ __forceinline__ float dppf(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+// 4 entries at padded position k of a row [b, en); lanes past the row get
+// value 0 and rank 0 from a load of the row's first entries (position b is
+// inside the arrays' padding even for an empty last row), so the loads are
+// not branched around and the next row's stay in flight
+template <int VT>
+__device__ __forceinline__ void fb_load(const uint16_t* __restrict__ lcol,
+                                        const typename Val<VT>::T* __restrict__ val, int64_t k,
+                                        int64_t b, int64_t en, vus4& c, float (&v)[4]) {
+  const bool ok = k < en;
+  const int64_t kk = ok ? k : b;
+  const vus4 cc = *reinterpret_cast<const vus4*>(lcol + kk);
+  float t[4];
+  load4c<VT>(val, kk, t[0], t[1], t[2], t[3]);
+  c = ok ? cc : vus4{0, 0, 0, 0};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) v[q] = ok ? t[q] : 0.0f;
+}
+
+// sum over the LANES lanes of each row slot, the same bits in all of them
+// (DPP quad swaps and row mirrors inside 16-lane rows, then swaps across
+// rows); every lane must be active
+template <int LANES>
+__device__ __forceinline__ float slot_sum(float v) {
+  v += dppf<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dppf<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dppf<0x141>(v);  // row_half_mirror
+  v += dppf<0x140>(v);  // row_mirror
+  if constexpr (LANES >= 32) v += __shfl_xor(v, 16, 64);
+  if constexpr (LANES >= 64) v += __shfl_xor(v, 32, 64);
+  return v;
+}
+
+// W waves per group; LANES lanes per row (64 / LANES rows per wave and
+// batch); S steps of 4 x LANES entries per row held in registers (longer
+// rows continue from memory); the next batch in flight while one is
+// summed.  SH: one column-sum region for the workgroup, the waves adding
+// their rows in turn (wave order, a barrier between turns) -- a third of
+// the LDS of W regions, so about three times the workgroups per CU
+template <int VT, int W, int LANES, int S, bool SH>
+__global__ __launch_bounds__(64 * W) void k_fb1l(int64_t m, const int64_t* __restrict__ rp,
+                                                 const uint16_t* __restrict__ lcol,
+                                                 const int64_t* __restrict__ up,
+                                                 const int* __restrict__ ucol,
+                                                 const typename Val<VT>::T* __restrict__ val,
+                                                 const float* __restrict__ rscale,
+                                                 const float* __restrict__ x,
+                                                 const float* __restrict__ y,
+                                                 float* __restrict__ e,
+                                                 float* __restrict__ cpart) {
+  constexpr int RPW = 64 / LANES;  // rows per wave and batch
+  constexpr int STEP = 4 * LANES;  // entries of a row per step
+  constexpr int NT = 64 * W;
+  constexpr int kBatches = (kLocRows + W * RPW - 1) / (W * RPW);
+  constexpr int NR = SH ? 1 : W;  // column-sum regions
+  // the group's kLocRows + 1 row bounds, x over the union (nus floats),
+  // then NR column-sum regions of nus + 1 floats each (the last a spare
+  // slot)
+  extern __shared__ int64_t fb_lds[];
+  int64_t* rps = fb_lds;
+  float* xs = reinterpret_cast<float*>(fb_lds + kLocRows + 1);
+  const int tid = int(threadIdx.x);
+  const int64_t grp = blockIdx.x;
+  const int64_t ub = up[grp];
+  const int nu = int(up[grp + 1] - ub);
+  const int nus = nu > 0 ? nu : 1;
+  const int64_t r0 = grp * kLocRows;
+  const int nr = int(r0 + kLocRows < m ? kLocRows : m - r0);  // >= 1
+  if (tid <= kLocRows) rps[tid] = rp[r0 + (tid < nr ? tid : nr)];
+  // x over the union, 8 gathers in flight per thread
+  for (int j0 = 0; j0 < nu; j0 += NT * 8) {
+    int cc[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int j = j0 + tid + u * NT;
+      cc[u] = ucol[ub + (j < nu ? j : 0)];
+    }
+    float xv[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) xv[u] = x[cc[u]];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int j = j0 + tid + u * NT;
+      if (j < nu) xs[j] = xv[u];
+    }
+  }
+  for (int j = tid; j < NR * (nus + 1); j += NT) xs[nus + j] = 0.0f;
+  if (nu == 0 && tid == 0) xs[0] = 0.0f;  // lanes past a row read rank 0
+  __syncthreads();
+  const int w = tid / 64, lane = tid % 64, q = lane / LANES, g = lane % LANES;
+  float* cw = xs + nus + (SH ? 0 : int64_t(nus + 1) * w);
+  // row of this lane's slot in batch bt (rows past the group are empty)
+  auto row_of = [&](int bt) { return bt * (W * RPW) + w * RPW + q; };
+  int t = row_of(0);
+  int64_t b = rps[t < nr ? t : nr], en = t < nr ? rps[t + 1] : b;
+  float yv = y[r0 + (t < nr ? t : nr - 1)];
+  float sv = VT != kF32 ? rscale[r0 + (t < nr ? t : nr - 1)] : 1.0f;
+  vus4 c[S];
+  float v[S][4];
+#pragma unroll
+  for (int s = 0; s < S; ++s) fb_load<VT>(lcol, val, b + 4 * g + STEP * s, b, en, c[s], v[s]);
+  for (int bt = 0; bt < kBatches; ++bt) {
+    // the next batch's rows in flight
+    const int tn = row_of(bt + 1);
+    const bool has_next = bt + 1 < kBatches && tn < nr;
+    const int64_t bn = rps[tn < nr ? tn : nr], enn = has_next ? rps[tn + 1] : bn;
+    const float yn = y[r0 + (tn < nr ? tn : nr - 1)];
+    const float sn = VT != kF32 ? rscale[r0 + (tn < nr ? tn : nr - 1)] : 1.0f;
+    vus4 cn[S];
+    float vn[S][4];
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+      fb_load<VT>(lcol, val, bn + 4 * g + STEP * s, bn, enn, cn[s], vn[s]);
+    float acc = 0.0f;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      acc = fmaf(v[s][0], xs[c[s].x], acc);
+      acc = fmaf(v[s][1], xs[c[s].y], acc);
+      acc = fmaf(v[s][2], xs[c[s].z], acc);
+      acc = fmaf(v[s][3], xs[c[s].w], acc);
+    }
+    const int64_t kx = b + 4 * g + int64_t(STEP) * S;  // rows past S steps
+    for (int64_t kk = kx; kk < en; kk += STEP) {
+      const vus4 cc = *reinterpret_cast<const vus4*>(lcol + kk);
+      float vv[4];
+      load4c<VT>(val, kk, vv[0], vv[1], vv[2], vv[3]);
+      acc = fmaf(vv[0], xs[cc.x], acc);
+      acc = fmaf(vv[1], xs[cc.y], acc);
+      acc = fmaf(vv[2], xs[cc.z], acc);
+      acc = fmaf(vv[3], xs[cc.w], acc);
+    }
+    const float yt = sv * slot_sum<LANES>(acc);
+    const float r = yt != 0.0f ? yv / yt : 0.0f;
+    const float ev = VT == kF32 ? r : sv * r;  // ratio folded with the row scale
+    if (t < nr && g == 0) e[r0 + t] = ev;
+    // Column sums, one row slot at a time: the lanes of one LDS instruction
+    // then hold distinct columns of one row (the plan has no repeated
+    // column in a row), so a plain read-add-write per entry is exact and
+    // ordered; entries of value 0 (padding, lanes past the row) go to the
+    // region's spare slot nus.  LDS float atomics are far slower.
+    int si[S][4];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      const bool ok = b + 4 * g + STEP * s < en;
+      si[s][0] = ok && v[s][0] != 0.0f ? int(c[s].x) : nus;
+      si[s][1] = ok && v[s][1] != 0.0f ? int(c[s].y) : nus;
+      si[s][2] = ok && v[s][2] != 0.0f ? int(c[s].z) : nus;
+      si[s][3] = ok && v[s][3] != 0.0f ? int(c[s].w) : nus;
+    }
+    for (int turn = 0; turn < (SH ? W : 1); ++turn) {
+      if (!SH || w == turn) {
+#pragma unroll
+        for (int qq = 0; qq < RPW; ++qq) {
+          if (q == qq) {
+            float old[S][4];
+#pragma unroll
+            for (int s = 0; s < S; ++s)
+#pragma unroll
+              for (int j = 0; j < 4; ++j) old[s][j] = cw[si[s][j]];
+#pragma unroll
+            for (int s = 0; s < S; ++s)
+#pragma unroll
+              for (int j = 0; j < 4; ++j) cw[si[s][j]] = fmaf(v[s][j], ev, old[s][j]);
+            for (int64_t kk = kx; kk < en; kk += STEP) {
+              const vus4 cc = *reinterpret_cast<const vus4*>(lcol + kk);
+              float vv[4];
+              load4c<VT>(val, kk, vv[0], vv[1], vv[2], vv[3]);
+              const int i0 = vv[0] != 0.0f ? int(cc.x) : nus;
+              const int i1 = vv[1] != 0.0f ? int(cc.y) : nus;
+              const int i2 = vv[2] != 0.0f ? int(cc.z) : nus;
+              const int i3 = vv[3] != 0.0f ? int(cc.w) : nus;
+              const float o0 = cw[i0], o1 = cw[i1], o2 = cw[i2], o3 = cw[i3];
+              cw[i0] = fmaf(vv[0], ev, o0);
+              cw[i1] = fmaf(vv[1], ev, o1);
+              cw[i2] = fmaf(vv[2], ev, o2);
+              cw[i3] = fmaf(vv[3], ev, o3);
+            }
+          }
+        }
+      }
+      if (SH) __syncthreads();
+    }
+    t = tn;
+    b = bn;
+    en = enn;
+    yv = yn;
+    sv = sn;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      c[s] = cn[s];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) v[s][j] = vn[s][j];
+    }
+  }
+  __syncthreads();
+  for (int j = tid; j < nu; j += NT) {
+    float sum = xs[nus + j];
+#pragma unroll
+    for (int k = 1; k < NR; ++k) sum += xs[nus + int64_t(nus + 1) * k + j];
+    cpart[ub + j] = sum;
+  }
+}
+
+// c_j = the sum of column j's group partials in group order (vpos: the
+// union positions holding column j, ascending; LPC lanes per column, each
+// lane's terms in order, then a fixed tree over the lanes), then
+// x'_j = x_j c_j / (s_j + (s_j == 0)) and an FP64 partial sum per
+// workgroup.  Workgroups of one XCD take consecutive columns, whose
+// partials lie side by side in each group's sorted union.
+template <int LPC, int U>
+__global__ __launch_bounds__(256) void k_fb1_reduce(int n, const int64_t* __restrict__ vptr,
+                                                    const int* __restrict__ vpos,
+                                                    const float* __restrict__ cpart,
+                                                    const float* __restrict__ x,
+                                                    const float* __restrict__ s,
+                                                    float* __restrict__ xx,
+                                                    double* __restrict__ partials) {
+  const int lane = int(threadIdx.x) % LPC;
+  const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t j = blk * (256 / LPC) + int(threadIdx.x) / LPC;
+  float acc = 0.0f;
+  if (j < n) {
+    const int64_t k1 = vptr[j + 1];
+    int64_t k = vptr[j] + lane;
+    for (; k + int64_t(LPC) * (U - 1) < k1; k += int64_t(LPC) * U) {
+      int p[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) p[u] = vpos[k + LPC * u];
+      float g[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) g[u] = cpart[p[u]];
+#pragma unroll
+      for (int u = 0; u < U; ++u) acc += g[u];
+    }
+    for (; k < k1; k += LPC) acc += cpart[vpos[k]];
+  }
+#pragma unroll
+  for (int off = LPC / 2; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, LPC);
+  double part = 0.0;
+  if (j < n && lane == 0) {
+    float d = s[j];
+    d += float(d == 0.0f);
+    const float nv = x[j] * acc / d;
+    xx[j] = nv;
+    part = double(nv);
+  }
+  __shared__ double red[256];
+  red[threadIdx.x] = part;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
+}
+
+// bits of the keys 0 .. count - 1
+inline int bits_for_count(int64_t count) {
+  int b = 1;
+  while ((int64_t(1) << b) < count) ++b;
+  return b;
+}
+
+// (column, position) pairs of the unions, for the column-major index
+__global__ void k_upos_keys(const int* __restrict__ ucol, int64_t nu, uint64_t* __restrict__ keys,
+                            uint32_t* __restrict__ idx) {
+  for (int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; p < nu;
+       p += int64_t(gridDim.x) * blockDim.x) {
+    keys[p] = uint64_t(uint32_t(ucol[p]));
+    idx[p] = uint32_t(p);
+  }
+}
+
+__global__ void k_u32_to_i32_64(const uint32_t* __restrict__ a, int64_t m, int* __restrict__ b) {
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < m;
+       i += int64_t(gridDim.x) * blockDim.x)
+    b[i] = int(a[i]);
+}
+
+// vptr[j] = first sorted position whose column is >= j, j = 0 .. n
+__global__ void k_vptr(const uint64_t* __restrict__ keys, int64_t nu, int n,
+                       int64_t* __restrict__ vptr) {
+  for (int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; j <= n;
+       j += int64_t(gridDim.x) * blockDim.x) {
+    int64_t lo = 0, hi = nu;
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) / 2;
+      if (int64_t(keys[mid]) < j) lo = mid + 1; else hi = mid;
+    }
+    vptr[j] = lo;
+  }
+}
+
+// *dup = 1 if a row holds one union rank twice among its entries of
+// nonzero stored value (the fused pass adds each such entry to its column
+// sum with a plain read-add-write, exact only for distinct columns).  One
+// workgroup per group of kLocRows rows, a bitmap of `words` words per wave
+template <int VT>
+__global__ __launch_bounds__(256) void k_fb_distinct(int64_t m, const int64_t* __restrict__ rp,
+                                                     const uint16_t* __restrict__ lcol,
+                                                     const typename Val<VT>::T* __restrict__ val,
+                                                     int words, int* __restrict__ dup) {
+  extern __shared__ uint32_t fb_bm[];
+  const int w = int(threadIdx.x) / 64, lane = int(threadIdx.x) % 64;
+  uint32_t* bm = fb_bm + int64_t(w) * words;
+  const int64_t r0 = int64_t(blockIdx.x) * kLocRows;
+  const int nr = int(r0 + kLocRows < m ? kLocRows : m - r0);
+  int found = 0;
+  for (int t = w; t < nr; t += 4) {
+    for (int k = lane; k < words; k += 64) bm[k] = 0u;
+    const int64_t b = rp[r0 + t], en = rp[r0 + t + 1];
+    for (int64_t k = b + lane; k < en; k += 64) {
+      const bool nz = VT == kF32 ? reinterpret_cast<const float*>(val)[k] != 0.0f
+                                 : reinterpret_cast<const uint16_t*>(val)[k] != 0;
+      if (nz) {
+        const uint32_t r = lcol[k], bit = 1u << (r & 31u);
+        if (atomicOr(&bm[r >> 5], bit) & bit) found = 1;
+      }
+    }
+  }
+  if (found) atomicOr(dup, 1);
+}
+
+
 // brick of the backward groups: G voxels (x, y, z extents), wide in x and z
 // and thin in y.  The lines of a two-head setup rotated about y run nearly
 // parallel to the x-z planes, so a line that crosses a 4 x 1 x 4 brick meets
@@ -2010,9 +2352,81 @@ int launch_bwd_xcd(const raptr_recon* R, const float* x, hipStream_t st, int* nb
   return RAPTR_OK;
 }
 
+// LDS bytes of the fused pass with W waves per group: the group's row
+// bounds, x over the largest union and W column-sum regions, or one when
+// the waves share it (one spare slot each)
+size_t fused_lds_bytes(const raptr_recon* R, int W, bool shared) {
+  const size_t u = size_t(std::max(R->lmax, 1));
+  return size_t(kLocRows + 1) * sizeof(int64_t) +
+         (u + size_t(shared ? 1 : W) * (u + 1)) * sizeof(float);
+}
+constexpr size_t kLdsMax = 160 * 1024;
+
+// single frame: forward, ratio and backward in one pass (k_fb1l), then the
+// fixed-order column reduce + update
+int launch_fused1(const raptr_recon* R, const float* x, const float* y, hipStream_t st,
+                  int* nb_out) {
+  if (!R->cpart || !R->vpos) return fail(RAPTR_ERR_INVALID_ARG, "plan has no fused pass");
+  const int64_t m = R->m;
+  const int n = R->n;
+  const int W = R->fb_waves, LR = R->fb_lanes;
+  const bool SHR = R->fused == 2;
+  const size_t shm = fused_lds_bytes(R, W, SHR);
+  if (shm > kLdsMax) return fail(RAPTR_ERR_INVALID_ARG, "fused pass: %zu B of LDS", shm);
+  if (m > 0) {
+    const unsigned g = unsigned((m + kLocRows - 1) / kLocRows);
+    bool launched = false;
+#define RAPTR_FB(VV, WW, LL, SS, SH)                                                         \
+  if (R->vt == VV && W == WW && LR == LL && SHR == SH) {                                     \
+    static bool configured = false;                                                          \
+    if (!configured) {                                                                       \
+      if (hipFuncSetAttribute((const void*)k_fb1l<VV, WW, LL, SS, SH>,                       \
+                              hipFuncAttributeMaxDynamicSharedMemorySize,                    \
+                              int(kLdsMax)) != hipSuccess)                                   \
+        (void)hipGetLastError();                                                             \
+      configured = true;                                                                     \
+    }                                                                                        \
+    hipLaunchKernelGGL((k_fb1l<VV, WW, LL, SS, SH>), dim3(g), dim3(64 * WW), shm, st, m,      \
+                       R->frp,                                                               \
+                       (const uint16_t*)R->lcol, (const int64_t*)R->lup,                     \
+                       (const int*)R->lucol, (const typename Val<VV>::T*)R->fval, R->rscale, \
+                       x, y, R->e, R->cpart);                                                \
+    launched = true;                                                                         \
+  }
+#define RAPTR_FBS(VV, WW, SH)                                                                \
+  RAPTR_FB(VV, WW, 16, 6, SH) RAPTR_FB(VV, WW, 32, 3, SH) RAPTR_FB(VV, WW, 64, 2, SH)
+#define RAPTR_FBW(VV, WW) RAPTR_FBS(VV, WW, false) RAPTR_FBS(VV, WW, true)
+#define RAPTR_FBV(VV) RAPTR_FBW(VV, 2) RAPTR_FBW(VV, 4) RAPTR_FBS(VV, 8, true)
+    RAPTR_FBV(kF32) RAPTR_FBV(kF16) RAPTR_FBV(kBF16) RAPTR_FBV(kU16)
+#undef RAPTR_FBV
+#undef RAPTR_FBW
+#undef RAPTR_FBS
+#undef RAPTR_FB
+    if (!launched)
+      return fail(RAPTR_ERR_INVALID_ARG, "fused pass: %d waves, %d lanes per row unsupported", W,
+                  LR);
+    RAPTR_LAUNCH_CHECK();
+  }
+  const int L = R->fb_lpc;
+  const unsigned nr = unsigned((int64_t(n) + (256 / L) - 1) / (256 / L));
+#define RAPTR_FR(LL)                                                                           \
+  if (L == LL)                                                                                 \
+    hipLaunchKernelGGL((k_fb1_reduce<LL, 4>), dim3(nr), dim3(256), 0, st, n,                    \
+                       (const int64_t*)R->vptr, (const int*)R->vpos, (const float*)R->cpart,   \
+                       x, (const float*)R->s, R->xx, R->partials);
+  RAPTR_FR(16) else RAPTR_FR(32) else RAPTR_FR(64) else return fail(RAPTR_ERR_INVALID_ARG,
+                                                                    "reduce lanes %d", L);
+#undef RAPTR_FR
+  RAPTR_LAUNCH_CHECK();
+  if (nb_out) *nb_out = int(nr);
+  return RAPTR_OK;
+}
+
 int iteration(raptr_recon* R, int F, const float* y, float* x, hipStream_t st) {
   int nb = 0;
-  if (F == 1 && R->bwd_lds == 3) {
+  if (F == 1 && R->fused && R->cpart) {
+    RAPTR_TRY(launch_fused1(R, x, y, st, &nb));
+  } else if (F == 1 && R->bwd_lds == 3) {
     RAPTR_TRY(launch_fwd_any(R, F, x, y, R->e, st));
     RAPTR_TRY(launch_bwd1_xcd(R, x, st, &nb));
   } else if (F == 64 && R->grp64 && R->grp_pipe == 3) {
@@ -2249,6 +2663,86 @@ int build_local_columns(raptr_recon* R, hipStream_t st) {
   rfree(R, cnt);
   R->lmax = int(mx);
   R->lnu = h;
+  return RAPTR_OK;
+}
+
+// the fused pass's index: the union positions of every column, ascending
+// (a stable radix sort of the positions by column), and the groups'
+// partial column sums; none where the unions do not fit LDS or a row
+// repeats a column
+int build_union_index(raptr_recon* R, hipStream_t st) {
+  const int64_t nu = R->lnu;
+  const int n = R->n;
+  if (!R->lucol || nu <= 0 || nu > int64_t(INT32_MAX) || fused_lds_bytes(R, 2, true) > kLdsMax)
+    return RAPTR_OK;
+  int rc;
+  {  // rows must not repeat a column
+    int* dup = nullptr;
+    if ((rc = ralloc(R, &dup, 1))) return rc;
+    (void)hipMemsetAsync(dup, 0, sizeof(int), st);
+    const int words = (R->lmax + 31) / 32;
+    const unsigned ng = unsigned((int64_t(R->m) + kLocRows - 1) / kLocRows);
+    const size_t shm = size_t(4) * size_t(std::max(words, 1)) * sizeof(uint32_t);
+    switch (R->vt) {
+#define RAPTR_FD(VV)                                                                         \
+  case VV:                                                                                   \
+    hipLaunchKernelGGL(k_fb_distinct<VV>, dim3(ng), dim3(256), shm, st, int64_t(R->m),        \
+                       (const int64_t*)R->frp, (const uint16_t*)R->lcol,                     \
+                       (const typename Val<VV>::T*)R->fval, words, dup);                     \
+    break;
+      RAPTR_FD(kF32) RAPTR_FD(kF16) RAPTR_FD(kBF16) RAPTR_FD(kU16)
+#undef RAPTR_FD
+    }
+    int h = 0;
+    hipError_t le = hipGetLastError();
+    if (le == hipSuccess) le = hipMemcpyAsync(&h, dup, sizeof(int), hipMemcpyDeviceToHost, st);
+    if (le == hipSuccess) le = hipStreamSynchronize(st);
+    rfree(R, dup);
+    if (le != hipSuccess) return fail(RAPTR_ERR_DEVICE, "union index: %s", hipGetErrorString(le));
+    if (h) return RAPTR_OK;  // a repeated column: no fused pass
+  }
+  if ((rc = ralloc(R, &R->vptr, size_t(n) + 1)) || (rc = ralloc(R, &R->vpos, size_t(nu))) ||
+      (rc = ralloc(R, &R->cpart, size_t(nu))))
+    return rc;
+  const int64_t ntiles = (nu + 2047) / 2048;
+  Sizer z;
+  z.take<uint64_t>(size_t(nu));
+  z.take<uint64_t>(size_t(nu));
+  z.take<uint32_t>(size_t(nu));
+  z.take<uint32_t>(size_t(nu));
+  z.take<uint32_t>(size_t(ntiles) * 256);
+  z.take<uint32_t>(size_t(ntiles) * 256);
+  z.take<char>(scan_workspace_bytes(ntiles * 256));
+  void* tmp = nullptr;
+  hipError_t ae = hipMalloc(&tmp, z.bytes() + 4096);
+  if (ae != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(RAPTR_ERR_OUT_OF_MEMORY, "union index scratch: %s", hipGetErrorString(ae));
+  }
+  Carver cv(tmp, z.bytes() + 4096);
+  uint64_t* k0 = cv.take<uint64_t>(size_t(nu));
+  uint64_t* k1 = cv.take<uint64_t>(size_t(nu));
+  uint32_t* v0 = cv.take<uint32_t>(size_t(nu));
+  uint32_t* v1 = cv.take<uint32_t>(size_t(nu));
+  hipLaunchKernelGGL(k_upos_keys, dim3(grid_for(nu, 256, 1 << 16)), dim3(256), 0, st,
+                     (const int*)R->lucol, nu, k0, v0);
+  int res = 0;
+  rc = radix_sort_pairs(k0, k1, v0, v1, nu, bits_for_count(n), st, &res, cv);
+  if (!rc) {
+    hipLaunchKernelGGL(k_vptr, dim3(grid_for(int64_t(n) + 1, 256, 1 << 16)), dim3(256), 0, st,
+                       (const uint64_t*)(res ? k1 : k0), nu, n, R->vptr);
+    hipLaunchKernelGGL(k_u32_to_i32_64, dim3(grid_for(nu, 256, 1 << 16)), dim3(256), 0, st,
+                       (const uint32_t*)(res ? v1 : v0), nu, R->vpos);
+    hipError_t le = hipGetLastError();
+    if (le == hipSuccess) le = hipStreamSynchronize(st);
+    if (le != hipSuccess) rc = fail(RAPTR_ERR_DEVICE, "union index: %s", hipGetErrorString(le));
+  }
+  (void)hipStreamSynchronize(st);
+  (void)hipFree(tmp);
+  if (rc) return rc;
+  // the most waves per group that fit LDS with one region per wave, at
+  // most 4
+  R->fb_waves = fused_lds_bytes(R, 4, false) <= kLdsMax ? 4 : 2;
   return RAPTR_OK;
 }
 
@@ -2605,6 +3099,7 @@ int create_plan(raptr_recon** out, int m, int n, int64_t nnz, const int64_t* d_r
       R->fwd_kind = 2;
       R->fwd_lanes = 16;  // the fastest shape of the full-size workload
       R->fwd_unroll = 4;
+      if ((rc = build_union_index(R, st))) return bail(rc);
     }
   }
   // single-frame row-blocked plans of >= 8 row blocks (one or more per
@@ -2981,6 +3476,44 @@ int raptr_recon_get_backward(const raptr_recon* R, int* staged, int* lanes_per_c
   if (staged) *staged = R->bwd_lds;
   if (lanes_per_column) *lanes_per_column = R->bwd_lpc;
   if (workgroups) *workgroups = R->bwd_wgs;
+  return RAPTR_OK;
+}
+
+int raptr_recon_set_fused(raptr_recon* R, int on, int waves, int lanes_per_row,
+                          int lanes_per_column) {
+  if (!R) return fail(RAPTR_ERR_INVALID_ARG, "null plan");
+  if (on < 0 || on > 2) return fail(RAPTR_ERR_INVALID_ARG, "on must be 0, 1 or 2");
+  if (on && !R->cpart)
+    return fail(RAPTR_ERR_INVALID_ARG,
+                "no fused pass (single-frame blocked plans with column unions that fit LDS)");
+  if ((waves != 2 && waves != 4 && waves != 8) || (waves == 8 && on == 1))
+    return fail(RAPTR_ERR_INVALID_ARG, "waves must be 2 or 4 (or 8 with a shared region)");
+  if (lanes_per_row != 16 && lanes_per_row != 32 && lanes_per_row != 64)
+    return fail(RAPTR_ERR_INVALID_ARG, "lanes per row must be 16, 32 or 64");
+  if (lanes_per_column != 16 && lanes_per_column != 32 && lanes_per_column != 64)
+    return fail(RAPTR_ERR_INVALID_ARG, "lanes per column must be 16, 32 or 64");
+  if (R->cpart && on && fused_lds_bytes(R, waves, on == 2) > kLdsMax)
+    return fail(RAPTR_ERR_INVALID_ARG, "%d waves need %zu B of LDS (> 160 KiB)", waves,
+                fused_lds_bytes(R, waves, on == 2));
+  const int nf = on;
+  if (R->fused != nf || R->fb_waves != waves || R->fb_lanes != lanes_per_row ||
+      R->fb_lpc != lanes_per_column)
+    clear_graphs(R);
+  R->fused = nf;
+  R->fb_waves = waves;
+  R->fb_lanes = lanes_per_row;
+  R->fb_lpc = lanes_per_column;
+  return RAPTR_OK;
+}
+
+int raptr_recon_get_fused(const raptr_recon* R, int* available, int* on, int* waves,
+                          int* lanes_per_row, int* lanes_per_column) {
+  if (!R) return fail(RAPTR_ERR_INVALID_ARG, "null plan");
+  if (available) *available = R->cpart ? 1 : 0;
+  if (on) *on = R->fused;
+  if (waves) *waves = R->fb_waves;
+  if (lanes_per_row) *lanes_per_row = R->fb_lanes;
+  if (lanes_per_column) *lanes_per_column = R->fb_lpc;
   return RAPTR_OK;
 }
 

@@ -153,6 +153,16 @@ struct raptr_recon {
   uint16_t* lcol = nullptr;
   int lmax = 0;                // largest union
   int64_t lnu = 0;             // union entries of all groups
+  // fused single-frame pass (k_fb1l + k_fb1_reduce): the groups' partial
+  // column sums over their unions (cpart) and the unions' positions by
+  // column (vptr: n + 1 offsets, vpos: positions ascending per column)
+  float* cpart = nullptr;
+  int64_t* vptr = nullptr;
+  int* vpos = nullptr;
+  int fused = 0;      // single-frame iterations run the fused pass
+  int fb_waves = 4;   // waves per row group (2 or 4)
+  int fb_lanes = 32;  // lanes per row (16, 32 or 64)
+  int fb_lpc = 64;    // lanes per column of the reduce (16, 32 or 64)
   int64_t ngf = 0, ngc = 0, gfnnz = 0, gbnnz = 0;
   int64_t* gfp = nullptr;  // ngf + 1
   int* gfcol = nullptr;

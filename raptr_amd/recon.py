@@ -95,6 +95,10 @@ class Reconstructor:
                "row_ordered": bool(capi.lib().raptr_recon_row_ordered(self.handle)),
                "compact_rows": capi.lib().raptr_recon_get_compact_rows(self.handle),
                "fwd_kind": self.forward_kind()[0]}
+        fz = self.fused()
+        out.update({"fused": fz["on"], "fused_available": fz["available"],
+                    "fused_waves": fz["waves"], "fused_lanes_per_row": fz["lanes_per_row"],
+                    "fused_lanes_per_column": fz["lanes_per_column"]})
         gi = self.grouped()
         if gi["group"]:
             out.update({"grouped64": gi["on"], "group": gi["group"],
@@ -144,6 +148,31 @@ class Reconstructor:
         k, a = C.c_int(), C.c_int()
         capi.call("raptr_recon_get_forward_kind", self.handle, C.byref(k), C.byref(a), None)
         return k.value, [j for j in range(3) if a.value >> j & 1]
+
+    def set_fused(self, on, waves=None, lanes_per_row=None, lanes_per_column=None):
+        """Single-frame iterations in one pass over the matrix (forward,
+        ratio and backward per group of 64 plan rows, then a fixed-order
+        column reduce).  on: False / 0 off; True / 1 one column-sum region
+        per wave; 2 one region per workgroup, the waves adding in turn.
+        Waves per group 2 or 4 (or 8 with a shared region), lanes per row
+        16, 32 or 64, reduce lanes per column 16, 32 or 64 (None: keep the
+        plan's)."""
+        mode = (1 if on else 0) if isinstance(on, bool) else int(on)
+        cur = self.fused()
+        waves = cur["waves"] if waves is None else waves
+        lanes_per_row = cur["lanes_per_row"] if lanes_per_row is None else lanes_per_row
+        lanes_per_column = cur["lanes_per_column"] if lanes_per_column is None else lanes_per_column
+        capi.call("raptr_recon_set_fused", self.handle, mode, int(waves),
+                  int(lanes_per_row), int(lanes_per_column))
+
+    def fused(self):
+        """{"available", "on", "waves", "lanes_per_row", "lanes_per_column"}
+        of the fused pass."""
+        a, o, w, r, l = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        capi.call("raptr_recon_get_fused", self.handle, C.byref(a), C.byref(o), C.byref(w),
+                  C.byref(r), C.byref(l))
+        return {"available": a.value, "on": o.value, "waves": w.value,
+                "lanes_per_row": r.value, "lanes_per_column": l.value}
 
     def forward_union_entries(self):
         """Total size of the LDS-staged forward's column unions (0: none)."""
@@ -264,6 +293,23 @@ class Reconstructor:
             self.set_backward(st, lpc, wgs)
         self.set_config(fl, fu, u)
         self.set_blocks_per_launch(bpl)
+        fz_cfg = (0, 0, 0, 0)
+        if nframes == 1 and self.fused()["available"]:
+            best_fz = None
+            for fmode in (1, 2):
+                for fw in ((2, 4) if fmode == 1 else (2, 4, 8)):
+                    for flr in (16, 32, 64):
+                        try:
+                            self.set_fused(fmode, fw, flr, 64)
+                        except capi.RaptrError:  # too little LDS for that many waves
+                            continue
+                        ms = timed()
+                        if best_fz is None or ms < best_fz[1]:
+                            best_fz = ((fmode, fw, flr, 64), ms)
+            self.set_fused(False)
+            if best_fz is not None and best_fz[1] < timed():
+                fz_cfg = best_fz[0]
+                self.set_fused(fz_cfg[0], *fz_cfg[1:])
         bc = 0
         if nframes == 64:
             # the 64-frame kernels with in-row broadcast of values and indices
@@ -289,7 +335,9 @@ class Reconstructor:
                "bwd_unroll": u, "bwd_staged": st,
                "bwd_lanes_per_column": lpc, "bwd_workgroups": wgs, "bwd_blocks_per_launch": bpl,
                "broadcast64": bc, "grouped64": g_on, "grouped_unroll": g_u,
-               "grouped_kind": g_p, "xcd_blocks": xb}
+               "grouped_kind": g_p, "xcd_blocks": xb, "fused": fz_cfg[0],
+               "fused_waves": fz_cfg[1], "fused_lanes_per_row": fz_cfg[2],
+               "fused_lanes_per_column": fz_cfg[3]}
         return cfg, ms_best
 
     # rows each XCD walks per launch of kind 3's backward: windows of about

@@ -92,6 +92,30 @@ def test_full_plain_mlem_vs_fp64(full, oracle):
     print(f"plain f32: rel L2 {rel:.2e}, max voxel {mx:.2e}")
     assert rel <= 1e-4
     assert abs(float(np.sum(x, dtype=np.float64)) - 1.0) <= 1e-5
+    F["cache"]["plain_x"] = x
+
+
+def test_full_plain_fused_vs_fp64(full, oracle):
+    """The fused single-frame iteration (one pass over the matrix per
+    iteration) over 50 iterations: within 1e-4 of the FP64 oracle, within
+    2e-6 of the separate passes, and the same bits on a second run."""
+    from raptr_amd.recon import Reconstructor
+    F = full
+    n = F["grid"].nvox
+    R = Reconstructor(F["A"], F["sens"], "f32", 1, row_order=F["order"])
+    assert R.fused()["available"] == 1
+    R.set_fused(True)
+    y, x0 = F["Y"][:, 0].copy(), np.full(n, 1.0 / n, np.float32)
+    x = R.mlem(y, x0, ITERS)
+    np.testing.assert_array_equal(x, R.mlem(y, x0, ITERS))
+    R.close()
+    x64 = _oracle(oracle, F, 0, ITERS)
+    rel, mx = _rel(x, x64), _maxerr(x, x64)
+    print(f"plain fused f32: rel L2 {rel:.2e}, max voxel {mx:.2e}")
+    assert rel <= 1e-4
+    assert abs(float(np.sum(x, dtype=np.float64)) - 1.0) <= 1e-5
+    if "plain_x" in F["cache"]:
+        assert _rel(x, F["cache"]["plain_x"].astype(np.float64)) <= 2e-6
 
 
 def test_full_batched64_vs_fp64_and_plain(full, oracle):

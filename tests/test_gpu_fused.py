@@ -1,0 +1,216 @@
+"""GPU tests of the fused single-frame iteration (raptr_recon_set_fused):
+one pass over the matrix per iteration (forward, ratio and backward per
+group of 64 plan rows with per-wave column sums in LDS, then a fixed-order
+reduce of the groups' partial sums).  Checked against the FP64 oracle
+recurrence, against the separate forward and backward passes, and against
+itself run to run (bit for bit); edge cases: empty row groups, a ragged last
+group, a one-row plan, rows longer than the two register-held steps, zero
+counts, unions too large for LDS."""
+import numpy as np
+import pytest
+
+from test_gpu_recon import _device_csr, _oracle_mlem, _rel, small_system  # noqa: F401
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(o, w, r, l) for o in (1, 2) for w in ((2, 4) if o == 1 else (2, 4, 8))
+          for r in (16, 32, 64) for l in ((16, 32, 64) if r == 32 else (64,))]
+
+
+def test_fused_small_system(gpu, oracle, small_system):
+    """Small workload in locality row order: the fused pass is built; every
+    shape (a column-sum region per wave or one shared in turn; waves, lanes
+    per row, reduce lanes per column) is within 1e-4 of the FP64 oracle
+    after 5 iterations, within 1e-6 of the separate passes, and repeats bit
+    for bit; graph replay equals eager."""
+    from raptr_amd.device import Stream
+    from raptr_amd.recon import Reconstructor
+    S = small_system
+    n = S["grid"].nvox
+    x0 = np.full(n, 1.0 / n, np.float32)
+    x64 = _oracle_mlem(oracle, S, S["y"], 5)
+    R = Reconstructor(S["A"], S["sens"], "f32", 1, layout="blocked", row_order=S["setup"])
+    fz = R.fused()
+    assert fz["available"] == 1
+    assert fz["on"] == 0  # off until selected
+    sep = R.mlem(S["y"], x0, 5)
+    assert _rel(sep, x64) <= 1e-4
+    st = Stream()
+    for o, w, lr, rw in SHAPES:
+        R.set_fused(o, w, lr, rw)
+        assert R.fused() == {"available": 1, "on": o, "waves": w, "lanes_per_row": lr,
+                             "lanes_per_column": rw}
+        a = R.mlem(S["y"], x0, 5)
+        b = R.mlem(S["y"], x0, 5)
+        np.testing.assert_array_equal(a, b)
+        assert _rel(a, x64) <= 1e-4, (o, w, lr, rw)
+        assert _rel(a, sep) <= 1e-6, (o, w, lr, rw)
+        assert abs(float(a.astype(np.float64).sum()) - 1.0) < 1e-5
+        g = R.mlem(S["y"], x0, 5, use_graph=True, stream=st.handle)
+        np.testing.assert_array_equal(a, g)
+    R.set_fused(False)
+    np.testing.assert_array_equal(R.mlem(S["y"], x0, 5), sep)
+
+
+@pytest.mark.parametrize("vt", ["f16", "bf16", "u16"])
+def test_fused_16bit_values(gpu, small_system, vt):
+    """16-bit values with per-row scales: the fused pass computes the same
+    recurrence as the separate passes on the same stored values (within
+    1e-5 after 5 iterations) and repeats bit for bit."""
+    from raptr_amd.recon import Reconstructor
+    S = small_system
+    n = S["grid"].nvox
+    x0 = np.full(n, 1.0 / n, np.float32)
+    R = Reconstructor(S["A"], S["sens"], vt, 1, layout="blocked", row_order=S["setup"])
+    sep = R.mlem(S["y"], x0, 5)
+    for o in (1, 2):
+        R.set_fused(o, 4, 32, 64)
+        a = R.mlem(S["y"], x0, 5)
+        np.testing.assert_array_equal(a, R.mlem(S["y"], x0, 5))
+        assert _rel(a, sep) <= 1e-5
+
+
+def _long_row_csr(n, m, seed, long_rows=True, empty=None, window=5000):
+    """Host CSR whose groups of 64 consecutive rows draw their columns from
+    one window of `window` voxels; lengths 1..500, with rows of 513..2,000
+    entries (past the register-held steps of every shape) where long_rows;
+    rows in `empty` have no entries."""
+    rng = np.random.default_rng(seed)
+    lens = rng.integers(1, 500, m)
+    if long_rows:
+        k = max(1, m // 10)
+        lens[rng.choice(m, k, replace=False)] = rng.integers(513, 2000, k)
+        lens[m - 1] = 1999  # the last row of the ragged last group
+    if empty is not None:
+        lens[empty] = 0
+    cols = []
+    for r, L in enumerate(lens):
+        if r % 64 == 0:
+            w0 = int(rng.integers(0, n - window + 1))
+        cols.append(np.sort(rng.choice(window, L, replace=False)) + w0)
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    col = np.concatenate(cols).astype(np.int32)
+    val = rng.random(len(col)).astype(np.float32) + 0.01
+    return rp, col, val
+
+
+@pytest.mark.parametrize("case", ["long_rows", "empty_group", "one_row"])
+def test_fused_edges(gpu, oracle, case):
+    """Rows longer than the register-held steps (up to 2,000 entries) and a
+    ragged last group (m = 785); four empty row groups; a one-row plan;
+    zero counts on some rows.  Every shape: 3 iterations within 1e-5 of the
+    FP64 oracle recurrence on the same FP32 matrix, bit for bit run to run,
+    and within 1e-6 of the separate passes."""
+    from raptr_amd.device import DeviceArray
+    from raptr_amd.recon import Reconstructor
+    n = 20000
+    if case == "long_rows":
+        rp, col, val = _long_row_csr(n, 785, 21)
+    elif case == "empty_group":
+        rp, col, val = _long_row_csr(n, 785, 22, empty=slice(256, 512))
+    else:
+        rp, col, val = _long_row_csr(n, 1, 23)
+    m = len(rp) - 1
+    rng = np.random.default_rng(7)
+    y = (rng.random(m) * 50).astype(np.float32)
+    y[rng.random(m) < 0.1] = 0.0
+    if m == 1:
+        y[0] = 5.0
+    R = Reconstructor(_device_csr(rp, col, val, n), DeviceArray.from_numpy(np.ones(n)), "f32", 1,
+                      layout="blocked")
+    assert R.fused()["available"] == 1
+    x0 = np.full(n, 1.0 / n, np.float32)
+    x64 = oracle.mlem_f64(rp, col, val.astype(np.float64), y, np.ones(n), x0, 3)
+    sep = R.mlem(y, x0, 3)
+    assert _rel(sep, x64) <= 1e-5
+    for o, w, lr, rw in [(1, 2, 16, 16), (1, 4, 16, 64), (1, 2, 32, 32), (1, 4, 64, 64),
+                         (2, 4, 32, 16), (2, 2, 16, 64), (2, 4, 64, 32), (2, 8, 64, 64)]:
+        R.set_fused(o, w, lr, rw)
+        a = R.mlem(y, x0, 3)
+        np.testing.assert_array_equal(a, R.mlem(y, x0, 3))
+        assert _rel(a, x64) <= 1e-5, (o, w, lr, rw)
+        assert _rel(a, sep) <= 1e-6, (o, w, lr, rw)
+        if case == "one_row":  # voxels of no row keep x * 0 = 0
+            assert np.all(a[np.setdiff1d(np.arange(n), col)] == 0)
+
+
+def test_fused_lds_limits(gpu):
+    """Unions of about 9,900 voxels: with a column-sum region per wave 2
+    waves per group fit 160 KiB of LDS (520 + 3 x 4 bytes per union voxel),
+    4 waves do not and are refused; one shared region fits with 4 waves;
+    bad shapes are refused and leave the plan's shape."""
+    from raptr_amd import RaptrError
+    from raptr_amd.device import DeviceArray
+    from raptr_amd.recon import Reconstructor
+    n = 20000
+    rp, col, val = _long_row_csr(n, 640, 24, window=10000)
+    R = Reconstructor(_device_csr(rp, col, val, n), DeviceArray.from_numpy(np.ones(n)), "f32", 1,
+                      layout="blocked")
+    assert R.fused()["available"] == 1
+    assert R.fused()["waves"] == 2  # the plan's default: the most waves that fit
+    R.set_fused(2, 4, 16, 64)
+    R.set_fused(2, 8, 64, 64)
+    R.set_fused(True, 2, 32, 64)
+    with pytest.raises(RaptrError):
+        R.set_fused(True, 4, 16, 64)
+    for o, w, lr, rw in [(1, 3, 16, 64), (1, 8, 16, 64), (1, 2, 8, 64), (1, 2, 128, 64),
+                         (1, 2, 16, 8), (1, 2, 16, 128), (1, 0, 16, 64), (3, 2, 16, 64),
+                         (-1, 2, 16, 64)]:
+        with pytest.raises(RaptrError):
+            R.set_fused(o, w, lr, rw)
+    assert R.fused() == {"available": 1, "on": 1, "waves": 2, "lanes_per_row": 32,
+                         "lanes_per_column": 64}
+    R.set_fused(2, 8, 64, 64)
+    R.set_fused(False)  # keeps the shape
+    with pytest.raises(RaptrError):
+        R.set_fused(True)  # 8 waves need the shared region
+    assert R.fused()["on"] == 0
+    x0 = np.full(n, 1.0 / n, np.float32)
+    y = np.full(len(rp) - 1, 3.0, np.float32)
+    a = R.mlem(y, x0, 2)
+    R.set_fused(False)
+    assert _rel(a, R.mlem(y, x0, 2)) <= 1e-6
+
+
+def test_fused_absent(gpu, small_system):
+    """No fused pass without column unions (64-frame plans, and
+    single-frame plans whose groups touch more than 16,384 voxels) or when
+    a row repeats a column (its column sums add each entry with one
+    unguarded read-add-write): set_fused(on) is refused, off is accepted;
+    the separate passes still add the repeated entries."""
+    from raptr_amd import RaptrError
+    from raptr_amd.device import DeviceArray
+    from raptr_amd.recon import Reconstructor
+    from test_gpu_recon import _wide_csr
+    S = small_system
+    R64 = Reconstructor(S["A"], S["sens"], "f32", 64, row_order=S["setup"], grid=S["grid"],
+                        group=16)
+    assert R64.fused()["available"] == 0
+    with pytest.raises(RaptrError):
+        R64.set_fused(True, 4, 16, 64)
+    R64.set_fused(False, 4, 16, 64)
+    n = 200000
+    rp, col, val = _wide_csr(n, m=600)
+    R = Reconstructor(_device_csr(rp, col, val, n), DeviceArray.from_numpy(np.ones(n)), "f32", 1,
+                      layout="blocked")
+    assert R.fused()["available"] == 0
+    with pytest.raises(RaptrError):
+        R.set_fused(True, 2, 16, 64)
+    n = 20000
+    rp, col, val = _long_row_csr(n, 300, 25)
+    col = col.copy()
+    col[rp[100] + 1] = col[rp[100]]  # row 100 holds its first column twice
+    R = Reconstructor(_device_csr(rp, col, val, n), DeviceArray.from_numpy(np.ones(n)), "f32", 1,
+                      layout="blocked")
+    assert R.forward_kind()[0] == 2 and R.fused()["available"] == 0
+    with pytest.raises(RaptrError):
+        R.set_fused(True, 2, 64, 64)
+    from raptr_amd import _capi as capi
+    x = np.random.default_rng(3).random(n).astype(np.float32)
+    m = len(rp) - 1
+    y = np.full(m, 2.0, np.float32)
+    dx, dy, de = DeviceArray.from_numpy(x), DeviceArray.from_numpy(y), DeviceArray(m, np.float32)
+    capi.call("raptr_recon_forward", R.handle, 1, dx.ptr, dy.ptr, de.ptr, None)
+    yt = np.array([np.dot(val[rp[i]:rp[i + 1]].astype(np.float64), x[col[rp[i]:rp[i + 1]]])
+                   for i in range(m)])
+    np.testing.assert_allclose(de.numpy(), y / yt, rtol=1e-5)
