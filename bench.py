@@ -55,6 +55,11 @@ MEASURED_TRAFFIC = {
     # k_bwd1_xreduce, moves 0.006 GB more)
     ("k_bwd1x", 2, 69, "y", "r16", 32): 2 * 5467831 * 1024 + 10202 * 1024,
     ("k_bwd1x", 2, 368, "y", "r16", 32): 2 * 5855002 * 1024 + 8618 * 1024,
+    # the fused iteration's products kernel (kernel, mode, waves, lanes per
+    # row, row order) and its per-XCD column reduce (kernel, lanes per
+    # column, row order); one launch each per iteration
+    ("k_fb1l", 2, 4, 64, "y"): 2 * 5035683 * 1024 + 465498 * 1024,
+    ("k_fb1_reduce8", 16, "y"): 2 * 1028086 * 1024 + 6883 * 1024,
 }
 
 
@@ -375,6 +380,56 @@ def main():
         dom = {"kernel": bwd_name, "ms": ms_b, "bytes": bytes_b, "gbs": gbs_b, "launches": nl_b,
                "key": key_b, "stored": stored_b}
     traffic = MEASURED_TRAFFIC.get(dom["key"]) if dom["key"] else None
+    # the fused iteration (the plain plan's default where built): the
+    # products kernel k_fb1l reads the matrix once for both products; its
+    # algorithmic bytes are those of the two SpMVs it replaces
+    fused = None
+    if info.get("fused"):
+        def fb1():
+            capi.call("raptr_recon_fused_pass", R.handle, dx.ptr, dy.ptr, 1, st)
+
+        def fb2():
+            capi.call("raptr_recon_fused_pass", R.handle, dx.ptr, dy.ptr, 2, st)
+
+        fb1(), fb2()
+        ms_fb = timed(lambda: [fb1() for _ in range(reps)]) / reps
+        ms_red = timed(lambda: [fb2() for _ in range(reps)]) / reps
+        nu = R.forward_union_entries()
+        spmv = 8 * A.nnz + 4 * (m + 1) + 4 * m + 4 * n  # one direction
+        bytes_fb = 2 * spmv
+        # stored: 6 bytes per padded entry (FP32 value, 16-bit union rank),
+        # row pointers, y, e, per union entry its column and x, the partial
+        # sums written
+        stored_fb = (6 * info["padded_nnz"] + 8 * (m + 1) + 8 * m + 8 * nu + 4 * nu
+                     + 8 * (-(-m // 64) + 1))
+        it_bytes = 2 * 8 * A.nnz + 8 * (m + 1) + 4 * 3 * m + 4 * 6 * n  # one iteration
+        ms_it = modes["plain"]["ms_per_iteration"]
+        key = ("k_fb1l", info["fused"], info["fused_waves"], info["fused_lanes_per_row"],
+               cfg_p["row_order"])
+        fused = {"kernel": f"k_fb1l<f32, {info['fused_waves']} waves, "
+                           f"{info['fused_lanes_per_row']} lanes per row, "
+                           + ("one shared column-sum region" if info["fused"] == 2 else
+                              "a column-sum region per wave")
+                           + "> (forward + ratio + backward partial sums over each 64-row "
+                             "group's column union, one pass over the matrix)",
+                 "ms": ms_fb, "bytes": bytes_fb, "gbs": bytes_fb / (ms_fb * 1e-3) / 1e9,
+                 "stored": stored_fb, "key": key,
+                 "reduce": {"kernel": ("k_fb1_reduce8 + k_bwd1_xreduce (group partials per "
+                                       "column in per-XCD group ranges, then the update)"
+                                       if info["fused_reduce_per_xcd"] else
+                                       "k_fb1_reduce (group partials per column, update)"),
+                            "ms": round(ms_red, 4),
+                            "algorithmic_bytes": int(4 * 2 * nu + 8 * (n + 1) + 4 * 4 * n),
+                            "traffic": MEASURED_TRAFFIC.get(("k_fb1_reduce8",
+                                                             info["fused_lanes_per_column"],
+                                                             cfg_p["row_order"]))},
+                 "iteration": {"ms": ms_it, "algorithmic_bytes": it_bytes,
+                               "achieved": round(it_bytes / (ms_it * 1e-3) / 1e9, 1),
+                               "frac": round(it_bytes / (ms_it * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                               "bytes_basis": "one MLEM iteration by the survey's formula: "
+                                              "the matrix counted once per product"}}
+        log(f"fused products {ms_fb:.3f} ms ({fused['gbs']:.0f} GB/s of the two products' "
+            f"bytes), reduce {ms_red:.3f} ms, iteration {ms_it:.3f} ms")
 
     # ------------------------ FP64 host recurrence: errors + CPU baseline
     cpu = None
@@ -479,6 +534,18 @@ def main():
         serve = bench_serving(log, args.serving_requests, args.serving_rate, args.serving_slots,
                               args.serving_chunk)
 
+    separate = None
+    if fused is not None:  # the plain path's dominant kernel is the fused one
+        separate = {"forward": {"kernel": fwd_name, "ms": round(ms_f, 4),
+                                "GBps": round(gbs_f, 1)},
+                    "backward": {"kernel": bwd_name, "ms": round(ms_b, 4),
+                                 "GBps": round(gbs_b, 1), "frac": round(gbs_b / HBM_PEAK_GBS, 4)},
+                    "ms_per_iteration_measured_by": "autotune (the separate passes are "
+                                                    "slower; not run in the timed region)"}
+        traffic = MEASURED_TRAFFIC.get(fused["key"])
+        dom = {"kernel": fused["kernel"], "ms": fused["ms"], "bytes": fused["bytes"],
+               "gbs": fused["gbs"], "launches": 1, "key": fused["key"],
+               "stored": fused["stored"]}
     plain_roofline = {"bound": "hbm", "kernel": dom["kernel"], "achieved": round(dom["gbs"], 1),
                  "peak": HBM_PEAK_GBS, "unit": "GB/s",
                  "frac": round(dom["gbs"] / HBM_PEAK_GBS, 4),
@@ -486,7 +553,11 @@ def main():
                  "traffic_source": ("rocprofv3 FETCH_SIZE x2.00 + WRITE_SIZE x1.00 per launch, "
                                     "factors from a 2 GiB streaming copy") if traffic else None,
                  "algorithmic_bytes": int(dom["bytes"] / dom["launches"]),
-                 "bytes_basis": "4-byte value + 4-byte index per stored entry",
+                 "bytes_basis": ("both products' bytes (8 per matrix entry each, the "
+                                 "survey's SpMV formula twice): the kernel reads the matrix once "
+                                 "for two products, so frac can exceed 1; traffic/peak is the "
+                                 "HBM use" if fused is not None else
+                                 "4-byte value + 4-byte index per stored entry"),
                  "stored_bytes": (int(dom["stored"] / dom["launches"])
                                   if dom["stored"] else None),
                  "stored_GBps": (round(dom["stored"] / (dom["ms"] * 1e-3) / 1e9, 1)
@@ -494,6 +565,13 @@ def main():
                  "avg_launch_ms": round(dom["ms"] / dom["launches"], 4),
                  "launches_per_pass": dom["launches"], "pass_ms": round(dom["ms"], 4),
                  "copy_GBps_measured": copy["GBps"]}
+    if traffic:
+        plain_roofline["traffic_frac"] = round(traffic / (dom["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                               4)
+    if fused is not None:
+        plain_roofline["reduce"] = fused["reduce"]
+        plain_roofline["iteration"] = fused["iteration"]
+        plain_roofline["separate_passes"] = separate
     # the roofline line describes the headline mode's dominant kernel (the
     # batched backward when a batched mode is the value); the plain path's is
     # kept beside it

@@ -514,6 +514,23 @@ int raptr_recon_set_fused(raptr_recon* plan, int on, int waves, int lanes_per_ro
                           int lanes_per_column);
 int raptr_recon_get_fused(const raptr_recon* plan, int* available, int* on, int* waves,
                           int* lanes_per_row, int* lanes_per_column);
+/* Column reduce of the fused iteration: per_xcd = 1 (the default) splits
+ * the groups into 8 ranges, one per XCD (workgroup i runs on XCD i % 8),
+ * so each XCD's L2 holds an eighth of the partial sums, and adds the 8
+ * per-XCD sums of a column in XCD order; 0 adds all of a column's group
+ * partials in one pass.  Both are fixed-order; their sums differ in
+ * rounding.  get: the setting (-1 for a null plan). */
+int raptr_recon_set_fused_reduce(raptr_recon* plan, int per_xcd);
+int raptr_recon_get_fused_reduce(const raptr_recon* plan);
+/* The fused iteration's kernels alone (timing, tests), as the plan is set:
+ * part 0 the products and the column reduce with the update into the
+ * plan's internal image buffer (d_y in the caller's row order, as
+ * raptr_recon_forward); 1 the products alone (k_fb1l: ratios and the
+ * groups' partial sums; d_y in the plan's row order); 2 the column reduce
+ * and update alone (from the last products; d_y unused).  No counterpart
+ * in the reference, whose iteration is the two csrmv calls. */
+int raptr_recon_fused_pass(raptr_recon* plan, const float* d_x, const float* d_y, int part,
+                           raptr_stream_t stream);
 /* Row blocks per backward launch of the blocked layout (1..64): a wave
  * walks its column through that many consecutive blocks, keeping the sum in
  * registers (longer waves, fewer launches). */

@@ -1702,6 +1702,59 @@ inline int bits_for_count(int64_t count) {
   return b;
 }
 
+// Per-XCD form of the same sums: workgroup i runs on XCD x = i % 8 and
+// adds, for its columns, the partials of groups [ng x / 8, ng (x+1) / 8)
+// only (the positions of column j in [vb[x-1][j], vb[x][j]), with vb[-1] =
+// vptr[j] and vb[7] = vptr[j+1]), so an XCD's L2 holds the lines of an
+// eighth of the groups; c8[x][j] takes the XCD's sum, and k_bwd1_xreduce
+// adds the 8 in XCD order and applies the update.
+template <int LPC, int U>
+__global__ __launch_bounds__(256) void k_fb1_reduce8(int n, const int64_t* __restrict__ vptr,
+                                                     const int64_t* __restrict__ vb,
+                                                     const int* __restrict__ vpos,
+                                                     const float* __restrict__ cpart,
+                                                     float* __restrict__ c8) {
+  const int lane = int(threadIdx.x) % LPC;
+  const int xcd = int(blockIdx.x % 8);
+  const int64_t j = int64_t(blockIdx.x / 8) * (256 / LPC) + int(threadIdx.x) / LPC;
+  if (j >= n) return;  // the column's LPC lanes leave together
+  const int64_t k1 = xcd == 7 ? vptr[j + 1] : vb[int64_t(xcd) * n + j];
+  int64_t k = (xcd == 0 ? vptr[j] : vb[int64_t(xcd - 1) * n + j]) + lane;
+  float acc = 0.0f;
+  for (; k + int64_t(LPC) * (U - 1) < k1; k += int64_t(LPC) * U) {
+    int p[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) p[u] = vpos[k + LPC * u];
+    float g[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) g[u] = cpart[p[u]];
+#pragma unroll
+    for (int u = 0; u < U; ++u) acc += g[u];
+  }
+  for (; k < k1; k += LPC) acc += cpart[vpos[k]];
+#pragma unroll
+  for (int off = LPC / 2; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, LPC);
+  if (lane == 0) c8[int64_t(xcd) * n + j] = acc;
+}
+
+// vb[x-1][j] = first position of column j (in [vptr[j], vptr[j+1])) that
+// belongs to a group of XCD range x or later, x = 1 .. 7
+__global__ void k_vbounds(int n, const int64_t* __restrict__ vptr, const int* __restrict__ vpos,
+                          const int64_t* __restrict__ up, int64_t ng, int64_t* __restrict__ vb) {
+  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < int64_t(7) * n;
+       t += int64_t(gridDim.x) * blockDim.x) {
+    const int x = int(t / n) + 1;
+    const int64_t j = t % n;
+    const int64_t pstart = up[ng * x / 8];
+    int64_t lo = vptr[j], hi = vptr[j + 1];
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) / 2;
+      if (int64_t(vpos[mid]) < pstart) lo = mid + 1; else hi = mid;
+    }
+    vb[t] = lo;
+  }
+}
+
 // (column, position) pairs of the unions, for the column-major index
 __global__ void k_upos_keys(const int* __restrict__ ucol, int64_t nu, uint64_t* __restrict__ keys,
                             uint32_t* __restrict__ idx) {
@@ -2364,8 +2417,10 @@ constexpr size_t kLdsMax = 160 * 1024;
 
 // single frame: forward, ratio and backward in one pass (k_fb1l), then the
 // fixed-order column reduce + update
+// part: 0 both kernels, 1 the products (k_fb1l) only, 2 the column reduce
+// and update only
 int launch_fused1(const raptr_recon* R, const float* x, const float* y, hipStream_t st,
-                  int* nb_out) {
+                  int* nb_out, int part = 0) {
   if (!R->cpart || !R->vpos) return fail(RAPTR_ERR_INVALID_ARG, "plan has no fused pass");
   const int64_t m = R->m;
   const int n = R->n;
@@ -2373,7 +2428,7 @@ int launch_fused1(const raptr_recon* R, const float* x, const float* y, hipStrea
   const bool SHR = R->fused == 2;
   const size_t shm = fused_lds_bytes(R, W, SHR);
   if (shm > kLdsMax) return fail(RAPTR_ERR_INVALID_ARG, "fused pass: %zu B of LDS", shm);
-  if (m > 0) {
+  if (m > 0 && part != 2) {
     const unsigned g = unsigned((m + kLocRows - 1) / kLocRows);
     bool launched = false;
 #define RAPTR_FB(VV, WW, LL, SS, SH)                                                         \
@@ -2407,7 +2462,26 @@ int launch_fused1(const raptr_recon* R, const float* x, const float* y, hipStrea
                   LR);
     RAPTR_LAUNCH_CHECK();
   }
+  if (part == 1) return RAPTR_OK;
   const int L = R->fb_lpc;
+  if (R->fb_xcd) {
+    const unsigned gg = unsigned(8 * ((int64_t(n) + (256 / L) - 1) / (256 / L)));
+#define RAPTR_FR8(LL)                                                                          \
+  if (L == LL)                                                                                 \
+    hipLaunchKernelGGL((k_fb1_reduce8<LL, 4>), dim3(gg), dim3(256), 0, st, n,                   \
+                       (const int64_t*)R->vptr, (const int64_t*)R->vb, (const int*)R->vpos,    \
+                       (const float*)R->cpart, R->c8);
+    RAPTR_FR8(16) else RAPTR_FR8(32) else RAPTR_FR8(64) else return fail(RAPTR_ERR_INVALID_ARG,
+                                                                      "reduce lanes %d", L);
+#undef RAPTR_FR8
+    RAPTR_LAUNCH_CHECK();
+    const unsigned nr = unsigned((int64_t(n) + 255) / 256);
+    hipLaunchKernelGGL(k_bwd1_xreduce, dim3(nr), dim3(256), 0, st, n, (const float*)R->c8, x,
+                       (const float*)R->s, R->xx, R->partials);
+    RAPTR_LAUNCH_CHECK();
+    if (nb_out) *nb_out = int(nr);
+    return RAPTR_OK;
+  }
   const unsigned nr = unsigned((int64_t(n) + (256 / L) - 1) / (256 / L));
 #define RAPTR_FR(LL)                                                                           \
   if (L == LL)                                                                                 \
@@ -2702,7 +2776,8 @@ int build_union_index(raptr_recon* R, hipStream_t st) {
     if (h) return RAPTR_OK;  // a repeated column: no fused pass
   }
   if ((rc = ralloc(R, &R->vptr, size_t(n) + 1)) || (rc = ralloc(R, &R->vpos, size_t(nu))) ||
-      (rc = ralloc(R, &R->cpart, size_t(nu))))
+      (rc = ralloc(R, &R->vb, size_t(7) * n)) || (rc = ralloc(R, &R->cpart, size_t(nu))) ||
+      (!R->c8 && (rc = ralloc(R, &R->c8, size_t(8) * n))))
     return rc;
   const int64_t ntiles = (nu + 2047) / 2048;
   Sizer z;
@@ -2733,6 +2808,10 @@ int build_union_index(raptr_recon* R, hipStream_t st) {
                        (const uint64_t*)(res ? k1 : k0), nu, n, R->vptr);
     hipLaunchKernelGGL(k_u32_to_i32_64, dim3(grid_for(nu, 256, 1 << 16)), dim3(256), 0, st,
                        (const uint32_t*)(res ? v1 : v0), nu, R->vpos);
+    const int64_t ng = (int64_t(R->m) + kLocRows - 1) / kLocRows;
+    hipLaunchKernelGGL(k_vbounds, dim3(grid_for(int64_t(7) * n, 256, 1 << 16)), dim3(256), 0, st,
+                       n, (const int64_t*)R->vptr, (const int*)R->vpos, (const int64_t*)R->lup,
+                       ng, R->vb);
     hipError_t le = hipGetLastError();
     if (le == hipSuccess) le = hipStreamSynchronize(st);
     if (le != hipSuccess) rc = fail(RAPTR_ERR_DEVICE, "union index: %s", hipGetErrorString(le));
@@ -2740,9 +2819,14 @@ int build_union_index(raptr_recon* R, hipStream_t st) {
   (void)hipStreamSynchronize(st);
   (void)hipFree(tmp);
   if (rc) return rc;
-  // the most waves per group that fit LDS with one region per wave, at
-  // most 4
-  R->fb_waves = fused_lds_bytes(R, 4, false) <= kLdsMax ? 4 : 2;
+  // the default single-frame iteration: one shared column-sum region, 4
+  // waves per group, 64 lanes per row, the per-XCD reduce with 16 lanes per
+  // column (the fastest shape at full size)
+  R->fused = 2;
+  R->fb_waves = 4;
+  R->fb_lanes = 64;
+  R->fb_lpc = 16;
+  R->fb_xcd = 1;
   return RAPTR_OK;
 }
 
@@ -3506,6 +3590,16 @@ int raptr_recon_set_fused(raptr_recon* R, int on, int waves, int lanes_per_row,
   return RAPTR_OK;
 }
 
+int raptr_recon_set_fused_reduce(raptr_recon* R, int per_xcd) {
+  if (!R) return fail(RAPTR_ERR_INVALID_ARG, "null plan");
+  const int v = per_xcd ? 1 : 0;
+  if (R->fb_xcd != v) clear_graphs(R);
+  R->fb_xcd = v;
+  return RAPTR_OK;
+}
+
+int raptr_recon_get_fused_reduce(const raptr_recon* R) { return R ? R->fb_xcd : -1; }
+
 int raptr_recon_get_fused(const raptr_recon* R, int* available, int* on, int* waves,
                           int* lanes_per_row, int* lanes_per_column) {
   if (!R) return fail(RAPTR_ERR_INVALID_ARG, "null plan");
@@ -3623,6 +3717,24 @@ int raptr_recon_forward(raptr_recon* R, int nframes, const float* d_x, const flo
     return RAPTR_OK;
   }
   return launch_fwd_any(R, nframes, d_x, d_y, d_e, st);
+}
+
+int raptr_recon_fused_pass(raptr_recon* R, const float* d_x, const float* d_y, int part,
+                           raptr_stream_t stream) {
+  if (!R) return fail(RAPTR_ERR_INVALID_ARG, "null plan");
+  if (!R->cpart || R->fused == 0)
+    return fail(RAPTR_ERR_INVALID_ARG, "the fused iteration is off or absent on this plan");
+  if (part < 0 || part > 2) return fail(RAPTR_ERR_INVALID_ARG, "part must be 0, 1 or 2");
+  if (!d_x || (part != 2 && !d_y)) return fail(RAPTR_ERR_INVALID_ARG, "null buffer");
+  hipStream_t st = S(stream);
+  if (R->rperm && R->m > 0 && part == 0) {
+    const unsigned g = grid_for(int64_t(R->m), 256, 1 << 15);
+    hipLaunchKernelGGL(k_rows_gather, dim3(g), dim3(256), 0, st, d_y, R->yp,
+                       (const int*)R->rperm, int64_t(R->m), 1);
+    RAPTR_LAUNCH_CHECK();
+    d_y = R->yp;
+  }
+  return launch_fused1(R, d_x, d_y, st, nullptr, part);
 }
 
 int raptr_recon_backward(raptr_recon* R, int nframes, const float* d_x, raptr_stream_t stream) {

@@ -159,7 +159,9 @@ struct raptr_recon {
   float* cpart = nullptr;
   int64_t* vptr = nullptr;
   int* vpos = nullptr;
+  int64_t* vb = nullptr;  // 7 x n: per XCD range of groups, where column j's positions start
   int fused = 0;      // single-frame iterations run the fused pass
+  int fb_xcd = 1;     // the reduce in per-XCD group ranges (k_fb1_reduce8)
   int fb_waves = 4;   // waves per row group (2 or 4)
   int fb_lanes = 32;  // lanes per row (16, 32 or 64)
   int fb_lpc = 64;    // lanes per column of the reduce (16, 32 or 64)

@@ -2,12 +2,14 @@
 
     python -m raptr_amd.passes [--frames F] [--value-type f32] [--block-rows R]
         [--order y|slope|angle|angle_slope] [--xcd-blocks B] [--active A] [--reps N]
+        [--separate]
 
 Builds the benchmark's full-size workload (13x13-pixel heads, 180 angles,
 52^3 grid at 2 mm, 30 rays, six-ellipsoid phantom, 2e7 Poisson events), the
 matrix of its measured channels and the FP64 sensitivity, then one plan
 with the given shape (no autotuning), and runs `reps` forward and backward
-passes.  A 2 GiB streaming copy (k_stream_copy) runs first: the known-size
+passes; a single-frame plan runs `reps` fused iterations (its default)
+unless --separate is given.  A 2 GiB streaming copy (k_stream_copy) runs first: the known-size
 dispatch that calibrates the HBM traffic counters.  Progress goes to
 stderr.  Nothing is timed here; rocprofv3 measures the kernels.
 """
@@ -39,6 +41,9 @@ def main(argv=None):
     ap.add_argument("--active", type=int, default=64, choices=(16, 32, 64),
                     help="64 frames: frames each pass computes (an MLEM iteration per rep)")
     ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--separate", action="store_true",
+                    help="one frame: the separate forward and backward passes instead of the "
+                         "fused iteration")
     a = ap.parse_args(argv)
     st_obj = Stream()
     st = st_obj.handle
@@ -66,8 +71,13 @@ def main(argv=None):
     dy = DeviceArray.from_numpy(Y.reshape(-1))
     dx = DeviceArray.from_numpy(np.full(n * F, 1.0 / n, np.float32))
     de = DeviceArray(m * F, np.float32)
+    fused = F == 1 and not a.separate and R.fused()["on"]
+    if F == 1 and a.separate and R.fused()["available"]:
+        R.set_fused(False)
     for r in range(a.reps):
-        if a.active < 64:
+        if fused:
+            R.run(dy, dx, 1, 1, use_graph=False, stream=st)
+        elif a.active < 64:
             R.run(dy, dx, 1, F, use_graph=False, stream=st, active=a.active)
         else:
             capi.call("raptr_recon_forward", R.handle, F, dx.ptr, dy.ptr, de.ptr, st)

@@ -98,7 +98,8 @@ class Reconstructor:
         fz = self.fused()
         out.update({"fused": fz["on"], "fused_available": fz["available"],
                     "fused_waves": fz["waves"], "fused_lanes_per_row": fz["lanes_per_row"],
-                    "fused_lanes_per_column": fz["lanes_per_column"]})
+                    "fused_lanes_per_column": fz["lanes_per_column"],
+                    "fused_reduce_per_xcd": fz["per_xcd"]})
         gi = self.grouped()
         if gi["group"]:
             out.update({"grouped64": gi["on"], "group": gi["group"],
@@ -165,14 +166,20 @@ class Reconstructor:
         capi.call("raptr_recon_set_fused", self.handle, mode, int(waves),
                   int(lanes_per_row), int(lanes_per_column))
 
+    def set_fused_reduce(self, per_xcd):
+        """Fused iteration's column reduce: per XCD group ranges (True, the
+        default) or all of a column's group partials in one pass."""
+        capi.call("raptr_recon_set_fused_reduce", self.handle, 1 if per_xcd else 0)
+
     def fused(self):
-        """{"available", "on", "waves", "lanes_per_row", "lanes_per_column"}
-        of the fused pass."""
+        """{"available", "on", "waves", "lanes_per_row", "lanes_per_column",
+        "per_xcd"} of the fused pass."""
         a, o, w, r, l = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_int()
         capi.call("raptr_recon_get_fused", self.handle, C.byref(a), C.byref(o), C.byref(w),
                   C.byref(r), C.byref(l))
         return {"available": a.value, "on": o.value, "waves": w.value,
-                "lanes_per_row": r.value, "lanes_per_column": l.value}
+                "lanes_per_row": r.value, "lanes_per_column": l.value,
+                "per_xcd": capi.lib().raptr_recon_get_fused_reduce(self.handle)}
 
     def forward_union_entries(self):
         """Total size of the LDS-staged forward's column unions (0: none)."""
@@ -251,6 +258,9 @@ class Reconstructor:
                     ms_best)
         if grp:  # tune the general kernels first
             self.set_grouped(False)
+        fz0 = self.fused() if nframes == 1 else {"available": 0}
+        if fz0["available"]:  # the separate passes first
+            self.set_fused(False)
         if nframes == 1:
             fshapes = [(fl, fu) for fl in (16, 32, 64) for fu in (1, 2, 4)]
         else:
@@ -293,23 +303,25 @@ class Reconstructor:
             self.set_backward(st, lpc, wgs)
         self.set_config(fl, fu, u)
         self.set_blocks_per_launch(bpl)
-        fz_cfg = (0, 0, 0, 0)
-        if nframes == 1 and self.fused()["available"]:
+        fz_cfg = (0, 0, 0, 0, 0)
+        if fz0["available"]:
             best_fz = None
-            for fmode in (1, 2):
-                for fw in ((2, 4) if fmode == 1 else (2, 4, 8)):
-                    for flr in (16, 32, 64):
-                        try:
-                            self.set_fused(fmode, fw, flr, 64)
-                        except capi.RaptrError:  # too little LDS for that many waves
-                            continue
-                        ms = timed()
-                        if best_fz is None or ms < best_fz[1]:
-                            best_fz = ((fmode, fw, flr, 64), ms)
+            for fmode, fw, flr in ((2, 4, 64), (2, 2, 64), (2, 4, 32), (2, 8, 64), (1, 4, 32),
+                                   (1, 2, 32)):
+                for flpc, fxr in ((16, 1), (32, 1), (64, 0)):
+                    try:
+                        self.set_fused(fmode, fw, flr, flpc)
+                        self.set_fused_reduce(fxr)
+                    except capi.RaptrError:  # too little LDS for that many waves
+                        continue
+                    ms = timed()
+                    if best_fz is None or ms < best_fz[1]:
+                        best_fz = ((fmode, fw, flr, flpc, fxr), ms)
             self.set_fused(False)
             if best_fz is not None and best_fz[1] < timed():
                 fz_cfg = best_fz[0]
-                self.set_fused(fz_cfg[0], *fz_cfg[1:])
+                self.set_fused(*fz_cfg[:4])
+                self.set_fused_reduce(fz_cfg[4])
         bc = 0
         if nframes == 64:
             # the 64-frame kernels with in-row broadcast of values and indices
@@ -337,7 +349,7 @@ class Reconstructor:
                "broadcast64": bc, "grouped64": g_on, "grouped_unroll": g_u,
                "grouped_kind": g_p, "xcd_blocks": xb, "fused": fz_cfg[0],
                "fused_waves": fz_cfg[1], "fused_lanes_per_row": fz_cfg[2],
-               "fused_lanes_per_column": fz_cfg[3]}
+               "fused_lanes_per_column": fz_cfg[3], "fused_reduce_per_xcd": fz_cfg[4]}
         return cfg, ms_best
 
     # rows each XCD walks per launch of kind 3's backward: windows of about

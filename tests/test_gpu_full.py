@@ -79,12 +79,14 @@ def _oracle(oracle, F, frame, niter, val64=None, key=None):
 
 
 def test_full_plain_mlem_vs_fp64(full, oracle):
-    """Plain single-frame FP32 plan (locality row order, LDS-staged forward,
-    per-XCD backward, 16-bit indices) over 50 iterations."""
+    """Plain single-frame FP32 plan with the separate passes (locality row
+    order, LDS-staged forward, per-XCD backward, 16-bit indices) over 50
+    iterations."""
     from raptr_amd.recon import Reconstructor
     F = full
     n = F["grid"].nvox
     R = Reconstructor(F["A"], F["sens"], "f32", 1, row_order=F["order"])
+    R.set_fused(False)  # the separate passes; the fused iteration is tested below
     x = R.mlem(F["Y"][:, 0].copy(), np.full(n, 1.0 / n, np.float32), ITERS)
     R.close()
     x64 = _oracle(oracle, F, 0, ITERS)
@@ -103,8 +105,7 @@ def test_full_plain_fused_vs_fp64(full, oracle):
     F = full
     n = F["grid"].nvox
     R = Reconstructor(F["A"], F["sens"], "f32", 1, row_order=F["order"])
-    assert R.fused()["available"] == 1
-    R.set_fused(True)
+    assert R.fused()["available"] == 1 and R.fused()["on"] == 2  # the default
     y, x0 = F["Y"][:, 0].copy(), np.full(n, 1.0 / n, np.float32)
     x = R.mlem(y, x0, ITERS)
     np.testing.assert_array_equal(x, R.mlem(y, x0, ITERS))

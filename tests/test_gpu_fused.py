@@ -30,24 +30,28 @@ def test_fused_small_system(gpu, oracle, small_system):
     x0 = np.full(n, 1.0 / n, np.float32)
     x64 = _oracle_mlem(oracle, S, S["y"], 5)
     R = Reconstructor(S["A"], S["sens"], "f32", 1, layout="blocked", row_order=S["setup"])
-    fz = R.fused()
-    assert fz["available"] == 1
-    assert fz["on"] == 0  # off until selected
+    # the default shape: one shared region, 4 waves, 64 lanes per row, the
+    # per-XCD reduce with 16 lanes per column
+    assert R.fused() == {"available": 1, "on": 2, "waves": 4, "lanes_per_row": 64,
+                         "lanes_per_column": 16, "per_xcd": 1}
+    R.set_fused(False)
     sep = R.mlem(S["y"], x0, 5)
     assert _rel(sep, x64) <= 1e-4
     st = Stream()
     for o, w, lr, rw in SHAPES:
         R.set_fused(o, w, lr, rw)
-        assert R.fused() == {"available": 1, "on": o, "waves": w, "lanes_per_row": lr,
-                             "lanes_per_column": rw}
-        a = R.mlem(S["y"], x0, 5)
-        b = R.mlem(S["y"], x0, 5)
-        np.testing.assert_array_equal(a, b)
-        assert _rel(a, x64) <= 1e-4, (o, w, lr, rw)
-        assert _rel(a, sep) <= 1e-6, (o, w, lr, rw)
-        assert abs(float(a.astype(np.float64).sum()) - 1.0) < 1e-5
-        g = R.mlem(S["y"], x0, 5, use_graph=True, stream=st.handle)
-        np.testing.assert_array_equal(a, g)
+        for xr in (True, False):
+            R.set_fused_reduce(xr)
+            assert R.fused() == {"available": 1, "on": o, "waves": w, "lanes_per_row": lr,
+                                 "lanes_per_column": rw, "per_xcd": int(xr)}
+            a = R.mlem(S["y"], x0, 5)
+            b = R.mlem(S["y"], x0, 5)
+            np.testing.assert_array_equal(a, b)
+            assert _rel(a, x64) <= 1e-4, (o, w, lr, rw, xr)
+            assert _rel(a, sep) <= 1e-6, (o, w, lr, rw, xr)
+            assert abs(float(a.astype(np.float64).sum()) - 1.0) < 1e-5
+            g = R.mlem(S["y"], x0, 5, use_graph=True, stream=st.handle)
+            np.testing.assert_array_equal(a, g)
     R.set_fused(False)
     np.testing.assert_array_equal(R.mlem(S["y"], x0, 5), sep)
 
@@ -62,6 +66,7 @@ def test_fused_16bit_values(gpu, small_system, vt):
     n = S["grid"].nvox
     x0 = np.full(n, 1.0 / n, np.float32)
     R = Reconstructor(S["A"], S["sens"], vt, 1, layout="blocked", row_order=S["setup"])
+    R.set_fused(False)
     sep = R.mlem(S["y"], x0, 5)
     for o in (1, 2):
         R.set_fused(o, 4, 32, 64)
@@ -121,11 +126,16 @@ def test_fused_edges(gpu, oracle, case):
     assert R.fused()["available"] == 1
     x0 = np.full(n, 1.0 / n, np.float32)
     x64 = oracle.mlem_f64(rp, col, val.astype(np.float64), y, np.ones(n), x0, 3)
+    assert R.fused()["on"] == 2  # the default
+    np.testing.assert_array_equal(R.mlem(y, x0, 3), R.mlem(y, x0, 3))
+    R.set_fused(False)
     sep = R.mlem(y, x0, 3)
     assert _rel(sep, x64) <= 1e-5
-    for o, w, lr, rw in [(1, 2, 16, 16), (1, 4, 16, 64), (1, 2, 32, 32), (1, 4, 64, 64),
-                         (2, 4, 32, 16), (2, 2, 16, 64), (2, 4, 64, 32), (2, 8, 64, 64)]:
+    for o, w, lr, rw, xr in [(1, 2, 16, 16, 1), (1, 4, 16, 64, 0), (1, 2, 32, 32, 1),
+                             (1, 4, 64, 64, 1), (2, 4, 32, 16, 0), (2, 2, 16, 64, 1),
+                             (2, 4, 64, 32, 1), (2, 8, 64, 64, 0)]:
         R.set_fused(o, w, lr, rw)
+        R.set_fused_reduce(xr)
         a = R.mlem(y, x0, 3)
         np.testing.assert_array_equal(a, R.mlem(y, x0, 3))
         assert _rel(a, x64) <= 1e-5, (o, w, lr, rw)
@@ -147,7 +157,7 @@ def test_fused_lds_limits(gpu):
     R = Reconstructor(_device_csr(rp, col, val, n), DeviceArray.from_numpy(np.ones(n)), "f32", 1,
                       layout="blocked")
     assert R.fused()["available"] == 1
-    assert R.fused()["waves"] == 2  # the plan's default: the most waves that fit
+    assert R.fused()["on"] == 2 and R.fused()["waves"] == 4  # a shared region fits 4 waves
     R.set_fused(2, 4, 16, 64)
     R.set_fused(2, 8, 64, 64)
     R.set_fused(True, 2, 32, 64)
@@ -159,7 +169,7 @@ def test_fused_lds_limits(gpu):
         with pytest.raises(RaptrError):
             R.set_fused(o, w, lr, rw)
     assert R.fused() == {"available": 1, "on": 1, "waves": 2, "lanes_per_row": 32,
-                         "lanes_per_column": 64}
+                         "lanes_per_column": 64, "per_xcd": 1}
     R.set_fused(2, 8, 64, 64)
     R.set_fused(False)  # keeps the shape
     with pytest.raises(RaptrError):

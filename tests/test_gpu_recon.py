@@ -78,6 +78,7 @@ def test_recon_backward_per_xcd_row_ranges(gpu, oracle, small_system, block_rows
     x64 = _oracle_mlem(oracle, S, S["y"], 5)
     R = Reconstructor(S["A"], S["sens"], "f32", 1, block_rows=block_rows, layout="blocked",
                       row_order=S["setup"])
+    R.set_fused(False)  # the separate forward and backward passes
     R.set_backward(0)
     b = R.mlem(S["y"], x0, 5)
     from raptr_amd.device import Stream
@@ -129,13 +130,14 @@ def test_recon_row_block_cut_is_immaterial(gpu, small_system):
     S = small_system
     n = S["grid"].nvox
     x0 = np.full(n, 1.0 / n, np.float32)
-    a = Reconstructor(S["A"], S["sens"], "f32", 1, block_rows=0, layout="blocked").mlem(
-        S["y"], x0, 5)
-    b = Reconstructor(S["A"], S["sens"], "f32", 1, block_rows=999, layout="blocked").mlem(
-        S["y"], x0, 5)
+    Ra = Reconstructor(S["A"], S["sens"], "f32", 1, block_rows=0, layout="blocked")
+    Ra.set_fused(False)  # the separate passes: the backward walks the row blocks
+    a = Ra.mlem(S["y"], x0, 5)
+    Rb = Reconstructor(S["A"], S["sens"], "f32", 1, block_rows=999, layout="blocked")
+    Rb.set_fused(False)
+    b = Rb.mlem(S["y"], x0, 5)
     assert _rel(a, b) <= 1e-6
     # several row blocks per backward launch (sums carried in registers)
-    Rb = Reconstructor(S["A"], S["sens"], "f32", 1, block_rows=999, layout="blocked")
     assert Rb.info()["row_blocks"] > 8
     for bpl in (3, 64):
         Rb.set_blocks_per_launch(bpl)
@@ -798,6 +800,7 @@ def test_recon_forward_lds_staged_small_system(gpu, oracle, small_system):
     x0 = np.full(n, 1.0 / n, np.float32)
     R = Reconstructor(S["A"], S["sens"], "f32", 1, layout="blocked", row_order=S["setup"])
     assert R.forward_kind() == (2, [0, 2])
+    R.set_fused(False)  # the separate passes (the fused iteration has its own forward)
     a = R.mlem(S["y"], x0, 5)
     R.set_forward_kind(0)
     b = R.mlem(S["y"], x0, 5)
