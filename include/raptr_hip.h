@@ -504,8 +504,9 @@ int raptr_recon_get_forward_kind(const raptr_recon* plan, int* kind, int* availa
  *       region per wave (520 + 4 x (largest union + waves x (largest union
  *       + 1)) bytes of LDS must fit 160 KiB); 2 fused, one region per
  *       workgroup that the waves add to in turn (520 + 4 x (2 x largest
- *       union + 1) bytes);
- *   waves: waves per row group (2 or 4; 8 with on = 2);
+ *       union + 1) bytes; the default);
+ *   waves: waves per row group (2 or 4; 8 with on = 2; default 4 where five
+ *          such workgroups fit a CU's 160 KiB of LDS, else 8);
  *   lanes_per_row: lanes per matrix row (16, 32 or 64);
  *   lanes_per_column: lanes per column of the reduce (16, 32 or 64).
  * get: available = 1 if the plan has the fused pass; null outputs are
@@ -522,6 +523,13 @@ int raptr_recon_get_fused(const raptr_recon* plan, int* available, int* on, int*
  * rounding.  get: the setting (-1 for a null plan). */
 int raptr_recon_set_fused_reduce(raptr_recon* plan, int per_xcd);
 int raptr_recon_get_fused_reduce(const raptr_recon* plan);
+/* Register-held steps of 4 x 64 entries per row of the fused products
+ * kernel with 64 lanes per row (2, 3 or 4; rows past them continue from
+ * memory).  The plan's default: (mean padded row + 64) / 256, rounded up
+ * and clamped to 2-4.  With 16 or 32 lanes per row the steps are fixed (6
+ * and 3).  get: the setting (-1 for a null plan). */
+int raptr_recon_set_fused_steps(raptr_recon* plan, int steps);
+int raptr_recon_get_fused_steps(const raptr_recon* plan);
 /* The fused iteration's kernels alone (timing, tests), as the plan is set:
  * part 0 the products and the column reduce with the update into the
  * plan's internal image buffer (d_y in the caller's row order, as

@@ -138,6 +138,8 @@ _SIGS = {
     "raptr_recon_set_fused": (i32, [vp, i32, i32, i32, i32]),
     "raptr_recon_get_fused": (i32, [vp, P(i32), P(i32), P(i32), P(i32), P(i32)]),
     "raptr_recon_set_fused_reduce": (i32, [vp, i32]),
+    "raptr_recon_set_fused_steps": (i32, [vp, i32]),
+    "raptr_recon_get_fused_steps": (i32, [vp]),
     "raptr_recon_get_fused_reduce": (i32, [vp]),
     "raptr_recon_fused_pass": (i32, [vp, vp, vp, i32, vp]),
     "raptr_locality_order": (i32, [vp, vp, i32, i32, vp, vp]),

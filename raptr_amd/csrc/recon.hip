@@ -2432,7 +2432,7 @@ int launch_fused1(const raptr_recon* R, const float* x, const float* y, hipStrea
     const unsigned g = unsigned((m + kLocRows - 1) / kLocRows);
     bool launched = false;
 #define RAPTR_FB(VV, WW, LL, SS, SH)                                                         \
-  if (R->vt == VV && W == WW && LR == LL && SHR == SH) {                                     \
+  if (R->vt == VV && W == WW && LR == LL && SHR == SH && (LL != 64 || SS == R->fb_steps)) {  \
     static bool configured = false;                                                          \
     if (!configured) {                                                                       \
       if (hipFuncSetAttribute((const void*)k_fb1l<VV, WW, LL, SS, SH>,                       \
@@ -2449,7 +2449,8 @@ int launch_fused1(const raptr_recon* R, const float* x, const float* y, hipStrea
     launched = true;                                                                         \
   }
 #define RAPTR_FBS(VV, WW, SH)                                                                \
-  RAPTR_FB(VV, WW, 16, 6, SH) RAPTR_FB(VV, WW, 32, 3, SH) RAPTR_FB(VV, WW, 64, 2, SH)
+  RAPTR_FB(VV, WW, 16, 6, SH) RAPTR_FB(VV, WW, 32, 3, SH) RAPTR_FB(VV, WW, 64, 2, SH)        \
+  RAPTR_FB(VV, WW, 64, 3, SH) RAPTR_FB(VV, WW, 64, 4, SH)
 #define RAPTR_FBW(VV, WW) RAPTR_FBS(VV, WW, false) RAPTR_FBS(VV, WW, true)
 #define RAPTR_FBV(VV) RAPTR_FBW(VV, 2) RAPTR_FBW(VV, 4) RAPTR_FBS(VV, 8, true)
     RAPTR_FBV(kF32) RAPTR_FBV(kF16) RAPTR_FBV(kBF16) RAPTR_FBV(kU16)
@@ -2819,14 +2820,19 @@ int build_union_index(raptr_recon* R, hipStream_t st) {
   (void)hipStreamSynchronize(st);
   (void)hipFree(tmp);
   if (rc) return rc;
-  // the default single-frame iteration: one shared column-sum region, 4
-  // waves per group, 64 lanes per row, the per-XCD reduce with 16 lanes per
-  // column (the fastest shape at full size)
+  // the default single-frame iteration: one shared column-sum region, 64
+  // lanes per row, the per-XCD reduce with 16 lanes per column; register-held
+  // steps of 256 entries for the mean padded row plus 64 (2 for the
+  // full-size workload's 360 entries, 3 for the 96^3 geometry's 642); 4
+  // waves per group where 5 or more such workgroups fit a CU's LDS, else 8
+  // (the fastest shapes measured on both)
   R->fused = 2;
-  R->fb_waves = 4;
   R->fb_lanes = 64;
   R->fb_lpc = 16;
   R->fb_xcd = 1;
+  const double mean_row = double(R->pnnz) / double(std::max(R->m, 1));
+  R->fb_steps = std::min(4, std::max(2, int(std::ceil((mean_row + 64.0) / 256.0))));
+  R->fb_waves = fused_lds_bytes(R, 4, true) * 5 <= kLdsMax ? 4 : 8;
   return RAPTR_OK;
 }
 
@@ -3589,6 +3595,16 @@ int raptr_recon_set_fused(raptr_recon* R, int on, int waves, int lanes_per_row,
   R->fb_lpc = lanes_per_column;
   return RAPTR_OK;
 }
+
+int raptr_recon_set_fused_steps(raptr_recon* R, int steps) {
+  if (!R) return fail(RAPTR_ERR_INVALID_ARG, "null plan");
+  if (steps < 2 || steps > 4) return fail(RAPTR_ERR_INVALID_ARG, "steps must be 2, 3 or 4");
+  if (R->fb_steps != steps) clear_graphs(R);
+  R->fb_steps = steps;
+  return RAPTR_OK;
+}
+
+int raptr_recon_get_fused_steps(const raptr_recon* R) { return R ? R->fb_steps : -1; }
 
 int raptr_recon_set_fused_reduce(raptr_recon* R, int per_xcd) {
   if (!R) return fail(RAPTR_ERR_INVALID_ARG, "null plan");

@@ -164,6 +164,7 @@ struct raptr_recon {
   int fb_xcd = 1;     // the reduce in per-XCD group ranges (k_fb1_reduce8)
   int fb_waves = 4;   // waves per row group (2 or 4)
   int fb_lanes = 32;  // lanes per row (16, 32 or 64)
+  int fb_steps = 2;   // register-held steps per row with 64 lanes (2, 3 or 4)
   int fb_lpc = 64;    // lanes per column of the reduce (16, 32 or 64)
   int64_t ngf = 0, ngc = 0, gfnnz = 0, gbnnz = 0;
   int64_t* gfp = nullptr;  // ngf + 1

@@ -99,7 +99,7 @@ class Reconstructor:
         out.update({"fused": fz["on"], "fused_available": fz["available"],
                     "fused_waves": fz["waves"], "fused_lanes_per_row": fz["lanes_per_row"],
                     "fused_lanes_per_column": fz["lanes_per_column"],
-                    "fused_reduce_per_xcd": fz["per_xcd"]})
+                    "fused_reduce_per_xcd": fz["per_xcd"], "fused_steps": fz["steps"]})
         gi = self.grouped()
         if gi["group"]:
             out.update({"grouped64": gi["on"], "group": gi["group"],
@@ -166,6 +166,11 @@ class Reconstructor:
         capi.call("raptr_recon_set_fused", self.handle, mode, int(waves),
                   int(lanes_per_row), int(lanes_per_column))
 
+    def set_fused_steps(self, steps):
+        """Register-held steps of 256 entries per row of the fused products
+        kernel with 64 lanes per row (2, 3 or 4)."""
+        capi.call("raptr_recon_set_fused_steps", self.handle, int(steps))
+
     def set_fused_reduce(self, per_xcd):
         """Fused iteration's column reduce: per XCD group ranges (True, the
         default) or all of a column's group partials in one pass."""
@@ -173,13 +178,14 @@ class Reconstructor:
 
     def fused(self):
         """{"available", "on", "waves", "lanes_per_row", "lanes_per_column",
-        "per_xcd"} of the fused pass."""
+        "per_xcd", "steps"} of the fused pass."""
         a, o, w, r, l = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_int()
         capi.call("raptr_recon_get_fused", self.handle, C.byref(a), C.byref(o), C.byref(w),
                   C.byref(r), C.byref(l))
         return {"available": a.value, "on": o.value, "waves": w.value,
                 "lanes_per_row": r.value, "lanes_per_column": l.value,
-                "per_xcd": capi.lib().raptr_recon_get_fused_reduce(self.handle)}
+                "per_xcd": capi.lib().raptr_recon_get_fused_reduce(self.handle),
+                "steps": capi.lib().raptr_recon_get_fused_steps(self.handle)}
 
     def forward_union_entries(self):
         """Total size of the LDS-staged forward's column unions (0: none)."""
@@ -303,25 +309,29 @@ class Reconstructor:
             self.set_backward(st, lpc, wgs)
         self.set_config(fl, fu, u)
         self.set_blocks_per_launch(bpl)
-        fz_cfg = (0, 0, 0, 0, 0)
+        fz_cfg = (0, 0, 0, 0, 0, 0)
         if fz0["available"]:
             best_fz = None
-            for fmode, fw, flr in ((2, 4, 64), (2, 2, 64), (2, 4, 32), (2, 8, 64), (1, 4, 32),
-                                   (1, 2, 32)):
+            s0 = fz0["steps"]
+            shapes = [(2, w, 64, st_) for w in (4, 8) for st_ in sorted({s0, min(s0 + 1, 4)})]
+            shapes += [(2, 2, 64, s0), (2, 4, 32, s0), (1, 4, 32, s0), (1, 2, 64, s0)]
+            for fmode, fw, flr, fst in shapes:
                 for flpc, fxr in ((16, 1), (32, 1), (64, 0)):
                     try:
                         self.set_fused(fmode, fw, flr, flpc)
                         self.set_fused_reduce(fxr)
+                        self.set_fused_steps(fst)
                     except capi.RaptrError:  # too little LDS for that many waves
                         continue
                     ms = timed()
                     if best_fz is None or ms < best_fz[1]:
-                        best_fz = ((fmode, fw, flr, flpc, fxr), ms)
+                        best_fz = ((fmode, fw, flr, flpc, fxr, fst), ms)
             self.set_fused(False)
             if best_fz is not None and best_fz[1] < timed():
                 fz_cfg = best_fz[0]
                 self.set_fused(*fz_cfg[:4])
                 self.set_fused_reduce(fz_cfg[4])
+                self.set_fused_steps(fz_cfg[5])
         bc = 0
         if nframes == 64:
             # the 64-frame kernels with in-row broadcast of values and indices
@@ -349,7 +359,8 @@ class Reconstructor:
                "broadcast64": bc, "grouped64": g_on, "grouped_unroll": g_u,
                "grouped_kind": g_p, "xcd_blocks": xb, "fused": fz_cfg[0],
                "fused_waves": fz_cfg[1], "fused_lanes_per_row": fz_cfg[2],
-               "fused_lanes_per_column": fz_cfg[3], "fused_reduce_per_xcd": fz_cfg[4]}
+               "fused_lanes_per_column": fz_cfg[3], "fused_reduce_per_xcd": fz_cfg[4],
+               "fused_steps": fz_cfg[5]}
         return cfg, ms_best
 
     # rows each XCD walks per launch of kind 3's backward: windows of about

@@ -30,10 +30,11 @@ def test_fused_small_system(gpu, oracle, small_system):
     x0 = np.full(n, 1.0 / n, np.float32)
     x64 = _oracle_mlem(oracle, S, S["y"], 5)
     R = Reconstructor(S["A"], S["sens"], "f32", 1, layout="blocked", row_order=S["setup"])
-    # the default shape: one shared region, 4 waves, 64 lanes per row, the
-    # per-XCD reduce with 16 lanes per column
+    # the default shape: one shared region, 4 waves, 64 lanes per row with 2
+    # register-held steps (rows of about 140 entries), the per-XCD reduce
+    # with 16 lanes per column
     assert R.fused() == {"available": 1, "on": 2, "waves": 4, "lanes_per_row": 64,
-                         "lanes_per_column": 16, "per_xcd": 1}
+                         "lanes_per_column": 16, "per_xcd": 1, "steps": 2}
     R.set_fused(False)
     sep = R.mlem(S["y"], x0, 5)
     assert _rel(sep, x64) <= 1e-4
@@ -43,7 +44,7 @@ def test_fused_small_system(gpu, oracle, small_system):
         for xr in (True, False):
             R.set_fused_reduce(xr)
             assert R.fused() == {"available": 1, "on": o, "waves": w, "lanes_per_row": lr,
-                                 "lanes_per_column": rw, "per_xcd": int(xr)}
+                                 "lanes_per_column": rw, "per_xcd": int(xr), "steps": 2}
             a = R.mlem(S["y"], x0, 5)
             b = R.mlem(S["y"], x0, 5)
             np.testing.assert_array_equal(a, b)
@@ -52,6 +53,12 @@ def test_fused_small_system(gpu, oracle, small_system):
             assert abs(float(a.astype(np.float64).sum()) - 1.0) < 1e-5
             g = R.mlem(S["y"], x0, 5, use_graph=True, stream=st.handle)
             np.testing.assert_array_equal(a, g)
+    for stp in (3, 4, 2):  # register-held steps of the 64-lane kernel
+        R.set_fused(2, 4, 64, 16)
+        R.set_fused_steps(stp)
+        a = R.mlem(S["y"], x0, 5)
+        np.testing.assert_array_equal(a, R.mlem(S["y"], x0, 5))
+        assert _rel(a, sep) <= 1e-6, stp
     R.set_fused(False)
     np.testing.assert_array_equal(R.mlem(S["y"], x0, 5), sep)
 
@@ -75,13 +82,13 @@ def test_fused_16bit_values(gpu, small_system, vt):
         assert _rel(a, sep) <= 1e-5
 
 
-def _long_row_csr(n, m, seed, long_rows=True, empty=None, window=5000):
+def _long_row_csr(n, m, seed, long_rows=True, empty=None, window=5000, lmax=500):
     """Host CSR whose groups of 64 consecutive rows draw their columns from
-    one window of `window` voxels; lengths 1..500, with rows of 513..2,000
+    one window of `window` voxels; lengths 1..lmax, with rows of 513..2,000
     entries (past the register-held steps of every shape) where long_rows;
     rows in `empty` have no entries."""
     rng = np.random.default_rng(seed)
-    lens = rng.integers(1, 500, m)
+    lens = rng.integers(1, lmax, m)
     if long_rows:
         k = max(1, m // 10)
         lens[rng.choice(m, k, replace=False)] = rng.integers(513, 2000, k)
@@ -99,7 +106,7 @@ def _long_row_csr(n, m, seed, long_rows=True, empty=None, window=5000):
     return rp, col, val
 
 
-@pytest.mark.parametrize("case", ["long_rows", "empty_group", "one_row"])
+@pytest.mark.parametrize("case", ["long_rows", "empty_group", "one_row", "long_mean"])
 def test_fused_edges(gpu, oracle, case):
     """Rows longer than the register-held steps (up to 2,000 entries) and a
     ragged last group (m = 785); four empty row groups; a one-row plan;
@@ -113,8 +120,10 @@ def test_fused_edges(gpu, oracle, case):
         rp, col, val = _long_row_csr(n, 785, 21)
     elif case == "empty_group":
         rp, col, val = _long_row_csr(n, 785, 22, empty=slice(256, 512))
-    else:
+    elif case == "one_row":
         rp, col, val = _long_row_csr(n, 1, 23)
+    else:  # rows of 1..1,300 entries: a mean of about 650 + padding -> 3 steps
+        rp, col, val = _long_row_csr(n, 785, 26, long_rows=False, lmax=1300)
     m = len(rp) - 1
     rng = np.random.default_rng(7)
     y = (rng.random(m) * 50).astype(np.float32)
@@ -127,15 +136,19 @@ def test_fused_edges(gpu, oracle, case):
     x0 = np.full(n, 1.0 / n, np.float32)
     x64 = oracle.mlem_f64(rp, col, val.astype(np.float64), y, np.ones(n), x0, 3)
     assert R.fused()["on"] == 2  # the default
+    if case == "long_mean":
+        assert R.fused()["steps"] == 3
     np.testing.assert_array_equal(R.mlem(y, x0, 3), R.mlem(y, x0, 3))
     R.set_fused(False)
     sep = R.mlem(y, x0, 3)
     assert _rel(sep, x64) <= 1e-5
-    for o, w, lr, rw, xr in [(1, 2, 16, 16, 1), (1, 4, 16, 64, 0), (1, 2, 32, 32, 1),
-                             (1, 4, 64, 64, 1), (2, 4, 32, 16, 0), (2, 2, 16, 64, 1),
-                             (2, 4, 64, 32, 1), (2, 8, 64, 64, 0)]:
+    for o, w, lr, rw, xr, stp in [(1, 2, 16, 16, 1, 2), (1, 4, 16, 64, 0, 2),
+                                  (1, 2, 32, 32, 1, 2), (1, 4, 64, 64, 1, 3), (2, 4, 32, 16, 0, 2),
+                                  (2, 2, 16, 64, 1, 2), (2, 4, 64, 32, 1, 2), (2, 8, 64, 64, 0, 4),
+                                  (2, 4, 64, 16, 1, 3), (2, 8, 64, 16, 1, 2)]:
         R.set_fused(o, w, lr, rw)
         R.set_fused_reduce(xr)
+        R.set_fused_steps(stp)
         a = R.mlem(y, x0, 3)
         np.testing.assert_array_equal(a, R.mlem(y, x0, 3))
         assert _rel(a, x64) <= 1e-5, (o, w, lr, rw)
@@ -157,7 +170,9 @@ def test_fused_lds_limits(gpu):
     R = Reconstructor(_device_csr(rp, col, val, n), DeviceArray.from_numpy(np.ones(n)), "f32", 1,
                       layout="blocked")
     assert R.fused()["available"] == 1
-    assert R.fused()["on"] == 2 and R.fused()["waves"] == 4  # a shared region fits 4 waves
+    # the default: a shared region (79.7 KB of LDS: two workgroups per CU,
+    # so 8 waves per group)
+    assert R.fused()["on"] == 2 and R.fused()["waves"] == 8
     R.set_fused(2, 4, 16, 64)
     R.set_fused(2, 8, 64, 64)
     R.set_fused(True, 2, 32, 64)
@@ -169,7 +184,10 @@ def test_fused_lds_limits(gpu):
         with pytest.raises(RaptrError):
             R.set_fused(o, w, lr, rw)
     assert R.fused() == {"available": 1, "on": 1, "waves": 2, "lanes_per_row": 32,
-                         "lanes_per_column": 64, "per_xcd": 1}
+                         "lanes_per_column": 64, "per_xcd": 1, "steps": 2}
+    for bad in (1, 5, 0):
+        with pytest.raises(RaptrError):
+            R.set_fused_steps(bad)
     R.set_fused(2, 8, 64, 64)
     R.set_fused(False)  # keeps the shape
     with pytest.raises(RaptrError):
