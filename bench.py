@@ -22,7 +22,9 @@ is <= 1e-4 relative L2; every mode's frames/s and error are in `modes`.
 
 Also reported: the 64-frame batched workload on the small geometry
 (20,480 channels, 32^3 at 2.5 mm, 10 rays, decaying ellipsoids, counts
-log-uniform 1e4-2e6 per frame, 30 iterations) in `batched_small`.
+log-uniform 1e4-2e6 per frame, 30 iterations) in `batched_small`; the
+largest serving geometry (16x16 heads, 60 angles, 96^3 at 2 mm, 20 rays,
+5e6 events) as a single-frame plan in `plain_96`.
 
 Prints one JSON line on stdout; progress on stderr.
 """
@@ -518,6 +520,12 @@ def main():
                             0 if args.no_cpu_baseline else
                             int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
 
+    p96 = None
+    if not args.no_small:
+        p96 = bench_plain_96(log, timed, stream, st, niter,
+                             0 if args.no_cpu_baseline else
+                             int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+
     phases = None
     if not args.no_small:
         phases = bench_build_phases(log, timed, stream, st,
@@ -608,6 +616,7 @@ def main():
         "build": {"matrix_ms": round(ms_build, 1), "sensitivity_ms": round(ms_sens, 1),
                   "data_s": round(t_data, 2)},
         "batched_small": small,
+        "plain_96": p96,
         "build_phases_small": phases,
         "serving": serve,
         "device_copy": copy,
@@ -841,6 +850,64 @@ def bench_build_phases(log, timed, stream, st, cpu_threads=0):
         log(f"build phases (small), CPU oracle: workqueue {out['cpu_ms']['workqueue']:.0f} ms "
             f"(1 thread), sampling {out['cpu_ms']['sm_from_wq']:.0f} ms, fused build "
             f"{out['cpu_ms']['csr_build_fused']:.0f} ms ({cpu_threads} threads)")
+    return out
+
+
+def bench_plain_96(log, timed, stream, st, niter, cpu_threads=0):
+    """The largest serving geometry as a single-frame plan: 16x16-pixel
+    heads, 60 angles at 3 deg, 96^3 grid at 2 mm, 20 rays, 5e6 events, niter
+    iterations per frame; the autotuned plan (the fused iteration where it
+    wins) against the separate passes, and the image against the FP64
+    oracle on the same matrix."""
+    from raptr_amd import geometry as G
+    from raptr_amd import synthetic, sysmat
+    from raptr_amd.device import DeviceArray
+    from raptr_amd.recon import Reconstructor
+    setup, grid, nrays = G.two_head_setup(16, 60, 3.0), G.centred_grid(96, 2.0), 20
+    image, _ = synthetic.random_ellipsoids(grid, n=6, seed=2)
+    meas, y, _ = synthetic.measurement(setup, grid, nrays, image, 5e6, seed=1)
+    m, n = len(meas), grid.nvox
+    d_list = DeviceArray.from_numpy(meas)
+    A = sysmat.build_csr(setup, grid, meas, nrays, d_list=d_list, stream=st, capacity="auto")
+    sens = sysmat.sensitivity(setup, grid, nrays, stream=st)
+    from raptr_amd.recon import default_order_kind
+    order = sysmat.locality_order(setup, d_list, stream=st, kind=default_order_kind(1))
+    R = Reconstructor(A, sens, "f32", 1, stream=st, row_order=order)
+    dy = DeviceArray.from_numpy(y.astype(np.float32))
+    x0 = np.full(n, 1.0 / n, np.float32)
+    dx = DeviceArray.from_numpy(x0)
+    cfg, _ = R.autotune(dy, dx, 1, reps=2, stream=st)
+
+    def frame():
+        dx.copy_from(x0, stream=st)
+        R.run(dy, dx, niter, 1, use_graph=True, stream=st)
+
+    frame()
+    stream.sync()
+    ms = timed(frame)
+    x = dx.numpy(stream=st)
+    sep = None
+    if R.fused()["on"]:
+        R.set_fused(False)
+        frame()
+        sep = timed(frame)
+    out = {"workload": f"16x16 heads x 60 angles (3,932,160 channels), 96^3 grid @ 2 mm, "
+                       f"20 rays, 5e6 events, {niter} iterations, FP32, one frame",
+           "measured_channels": m, "nnz": int(A.nnz), "voxels": n,
+           "frames_per_s": round(1000.0 / ms, 3), "ms_per_iteration": round(ms / niter, 4),
+           "config": cfg,
+           "separate_passes_ms_per_iteration": round(sep / niter, 4) if sep else None}
+    R.close()
+    log(f"plain 96^3: {ms / niter:.3f} ms/iteration (separate passes "
+        f"{(sep / niter) if sep else float('nan'):.3f}), cfg {cfg}")
+    if cpu_threads:
+        from oracle import pyoracle as O
+        rp, col, val = A.to_host()
+        O.lib().ro_set_threads(cpu_threads)
+        x64 = O.mlem_f64(rp, col, val.astype(np.float64), y.astype(np.float64), sens.numpy(),
+                         x0.astype(np.float64), niter)
+        out["rel_l2_vs_fp64"] = rel_l2(x, x64)
+        log(f"plain 96^3: rel L2 vs FP64 {out['rel_l2_vs_fp64']:.2e}")
     return out
 
 

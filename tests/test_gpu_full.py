@@ -14,6 +14,8 @@ workloads), with the GPU's FP64 sensitivity.  Bounds:
     iterations, two frames: every frame within 1e-5 of the FP64 recurrence
     on the dequantized matrix (the kernels' arithmetic), and within the
     stated bounds of the FP32-matrix oracle (the storage's own error).
+And the 96^3 serving geometry as a single-frame plan (fused and separate
+passes, 20 iterations, <= 1e-4).
 """
 import os
 
@@ -194,3 +196,42 @@ def test_full_mixed_precision_per_frame(full, oracle, vt):
               f"max voxel {mx:.2e}")
         assert rel_q <= 1e-5
         assert rel <= BOUNDS_16[vt][0] and mx <= BOUNDS_16[vt][1]
+
+
+def test_full_96_plain_fused_vs_fp64(gpu, oracle):
+    """The largest serving geometry as a single-frame plan: 16x16-pixel
+    heads, 60 angles at 3 deg (3,932,160 channels), 96^3 grid at 2 mm, 20
+    rays, 5e6 events; rows of about 640 entries.  The fused iteration (its
+    default shape here: 3 register-held steps, 8 waves per group) and the
+    separate passes, 20 iterations, against the FP64 oracle on the same
+    matrix (<= 1e-4) and each other (<= 2e-6); the fused run repeats bit for
+    bit."""
+    from raptr_amd import synthetic, sysmat
+    from raptr_amd.device import DeviceArray
+    from raptr_amd.recon import Reconstructor
+    setup, grid = G.two_head_setup(16, 60, 3.0), G.centred_grid(96, 2.0)
+    image, _ = synthetic.random_ellipsoids(grid, n=6, seed=2)
+    meas, y, _ = synthetic.measurement(setup, grid, 20, image, 5e6, seed=1)
+    d_list = DeviceArray.from_numpy(meas)
+    A = sysmat.build_csr(setup, grid, meas, 20, d_list=d_list, capacity="auto")
+    sens = sysmat.sensitivity(setup, grid, 20)
+    order = sysmat.locality_order(setup, d_list)
+    n, it = grid.nvox, 20
+    print(f"96^3 workload: m={len(meas)} nnz={A.nnz}", flush=True)
+    R = Reconstructor(A, sens, "f32", 1, row_order=order)
+    fz = R.fused()
+    assert fz["on"] == 2 and fz["steps"] == 3 and fz["waves"] == 8
+    y32, x0 = y.astype(np.float32), np.full(n, 1.0 / n, np.float32)
+    xf = R.mlem(y32, x0, it)
+    np.testing.assert_array_equal(xf, R.mlem(y32, x0, it))
+    R.set_fused(False)
+    xs = R.mlem(y32, x0, it)
+    R.close()
+    rp, col, val = A.to_host()
+    oracle.lib().ro_set_threads(int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+    x64 = oracle.mlem_f64(rp, col, val.astype(np.float64), y.astype(np.float64), sens.numpy(),
+                          np.full(n, 1.0 / n), it)
+    rel_f, rel_s = _rel(xf, x64), _rel(xs, x64)
+    print(f"96^3: fused rel L2 {rel_f:.2e}, separate {rel_s:.2e}")
+    assert rel_f <= 1e-4 and rel_s <= 1e-4
+    assert _rel(xf, xs.astype(np.float64)) <= 2e-6
