@@ -215,7 +215,7 @@ def test_full_96_plain_fused_vs_fp64(gpu, oracle):
     d_list = DeviceArray.from_numpy(meas)
     A = sysmat.build_csr(setup, grid, meas, 20, d_list=d_list, capacity="auto")
     sens = sysmat.sensitivity(setup, grid, 20)
-    order = sysmat.locality_order(setup, d_list)
+    order = sysmat.locality_order(setup, d_list, kind="y")  # the single-frame default
     n, it = grid.nvox, 20
     print(f"96^3 workload: m={len(meas)} nnz={A.nnz}", flush=True)
     R = Reconstructor(A, sens, "f32", 1, row_order=order)
