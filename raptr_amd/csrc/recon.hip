@@ -19,6 +19,7 @@
 // stream is shared by the batch.  The normalisation sums are FP64.
 #include <algorithm>
 #include <climits>
+#include <mutex>
 #include <type_traits>
 #include <vector>
 
@@ -1707,7 +1708,8 @@ inline int bits_for_count(int64_t count) {
 // only (the positions of column j in [vb[x-1][j], vb[x][j]), with vb[-1] =
 // vptr[j] and vb[7] = vptr[j+1]), so an XCD's L2 holds the lines of an
 // eighth of the groups; c8[x][j] takes the XCD's sum, and k_bwd1_xreduce
-// adds the 8 in XCD order and applies the update.
+// adds the 8 in XCD order and applies the update.  The positions stream
+// past L2 (non-temporal) so that they do not evict the partial sums' lines.
 template <int LPC, int U>
 __global__ __launch_bounds__(256) void k_fb1_reduce8(int n, const int64_t* __restrict__ vptr,
                                                      const int64_t* __restrict__ vb,
@@ -1724,14 +1726,14 @@ __global__ __launch_bounds__(256) void k_fb1_reduce8(int n, const int64_t* __res
   for (; k + int64_t(LPC) * (U - 1) < k1; k += int64_t(LPC) * U) {
     int p[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) p[u] = vpos[k + LPC * u];
+    for (int u = 0; u < U; ++u) p[u] = __builtin_nontemporal_load(vpos + k + LPC * u);
     float g[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) g[u] = cpart[p[u]];
 #pragma unroll
     for (int u = 0; u < U; ++u) acc += g[u];
   }
-  for (; k < k1; k += LPC) acc += cpart[vpos[k]];
+  for (; k < k1; k += LPC) acc += cpart[__builtin_nontemporal_load(vpos + k)];
 #pragma unroll
   for (int off = LPC / 2; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, LPC);
   if (lane == 0) c8[int64_t(xcd) * n + j] = acc;
@@ -2417,6 +2419,30 @@ constexpr size_t kLdsMax = 160 * 1024;
 
 // single frame: forward, ratio and backward in one pass (k_fb1l), then the
 // fixed-order column reduce + update
+// the dynamic-LDS limit of every fused products kernel raised to the CU's
+// 160 KiB, once per process (at plan build, outside any graph capture);
+// not fatal where the default already suffices
+void fused_lds_attributes() {
+  static std::once_flag once;
+  std::call_once(once, [] {
+#define RAPTR_FA(VV, WW, LL, SS, SH)                                                         \
+  if (hipFuncSetAttribute((const void*)k_fb1l<VV, WW, LL, SS, SH>,                           \
+                          hipFuncAttributeMaxDynamicSharedMemorySize, int(kLdsMax)) !=       \
+      hipSuccess)                                                                            \
+    (void)hipGetLastError();
+#define RAPTR_FAS(VV, WW, SH)                                                                \
+  RAPTR_FA(VV, WW, 16, 6, SH) RAPTR_FA(VV, WW, 32, 3, SH) RAPTR_FA(VV, WW, 64, 2, SH)        \
+  RAPTR_FA(VV, WW, 64, 3, SH) RAPTR_FA(VV, WW, 64, 4, SH)
+#define RAPTR_FAV(VV)                                                                        \
+  RAPTR_FAS(VV, 2, false) RAPTR_FAS(VV, 2, true) RAPTR_FAS(VV, 4, false)                     \
+  RAPTR_FAS(VV, 4, true) RAPTR_FAS(VV, 8, true)
+    RAPTR_FAV(kF32) RAPTR_FAV(kF16) RAPTR_FAV(kBF16) RAPTR_FAV(kU16)
+#undef RAPTR_FAV
+#undef RAPTR_FAS
+#undef RAPTR_FA
+  });
+}
+
 // part: 0 both kernels, 1 the products (k_fb1l) only, 2 the column reduce
 // and update only
 int launch_fused1(const raptr_recon* R, const float* x, const float* y, hipStream_t st,
@@ -2433,14 +2459,6 @@ int launch_fused1(const raptr_recon* R, const float* x, const float* y, hipStrea
     bool launched = false;
 #define RAPTR_FB(VV, WW, LL, SS, SH)                                                         \
   if (R->vt == VV && W == WW && LR == LL && SHR == SH && (LL != 64 || SS == R->fb_steps)) {  \
-    static bool configured = false;                                                          \
-    if (!configured) {                                                                       \
-      if (hipFuncSetAttribute((const void*)k_fb1l<VV, WW, LL, SS, SH>,                       \
-                              hipFuncAttributeMaxDynamicSharedMemorySize,                    \
-                              int(kLdsMax)) != hipSuccess)                                   \
-        (void)hipGetLastError();                                                             \
-      configured = true;                                                                     \
-    }                                                                                        \
     hipLaunchKernelGGL((k_fb1l<VV, WW, LL, SS, SH>), dim3(g), dim3(64 * WW), shm, st, m,      \
                        R->frp,                                                               \
                        (const uint16_t*)R->lcol, (const int64_t*)R->lup,                     \
@@ -2833,6 +2851,7 @@ int build_union_index(raptr_recon* R, hipStream_t st) {
   const double mean_row = double(R->pnnz) / double(std::max(R->m, 1));
   R->fb_steps = std::min(4, std::max(2, int(std::ceil((mean_row + 64.0) / 256.0))));
   R->fb_waves = fused_lds_bytes(R, 4, true) * 5 <= kLdsMax ? 4 : 8;
+  fused_lds_attributes();
   return RAPTR_OK;
 }
 
