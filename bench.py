@@ -11,10 +11,11 @@ transpose and the FP64 sensitivity are built once and stay in HBM.
 Modes on that workload (a step = one batch of frames, each frame run for
 50 iterations, replayed from a HIP graph):
   plain    one frame at a time, FP32 values            (the plain GPU path)
-  batched  F frames per launch, FP32 values
-  mixed    F frames per launch, 16-bit values (UNORM16 fixed point, FP16,
-           BF16) with per-row FP32 scale, FP32 accumulation, FP64
-           normalisation
+  batched  F frames per launch, FP32 values; two batches side by side (the
+           plan and a clone sharing its matrix, on two streams), so a step
+           is 2 x F frames (--side-by-side 0: one batch)
+  mixed    as batched, 16-bit values (UNORM16 fixed point, FP16, BF16) with
+           per-row FP32 scale, FP32 accumulation, FP64 normalisation
 Frame 0 of every mode is the 2e7-event data set; its image is compared with
 the FP64 host recurrence (oracle) run on the same matrix -- that run is also
 the timed CPU baseline.  `value` = frames/s of the fastest mode whose error
@@ -80,6 +81,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true",
                     help="skip the FP64 host recurrence (no error figures either)")
     ap.add_argument("--no-small", action="store_true")
+    ap.add_argument("--side-by-side", type=int, default=1,
+                    help="batched modes: 1 runs a second batch of the same frames on a plan "
+                         "clone on a second stream (the step is 2 x F frames); 0 one batch")
     ap.add_argument("--kernel-reps", type=int, default=10)
     ap.add_argument("--modes",
                     default="plain,batched,mixed_u16,mixed_f16,mixed_bf16,plain_u16,plain_f16",
@@ -124,6 +128,9 @@ def main():
     log("device:", dev)
     stream = Stream()
     st = stream.handle
+    stream2 = Stream()  # the second batch of a batched mode (a plan clone)
+    st2 = stream2.handle
+    joined = Event()
     ev0, ev1 = Event(), Event()
 
     def timed(fn):
@@ -238,24 +245,40 @@ def main():
         if nf > 1:
             best_rows, best_order = chosen_rows, chosen_order
         plans[name] = (R, dy, dx, xx0, nf)
-        for _ in range(args.warmup):
-            dx.copy_from(xx0, stream=st)
-            R.run(dy, dx, niter, nf, use_graph=True, stream=st)
-        stream.sync()
+        # batches: a second batch of the same frames on a clone of the plan
+        # (shared matrix, own work vectors) on a second stream, side by side
+        side = None
+        if nf > 1 and args.side_by_side:
+            side = (R.clone(stream=st2), DeviceArray.from_numpy(yy.reshape(-1)),
+                    DeviceArray.from_numpy(xx0.reshape(-1)))
+        nb = 2 if side else 1
 
         def batch():
             dx.copy_from(xx0, stream=st)
             R.run(dy, dx, niter, nf, use_graph=True, stream=st)
+            if side:
+                side[2].copy_from(xx0, stream=st2)
+                side[0].run(side[1], side[2], niter, nf, use_graph=True, stream=st2)
+                joined.record(st2)
+                stream.wait(joined)
 
+        for _ in range(args.warmup):
+            batch()
+        stream.sync()
         ms = timed(lambda: [batch() for _ in range(args.steps)]) / args.steps
         xr = dx.numpy().reshape(n, nf)
-        fps = nf * 1000.0 / ms
+        if side:
+            same = bool(np.array_equal(side[2].numpy(stream=st2).reshape(n, nf), xr))
+            side[0].close()
+            side = None
+        fps = nb * nf * 1000.0 / ms
         # the mixed-precision error figure is quoted at 30 iterations
         dx.copy_from(xx0, stream=st)
         R.run(dy, dx, ERR_ITERS, nf, use_graph=True, stream=st)
         x30 = dx.numpy(stream=st).reshape(n, nf)[:, 0].copy()
         info = R.info()
-        modes[name] = {"value_type": vt, "frames_per_batch": nf, "ms_per_batch": round(ms, 3),
+        modes[name] = {"value_type": vt, "frames_per_batch": nf, "batches_side_by_side": nb,
+                       "ms_per_step": round(ms, 3),
                        "frames_per_s": round(fps, 3), "ms_per_iteration": round(ms / niter, 4),
                        "config": dict(cfg, row_blocks=info["row_blocks"]),
                        "plan_build_s": round(t_plan, 2),
@@ -273,7 +296,9 @@ def main():
         modes[name]["passes_ms"] = {"forward": round(timed(lambda: [pf() for _ in range(3)]) / 3, 3),
                                     "backward": round(timed(lambda: [pb() for _ in range(3)]) / 3, 3)}
         del de_m
-        log(f"{name}: {ms:.1f} ms/batch of {nf} -> {fps:.2f} frames/s "
+        if nb > 1:
+            modes[name]["side_by_side_images_equal"] = same
+        log(f"{name}: {ms:.1f} ms per step of {nb} x {nf} frames -> {fps:.2f} frames/s "
             f"(cfg {cfg}, {info['row_blocks']} row blocks, plan {t_plan:.1f} s, "
             f"passes {modes[name]['passes_ms']})")
         if name == "plain" and nf < F and F > 1:
@@ -593,7 +618,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": modes[best]["ms_per_batch"],
+        "ms_per_step": modes[best]["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -603,7 +628,8 @@ def main():
                                "52^3 grid @ 2 mm, 30 rays, 2e7 events, "
                                f"{niter} iterations/frame, mode '{best}'",
                    "measured_channels": m, "nnz": nnz_full, "voxels": n,
-                   "frames_per_batch": modes[best]["frames_per_batch"]},
+                   "frames_per_batch": modes[best]["frames_per_batch"],
+                   "batches_side_by_side": modes[best]["batches_side_by_side"]},
         "modes": modes,
         "plain_frames_per_s": plain_fps,
         "speedup_vs_plain": modes[best]["speedup_vs_plain"],

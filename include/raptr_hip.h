@@ -93,6 +93,9 @@ int raptr_event_create(void** ev);
 int raptr_event_destroy(void* ev);
 int raptr_event_record(void* ev, raptr_stream_t s);
 int raptr_event_elapsed_ms(float* ms, void* start, void* stop);
+/* Work queued on s after this call waits for ev (hipStreamWaitEvent): joins
+ * two streams' work, e.g. two plans run side by side. */
+int raptr_stream_wait_event(raptr_stream_t s, void* ev);
 int raptr_mem_info(size_t* free_bytes, size_t* total_bytes);
 
 /* ----------------------------------------------- geometry (host side) */
@@ -443,7 +446,22 @@ enum {
 int raptr_locality_order_ex(const raptr_setup_t* setup, const int* d_channels, int m, int tile,
                             int kind, int* d_order, raptr_stream_t stream);
 int raptr_recon_layout(const raptr_recon* plan);
+/* Fails (RAPTR_ERR_INVALID_ARG) while clones of the plan live. */
 int raptr_recon_destroy(raptr_recon* plan);
+/* A second plan over the same matrix (no counterpart in the reference): it
+ * shares the plan's read-only layouts (matrix formats, unions, row order,
+ * sensitivity) and owns only its work vectors (ratios, partial sums,
+ * images, normalisation; 1.3 GB for a 64-frame plan at full size, against
+ * 63 GB of layouts), with the plan's current kernel shapes, which it can
+ * change on its own.  Clones run on their own streams at the same time as
+ * the plan and each other: two 64-frame batches side by side overlap their
+ * gather-bound passes (full size: 80 against 73 frames/s).  A clone of a
+ * clone shares the first plan's layouts.  The plan must outlive its
+ * clones (raptr_recon_destroy refuses it while they live); clone and
+ * destroy calls on one plan must not run concurrently. */
+int raptr_recon_clone(raptr_recon* plan, raptr_recon** out, raptr_stream_t stream);
+/* Live clones of a plan (-1 for a null plan). */
+int raptr_recon_clone_count(const raptr_recon* plan);
 int raptr_recon_info(const raptr_recon* plan, int64_t* device_bytes,
                      int64_t* padded_nnz, int* row_blocks, int* fwd_lanes,
                      int* fwd_unroll, int* bwd_unroll);

@@ -136,6 +136,9 @@ _SIGS = {
     "raptr_recon_set_forward_kind": (i32, [vp, i32]),
     "raptr_recon_get_forward_kind": (i32, [vp, P(i32), P(i32), P(i64)]),
     "raptr_recon_set_fused": (i32, [vp, i32, i32, i32, i32]),
+    "raptr_recon_clone": (i32, [vp, P(vp), vp]),
+    "raptr_recon_clone_count": (i32, [vp]),
+    "raptr_stream_wait_event": (i32, [vp, vp]),
     "raptr_recon_get_fused": (i32, [vp, P(i32), P(i32), P(i32), P(i32), P(i32)]),
     "raptr_recon_set_fused_reduce": (i32, [vp, i32]),
     "raptr_recon_set_fused_steps": (i32, [vp, i32]),

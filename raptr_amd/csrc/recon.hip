@@ -3408,11 +3408,61 @@ int raptr_locality_order_ex(const raptr_setup_t* setup, const int* d_channels, i
 
 int raptr_recon_destroy(raptr_recon* R) {
   if (!R) return RAPTR_OK;
+  if (R->nclones > 0)
+    return fail(RAPTR_ERR_INVALID_ARG, "plan has %d live clone(s): destroy them first",
+                R->nclones);
   clear_graphs(R);
   for (void* p : R->allocs) (void)hipFree(p);
+  if (R->parent) --R->parent->nclones;
   delete R;
   return RAPTR_OK;
 }
+
+int raptr_recon_clone(raptr_recon* P, raptr_recon** out, raptr_stream_t stream) {
+  if (!P || !out) return fail(RAPTR_ERR_INVALID_ARG, "null plan or output");
+  *out = nullptr;
+  if (P->parent) P = P->parent;  // clones of a clone share the same layouts
+  raptr_recon* R = new raptr_recon(*P);
+  // the copy's graphs, allocations and clone count are the parent's: drop
+  // them (not destroyed: they stay the parent's)
+  R->graphs.clear();
+  R->graph_tick = 0;
+  R->allocs.clear();
+  R->asizes.clear();
+  R->bytes = 0;
+  R->nclones = 0;
+  R->parent = P;
+  ++P->nclones;
+  // the work vectors, each the size of the parent's
+  auto own = [&](auto*& p) -> int {
+    using T = std::remove_pointer_t<std::remove_reference_t<decltype(p)>>;
+    if (!p) return RAPTR_OK;
+    size_t bytes = 0;
+    for (size_t i = 0; i < P->allocs.size(); ++i)
+      if (P->allocs[i] == static_cast<const void*>(p)) bytes = P->asizes[i];
+    p = nullptr;
+    return ralloc(R, &p, bytes / sizeof(T) + 1);
+  };
+  int rc = RAPTR_OK;
+  if (!rc) rc = own(R->e);
+  if (!rc) rc = own(R->c);
+  if (!rc) rc = own(R->xx);
+  if (!rc) rc = own(R->partials);
+  if (!rc) rc = own(R->scale);
+  if (!rc) rc = own(R->norm);
+  if (!rc) rc = own(R->c8);
+  if (!rc) rc = own(R->yp);
+  if (!rc) rc = own(R->cpart);
+  if (rc) {
+    raptr_recon_destroy(R);
+    return rc;
+  }
+  (void)stream;
+  *out = R;
+  return RAPTR_OK;
+}
+
+int raptr_recon_clone_count(const raptr_recon* R) { return R ? R->nclones : -1; }
 
 int raptr_recon_info(const raptr_recon* R, int64_t* device_bytes, int64_t* padded_nnz,
                      int* row_blocks, int* fwd_lanes, int* fwd_unroll, int* bwd_unroll) {

@@ -173,6 +173,10 @@ struct raptr_recon {
   int64_t* gbp = nullptr;  // nblk * ngc + 1, (block, column group) order
   int* gbrow = nullptr;
   void* gbval = nullptr;
+  // a clone (raptr_recon_clone) shares its parent's read-only layouts and
+  // owns only its work vectors; a parent counts its live clones
+  raptr_recon* parent = nullptr;
+  int nclones = 0;
   // plan row order: plan row p holds the caller's row rperm[p] (null:
   // identity); yp holds the counts in plan order
   int* rperm = nullptr;

@@ -186,6 +186,11 @@ int raptr_event_record(void* ev, raptr_stream_t s) {
   RAPTR_HIP_TRY(hipEventRecord(reinterpret_cast<hipEvent_t>(ev), S(s)));
   return RAPTR_OK;
 }
+int raptr_stream_wait_event(raptr_stream_t s, void* ev) {
+  if (!ev) return fail(RAPTR_ERR_INVALID_ARG, "null event");
+  RAPTR_HIP_TRY(hipStreamWaitEvent(S(s), reinterpret_cast<hipEvent_t>(ev), 0));
+  return RAPTR_OK;
+}
 int raptr_event_elapsed_ms(float* ms, void* a, void* b) {
   RAPTR_HIP_TRY(hipEventSynchronize(reinterpret_cast<hipEvent_t>(b)));
   RAPTR_HIP_TRY(hipEventElapsedTime(ms, reinterpret_cast<hipEvent_t>(a),

@@ -85,6 +85,10 @@ class Stream:
     def sync(self):
         capi.call("raptr_stream_sync", self.handle)
 
+    def wait(self, event):
+        """Work queued on this stream from now on waits for `event`."""
+        capi.call("raptr_stream_wait_event", self.handle, event.handle)
+
     def __del__(self):
         try:
             if self.handle:

@@ -398,9 +398,26 @@ class Reconstructor:
         self.set_xcd_blocks(xb)
         return best
 
+    def clone(self, stream=None):
+        """A second plan over the same matrix (raptr_recon_clone): it shares
+        this plan's layouts and owns only its work vectors, with this plan's
+        current kernel shapes.  Run it on its own stream beside this plan
+        (two batches at once).  It keeps this plan alive; close it first."""
+        h = C.c_void_p()
+        capi.call("raptr_recon_clone", self.handle, C.byref(h), stream)
+        c = object.__new__(type(self))
+        c.__dict__.update({k: v for k, v in self.__dict__.items() if k != "handle"})
+        c.handle = h.value
+        c._parent = self
+        return c
+
+    def clone_count(self):
+        return capi.lib().raptr_recon_clone_count(self.handle)
+
     def close(self):
+        """Destroy the plan (raises while clones of it live)."""
         if getattr(self, "handle", None):
-            capi.lib().raptr_recon_destroy(self.handle)
+            capi.call("raptr_recon_destroy", self.handle)
             self.handle = None
 
     def __del__(self):
