@@ -11,9 +11,9 @@ transpose and the FP64 sensitivity are built once and stay in HBM.
 Modes on that workload (a step = one batch of frames, each frame run for
 50 iterations, replayed from a HIP graph):
   plain    one frame at a time, FP32 values            (the plain GPU path)
-  batched  F frames per launch, FP32 values; two batches side by side (the
-           plan and a clone sharing its matrix, on two streams), so a step
-           is 2 x F frames (--side-by-side 0: one batch)
+  batched  F frames per launch, FP32 values; three batches side by side
+           (the plan and two clones sharing its matrix, on their own
+           streams), so a step is 3 x F frames (--side-by-side 1: one batch)
   mixed    as batched, 16-bit values (UNORM16 fixed point, FP16, BF16) with
            per-row FP32 scale, FP32 accumulation, FP64 normalisation
 Frame 0 of every mode is the 2e7-event data set; its image is compared with
@@ -81,9 +81,10 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true",
                     help="skip the FP64 host recurrence (no error figures either)")
     ap.add_argument("--no-small", action="store_true")
-    ap.add_argument("--side-by-side", type=int, default=1,
-                    help="batched modes: 1 runs a second batch of the same frames on a plan "
-                         "clone on a second stream (the step is 2 x F frames); 0 one batch")
+    ap.add_argument("--side-by-side", type=int, default=3,
+                    help="batched modes: batches of the same frames run side by side, the "
+                         "plan and plan clones on their own streams (a step is this many x F "
+                         "frames); 1: one batch")
     ap.add_argument("--kernel-reps", type=int, default=10)
     ap.add_argument("--modes",
                     default="plain,batched,mixed_u16,mixed_f16,mixed_bf16,plain_u16,plain_f16",
@@ -128,9 +129,9 @@ def main():
     log("device:", dev)
     stream = Stream()
     st = stream.handle
-    stream2 = Stream()  # the second batch of a batched mode (a plan clone)
-    st2 = stream2.handle
-    joined = Event()
+    # the other batches of a batched mode (plan clones), one stream each
+    st_more = [Stream() for _ in range(max(0, args.side_by_side - 1))]
+    joined = [Event() for _ in st_more]
     ev0, ev1 = Event(), Event()
 
     def timed(fn):
@@ -247,30 +248,38 @@ def main():
         plans[name] = (R, dy, dx, xx0, nf)
         # batches: a second batch of the same frames on a clone of the plan
         # (shared matrix, own work vectors) on a second stream, side by side
-        side = None
-        if nf > 1 and args.side_by_side:
-            side = (R.clone(stream=st2), DeviceArray.from_numpy(yy.reshape(-1)),
-                    DeviceArray.from_numpy(xx0.reshape(-1)))
-        nb = 2 if side else 1
+        sides = []
+        if nf > 1:
+            for k in range(max(0, args.side_by_side - 1)):
+                sides.append((R.clone(stream=st_more[k].handle),
+                              DeviceArray.from_numpy(yy.reshape(-1)),
+                              DeviceArray.from_numpy(xx0.reshape(-1)), st_more[k]))
+        nb = 1 + len(sides)
 
         def batch():
             dx.copy_from(xx0, stream=st)
             R.run(dy, dx, niter, nf, use_graph=True, stream=st)
-            if side:
-                side[2].copy_from(xx0, stream=st2)
-                side[0].run(side[1], side[2], niter, nf, use_graph=True, stream=st2)
-                joined.record(st2)
-                stream.wait(joined)
+            for C_, dy_, dx_, s_ in sides:
+                dx_.copy_from(xx0, stream=s_.handle)
+                C_.run(dy_, dx_, niter, nf, use_graph=True, stream=s_.handle)
+            for k, (_, _, _, s_) in enumerate(sides):
+                joined[k].record(s_.handle)
+                stream.wait(joined[k])
 
-        for _ in range(args.warmup):
+        # the 16-bit batched modes miss the error bar (never the value):
+        # a shorter timing
+        steps, warm = ((min(args.steps, 5), min(args.warmup, 1)) if name.startswith("mixed")
+                       else (args.steps, args.warmup))
+        for _ in range(warm):
             batch()
         stream.sync()
-        ms = timed(lambda: [batch() for _ in range(args.steps)]) / args.steps
+        ms = timed(lambda: [batch() for _ in range(steps)]) / steps
         xr = dx.numpy().reshape(n, nf)
-        if side:
-            same = bool(np.array_equal(side[2].numpy(stream=st2).reshape(n, nf), xr))
-            side[0].close()
-            side = None
+        same = all(np.array_equal(dx_.numpy(stream=s_.handle).reshape(n, nf), xr)
+                   for _, _, dx_, s_ in sides)
+        for C_, _, _, _ in sides:
+            C_.close()
+        sides = []
         fps = nb * nf * 1000.0 / ms
         # the mixed-precision error figure is quoted at 30 iterations
         dx.copy_from(xx0, stream=st)
@@ -278,7 +287,7 @@ def main():
         x30 = dx.numpy(stream=st).reshape(n, nf)[:, 0].copy()
         info = R.info()
         modes[name] = {"value_type": vt, "frames_per_batch": nf, "batches_side_by_side": nb,
-                       "ms_per_step": round(ms, 3),
+                       "ms_per_step": round(ms, 3), "steps_timed": steps,
                        "frames_per_s": round(fps, 3), "ms_per_iteration": round(ms / niter, 4),
                        "config": dict(cfg, row_blocks=info["row_blocks"]),
                        "plan_build_s": round(t_plan, 2),
