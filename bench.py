@@ -59,10 +59,11 @@ MEASURED_TRAFFIC = {
     ("k_bwd1x", 2, 69, "y", "r16", 32): 2 * 5467831 * 1024 + 10202 * 1024,
     ("k_bwd1x", 2, 368, "y", "r16", 32): 2 * 5855002 * 1024 + 8618 * 1024,
     # the fused iteration's products kernel (kernel, mode, waves, lanes per
-    # row, row order) and its per-XCD column reduce (kernel, lanes per
-    # column, row order); one launch each per iteration
-    ("k_fb1l", 2, 4, 64, "y"): 2 * 5035683 * 1024 + 465498 * 1024,
-    ("k_fb1_reduce8", 16, "y"): 2 * 1028086 * 1024 + 6883 * 1024,
+    # row, row order, plan rows per union group) and its column reduce
+    # (kernel, lanes per column or segment slot, row order, rows per union
+    # group); one launch each per iteration
+    ("k_fb1l", 2, 4, 64, "y", 64): 2 * 5035683 * 1024 + 465498 * 1024,
+    ("k_fb1_reduce8", 16, "y", 64): 2 * 1028086 * 1024 + 6883 * 1024,
 }
 
 
@@ -431,34 +432,50 @@ def main():
         ms_fb = timed(lambda: [fb1() for _ in range(reps)]) / reps
         ms_red = timed(lambda: [fb2() for _ in range(reps)]) / reps
         nu = R.forward_union_entries()
+        urows = R.unions()["rows"]
+        tz = R.fused_tiles()
         spmv = 8 * A.nnz + 4 * (m + 1) + 4 * m + 4 * n  # one direction
         bytes_fb = 2 * spmv
         # stored: 6 bytes per padded entry (FP32 value, 16-bit union rank),
         # row pointers, y, e, per union entry its column and x, the partial
         # sums written
         stored_fb = (6 * info["padded_nnz"] + 8 * (m + 1) + 8 * m + 8 * nu + 4 * nu
-                     + 8 * (-(-m // 64) + 1))
+                     + 8 * (-(-m // urows) + 1))
         it_bytes = 2 * 8 * A.nnz + 8 * (m + 1) + 4 * 3 * m + 4 * 6 * n  # one iteration
         ms_it = modes["plain"]["ms_per_iteration"]
         key = ("k_fb1l", info["fused"], info["fused_waves"], info["fused_lanes_per_row"],
-               cfg_p["row_order"])
+               cfg_p["row_order"], urows)
         fused = {"kernel": f"k_fb1l<f32, {info['fused_waves']} waves, "
                            f"{info['fused_lanes_per_row']} lanes per row, "
                            + ("one shared column-sum region" if info["fused"] == 2 else
                               "a column-sum region per wave")
-                           + "> (forward + ratio + backward partial sums over each 64-row "
+                           + f"> (forward + ratio + backward partial sums over each {urows}-row "
                              "group's column union, one pass over the matrix)",
                  "ms": ms_fb, "bytes": bytes_fb, "gbs": bytes_fb / (ms_fb * 1e-3) / 1e9,
                  "stored": stored_fb, "key": key,
-                 "reduce": {"kernel": ("k_fb1_reduce8 + k_bwd1_xreduce (group partials per "
-                                       "column in per-XCD group ranges, then the update)"
-                                       if info["fused_reduce_per_xcd"] else
-                                       "k_fb1_reduce (group partials per column, update)"),
+                 "reduce": {"kernel": (
+                                f"k_fb1_treduce<{info['fused_lanes_per_column']} lanes per "
+                                f"slot> + k_fb1_tcombine (per part of a tile of "
+                                f"{tz['columns']} columns, {tz['segments_per_workgroup']} "
+                                f"segments: the groups' partials in written order into LDS "
+                                f"column sums; then the parts in order and the update)"
+                                if info["fused_reduce_per_xcd"] == 2 else
+                                "k_fb1_reduce8 + k_bwd1_xreduce (group partials per column "
+                                "in per-XCD group ranges, then the update)"
+                                if info["fused_reduce_per_xcd"] else
+                                "k_fb1_reduce (group partials per column, update)"),
                             "ms": round(ms_red, 4),
-                            "algorithmic_bytes": int(4 * 2 * nu + 8 * (n + 1) + 4 * 4 * n),
-                            "traffic": MEASURED_TRAFFIC.get(("k_fb1_reduce8",
-                                                             info["fused_lanes_per_column"],
-                                                             cfg_p["row_order"]))},
+                            # mode 2: each partial and its column once, the
+                            # segment bounds, the part sums and x, s, x';
+                            # else each partial and its position once
+                            "algorithmic_bytes": int(
+                                8 * nu + 8 * tz["segments"] + 4 * tz["max_parts"] * n + 12 * n
+                                if info["fused_reduce_per_xcd"] == 2 else
+                                4 * 2 * nu + 8 * (n + 1) + 4 * 4 * n),
+                            "traffic": MEASURED_TRAFFIC.get((
+                                "k_fb1_treduce" if info["fused_reduce_per_xcd"] == 2 else
+                                "k_fb1_reduce8", info["fused_lanes_per_column"],
+                                cfg_p["row_order"], urows))},
                  "iteration": {"ms": ms_it, "algorithmic_bytes": it_bytes,
                                "achieved": round(it_bytes / (ms_it * 1e-3) / 1e9, 1),
                                "frac": round(it_bytes / (ms_it * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),

@@ -510,37 +510,70 @@ int raptr_recon_get_forward_kind(const raptr_recon* plan, int* kind, int* availa
 /* Fused single-frame iteration.  Replaces the pair of products
  * cusparseScsrmv(NON_TRANSPOSE) / cusparseScsrmv(TRANSPOSE) of one MLEM
  * iteration (reco.cu:257-270, csrmv.hpp:72-83) by one pass over the
- * matrix: per group of 64 consecutive plan rows, x over the group's column
- * union in LDS, each row's ratio e_i = y_i / (A x)_i, and A_ij e_i added
- * into column sums over the union in LDS, written as the group's partial
- * sums; a second kernel adds each column's partials in group order and
+ * matrix: per group of consecutive plan rows (raptr_recon_set_union_rows),
+ * x over the group's column union in LDS, each row's ratio
+ * e_i = y_i / (A x)_i, and A_ij e_i added into column sums over the union
+ * in LDS, written as the group's partial sums; a second kernel adds each
+ * column's partials in a fixed order (raptr_recon_set_fused_reduce) and
  * applies the update.  Bit-reproducible run to run (fixed orders, no
  * atomics).  Built with the LDS-staged forward's unions where no row
- * repeats a column; adds 8 bytes per union entry and 8 per voxel to the
- * plan.
+ * repeats a column; adds 8 bytes per union entry, 8 per segment of the
+ * column-tile reduce and 8 per voxel to the plan.
  *   on: 0 the separate forward and backward passes; 1 fused, one column-sum
- *       region per wave (520 + 4 x (largest union + waves x (largest union
- *       + 1)) bytes of LDS must fit 160 KiB); 2 fused, one region per
- *       workgroup that the waves add to in turn (520 + 4 x (2 x largest
- *       union + 1) bytes; the default);
+ *       region per wave (8 x (rows + 1) + 4 x (largest union + waves x
+ *       (largest union + 1)) bytes of LDS must fit 160 KiB); 2 fused, one
+ *       region per workgroup that the waves add to in turn (8 x (rows + 1)
+ *       + 4 x (2 x largest union + 1) bytes; the default);
  *   waves: waves per row group (2 or 4; 8 with on = 2; default 4 where five
  *          such workgroups fit a CU's 160 KiB of LDS, else 8);
  *   lanes_per_row: lanes per matrix row (16, 32 or 64);
- *   lanes_per_column: lanes per column of the reduce (16, 32 or 64).
+ *   lanes_per_column: lanes per column of the reduce, or per segment slot
+ *       of the column-tile reduce (16, 32 or 64).
  * get: available = 1 if the plan has the fused pass; null outputs are
  * skipped. */
 int raptr_recon_set_fused(raptr_recon* plan, int on, int waves, int lanes_per_row,
                           int lanes_per_column);
 int raptr_recon_get_fused(const raptr_recon* plan, int* available, int* on, int* waves,
                           int* lanes_per_row, int* lanes_per_column);
-/* Column reduce of the fused iteration: per_xcd = 1 (the default) splits
- * the groups into 8 ranges, one per XCD (workgroup i runs on XCD i % 8),
- * so each XCD's L2 holds an eighth of the partial sums, and adds the 8
- * per-XCD sums of a column in XCD order; 0 adds all of a column's group
- * partials in one pass.  Both are fixed-order; their sums differ in
- * rounding.  get: the setting (-1 for a null plan). */
-int raptr_recon_set_fused_reduce(raptr_recon* plan, int per_xcd);
+/* Column reduce of the fused iteration (the transposed product's sums,
+ * reco.cu:268-270), by mode:
+ *   0: all of a column's group partials in one pass, by column;
+ *   1: the groups split into 8 ranges, one per XCD (workgroup i runs on
+ *      XCD i % 8), so each XCD's L2 holds an eighth of the partial sums;
+ *      the 8 range sums of a column added in order;
+ *   2 (the default where the plan has its segment table): column tiles:
+ *      per part of a tile's groups, their partials read in the order the
+ *      products kernel wrote them and added into per-slot column sums in
+ *      LDS, then in slot and part order (raptr_recon_set_fused_tiles).
+ * All are fixed-order; their sums differ in rounding.  get: the mode (-1
+ * for a null plan). */
+int raptr_recon_set_fused_reduce(raptr_recon* plan, int mode);
 int raptr_recon_get_fused_reduce(const raptr_recon* plan);
+/* Plan rows per column-union group (the LDS-staged forward and the fused
+ * iteration): rows = 0 lets the plan choose (the largest of 256, 128, 64
+ * and 32 whose fused workgroup of 8 waves leaves room in LDS for three on a
+ * CU, else the largest that leaves room for two, else 64), else a power
+ * of two in 16..512.  Rebuilds the unions, the
+ * fused pass's index and the column-tile table, drops the plan's captured
+ * iterations and restores the fused defaults.  Refused on a clone or a plan
+ * with clones, which share them, and on plans other than single-frame
+ * blocked ones.  get: rows, the largest union and the union entries of all
+ * groups (zeros: no unions).  No counterpart in the reference. */
+int raptr_recon_set_union_rows(raptr_recon* plan, int rows, raptr_stream_t stream);
+int raptr_recon_get_unions(const raptr_recon* plan, int* rows, int* largest, int64_t* entries);
+/* Shape of the column-tile reduce: columns per tile (a power of two,
+ * 128-2048) and segments per workgroup (64-1,048,576; a tile's segments,
+ * a group's union positions inside the tile, 8 bytes each, are cut into
+ * parts of about that many, one workgroup and one partial image row each).
+ * The plan builds 1,024 columns and 512 segments.  Rebuilds the table and
+ * drops the plan's captured iterations.  Refused on a clone or a plan with
+ * clones, which share the table.  get: the shape (0: no table), the
+ * segments, workgroups and the largest number of parts of a tile; null
+ * outputs are skipped.  No counterpart in the reference. */
+int raptr_recon_set_fused_tiles(raptr_recon* plan, int columns, int segments_per_workgroup,
+                                raptr_stream_t stream);
+int raptr_recon_get_fused_tiles(const raptr_recon* plan, int* columns, int* segments_per_workgroup,
+                                int64_t* segments, int* workgroups, int* max_parts);
 /* Register-held steps of 4 x 64 entries per row of the fused products
  * kernel with 64 lanes per row (2, 3 or 4; rows past them continue from
  * memory).  The plan's default: (mean padded row + 64) / 256, rounded up

@@ -144,6 +144,10 @@ _SIGS = {
     "raptr_recon_set_fused_steps": (i32, [vp, i32]),
     "raptr_recon_get_fused_steps": (i32, [vp]),
     "raptr_recon_get_fused_reduce": (i32, [vp]),
+    "raptr_recon_set_fused_tiles": (i32, [vp, i32, i32, vp]),
+    "raptr_recon_set_union_rows": (i32, [vp, i32, vp]),
+    "raptr_recon_get_unions": (i32, [vp, P(i32), P(i32), P(i64)]),
+    "raptr_recon_get_fused_tiles": (i32, [vp, P(i32), P(i32), P(i64), P(i32), P(i32)]),
     "raptr_recon_fused_pass": (i32, [vp, vp, vp, i32, vp]),
     "raptr_locality_order": (i32, [vp, vp, i32, i32, vp, vp]),
     "raptr_locality_order_ex": (i32, [vp, vp, i32, i32, i32, vp, vp]),

@@ -330,8 +330,8 @@ __global__ __launch_bounds__(256) void k_fwd1c(int64_t m, const int64_t* __restr
   }
 }
 
-// Single frame with x staged in LDS: a workgroup takes kLocRows
-// consecutive plan rows (neighbouring lines through the same voxels in the
+// Single frame with x staged in LDS: a workgroup takes `rows` (the plan's
+// union group size) consecutive plan rows (neighbouring lines through the same voxels in the
 // locality row order), copies x over the sorted union of their columns into
 // LDS, then runs k_fwd1c's loop with each entry's 16-bit index into that
 // union.  The matrix stream stays 6 bytes per FP32 entry; the gathers read
@@ -339,11 +339,13 @@ __global__ __launch_bounds__(256) void k_fwd1c(int64_t m, const int64_t* __restr
 // stream uses ordinary cached loads: rows share the cache lines at their
 // boundaries, which non-temporal loads fetched twice (12.2 -> 10.3 GB of
 // HBM reads per pass at full size; x lives in LDS, so nothing is evicted).
-constexpr int kLocRows = 64;
 constexpr int kLocMaxUnion = 16384;  // 64 KB of LDS per workgroup
+// union group sizes a plan may take (plan rows per group), largest first
+constexpr int kLocRowsChoices[] = {256, 128, 64, 32};
 
 template <int VT, int LANES, int U>
-__global__ __launch_bounds__(256) void k_fwd1l(int64_t m, const int64_t* __restrict__ rp,
+__global__ __launch_bounds__(256) void k_fwd1l(int64_t m, int rows,
+                                               const int64_t* __restrict__ rp,
                                                const uint16_t* __restrict__ lcol,
                                                const int64_t* __restrict__ up,
                                                const int* __restrict__ ucol,
@@ -359,7 +361,7 @@ __global__ __launch_bounds__(256) void k_fwd1l(int64_t m, const int64_t* __restr
   for (int j = threadIdx.x; j < nu; j += 256) xs[j] = x[ucol[ub + j]];
   __syncthreads();
   const int g = threadIdx.x % LANES;
-  const int64_t r0 = grp * kLocRows, r1 = r0 + kLocRows < m ? r0 + kLocRows : m;
+  const int64_t r0 = grp * rows, r1 = r0 + rows < m ? r0 + rows : m;
   for (int64_t i = r0 + threadIdx.x / LANES; i < r1; i += 256 / LANES) {
     const int64_t b = rp[i], en = rp[i + 1];  // multiples of 4
     float acc = 0.0f;
@@ -1424,7 +1426,7 @@ __global__ __launch_bounds__(256) void k_bwd1_xreduce(int n, const float* __rest
 
 // ------------------------------------------------------------------------
 // Single frame, both products in one pass over the matrix.  A workgroup of
-// W waves takes one group of kLocRows consecutive plan rows (the groups of
+// W waves takes one group of `rows` consecutive plan rows (the groups of
 // the LDS-staged forward) and copies x over the group's column union into
 // LDS.  The group's rows go out in batches: in each, every wave takes
 // 64 / LANES rows, LANES lanes per row, with S steps of each row's entries
@@ -1480,7 +1482,8 @@ __device__ __forceinline__ float slot_sum(float v) {
 // their rows in turn (wave order, a barrier between turns) -- a third of
 // the LDS of W regions, so about three times the workgroups per CU
 template <int VT, int W, int LANES, int S, bool SH>
-__global__ __launch_bounds__(64 * W) void k_fb1l(int64_t m, const int64_t* __restrict__ rp,
+__global__ __launch_bounds__(64 * W) void k_fb1l(int64_t m, int rows,
+                                                 const int64_t* __restrict__ rp,
                                                  const uint16_t* __restrict__ lcol,
                                                  const int64_t* __restrict__ up,
                                                  const int* __restrict__ ucol,
@@ -1493,22 +1496,22 @@ __global__ __launch_bounds__(64 * W) void k_fb1l(int64_t m, const int64_t* __res
   constexpr int RPW = 64 / LANES;  // rows per wave and batch
   constexpr int STEP = 4 * LANES;  // entries of a row per step
   constexpr int NT = 64 * W;
-  constexpr int kBatches = (kLocRows + W * RPW - 1) / (W * RPW);
+  const int kBatches = (rows + W * RPW - 1) / (W * RPW);
   constexpr int NR = SH ? 1 : W;  // column-sum regions
-  // the group's kLocRows + 1 row bounds, x over the union (nus floats),
+  // the group's rows + 1 row bounds, x over the union (nus floats),
   // then NR column-sum regions of nus + 1 floats each (the last a spare
   // slot)
   extern __shared__ int64_t fb_lds[];
   int64_t* rps = fb_lds;
-  float* xs = reinterpret_cast<float*>(fb_lds + kLocRows + 1);
+  float* xs = reinterpret_cast<float*>(fb_lds + rows + 1);
   const int tid = int(threadIdx.x);
   const int64_t grp = blockIdx.x;
   const int64_t ub = up[grp];
   const int nu = int(up[grp + 1] - ub);
   const int nus = nu > 0 ? nu : 1;
-  const int64_t r0 = grp * kLocRows;
-  const int nr = int(r0 + kLocRows < m ? kLocRows : m - r0);  // >= 1
-  if (tid <= kLocRows) rps[tid] = rp[r0 + (tid < nr ? tid : nr)];
+  const int64_t r0 = grp * rows;
+  const int nr = int(r0 + rows < m ? rows : m - r0);  // >= 1
+  for (int i = tid; i <= rows; i += NT) rps[i] = rp[r0 + (i < nr ? i : nr)];
   // x over the union, 8 gathers in flight per thread
   for (int j0 = 0; j0 < nu; j0 += NT * 8) {
     int cc[8];
@@ -1787,20 +1790,199 @@ __global__ void k_vptr(const uint64_t* __restrict__ keys, int64_t nu, int n,
   }
 }
 
+// Column-tile form of the sums: workgroup w takes part k of tile t
+// (twg[w] = {t, k, first segment, end segment}: a run of the tile's
+// segments in group order) and adds its partials for the 2^tshift columns
+// of the tile.  The segments (a group's union positions inside the tile:
+// distinct columns, ascending) are split into NREG consecutive runs, one
+// per slot of LPS lanes, each with its own column sums in LDS, so the
+// lanes of one read-add-write never share a column.  A slot reads its
+// segments' partials and columns in the order k_fb1l wrote them (no
+// gather): it holds the bounds of LPS segments in its lanes (the next LPS
+// already in flight), cuts them into chunks of LPS positions and loads U
+// chunks at a time before adding them in order.  Then each column's NREG
+// sums are added in slot order into ck[k][j]; k_fb1_tcombine adds a
+// column's parts in order.  Fixed order, deterministic.
+template <int LPS, int W>
+__global__ __launch_bounds__(64 * W) void k_fb1_treduce(int n, int tshift,
+                                                        const int4* __restrict__ twg,
+                                                        const int2* __restrict__ tseg,
+                                                        const int* __restrict__ ucol,
+                                                        const float* __restrict__ cpart,
+                                                        float* __restrict__ ck) {
+  constexpr int SUB = 64 / LPS;
+  constexpr int NREG = W * SUB;
+  constexpr int U = 8;
+  extern __shared__ float tr_lds[];
+  const int T = 1 << tshift;
+  const int tid = int(threadIdx.x);
+  const int4 wg = twg[blockIdx.x];
+  const int t = wg.x, kp = wg.y;
+  const int j0 = t << tshift;
+  for (int i = tid; i < NREG * T; i += 64 * W) tr_lds[i] = 0.0f;
+  __syncthreads();
+  const int s0 = wg.z, s1 = wg.w;
+  const int lane = tid % 64, sl = lane % LPS, slot = lane / LPS;
+  const int r = (tid / 64) * SUB + slot;
+  const int64_t cnt = s1 - s0;
+  const int a = s0 + int(cnt * r / NREG), b = s0 + int(cnt * (r + 1) / NREG);
+  float* sums = tr_lds + r * T;
+  const uint64_t smask = LPS == 64 ? ~uint64_t(0) : ((uint64_t(1) << LPS) - 1) << (slot * LPS);
+  // the first window's bounds; lanes past the slot's segments load segment
+  // a (the loop runs only if a < b) and hold an empty range
+  int2 nxt;
+  {
+    const bool in = a + sl < b;
+    const int2 v = tseg[in ? a + sl : a];
+    nxt = in ? v : int2{0, 0};
+  }
+  for (int s = a; s < b; s += LPS) {
+    const int2 se = nxt;
+    {
+      const bool in = s + LPS + sl < b;
+      const int2 v = tseg[in ? s + LPS + sl : a];
+      nxt = in ? v : int2{0, 0};
+    }
+    // chunks per segment and their inclusive prefix over the slot's lanes
+    const int nch = (se.y - se.x + LPS - 1) / LPS;
+    int incl = nch;
+#pragma unroll
+    for (int off = 1; off < LPS; off <<= 1) {
+      const int o = __shfl_up(incl, off, LPS);
+      if (sl >= off) incl += o;
+    }
+    const int excl = incl - nch;
+    const int total = __shfl(incl, LPS - 1, LPS);
+    for (int q0 = 0; q0 < total; q0 += U) {
+      int c[U];
+      float v[U];
+      bool ok[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int q = q0 + u;
+        // segment of chunk q: the lanes whose chunks end at or before q
+        const int i = min(__popcll(__ballot(incl <= q) & smask), LPS - 1);
+        const int p0 = __shfl(se.x, i, LPS), p1 = __shfl(se.y, i, LPS);
+        const int p = p0 + (q - __shfl(excl, i, LPS)) * LPS + sl;
+        ok[u] = q < total && p < p1;
+        const int pp = ok[u] ? p : 0;  // position 0 exists (the unions are not empty)
+        c[u] = ucol[pp] - j0;
+        v[u] = cpart[pp];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (ok[u]) sums[c[u]] += v[u];
+    }
+  }
+  __syncthreads();
+  for (int i = tid; i < T; i += 64 * W) {
+    const int j = j0 + i;
+    if (j >= n) break;
+    float acc = 0.0f;
+#pragma unroll
+    for (int k = 0; k < NREG; ++k) acc += tr_lds[k * T + i];
+    ck[int64_t(kp) * n + j] = acc;
+  }
+}
+
+// c_j = the sum of column j's tk[tile] part sums in part order, then the
+// update x'_j = x_j c_j / (s_j + (s_j == 0)) and an FP64 partial sum per
+// workgroup (as k_bwd1_xreduce)
+__global__ __launch_bounds__(256) void k_fb1_tcombine(int n, int tshift, const int* __restrict__ tk,
+                                                      const float* __restrict__ ck,
+                                                      const float* __restrict__ x,
+                                                      const float* __restrict__ s,
+                                                      float* __restrict__ xx,
+                                                      double* __restrict__ partials) {
+  const int64_t j = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  double part = 0.0;
+  if (j < n) {
+    const int K = tk[j >> tshift];
+    float cv = 0.0f;
+    for (int q = 0; q < K; ++q) cv += ck[int64_t(q) * n + j];
+    float d = s[j];
+    d += float(d == 0.0f);
+    const float nv = x[j] * cv / d;
+    xx[j] = nv;
+    part = double(nv);
+  }
+  __shared__ double red[256];
+  red[threadIdx.x] = part;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
+}
+
+// segment starts: the first position of every group and every position
+// whose column lies in another tile than the one before it
+__global__ void k_seg_flags(const int* __restrict__ ucol, int64_t nu, int tshift,
+                            uint32_t* __restrict__ flag) {
+  for (int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; p < nu;
+       p += int64_t(gridDim.x) * blockDim.x)
+    flag[p] = (p == 0 || (ucol[p] >> tshift) != (ucol[p - 1] >> tshift)) ? 1u : 0u;
+}
+__global__ void k_seg_gstarts(const int64_t* __restrict__ up, int64_t ng, int64_t nu,
+                              uint32_t* __restrict__ flag) {
+  for (int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; g < ng;
+       g += int64_t(gridDim.x) * blockDim.x)
+    if (up[g] < nu) flag[up[g]] = 1u;
+}
+// segment s (in position order) starts at ps[s]; ps[nseg] = nu; key: its tile
+__global__ void k_seg_emit(const int* __restrict__ ucol, int64_t nu, int tshift,
+                           const uint32_t* __restrict__ flag, const uint32_t* __restrict__ idx,
+                           uint64_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                           int* __restrict__ ps) {
+  for (int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; p < nu;
+       p += int64_t(gridDim.x) * blockDim.x) {
+    if (flag[p]) {
+      const uint32_t s = idx[p];
+      keys[s] = uint64_t(uint32_t(ucol[p] >> tshift));
+      vals[s] = s;
+      ps[s] = int(p);
+    }
+    if (p == nu - 1) ps[idx[p] + flag[p]] = int(nu);
+  }
+}
+__global__ void k_seg_pairs(const uint32_t* __restrict__ sorted, int64_t nseg,
+                            const int* __restrict__ ps, int2* __restrict__ tseg) {
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < nseg;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const uint32_t s = sorted[i];
+    tseg[i] = int2{ps[s], ps[s + 1]};
+  }
+}
+// tsb[t] = first segment (sorted by tile) of tile t, t = 0 .. ntiles
+__global__ void k_seg_bounds(const uint64_t* __restrict__ keys, int64_t nseg, int ntiles,
+                             int* __restrict__ tsb) {
+  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t <= ntiles;
+       t += int64_t(gridDim.x) * blockDim.x) {
+    int64_t lo = 0, hi = nseg;
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) / 2;
+      if (keys[mid] < uint64_t(t)) lo = mid + 1; else hi = mid;
+    }
+    tsb[t] = int(lo);
+  }
+}
+
 // *dup = 1 if a row holds one union rank twice among its entries of
 // nonzero stored value (the fused pass adds each such entry to its column
 // sum with a plain read-add-write, exact only for distinct columns).  One
-// workgroup per group of kLocRows rows, a bitmap of `words` words per wave
+// workgroup per group of `rows` rows, a bitmap of `words` words per wave
 template <int VT>
-__global__ __launch_bounds__(256) void k_fb_distinct(int64_t m, const int64_t* __restrict__ rp,
+__global__ __launch_bounds__(256) void k_fb_distinct(int64_t m, int rows,
+                                                     const int64_t* __restrict__ rp,
                                                      const uint16_t* __restrict__ lcol,
                                                      const typename Val<VT>::T* __restrict__ val,
                                                      int words, int* __restrict__ dup) {
   extern __shared__ uint32_t fb_bm[];
   const int w = int(threadIdx.x) / 64, lane = int(threadIdx.x) % 64;
   uint32_t* bm = fb_bm + int64_t(w) * words;
-  const int64_t r0 = int64_t(blockIdx.x) * kLocRows;
-  const int nr = int(r0 + kLocRows < m ? kLocRows : m - r0);
+  const int64_t r0 = int64_t(blockIdx.x) * rows;
+  const int nr = int(r0 + rows < m ? rows : m - r0);
   int found = 0;
   for (int t = w; t < nr; t += 4) {
     for (int k = lane; k < words; k += 64) bm[k] = 0u;
@@ -1921,7 +2103,7 @@ __global__ __launch_bounds__(256) void k_fwd_cols16(const int64_t* __restrict__ 
 }
 
 // Column unions of the LDS-staged forward (k_fwd1l), one workgroup per
-// group of kLocRows plan rows.  The voxels are taken in windows of
+// group of `rows` plan rows.  The voxels are taken in windows of
 // kUnionWin (256,000: a window's bitmap and popcount prefix, 8,000 words
 // each, and 257 partials fit 64 KB of LDS); per window a bitmap of the group's columns in it
 // and the words' exclusive popcount prefix, offset by the union entries of
@@ -1933,7 +2115,8 @@ constexpr int kUnionWinWords = 8000;
 constexpr int kUnionWin = kUnionWinWords * 32;
 __global__ __launch_bounds__(256) void k_loc_union(const int64_t* __restrict__ frp,
                                                    const int* __restrict__ fcol, int64_t m,
-                                                   int n, int pass, int64_t* __restrict__ cnt,
+                                                   int rows, int n, int pass,
+                                                   int64_t* __restrict__ cnt,
                                                    const int64_t* __restrict__ up,
                                                    int* __restrict__ ucol,
                                                    uint16_t* __restrict__ lcol) {
@@ -1946,7 +2129,7 @@ __global__ __launch_bounds__(256) void k_loc_union(const int64_t* __restrict__ f
   uint32_t* part = lw + 2 * nwmax;
   uint32_t& win_total = part[256];
   const int64_t grp = blockIdx.x;
-  const int64_t r0 = grp * kLocRows, r1 = r0 + kLocRows < m ? r0 + kLocRows : m;
+  const int64_t r0 = grp * rows, r1 = r0 + rows < m ? r0 + rows : m;
   const int64_t d0 = frp[r0], d1 = frp[r1];
   const int64_t ub = pass ? up[grp] : 0;
   uint32_t base = 0;  // union entries of the windows before (same in every thread)
@@ -2149,11 +2332,12 @@ int launch_fwd_any(const raptr_recon* R, int F, const float* x, const float* y, 
   const int U = R->fwd_unroll;
   if (F == 1 && R->fwd_kind == 2 && R->lcol) {
     const int L = R->fwd_lanes;
-    const unsigned g = unsigned((int64_t(m) + kLocRows - 1) / kLocRows);
+    const unsigned g = unsigned((int64_t(m) + R->lrows - 1) / R->lrows);
     const size_t shm = size_t(R->lmax > 0 ? R->lmax : 1) * sizeof(float);
 #define RAPTR_FL(VV, LL, UU)                                                                 \
   if (R->vt == VV && L == LL && U == UU) {                                                   \
-    hipLaunchKernelGGL((k_fwd1l<VV, LL, UU>), dim3(g), dim3(256), shm, st, m, R->frp,         \
+    hipLaunchKernelGGL((k_fwd1l<VV, LL, UU>), dim3(g), dim3(256), shm, st, m, R->lrows,       \
+                       R->frp,                                                               \
                        (const uint16_t*)R->lcol, (const int64_t*)R->lup,                     \
                        (const int*)R->lucol, (const typename Val<VV>::T*)R->fval, R->rscale, \
                        x, y, e);                                                             \
@@ -2412,7 +2596,7 @@ int launch_bwd_xcd(const raptr_recon* R, const float* x, hipStream_t st, int* nb
 // the waves share it (one spare slot each)
 size_t fused_lds_bytes(const raptr_recon* R, int W, bool shared) {
   const size_t u = size_t(std::max(R->lmax, 1));
-  return size_t(kLocRows + 1) * sizeof(int64_t) +
+  return size_t(R->lrows + 1) * sizeof(int64_t) +
          (u + size_t(shared ? 1 : W) * (u + 1)) * sizeof(float);
 }
 constexpr size_t kLdsMax = 160 * 1024;
@@ -2445,6 +2629,15 @@ void fused_lds_attributes() {
 
 // part: 0 both kernels, 1 the products (k_fb1l) only, 2 the column reduce
 // and update only
+// waves per workgroup of the column-tile reduce: as many as keep its LDS
+// (one region of 2^tshift floats per slot of lpc lanes) within 32 KiB, 1-8
+inline int treduce_waves(int tshift, int lpc) {
+  const int64_t per_wave = (int64_t(64) / lpc) << tshift;  // floats
+  int w = 1;
+  while (w < 8 && per_wave * 2 * w * int64_t(sizeof(float)) <= 32 * 1024) w *= 2;
+  return w;
+}
+
 int launch_fused1(const raptr_recon* R, const float* x, const float* y, hipStream_t st,
                   int* nb_out, int part = 0) {
   if (!R->cpart || !R->vpos) return fail(RAPTR_ERR_INVALID_ARG, "plan has no fused pass");
@@ -2455,12 +2648,12 @@ int launch_fused1(const raptr_recon* R, const float* x, const float* y, hipStrea
   const size_t shm = fused_lds_bytes(R, W, SHR);
   if (shm > kLdsMax) return fail(RAPTR_ERR_INVALID_ARG, "fused pass: %zu B of LDS", shm);
   if (m > 0 && part != 2) {
-    const unsigned g = unsigned((m + kLocRows - 1) / kLocRows);
+    const unsigned g = unsigned((m + R->lrows - 1) / R->lrows);
     bool launched = false;
 #define RAPTR_FB(VV, WW, LL, SS, SH)                                                         \
   if (R->vt == VV && W == WW && LR == LL && SHR == SH && (LL != 64 || SS == R->fb_steps)) {  \
     hipLaunchKernelGGL((k_fb1l<VV, WW, LL, SS, SH>), dim3(g), dim3(64 * WW), shm, st, m,      \
-                       R->frp,                                                               \
+                       R->lrows, R->frp,                                                     \
                        (const uint16_t*)R->lcol, (const int64_t*)R->lup,                     \
                        (const int*)R->lucol, (const typename Val<VV>::T*)R->fval, R->rscale, \
                        x, y, R->e, R->cpart);                                                \
@@ -2483,6 +2676,32 @@ int launch_fused1(const raptr_recon* R, const float* x, const float* y, hipStrea
   }
   if (part == 1) return RAPTR_OK;
   const int L = R->fb_lpc;
+  if (R->fb_xcd == 2) {
+    if (!R->tseg) return fail(RAPTR_ERR_INVALID_ARG, "fused pass: no column-tile segments");
+    const int TW = treduce_waves(R->tshift, L);
+    const size_t tshm = size_t(TW) * size_t(64 / L) * (size_t(1) << R->tshift) * sizeof(float);
+    bool launched = false;
+#define RAPTR_FT(LL, WW)                                                                       \
+  if (L == LL && TW == WW) {                                                                   \
+    hipLaunchKernelGGL((k_fb1_treduce<LL, WW>), dim3(R->tnwg), dim3(64 * WW), tshm, st, n,     \
+                       R->tshift, (const int4*)R->twg, (const int2*)R->tseg,                   \
+                       (const int*)R->lucol, (const float*)R->cpart, R->ck);                   \
+    launched = true;                                                                           \
+  }
+#define RAPTR_FTL(LL) RAPTR_FT(LL, 1) RAPTR_FT(LL, 2) RAPTR_FT(LL, 4) RAPTR_FT(LL, 8)
+    RAPTR_FTL(16) RAPTR_FTL(32) RAPTR_FTL(64)
+#undef RAPTR_FTL
+#undef RAPTR_FT
+    if (!launched) return fail(RAPTR_ERR_INVALID_ARG, "reduce lanes %d", L);
+    RAPTR_LAUNCH_CHECK();
+    const unsigned nr = unsigned((int64_t(n) + 255) / 256);
+    hipLaunchKernelGGL(k_fb1_tcombine, dim3(nr), dim3(256), 0, st, n, R->tshift,
+                       (const int*)R->tk, (const float*)R->ck, x, (const float*)R->s, R->xx,
+                       R->partials);
+    RAPTR_LAUNCH_CHECK();
+    if (nb_out) *nb_out = int(nr);
+    return RAPTR_OK;
+  }
   if (R->fb_xcd) {
     const unsigned gg = unsigned(8 * ((int64_t(n) + (256 / L) - 1) / (256 / L)));
 #define RAPTR_FR8(LL)                                                                          \
@@ -2710,59 +2929,249 @@ namespace raptr {
 namespace {
 // column unions of the LDS-staged forward (k_fwd1l): sizes, scan, unions
 // and local indices; none when a group's union exceeds kLocMaxUnion
-int build_local_columns(raptr_recon* R, hipStream_t st) {
+// The column unions of groups of `rows` consecutive plan rows.  rows = 0:
+// the largest of kLocRowsChoices whose fused workgroup (8 waves adding to
+// one shared region) leaves three of them room in a CU's LDS, else the
+// largest that leaves two, else 64.  Larger groups share more of their
+// columns (full-size workload: 112M union entries for 64 rows, 51M for
+// 256; 96^3 geometry: 306M for 32, 217M for 64), so the staging of x and
+// the partial sums shrink, while fewer workgroups fit a CU.  Measured best
+// at full size: 256 rows (48 KB, 2.09 ms per iteration; 128: 2.21, 512:
+// 2.36); 96^3: 32 rows (74.5 KB, 3.58 ms; 64: 4.39, 128: 3.86).
+constexpr size_t kUnionLdsTiers[] = {160 * 1024 / 3, 160 * 1024 / 2};
+int build_local_columns(raptr_recon* R, hipStream_t st, int rows_req = 0) {
   const int64_t m = R->m;
   const int n = R->n;
-  const int64_t ng = (m + kLocRows - 1) / kLocRows;
   const size_t shm = (2 * size_t((std::min(n, kUnionWin) + 31) >> 5) + 257) * sizeof(uint32_t);
   int rc;
   int64_t* cnt = nullptr;
-  if ((rc = ralloc(R, &R->lup, size_t(ng) + 1)) || (rc = ralloc(R, &cnt, size_t(ng))))
-    return rc;
-  hipLaunchKernelGGL(k_loc_union, dim3(unsigned(ng)), dim3(256), shm, st,
-                     (const int64_t*)R->frp, (const int*)R->fcol, m, n, 0, cnt,
-                     (const int64_t*)nullptr, (int*)nullptr, (uint16_t*)nullptr);
-  void* tmp = nullptr;
-  const size_t sb = scan_workspace_bytes(ng);
-  RAPTR_HIP_TRY(hipMalloc(&tmp, 512 + sb));
-  int64_t* tot = static_cast<int64_t*>(tmp);
-  rc = scan_exclusive_i64(cnt, R->lup, ng, tot, static_cast<char*>(tmp) + 256, sb, st);
-  if (!rc) hipLaunchKernelGGL(k_set_last, dim3(1), dim3(1), 0, st, R->lup, int(ng),
-                              (const int64_t*)tot);
-  std::vector<int64_t> hc(static_cast<size_t>(ng));
-  int64_t h = 0;
-  hipError_t le = hipGetLastError();
-  if (le == hipSuccess) le = hipMemcpyAsync(&h, tot, 8, hipMemcpyDeviceToHost, st);
-  if (le == hipSuccess)
-    le = hipMemcpyAsync(hc.data(), cnt, size_t(ng) * 8, hipMemcpyDeviceToHost, st);
-  if (le == hipSuccess) le = hipStreamSynchronize(st);
-  (void)hipFree(tmp);
-  if (rc) return rc;
-  if (le != hipSuccess) return fail(RAPTR_ERR_DEVICE, "column unions: %s", hipGetErrorString(le));
-  const int64_t mx = hc.empty() ? 0 : *std::max_element(hc.begin(), hc.end());
-  if (mx > kLocMaxUnion) {  // too spread out for LDS: the plan keeps no unions
-    rfree(R, R->lup);
-    rfree(R, cnt);
-    return RAPTR_OK;
+  int64_t h = 0, mx = 0;
+  // without a request: the group sizes' union counts first (pass 0 each),
+  // then the choice
+  if (!rows_req) {
+    const int nc = int(sizeof(kLocRowsChoices) / sizeof(int));
+    std::vector<size_t> lds(nc, ~size_t(0));
+    for (int ci = 0; ci < nc; ++ci) {
+      const int rr = kLocRowsChoices[ci];
+      const int64_t ng = (m + rr - 1) / rr;
+      if ((rc = ralloc(R, &cnt, size_t(ng)))) return rc;
+      hipLaunchKernelGGL(k_loc_union, dim3(unsigned(ng)), dim3(256), shm, st,
+                         (const int64_t*)R->frp, (const int*)R->fcol, m, rr, n, 0, cnt,
+                         (const int64_t*)nullptr, (int*)nullptr, (uint16_t*)nullptr);
+      std::vector<int64_t> hc(static_cast<size_t>(ng));
+      hipError_t le = hipGetLastError();
+      if (le == hipSuccess)
+        le = hipMemcpyAsync(hc.data(), cnt, size_t(ng) * 8, hipMemcpyDeviceToHost, st);
+      if (le == hipSuccess) le = hipStreamSynchronize(st);
+      rfree(R, cnt);
+      if (le != hipSuccess)
+        return fail(RAPTR_ERR_DEVICE, "column unions: %s", hipGetErrorString(le));
+      const int64_t mxc = hc.empty() ? 0 : *std::max_element(hc.begin(), hc.end());
+      if (mxc <= kLocMaxUnion)
+        lds[ci] = size_t(rr + 1) * sizeof(int64_t) + size_t(2 * mxc + 1) * sizeof(float);
+    }
+    int pick = 64;
+    bool found = false;
+    for (size_t tier : kUnionLdsTiers) {
+      for (int ci = 0; ci < nc && !found; ++ci)
+        if (lds[ci] <= tier) {
+          pick = kLocRowsChoices[ci];
+          found = true;
+        }
+      if (found) break;
+    }
+    return build_local_columns(R, st, pick);
   }
+  const int rows = rows_req;
+  {
+    const int64_t ng = (m + rows - 1) / rows;
+    if ((rc = ralloc(R, &R->lup, size_t(ng) + 1)) || (rc = ralloc(R, &cnt, size_t(ng)))) return rc;
+    hipLaunchKernelGGL(k_loc_union, dim3(unsigned(ng)), dim3(256), shm, st,
+                       (const int64_t*)R->frp, (const int*)R->fcol, m, rows, n, 0, cnt,
+                       (const int64_t*)nullptr, (int*)nullptr, (uint16_t*)nullptr);
+    void* tmp = nullptr;
+    const size_t sb = scan_workspace_bytes(ng);
+    RAPTR_HIP_TRY(hipMalloc(&tmp, 512 + sb));
+    int64_t* tot = static_cast<int64_t*>(tmp);
+    rc = scan_exclusive_i64(cnt, R->lup, ng, tot, static_cast<char*>(tmp) + 256, sb, st);
+    if (!rc) hipLaunchKernelGGL(k_set_last, dim3(1), dim3(1), 0, st, R->lup, int(ng),
+                                (const int64_t*)tot);
+    std::vector<int64_t> hc(static_cast<size_t>(ng));
+    hipError_t le = hipGetLastError();
+    if (le == hipSuccess) le = hipMemcpyAsync(&h, tot, 8, hipMemcpyDeviceToHost, st);
+    if (le == hipSuccess)
+      le = hipMemcpyAsync(hc.data(), cnt, size_t(ng) * 8, hipMemcpyDeviceToHost, st);
+    if (le == hipSuccess) le = hipStreamSynchronize(st);
+    (void)hipFree(tmp);
+    if (rc) return rc;
+    if (le != hipSuccess)
+      return fail(RAPTR_ERR_DEVICE, "column unions: %s", hipGetErrorString(le));
+    mx = hc.empty() ? 0 : *std::max_element(hc.begin(), hc.end());
+    if (mx > kLocMaxUnion) {  // too spread out for LDS: the plan keeps no unions
+      rfree(R, R->lup);
+      rfree(R, cnt);
+      return RAPTR_OK;
+    }
+  }
+  const int64_t ng = (m + rows - 1) / rows;
   if ((rc = ralloc(R, &R->lucol, size_t(h) + 1)) || (rc = ralloc(R, &R->lcol, size_t(R->pnnz) + 8)))
     return rc;
   hipLaunchKernelGGL(k_loc_union, dim3(unsigned(ng)), dim3(256), shm, st,
-                     (const int64_t*)R->frp, (const int*)R->fcol, m, n, 1, cnt,
+                     (const int64_t*)R->frp, (const int*)R->fcol, m, rows, n, 1, cnt,
                      (const int64_t*)R->lup, R->lucol, R->lcol);
-  le = hipGetLastError();
+  hipError_t le = hipGetLastError();
   if (le == hipSuccess) le = hipStreamSynchronize(st);
   if (le != hipSuccess) return fail(RAPTR_ERR_DEVICE, "column unions: %s", hipGetErrorString(le));
   rfree(R, cnt);
+  R->lrows = rows;
   R->lmax = int(mx);
   R->lnu = h;
   return RAPTR_OK;
 }
 
+// the segment table of the column-tile reduce for tiles of 2^tshift
+// columns, each tile's segments cut into parts of about `target` segments
+// (replaces the plan's table; the plan has no clones)
+int build_tile_segments(raptr_recon* R, int tshift, int target, hipStream_t st) {
+  const int64_t nu = R->lnu;
+  const int64_t ng = (int64_t(R->m) + R->lrows - 1) / R->lrows;
+  const int ntiles = int((int64_t(R->n) + (int64_t(1) << tshift) - 1) >> tshift);
+  rfree(R, R->tseg);
+  rfree(R, R->twg);
+  rfree(R, R->tk);
+  rfree(R, R->ck);
+  R->nseg = 0;
+  R->tshift = R->tseg_target = R->tnwg = R->tkmax = 0;
+  Sizer z;
+  z.take<uint32_t>(size_t(nu));
+  z.take<uint32_t>(size_t(nu));
+  z.take<uint32_t>(1);
+  z.take<char>(scan_workspace_bytes(nu));
+  void* tmp = nullptr;
+  hipError_t ae = hipMalloc(&tmp, z.bytes() + 4096);
+  if (ae != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(RAPTR_ERR_OUT_OF_MEMORY, "tile segments scratch: %s", hipGetErrorString(ae));
+  }
+  Carver cv(tmp, z.bytes() + 4096);
+  uint32_t* flag = cv.take<uint32_t>(size_t(nu));
+  uint32_t* idx = cv.take<uint32_t>(size_t(nu));
+  uint32_t* d_total = cv.take<uint32_t>(1);
+  const size_t wsb = scan_workspace_bytes(nu);
+  char* ws = cv.take<char>(wsb);
+  const unsigned gp = unsigned(grid_for(nu, 256, 1 << 16));
+  hipLaunchKernelGGL(k_seg_flags, dim3(gp), dim3(256), 0, st, (const int*)R->lucol, nu, tshift,
+                     flag);
+  hipLaunchKernelGGL(k_seg_gstarts, dim3(grid_for(ng, 256, 1 << 16)), dim3(256), 0, st,
+                     (const int64_t*)R->lup, ng, nu, flag);
+  int rc = scan_exclusive_u32(flag, idx, nu, d_total, ws, wsb, st);
+  uint32_t total = 0;
+  if (!rc) {
+    hipError_t le = hipGetLastError();
+    if (le == hipSuccess)
+      le = hipMemcpyAsync(&total, d_total, sizeof(uint32_t), hipMemcpyDeviceToHost, st);
+    if (le == hipSuccess) le = hipStreamSynchronize(st);
+    if (le != hipSuccess) rc = fail(RAPTR_ERR_DEVICE, "tile segments: %s", hipGetErrorString(le));
+  }
+  const int64_t S = total;
+  void* tmp2 = nullptr;
+  std::vector<int> tsb_h(size_t(ntiles) + 1);
+  if (!rc) {
+    Sizer z2;
+    z2.take<uint64_t>(size_t(S));
+    z2.take<uint64_t>(size_t(S));
+    z2.take<uint32_t>(size_t(S));
+    z2.take<uint32_t>(size_t(S));
+    z2.take<int>(size_t(S) + 1);
+    z2.take<int>(size_t(ntiles) + 1);
+    const int64_t nt = (S + 2047) / 2048;
+    z2.take<uint32_t>(size_t(nt) * 256);
+    z2.take<uint32_t>(size_t(nt) * 256);
+    z2.take<char>(scan_workspace_bytes(nt * 256));
+    ae = hipMalloc(&tmp2, z2.bytes() + 4096);
+    if (ae != hipSuccess) {
+      (void)hipGetLastError();
+      tmp2 = nullptr;
+      rc = fail(RAPTR_ERR_OUT_OF_MEMORY, "tile segments scratch: %s", hipGetErrorString(ae));
+    }
+    if (!rc) rc = ralloc(R, &R->tseg, size_t(S));
+    if (!rc) {
+      Carver c2(tmp2, z2.bytes() + 4096);
+      uint64_t* k0 = c2.take<uint64_t>(size_t(S));
+      uint64_t* k1 = c2.take<uint64_t>(size_t(S));
+      uint32_t* v0 = c2.take<uint32_t>(size_t(S));
+      uint32_t* v1 = c2.take<uint32_t>(size_t(S));
+      int* ps = c2.take<int>(size_t(S) + 1);
+      int* tsb = c2.take<int>(size_t(ntiles) + 1);
+      hipLaunchKernelGGL(k_seg_emit, dim3(gp), dim3(256), 0, st, (const int*)R->lucol, nu,
+                         tshift, (const uint32_t*)flag, (const uint32_t*)idx, k0, v0, ps);
+      int res = 0;
+      rc = radix_sort_pairs(k0, k1, v0, v1, S, bits_for_count(ntiles), st, &res, c2);
+      if (!rc) {
+        hipLaunchKernelGGL(k_seg_pairs, dim3(grid_for(S, 256, 1 << 16)), dim3(256), 0, st,
+                           (const uint32_t*)(res ? v1 : v0), S, (const int*)ps, R->tseg);
+        hipLaunchKernelGGL(k_seg_bounds, dim3(grid_for(int64_t(ntiles) + 1, 256, 1 << 16)),
+                           dim3(256), 0, st, (const uint64_t*)(res ? k1 : k0), S, ntiles, tsb);
+        hipError_t le = hipGetLastError();
+        if (le == hipSuccess)
+          le = hipMemcpyAsync(tsb_h.data(), tsb, tsb_h.size() * sizeof(int),
+                              hipMemcpyDeviceToHost, st);
+        if (le == hipSuccess) le = hipStreamSynchronize(st);
+        if (le != hipSuccess)
+          rc = fail(RAPTR_ERR_DEVICE, "tile segments: %s", hipGetErrorString(le));
+      }
+    }
+  }
+  (void)hipStreamSynchronize(st);
+  (void)hipFree(tmp);
+  if (tmp2) (void)hipFree(tmp2);
+  // the workgroups: tile t's segments in ceil(count / target) parts of
+  // equal size (one part for a tile with none)
+  std::vector<int4> wg;
+  std::vector<int> tk(size_t(ntiles), 1);
+  int kmax = 1;
+  if (!rc) {
+    for (int t = 0; t < ntiles; ++t) {
+      const int64_t a = tsb_h[size_t(t)], c = tsb_h[size_t(t) + 1] - a;
+      const int K = int(std::max<int64_t>(1, (c + target - 1) / target));
+      tk[size_t(t)] = K;
+      kmax = std::max(kmax, K);
+      for (int k = 0; k < K; ++k)
+        wg.push_back(int4{t, k, int(a + c * k / K), int(a + c * (k + 1) / K)});
+    }
+    if (wg.size() > size_t(INT32_MAX))
+      rc = fail(RAPTR_ERR_INVALID_ARG, "tile segments: too many workgroups");
+  }
+  if (!rc) rc = ralloc(R, &R->twg, wg.size());
+  if (!rc) rc = ralloc(R, &R->tk, tk.size());
+  if (!rc) rc = ralloc(R, &R->ck, size_t(kmax) * size_t(R->n));
+  if (!rc) {
+    hipError_t le = hipMemcpyAsync(R->twg, wg.data(), wg.size() * sizeof(int4),
+                                   hipMemcpyHostToDevice, st);
+    if (le == hipSuccess)
+      le = hipMemcpyAsync(R->tk, tk.data(), tk.size() * sizeof(int), hipMemcpyHostToDevice, st);
+    if (le == hipSuccess) le = hipStreamSynchronize(st);
+    if (le != hipSuccess) rc = fail(RAPTR_ERR_DEVICE, "tile segments: %s", hipGetErrorString(le));
+  }
+  if (rc) {
+    rfree(R, R->tseg);
+    rfree(R, R->twg);
+    rfree(R, R->tk);
+    rfree(R, R->ck);
+    return rc;
+  }
+  R->nseg = S;
+  R->tshift = tshift;
+  R->tseg_target = target;
+  R->tnwg = int(wg.size());
+  R->tkmax = kmax;
+  return RAPTR_OK;
+}
+
 // the fused pass's index: the union positions of every column, ascending
-// (a stable radix sort of the positions by column), and the groups'
-// partial column sums; none where the unions do not fit LDS or a row
-// repeats a column
+// (a stable radix sort of the positions by column), the groups' partial
+// column sums and the column-tile reduce's segments; none where the unions
+// do not fit LDS or a row repeats a column
 int build_union_index(raptr_recon* R, hipStream_t st) {
   const int64_t nu = R->lnu;
   const int n = R->n;
@@ -2774,12 +3183,13 @@ int build_union_index(raptr_recon* R, hipStream_t st) {
     if ((rc = ralloc(R, &dup, 1))) return rc;
     (void)hipMemsetAsync(dup, 0, sizeof(int), st);
     const int words = (R->lmax + 31) / 32;
-    const unsigned ng = unsigned((int64_t(R->m) + kLocRows - 1) / kLocRows);
+    const unsigned ng = unsigned((int64_t(R->m) + R->lrows - 1) / R->lrows);
     const size_t shm = size_t(4) * size_t(std::max(words, 1)) * sizeof(uint32_t);
     switch (R->vt) {
 #define RAPTR_FD(VV)                                                                         \
   case VV:                                                                                   \
     hipLaunchKernelGGL(k_fb_distinct<VV>, dim3(ng), dim3(256), shm, st, int64_t(R->m),        \
+                       R->lrows,                                                             \
                        (const int64_t*)R->frp, (const uint16_t*)R->lcol,                     \
                        (const typename Val<VV>::T*)R->fval, words, dup);                     \
     break;
@@ -2827,7 +3237,7 @@ int build_union_index(raptr_recon* R, hipStream_t st) {
                        (const uint64_t*)(res ? k1 : k0), nu, n, R->vptr);
     hipLaunchKernelGGL(k_u32_to_i32_64, dim3(grid_for(nu, 256, 1 << 16)), dim3(256), 0, st,
                        (const uint32_t*)(res ? v1 : v0), nu, R->vpos);
-    const int64_t ng = (int64_t(R->m) + kLocRows - 1) / kLocRows;
+    const int64_t ng = (int64_t(R->m) + R->lrows - 1) / R->lrows;
     hipLaunchKernelGGL(k_vbounds, dim3(grid_for(int64_t(7) * n, 256, 1 << 16)), dim3(256), 0, st,
                        n, (const int64_t*)R->vptr, (const int*)R->vpos, (const int64_t*)R->lup,
                        ng, R->vb);
@@ -2852,6 +3262,15 @@ int build_union_index(raptr_recon* R, hipStream_t st) {
   R->fb_steps = std::min(4, std::max(2, int(std::ceil((mean_row + 64.0) / 256.0))));
   R->fb_waves = fused_lds_bytes(R, 4, true) * 5 <= kLdsMax ? 4 : 8;
   fused_lds_attributes();
+  // the column-tile reduce (tiles of 1,024 columns, parts of 512 segments,
+  // 64 lanes per slot: the fastest shape measured at full size and at 96^3)
+  // where its table builds, else the per-XCD reduce
+  if (build_tile_segments(R, 10, 512, st) == RAPTR_OK) {
+    R->fb_xcd = 2;
+    R->fb_lpc = 64;
+  } else {
+    (void)hipGetLastError();
+  }
   return RAPTR_OK;
 }
 
@@ -3453,6 +3872,7 @@ int raptr_recon_clone(raptr_recon* P, raptr_recon** out, raptr_stream_t stream) 
   if (!rc) rc = own(R->c8);
   if (!rc) rc = own(R->yp);
   if (!rc) rc = own(R->cpart);
+  if (!rc) rc = own(R->ck);
   if (rc) {
     raptr_recon_destroy(R);
     return rc;
@@ -3675,15 +4095,90 @@ int raptr_recon_set_fused_steps(raptr_recon* R, int steps) {
 
 int raptr_recon_get_fused_steps(const raptr_recon* R) { return R ? R->fb_steps : -1; }
 
-int raptr_recon_set_fused_reduce(raptr_recon* R, int per_xcd) {
+int raptr_recon_set_fused_reduce(raptr_recon* R, int mode) {
   if (!R) return fail(RAPTR_ERR_INVALID_ARG, "null plan");
-  const int v = per_xcd ? 1 : 0;
-  if (R->fb_xcd != v) clear_graphs(R);
-  R->fb_xcd = v;
+  if (mode < 0 || mode > 2) return fail(RAPTR_ERR_INVALID_ARG, "reduce mode must be 0, 1 or 2");
+  if (mode == 2 && !R->tseg) return fail(RAPTR_ERR_INVALID_ARG, "plan has no column-tile segments");
+  if (R->fb_xcd != mode) clear_graphs(R);
+  R->fb_xcd = mode;
   return RAPTR_OK;
 }
 
 int raptr_recon_get_fused_reduce(const raptr_recon* R) { return R ? R->fb_xcd : -1; }
+
+int raptr_recon_set_fused_tiles(raptr_recon* R, int columns, int segments_per_workgroup,
+                                raptr_stream_t stream) {
+  if (!R) return fail(RAPTR_ERR_INVALID_ARG, "null plan");
+  if (!R->cpart) return fail(RAPTR_ERR_INVALID_ARG, "plan has no fused pass");
+  if (R->parent || R->nclones)
+    return fail(RAPTR_ERR_INVALID_ARG, "the segment table is shared with clones: set it before cloning");
+  int sh = 7;
+  while (sh < 11 && (1 << sh) != columns) ++sh;
+  if ((1 << sh) != columns)
+    return fail(RAPTR_ERR_INVALID_ARG, "tile columns must be a power of two in 128..2048");
+  if (segments_per_workgroup < 64 || segments_per_workgroup > (1 << 20))
+    return fail(RAPTR_ERR_INVALID_ARG, "segments per workgroup must be in 64..1048576");
+  if (R->tseg && R->tshift == sh && R->tseg_target == segments_per_workgroup) return RAPTR_OK;
+  clear_graphs(R);
+  (void)hipDeviceSynchronize();  // no queued pass may still read the old table
+  const int rc = build_tile_segments(R, sh, segments_per_workgroup,
+                                     static_cast<hipStream_t>(stream));
+  if (rc && R->fb_xcd == 2) R->fb_xcd = 1;
+  return rc;
+}
+
+int raptr_recon_set_union_rows(raptr_recon* R, int rows, raptr_stream_t stream) {
+  if (!R) return fail(RAPTR_ERR_INVALID_ARG, "null plan");
+  if (R->parent || R->nclones)
+    return fail(RAPTR_ERR_INVALID_ARG, "the unions are shared with clones: set them before cloning");
+  if (rows != 0 && (rows < 16 || rows > 512 || (rows & (rows - 1))))
+    return fail(RAPTR_ERR_INVALID_ARG, "union rows must be 0 or a power of two in 16..512");
+  if (R->fmax != 1 || R->layout != RAPTR_LAYOUT_BLOCKED || !R->fcol || R->m == 0 || R->pnnz == 0)
+    return fail(RAPTR_ERR_INVALID_ARG,
+                "column unions need a single-frame blocked plan with a padded CSR");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  clear_graphs(R);
+  (void)hipDeviceSynchronize();  // no queued pass may still read the old unions
+  rfree(R, R->lup);
+  rfree(R, R->lucol);
+  rfree(R, R->lcol);
+  rfree(R, R->cpart);
+  rfree(R, R->vptr);
+  rfree(R, R->vpos);
+  rfree(R, R->vb);
+  rfree(R, R->tseg);
+  rfree(R, R->twg);
+  rfree(R, R->tk);
+  rfree(R, R->ck);
+  R->lmax = 0;
+  R->lnu = 0;
+  R->nseg = 0;
+  R->tshift = R->tseg_target = R->tnwg = R->tkmax = 0;
+  R->fused = 0;
+  int rc = build_local_columns(R, st, rows);
+  if (!rc && R->lcol) rc = build_union_index(R, st);
+  if (!R->lcol && R->fwd_kind == 2) R->fwd_kind = 0;
+  return rc;
+}
+
+int raptr_recon_get_unions(const raptr_recon* R, int* rows, int* largest, int64_t* entries) {
+  if (!R) return fail(RAPTR_ERR_INVALID_ARG, "null plan");
+  if (rows) *rows = R->lcol ? R->lrows : 0;
+  if (largest) *largest = R->lcol ? R->lmax : 0;
+  if (entries) *entries = R->lcol ? R->lnu : 0;
+  return RAPTR_OK;
+}
+
+int raptr_recon_get_fused_tiles(const raptr_recon* R, int* columns, int* segments_per_workgroup,
+                                int64_t* segments, int* workgroups, int* max_parts) {
+  if (!R) return fail(RAPTR_ERR_INVALID_ARG, "null plan");
+  if (columns) *columns = R->tseg ? (1 << R->tshift) : 0;
+  if (segments_per_workgroup) *segments_per_workgroup = R->tseg ? R->tseg_target : 0;
+  if (segments) *segments = R->tseg ? R->nseg : 0;
+  if (workgroups) *workgroups = R->tseg ? R->tnwg : 0;
+  if (max_parts) *max_parts = R->tseg ? R->tkmax : 0;
+  return RAPTR_OK;
+}
 
 int raptr_recon_get_fused(const raptr_recon* R, int* available, int* on, int* waves,
                           int* lanes_per_row, int* lanes_per_column) {

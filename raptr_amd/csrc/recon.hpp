@@ -151,6 +151,7 @@ struct raptr_recon {
   int64_t* lup = nullptr;
   int* lucol = nullptr;
   uint16_t* lcol = nullptr;
+  int lrows = 64;              // plan rows per union group
   int lmax = 0;                // largest union
   int64_t lnu = 0;             // union entries of all groups
   // fused single-frame pass (k_fb1l + k_fb1_reduce): the groups' partial
@@ -160,8 +161,20 @@ struct raptr_recon {
   int64_t* vptr = nullptr;
   int* vpos = nullptr;
   int64_t* vb = nullptr;  // 7 x n: per XCD range of groups, where column j's positions start
+  // the unions cut at column tiles of 2^tshift columns: segments (union
+  // position ranges of one group inside one tile) sorted by (tile, group);
+  // each tile's segments split into tk[t] parts of about tseg_target
+  // segments, one workgroup each (twg: tile, part, first and end segment);
+  // ck[k][j] takes part k's sums of column j
+  int2* tseg = nullptr;
+  int4* twg = nullptr;
+  int* tk = nullptr;
+  float* ck = nullptr;
+  int tshift = 0, tseg_target = 0, tnwg = 0, tkmax = 0;
+  int64_t nseg = 0;
   int fused = 0;      // single-frame iterations run the fused pass
-  int fb_xcd = 1;     // the reduce in per-XCD group ranges (k_fb1_reduce8)
+  int fb_xcd = 1;     // the reduce: 0 one pass by column, 1 per-XCD group ranges
+                      // (k_fb1_reduce8), 2 column tiles (k_fb1_treduce)
   int fb_waves = 4;   // waves per row group (2 or 4)
   int fb_lanes = 32;  // lanes per row (16, 32 or 64)
   int fb_steps = 2;   // register-held steps per row with 64 lanes (2, 3 or 4)

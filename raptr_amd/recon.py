@@ -99,7 +99,8 @@ class Reconstructor:
         out.update({"fused": fz["on"], "fused_available": fz["available"],
                     "fused_waves": fz["waves"], "fused_lanes_per_row": fz["lanes_per_row"],
                     "fused_lanes_per_column": fz["lanes_per_column"],
-                    "fused_reduce_per_xcd": fz["per_xcd"], "fused_steps": fz["steps"]})
+                    "fused_reduce_per_xcd": fz["per_xcd"], "fused_steps": fz["steps"],
+                    "union_rows": self.unions()["rows"]})
         gi = self.grouped()
         if gi["group"]:
             out.update({"grouped64": gi["on"], "group": gi["group"],
@@ -171,10 +172,40 @@ class Reconstructor:
         kernel with 64 lanes per row (2, 3 or 4)."""
         capi.call("raptr_recon_set_fused_steps", self.handle, int(steps))
 
-    def set_fused_reduce(self, per_xcd):
-        """Fused iteration's column reduce: per XCD group ranges (True, the
-        default) or all of a column's group partials in one pass."""
-        capi.call("raptr_recon_set_fused_reduce", self.handle, 1 if per_xcd else 0)
+    def set_fused_reduce(self, mode):
+        """Fused iteration's column reduce: 0 (or False) all of a column's
+        group partials in one pass; 1 (or True, the default) per-XCD group
+        ranges; 2 column tiles over the partials in written order."""
+        capi.call("raptr_recon_set_fused_reduce", self.handle, int(mode))
+
+    def set_union_rows(self, rows=0, stream=None):
+        """Plan rows per column-union group (0: the plan's own choice, else a
+        power of two in 16..512); rebuilds the unions and the fused pass's
+        index (before any clone)."""
+        capi.call("raptr_recon_set_union_rows", self.handle, int(rows), stream)
+
+    def unions(self):
+        """{"rows", "largest", "entries"} of the column unions (zeros: none)."""
+        r, l, e = C.c_int(), C.c_int(), C.c_int64()
+        capi.call("raptr_recon_get_unions", self.handle, C.byref(r), C.byref(l), C.byref(e))
+        return {"rows": r.value, "largest": l.value, "entries": e.value}
+
+    def set_fused_tiles(self, columns, segments_per_workgroup=None, stream=None):
+        """Shape of the column-tile reduce: columns per tile (power of two,
+        128-2048) and segments per workgroup (None: keep the plan's);
+        rebuilds its segment table (before any clone)."""
+        spw = self.fused_tiles()["segments_per_workgroup"] if segments_per_workgroup is None \
+            else segments_per_workgroup
+        capi.call("raptr_recon_set_fused_tiles", self.handle, int(columns), int(spw), stream)
+
+    def fused_tiles(self):
+        """{"columns", "segments_per_workgroup", "segments", "workgroups",
+        "max_parts"} of the column-tile reduce's table (zeros: none)."""
+        c, p, n, w, k = C.c_int(), C.c_int(), C.c_int64(), C.c_int(), C.c_int()
+        capi.call("raptr_recon_get_fused_tiles", self.handle, C.byref(c), C.byref(p),
+                  C.byref(n), C.byref(w), C.byref(k))
+        return {"columns": c.value, "segments_per_workgroup": p.value, "segments": n.value,
+                "workgroups": w.value, "max_parts": k.value}
 
     def fused(self):
         """{"available", "on", "waves", "lanes_per_row", "lanes_per_column",
@@ -309,29 +340,9 @@ class Reconstructor:
             self.set_backward(st, lpc, wgs)
         self.set_config(fl, fu, u)
         self.set_blocks_per_launch(bpl)
-        fz_cfg = (0, 0, 0, 0, 0, 0)
+        fz_cfg = (0, 0, 0, 0, 0, 0, 0, 0, 0)
         if fz0["available"]:
-            best_fz = None
-            s0 = fz0["steps"]
-            shapes = [(2, w, 64, st_) for w in (4, 8) for st_ in sorted({s0, min(s0 + 1, 4)})]
-            shapes += [(2, 2, 64, s0), (2, 4, 32, s0), (1, 4, 32, s0), (1, 2, 64, s0)]
-            for fmode, fw, flr, fst in shapes:
-                for flpc, fxr in ((16, 1), (32, 1), (64, 0)):
-                    try:
-                        self.set_fused(fmode, fw, flr, flpc)
-                        self.set_fused_reduce(fxr)
-                        self.set_fused_steps(fst)
-                    except capi.RaptrError:  # too little LDS for that many waves
-                        continue
-                    ms = timed()
-                    if best_fz is None or ms < best_fz[1]:
-                        best_fz = ((fmode, fw, flr, flpc, fxr, fst), ms)
-            self.set_fused(False)
-            if best_fz is not None and best_fz[1] < timed():
-                fz_cfg = best_fz[0]
-                self.set_fused(*fz_cfg[:4])
-                self.set_fused_reduce(fz_cfg[4])
-                self.set_fused_steps(fz_cfg[5])
+            fz_cfg = self._tune_fused(fz0, timed, stream)
         bc = 0
         if nframes == 64:
             # the 64-frame kernels with in-row broadcast of values and indices
@@ -360,8 +371,79 @@ class Reconstructor:
                "grouped_kind": g_p, "xcd_blocks": xb, "fused": fz_cfg[0],
                "fused_waves": fz_cfg[1], "fused_lanes_per_row": fz_cfg[2],
                "fused_lanes_per_column": fz_cfg[3], "fused_reduce_per_xcd": fz_cfg[4],
-               "fused_steps": fz_cfg[5]}
+               "fused_steps": fz_cfg[5], "union_rows": fz_cfg[6] or self.unions()["rows"],
+               "fused_tiles": fz_cfg[7], "fused_segments_per_workgroup": fz_cfg[8]}
         return cfg, ms_best
+
+    def _tune_fused(self, fz0, timed, stream):
+        """The fused iteration's shape: plan rows per union group (the
+        plan's choice, half and twice it), then per group size the
+        products kernel (4 or 8 waves, the plan's register-held steps and
+        one more), then the column reduce (column tiles of 512-2,048
+        columns with 32 or 64 lanes per slot, the per-XCD and the by-column
+        passes).  Kept if faster than the separate passes, which are timed
+        last over the kept unions.  Returns (mode, waves, lanes per row,
+        lanes per column, reduce mode, steps, union rows, tile columns,
+        segments per workgroup), zeros if the separate passes win."""
+        rows0 = self.unions()["rows"]
+        red0 = (64, 2) if self.fused_tiles()["columns"] else (16, 1)
+        best = None
+        for rows in [rows0] + [r for r in (rows0 // 2, rows0 * 2) if 16 <= r <= 512]:
+            if rows != self.unions()["rows"]:
+                try:
+                    self.set_union_rows(rows, stream=stream)
+                except capi.RaptrError:
+                    continue
+            fz = self.fused()
+            if not fz["available"]:
+                continue
+            s0 = fz["steps"]
+            shapes = [(2, w, 64, st_) for w in (4, 8) for st_ in sorted({s0, min(s0 + 1, 4)})]
+            if rows == rows0:
+                shapes += [(2, 2, 64, s0), (2, 4, 32, s0), (1, 2, 64, s0)]
+            for fmode, fw, flr, fst in shapes:
+                try:
+                    self.set_fused(fmode, fw, flr, red0[0])
+                    self.set_fused_reduce(red0[1])
+                    self.set_fused_steps(fst)
+                except capi.RaptrError:  # too little LDS for that many waves
+                    continue
+                ms = timed()
+                if best is None or ms < best[1]:
+                    best = ((fmode, fw, flr, fst, rows), ms)
+        if best is None:
+            self.set_fused(False)
+            return (0,) * 9
+        fmode, fw, flr, fst, rows = best[0]
+        if rows != self.unions()["rows"]:
+            self.set_union_rows(rows, stream=stream)
+        self.set_fused_steps(fst)
+        reds = [(64, 1, None), (16, 1, None), (64, 0, None)]
+        if self.fused_tiles()["columns"]:
+            reds = [(lpc, 2, t) for t in ((1024, 512), (512, 1024), (2048, 512))
+                    for lpc in (64, 32)] + reds
+        best_r = None
+        for lpc, xr, tiles in reds:
+            if tiles is not None:
+                try:
+                    self.set_fused_tiles(*tiles, stream=stream)
+                except capi.RaptrError:  # a table shared with clones stays
+                    continue
+            self.set_fused(fmode, fw, flr, lpc)
+            self.set_fused_reduce(xr)
+            ms = timed()
+            if best_r is None or ms < best_r[1]:
+                best_r = ((lpc, xr, tiles), ms)
+        lpc, xr, tiles = best_r[0]
+        if tiles is not None:
+            self.set_fused_tiles(*tiles, stream=stream)
+        self.set_fused(False)
+        if best_r[1] >= timed():
+            return (0,) * 9
+        self.set_fused(fmode, fw, flr, lpc)
+        self.set_fused_reduce(xr)
+        tz = self.fused_tiles()
+        return (fmode, fw, flr, lpc, xr, fst, rows, tz["columns"], tz["segments_per_workgroup"])
 
     # rows each XCD walks per launch of kind 3's backward: windows of about
     # 131,072 plan rows (32 MB of 64-frame ratios) measured best at full size

@@ -30,21 +30,24 @@ def test_fused_small_system(gpu, oracle, small_system):
     x0 = np.full(n, 1.0 / n, np.float32)
     x64 = _oracle_mlem(oracle, S, S["y"], 5)
     R = Reconstructor(S["A"], S["sens"], "f32", 1, layout="blocked", row_order=S["setup"])
-    # the default shape: one shared region, 4 waves, 64 lanes per row with 2
-    # register-held steps (rows of about 140 entries), the per-XCD reduce
-    # with 16 lanes per column
-    assert R.fused() == {"available": 1, "on": 2, "waves": 4, "lanes_per_row": 64,
-                         "lanes_per_column": 16, "per_xcd": 1, "steps": 2}
+    # the default shape: one shared region, 4 waves where five such
+    # workgroups fit a CU's LDS (else 8), 64 lanes per row with 2
+    # register-held steps (rows of about 140 entries), the column-tile
+    # reduce with 64 lanes per segment slot
+    u = R.unions()
+    lds = 8 * (u["rows"] + 1) + 4 * (2 * u["largest"] + 1)
+    assert R.fused() == {"available": 1, "on": 2, "waves": 4 if 5 * lds <= 160 * 1024 else 8,
+                         "lanes_per_row": 64, "lanes_per_column": 64, "per_xcd": 2, "steps": 2}
     R.set_fused(False)
     sep = R.mlem(S["y"], x0, 5)
     assert _rel(sep, x64) <= 1e-4
     st = Stream()
     for o, w, lr, rw in SHAPES:
         R.set_fused(o, w, lr, rw)
-        for xr in (True, False):
+        for xr in (2, 1, 0):
             R.set_fused_reduce(xr)
             assert R.fused() == {"available": 1, "on": o, "waves": w, "lanes_per_row": lr,
-                                 "lanes_per_column": rw, "per_xcd": int(xr), "steps": 2}
+                                 "lanes_per_column": rw, "per_xcd": xr, "steps": 2}
             a = R.mlem(S["y"], x0, 5)
             b = R.mlem(S["y"], x0, 5)
             np.testing.assert_array_equal(a, b)
@@ -61,6 +64,112 @@ def test_fused_small_system(gpu, oracle, small_system):
         assert _rel(a, sep) <= 1e-6, stp
     R.set_fused(False)
     np.testing.assert_array_equal(R.mlem(S["y"], x0, 5), sep)
+
+
+def test_fused_tile_reduce(gpu, oracle, small_system):
+    """The column-tile reduce (mode 2): for tiles of 128-2,048 columns and
+    16, 32 or 64 lanes per segment, 5 iterations within 1e-4 of the FP64
+    oracle and 1e-6 of the separate passes, bit for bit run to run and in a
+    graph replay.  The segment count lies between the number of row groups
+    and the number of union entries and falls as the tiles grow; bad tile
+    sizes and tables shared with clones are refused."""
+    from raptr_amd import _capi as capi
+    from raptr_amd.device import Stream
+    from raptr_amd.recon import Reconstructor
+    S = small_system
+    n = S["grid"].nvox
+    x0 = np.full(n, 1.0 / n, np.float32)
+    x64 = _oracle_mlem(oracle, S, S["y"], 5)
+    R = Reconstructor(S["A"], S["sens"], "f32", 1, layout="blocked", row_order=S["setup"])
+    tz = R.fused_tiles()
+    assert tz["columns"] == 1024 and tz["segments_per_workgroup"] == 512 and tz["segments"] > 0
+    R.set_fused(False)
+    sep = R.mlem(S["y"], x0, 5)
+    nu = R.forward_union_entries()
+    ngroups = (R.m + 63) // 64
+    st = Stream()
+    last = None
+    for tiles, spw in ((128, 64), (128, 1024), (256, 100), (512, 4096), (1024, 64),
+                       (2048, 1 << 20)):
+        R.set_fused_tiles(tiles, spw)
+        tz = R.fused_tiles()
+        nseg = tz["segments"]
+        assert tz["columns"] == tiles and tz["segments_per_workgroup"] == spw
+        assert ngroups <= nseg <= nu
+        # every tile in ceil(segments / spw) parts, at least one
+        ntiles = -(-n // tiles)
+        assert max(ntiles, -(-nseg // spw)) <= tz["workgroups"] <= ntiles + nseg // spw + ntiles
+        assert 1 <= tz["max_parts"] <= max(1, -(-nseg // spw))
+        if last is not None and last[0] == tiles - tiles // 2:
+            assert nseg <= last[1]
+        last = (tiles, nseg)
+        for lpc in (16, 32, 64):
+            R.set_fused(2, 4, 64, lpc)
+            R.set_fused_reduce(2)
+            assert R.fused()["per_xcd"] == 2
+            a = R.mlem(S["y"], x0, 5)
+            np.testing.assert_array_equal(a, R.mlem(S["y"], x0, 5))
+            assert _rel(a, x64) <= 1e-4, (tiles, lpc)
+            assert _rel(a, sep) <= 1e-6, (tiles, lpc)
+            g = R.mlem(S["y"], x0, 5, use_graph=True, stream=st.handle)
+            np.testing.assert_array_equal(a, g)
+    for bad, spw in ((0, 1024), (64, 1024), (100, 1024), (4096, 1024), (512, 63),
+                     (512, (1 << 20) + 1)):
+        with pytest.raises(capi.RaptrError):
+            R.set_fused_tiles(bad, spw)
+    assert R.fused_tiles()["columns"] == 2048  # unchanged by the refusals
+    C = R.clone()
+    with pytest.raises(capi.RaptrError):
+        R.set_fused_tiles(512)
+    with pytest.raises(capi.RaptrError):
+        C.set_fused_tiles(512)
+    np.testing.assert_array_equal(C.mlem(S["y"], x0, 5), R.mlem(S["y"], x0, 5))
+    C.close()
+    R.set_fused_tiles(1024, 512)
+
+
+def test_fused_union_rows(gpu, oracle, small_system):
+    """Plan rows per union group: the plan's choice is one of 32-256 rows;
+    for groups of 16-512 rows the union entries do not grow with the group
+    size and the fused iteration stays within 1e-4 of the FP64 oracle and
+    1e-6 of the separate passes, bit for bit run to run; bad sizes and
+    unions shared with clones are refused."""
+    from raptr_amd import _capi as capi
+    from raptr_amd.recon import Reconstructor
+    S = small_system
+    n = S["grid"].nvox
+    x0 = np.full(n, 1.0 / n, np.float32)
+    x64 = _oracle_mlem(oracle, S, S["y"], 5)
+    R = Reconstructor(S["A"], S["sens"], "f32", 1, layout="blocked", row_order=S["setup"])
+    u0 = R.unions()
+    assert u0["rows"] in (32, 64, 128, 256) and 0 < u0["largest"] <= u0["entries"]
+    R.set_fused(False)
+    sep = R.mlem(S["y"], x0, 5)
+    last = None
+    for rows in (16, 32, 64, 128, 256, 512):
+        R.set_union_rows(rows)
+        u = R.unions()
+        assert u["rows"] == rows and R.fused()["on"] == 2
+        if last is not None:
+            assert u["entries"] <= last
+        last = u["entries"]
+        a = R.mlem(S["y"], x0, 5)
+        np.testing.assert_array_equal(a, R.mlem(S["y"], x0, 5))
+        assert _rel(a, x64) <= 1e-4, rows
+        assert _rel(a, sep) <= 1e-6, rows
+        R.set_fused(False)  # the LDS-staged forward over the same unions
+        assert R.forward_kind()[0] == 2
+        np.testing.assert_array_equal(R.mlem(S["y"], x0, 5), sep)
+    for bad in (8, 24, 1024, -64):
+        with pytest.raises(capi.RaptrError):
+            R.set_union_rows(bad)
+    assert R.unions()["rows"] == 512
+    R.set_union_rows(0)
+    assert R.unions() == u0
+    C = R.clone()
+    with pytest.raises(capi.RaptrError):
+        R.set_union_rows(64)
+    C.close()
 
 
 @pytest.mark.parametrize("vt", ["f16", "bf16", "u16"])
@@ -142,10 +251,15 @@ def test_fused_edges(gpu, oracle, case):
     R.set_fused(False)
     sep = R.mlem(y, x0, 3)
     assert _rel(sep, x64) <= 1e-5
+    if case == "long_rows":  # tiles of 128 columns in parts of 64 segments
+        R.set_fused_tiles(128, 64)
+    elif case == "empty_group":  # whole tiles of 2,048 columns
+        R.set_fused_tiles(2048, 1 << 20)
     for o, w, lr, rw, xr, stp in [(1, 2, 16, 16, 1, 2), (1, 4, 16, 64, 0, 2),
                                   (1, 2, 32, 32, 1, 2), (1, 4, 64, 64, 1, 3), (2, 4, 32, 16, 0, 2),
                                   (2, 2, 16, 64, 1, 2), (2, 4, 64, 32, 1, 2), (2, 8, 64, 64, 0, 4),
-                                  (2, 4, 64, 16, 1, 3), (2, 8, 64, 16, 1, 2)]:
+                                  (2, 4, 64, 16, 1, 3), (2, 8, 64, 16, 1, 2), (2, 4, 64, 16, 2, 2),
+                                  (2, 8, 64, 64, 2, 4), (1, 2, 32, 32, 2, 2)]:
         R.set_fused(o, w, lr, rw)
         R.set_fused_reduce(xr)
         R.set_fused_steps(stp)
@@ -169,7 +283,8 @@ def test_fused_lds_limits(gpu):
     rp, col, val = _long_row_csr(n, 640, 24, window=10000)
     R = Reconstructor(_device_csr(rp, col, val, n), DeviceArray.from_numpy(np.ones(n)), "f32", 1,
                       layout="blocked")
-    assert R.fused()["available"] == 1
+    R.set_union_rows(64)  # one column window per group
+    assert R.unions()["rows"] == 64 and R.fused()["available"] == 1
     # the default: a shared region (79.7 KB of LDS: two workgroups per CU,
     # so 8 waves per group)
     assert R.fused()["on"] == 2 and R.fused()["waves"] == 8
@@ -184,7 +299,7 @@ def test_fused_lds_limits(gpu):
         with pytest.raises(RaptrError):
             R.set_fused(o, w, lr, rw)
     assert R.fused() == {"available": 1, "on": 1, "waves": 2, "lanes_per_row": 32,
-                         "lanes_per_column": 64, "per_xcd": 1, "steps": 2}
+                         "lanes_per_column": 64, "per_xcd": 2, "steps": 2}
     for bad in (1, 5, 0):
         with pytest.raises(RaptrError):
             R.set_fused_steps(bad)
