@@ -64,6 +64,8 @@ MEASURED_TRAFFIC = {
     # group); one launch each per iteration
     ("k_fb1l", 2, 4, 64, "y", 64): 2 * 5035683 * 1024 + 465498 * 1024,
     ("k_fb1_reduce8", 16, "y", 64): 2 * 1028086 * 1024 + 6883 * 1024,
+    ("k_fb1l", 2, 8, 64, "y", 256): 2 * 4908508 * 1024 + 221522 * 1024,
+    ("k_fb1_treduce", 64, "y", 256): 2 * 318335 * 1024 + 7869 * 1024,
 }
 
 
