@@ -65,7 +65,7 @@ MEASURED_TRAFFIC = {
     ("k_fb1l", 2, 4, 64, "y", 64): 2 * 5035683 * 1024 + 465498 * 1024,
     ("k_fb1_reduce8", 16, "y", 64): 2 * 1028086 * 1024 + 6883 * 1024,
     ("k_fb1l", 2, 8, 64, "y", 256): 2 * 4908508 * 1024 + 221522 * 1024,
-    ("k_fb1_treduce", 64, "y", 256): 2 * 318335 * 1024 + 7869 * 1024,
+    ("k_fb1_treduce", 64, "y", 256, 1024, 512): 2 * 318335 * 1024 + 7869 * 1024,
 }
 
 
@@ -366,8 +366,9 @@ def main():
     cols16, rows16 = bool(cmp_ & 2), bool(cmp_ & 1)
     fwd_kernel = ("k_fwd1l" if info["fwd_kind"] == 2 else
                   "k_fwd1c" if cols16 else "k_fwd1")
+    urows = R.unions()["rows"]
     fwd_name = f"{fwd_kernel} (forward + ratio, one launch" + (
-        ", x staged in LDS per 64 rows, 16-bit ranks)" if fwd_kernel == "k_fwd1l" else
+        f", x staged in LDS per {urows} rows, 16-bit ranks)" if fwd_kernel == "k_fwd1l" else
         ", 16-bit columns)" if cols16 else ")")
     # bytes as stored: 16-bit indices move 2 bytes less per entry than the
     # 4-byte index of the algorithmic count (plus 8 bytes of split points per
@@ -377,10 +378,10 @@ def main():
     bytes_f = 8 * info["padded_nnz"] + 8 * (m + 1) + 4 * m + 4 * m + 4 * n
     if fwd_kernel == "k_fwd1l":
         # 2-byte ranks; per union entry its column and x (4 + 4 bytes);
-        # union offsets per 64 rows; no x term (read through the unions)
+        # union offsets per group of rows; no x term (read through the unions)
         nu = R.forward_union_entries()
         stored_f = (bytes_f - 2 * info["padded_nnz"] - 4 * n + 8 * nu
-                    + 8 * (-(-m // 64) + 1))
+                    + 8 * (-(-m // urows) + 1))
     elif cols16:
         stored_f = bytes_f - 2 * info["padded_nnz"] + 8 * m
     if info["bwd_staged"] == 3:
@@ -474,10 +475,13 @@ def main():
                                 8 * nu + 8 * tz["segments"] + 4 * tz["max_parts"] * n + 12 * n
                                 if info["fused_reduce_per_xcd"] == 2 else
                                 4 * 2 * nu + 8 * (n + 1) + 4 * 4 * n),
-                            "traffic": MEASURED_TRAFFIC.get((
-                                "k_fb1_treduce" if info["fused_reduce_per_xcd"] == 2 else
-                                "k_fb1_reduce8", info["fused_lanes_per_column"],
-                                cfg_p["row_order"], urows))},
+                            "traffic": MEASURED_TRAFFIC.get(
+                                ("k_fb1_treduce", info["fused_lanes_per_column"],
+                                 cfg_p["row_order"], urows, tz["columns"],
+                                 tz["segments_per_workgroup"])
+                                if info["fused_reduce_per_xcd"] == 2 else
+                                ("k_fb1_reduce8", info["fused_lanes_per_column"],
+                                 cfg_p["row_order"], urows))},
                  "iteration": {"ms": ms_it, "algorithmic_bytes": it_bytes,
                                "achieved": round(it_bytes / (ms_it * 1e-3) / 1e9, 1),
                                "frac": round(it_bytes / (ms_it * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),

@@ -541,7 +541,8 @@ int raptr_recon_get_fused(const raptr_recon* plan, int* available, int* on, int*
  *   1: the groups split into 8 ranges, one per XCD (workgroup i runs on
  *      XCD i % 8), so each XCD's L2 holds an eighth of the partial sums;
  *      the 8 range sums of a column added in order;
- *   2 (the default where the plan has its segment table): column tiles:
+ *   2 (the default where the plan's segment table gives at least 512
+ *      workgroups; else 1): column tiles:
  *      per part of a tile's groups, their partials read in the order the
  *      products kernel wrote them and added into per-slot column sums in
  *      LDS, then in slot and part order (raptr_recon_set_fused_tiles).
@@ -550,10 +551,10 @@ int raptr_recon_get_fused(const raptr_recon* plan, int* available, int* on, int*
 int raptr_recon_set_fused_reduce(raptr_recon* plan, int mode);
 int raptr_recon_get_fused_reduce(const raptr_recon* plan);
 /* Plan rows per column-union group (the LDS-staged forward and the fused
- * iteration): rows = 0 lets the plan choose (the largest of 256, 128, 64
- * and 32 whose fused workgroup of 8 waves leaves room in LDS for three on a
- * CU, else the largest that leaves room for two, else 64), else a power
- * of two in 16..512.  Rebuilds the unions, the
+ * iteration): rows = 0 lets the plan choose (among 256, 128, 64, 32 and 16
+ * rows giving at least 1,024 groups, the largest whose fused workgroup of
+ * 8 waves leaves room in LDS for three on a CU, else the largest that
+ * leaves room for two; else 16), else a power of two in 16..512.  Rebuilds the unions, the
  * fused pass's index and the column-tile table, drops the plan's captured
  * iterations and restores the fused defaults.  Refused on a clone or a plan
  * with clones, which share them, and on plans other than single-frame

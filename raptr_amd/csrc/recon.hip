@@ -341,7 +341,7 @@ __global__ __launch_bounds__(256) void k_fwd1c(int64_t m, const int64_t* __restr
 // HBM reads per pass at full size; x lives in LDS, so nothing is evicted).
 constexpr int kLocMaxUnion = 16384;  // 64 KB of LDS per workgroup
 // union group sizes a plan may take (plan rows per group), largest first
-constexpr int kLocRowsChoices[] = {256, 128, 64, 32};
+constexpr int kLocRowsChoices[] = {256, 128, 64, 32, 16};
 
 template <int VT, int LANES, int U>
 __global__ __launch_bounds__(256) void k_fwd1l(int64_t m, int rows,
@@ -2930,15 +2930,20 @@ namespace {
 // column unions of the LDS-staged forward (k_fwd1l): sizes, scan, unions
 // and local indices; none when a group's union exceeds kLocMaxUnion
 // The column unions of groups of `rows` consecutive plan rows.  rows = 0:
-// the largest of kLocRowsChoices whose fused workgroup (8 waves adding to
-// one shared region) leaves three of them room in a CU's LDS, else the
-// largest that leaves two, else 64.  Larger groups share more of their
+// among the sizes of kLocRowsChoices that give at least kMinGroups groups
+// (four workgroups per CU), the largest whose fused workgroup (8 waves
+// adding to one shared region) leaves three of them room in a CU's LDS,
+// else the largest that leaves two; without one, the smallest size (the
+// most workgroups; small plans, whose iterations take tens of
+// microseconds, are bound by occupancy).  Larger groups share more of their
 // columns (full-size workload: 112M union entries for 64 rows, 51M for
 // 256; 96^3 geometry: 306M for 32, 217M for 64), so the staging of x and
 // the partial sums shrink, while fewer workgroups fit a CU.  Measured best
 // at full size: 256 rows (48 KB, 2.09 ms per iteration; 128: 2.21, 512:
-// 2.36); 96^3: 32 rows (74.5 KB, 3.58 ms; 64: 4.39, 128: 3.86).
+// 2.36); 96^3: 32 rows (74.5 KB, 3.58 ms; 64: 4.39, 128: 3.86); small
+// geometry (20,480 rows): 16 rows (0.044 ms; 256: 0.094).
 constexpr size_t kUnionLdsTiers[] = {160 * 1024 / 3, 160 * 1024 / 2};
+constexpr int64_t kMinGroups = 1024;
 int build_local_columns(raptr_recon* R, hipStream_t st, int rows_req = 0) {
   const int64_t m = R->m;
   const int n = R->n;
@@ -2970,11 +2975,11 @@ int build_local_columns(raptr_recon* R, hipStream_t st, int rows_req = 0) {
       if (mxc <= kLocMaxUnion)
         lds[ci] = size_t(rr + 1) * sizeof(int64_t) + size_t(2 * mxc + 1) * sizeof(float);
     }
-    int pick = 64;
+    int pick = kLocRowsChoices[nc - 1];
     bool found = false;
     for (size_t tier : kUnionLdsTiers) {
       for (int ci = 0; ci < nc && !found; ++ci)
-        if (lds[ci] <= tier) {
+        if (lds[ci] <= tier && (m + kLocRowsChoices[ci] - 1) / kLocRowsChoices[ci] >= kMinGroups) {
           pick = kLocRowsChoices[ci];
           found = true;
         }
@@ -3264,10 +3269,13 @@ int build_union_index(raptr_recon* R, hipStream_t st) {
   fused_lds_attributes();
   // the column-tile reduce (tiles of 1,024 columns, parts of 512 segments,
   // 64 lanes per slot: the fastest shape measured at full size and at 96^3)
-  // where its table builds, else the per-XCD reduce
+  // where its table builds with at least 512 workgroups, else the per-XCD
+  // reduce (small plans: a few dozen tile workgroups leave the GPU idle)
   if (build_tile_segments(R, 10, 512, st) == RAPTR_OK) {
-    R->fb_xcd = 2;
-    R->fb_lpc = 64;
+    if (R->tnwg >= 512) {
+      R->fb_xcd = 2;
+      R->fb_lpc = 64;
+    }
   } else {
     (void)hipGetLastError();
   }

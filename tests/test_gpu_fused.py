@@ -33,11 +33,14 @@ def test_fused_small_system(gpu, oracle, small_system):
     # the default shape: one shared region, 4 waves where five such
     # workgroups fit a CU's LDS (else 8), 64 lanes per row with 2
     # register-held steps (rows of about 140 entries), the column-tile
-    # reduce with 64 lanes per segment slot
+    # reduce with 64 lanes per segment slot where its table gives 512
+    # workgroups, else the per-XCD reduce with 16 lanes per column
     u = R.unions()
     lds = 8 * (u["rows"] + 1) + 4 * (2 * u["largest"] + 1)
+    tiled = R.fused_tiles()["workgroups"] >= 512
     assert R.fused() == {"available": 1, "on": 2, "waves": 4 if 5 * lds <= 160 * 1024 else 8,
-                         "lanes_per_row": 64, "lanes_per_column": 64, "per_xcd": 2, "steps": 2}
+                         "lanes_per_row": 64, "lanes_per_column": 64 if tiled else 16,
+                         "per_xcd": 2 if tiled else 1, "steps": 2}
     R.set_fused(False)
     sep = R.mlem(S["y"], x0, 5)
     assert _rel(sep, x64) <= 1e-4
@@ -129,7 +132,7 @@ def test_fused_tile_reduce(gpu, oracle, small_system):
 
 
 def test_fused_union_rows(gpu, oracle, small_system):
-    """Plan rows per union group: the plan's choice is one of 32-256 rows;
+    """Plan rows per union group: the plan's choice is one of 16-256 rows;
     for groups of 16-512 rows the union entries do not grow with the group
     size and the fused iteration stays within 1e-4 of the FP64 oracle and
     1e-6 of the separate passes, bit for bit run to run; bad sizes and
@@ -142,7 +145,7 @@ def test_fused_union_rows(gpu, oracle, small_system):
     x64 = _oracle_mlem(oracle, S, S["y"], 5)
     R = Reconstructor(S["A"], S["sens"], "f32", 1, layout="blocked", row_order=S["setup"])
     u0 = R.unions()
-    assert u0["rows"] in (32, 64, 128, 256) and 0 < u0["largest"] <= u0["entries"]
+    assert u0["rows"] in (16, 32, 64, 128, 256) and 0 < u0["largest"] <= u0["entries"]
     R.set_fused(False)
     sep = R.mlem(S["y"], x0, 5)
     last = None
@@ -298,8 +301,10 @@ def test_fused_lds_limits(gpu):
                          (-1, 2, 16, 64)]:
         with pytest.raises(RaptrError):
             R.set_fused(o, w, lr, rw)
+    # ten groups: too few column-tile workgroups, the per-XCD reduce stays
+    assert R.fused_tiles()["workgroups"] < 512
     assert R.fused() == {"available": 1, "on": 1, "waves": 2, "lanes_per_row": 32,
-                         "lanes_per_column": 64, "per_xcd": 2, "steps": 2}
+                         "lanes_per_column": 64, "per_xcd": 1, "steps": 2}
     for bad in (1, 5, 0):
         with pytest.raises(RaptrError):
             R.set_fused_steps(bad)
@@ -336,7 +341,8 @@ def test_fused_absent(gpu, small_system):
     rp, col, val = _wide_csr(n, m=600)
     R = Reconstructor(_device_csr(rp, col, val, n), DeviceArray.from_numpy(np.ones(n)), "f32", 1,
                       layout="blocked")
-    assert R.fused()["available"] == 0
+    R.set_union_rows(64)  # groups of 64 wide rows touch more than 16,384 voxels
+    assert R.unions()["rows"] == 0 and R.fused()["available"] == 0
     with pytest.raises(RaptrError):
         R.set_fused(True, 2, 16, 64)
     n = 20000

@@ -619,7 +619,8 @@ def test_recon_forward_compact_columns(gpu, vt):
     y = rng.random(m).astype(np.float32) + 0.5
     R = Reconstructor(A, sens, vt, 1, layout="blocked")
     assert R.info()["compact_rows"] & 2
-    assert R.forward_kind() == (0, [0])
+    R.set_forward_kind(0)  # rows of global columns, not the LDS-staged unions
+    assert R.forward_kind()[0] == 0
     dx, dy, de = DeviceArray.from_numpy(x), DeviceArray.from_numpy(y), DeviceArray(m, np.float32)
     yt = np.array([np.dot(val[rp[i]:rp[i + 1]].astype(np.float64), x[col[rp[i]:rp[i + 1]]])
                    for i in range(m)])
@@ -657,6 +658,7 @@ def test_recon_forward_compact_columns_absent(gpu, case):
     R = Reconstructor(A, DeviceArray.from_numpy(np.ones(n)), "f32", 1, layout="blocked",
                       block_rows=512)
     assert R.info()["compact_rows"] & 2 == 0
+    R.set_union_rows(64)  # groups of 64 wide rows touch more than 16,384 voxels
     assert R.forward_kind() == (0, [0])
     with pytest.raises(RaptrError):
         R.set_forward_kind(2)
@@ -777,7 +779,9 @@ def test_recon_forward_lds_staged_large_grid(gpu, n):
 def test_recon_forward_lds_staged_absent(gpu):
     """No column unions when a 64-row group touches more than 16,384
     voxels (rows of random columns over the grid, one window of the union
-    builder or several); kind 2 is then refused."""
+    builder or several; groups of 64 rows set explicitly, since the plan's
+    own choice for 600 rows is the smallest group); kind 2 is then
+    refused."""
     from raptr_amd import RaptrError
     from raptr_amd.device import DeviceArray
     from raptr_amd.recon import Reconstructor
@@ -785,7 +789,8 @@ def test_recon_forward_lds_staged_absent(gpu):
                               (700000, _wide_csr(700000, m=600))):
         R = Reconstructor(_device_csr(rp, col, val, n), DeviceArray.from_numpy(np.ones(n)), "f32",
                           1, layout="blocked")
-        assert R.forward_kind() == (0, [0])
+        R.set_union_rows(64)
+        assert R.unions()["rows"] == 0 and R.forward_kind() == (0, [0])
         with pytest.raises(RaptrError):
             R.set_forward_kind(2)
 
