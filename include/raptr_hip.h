@@ -552,9 +552,9 @@ int raptr_recon_set_fused_reduce(raptr_recon* plan, int mode);
 int raptr_recon_get_fused_reduce(const raptr_recon* plan);
 /* Plan rows per column-union group (the LDS-staged forward and the fused
  * iteration): rows = 0 lets the plan choose (among 256, 128, 64, 32 and 16
- * rows giving at least 1,024 groups, the largest whose fused workgroup of
- * 8 waves leaves room in LDS for three on a CU, else the largest that
- * leaves room for two; else 16), else a power of two in 16..512.  Rebuilds the unions, the
+ * rows giving at least 1,024 groups, else 16, the size whose fused
+ * iteration a bytes-and-occupancy model of the measured shapes puts
+ * fastest: 256 at full size, 32 at 96^3), else a power of two in 16..512.  Rebuilds the unions, the
  * fused pass's index and the column-tile table, drops the plan's captured
  * iterations and restores the fused defaults.  Refused on a clone or a plan
  * with clones, which share them, and on plans other than single-frame

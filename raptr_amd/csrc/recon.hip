@@ -2931,19 +2931,26 @@ namespace {
 // and local indices; none when a group's union exceeds kLocMaxUnion
 // The column unions of groups of `rows` consecutive plan rows.  rows = 0:
 // among the sizes of kLocRowsChoices that give at least kMinGroups groups
-// (four workgroups per CU), the largest whose fused workgroup (8 waves
-// adding to one shared region) leaves three of them room in a CU's LDS,
-// else the largest that leaves two; without one, the smallest size (the
-// most workgroups; small plans, whose iterations take tens of
-// microseconds, are bound by occupancy).  Larger groups share more of their
-// columns (full-size workload: 112M union entries for 64 rows, 51M for
-// 256; 96^3 geometry: 306M for 32, 217M for 64), so the staging of x and
-// the partial sums shrink, while fewer workgroups fit a CU.  Measured best
-// at full size: 256 rows (48 KB, 2.09 ms per iteration; 128: 2.21, 512:
-// 2.36); 96^3: 32 rows (74.5 KB, 3.58 ms; 64: 4.39, 128: 3.86); small
-// geometry (20,480 rows): 16 rows (0.044 ms; 256: 0.094).
-constexpr size_t kUnionLdsTiers[] = {160 * 1024 / 3, 160 * 1024 / 2};
+// (four workgroups per CU; without one, the smallest size), the one whose
+// fused iteration the model below puts fastest.  Larger groups share more
+// of their columns (full-size workload: 112M union entries for 64 rows,
+// 51M for 256; 96^3 geometry: 441M for 16, 306M for 32, 153M for 128), so
+// the staging of x and the partial sums shrink, while fewer workgroups fit
+// a CU.  Model: the products kernel moves 6 bytes per padded entry and 12
+// per union entry (column, x, partial sum) at 5.2 TB/s with three or more
+// 8-wave workgroups per CU, 4.5 with two, 3.0 with one; the reduce reads 8
+// bytes per union entry at 3.3 TB/s.  Measured per iteration, full size:
+// 64 rows 2.38 ms, 128 2.21, 256 2.04-2.09, 512 2.36 (model 2.40, 2.22,
+// 2.11, 2.36); 96^3: 16 rows 4.44, 32 3.58, 64 4.39, 128 3.86 (model
+// 3.88, 3.63, 4.50, 4.07); small geometry (20,480 rows): 16 rows 0.044 ms
+// (256: 0.094).
 constexpr int64_t kMinGroups = 1024;
+double union_model_ms(int64_t pnnz, int64_t entries, size_t lds) {
+  const int per_cu = lds > 0 ? int(std::min<size_t>(4, (160 * 1024) / lds)) : 0;
+  if (per_cu == 0) return 1e30;
+  const double rate = per_cu >= 3 ? 5.2e9 : per_cu == 2 ? 4.5e9 : 3.0e9;  // bytes per ms
+  return (6.0 * double(pnnz) + 12.0 * double(entries)) / rate + 8.0 * double(entries) / 3.3e9;
+}
 int build_local_columns(raptr_recon* R, hipStream_t st, int rows_req = 0) {
   const int64_t m = R->m;
   const int n = R->n;
@@ -2955,7 +2962,7 @@ int build_local_columns(raptr_recon* R, hipStream_t st, int rows_req = 0) {
   // then the choice
   if (!rows_req) {
     const int nc = int(sizeof(kLocRowsChoices) / sizeof(int));
-    std::vector<size_t> lds(nc, ~size_t(0));
+    std::vector<double> cost(nc, 1e30);
     for (int ci = 0; ci < nc; ++ci) {
       const int rr = kLocRowsChoices[ci];
       const int64_t ng = (m + rr - 1) / rr;
@@ -2972,19 +2979,19 @@ int build_local_columns(raptr_recon* R, hipStream_t st, int rows_req = 0) {
       if (le != hipSuccess)
         return fail(RAPTR_ERR_DEVICE, "column unions: %s", hipGetErrorString(le));
       const int64_t mxc = hc.empty() ? 0 : *std::max_element(hc.begin(), hc.end());
+      int64_t tot = 0;
+      for (int64_t v : hc) tot += v;
       if (mxc <= kLocMaxUnion)
-        lds[ci] = size_t(rr + 1) * sizeof(int64_t) + size_t(2 * mxc + 1) * sizeof(float);
+        cost[ci] = union_model_ms(
+            R->pnnz, tot, size_t(rr + 1) * sizeof(int64_t) + size_t(2 * mxc + 1) * sizeof(float));
     }
     int pick = kLocRowsChoices[nc - 1];
-    bool found = false;
-    for (size_t tier : kUnionLdsTiers) {
-      for (int ci = 0; ci < nc && !found; ++ci)
-        if (lds[ci] <= tier && (m + kLocRowsChoices[ci] - 1) / kLocRowsChoices[ci] >= kMinGroups) {
-          pick = kLocRowsChoices[ci];
-          found = true;
-        }
-      if (found) break;
-    }
+    double best = 1e30;
+    for (int ci = 0; ci < nc; ++ci)  // largest first: ties keep the larger groups
+      if ((m + kLocRowsChoices[ci] - 1) / kLocRowsChoices[ci] >= kMinGroups && cost[ci] < best) {
+        best = cost[ci];
+        pick = kLocRowsChoices[ci];
+      }
     return build_local_columns(R, st, pick);
   }
   const int rows = rows_req;
