@@ -1794,7 +1794,8 @@ __global__ void k_vptr(const uint64_t* __restrict__ keys, int64_t nu, int n,
 // (twg[w] = {t, k, first segment, end segment}: a run of the tile's
 // segments in group order) and adds its partials for the 2^tshift columns
 // of the tile.  The segments (a group's union positions inside the tile:
-// distinct columns, ascending) are split into NREG consecutive runs, one
+// distinct columns, ascending; tcol holds each position's column inside
+// its tile) are split into NREG consecutive runs, one
 // per slot of LPS lanes, each with its own column sums in LDS, so the
 // lanes of one read-add-write never share a column.  A slot reads its
 // segments' partials and columns in the order k_fb1l wrote them (no
@@ -1807,7 +1808,7 @@ template <int LPS, int W>
 __global__ __launch_bounds__(64 * W) void k_fb1_treduce(int n, int tshift,
                                                         const int4* __restrict__ twg,
                                                         const int2* __restrict__ tseg,
-                                                        const int* __restrict__ ucol,
+                                                        const uint16_t* __restrict__ tcol,
                                                         const float* __restrict__ cpart,
                                                         float* __restrict__ ck) {
   constexpr int SUB = 64 / LPS;
@@ -1866,7 +1867,7 @@ __global__ __launch_bounds__(64 * W) void k_fb1_treduce(int n, int tshift,
         const int p = p0 + (q - __shfl(excl, i, LPS)) * LPS + sl;
         ok[u] = q < total && p < p1;
         const int pp = ok[u] ? p : 0;  // position 0 exists (the unions are not empty)
-        c[u] = ucol[pp] - j0;
+        c[u] = tcol[pp];
         v[u] = cpart[pp];
       }
 #pragma unroll
@@ -1914,6 +1915,15 @@ __global__ __launch_bounds__(256) void k_fb1_tcombine(int n, int tshift, const i
     __syncthreads();
   }
   if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
+}
+
+// column of every union position inside its tile of 2^tshift columns
+__global__ void k_tile_cols(const int* __restrict__ ucol, int64_t nu, int tshift,
+                            uint16_t* __restrict__ tcol) {
+  const int mask = (1 << tshift) - 1;
+  for (int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; p < nu;
+       p += int64_t(gridDim.x) * blockDim.x)
+    tcol[p] = uint16_t(ucol[p] & mask);
 }
 
 // segment starts: the first position of every group and every position
@@ -2685,7 +2695,7 @@ int launch_fused1(const raptr_recon* R, const float* x, const float* y, hipStrea
   if (L == LL && TW == WW) {                                                                   \
     hipLaunchKernelGGL((k_fb1_treduce<LL, WW>), dim3(R->tnwg), dim3(64 * WW), tshm, st, n,     \
                        R->tshift, (const int4*)R->twg, (const int2*)R->tseg,                   \
-                       (const int*)R->lucol, (const float*)R->cpart, R->ck);                   \
+                       (const uint16_t*)R->tcol, (const float*)R->cpart, R->ck);               \
     launched = true;                                                                           \
   }
 #define RAPTR_FTL(LL) RAPTR_FT(LL, 1) RAPTR_FT(LL, 2) RAPTR_FT(LL, 4) RAPTR_FT(LL, 8)
@@ -3049,6 +3059,7 @@ int build_tile_segments(raptr_recon* R, int tshift, int target, hipStream_t st) 
   const int64_t ng = (int64_t(R->m) + R->lrows - 1) / R->lrows;
   const int ntiles = int((int64_t(R->n) + (int64_t(1) << tshift) - 1) >> tshift);
   rfree(R, R->tseg);
+  rfree(R, R->tcol);
   rfree(R, R->twg);
   rfree(R, R->tk);
   rfree(R, R->ck);
@@ -3107,8 +3118,11 @@ int build_tile_segments(raptr_recon* R, int tshift, int target, hipStream_t st) 
       rc = fail(RAPTR_ERR_OUT_OF_MEMORY, "tile segments scratch: %s", hipGetErrorString(ae));
     }
     if (!rc) rc = ralloc(R, &R->tseg, size_t(S));
+    if (!rc) rc = ralloc(R, &R->tcol, size_t(nu));
     if (!rc) {
       Carver c2(tmp2, z2.bytes() + 4096);
+      hipLaunchKernelGGL(k_tile_cols, dim3(gp), dim3(256), 0, st, (const int*)R->lucol, nu,
+                         tshift, R->tcol);
       uint64_t* k0 = c2.take<uint64_t>(size_t(S));
       uint64_t* k1 = c2.take<uint64_t>(size_t(S));
       uint32_t* v0 = c2.take<uint32_t>(size_t(S));
@@ -3167,6 +3181,7 @@ int build_tile_segments(raptr_recon* R, int tshift, int target, hipStream_t st) 
   }
   if (rc) {
     rfree(R, R->tseg);
+    rfree(R, R->tcol);
     rfree(R, R->twg);
     rfree(R, R->tk);
     rfree(R, R->ck);
@@ -4162,6 +4177,7 @@ int raptr_recon_set_union_rows(raptr_recon* R, int rows, raptr_stream_t stream) 
   rfree(R, R->vpos);
   rfree(R, R->vb);
   rfree(R, R->tseg);
+  rfree(R, R->tcol);
   rfree(R, R->twg);
   rfree(R, R->tk);
   rfree(R, R->ck);

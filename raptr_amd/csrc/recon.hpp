@@ -167,6 +167,7 @@ struct raptr_recon {
   // segments, one workgroup each (twg: tile, part, first and end segment);
   // ck[k][j] takes part k's sums of column j
   int2* tseg = nullptr;
+  uint16_t* tcol = nullptr;  // per union position: its column inside its tile
   int4* twg = nullptr;
   int* tk = nullptr;
   float* ck = nullptr;
