@@ -64,8 +64,8 @@ MEASURED_TRAFFIC = {
     # group); one launch each per iteration
     ("k_fb1l", 2, 4, 64, "y", 64): 2 * 5035683 * 1024 + 465498 * 1024,
     ("k_fb1_reduce8", 16, "y", 64): 2 * 1028086 * 1024 + 6883 * 1024,
-    ("k_fb1l", 2, 8, 64, "y", 256): 2 * 4908508 * 1024 + 221522 * 1024,
-    ("k_fb1_treduce", 64, "y", 256, 1024, 512): 2 * 318335 * 1024 + 7869 * 1024,
+    ("k_fb1l", 2, 8, 64, "y", 256): 2 * 4908446 * 1024 + 221626 * 1024,
+    ("k_fb1_treduce", 64, "y", 256, 1024, 512): 2 * 267869 * 1024 + 7869 * 1024,
 }
 
 
@@ -468,11 +468,12 @@ def main():
                                 if info["fused_reduce_per_xcd"] else
                                 "k_fb1_reduce (group partials per column, update)"),
                             "ms": round(ms_red, 4),
-                            # mode 2: each partial and its column once, the
-                            # segment bounds, the part sums and x, s, x';
-                            # else each partial and its position once
+                            # mode 2: each partial and its 16-bit column
+                            # inside its tile once, the segment bounds, the
+                            # part sums and x, s, x'; else each partial and
+                            # its position once
                             "algorithmic_bytes": int(
-                                8 * nu + 8 * tz["segments"] + 4 * tz["max_parts"] * n + 12 * n
+                                6 * nu + 8 * tz["segments"] + 4 * tz["max_parts"] * n + 12 * n
                                 if info["fused_reduce_per_xcd"] == 2 else
                                 4 * 2 * nu + 8 * (n + 1) + 4 * 4 * n),
                             "traffic": MEASURED_TRAFFIC.get(
