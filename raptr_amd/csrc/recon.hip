@@ -1621,6 +1621,12 @@ __global__ __launch_bounds__(64 * W) void k_fb1l(int64_t m, int rows,
               cw[i3] = fmaf(vv[3], ev, o3);
             }
           }
+          // The row slots of a wave add in turn.  A lane runs only its own
+          // slot's block, so the compiler, which orders memory per lane,
+          // may otherwise move the next slot's LDS reads above this slot's
+          // writes (rows of one group share columns): a compiler barrier
+          // keeps the program order, which the wave's LDS accesses follow.
+          asm volatile("" ::: "memory");
         }
       }
       if (SH) __syncthreads();

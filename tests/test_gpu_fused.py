@@ -160,6 +160,18 @@ def test_fused_union_rows(gpu, oracle, small_system):
         np.testing.assert_array_equal(a, R.mlem(S["y"], x0, 5))
         assert _rel(a, x64) <= 1e-4, rows
         assert _rel(a, sep) <= 1e-6, rows
+        # groups of several batches of rows: every shape, in particular
+        # waves holding 2 or 4 rows whose column sums go in turn
+        for shape in ((2, 8, 32, 64), (2, 8, 16, 64), (2, 4, 32, 64), (2, 2, 16, 64),
+                      (1, 2, 32, 64), (1, 4, 16, 64), (2, 8, 64, 64)):
+            try:
+                R.set_fused(*shape)
+            except capi.RaptrError:  # a region per wave beyond LDS
+                continue
+            b = R.mlem(S["y"], x0, 5)
+            assert _rel(b, sep) <= 1e-6, (rows, shape)
+            np.testing.assert_array_equal(b, R.mlem(S["y"], x0, 5))
+        R.set_fused(2, 4, 64, 64)
         R.set_fused(False)  # the LDS-staged forward over the same unions
         assert R.forward_kind()[0] == 2
         np.testing.assert_array_equal(R.mlem(S["y"], x0, 5), sep)
