@@ -1432,13 +1432,14 @@ __global__ __launch_bounds__(256) void k_bwd1_xreduce(int n, const float* __rest
 // 64 / LANES rows, LANES lanes per row, with S steps of each row's entries
 // held in registers and the next batch's already in flight:
 //   yt_i = sum_j A_ij x_j (x from LDS), e_i = y_i / yt_i,
-// then A_ij e_i is added into the wave's own column sums over the union,
-// also in LDS (waves never share a sum, so no two waves' adds interleave;
-// inside one wave the adds follow its instruction order).  At the end the
-// W partial column sums are added in wave order and written over the
-// union (cpart[up[g] + rank]); k_fb1_reduce adds each column's group
-// partials in group order.  The matrix is streamed once per iteration
-// instead of once per product.
+// then A_ij e_i is added into column sums over the union, also in LDS:
+// one region per wave, or (SH) one region that the waves add to in turn,
+// a barrier between turns; inside one wave the adds follow its
+// instruction order.  At the end the regions are added in wave order and
+// written over the union (cpart[up[g] + rank]); a column reduce
+// (k_fb1_treduce + k_fb1_tcombine, k_fb1_reduce8 + k_bwd1_xreduce or
+// k_fb1_reduce) adds each column's group partials in a fixed order.  The
+// matrix is streamed once per iteration instead of once per product.
 template <int CTRL>
 __device__ __forceinline__ float dppf(float v) {
   return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));

@@ -145,18 +145,19 @@ struct raptr_recon {
   uint64_t* fsplit = nullptr;  // per row: sorted positions where the high part steps
   int fwd_c16 = 0;             // single-frame forward reads fcol16
   int fwd_kind = 0;            // 0 rows per lane group, 2 LDS-staged x
-  // LDS-staged forward: per group of consecutive plan rows the sorted union
-  // of its columns (lup: group offsets, lucol: columns) and every padded
-  // entry's 16-bit rank in its group's union
+  // LDS-staged forward: per group of lrows consecutive plan rows the
+  // sorted union of its columns (lup: group offsets, lucol: columns) and
+  // every padded entry's 16-bit rank in its group's union
   int64_t* lup = nullptr;
   int* lucol = nullptr;
   uint16_t* lcol = nullptr;
   int lrows = 64;              // plan rows per union group
   int lmax = 0;                // largest union
   int64_t lnu = 0;             // union entries of all groups
-  // fused single-frame pass (k_fb1l + k_fb1_reduce): the groups' partial
-  // column sums over their unions (cpart) and the unions' positions by
-  // column (vptr: n + 1 offsets, vpos: positions ascending per column)
+  // fused single-frame pass (k_fb1l + a column reduce): the groups'
+  // partial column sums over their unions (cpart) and, for the per-XCD and
+  // by-column reduces, the unions' positions by column (vptr: n + 1
+  // offsets, vpos: positions ascending per column)
   float* cpart = nullptr;
   int64_t* vptr = nullptr;
   int* vpos = nullptr;
