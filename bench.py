@@ -51,8 +51,9 @@ CPU_ITERS = 5  # iterations per timed run of the CPU baseline
 # kernel shapes the autotuner picks; other shapes report no traffic figure.
 # Row order: y-major locality order for one frame.
 MEASURED_TRAFFIC = {
-    # forward (kernel, lanes, unroll, row order); one launch per pass
-    ("k_fwd1l", 16, 4, "y"): 2 * 5156424 * 1024 + 27546 * 1024,
+    # forward (kernel, lanes, unroll, row order[, rows per union group]);
+    # one launch per pass
+    ("k_fwd1l", 16, 4, "y", 64): 2 * 5156424 * 1024 + 27546 * 1024,
     # backward with per-XCD row ranges (unroll, row blocks, row order,
     # 16-bit rows, lanes per column); one launch per pass (its reduce,
     # k_bwd1_xreduce, moves 0.006 GB more)
@@ -408,6 +409,8 @@ def main():
     nl_f = 1
     nl_b = 1 if info["bwd_staged"] else -(-nb // cfg_p["bwd_blocks_per_launch"])
     key_f = (fwd_kernel, cfg_p["fwd_lanes"], cfg_p["fwd_unroll"], cfg_p["row_order"])
+    if fwd_kernel == "k_fwd1l":
+        key_f += (urows,)
     key_b = ("k_bwd", cfg_p["bwd_unroll"], nb, cfg_p["bwd_blocks_per_launch"],
              cfg_p["bwd_staged"], cfg_p["row_order"])
     if info["bwd_staged"] == 3:
