@@ -106,8 +106,17 @@ def test_full_plain_fused_vs_fp64(full, oracle):
     from raptr_amd.recon import Reconstructor
     F = full
     n = F["grid"].nvox
-    R = Reconstructor(F["A"], F["sens"], "f32", 1, row_order=F["order"])
+    from raptr_amd import sysmat
+    from raptr_amd.device import DeviceArray
+    # the single-frame default row order (y-major), as bench.py's plain plan
+    order_y = sysmat.locality_order(F["setup"], DeviceArray.from_numpy(F["meas"]), kind="y")
+    R = Reconstructor(F["A"], F["sens"], "f32", 1, row_order=order_y)
     assert R.fused()["available"] == 1 and R.fused()["on"] == 2  # the default
+    # the plan's choices at this size: groups of 256 rows, 8 waves, the
+    # column-tile reduce (tiles of 1,024 columns, parts of 512 segments)
+    assert R.unions()["rows"] == 256
+    assert R.fused()["waves"] == 8 and R.fused()["per_xcd"] == 2
+    assert R.fused_tiles()["columns"] == 1024
     y, x0 = F["Y"][:, 0].copy(), np.full(n, 1.0 / n, np.float32)
     x = R.mlem(y, x0, ITERS)
     np.testing.assert_array_equal(x, R.mlem(y, x0, ITERS))
@@ -202,7 +211,8 @@ def test_full_96_plain_fused_vs_fp64(gpu, oracle):
     """The largest serving geometry as a single-frame plan: 16x16-pixel
     heads, 60 angles at 3 deg (3,932,160 channels), 96^3 grid at 2 mm, 20
     rays, 5e6 events; rows of about 640 entries.  The fused iteration (its
-    default shape here: 3 register-held steps, 8 waves per group) and the
+    default shape here: 32-row union groups, 3 register-held steps, 8
+    waves per group, the column-tile reduce) and the
     separate passes, 20 iterations, against the FP64 oracle on the same
     matrix (<= 1e-4) and each other (<= 2e-6); the fused run repeats bit for
     bit."""
@@ -220,7 +230,8 @@ def test_full_96_plain_fused_vs_fp64(gpu, oracle):
     print(f"96^3 workload: m={len(meas)} nnz={A.nnz}", flush=True)
     R = Reconstructor(A, sens, "f32", 1, row_order=order)
     fz = R.fused()
-    assert fz["on"] == 2 and fz["steps"] == 3 and fz["waves"] == 8
+    assert fz["on"] == 2 and fz["steps"] == 3 and fz["waves"] == 8 and fz["per_xcd"] == 2
+    assert R.unions()["rows"] == 32
     y32, x0 = y.astype(np.float32), np.full(n, 1.0 / n, np.float32)
     xf = R.mlem(y32, x0, it)
     np.testing.assert_array_equal(xf, R.mlem(y32, x0, it))
